@@ -1,0 +1,140 @@
+/* C ABI of the MI355X Poisson surface reconstruction.
+ *
+ * Replaces the reference's class entry point
+ *   void SparseSurfelFusion::PoissonReconstruction::SolvePoissionReconstructionMesh(
+ *        DeviceArrayView<DepthSurfel> denseSurfel);
+ *   (mesh/PoissonReconstruction.h:124, mesh/PoissonReconstruction.cpp:195-245)
+ * and its output getters
+ *   ComputeTriangleIndices::GetRebuildMeshVertices / GetRebuildMeshTriangleIndices
+ *   (mesh/ComputeTriangleIndices.h:331-342).
+ * Plain pointers and sizes only.  Input point buffers are device pointers
+ * borrowed for the call; the mesh buffers are device memory owned by the
+ * context and valid until the next psr_reconstruct or psr_destroy.
+ */
+#ifndef PSR_H
+#define PSR_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct psr_ctx psr_ctx;
+
+/* Status codes.  A CG that stops at its iteration limit is reported in
+ * psr_stats, not as an error (the reference is silent there,
+ * mesh/solver/CGAlgorithm.cu:36). */
+enum {
+    PSR_OK = 0,
+    PSR_ERR_INVALID = 1,  /* bad argument or degenerate input (reference divides by 0, BuildOctree.cu:149) */
+    PSR_ERR_CAPACITY = 2, /* input exceeds a configured capacity (reference throws, DeviceBufferArray.h:125-132) */
+    PSR_ERR_HIP = 3,      /* HIP runtime error (reference calls exit(1), safe_call.hpp:58-71) */
+    PSR_ERR_INTERNAL = 4
+};
+
+enum {
+    PSR_PRECISION_FP32 = 0, /* the reference's FP32 CG (CGAlgorithm.cu:10-80) */
+    PSR_PRECISION_MIXED = 1 /* FP16 field storage, compensated FP32 dots, FP64 outer refinement */
+};
+
+typedef struct {
+    int depth;             /* maximum octree depth D, 2..9 (MAX_DEPTH_OCTREE, GlobalConfigs.h:34) */
+    uint32_t max_points;   /* capacity; 0 = no limit (MAX_SURFEL_COUNT, GlobalConfigs.h:7) */
+    int precision;         /* PSR_PRECISION_* */
+    float cg_tol;          /* absolute tolerance on the updated residual norm, 1e-5 (CGAlgorithm.cu:298) */
+    int cg_max_iter;       /* 10000 (CGAlgorithm.cu:15) */
+    float refine_rel_tol;  /* PSR_PRECISION_MIXED: target true relative residual per depth, 1e-5 */
+    int refine_max_outer;  /* PSR_PRECISION_MIXED: outer refinement steps, 4 */
+} psr_config;
+
+typedef struct {
+    const float* xyz;      /* device pointer, x y z floats at the start of each record */
+    const float* normal;   /* device pointer, nx ny nz floats at the start of each record */
+    uint32_t count;
+    uint32_t stride_bytes; /* 12 for packed float3; 64 for the reference's DepthSurfel (xyz at +16, normal at +32) */
+} psr_points;
+
+typedef struct {
+    const float* vertices;     /* device, 3 floats per vertex, normalised frame (reference Q11) */
+    const int32_t* triangles;  /* device, 3 vertex indices per triangle */
+    uint32_t n_vertices;
+    uint32_t n_triangles;
+} psr_mesh;
+
+#define PSR_STATS_MAX_DEPTH 10
+typedef struct {
+    int depth;
+    uint32_t n_points;
+    uint32_t n_nodes;
+    uint32_t nodes_per_depth[PSR_STATS_MAX_DEPTH + 1];
+    int cg_iterations[PSR_STATS_MAX_DEPTH + 1];
+    double cg_updated_residual[PSR_STATS_MAX_DEPTH + 1];
+    double cg_true_relres[PSR_STATS_MAX_DEPTH + 1]; /* filled when stats_level >= 1 */
+    float iso_value;
+    uint32_t n_vertex_nodes, n_edge_nodes, n_face_nodes;
+    uint32_t n_subdivision_roots, n_coarse_roots;
+    uint32_t n_mesh_vertices, n_mesh_triangles;
+    int q1_fired;         /* the reference's level-local child 0 parent rule applied somewhere */
+    float ms_octree, ms_field, ms_solve, ms_mesh, ms_total;
+} psr_stats;
+
+/* Arrays that psr_export can copy to host memory (for parity checks). */
+enum {
+    PSR_ARRAY_SORTED_POS = 0,   /* float4 per sorted point (w = 0) */
+    PSR_ARRAY_SORTED_NRM,       /* float4 per sorted point */
+    PSR_ARRAY_CODES,            /* int32 Morton code per sorted point */
+    PSR_ARRAY_ORDER,            /* int32 input index per sorted point */
+    PSR_ARRAY_POINT2NODE,       /* int32 level-local depth-D node per sorted point */
+    PSR_ARRAY_NODE_KEY,         /* int32 per node */
+    PSR_ARRAY_NODE_PIDX,
+    PSR_ARRAY_NODE_PNUM,
+    PSR_ARRAY_NODE_PARENT,
+    PSR_ARRAY_NODE_CHILDREN,    /* int32 [M][8] */
+    PSR_ARRAY_NODE_NEIGHS,      /* int32 [M][27] */
+    PSR_ARRAY_NODE_DIDX,
+    PSR_ARRAY_NODE_DNUM,
+    PSR_ARRAY_NODE_ENCIDX,
+    PSR_ARRAY_LEVEL_BASE,       /* int32 [D+2] */
+    PSR_ARRAY_LEVEL_COUNT,      /* int32 [D+1] */
+    PSR_ARRAY_NORM_PARAMS,      /* float corner xyz, edge */
+    PSR_ARRAY_VFIELD,           /* float [nD][3] */
+    PSR_ARRAY_DIVERGENCE,       /* float [M] */
+    PSR_ARRAY_DX,               /* float [M] */
+    PSR_ARRAY_CHI_POINTS,       /* float [N] */
+    PSR_ARRAY_VERTS,            /* int32 [V][6]: x y z (float bits), owner, kind, depth */
+    PSR_ARRAY_EDGES,            /* int32 [E][2]: kind, owner */
+    PSR_ARRAY_FACES,            /* int32 [F][3]: kind, owner, hasParentFace */
+    PSR_ARRAY_NODE_VERTICES,    /* int32 [M][8], 1-based */
+    PSR_ARRAY_NODE_EDGES,       /* int32 [M][12], 1-based, 0 above depth D */
+    PSR_ARRAY_NODE_FACES,       /* int32 [M][6], 1-based */
+    PSR_ARRAY_VVALUE,           /* float [V] */
+    PSR_ARRAY_MESH_VERTICES,    /* float [n_vertices][3] */
+    PSR_ARRAY_MESH_TRIANGLES,   /* int32 [n_triangles][3] */
+    PSR_ARRAY_TABLE_DOT,        /* double [res][res] */
+    PSR_ARRAY_TABLE_DDOT,
+    PSR_ARRAY_TABLE_D2DOT,
+    PSR_ARRAY_CUBE,             /* int32 [nD] depth-D cube categories */
+    PSR_ARRAY_COUNT_
+};
+
+void psr_default_config(psr_config* cfg);
+/* hip_stream: a hipStream_t to work on, or NULL for a context-owned stream. */
+int psr_create(const psr_config* cfg, void* hip_stream, psr_ctx** out);
+/* Runs the whole pipeline; on return the mesh counts are final and the buffers
+ * are ready on the context's stream. */
+int psr_reconstruct(psr_ctx* ctx, const psr_points* points, psr_mesh* out);
+int psr_sync(psr_ctx* ctx);
+int psr_get_stats(psr_ctx* ctx, psr_stats* out);
+/* Copies array `array_id` of the last reconstruction to host memory.  With
+ * host_dst == NULL only *needed (bytes) is written. */
+int psr_export(psr_ctx* ctx, int array_id, void* host_dst, uint64_t bytes, uint64_t* needed);
+void psr_destroy(psr_ctx* ctx);
+const char* psr_status_string(int status);
+const char* psr_last_error(const psr_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PSR_H */

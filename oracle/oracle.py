@@ -1,0 +1,124 @@
+"""ctypes loader for the oracle library -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, the smoke check and the benchmark's CPU-baseline leg use this
+module, as the checker or as the timed CPU port.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+STAGE_OCTREE, STAGE_FIELD, STAGE_SOLVE, STAGE_MESH = 1, 2, 4, 8
+STAGE_ALL = 15
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.orc_run.restype = ctypes.c_void_p
+        L.orc_run.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                              ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.orc_free.argtypes = [ctypes.c_void_p]
+        L.orc_count.restype = ctypes.c_longlong
+        L.orc_count.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
+        L.orc_copy.restype = ctypes.c_int
+        L.orc_copy.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p, ctypes.c_longlong]
+        L.orc_scalar.restype = ctypes.c_double
+        L.orc_scalar.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
+        L.orc_inner.restype = ctypes.c_double
+        L.orc_inner.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.orc_basis.restype = ctypes.c_double
+        L.orc_basis.argtypes = [ctypes.c_int, ctypes.c_double]
+        L.orc_lut.restype = ctypes.c_int
+        L.orc_lut.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.orc_mc_word.restype = ctypes.c_longlong
+        L.orc_mc_word.argtypes = [ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+_DTYPES = {4: {"f": np.float32, "i": np.int32}, 8: {"f": np.float64, "i": np.int64}}
+_FLOAT_FIELDS = {"spos", "snrm", "center", "vfield", "div", "dx", "chi_pts", "cg_relres",
+                 "cg_recres", "vvalue", "mesh_v", "dot", "ddot", "d2dot"}
+_NODE_WIDTH = {"children": 8, "vertices": 8, "neighs": 27, "edges": 12, "faces": 6}
+
+
+class Result:
+    """One oracle run; arrays are fetched lazily by name."""
+
+    def __init__(self, xyz, nrm, depth, stages=STAGE_ALL, precision=1, threads=0):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float32)
+        nrm = np.ascontiguousarray(nrm, dtype=np.float32)
+        self.n = xyz.shape[0]
+        self.depth = depth
+        self._h = lib().orc_run(xyz.ctypes.data, nrm.ctypes.data, self.n, depth, stages, precision, threads)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orc_free(self._h)
+            self._h = None
+
+    def scalar(self, name):
+        return lib().orc_scalar(self._h, name.encode())
+
+    def count(self, name):
+        return int(lib().orc_count(self._h, name.encode()))
+
+    def array(self, name):
+        if name.startswith("node_"):
+            f = name[5:]
+            m = self.count("nodes")
+            w = _NODE_WIDTH.get(f, 1)
+            out = np.empty((m, w) if w > 1 else m, dtype=np.int32)
+        elif name in ("verts", "edges", "faces"):
+            k = self.count(name)
+            width = {"verts": 6, "edges": 2, "faces": 3}[name]
+            out = np.empty((k, width), dtype=np.int32)
+        else:
+            k = self.count(name)
+            if k < 0:
+                raise KeyError(name)
+            esz = 8 if name in ("vfield", "div", "dx", "chi_pts", "cg_relres", "cg_recres", "vvalue",
+                                "dot", "ddot", "d2dot") else 4
+            kind = "f" if name in _FLOAT_FIELDS else "i"
+            out = np.empty(k, dtype=_DTYPES[esz][kind])
+        rc = lib().orc_copy(self._h, name.encode(), out.ctypes.data, out.nbytes)
+        if rc != 0:
+            raise RuntimeError("orc_copy(%s) = %d" % (name, rc))
+        return out
+
+    @property
+    def base(self):
+        return [int(self.scalar("base%d" % d)) for d in range(self.depth + 2)]
+
+    @property
+    def counts(self):
+        return [int(self.scalar("count%d" % d)) for d in range(self.depth + 1)]
+
+
+def inner(a, b, da, db):
+    return lib().orc_inner(a, b, da, db)
+
+
+def basis(idx, x):
+    return lib().orc_basis(idx, x)
+
+
+def lut(which, s, i):
+    return lib().orc_lut(which, s, i)
+
+
+def mc_word(cube):
+    return lib().orc_mc_word(cube) & ((1 << 64) - 1)

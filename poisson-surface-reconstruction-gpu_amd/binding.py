@@ -1,0 +1,185 @@
+"""ctypes binding of libpsr.so with the reference's class interface.
+
+``PoissonReconstruction.SolvePoissionReconstructionMesh`` mirrors
+mesh/PoissonReconstruction.h:124 (name kept, misspelling included): it takes
+an oriented point array resident on the GPU and leaves the mesh in device
+buffers owned by the object, read back with ``GetRebuildMeshVertices`` /
+``GetRebuildMeshTriangleIndices`` (mesh/ComputeTriangleIndices.h:331-342).
+There is no CPU fallback: a missing library raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpsr.so")
+
+PSR_OK, PSR_ERR_INVALID, PSR_ERR_CAPACITY, PSR_ERR_HIP, PSR_ERR_INTERNAL = range(5)
+PRECISION_FP32, PRECISION_MIXED = 0, 1
+
+ARRAYS = ["SORTED_POS", "SORTED_NRM", "CODES", "ORDER", "POINT2NODE", "NODE_KEY", "NODE_PIDX", "NODE_PNUM",
+          "NODE_PARENT", "NODE_CHILDREN", "NODE_NEIGHS", "NODE_DIDX", "NODE_DNUM", "NODE_ENCIDX", "LEVEL_BASE",
+          "LEVEL_COUNT", "NORM_PARAMS", "VFIELD", "DIVERGENCE", "DX", "CHI_POINTS", "VERTS", "EDGES", "FACES",
+          "NODE_VERTICES", "NODE_EDGES", "NODE_FACES", "VVALUE", "MESH_VERTICES", "MESH_TRIANGLES", "TABLE_DOT",
+          "TABLE_DDOT", "TABLE_D2DOT", "CUBE"]
+ARRAY_ID = {n: i for i, n in enumerate(ARRAYS)}
+_ARRAY_LAYOUT = {
+    "SORTED_POS": (np.float32, 4), "SORTED_NRM": (np.float32, 4), "NODE_CHILDREN": (np.int32, 8),
+    "NODE_NEIGHS": (np.int32, 27), "NORM_PARAMS": (np.float32, 1), "VFIELD": (np.float32, 3),
+    "DIVERGENCE": (np.float32, 1), "DX": (np.float32, 1), "CHI_POINTS": (np.float32, 1), "VERTS": (np.int32, 6),
+    "EDGES": (np.int32, 2), "FACES": (np.int32, 3), "NODE_VERTICES": (np.int32, 8), "NODE_EDGES": (np.int32, 12),
+    "NODE_FACES": (np.int32, 6), "VVALUE": (np.float32, 1), "MESH_VERTICES": (np.float32, 3),
+    "MESH_TRIANGLES": (np.int32, 3), "TABLE_DOT": (np.float64, 1), "TABLE_DDOT": (np.float64, 1),
+    "TABLE_D2DOT": (np.float64, 1)}
+
+EXPORTED_SYMBOLS = ["psr_default_config", "psr_create", "psr_reconstruct", "psr_sync", "psr_get_stats",
+                    "psr_export", "psr_destroy", "psr_status_string", "psr_last_error"]
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("depth", ctypes.c_int), ("max_points", ctypes.c_uint32), ("precision", ctypes.c_int),
+                ("cg_tol", ctypes.c_float), ("cg_max_iter", ctypes.c_int), ("refine_rel_tol", ctypes.c_float),
+                ("refine_max_outer", ctypes.c_int)]
+
+
+class Points(ctypes.Structure):
+    _fields_ = [("xyz", ctypes.c_void_p), ("normal", ctypes.c_void_p), ("count", ctypes.c_uint32),
+                ("stride_bytes", ctypes.c_uint32)]
+
+
+class Mesh(ctypes.Structure):
+    _fields_ = [("vertices", ctypes.c_void_p), ("triangles", ctypes.c_void_p), ("n_vertices", ctypes.c_uint32),
+                ("n_triangles", ctypes.c_uint32)]
+
+
+_MAXD = 11
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("depth", ctypes.c_int), ("n_points", ctypes.c_uint32), ("n_nodes", ctypes.c_uint32),
+                ("nodes_per_depth", ctypes.c_uint32 * _MAXD), ("cg_iterations", ctypes.c_int * _MAXD),
+                ("cg_updated_residual", ctypes.c_double * _MAXD), ("cg_true_relres", ctypes.c_double * _MAXD),
+                ("iso_value", ctypes.c_float), ("n_vertex_nodes", ctypes.c_uint32),
+                ("n_edge_nodes", ctypes.c_uint32), ("n_face_nodes", ctypes.c_uint32),
+                ("n_subdivision_roots", ctypes.c_uint32), ("n_coarse_roots", ctypes.c_uint32),
+                ("n_mesh_vertices", ctypes.c_uint32), ("n_mesh_triangles", ctypes.c_uint32),
+                ("q1_fired", ctypes.c_int), ("ms_octree", ctypes.c_float), ("ms_field", ctypes.c_float),
+                ("ms_solve", ctypes.c_float), ("ms_mesh", ctypes.c_float), ("ms_total", ctypes.c_float)]
+
+    def as_dict(self):
+        d = self.depth
+        return {
+            "depth": d, "n_points": self.n_points, "n_nodes": self.n_nodes,
+            "nodes_per_depth": list(self.nodes_per_depth)[:d + 1],
+            "cg_iterations": list(self.cg_iterations)[:d + 1],
+            "cg_updated_residual": list(self.cg_updated_residual)[:d + 1],
+            "iso_value": self.iso_value, "V": self.n_vertex_nodes, "E": self.n_edge_nodes, "F": self.n_face_nodes,
+            "roots": self.n_subdivision_roots, "coarse_roots": self.n_coarse_roots,
+            "mesh_vertices": self.n_mesh_vertices, "mesh_triangles": self.n_mesh_triangles,
+            "ms_octree": self.ms_octree, "ms_field": self.ms_field, "ms_solve": self.ms_solve,
+            "ms_mesh": self.ms_mesh, "ms_total": self.ms_total,
+        }
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libpsr.so is not built (%s); run build()" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        L.psr_default_config.argtypes = [ctypes.POINTER(Config)]
+        L.psr_create.argtypes = [ctypes.POINTER(Config), ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]
+        L.psr_reconstruct.argtypes = [ctypes.c_void_p, ctypes.POINTER(Points), ctypes.POINTER(Mesh)]
+        L.psr_sync.argtypes = [ctypes.c_void_p]
+        L.psr_get_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(Stats)]
+        L.psr_export.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64,
+                                 ctypes.POINTER(ctypes.c_uint64)]
+        L.psr_destroy.argtypes = [ctypes.c_void_p]
+        L.psr_status_string.restype = ctypes.c_char_p
+        L.psr_status_string.argtypes = [ctypes.c_int]
+        L.psr_last_error.restype = ctypes.c_char_p
+        L.psr_last_error.argtypes = [ctypes.c_void_p]
+        _lib = L
+    return _lib
+
+
+class PsrError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (%d): %s" % (lib().psr_status_string(code).decode(), code, msg))
+        self.code = code
+
+
+class PoissonReconstruction:
+    """Host-side mirror of SparseSurfelFusion::PoissonReconstruction."""
+
+    def __init__(self, depth=7, precision=PRECISION_FP32, max_points=0, cg_tol=1e-5, cg_max_iter=10000,
+                 stream=None):
+        cfg = Config()
+        lib().psr_default_config(ctypes.byref(cfg))
+        cfg.depth = depth
+        cfg.precision = precision
+        cfg.max_points = max_points
+        cfg.cg_tol = cg_tol
+        cfg.cg_max_iter = cg_max_iter
+        self._ctx = ctypes.c_void_p()
+        rc = lib().psr_create(ctypes.byref(cfg), stream, ctypes.byref(self._ctx))
+        if rc != PSR_OK:
+            raise PsrError(rc, "psr_create")
+        self.depth = depth
+        self.mesh = Mesh()
+
+    def close(self):
+        if self._ctx:
+            lib().psr_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def solve_ptr(self, xyz_ptr, nrm_ptr, count, stride_bytes=12):
+        pts = Points(xyz_ptr, nrm_ptr, count, stride_bytes)
+        rc = lib().psr_reconstruct(self._ctx, ctypes.byref(pts), ctypes.byref(self.mesh))
+        if rc != PSR_OK:
+            raise PsrError(rc, lib().psr_last_error(self._ctx).decode())
+        return self.mesh
+
+    def SolvePoissionReconstructionMesh(self, xyz, normals):
+        """xyz, normals: contiguous float32 GPU tensors of shape (N, 3), or one
+        (N, 16) float32 tensor of DepthSurfel records (xyz at 4..6, normal at 8..10)."""
+        if normals is None:
+            base = xyz.data_ptr()
+            return self.solve_ptr(base + 16, base + 32, xyz.shape[0], 64)
+        assert xyz.is_cuda and normals.is_cuda and xyz.is_contiguous() and normals.is_contiguous()
+        return self.solve_ptr(xyz.data_ptr(), normals.data_ptr(), xyz.shape[0], 12)
+
+    def stats(self):
+        st = Stats()
+        lib().psr_get_stats(self._ctx, ctypes.byref(st))
+        return st.as_dict()
+
+    def export(self, name):
+        aid = ARRAY_ID[name]
+        need = ctypes.c_uint64(0)
+        rc = lib().psr_export(self._ctx, aid, None, 0, ctypes.byref(need))
+        if rc != PSR_OK:
+            raise PsrError(rc, "export %s" % name)
+        dtype, width = _ARRAY_LAYOUT.get(name, (np.int32, 1))
+        n = need.value // np.dtype(dtype).itemsize
+        out = np.empty(n, dtype=dtype)
+        rc = lib().psr_export(self._ctx, aid, out.ctypes.data, out.nbytes, ctypes.byref(need))
+        if rc != PSR_OK:
+            raise PsrError(rc, "export %s" % name)
+        return out.reshape(-1, width) if width > 1 else out
+
+    def GetRebuildMeshVertices(self):
+        return self.export("MESH_VERTICES")
+
+    def GetRebuildMeshTriangleIndices(self):
+        return self.export("MESH_TRIANGLES")

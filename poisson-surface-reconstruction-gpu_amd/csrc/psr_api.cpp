@@ -1,0 +1,371 @@
+// C ABI and orchestration (SolvePoissionReconstructionMesh,
+// mesh/PoissonReconstruction.cpp:195-245): octree -> field -> divergence ->
+// per-depth CG -> implicit function -> geometry -> marching cubes.
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "psr_internal.h"
+
+struct psr_ctx {
+    psr::Ctx c;
+    std::string lastError;
+};
+
+namespace psr {
+
+void* Arena::get(const std::string& name, size_t bytes) {
+    Block& b = blocks_[name];
+    if (b.bytes < bytes) {
+        if (b.ptr) {
+            PSR_HIP_CHECK(hipDeviceSynchronize());
+            PSR_HIP_CHECK(hipFree(b.ptr));
+            total_ -= b.bytes;
+        }
+        size_t want = bytes + bytes / 8 + 256;
+        PSR_HIP_CHECK(hipMalloc(&b.ptr, want));
+        b.bytes = want;
+        total_ += want;
+        ++allocs_;
+    }
+    return b.ptr;
+}
+
+void Arena::release() {
+    for (auto& kv : blocks_)
+        if (kv.second.ptr) (void)hipFree(kv.second.ptr);
+    blocks_.clear();
+    total_ = 0;
+}
+
+}  // namespace psr
+
+namespace {
+
+int fail(psr_ctx* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->lastError = msg;
+    return code;
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return -1.f;
+    return ms;
+}
+
+int run(psr_ctx* ctx, const psr_points* pts, psr_mesh* out) {
+    using namespace psr;
+    Ctx& c = ctx->c;
+    hipStream_t s = c.stream;
+    const int N = int(pts->count);
+    if (N < 8) return fail(ctx, PSR_ERR_INVALID, "need at least 8 points");
+    if (c.cfg.max_points && pts->count > c.cfg.max_points)
+        return fail(ctx, PSR_ERR_CAPACITY, "point count exceeds max_points");
+    if (!pts->xyz || !pts->normal) return fail(ctx, PSR_ERR_INVALID, "null point buffer");
+    if (pts->stride_bytes && (pts->stride_bytes < 12 || pts->stride_bytes % 4))
+        return fail(ctx, PSR_ERR_INVALID, "bad stride");
+    std::memset(&c.stats, 0, sizeof(c.stats));
+    PSR_HIP_CHECK(hipEventRecord(c.ev[0], s));
+    build_octree(c, *pts);
+    PSR_HIP_CHECK(hipEventRecord(c.ev[1], s));
+    compute_field(c);
+    PSR_HIP_CHECK(hipEventRecord(c.ev[2], s));
+    solve_depths(c);
+    points_implicit(c);
+    PSR_HIP_CHECK(hipEventRecord(c.ev[3], s));
+    build_geometry(c);
+    extract_mesh(c);
+    PSR_HIP_CHECK(hipEventRecord(c.ev[4], s));
+    PSR_HIP_CHECK(hipStreamSynchronize(s));
+    // statistics
+    psr_stats& st = c.stats;
+    const Octree& o = c.oct;
+    st.depth = o.L.D;
+    st.n_points = N;
+    st.n_nodes = o.M;
+    for (int d = 0; d <= o.L.D; ++d) st.nodes_per_depth[d] = o.L.count[d];
+    int iters[PSR_MAX_DEPTH + 1];
+    double resid[PSR_MAX_DEPTH + 1];
+    PSR_HIP_CHECK(hipMemcpy(iters, c.arena.get("cg_iters", 4 * (PSR_MAX_DEPTH + 1)), sizeof(iters),
+                            hipMemcpyDeviceToHost));
+    PSR_HIP_CHECK(hipMemcpy(resid, c.arena.get("cg_resid", 8 * (PSR_MAX_DEPTH + 1)), sizeof(resid),
+                            hipMemcpyDeviceToHost));
+    int err = 0;
+    PSR_HIP_CHECK(hipMemcpy(&err, c.arena.get("cg_err", 4), sizeof(int), hipMemcpyDeviceToHost));
+    if (err) return fail(ctx, PSR_ERR_INTERNAL, "CG group barrier timed out");
+    for (int d = 0; d <= o.L.D; ++d) {
+        st.cg_iterations[d] = iters[d];
+        st.cg_updated_residual[d] = resid[d];
+    }
+    PSR_HIP_CHECK(hipMemcpy(&st.iso_value, c.arena.get("iso_value", 4), sizeof(float), hipMemcpyDeviceToHost));
+    st.n_vertex_nodes = c.geo.V;
+    st.n_edge_nodes = c.geo.E;
+    st.n_face_nodes = c.geo.F;
+    st.ms_octree = elapsed(c.ev[0], c.ev[1]);
+    st.ms_field = elapsed(c.ev[1], c.ev[2]);
+    st.ms_solve = elapsed(c.ev[2], c.ev[3]);
+    st.ms_mesh = elapsed(c.ev[3], c.ev[4]);
+    st.ms_total = elapsed(c.ev[0], c.ev[4]);
+    out->vertices = c.mesh.vertices;
+    out->triangles = c.mesh.triangles;
+    out->n_vertices = uint32_t(c.mesh.nv);
+    out->n_triangles = uint32_t(c.mesh.nt);
+    return PSR_OK;
+}
+
+// Host copy of a device int array, transformed column-wise into row-major [M][w].
+int export_columns(psr::Ctx& c, const int* dev, int M, int w, void* dst) {
+    std::vector<int> tmp(size_t(M) * w);
+    PSR_HIP_CHECK(hipMemcpy(tmp.data(), dev, sizeof(int) * tmp.size(), hipMemcpyDeviceToHost));
+    int* o = static_cast<int*>(dst);
+    for (int i = 0; i < M; ++i)
+        for (int k = 0; k < w; ++k) o[size_t(i) * w + k] = tmp[size_t(k) * M + i];
+    (void)c;
+    return PSR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void psr_default_config(psr_config* cfg) {
+    if (!cfg) return;
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->depth = 7;
+    cfg->max_points = 0;
+    cfg->precision = PSR_PRECISION_FP32;
+    cfg->cg_tol = 1e-5f;
+    cfg->cg_max_iter = 10000;
+    cfg->refine_rel_tol = 1e-5f;
+    cfg->refine_max_outer = 4;
+}
+
+int psr_create(const psr_config* cfg, void* hip_stream, psr_ctx** out) {
+    if (!out) return PSR_ERR_INVALID;
+    *out = nullptr;
+    psr_config def;
+    psr_default_config(&def);
+    const psr_config& k = cfg ? *cfg : def;
+    if (k.depth < 2 || k.depth > 9) return PSR_ERR_INVALID;
+    psr_ctx* ctx = new (std::nothrow) psr_ctx();
+    if (!ctx) return PSR_ERR_INTERNAL;
+    try {
+        psr::Ctx& c = ctx->c;
+        c.cfg = k;
+        PSR_HIP_CHECK(hipGetDevice(&c.device));
+        if (hip_stream) {
+            c.stream = static_cast<hipStream_t>(hip_stream);
+        } else {
+            PSR_HIP_CHECK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+            c.ownStream = true;
+        }
+        for (auto& e : c.ev) PSR_HIP_CHECK(hipEventCreate(&e));
+        PSR_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&c.hostPinned), 64 * sizeof(int)));
+        psr::build_tables(c);  // one-time, like ComputeVectorField's constructor (ComputeVectorField.cpp:11-15)
+        PSR_HIP_CHECK(hipStreamSynchronize(c.stream));
+    } catch (const psr::HipError& e) {
+        psr_destroy(ctx);
+        return PSR_ERR_HIP;
+    } catch (...) {
+        psr_destroy(ctx);
+        return PSR_ERR_INTERNAL;
+    }
+    *out = ctx;
+    return PSR_OK;
+}
+
+int psr_reconstruct(psr_ctx* ctx, const psr_points* points, psr_mesh* out) {
+    if (!ctx || !points || !out) return PSR_ERR_INVALID;
+    try {
+        return run(ctx, points, out);
+    } catch (const psr::HipError& e) {
+        return fail(ctx, PSR_ERR_HIP, e.what());
+    } catch (const std::exception& e) {
+        return fail(ctx, PSR_ERR_INTERNAL, e.what());
+    }
+}
+
+int psr_sync(psr_ctx* ctx) {
+    if (!ctx) return PSR_ERR_INVALID;
+    return hipStreamSynchronize(ctx->c.stream) == hipSuccess ? PSR_OK : PSR_ERR_HIP;
+}
+
+int psr_get_stats(psr_ctx* ctx, psr_stats* out) {
+    if (!ctx || !out) return PSR_ERR_INVALID;
+    *out = ctx->c.stats;
+    return PSR_OK;
+}
+
+int psr_export(psr_ctx* ctx, int id, void* dst, uint64_t bytes, uint64_t* needed) {
+    if (!ctx) return PSR_ERR_INVALID;
+    psr::Ctx& c = ctx->c;
+    const psr::Octree& o = c.oct;
+    const int M = o.M, N = o.N, D = o.L.D;
+    const int nD = M ? o.L.count[D] : 0;
+    struct Src {
+        const void* p;
+        size_t bytes;
+        int cols;  // >0: int column-major [cols][M] exported row-major
+    } src{nullptr, 0, 0};
+    std::vector<int> hostTmp;
+    switch (id) {
+        case PSR_ARRAY_SORTED_POS: src = {o.spos, size_t(16) * N, 0}; break;
+        case PSR_ARRAY_SORTED_NRM: src = {o.snrm, size_t(16) * N, 0}; break;
+        case PSR_ARRAY_CODES: src = {o.codes, size_t(4) * N, 0}; break;
+        case PSR_ARRAY_ORDER: src = {o.order, size_t(4) * N, 0}; break;
+        case PSR_ARRAY_POINT2NODE: src = {o.point2node, size_t(4) * N, 0}; break;
+        case PSR_ARRAY_NODE_KEY: src = {o.key, size_t(4) * M, 0}; break;
+        case PSR_ARRAY_NODE_PIDX: src = {o.pidx, size_t(4) * M, 0}; break;
+        case PSR_ARRAY_NODE_PNUM: src = {o.pnum, size_t(4) * M, 0}; break;
+        case PSR_ARRAY_NODE_PARENT: src = {o.parent, size_t(4) * M, 0}; break;
+        case PSR_ARRAY_NODE_DIDX: src = {o.didx, size_t(4) * M, 0}; break;
+        case PSR_ARRAY_NODE_DNUM: src = {o.dnum, size_t(4) * M, 0}; break;
+        case PSR_ARRAY_NODE_ENCIDX: src = {o.encIdx, size_t(4) * M, 0}; break;
+        case PSR_ARRAY_NODE_NEIGHS: src = {o.neigh, size_t(4) * 27 * M, 27}; break;
+        case PSR_ARRAY_NODE_CHILDREN: {
+            if (needed) *needed = size_t(32) * M;
+            if (!dst) return PSR_OK;
+            if (bytes < size_t(32) * M) return PSR_ERR_CAPACITY;
+            std::vector<int> cb(M);
+            if (hipMemcpy(cb.data(), o.childBase, 4 * size_t(M), hipMemcpyDeviceToHost) != hipSuccess)
+                return PSR_ERR_HIP;
+            int* out = static_cast<int*>(dst);
+            for (int i = 0; i < M; ++i)
+                for (int k = 0; k < 8; ++k) out[8 * size_t(i) + k] = cb[i] < 0 ? -1 : cb[i] + k;
+            return PSR_OK;
+        }
+        case PSR_ARRAY_LEVEL_BASE:
+        case PSR_ARRAY_LEVEL_COUNT: {
+            size_t n = (id == PSR_ARRAY_LEVEL_BASE) ? D + 2 : D + 1;
+            if (needed) *needed = 4 * n;
+            if (!dst) return PSR_OK;
+            if (bytes < 4 * n) return PSR_ERR_CAPACITY;
+            std::memcpy(dst, id == PSR_ARRAY_LEVEL_BASE ? o.L.base : o.L.count, 4 * n);
+            return PSR_OK;
+        }
+        case PSR_ARRAY_NORM_PARAMS: src = {o.normParams, 16, 0}; break;
+        case PSR_ARRAY_VFIELD: {
+            if (needed) *needed = size_t(12) * nD;
+            if (!dst) return PSR_OK;
+            if (bytes < size_t(12) * nD) return PSR_ERR_CAPACITY;
+            std::vector<float> t(size_t(3) * nD);
+            if (hipMemcpy(t.data(), c.fld.vfield, 12 * size_t(nD), hipMemcpyDeviceToHost) != hipSuccess)
+                return PSR_ERR_HIP;
+            float* out = static_cast<float*>(dst);
+            for (int i = 0; i < nD; ++i)
+                for (int a = 0; a < 3; ++a) out[3 * size_t(i) + a] = t[size_t(a) * nD + i];
+            return PSR_OK;
+        }
+        case PSR_ARRAY_DIVERGENCE: src = {c.fld.div, size_t(4) * M, 0}; break;
+        case PSR_ARRAY_DX: src = {c.fld.dx, size_t(4) * M, 0}; break;
+        case PSR_ARRAY_CHI_POINTS: src = {c.fld.chiPts, size_t(4) * N, 0}; break;
+        case PSR_ARRAY_VERTS: {
+            int V = c.geo.V;
+            if (needed) *needed = size_t(24) * V;
+            if (!dst) return PSR_OK;
+            if (bytes < size_t(24) * V) return PSR_ERR_CAPACITY;
+            std::vector<float4> pos(V);
+            std::vector<int> own(V), kind(V);
+            if (hipMemcpy(pos.data(), c.geo.vtxPos, 16 * size_t(V), hipMemcpyDeviceToHost) != hipSuccess ||
+                hipMemcpy(own.data(), c.geo.vtxOwner, 4 * size_t(V), hipMemcpyDeviceToHost) != hipSuccess ||
+                hipMemcpy(kind.data(), c.geo.vtxKind, 4 * size_t(V), hipMemcpyDeviceToHost) != hipSuccess)
+                return PSR_ERR_HIP;
+            int* out = static_cast<int*>(dst);
+            for (int v = 0; v < V; ++v) {
+                std::memcpy(out + 6 * size_t(v), &pos[v], 12);
+                out[6 * size_t(v) + 3] = own[v];
+                out[6 * size_t(v) + 4] = kind[v];
+                std::memcpy(out + 6 * size_t(v) + 5, &pos[v].w, 4);
+            }
+            return PSR_OK;
+        }
+        case PSR_ARRAY_EDGES:
+        case PSR_ARRAY_FACES: {
+            bool e = id == PSR_ARRAY_EDGES;
+            int n = e ? c.geo.E : c.geo.F, w = e ? 2 : 3;
+            if (needed) *needed = size_t(4) * w * n;
+            if (!dst) return PSR_OK;
+            if (bytes < size_t(4) * w * n) return PSR_ERR_CAPACITY;
+            std::vector<int> a(n), b(n), h(n);
+            if (hipMemcpy(a.data(), e ? c.geo.edgeKind : c.geo.faceKind, 4 * size_t(n), hipMemcpyDeviceToHost) !=
+                    hipSuccess ||
+                hipMemcpy(b.data(), e ? c.geo.edgeOwner : c.geo.faceOwner, 4 * size_t(n), hipMemcpyDeviceToHost) !=
+                    hipSuccess)
+                return PSR_ERR_HIP;
+            if (!e && hipMemcpy(h.data(), c.geo.faceHasParent, 4 * size_t(n), hipMemcpyDeviceToHost) != hipSuccess)
+                return PSR_ERR_HIP;
+            int* out = static_cast<int*>(dst);
+            for (int i = 0; i < n; ++i) {
+                out[w * size_t(i)] = a[i];
+                out[w * size_t(i) + 1] = b[i];
+                if (!e) out[w * size_t(i) + 2] = h[i];
+            }
+            return PSR_OK;
+        }
+        case PSR_ARRAY_NODE_VERTICES: src = {c.geo.nodeVert, size_t(4) * 8 * M, 8}; break;
+        case PSR_ARRAY_NODE_FACES: src = {c.geo.nodeFace, size_t(4) * 6 * M, 6}; break;
+        case PSR_ARRAY_NODE_EDGES: {
+            size_t need = size_t(48) * M;
+            if (needed) *needed = need;
+            if (!dst) return PSR_OK;
+            if (bytes < need) return PSR_ERR_CAPACITY;
+            std::vector<int> t(size_t(12) * nD);
+            if (hipMemcpy(t.data(), c.geo.nodeEdge, 4 * t.size(), hipMemcpyDeviceToHost) != hipSuccess)
+                return PSR_ERR_HIP;
+            int* out = static_cast<int*>(dst);
+            std::memset(out, 0, need);
+            int off = o.L.base[D];
+            for (int i = 0; i < nD; ++i)
+                for (int k = 0; k < 12; ++k) out[12 * size_t(off + i) + k] = t[size_t(k) * nD + i];
+            return PSR_OK;
+        }
+        case PSR_ARRAY_VVALUE: src = {c.geo.vvalue, size_t(4) * c.geo.V, 0}; break;
+        case PSR_ARRAY_MESH_VERTICES: src = {c.mesh.vertices, size_t(12) * c.mesh.nv, 0}; break;
+        case PSR_ARRAY_MESH_TRIANGLES: src = {c.mesh.triangles, size_t(12) * c.mesh.nt, 0}; break;
+        case PSR_ARRAY_TABLE_DOT: src = {c.tables.dot, size_t(8) * c.tables.res * c.tables.res, 0}; break;
+        case PSR_ARRAY_TABLE_DDOT: src = {c.tables.ddot, size_t(8) * c.tables.res * c.tables.res, 0}; break;
+        case PSR_ARRAY_TABLE_D2DOT: src = {c.tables.d2dot, size_t(8) * c.tables.res * c.tables.res, 0}; break;
+        case PSR_ARRAY_CUBE: src = {c.arena.get("mc_cube", 4), size_t(4) * nD, 0}; break;
+        default: return fail(ctx, PSR_ERR_INVALID, "unknown array id");
+    }
+    if (needed) *needed = src.bytes;
+    if (!dst) return PSR_OK;
+    if (bytes < src.bytes) return PSR_ERR_CAPACITY;
+    if (!src.p || src.bytes == 0) return PSR_OK;
+    try {
+        PSR_HIP_CHECK(hipStreamSynchronize(c.stream));
+        if (src.cols > 0) return export_columns(c, static_cast<const int*>(src.p), M, src.cols, dst);
+        PSR_HIP_CHECK(hipMemcpy(dst, src.p, src.bytes, hipMemcpyDeviceToHost));
+    } catch (const psr::HipError& e) {
+        return fail(ctx, PSR_ERR_HIP, e.what());
+    }
+    return PSR_OK;
+}
+
+void psr_destroy(psr_ctx* ctx) {
+    if (!ctx) return;
+    psr::Ctx& c = ctx->c;
+    if (c.stream) (void)hipStreamSynchronize(c.stream);
+    c.arena.release();
+    for (auto& e : c.ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c.hostPinned) (void)hipHostFree(c.hostPinned);
+    if (c.ownStream && c.stream) (void)hipStreamDestroy(c.stream);
+    delete ctx;
+}
+
+const char* psr_status_string(int status) {
+    switch (status) {
+        case PSR_OK: return "ok";
+        case PSR_ERR_INVALID: return "invalid argument";
+        case PSR_ERR_CAPACITY: return "capacity exceeded";
+        case PSR_ERR_HIP: return "HIP error";
+        case PSR_ERR_INTERNAL: return "internal error";
+        default: return "unknown status";
+    }
+}
+
+const char* psr_last_error(const psr_ctx* ctx) { return ctx ? ctx->lastError.c_str() : ""; }
+
+}  // extern "C"

@@ -1,0 +1,281 @@
+// Basis inner-product tables, vector-field splat and divergence.
+#include "psr_internal.h"
+
+namespace psr {
+
+namespace {
+
+// --- 1-D tables (FunctionData::setDotTables, mesh/FunctionData.h:134-190) ---
+// Exact FP64 integrals over the real line of products of the scaled/shifted
+// quadratic B-spline and its derivative, one thread per (a, b) entry:
+//   dot[a + res b]   = <F_a,  F_b>
+//   ddot[a + res b]  = <F_a', F_b>
+//   d2dot[a + res b] = <F_a', F_b'>
+// Each product is a polynomial of degree <= 4 between breakpoints; 3-point
+// Gauss-Legendre per piece integrates it exactly.
+__device__ double bsp_d(double u) {
+    u = fabs(u);
+    if (u < 0.5) return 1.0 - (4.0 / 3.0) * u * u;
+    if (u < 1.5) return (2.0 / 3.0) * (1.5 - u) * (1.5 - u);
+    return 0.0;
+}
+__device__ double bsp_deriv(double u) {
+    if (u <= -1.5 || u >= 1.5) return 0.0;
+    if (u < -0.5) return (4.0 / 3.0) * (1.5 + u);
+    if (u < 0.5) return -(8.0 / 3.0) * u;
+    return -(4.0 / 3.0) * (1.5 - u);
+}
+__device__ void cw_of(int idx, double& c, double& w) {
+    int depth = 31 - __clz(idx + 1);
+    int off = idx + 1 - (1 << depth);
+    w = 1.0 / double(1 << depth);
+    c = (0.5 + off) * w;
+}
+
+__global__ void __launch_bounds__(256) build_tables_kernel(int res, double* __restrict__ dot,
+                                                           double* __restrict__ ddot, double* __restrict__ d2dot) {
+    long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (t >= (long long)res * res) return;
+    int a = int(t % res), b = int(t / res);
+    double ca, wa, cb, wb;
+    cw_of(a, ca, wa);
+    cw_of(b, cb, wb);
+    double lo = fmax(ca - 1.5 * wa, cb - 1.5 * wb), hi = fmin(ca + 1.5 * wa, cb + 1.5 * wb);
+    double r0 = 0, r1 = 0, r2 = 0;
+    if (lo < hi) {
+        double br[6];
+        int nb = 0;
+        br[nb++] = lo;
+        double cand[4] = {ca - 0.5 * wa, ca + 0.5 * wa, cb - 0.5 * wb, cb + 0.5 * wb};
+        for (int k = 0; k < 4; ++k)
+            if (cand[k] > lo && cand[k] < hi) br[nb++] = cand[k];
+        br[nb++] = hi;
+        for (int i = 1; i < nb; ++i)  // insertion sort, <= 6 entries
+            for (int j = i; j > 0 && br[j - 1] > br[j]; --j) {
+                double x = br[j];
+                br[j] = br[j - 1];
+                br[j - 1] = x;
+            }
+        const double gx[3] = {-0.77459666924148337704, 0.0, 0.77459666924148337704};
+        const double gw[3] = {5.0 / 9.0, 8.0 / 9.0, 5.0 / 9.0};
+        for (int k = 0; k + 1 < nb; ++k) {
+            double x0 = br[k], x1 = br[k + 1];
+            if (!(x1 > x0)) continue;
+            double h = 0.5 * (x1 - x0), m = 0.5 * (x1 + x0);
+            for (int g = 0; g < 3; ++g) {
+                double x = m + h * gx[g];
+                double ua = (x - ca) / wa, ub = (x - cb) / wb;
+                double fa = bsp_d(ua), fb = bsp_d(ub);
+                double da = bsp_deriv(ua) / wa, db = bsp_deriv(ub) / wb;
+                double wg = gw[g] * h;
+                r0 += wg * fa * fb;
+                r1 += wg * da * fb;
+                r2 += wg * da * db;
+            }
+        }
+        if (fabs(r0) < 1e-15) r0 = r1 = r2 = 0;  // FunctionData.h:172
+    }
+    dot[t] = r0;
+    ddot[t] = r1;
+    d2dot[t] = r2;
+}
+
+// Concatenated ranges of the 27 neighbours of a node, held in LDS per wave.
+struct NbRanges {
+    int start[28];
+    int first[27];
+};
+
+// Fills r for node o with range (first, len) per neighbour; returns total.
+template <class LenFn>
+__device__ __forceinline__ int wave_ranges(NbRanges& r, const int* __restrict__ neigh, int M, int o, int lane,
+                                           LenFn fn) {
+    int nb = -1, len = 0, first = 0;
+    if (lane < 27) {
+        nb = neigh[lane * M + o];
+        if (nb != -1) fn(nb, first, len);
+    }
+    int inc = len;
+    for (int off = 1; off < PSR_WAVE; off <<= 1) {
+        int t = __shfl_up(inc, off, PSR_WAVE);
+        if (lane >= off) inc += t;
+    }
+    if (lane < 27) {
+        r.start[lane] = inc - len;
+        r.first[lane] = first;
+    }
+    int total = __shfl(inc, 26, PSR_WAVE);
+    if (lane == 0) r.start[27] = total;
+    return total;
+}
+
+__device__ __forceinline__ int range_of(const NbRanges& r, int t) {
+    int i = 0;
+    for (int step = 16; step > 0; step >>= 1)
+        if (i + step < 28 && r.start[i + step] <= t) i += step;
+    return i;
+}
+
+// --- vector field (ComputeVectorField.cu:61-100) ---------------------------
+// V(o) = sum over the 27 neighbours' points p of F_o(x)F_o(y)F_o(z) n_p,
+// one wave per depth-D node.
+constexpr int kVfWaves = 4;
+__global__ void __launch_bounds__(kVfWaves * PSR_WAVE) vector_field_kernel(
+    int nD, int offD, int M, int D, const int* __restrict__ neigh, const int* __restrict__ pidx,
+    const int* __restrict__ pnum, const int* __restrict__ key, const float4* __restrict__ pos,
+    const float4* __restrict__ nrm, float* __restrict__ vf) {
+    __shared__ NbRanges R[kVfWaves];
+    int wid = threadIdx.x / PSR_WAVE, lane = __lane_id();
+    int node = blockIdx.x * kVfWaves + wid;
+    if (node >= nD) return;
+    int o = offD + node;
+    NbRanges& r = R[wid];
+    int total = wave_ranges(r, neigh, M, o, lane, [&](int nb, int& first, int& len) {
+        first = pidx[nb];
+        len = pnum[nb];
+    });
+    __builtin_amdgcn_wave_barrier();
+    int3 f = key_func_index(key[o], D, D);
+    float cx, cy, cz, invw;
+    func_center_invw(f.x, cx, invw);
+    func_center_invw(f.y, cy, invw);
+    func_center_invw(f.z, cz, invw);
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    for (int t = lane; t < total; t += PSR_WAVE) {
+        int i = range_of(r, t);
+        int p = r.first[i] + (t - r.start[i]);
+        float4 q = pos[p];
+        float4 n = nrm[p];
+        float w = bspline((q.x - cx) * invw) * bspline((q.y - cy) * invw) * bspline((q.z - cz) * invw);
+        ax += w * n.x;
+        ay += w * n.y;
+        az += w * n.z;
+    }
+    for (int off = PSR_WAVE / 2; off > 0; off >>= 1) {
+        ax += __shfl_xor(ax, off, PSR_WAVE);
+        ay += __shfl_xor(ay, off, PSR_WAVE);
+        az += __shfl_xor(az, off, PSR_WAVE);
+    }
+    if (lane == 0) {
+        vf[node] = ax;
+        vf[nD + node] = ay;
+        vf[2 * nD + node] = az;
+    }
+}
+
+// --- divergence (ComputeNodesDivergence.cu:25-136) -------------------------
+// b(o) = sum over the 27 neighbours' depth-D descendants j of
+//        V_j . (<F'_ox, F_jx>, <F'_oy, F_jy>, <F'_oz, F_jz>)     (reference Q3)
+__device__ __forceinline__ double div_term(int j, int offD, int nD, int res, int D, int3 fo,
+                                           const int* __restrict__ enc, const float* __restrict__ vf,
+                                           const double* __restrict__ ddot) {
+    int3 fj = decode_func_index(enc[offD + j], D);
+    double s = double(vf[j]) * ddot[fo.x + res * fj.x];
+    s += double(vf[nD + j]) * ddot[fo.y + res * fj.y];
+    s += double(vf[2 * nD + j]) * ddot[fo.z + res * fj.z];
+    return s;
+}
+
+constexpr int kDivWaves = 4;
+__global__ void __launch_bounds__(kDivWaves * PSR_WAVE) divergence_fine(
+    int begin, int count, int M, int offD, int nD, int res, int D, const int* __restrict__ neigh,
+    const int* __restrict__ didx, const int* __restrict__ dnum, const int* __restrict__ enc,
+    const float* __restrict__ vf, const double* __restrict__ ddot, float* __restrict__ div) {
+    __shared__ NbRanges R[kDivWaves];
+    int wid = threadIdx.x / PSR_WAVE, lane = __lane_id();
+    int node = blockIdx.x * kDivWaves + wid;
+    if (node >= count) return;
+    int o = begin + node;
+    NbRanges& r = R[wid];
+    int total = wave_ranges(r, neigh, M, o, lane, [&](int nb, int& first, int& len) {
+        first = didx[nb];
+        len = dnum[nb];
+    });
+    __builtin_amdgcn_wave_barrier();
+    int3 fo = decode_func_index(enc[o], D);
+    double acc = 0;
+    for (int t = lane; t < total; t += PSR_WAVE) {
+        int i = range_of(r, t);
+        acc += div_term(r.first[i] + (t - r.start[i]), offD, nD, res, D, fo, enc, vf, ddot);
+    }
+    for (int off = PSR_WAVE / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, PSR_WAVE);
+    if (lane == 0) div[o] = float(acc);
+}
+
+// Coarse nodes (few, each covering a large share of depth D): a workgroup each.
+constexpr int kDivCoarseThreads = 1024;
+__global__ void __launch_bounds__(kDivCoarseThreads) divergence_coarse(
+    int count, int M, int offD, int nD, int res, int D, const int* __restrict__ neigh, const int* __restrict__ didx,
+    const int* __restrict__ dnum, const int* __restrict__ enc, const float* __restrict__ vf,
+    const double* __restrict__ ddot, float* __restrict__ div) {
+    int o = blockIdx.x;
+    if (o >= count) return;
+    __shared__ int start[28], first[27];
+    __shared__ double part[kDivCoarseThreads / PSR_WAVE];
+    if (threadIdx.x == 0) {
+        int s = 0;
+        for (int i = 0; i < 27; ++i) {
+            int nb = neigh[i * M + o];
+            start[i] = s;
+            first[i] = nb != -1 ? didx[nb] : 0;
+            s += nb != -1 ? dnum[nb] : 0;
+        }
+        start[27] = s;
+    }
+    __syncthreads();
+    int total = start[27];
+    int3 fo = decode_func_index(enc[o], D);
+    double acc = 0;
+    int i = 0;
+    for (int t = threadIdx.x; t < total; t += kDivCoarseThreads) {
+        while (start[i + 1] <= t) ++i;
+        acc += div_term(first[i] + (t - start[i]), offD, nD, res, D, fo, enc, vf, ddot);
+    }
+    for (int off = PSR_WAVE / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, PSR_WAVE);
+    if (__lane_id() == 0) part[threadIdx.x / PSR_WAVE] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0;
+        for (int w = 0; w < kDivCoarseThreads / PSR_WAVE; ++w) s += part[w];
+        div[o] = float(s);
+    }
+}
+
+}  // namespace
+
+void build_tables(Ctx& c) {
+    const int D = c.cfg.depth;
+    const int res = (1 << (D + 1)) - 1;
+    Tables& t = c.tables;
+    t.res = res;
+    size_t n = size_t(res) * res;
+    t.dot = c.buf<double>("table_dot", n);
+    t.ddot = c.buf<double>("table_ddot", n);
+    t.d2dot = c.buf<double>("table_d2dot", n);
+    build_tables_kernel<<<div_up(n, 256), 256, 0, c.stream>>>(res, t.dot, t.ddot, t.d2dot);
+    PSR_HIP_CHECK(hipGetLastError());
+}
+
+void compute_field(Ctx& c) {
+    Octree& o = c.oct;
+    const Levels& L = o.L;
+    const int D = L.D, nD = L.count[D], offD = L.base[D], M = o.M;
+    hipStream_t s = c.stream;
+    c.fld.vfield = c.buf<float>("vfield", size_t(3) * nD);
+    c.fld.div = c.buf<float>("divergence", M);
+    vector_field_kernel<<<div_up(nD, kVfWaves), kVfWaves * PSR_WAVE, 0, s>>>(
+        nD, offD, M, D, o.neigh, o.pidx, o.pnum, o.key, o.spos, o.snrm, c.fld.vfield);
+    // depths 0..kCoarse: one workgroup per node; deeper: one wave per node
+    const int kCoarse = D >= 5 ? 3 : 1;
+    int coarseCount = L.base[kCoarse + 1];
+    divergence_coarse<<<coarseCount, kDivCoarseThreads, 0, s>>>(coarseCount, M, offD, nD, c.tables.res, D, o.neigh,
+                                                                o.didx, o.dnum, o.encIdx, c.fld.vfield,
+                                                                c.tables.ddot, c.fld.div);
+    int fineCount = M - coarseCount;
+    divergence_fine<<<div_up(fineCount, kDivWaves), kDivWaves * PSR_WAVE, 0, s>>>(
+        coarseCount, fineCount, M, offD, nD, c.tables.res, D, o.neigh, o.didx, o.dnum, o.encIdx, c.fld.vfield,
+        c.tables.ddot, c.fld.div);
+    PSR_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace psr

@@ -1,0 +1,135 @@
+// Reconstruction context shared by the stage drivers.
+#pragma once
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/psr.h"
+#include "psr_common.h"
+
+namespace psr {
+
+// Grow-only device arena: one allocation per named buffer, reused across
+// calls; a buffer is reallocated only when a call needs more than it holds.
+class Arena {
+  public:
+    ~Arena() { release(); }
+    void* get(const std::string& name, size_t bytes);
+    void release();
+    size_t bytes_reserved() const { return total_; }
+    int allocations() const { return allocs_; }
+
+  private:
+    struct Block {
+        void* ptr = nullptr;
+        size_t bytes = 0;
+    };
+    std::map<std::string, Block> blocks_;
+    size_t total_ = 0;
+    int allocs_ = 0;
+};
+
+// 1-D inner-product tables for the configured depth, [a + res * b].
+struct Tables {
+    int res = 0;
+    double* dot = nullptr;
+    double* ddot = nullptr;
+    double* d2dot = nullptr;
+};
+
+// Point and node arrays of the octree (structure of arrays).
+struct Octree {
+    int N = 0;
+    Levels L{};
+    int M = 0;
+    int U[PSR_MAX_DEPTH + 1] = {0};  // occupied nodes per level
+    float4* spos = nullptr;          // sorted normalised positions
+    float4* snrm = nullptr;          // sorted scaled normals
+    int* codes = nullptr;            // sorted Morton codes
+    int* order = nullptr;            // sorted -> input index
+    int* point2node = nullptr;       // sorted point -> level-local depth-D node
+    int* key = nullptr;
+    int* pidx = nullptr;
+    int* pnum = nullptr;
+    int* parent = nullptr;
+    int* childBase = nullptr;        // children[k] = childBase + k, or -1
+    int* didx = nullptr;
+    int* dnum = nullptr;
+    int* neigh = nullptr;            // [27][M]
+    int* encIdx = nullptr;
+    int* depth = nullptr;            // depth of each node
+    float* normParams = nullptr;     // corner xyz, edge
+};
+
+struct Field {
+    float* vfield = nullptr;  // [3][nD]
+    float* div = nullptr;     // [M]
+    float* dx = nullptr;      // [M]
+    float* chiPts = nullptr;  // [N]
+    double* isoSum = nullptr; // device scalar
+    float iso = 0.f;
+};
+
+struct Geometry {
+    int* vtxOwner = nullptr;  // compacted vertices: owner node
+    int* vtxKind = nullptr;
+    float4* vtxPos = nullptr;
+    int* edgeOwner = nullptr;
+    int* edgeKind = nullptr;
+    int* faceOwner = nullptr;
+    int* faceKind = nullptr;
+    int* faceHasParent = nullptr;
+    int* nodeVert = nullptr;  // [8][M], 1-based
+    int* nodeEdge = nullptr;  // [12][nD] (depth-D nodes, level-local), 1-based
+    int* nodeFace = nullptr;  // [6][M], 1-based
+    int* counts = nullptr;    // device: V, E, F
+    int V = 0, E = 0, F = 0;
+    float* vvalue = nullptr;
+};
+
+struct Mesh {
+    float* vertices = nullptr;  // xyz
+    int* triangles = nullptr;
+    int nv = 0, nt = 0;
+    int* counts = nullptr;      // device: final vertex / triangle counts
+};
+
+struct Ctx {
+    psr_config cfg{};
+    hipStream_t stream = nullptr;
+    bool ownStream = false;
+    int device = 0;
+    Arena arena;
+    Tables tables;
+    Octree oct;
+    Field fld;
+    Geometry geo;
+    Mesh mesh;
+    psr_stats stats{};
+    int* scratch = nullptr;
+    size_t scratchInts = 0;
+    hipEvent_t ev[16] = {};
+    int* hostPinned = nullptr;   // small pinned buffer for size read-backs
+    template <class T>
+    T* buf(const char* name, size_t count) {
+        return static_cast<T*>(arena.get(name, sizeof(T) * (count ? count : 1)));
+    }
+    int* scratch_for(size_t ints) {
+        if (ints > scratchInts) {
+            scratchInts = ints + ints / 4 + 1024;
+            scratch = static_cast<int*>(arena.get("scratch", scratchInts * sizeof(int)));
+        }
+        return scratch;
+    }
+};
+
+// Stage drivers.
+void build_tables(Ctx& c);
+void build_octree(Ctx& c, const psr_points& pts);
+void compute_field(Ctx& c);
+void solve_depths(Ctx& c);
+void points_implicit(Ctx& c);
+void build_geometry(Ctx& c);
+void extract_mesh(Ctx& c);
+
+}  // namespace psr

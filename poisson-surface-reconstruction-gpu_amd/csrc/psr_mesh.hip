@@ -1,0 +1,478 @@
+// Mesh geometry (vertex / edge / face ownership), vertex implicit values,
+// depth-D marching cubes, face-intersection propagation and subdivision roots.
+#include "psr_mesh_common.h"
+#include "psr_prims.h"
+
+namespace psr {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+// --- ownership (BuildMeshGeometry.cu:42-400) -------------------------------
+// A corner / edge / face of node o is owned by the smallest-key neighbour whose
+// centre lies within one node width of it; the owner emits it once.
+
+struct NodeView {
+    int M, D;
+    const int* key;
+    const int* depth;
+    const int* neigh;
+};
+
+__device__ __forceinline__ float3 node_center(const NodeView& v, int o) {
+    return key_center(v.key[o], v.depth[o], v.D);
+}
+
+__global__ void __launch_bounds__(kThreads) vertex_owner(NodeView v, int* __restrict__ flag) {
+    int o = blockIdx.x * kThreads + threadIdx.x;
+    if (o >= v.M) return;
+    int depth = v.depth[o];
+    float half = 1.0f / float(1 << (depth + 1)), W = 1.0f / float(1 << depth), W2 = W * W;
+    int nb[27];
+    for (int i = 0; i < 27; ++i) nb[i] = v.neigh[i * v.M + o];
+    float3 nc = nb[13] != -1 ? node_center(v, nb[13]) : make_float3(0.f, 0.f, 0.f);
+    int ownKey[8], own[8];
+    for (int i = 0; i < 8; ++i) {
+        ownKey[i] = 0x7fffffff;
+        own[i] = -1;
+    }
+    for (int j = 0; j < 27; ++j) {
+        if (nb[j] == -1) continue;
+        float3 c = node_center(v, nb[j]);
+        int k = v.key[nb[j]];
+        for (int i = 0; i < 8; ++i) {
+            float3 p = corner_pos(nc, i, half);
+            if (sqdist(p, c) < double(W2) && ownKey[i] > k) {
+                ownKey[i] = k;
+                own[i] = nb[j];
+            }
+        }
+    }
+    for (int i = 0; i < 8; ++i) flag[8 * o + i] = (own[i] == o) ? 1 : 0;
+}
+
+__global__ void __launch_bounds__(kThreads) vertex_emit(NodeView v, const int* __restrict__ flag,
+                                                        const int* __restrict__ addr, int* __restrict__ vOwner,
+                                                        int* __restrict__ vKind, float4* __restrict__ vPos) {
+    int t = blockIdx.x * kThreads + threadIdx.x;
+    if (t >= 8 * v.M || !flag[t]) return;
+    int o = t >> 3, i = t & 7;
+    int depth = v.depth[o];
+    float half = 1.0f / float(1 << (depth + 1));
+    int self = v.neigh[13 * v.M + o];
+    float3 nc = self != -1 ? node_center(v, self) : make_float3(0.f, 0.f, 0.f);
+    float3 p = corner_pos(nc, i, half);
+    int a = addr[t];
+    vOwner[a] = o;
+    vKind[a] = i;
+    vPos[a] = make_float4(p.x, p.y, p.z, __int_as_float(depth));
+}
+
+__global__ void __launch_bounds__(kThreads) vertex_maintain(NodeView v, const int* __restrict__ count,
+                                                            const int* __restrict__ vOwner,
+                                                            const float4* __restrict__ vPos,
+                                                            int* __restrict__ nodeVert) {
+    int t = blockIdx.x * kThreads + threadIdx.x;
+    if (t >= *count) return;
+    int owner = vOwner[t];
+    int depth = v.depth[owner];
+    float W = 1.0f / float(1 << depth), W2 = W * W;
+    float4 q = vPos[t];
+    float3 p = make_float3(q.x, q.y, q.z);
+    for (int i = 0; i < 27; ++i) {
+        int nb = v.neigh[i * v.M + owner];
+        if (nb == -1) continue;
+        float3 c = node_center(v, nb);
+        if (!(sqdist(p, c) < double(W2))) continue;
+        nodeVert[corner_slot(c, p) * v.M + nb] = t + 1;
+    }
+}
+
+__global__ void __launch_bounds__(kThreads) edge_owner(NodeView v, int offD, int nD, int* __restrict__ flag) {
+    int t = blockIdx.x * kThreads + threadIdx.x;
+    if (t >= nD) return;
+    int o = offD + t;
+    float half = 1.0f / float(1 << (v.D + 1)), W = 1.0f / float(1 << v.D), W2 = W * W;
+    int nb[27];
+    for (int i = 0; i < 27; ++i) nb[i] = v.neigh[i * v.M + o];
+    float3 nc = nb[13] != -1 ? node_center(v, nb[13]) : make_float3(0.f, 0.f, 0.f);
+    int ownKey[12], own[12];
+    for (int e = 0; e < 12; ++e) {
+        ownKey[e] = 0x7fffffff;
+        own[e] = -1;
+    }
+    for (int j = 0; j < 27; ++j) {
+        if (nb[j] == -1) continue;
+        float3 c = node_center(v, nb[j]);
+        int k = v.key[nb[j]];
+        for (int e = 0; e < 12; ++e) {
+            float3 p = edge_center(nc, e, half);
+            if (sqdist(p, c) < double(W2) && ownKey[e] > k) {
+                ownKey[e] = k;
+                own[e] = nb[j];
+            }
+        }
+    }
+    for (int e = 0; e < 12; ++e) flag[12 * t + e] = (own[e] == o) ? 1 : 0;
+}
+
+__global__ void __launch_bounds__(kThreads) edge_emit(int offD, int nD, const int* __restrict__ flag,
+                                                      const int* __restrict__ addr, int* __restrict__ eOwner,
+                                                      int* __restrict__ eKind) {
+    int t = blockIdx.x * kThreads + threadIdx.x;
+    if (t >= 12 * nD || !flag[t]) return;
+    int a = addr[t];
+    eOwner[a] = offD + t / 12;
+    eKind[a] = t % 12;
+}
+
+__global__ void __launch_bounds__(kThreads) edge_maintain(NodeView v, int offD, const int* __restrict__ count,
+                                                          const int* __restrict__ eOwner,
+                                                          const int* __restrict__ eKind, int nD,
+                                                          int* __restrict__ nodeEdge) {
+    int t = blockIdx.x * kThreads + threadIdx.x;
+    if (t >= *count) return;
+    int owner = eOwner[t], kind = eKind[t];
+    float half = 1.0f / float(1 << (v.D + 1)), W = 1.0f / float(1 << v.D), W2 = W * W;
+    int self = v.neigh[13 * v.M + owner];
+    float3 nc = self != -1 ? node_center(v, self) : make_float3(0.f, 0.f, 0.f);
+    float3 p = edge_center(nc, kind, half);
+    for (int i = 0; i < 27; ++i) {
+        int nb = v.neigh[i * v.M + owner];
+        if (nb == -1) continue;
+        float3 c = node_center(v, nb);
+        if (!(sqdist(p, c) < double(W2))) continue;
+        nodeEdge[edge_slot(c, p, kind) * nD + (nb - offD)] = t + 1;
+    }
+}
+
+__global__ void __launch_bounds__(kThreads) face_owner(NodeView v, const int* __restrict__ parent,
+                                                       int* __restrict__ flag) {
+    int o = blockIdx.x * kThreads + threadIdx.x;
+    if (o >= v.M) return;
+    int depth = v.depth[o];
+    float half = 1.0f / float(1 << (depth + 1)), W = 1.0f / float(1 << depth), W2 = W * W;
+    int nb[27];
+    for (int i = 0; i < 27; ++i) nb[i] = v.neigh[i * v.M + o];
+    float3 nc = nb[13] != -1 ? node_center(v, nb[13]) : make_float3(0.f, 0.f, 0.f);
+    int ownKey[6], own[6];
+    for (int f = 0; f < 6; ++f) {
+        ownKey[f] = 0x7fffffff;
+        own[f] = -1;
+    }
+    for (int j = 0; j < 27; ++j) {
+        if (nb[j] == -1) continue;
+        float3 c = node_center(v, nb[j]);
+        int k = v.key[nb[j]];
+        for (int f = 0; f < 6; ++f) {
+            float3 p = face_center(nc, f, half);
+            if (sqdist(p, c) < double(W2) && ownKey[f] > k) {
+                ownKey[f] = k;
+                own[f] = nb[j];
+            }
+        }
+    }
+    for (int f = 0; f < 6; ++f) flag[6 * o + f] = (own[f] == o) ? 1 : 0;
+}
+
+__global__ void __launch_bounds__(kThreads) face_emit(NodeView v, const int* __restrict__ parent,
+                                                      const int* __restrict__ flag, const int* __restrict__ addr,
+                                                      int* __restrict__ fOwner, int* __restrict__ fKind,
+                                                      int* __restrict__ fHasParent) {
+    int t = blockIdx.x * kThreads + threadIdx.x;
+    if (t >= 6 * v.M || !flag[t]) return;
+    int o = t / 6, f = t % 6;
+    int a = addr[t];
+    fOwner[a] = o;
+    fKind[a] = f;
+    int son = (v.key[o] >> (3 * (v.D - v.depth[o]))) & 7;
+    fHasParent[a] = (parent[o] == -1) ? -1 : (kParentFaceKind[son][f] != -1 ? 1 : -1);
+}
+
+__global__ void __launch_bounds__(kThreads) face_maintain(NodeView v, const int* __restrict__ count,
+                                                          const int* __restrict__ fOwner,
+                                                          const int* __restrict__ fKind, int* __restrict__ nodeFace) {
+    int t = blockIdx.x * kThreads + threadIdx.x;
+    if (t >= *count) return;
+    int owner = fOwner[t], kind = fKind[t], orient = kind >> 1;
+    int depth = v.depth[owner];
+    float half = 1.0f / float(1 << (depth + 1)), W = 1.0f / float(1 << depth), W2 = W * W;
+    int self = v.neigh[13 * v.M + owner];
+    float3 nc = self != -1 ? node_center(v, self) : make_float3(0.f, 0.f, 0.f);
+    float3 p = face_center(nc, kind, half);
+    for (int i = 0; i < 27; ++i) {
+        int nb = v.neigh[i * v.M + owner];
+        if (nb == -1) continue;
+        float3 c = node_center(v, nb);
+        if (!(sqdist(p, c) < double(W2))) continue;
+        float cv = orient == 0 ? c.x : (orient == 1 ? c.y : c.z);
+        float pv = orient == 0 ? p.x : (orient == 1 ? p.y : p.z);
+        nodeFace[((orient << 1) + (cv - pv < 0 ? 1 : 0)) * v.M + nb] = t + 1;
+    }
+}
+
+__global__ void __launch_bounds__(kThreads) zero_ints(int* __restrict__ a, size_t n) {
+    size_t i = size_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (i < n) a[i] = 0;
+}
+
+// --- vertex implicit values (ComputeTriangleIndices.cu:346-399), quirk Q8 --
+__global__ void __launch_bounds__(kThreads) vertex_values(NodeView v, const int* __restrict__ count,
+                                                          const int* __restrict__ vOwner,
+                                                          const int* __restrict__ vKind,
+                                                          const float4* __restrict__ vPos,
+                                                          const int* __restrict__ parent,
+                                                          const int* __restrict__ childBase,
+                                                          const int* __restrict__ enc, const float* __restrict__ dx,
+                                                          const float* __restrict__ iso, float* __restrict__ vv) {
+    int t = blockIdx.x * kThreads + threadIdx.x;
+    if (t >= *count) return;
+    int owner = vOwner[t];
+    float4 q = vPos[t];
+    float3 p = make_float3(q.x, q.y, q.z);
+    int depth = __float_as_int(q.w);
+    float val = 0.f;
+    if (owner > 0) {
+        int now = owner;
+        while (now != -1) {
+            val += chi_node_sum(now, p, v.M, v.D, v.neigh, enc, dx);
+            now = parent[now];
+        }
+        // descent into the key-ordered child picked by the Bourke corner id
+        int ex = kChildrenVertexKind[vKind[t]];
+        now = owner;
+        while (depth < v.D) {
+            ++depth;
+            int cb = childBase[now];
+            if (cb < 0) break;
+            now = cb + ex;
+            val += chi_node_sum(now, p, v.M, v.D, v.neigh, enc, dx);
+        }
+    }
+    vv[t] = val - *iso;
+}
+
+// --- depth-D marching cubes (ComputeTriangleIndices.cu:401-526) ------------
+__global__ void __launch_bounds__(kThreads) edge_crossings(const int* __restrict__ count, int M,
+                                                           const int* __restrict__ eOwner,
+                                                           const int* __restrict__ eKind,
+                                                           const int* __restrict__ nodeVert,
+                                                           const float* __restrict__ vv, int* __restrict__ flag) {
+    int t = blockIdx.x * kThreads + threadIdx.x;
+    if (t >= *count) return;
+    int owner = eOwner[t], kind = eKind[t];
+    int v1 = nodeVert[kEdgeVertex[kind][0] * M + owner] - 1;
+    int v2 = nodeVert[kEdgeVertex[kind][1] * M + owner] - 1;
+    flag[t] = (vv[v1] * vv[v2] <= 0.f) ? 1 : 0;
+}
+
+__global__ void __launch_bounds__(kThreads) crossing_points(const int* __restrict__ count, int M,
+                                                            const int* __restrict__ eOwner,
+                                                            const int* __restrict__ eKind,
+                                                            const int* __restrict__ nodeVert,
+                                                            const float* __restrict__ vv,
+                                                            const float4* __restrict__ vPos,
+                                                            const int* __restrict__ flag,
+                                                            const int* __restrict__ addr,
+                                                            float* __restrict__ out) {
+    int t = blockIdx.x * kThreads + threadIdx.x;
+    if (t >= *count || !flag[t]) return;
+    int owner = eOwner[t], kind = eKind[t];
+    int v1 = nodeVert[kEdgeVertex[kind][0] * M + owner] - 1;
+    int v2 = nodeVert[kEdgeVertex[kind][1] * M + owner] - 1;
+    float4 a = vPos[v1], b = vPos[v2];
+    float3 p = interpolate_point(make_float3(a.x, a.y, a.z), make_float3(b.x, b.y, b.z), kind >> 2, vv[v1], vv[v2]);
+    int k = addr[t];
+    out[3 * k] = p.x;
+    out[3 * k + 1] = p.y;
+    out[3 * k + 2] = p.z;
+}
+
+__global__ void __launch_bounds__(kThreads) cube_classes(int offD, int nD, int M, const int* __restrict__ nodeVert,
+                                                         const float* __restrict__ vv, int* __restrict__ cube,
+                                                         int* __restrict__ triNum) {
+    int t = blockIdx.x * kThreads + threadIdx.x;
+    if (t >= nD) return;
+    int o = offD + t, c = 0;
+    for (int i = 0; i < 8; ++i)
+        if (vv[nodeVert[i * M + o] - 1] < 0.f) c |= 1 << i;
+    cube[t] = c;
+    triNum[t] = mc_tri_count(c);
+}
+
+__global__ void __launch_bounds__(kThreads) emit_triangles_and_faces(
+    int offD, int nD, int M, const int* __restrict__ cube, const int* __restrict__ triAddr,
+    const int* __restrict__ nodeEdge, const int* __restrict__ vexAddr, const int* __restrict__ nodeFace,
+    const int* __restrict__ parent, const int* __restrict__ faceHasParent, int* __restrict__ tri,
+    int* __restrict__ hasInter) {
+    int t = blockIdx.x * kThreads + threadIdx.x;
+    if (t >= nD) return;
+    int o = offD + t, c = cube[t];
+    uint64_t w = mc_word(c);
+    int nt = mc_tri_count(c), base = triAddr[t];
+    int edgeHas = 0;
+    for (int k = 0; k < 3 * nt; ++k) {
+        int e = int((w >> (4 * k)) & 0xF);
+        edgeHas |= 1 << e;
+        tri[3 * base + k] = vexAddr[nodeEdge[e * nD + t] - 1];
+    }
+    for (int f = 0; f < 6; ++f) {
+        bool mark = false;
+        for (int j = 0; j < 4; ++j) mark |= (edgeHas >> kFaceEdges[f][j]) & 1;
+        if (!mark) continue;
+        int pn = parent[o];
+        int cf = nodeFace[f * M + o] - 1;
+        hasInter[cf] = 1;
+        while (faceHasParent[cf] != -1) {
+            cf = nodeFace[f * M + pn] - 1;
+            pn = parent[pn];
+            hasInter[cf] = 1;
+        }
+    }
+}
+
+// --- subdivision roots (ProcessLeafNodesAtOtherDepth :555-587, copy_if) -----
+__global__ void __launch_bounds__(kThreads) root_flags(int count, int M, const int* __restrict__ nodeVert,
+                                                       const int* __restrict__ nodeFace,
+                                                       const int* __restrict__ childBase,
+                                                       const float* __restrict__ vv,
+                                                       const int* __restrict__ hasInter, int* __restrict__ flag) {
+    int o = blockIdx.x * kThreads + threadIdx.x;
+    if (o >= count) return;
+    float v0 = vv[nodeVert[o] - 1];
+    float sign = v0 < 0.f ? -1.f : 1.f;
+    int hasTri = 0;
+    for (int i = 1; i < 8; ++i)
+        if (sign * vv[nodeVert[i * M + o] - 1] < 0.f) {
+            hasTri = 1;
+            break;
+        }
+    int hi = 0;
+    for (int f = 0; f < 6; ++f)
+        if (hasInter[nodeFace[f * M + o] - 1]) {
+            hi = 1;
+            break;
+        }
+    flag[o] = (childBase[o] < 0 && (hasTri || hi)) ? 1 : 0;
+}
+
+__global__ void __launch_bounds__(kThreads) root_emit(int count, int M, const int* __restrict__ flag,
+                                                      const int* __restrict__ addr, const int* __restrict__ neigh,
+                                                      const int* __restrict__ depth, int* __restrict__ roots,
+                                                      int* __restrict__ depthHist) {
+    int o = blockIdx.x * kThreads + threadIdx.x;
+    if (o >= count || !flag[o]) return;
+    roots[addr[o]] = o;
+    int self = neigh[13 * M + o];  // precomputeSubdivideDepth reads the depth through neighs[13]
+    atomicAdd(&depthHist[depth[self >= 0 ? self : o]], 1);
+}
+
+}  // namespace
+
+void build_geometry(Ctx& c) {
+    Octree& o = c.oct;
+    Geometry& g = c.geo;
+    const Levels& L = o.L;
+    const int M = o.M, D = L.D, nD = L.count[D], offD = L.base[D];
+    hipStream_t s = c.stream;
+    NodeView v{M, D, o.key, o.depth, o.neigh};
+    size_t nMax = std::max<size_t>(size_t(8) * M, size_t(12) * nD);
+    int* flag = c.buf<int>("geo_flag", nMax);
+    int* addr = c.buf<int>("geo_addr", nMax);
+    int* scratch = c.scratch_for(scan_scratch_ints(nMax) + 64);
+    g.counts = c.buf<int>("geo_counts", 4);
+    g.vtxOwner = c.buf<int>("vtx_owner", size_t(8) * M);
+    g.vtxKind = c.buf<int>("vtx_kind", size_t(8) * M);
+    g.vtxPos = c.buf<float4>("vtx_pos", size_t(8) * M);
+    g.nodeVert = c.buf<int>("node_vert", size_t(8) * M);
+    g.nodeFace = c.buf<int>("node_face", size_t(6) * M);
+    g.nodeEdge = c.buf<int>("node_edge", size_t(12) * nD);
+    zero_ints<<<div_up(size_t(8) * M, kThreads), kThreads, 0, s>>>(g.nodeVert, size_t(8) * M);
+    zero_ints<<<div_up(size_t(6) * M, kThreads), kThreads, 0, s>>>(g.nodeFace, size_t(6) * M);
+    zero_ints<<<div_up(size_t(12) * nD, kThreads), kThreads, 0, s>>>(g.nodeEdge, size_t(12) * nD);
+    // vertices (all depths)
+    vertex_owner<<<div_up(M, kThreads), kThreads, 0, s>>>(v, flag);
+    scan_exclusive(flag, addr, 8 * M, g.counts + 0, scratch, s);
+    vertex_emit<<<div_up(size_t(8) * M, kThreads), kThreads, 0, s>>>(v, flag, addr, g.vtxOwner, g.vtxKind, g.vtxPos);
+    vertex_maintain<<<div_up(size_t(8) * M, kThreads), kThreads, 0, s>>>(v, g.counts + 0, g.vtxOwner, g.vtxPos,
+                                                                        g.nodeVert);
+    // edges (depth D)
+    g.edgeOwner = c.buf<int>("edge_owner", size_t(12) * nD);
+    g.edgeKind = c.buf<int>("edge_kind", size_t(12) * nD);
+    edge_owner<<<div_up(nD, kThreads), kThreads, 0, s>>>(v, offD, nD, flag);
+    scan_exclusive(flag, addr, 12 * nD, g.counts + 1, scratch, s);
+    edge_emit<<<div_up(size_t(12) * nD, kThreads), kThreads, 0, s>>>(offD, nD, flag, addr, g.edgeOwner, g.edgeKind);
+    edge_maintain<<<div_up(size_t(12) * nD, kThreads), kThreads, 0, s>>>(v, offD, g.counts + 1, g.edgeOwner,
+                                                                        g.edgeKind, nD, g.nodeEdge);
+    // faces (all depths)
+    g.faceOwner = c.buf<int>("face_owner", size_t(6) * M);
+    g.faceKind = c.buf<int>("face_kind", size_t(6) * M);
+    g.faceHasParent = c.buf<int>("face_hasparent", size_t(6) * M);
+    face_owner<<<div_up(M, kThreads), kThreads, 0, s>>>(v, o.parent, flag);
+    scan_exclusive(flag, addr, 6 * M, g.counts + 2, scratch, s);
+    face_emit<<<div_up(size_t(6) * M, kThreads), kThreads, 0, s>>>(v, o.parent, flag, addr, g.faceOwner, g.faceKind,
+                                                                   g.faceHasParent);
+    face_maintain<<<div_up(size_t(6) * M, kThreads), kThreads, 0, s>>>(v, g.counts + 2, g.faceOwner, g.faceKind,
+                                                                      g.nodeFace);
+    // vertex implicit values
+    float* isoDev = c.buf<float>("iso_value", 1);
+    g.vvalue = c.buf<float>("vvalue", size_t(8) * M);
+    vertex_values<<<div_up(size_t(8) * M, kThreads), kThreads, 0, s>>>(v, g.counts + 0, g.vtxOwner, g.vtxKind,
+                                                                       g.vtxPos, o.parent, o.childBase, o.encIdx,
+                                                                       c.fld.dx, isoDev, g.vvalue);
+    PSR_HIP_CHECK(hipMemcpyAsync(c.hostPinned, g.counts, sizeof(int) * 3, hipMemcpyDeviceToHost, s));
+    PSR_HIP_CHECK(hipStreamSynchronize(s));
+    g.V = c.hostPinned[0];
+    g.E = c.hostPinned[1];
+    g.F = c.hostPinned[2];
+    PSR_HIP_CHECK(hipGetLastError());
+}
+
+void depth_mc(Ctx& c, MainMc& mm, float* rawV, int* rawT) {
+    Octree& o = c.oct;
+    Geometry& g = c.geo;
+    const Levels& L = o.L;
+    const int M = o.M, D = L.D, nD = L.count[D], offD = L.base[D];
+    hipStream_t s = c.stream;
+    const int E = g.E, F = g.F;
+    int* scratch = c.scratch_for(scan_scratch_ints(std::max<size_t>(size_t(E), size_t(M))) + 64);
+    mm.counts = c.buf<int>("mc_counts", PSR_MAX_DEPTH + 8);
+    zero_ints<<<1, kThreads, 0, s>>>(mm.counts, PSR_MAX_DEPTH + 8);
+    int* cflag = c.buf<int>("mc_cross_flag", E);
+    mm.vexAddr = c.buf<int>("mc_vex_addr", E);
+    edge_crossings<<<div_up(E, kThreads), kThreads, 0, s>>>(g.counts + 1, M, g.edgeOwner, g.edgeKind, g.nodeVert,
+                                                            g.vvalue, cflag);
+    scan_exclusive(cflag, mm.vexAddr, E, mm.counts + 0, scratch, s);
+    crossing_points<<<div_up(E, kThreads), kThreads, 0, s>>>(g.counts + 1, M, g.edgeOwner, g.edgeKind, g.nodeVert,
+                                                             g.vvalue, g.vtxPos, cflag, mm.vexAddr, rawV);
+    mm.cube = c.buf<int>("mc_cube", nD);
+    int* triNum = c.buf<int>("mc_trinum", nD);
+    mm.triAddr = c.buf<int>("mc_triaddr", nD);
+    cube_classes<<<div_up(nD, kThreads), kThreads, 0, s>>>(offD, nD, M, g.nodeVert, g.vvalue, mm.cube, triNum);
+    scan_exclusive(triNum, mm.triAddr, nD, mm.counts + 1, scratch, s);
+    mm.hasInter = c.buf<int>("mc_hasinter", F);
+    zero_ints<<<div_up(F, kThreads), kThreads, 0, s>>>(mm.hasInter, size_t(F));
+    emit_triangles_and_faces<<<div_up(nD, kThreads), kThreads, 0, s>>>(offD, nD, M, mm.cube, mm.triAddr, g.nodeEdge,
+                                                                       mm.vexAddr, g.nodeFace, o.parent,
+                                                                       g.faceHasParent, rawT, mm.hasInter);
+    // subdivision roots: leaves above depth D with a sign change or a marked face
+    int* rflag = c.buf<int>("root_flag", offD);
+    int* raddr = c.buf<int>("root_addr", offD);
+    mm.roots = c.buf<int>("roots", offD);
+    root_flags<<<div_up(offD, kThreads), kThreads, 0, s>>>(offD, M, g.nodeVert, g.nodeFace, o.childBase, g.vvalue,
+                                                           mm.hasInter, rflag);
+    scan_exclusive(rflag, raddr, offD, mm.counts + 2, scratch, s);
+    root_emit<<<div_up(offD, kThreads), kThreads, 0, s>>>(offD, M, rflag, raddr, o.neigh, o.depth, mm.roots,
+                                                          mm.counts + 4);
+    PSR_HIP_CHECK(hipMemcpyAsync(c.hostPinned, mm.counts, sizeof(int) * (PSR_MAX_DEPTH + 8), hipMemcpyDeviceToHost,
+                                 s));
+    PSR_HIP_CHECK(hipStreamSynchronize(s));
+    mm.nCross = c.hostPinned[0];
+    mm.nTri = c.hostPinned[1];
+    mm.nRoots = c.hostPinned[2];
+    for (int d = 0; d <= D; ++d) mm.rootsPerDepth[d] = c.hostPinned[4 + d];
+    PSR_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace psr

@@ -1,0 +1,126 @@
+// Geometry helpers and tables shared by the mesh and subdivision kernels.
+#pragma once
+#include "psr_internal.h"
+
+namespace psr {
+
+// Bourke corner order of OctNode.vertices (BuildMeshGeometry.cu:133-144).
+static __constant__ int kEdgeVertex[12][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {0, 3}, {1, 2},
+                                              {4, 7}, {5, 6}, {0, 4}, {1, 5}, {3, 7}, {2, 6}};
+static __constant__ int kFaceEdges[6][4] = {{4, 6, 8, 10}, {5, 7, 9, 11}, {0, 2, 8, 9},
+                                            {1, 3, 10, 11}, {0, 1, 4, 5}, {2, 3, 6, 7}};
+// Child picked by the vertex-value descent for a Bourke corner id
+// (ComputeTriangleIndices.cu:31; applied to key-ordered children, quirk Q8).
+static __constant__ int kChildrenVertexKind[8] = {0, 1, 3, 2, 4, 5, 7, 6};
+// BuildMeshGeometry.cu:19-26, indexed by the key-ordered son key as the
+// reference does.
+static __constant__ int kParentFaceKind[8][6] = {{0, -1, 2, -1, 4, -1}, {-1, 1, 2, -1, 4, -1},
+                                                 {0, -1, -1, 3, 4, -1}, {-1, 1, -1, 3, 4, -1},
+                                                 {0, -1, 2, -1, -1, 5}, {-1, 1, 2, -1, -1, 5},
+                                                 {0, -1, -1, 3, -1, 5}, {0, 1, -1, 3, -1, 5}};
+static __constant__ uint64_t kMcWords[256] = {
+#include "psr_mc_table_body.inc"
+};
+
+__device__ __forceinline__ uint64_t mc_word(int cube) { return kMcWords[cube]; }
+__device__ __forceinline__ int mc_tri_count(int cube) {
+    uint64_t w = kMcWords[cube];
+    int n = 0;
+    while (n < 15 && ((w >> (4 * n)) & 0xF) != 0xF) ++n;
+    return n / 3;
+}
+
+__device__ __forceinline__ double sqdist(float3 a, float3 b) {
+    float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+    return double(dx * dx + dy * dy + dz * dz);
+}
+
+// Corner i of the node centred at c: bit0 -> x, bit1 -> y, bit2 -> z.
+__device__ __forceinline__ float3 corner_pos(float3 c, int i, float half) {
+    return make_float3(c.x + float(2 * (i & 1) - 1) * half, c.y + float(2 * ((i >> 1) & 1) - 1) * half,
+                       c.z + float(2 * ((i >> 2) & 1) - 1) * half);
+}
+
+// Centre of edge `kind` = 4 * axis + 2-bit offset of the other two axes.
+__device__ __forceinline__ float3 edge_center(float3 c, int kind, float half) {
+    int orient = kind >> 2, o0 = kind & 1, o1 = (kind >> 1) & 1;
+    float m[3];
+    int dim = 0;
+    for (int a = 0; a < 3; ++a) {
+        if (a == orient) {
+            m[a] = 0.f;
+        } else {
+            m[a] = float(2 * (dim == 0 ? o0 : o1) - 1);
+            ++dim;
+        }
+    }
+    return make_float3(c.x + m[0] * half, c.y + m[1] * half, c.z + m[2] * half);
+}
+
+__device__ __forceinline__ float3 face_center(float3 c, int kind, float half) {
+    float d = float(2 * (kind & 1) - 1) * half;
+    int o = kind >> 1;
+    return make_float3(c.x + (o == 0 ? d : 0.f), c.y + (o == 1 ? d : 0.f), c.z + (o == 2 ? d : 0.f));
+}
+
+// Bourke slot of the vertex at p within the node centred at c
+// (maintainVertexNodePointer, BuildMeshGeometry.cu:133-144).
+__device__ __forceinline__ int corner_slot(float3 c, float3 p) {
+    int index = 0;
+    if (c.x - p.x < 0) index |= 1;
+    if (c.z - p.z < 0) index |= 4;
+    if (c.y - p.y < 0) index += (index & 1) ? 1 : 3;
+    return index;
+}
+
+// Edge slot within the node centred at c (maintainEdgeNodePointer, :264-277).
+__device__ __forceinline__ int edge_slot(float3 c, float3 p, int kind) {
+    int orient = kind >> 2, index = orient << 2, dim = 0;
+    float cc[3] = {c.x, c.y, c.z}, pp[3] = {p.x, p.y, p.z};
+    for (int a = 0; a < 3; ++a)
+        if (a != orient) {
+            if (cc[a] - pp[a] < 0) index |= 1 << dim;
+            ++dim;
+        }
+    return index;
+}
+
+// Linear crossing along axis dim (ComputeTriangleIndices.cu:464-474).
+__device__ __forceinline__ float3 interpolate_point(float3 p1, float3 p2, int dim, float v1, float v2) {
+    float pivot = v1 / (v1 - v2), another = 1.f - pivot;
+    float a[3] = {p1.x, p1.y, p1.z}, b[3] = {p2.x, p2.y, p2.z};
+    a[dim] = b[dim] * pivot + a[dim] * another;
+    return make_float3(a[0], a[1], a[2]);
+}
+
+// sum over the 27 neighbours of `node` of dx * F(x) F(y) F(z)
+__device__ __forceinline__ float chi_node_sum(int node, float3 p, int M, int D, const int* __restrict__ neigh,
+                                              const int* __restrict__ enc, const float* __restrict__ dx) {
+    float v = 0.f;
+    for (int i = 0; i < 27; ++i) {
+        int nb = neigh[i * M + node];
+        if (nb == -1) continue;
+        int3 f = decode_func_index(enc[nb], D);
+        v += dx[nb] * basis_value(f.x, p.x) * basis_value(f.y, p.y) * basis_value(f.z, p.z);
+    }
+    return v;
+}
+
+// Per-call state of the depth-D marching cubes.
+struct MainMc {
+    int* vexAddr = nullptr;   // crossing index per depth-D edge
+    int* cube = nullptr;      // cube category per depth-D node
+    int* triAddr = nullptr;   // first triangle per depth-D node
+    int* hasInter = nullptr;  // surface-edge intersection flag per face
+    int* roots = nullptr;     // subdivision roots, ascending node index
+    int* counts = nullptr;    // device: [0] crossings, [1] triangles, [2] roots, [4..] roots per depth
+    int nCross = 0, nTri = 0, nRoots = 0;
+    int rootsPerDepth[PSR_MAX_DEPTH + 1] = {0};
+};
+
+// Depth-D marching cubes into the raw mesh buffers (vertices at [0, nCross),
+// triangles at [0, nTri)), then the subdivision roots.  Ends with one
+// read-back of the counts.
+void depth_mc(Ctx& c, MainMc& mm, float* rawV, int* rawT);
+
+}  // namespace psr

@@ -1,0 +1,475 @@
+// Octree construction: normalisation, Morton codes, sort, node arrays,
+// neighbour tables, packed function index.
+//
+// Same node arrays as the reference's level-by-level build
+// (mesh/BuildOctree.cu:135-592, mesh/BuildOctree.cpp:75-194), computed without
+// a per-level host loop: every sorted point knows the first level at which
+// its code differs from its predecessor, one multi-level counting scan gives
+// the rank of each occupied node at every level, and each node's final record
+// is then written directly.
+#include "psr_internal.h"
+#include "psr_prims.h"
+
+namespace psr {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+// --- bounding box (BuildOctree.cu:51-133) and host adjustment (:619-631) ---
+__global__ void __launch_bounds__(kThreads) bbox_partial(const char* __restrict__ xyz, unsigned stride, int n,
+                                                         float* __restrict__ part) {
+    float mn[3] = {1e6f, 1e6f, 1e6f}, mx[3] = {-1e6f, -1e6f, -1e6f};
+    for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
+        const float* p = reinterpret_cast<const float*>(xyz + size_t(i) * stride);
+        for (int a = 0; a < 3; ++a) {
+            float v = p[a];
+            mx[a] = mx[a] < v ? v : mx[a];
+            mn[a] = mn[a] > v ? v : mn[a];
+        }
+    }
+    __shared__ float s[6][kThreads];
+    for (int a = 0; a < 3; ++a) {
+        s[a][threadIdx.x] = mx[a];
+        s[3 + a][threadIdx.x] = mn[a];
+    }
+    __syncthreads();
+    for (int st = kThreads / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < st)
+            for (int a = 0; a < 3; ++a) {
+                float l = s[a][threadIdx.x], r = s[a][threadIdx.x + st];
+                s[a][threadIdx.x] = l < r ? r : l;
+                l = s[3 + a][threadIdx.x];
+                r = s[3 + a][threadIdx.x + st];
+                s[3 + a][threadIdx.x] = l > r ? r : l;
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = s[threadIdx.x][0];
+}
+
+__global__ void bbox_final(const float* __restrict__ part, int nb, float* __restrict__ params) {
+#pragma clang fp contract(off)
+    if (threadIdx.x != 0) return;
+    float mx[3] = {-1e6f, -1e6f, -1e6f}, mn[3] = {1e6f, 1e6f, 1e6f};
+    for (int b = 0; b < nb; ++b)
+        for (int a = 0; a < 3; ++a) {
+            float v = part[b * 6 + a];
+            mx[a] = mx[a] < v ? v : mx[a];
+            v = part[b * 6 + 3 + a];
+            mn[a] = mn[a] > v ? v : mn[a];
+        }
+    float maxEdge = 1.0f, center[3];
+    for (int a = 0; a < 3; ++a) {
+        float ext = __fsub_rn(mx[a], mn[a]);
+        if (a == 0 || maxEdge < ext) maxEdge = ext;
+        center[a] = __fdiv_rn(__fadd_rn(mx[a], mn[a]), 2.0f);
+    }
+    maxEdge = __fmul_rn(maxEdge, 1.25f);
+    for (int a = 0; a < 3; ++a) params[a] = __fsub_rn(center[a], __fdiv_rn(maxEdge, 2.0f));
+    params[3] = maxEdge;
+}
+
+// --- normalise + Morton code (BuildOctree.cu:144-208) ---
+__global__ void __launch_bounds__(kThreads) normalize_encode(const char* __restrict__ xyz, unsigned xs,
+                                                             const char* __restrict__ nrm, unsigned ns, int n,
+                                                             int D, const float* __restrict__ params,
+                                                             float4* __restrict__ posOut,
+                                                             float4* __restrict__ nrmOut,
+                                                             unsigned* __restrict__ keys,
+                                                             unsigned* __restrict__ vals) {
+#pragma clang fp contract(off)
+    int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const float* p = reinterpret_cast<const float*>(xyz + size_t(i) * xs);
+    const float* q = reinterpret_cast<const float*>(nrm + size_t(i) * ns);
+    float edge = params[3];
+    float x = __fdiv_rn(__fsub_rn(p[0], params[0]), edge);
+    float y = __fdiv_rn(__fsub_rn(p[1], params[1]), edge);
+    float z = __fdiv_rn(__fsub_rn(p[2], params[2]), edge);
+    float nx = q[0], ny = q[1], nz = q[2];
+    float len = __fsqrt_rn(__fadd_rn(__fadd_rn(__fmul_rn(nx, nx), __fmul_rn(ny, ny)), __fmul_rn(nz, nz)));
+    if (double(len) > double(1e-6f)) len = __fdiv_rn(1.0f, len);
+    len = __fmul_rn(len, float(2 << D));
+    posOut[i] = make_float4(x, y, z, 0.f);
+    nrmOut[i] = make_float4(__fmul_rn(nx, len), __fmul_rn(ny, len), __fmul_rn(nz, len), 0.f);
+    unsigned code = 0;
+    float cx = 0.5f, cy = 0.5f, cz = 0.5f, w = 0.25f;
+    for (int l = D - 1; l >= 0; --l) {
+        if (x > cx) { code |= 4u << (3 * l); cx += w; } else { cx -= w; }
+        if (y > cy) { code |= 2u << (3 * l); cy += w; } else { cy -= w; }
+        if (z > cz) { code |= 1u << (3 * l); cz += w; } else { cz -= w; }
+        w *= 0.5f;
+    }
+    keys[i] = code;
+    vals[i] = unsigned(i);
+}
+
+__global__ void __launch_bounds__(kThreads) gather_points(const unsigned* __restrict__ perm,
+                                                          const float4* __restrict__ posIn,
+                                                          const float4* __restrict__ nrmIn, int n,
+                                                          float4* __restrict__ posOut, float4* __restrict__ nrmOut) {
+    int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    unsigned o = perm[i];
+    posOut[i] = posIn[o];
+    nrmOut[i] = nrmIn[o];
+}
+
+// --- level ranks -----------------------------------------------------------
+// lvl(i): first level 1..D at which the code of sorted point i differs from
+// point i-1; D+1 when both lie in the same leaf; 1 for i = 0.
+__device__ __forceinline__ int first_level(const unsigned* codes, int i, int D) {
+    if (i == 0) return 1;
+    unsigned x = codes[i] ^ codes[i - 1];
+    if (x == 0) return D + 1;
+    int hb = 31 - __clz(int(x));  // highest differing bit
+    return D - hb / 3;           // bit 3(D-l)+k belongs to level l
+}
+
+constexpr int kRankItems = 8;
+constexpr int kRankTile = kThreads * kRankItems;
+
+__global__ void __launch_bounds__(kThreads) rank_tile_hist(const unsigned* __restrict__ codes, int n, int D,
+                                                           int tiles, int* __restrict__ hist) {
+    __shared__ int h[PSR_MAX_DEPTH + 2];
+    if (threadIdx.x < PSR_MAX_DEPTH + 2) h[threadIdx.x] = 0;
+    __syncthreads();
+    int base = blockIdx.x * kRankTile;
+    for (int k = 0; k < kRankItems; ++k) {
+        int i = base + k * kThreads + threadIdx.x;
+        if (i < n) atomicAdd(&h[first_level(codes, i, D)], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x <= D) hist[threadIdx.x * tiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// One workgroup per level d: exclusive offsets of the tiles' new-node counts.
+__global__ void __launch_bounds__(1024) rank_tile_scan(const int* __restrict__ hist, int tiles, int D,
+                                                       int* __restrict__ tileOff, int* __restrict__ U) {
+    int d = blockIdx.x + 1;
+    __shared__ int run;
+    if (threadIdx.x == 0) run = 0;
+    __syncthreads();
+    for (int t0 = 0; t0 < tiles; t0 += 1024) {
+        int t = t0 + threadIdx.x;
+        int c = 0;
+        if (t < tiles)
+            for (int l = 1; l <= d; ++l) c += hist[l * tiles + t];
+        // block exclusive scan
+        __shared__ int sbuf[1024];
+        sbuf[threadIdx.x] = c;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            int v = threadIdx.x >= off ? sbuf[threadIdx.x - off] : 0;
+            __syncthreads();
+            sbuf[threadIdx.x] += v;
+            __syncthreads();
+        }
+        if (t < tiles) tileOff[d * tiles + t] = run + sbuf[threadIdx.x] - c;
+        __syncthreads();
+        if (threadIdx.x == 1023) run += sbuf[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) U[d] = run;
+}
+
+struct LevelOffsets {
+    int lo[PSR_MAX_DEPTH + 2];  // start of level d in the per-level node lists
+};
+
+// Per point: rank of its node at every level.  New nodes record their first
+// point and their parent's rank; every point records its level-(D-1) rank and
+// its level-local depth-D slot.
+__global__ void __launch_bounds__(kThreads) rank_points(const unsigned* __restrict__ codes, int n, int D, int tiles,
+                                                        const int* __restrict__ tileOff, LevelOffsets lo,
+                                                        int* __restrict__ firstPoint, int* __restrict__ parentRank,
+                                                        int* __restrict__ rankDm1, int* __restrict__ point2node) {
+    int base = blockIdx.x * kRankTile + threadIdx.x * kRankItems;
+    int lv[kRankItems];
+    int cnt[PSR_MAX_DEPTH + 1];
+    for (int d = 1; d <= D; ++d) cnt[d] = 0;
+    for (int k = 0; k < kRankItems; ++k) {
+        int i = base + k;
+        lv[k] = (i < n) ? first_level(codes, i, D) : PSR_MAX_DEPTH + 2;
+        for (int d = 1; d <= D; ++d) cnt[d] += (lv[k] <= d);
+    }
+    // block exclusive scan per level
+    __shared__ int wsum[kThreads / PSR_WAVE];
+    int excl[PSR_MAX_DEPTH + 1];
+    int lane = __lane_id(), wid = threadIdx.x / PSR_WAVE;
+    for (int d = 1; d <= D; ++d) {
+        int v = cnt[d], inc = v;
+        for (int off = 1; off < PSR_WAVE; off <<= 1) {
+            int t = __shfl_up(inc, off, PSR_WAVE);
+            if (lane >= off) inc += t;
+        }
+        if (lane == PSR_WAVE - 1) wsum[wid] = inc;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int r = 0;
+            for (int w = 0; w < kThreads / PSR_WAVE; ++w) {
+                int sw = wsum[w];
+                wsum[w] = r;
+                r += sw;
+            }
+        }
+        __syncthreads();
+        excl[d] = tileOff[d * tiles + blockIdx.x] + wsum[wid] + inc - v;
+        __syncthreads();
+    }
+    for (int k = 0; k < kRankItems; ++k) {
+        int i = base + k;
+        if (i >= n) break;
+        unsigned code = codes[i];
+        int rank[PSR_MAX_DEPTH + 1];
+        rank[0] = 0;
+        for (int d = 1; d <= D; ++d) {
+            if (lv[k] <= d) {
+                int q = excl[d];
+                firstPoint[lo.lo[d] + q] = i;
+                parentRank[lo.lo[d] + q] = rank[d - 1];
+                excl[d] = q + 1;
+            }
+            rank[d] = excl[d] - 1;
+        }
+        rankDm1[i] = rank[D - 1];
+        point2node[i] = 8 * rank[D - 1] + int(code & 7u);
+    }
+}
+
+// Writes the record of every occupied node (levels 1..D).
+__global__ void __launch_bounds__(kThreads) fill_occupied(const unsigned* __restrict__ codes, int n, Levels L,
+                                                          LevelOffsets lo, const int* __restrict__ U,
+                                                          const int* __restrict__ firstPoint,
+                                                          const int* __restrict__ parentRank,
+                                                          const int* __restrict__ rankDm1, int total,
+                                                          int* __restrict__ key, int* __restrict__ pidx,
+                                                          int* __restrict__ pnum, int* __restrict__ parent,
+                                                          int* __restrict__ childBase, int* __restrict__ didx,
+                                                          int* __restrict__ dnum) {
+    int t = blockIdx.x * kThreads + threadIdx.x;
+    if (t >= total) return;
+    const int D = L.D;
+    int d = 1;
+    while (d < D && t >= lo.lo[d + 1]) ++d;
+    int q = t - lo.lo[d];
+    int first = firstPoint[t];
+    int next = (q + 1 < U[d]) ? firstPoint[t + 1] : n;
+    unsigned code = codes[first];
+    int sh = 3 * (D - d);
+    int son = int((code >> sh) & 7u);
+    int pr = parentRank[t];
+    int local = 8 * pr + son;
+    int slot = L.base[d] + local;
+    key[slot] = int((code >> sh) << sh);
+    pidx[slot] = first;
+    pnum[slot] = next - first;
+    int par;
+    if (d == 1) {
+        par = 0;
+    } else if (local == 0) {
+        par = L.base[d - 1];  // Q1: level-local child 0 reads as "no child" (BuildOctree.cu:384-389, 412)
+    } else {
+        int sonUp = int((code >> (sh + 3)) & 7u);
+        int ppr = (d >= 3) ? parentRank[lo.lo[d - 1] + pr] : 0;
+        par = L.base[d - 1] + 8 * ppr + sonUp;
+    }
+    parent[slot] = par;
+    childBase[slot] = (d < D) ? L.base[d + 1] + 8 * q : -1;
+    if (d == D) {
+        didx[slot] = local;
+        dnum[slot] = 1;
+    } else {
+        didx[slot] = 8 * rankDm1[first];
+        dnum[slot] = 8 * (rankDm1[next - 1] - rankDm1[first] + 1);
+    }
+}
+
+// Completes each block of 8 siblings: empty slots and the shared parent
+// (updateEmptyNodeInfo, BuildOctree.cu:426-488).  Block b of level d holds the
+// children of the b-th occupied node of level d-1.
+__global__ void __launch_bounds__(kThreads) fill_blocks(Levels L, int totalBlocks, int N, int nDm1,
+                                                        int* __restrict__ key, int* __restrict__ pidx,
+                                                        int* __restrict__ pnum, int* __restrict__ parent,
+                                                        int* __restrict__ childBase, int* __restrict__ didx,
+                                                        int* __restrict__ dnum) {
+    int t = blockIdx.x * kThreads + threadIdx.x;
+    if (t > totalBlocks) return;
+    const int D = L.D;
+    if (t == totalBlocks) {  // root
+        key[0] = 0;
+        pidx[0] = 0;
+        pnum[0] = N;
+        parent[0] = -1;
+        childBase[0] = 1;
+        didx[0] = 0;
+        dnum[0] = 8 * nDm1;
+        return;
+    }
+    int idx = 1 + 8 * t;
+    int d = 1;
+    while (d < D && idx >= L.base[d + 1]) ++d;
+    int v = 0;
+    while (v < 7 && pnum[idx + v] == 0) ++v;
+    int sh = 3 * (D - d);
+    int vk = key[idx + v];
+    int baseKey = vk - (vk & (7 << sh));
+    int nowP = pidx[idx + v], nowD = didx[idx + v], common = parent[idx + v];
+    for (int j = 0; j < 8; ++j) {
+        int o = idx + j;
+        int pn = pnum[o];
+        if (pn == 0) {
+            childBase[o] = -1;
+            if (d == D) {
+                didx[o] = o - L.base[D];
+                dnum[o] = 1;
+            } else {
+                dnum[o] = 0;
+            }
+        }
+        key[o] = baseKey + (j << sh);
+        pidx[o] = nowP;
+        nowP += pn;
+        if (d != D) {
+            didx[o] = nowD;
+            nowD += dnum[o];
+        }
+        parent[o] = common;
+    }
+}
+
+__global__ void __launch_bounds__(kThreads) zero_int(int* __restrict__ a, int n) {
+    int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i < n) a[i] = 0;
+}
+
+// Same-depth neighbours from the parent's neighbours (BuildOctree.cu:490-512).
+__global__ void __launch_bounds__(kThreads) neighbors_level(int begin, int count, int D, int depth, int M,
+                                                            const int* __restrict__ key,
+                                                            const int* __restrict__ parent,
+                                                            const int* __restrict__ childBase,
+                                                            int* __restrict__ neigh, NeighborLut lut) {
+    int t = blockIdx.x * kThreads + threadIdx.x;
+    if (t >= count) return;
+    int o = begin + t;
+    int s = (key[o] >> (3 * (D - depth))) & 7;
+    int p = parent[o];
+    for (int i = 0; i < 27; ++i) {
+        int np = neigh[lut.parent[s][i] * M + p];
+        int r = -1;
+        if (np != -1) {
+            int cb = childBase[np];
+            r = cb < 0 ? -1 : cb + lut.child[s][i];
+        }
+        neigh[i * M + o] = r;
+    }
+}
+
+__global__ void root_neighbors(int* __restrict__ neigh, int M) {
+    int i = threadIdx.x;
+    if (i < 27) neigh[i * M] = (i == 13) ? 0 : -1;
+}
+
+// Packed 1-D function index (BuildOctree.cu:514-550).
+__global__ void __launch_bounds__(kThreads) encode_function_index(Levels L, int M, const int* __restrict__ key,
+                                                                  int* __restrict__ enc, int* __restrict__ depth) {
+    int o = blockIdx.x * kThreads + threadIdx.x;
+    if (o >= M) return;
+    int d = depth_of(L, o);
+    depth[o] = d;
+    int3 f = key_func_index(key[o], d, L.D);
+    enc[o] = f.x | (f.y << (L.D + 1)) | (f.z << (2 * (L.D + 1)));
+}
+
+}  // namespace
+
+void build_octree(Ctx& c, const psr_points& pts) {
+    Octree& o = c.oct;
+    const int N = int(pts.count), D = c.cfg.depth;
+    hipStream_t s = c.stream;
+    o.N = N;
+    const unsigned xs = pts.stride_bytes ? pts.stride_bytes : 12, ns = xs;
+    // bbox + normalisation parameters
+    int nbb = std::min<int>(1024, int(div_up(N, kThreads)));
+    float* part = c.buf<float>("bbox_part", size_t(nbb) * 6);
+    o.normParams = c.buf<float>("norm_params", 4);
+    bbox_partial<<<nbb, kThreads, 0, s>>>(reinterpret_cast<const char*>(pts.xyz), xs, N, part);
+    bbox_final<<<1, 64, 0, s>>>(part, nbb, o.normParams);
+    // normalise + encode
+    float4* posU = c.buf<float4>("pos_unsorted", N);
+    float4* nrmU = c.buf<float4>("nrm_unsorted", N);
+    unsigned* keys = c.buf<unsigned>("sort_keys", N);
+    unsigned* vals = c.buf<unsigned>("sort_vals", N);
+    unsigned* ktmp = c.buf<unsigned>("sort_ktmp", N);
+    unsigned* vtmp = c.buf<unsigned>("sort_vtmp", N);
+    normalize_encode<<<div_up(N, kThreads), kThreads, 0, s>>>(
+        reinterpret_cast<const char*>(pts.xyz), xs, reinterpret_cast<const char*>(pts.normal), ns, N, D,
+        o.normParams, posU, nrmU, keys, vals);
+    int* scratch = c.scratch_for(sort_scratch_ints(N) + scan_scratch_ints(N));
+    radix_sort_pairs(keys, vals, ktmp, vtmp, N, 3 * D, scratch, s);
+    o.spos = c.buf<float4>("spos", N);
+    o.snrm = c.buf<float4>("snrm", N);
+    gather_points<<<div_up(N, kThreads), kThreads, 0, s>>>(vals, posU, nrmU, N, o.spos, o.snrm);
+    o.codes = reinterpret_cast<int*>(keys);
+    o.order = reinterpret_cast<int*>(vals);
+    // per-level occupied-node counts
+    int tiles = int(div_up(N, kRankTile));
+    int* hist = c.buf<int>("rank_hist", size_t(PSR_MAX_DEPTH + 2) * tiles);
+    int* tileOff = c.buf<int>("rank_tileoff", size_t(PSR_MAX_DEPTH + 2) * tiles);
+    int* Ud = c.buf<int>("level_U", PSR_MAX_DEPTH + 2);
+    rank_tile_hist<<<tiles, kThreads, 0, s>>>(keys, N, D, tiles, hist);
+    rank_tile_scan<<<D, 1024, 0, s>>>(hist, tiles, D, tileOff, Ud);
+    PSR_HIP_CHECK(hipMemcpyAsync(c.hostPinned, Ud, sizeof(int) * (D + 1), hipMemcpyDeviceToHost, s));
+    PSR_HIP_CHECK(hipStreamSynchronize(s));
+    o.U[0] = 1;
+    for (int d = 1; d <= D; ++d) o.U[d] = c.hostPinned[d];
+    // level layout
+    Levels& L = o.L;
+    L.D = D;
+    L.count[0] = 1;
+    for (int d = 1; d <= D; ++d) L.count[d] = 8 * o.U[d - 1];
+    L.base[0] = 0;
+    for (int d = 1; d <= D + 1; ++d) L.base[d] = L.base[d - 1] + L.count[d - 1];
+    o.M = L.base[D + 1];
+    const int M = o.M;
+    LevelOffsets lo{};
+    lo.lo[1] = 0;
+    for (int d = 1; d <= D; ++d) lo.lo[d + 1] = lo.lo[d] + o.U[d];
+    int totalNodes = lo.lo[D + 1];
+    int* firstPoint = c.buf<int>("first_point", totalNodes);
+    int* parentRank = c.buf<int>("parent_rank", totalNodes);
+    int* rankDm1 = c.buf<int>("rank_dm1", N);
+    o.point2node = c.buf<int>("point2node", N);
+    rank_points<<<tiles, kThreads, 0, s>>>(keys, N, D, tiles, tileOff, lo, firstPoint, parentRank, rankDm1,
+                                           o.point2node);
+    // node arrays
+    o.key = c.buf<int>("node_key", M);
+    o.pidx = c.buf<int>("node_pidx", M);
+    o.pnum = c.buf<int>("node_pnum", M);
+    o.parent = c.buf<int>("node_parent", M);
+    o.childBase = c.buf<int>("node_childbase", M);
+    o.didx = c.buf<int>("node_didx", M);
+    o.dnum = c.buf<int>("node_dnum", M);
+    o.neigh = c.buf<int>("node_neigh", size_t(27) * M);
+    o.encIdx = c.buf<int>("node_enc", M);
+    zero_int<<<div_up(M, kThreads), kThreads, 0, s>>>(o.pnum, M);
+    fill_occupied<<<div_up(totalNodes, kThreads), kThreads, 0, s>>>(keys, N, L, lo, Ud, firstPoint, parentRank,
+                                                                     rankDm1, totalNodes, o.key, o.pidx, o.pnum,
+                                                                     o.parent, o.childBase, o.didx, o.dnum);
+    int totalBlocks = (M - 1) / 8;
+    fill_blocks<<<div_up(totalBlocks + 1, kThreads), kThreads, 0, s>>>(L, totalBlocks, N, o.U[D - 1], o.key, o.pidx,
+                                                                        o.pnum, o.parent, o.childBase, o.didx,
+                                                                        o.dnum);
+    NeighborLut lut;
+    make_neighbor_lut(lut);
+    root_neighbors<<<1, 64, 0, s>>>(o.neigh, M);
+    for (int d = 1; d <= D; ++d)
+        neighbors_level<<<div_up(L.count[d], kThreads), kThreads, 0, s>>>(L.base[d], L.count[d], D, d, M, o.key,
+                                                                           o.parent, o.childBase, o.neigh, lut);
+    o.depth = c.buf<int>("node_depth", M);
+    encode_function_index<<<div_up(M, kThreads), kThreads, 0, s>>>(L, M, o.key, o.encIdx, o.depth);
+    PSR_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace psr
