@@ -3,6 +3,7 @@
 // per-depth CG -> implicit function -> geometry -> marching cubes.
 #include <cstring>
 #include <new>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -181,6 +182,8 @@ int psr_reconstruct(psr_ctx* ctx, const psr_points* points, psr_mesh* out) {
         return run(ctx, points, out);
     } catch (const psr::HipError& e) {
         return fail(ctx, PSR_ERR_HIP, e.what());
+    } catch (const std::invalid_argument& e) {
+        return fail(ctx, PSR_ERR_INVALID, e.what());
     } catch (const std::exception& e) {
         return fail(ctx, PSR_ERR_INTERNAL, e.what());
     }

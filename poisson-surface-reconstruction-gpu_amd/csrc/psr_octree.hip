@@ -10,6 +10,10 @@
 #include "psr_internal.h"
 #include "psr_prims.h"
 
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+
 namespace psr {
 
 namespace {
@@ -421,7 +425,12 @@ void build_octree(Ctx& c, const psr_points& pts) {
     rank_tile_hist<<<tiles, kThreads, 0, s>>>(keys, N, D, tiles, hist);
     rank_tile_scan<<<D, 1024, 0, s>>>(hist, tiles, D, tileOff, Ud);
     PSR_HIP_CHECK(hipMemcpyAsync(c.hostPinned, Ud, sizeof(int) * (D + 1), hipMemcpyDeviceToHost, s));
+    PSR_HIP_CHECK(hipMemcpyAsync(c.hostPinned + 32, o.normParams, sizeof(float) * 4, hipMemcpyDeviceToHost, s));
     PSR_HIP_CHECK(hipStreamSynchronize(s));
+    float edge;
+    std::memcpy(&edge, c.hostPinned + 35, sizeof(float));
+    if (!(edge > 0.f) || !std::isfinite(edge))  // the reference divides by it (BuildOctree.cu:149)
+        throw std::invalid_argument("degenerate input: zero or non-finite bounding box");
     o.U[0] = 1;
     for (int d = 1; d <= D; ++d) o.U[d] = c.hostPinned[d];
     // level layout

@@ -1,0 +1,56 @@
+"""CPU checks of the C ABI library (loads, exports every declared symbol,
+struct layouts) and of the seeded synthetic workloads."""
+import ctypes
+import os
+import re
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol(pkg):
+    header = open(os.path.join(ROOT, "include", "psr.h")).read()
+    declared = set(re.findall(r"^\s*(?:[A-Za-z_][\w\s\*]*?)\b(psr_\w+)\s*\(", header, re.M))
+    assert declared == set(pkg.binding.EXPORTED_SYMBOLS)
+    lib = ctypes.CDLL(pkg.binding.LIB_PATH)
+    for name in declared:
+        assert hasattr(lib, name), name
+
+
+def test_struct_layouts_match_header(pkg):
+    b = pkg.binding
+    assert ctypes.sizeof(b.Points) == 24
+    assert ctypes.sizeof(b.Mesh) == 24
+    assert ctypes.sizeof(b.Config) == 28
+    cfg = b.Config()
+    b.lib().psr_default_config(ctypes.byref(cfg))
+    assert cfg.depth == 7 and abs(cfg.cg_tol - 1e-5) < 1e-12 and cfg.cg_max_iter == 10000
+
+
+def test_status_strings(pkg):
+    L = pkg.binding.lib()
+    assert L.psr_status_string(0) == b"ok"
+    assert L.psr_status_string(2) == b"capacity exceeded"
+
+
+def test_synthetic_workloads_deterministic(pkg):
+    s = pkg.synth
+    a = s.torus(n=5000)
+    b = s.torus(n=5000)
+    assert all(np.array_equal(x, y) for x, y in zip(a, b))
+    assert a[0].flags["C_CONTIGUOUS"] and a[0].dtype == np.float32
+    c = a[0].astype(np.float64) - 0.5
+    rho = np.hypot(c[:, 0], c[:, 1])
+    assert np.abs(np.hypot(rho - 0.3, c[:, 2]) - 0.1).max() < 1e-6
+    n = a[1].astype(np.float64)
+    assert np.abs(np.linalg.norm(n, axis=1) - 1).max() < 1e-6
+    p, q = s.sphere(n=4000)
+    r = np.linalg.norm(p.astype(np.float64) - 0.5, axis=1)
+    assert abs(r.mean() - 0.4) < 1e-3 and r.std() < 0.003
+
+
+def test_batch_sizes(pkg):
+    clouds = pkg.synth.batch64()
+    sizes = [c[0].shape[0] for c in clouds]
+    assert len(clouds) == 64 and min(sizes) >= 5000 and max(sizes) <= 150000
