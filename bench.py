@@ -1,0 +1,139 @@
+"""Throughput of the Poisson reconstruction path on one MI355X.
+
+A step is one whole reconstruction (normalise, octree, field, per-depth CG,
+iso-value, geometry, marching cubes with subdivision) of a 200,000-point
+torus at octree depth 7, the points already resident in HBM.  Prints one JSON
+line.  The dominant kernel is the concurrent CG solve; its roofline uses the
+algorithmic bytes sum_d k_d (8 nnz_d + 48 n_d) over its launch time, which
+the library measures with hipEvents on its own stream.  The CPU baseline is
+the oracle's FP32 restatement of the same pipeline, timed on a bounded
+sample on the host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+WORKLOADS = {"torus": "200,000-point torus (R 0.3, r 0.1), normals rotated <= 2 deg, depth 7",
+             "sphere": "20,000-point sphere (r 0.4), jittered, depth 6",
+             "displaced": "2,000,000-point displaced sphere, depth 9"}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="torus", choices=sorted(WORKLOADS))
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "mixed"])
+    ap.add_argument("--cpu-sample", type=int, default=3, help="reconstructions in the CPU-baseline sample (0: skip)")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="OpenMP threads of the CPU baseline (0: all, max 16)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain elementwise copy of MIB MiB three times (HBM counter calibration)")
+    return ap.parse_args()
+
+
+def copy_probe(mib):
+    """A kernel of known traffic: y = x + 0 over mib MiB of float32 reads and
+    writes, for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = mib * (1 << 20) // 4
+    x = torch.ones(n, device="cuda:0")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        torch.add(x, 0.0, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "launches": 3}))
+
+
+def cpu_baseline(xyz, nrm, depth, reps, threads):
+    """The oracle's FP32 restatement (same stages and CG rule) on host cores."""
+    import __graft_entry__ as ge
+    orc = ge.load_oracle()
+    if threads <= 0:
+        threads = min(16, os.cpu_count() or 1)
+    orc.Result(xyz[:20000], nrm[:20000], min(depth, 6), precision=0, threads=threads)  # warm the library
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        orc.Result(xyz, nrm, depth, precision=0, threads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": reps / dt, "unit": "reconstructions/sec", "cores": threads, "kind": "port",
+            "sample": "%d whole FP32 reconstructions of the same input (%.1f s); octree, field and CG use "
+                      "OpenMP, geometry and marching cubes run on one thread" % (reps, dt)}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py measures one GPU only; multi-GPU runs are not part of this build "
+              "(--gpus %d requested)" % a.gpus, file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import __graft_entry__ as ge
+    if a.copy_probe:
+        copy_probe(a.copy_probe)
+        return
+    pkg = ge.load_package()
+    if not torch.cuda.is_available():
+        raise RuntimeError("bench.py needs a GPU")
+    xyz, nrm, depth = pkg.synth.workload(a.workload)
+    dev = torch.device("cuda:0")
+    tx = torch.from_numpy(xyz).to(dev)
+    tn = torch.from_numpy(nrm).to(dev)
+    prec = pkg.binding.PRECISION_MIXED if a.precision == "mixed" else pkg.binding.PRECISION_FP32
+    rec = pkg.PoissonReconstruction(depth=depth, precision=prec)
+    torch.cuda.synchronize()
+    for _ in range(a.warmup):
+        rec.SolvePoissionReconstructionMesh(tx, tn)
+    torch.cuda.synchronize()
+    cg_ms, cg_bytes = [], []
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        rec.SolvePoissionReconstructionMesh(tx, tn)
+        st = rec.stats()
+        cg_ms.append(st["ms_cg_kernel"])
+        cg_bytes.append(sum(k * (8 * z + 48 * n) for k, z, n in
+                            zip(st["cg_iterations"], st["cg_nnz"], st["nodes_per_depth"])))
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    st = rec.stats()
+    rec.close()
+    ms_cg = float(np.mean(cg_ms))
+    achieved = float(np.mean(cg_bytes)) / (ms_cg * 1e-3) / 1e9
+    line = {
+        "metric": "reconstructions/sec at depth 7, 200k pts; mixed-prec vs FP32 at <=1e-5 residual",
+        "value": a.steps / dt,
+        "unit": "reconstructions/sec",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": dt * 1e3 / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": WORKLOADS[a.workload], "points": int(len(xyz)), "depth": depth,
+                   "precision": a.precision, "nodes": st["n_nodes"], "cg_iterations": st["cg_iterations"],
+                   "mesh_triangles": st["mesh_triangles"]},
+        "roofline": {"kernel": "cg_all_depths", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "ms_per_launch": ms_cg, "bytes_per_launch": float(np.mean(cg_bytes))},
+        "stage_ms": {k: st[k] for k in ("ms_octree", "ms_field", "ms_solve", "ms_mesh", "ms_total")},
+    }
+    if a.cpu_sample > 0:
+        line["cpu_baseline"] = cpu_baseline(xyz, nrm, depth, a.cpu_sample, a.cpu_threads)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -46,6 +46,8 @@ typedef struct {
     int cg_max_iter;       /* 10000 (CGAlgorithm.cu:15) */
     float refine_rel_tol;  /* PSR_PRECISION_MIXED: target true relative residual per depth, 1e-5 */
     int refine_max_outer;  /* PSR_PRECISION_MIXED: outer refinement steps, 4 */
+    uint32_t subdiv_chunk_nodes; /* virtual nodes per subdivision working set; 0 = 16M (about 2 GB of
+                                    neighbour tables); results do not depend on it */
 } psr_config;
 
 typedef struct {
@@ -77,6 +79,9 @@ typedef struct {
     uint32_t n_mesh_vertices, n_mesh_triangles;
     int q1_fired;         /* the reference's level-local child 0 parent rule applied somewhere */
     float ms_octree, ms_field, ms_solve, ms_mesh, ms_total;
+    float ms_cg_kernel;   /* the concurrent CG kernel alone, hipEvents on the context's stream */
+    uint32_t cg_workgroups;
+    int cg_nnz[PSR_STATS_MAX_DEPTH + 1]; /* stored operator entries per depth */
 } psr_stats;
 
 /* Arrays that psr_export can copy to host memory (for parity checks). */

@@ -40,7 +40,7 @@ EXPORTED_SYMBOLS = ["psr_default_config", "psr_create", "psr_reconstruct", "psr_
 class Config(ctypes.Structure):
     _fields_ = [("depth", ctypes.c_int), ("max_points", ctypes.c_uint32), ("precision", ctypes.c_int),
                 ("cg_tol", ctypes.c_float), ("cg_max_iter", ctypes.c_int), ("refine_rel_tol", ctypes.c_float),
-                ("refine_max_outer", ctypes.c_int)]
+                ("refine_max_outer", ctypes.c_int), ("subdiv_chunk_nodes", ctypes.c_uint32)]
 
 
 class Points(ctypes.Structure):
@@ -65,7 +65,9 @@ class Stats(ctypes.Structure):
                 ("n_subdivision_roots", ctypes.c_uint32), ("n_coarse_roots", ctypes.c_uint32),
                 ("n_mesh_vertices", ctypes.c_uint32), ("n_mesh_triangles", ctypes.c_uint32),
                 ("q1_fired", ctypes.c_int), ("ms_octree", ctypes.c_float), ("ms_field", ctypes.c_float),
-                ("ms_solve", ctypes.c_float), ("ms_mesh", ctypes.c_float), ("ms_total", ctypes.c_float)]
+                ("ms_solve", ctypes.c_float), ("ms_mesh", ctypes.c_float), ("ms_total", ctypes.c_float),
+                ("ms_cg_kernel", ctypes.c_float), ("cg_workgroups", ctypes.c_uint32),
+                ("cg_nnz", ctypes.c_int * _MAXD)]
 
     def as_dict(self):
         d = self.depth
@@ -78,7 +80,8 @@ class Stats(ctypes.Structure):
             "roots": self.n_subdivision_roots, "coarse_roots": self.n_coarse_roots,
             "mesh_vertices": self.n_mesh_vertices, "mesh_triangles": self.n_mesh_triangles,
             "ms_octree": self.ms_octree, "ms_field": self.ms_field, "ms_solve": self.ms_solve,
-            "ms_mesh": self.ms_mesh, "ms_total": self.ms_total,
+            "ms_mesh": self.ms_mesh, "ms_total": self.ms_total, "ms_cg_kernel": self.ms_cg_kernel,
+            "cg_workgroups": self.cg_workgroups, "cg_nnz": list(self.cg_nnz)[:d + 1],
         }
 
 
@@ -117,7 +120,7 @@ class PoissonReconstruction:
     """Host-side mirror of SparseSurfelFusion::PoissonReconstruction."""
 
     def __init__(self, depth=7, precision=PRECISION_FP32, max_points=0, cg_tol=1e-5, cg_max_iter=10000,
-                 stream=None):
+                 stream=None, subdiv_chunk_nodes=0):
         cfg = Config()
         lib().psr_default_config(ctypes.byref(cfg))
         cfg.depth = depth
@@ -125,6 +128,7 @@ class PoissonReconstruction:
         cfg.max_points = max_points
         cfg.cg_tol = cg_tol
         cfg.cg_max_iter = cg_max_iter
+        cfg.subdiv_chunk_nodes = subdiv_chunk_nodes
         self._ctx = ctypes.c_void_p()
         rc = lib().psr_create(ctypes.byref(cfg), stream, ctypes.byref(self._ctx))
         if rc != PSR_OK:

@@ -1,6 +1,7 @@
 // C ABI and orchestration (SolvePoissionReconstructionMesh,
 // mesh/PoissonReconstruction.cpp:195-245): octree -> field -> divergence ->
 // per-depth CG -> implicit function -> geometry -> marching cubes.
+#include <algorithm>
 #include <cstring>
 #include <new>
 #include <stdexcept>
@@ -26,6 +27,26 @@ void* Arena::get(const std::string& name, size_t bytes) {
         }
         size_t want = bytes + bytes / 8 + 256;
         PSR_HIP_CHECK(hipMalloc(&b.ptr, want));
+        b.bytes = want;
+        total_ += want;
+        ++allocs_;
+    }
+    return b.ptr;
+}
+
+void* Arena::get_keep(const std::string& name, size_t bytes, size_t keepBytes) {
+    Block& b = blocks_[name];
+    if (b.bytes < bytes) {
+        size_t want = bytes + bytes / 2 + 256;
+        void* p = nullptr;
+        PSR_HIP_CHECK(hipMalloc(&p, want));
+        if (b.ptr) {
+            PSR_HIP_CHECK(hipDeviceSynchronize());
+            if (keepBytes) PSR_HIP_CHECK(hipMemcpy(p, b.ptr, std::min(keepBytes, b.bytes), hipMemcpyDeviceToDevice));
+            PSR_HIP_CHECK(hipFree(b.ptr));
+            total_ -= b.bytes;
+        }
+        b.ptr = p;
         b.bytes = want;
         total_ += want;
         ++allocs_;
@@ -92,6 +113,8 @@ int run(psr_ctx* ctx, const psr_points* pts, psr_mesh* out) {
                             hipMemcpyDeviceToHost));
     PSR_HIP_CHECK(hipMemcpy(resid, c.arena.get("cg_resid", 8 * (PSR_MAX_DEPTH + 1)), sizeof(resid),
                             hipMemcpyDeviceToHost));
+    PSR_HIP_CHECK(hipMemcpy(st.cg_nnz, c.arena.get("cg_nnz", 4 * (PSR_MAX_DEPTH + 1)), sizeof(st.cg_nnz),
+                            hipMemcpyDeviceToHost));
     int err = 0;
     PSR_HIP_CHECK(hipMemcpy(&err, c.arena.get("cg_err", 4), sizeof(int), hipMemcpyDeviceToHost));
     if (err) return fail(ctx, PSR_ERR_INTERNAL, "CG group barrier timed out");
@@ -108,6 +131,7 @@ int run(psr_ctx* ctx, const psr_points* pts, psr_mesh* out) {
     st.ms_solve = elapsed(c.ev[2], c.ev[3]);
     st.ms_mesh = elapsed(c.ev[3], c.ev[4]);
     st.ms_total = elapsed(c.ev[0], c.ev[4]);
+    st.ms_cg_kernel = elapsed(c.ev[8], c.ev[9]);
     out->vertices = c.mesh.vertices;
     out->triangles = c.mesh.triangles;
     out->n_vertices = uint32_t(c.mesh.nv);
@@ -140,6 +164,7 @@ void psr_default_config(psr_config* cfg) {
     cfg->cg_max_iter = 10000;
     cfg->refine_rel_tol = 1e-5f;
     cfg->refine_max_outer = 4;
+    cfg->subdiv_chunk_nodes = 0;
 }
 
 int psr_create(const psr_config* cfg, void* hip_stream, psr_ctx** out) {

@@ -15,6 +15,8 @@ class Arena {
   public:
     ~Arena() { release(); }
     void* get(const std::string& name, size_t bytes);
+    // As get, but a reallocation carries over the first keepBytes of the old contents.
+    void* get_keep(const std::string& name, size_t bytes, size_t keepBytes);
     void release();
     size_t bytes_reserved() const { return total_; }
     int allocations() const { return allocs_; }
@@ -110,9 +112,15 @@ struct Ctx {
     size_t scratchInts = 0;
     hipEvent_t ev[16] = {};
     int* hostPinned = nullptr;   // small pinned buffer for size read-backs
+    int cgMaxWgs = 0;            // resident workgroups of the CG kernel
+    int cgRowsPerWg = 0;         // rows per CG workgroup (0: default)
     template <class T>
     T* buf(const char* name, size_t count) {
         return static_cast<T*>(arena.get(name, sizeof(T) * (count ? count : 1)));
+    }
+    template <class T>
+    T* buf_keep(const char* name, size_t count, size_t keepCount) {
+        return static_cast<T*>(arena.get_keep(name, sizeof(T) * (count ? count : 1), sizeof(T) * keepCount));
     }
     int* scratch_for(size_t ints) {
         if (ints > scratchInts) {

@@ -52,22 +52,40 @@ __global__ void __launch_bounds__(kThreads) bbox_partial(const char* __restrict_
     if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = s[threadIdx.x][0];
 }
 
-__global__ void bbox_final(const float* __restrict__ part, int nb, float* __restrict__ params) {
+__global__ void __launch_bounds__(kThreads) bbox_final(const float* __restrict__ part, int nb,
+                                                       float* __restrict__ params) {
 #pragma clang fp contract(off)
-    if (threadIdx.x != 0) return;
+    __shared__ float s[6][kThreads];
     float mx[3] = {-1e6f, -1e6f, -1e6f}, mn[3] = {1e6f, 1e6f, 1e6f};
-    for (int b = 0; b < nb; ++b)
+    for (int b = threadIdx.x; b < nb; b += kThreads)
         for (int a = 0; a < 3; ++a) {
             float v = part[b * 6 + a];
             mx[a] = mx[a] < v ? v : mx[a];
             v = part[b * 6 + 3 + a];
             mn[a] = mn[a] > v ? v : mn[a];
         }
+    for (int a = 0; a < 3; ++a) {
+        s[a][threadIdx.x] = mx[a];
+        s[3 + a][threadIdx.x] = mn[a];
+    }
+    __syncthreads();
+    for (int st = kThreads / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < st)
+            for (int a = 0; a < 3; ++a) {
+                float l = s[a][threadIdx.x], r = s[a][threadIdx.x + st];
+                s[a][threadIdx.x] = l < r ? r : l;
+                l = s[3 + a][threadIdx.x];
+                r = s[3 + a][threadIdx.x + st];
+                s[3 + a][threadIdx.x] = l > r ? r : l;
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
     float maxEdge = 1.0f, center[3];
     for (int a = 0; a < 3; ++a) {
-        float ext = __fsub_rn(mx[a], mn[a]);
+        float ext = __fsub_rn(s[a][0], s[3 + a][0]);
         if (a == 0 || maxEdge < ext) maxEdge = ext;
-        center[a] = __fdiv_rn(__fadd_rn(mx[a], mn[a]), 2.0f);
+        center[a] = __fdiv_rn(__fadd_rn(s[a][0], s[3 + a][0]), 2.0f);
     }
     maxEdge = __fmul_rn(maxEdge, 1.25f);
     for (int a = 0; a < 3; ++a) params[a] = __fsub_rn(center[a], __fdiv_rn(maxEdge, 2.0f));
@@ -399,7 +417,7 @@ void build_octree(Ctx& c, const psr_points& pts) {
     float* part = c.buf<float>("bbox_part", size_t(nbb) * 6);
     o.normParams = c.buf<float>("norm_params", 4);
     bbox_partial<<<nbb, kThreads, 0, s>>>(reinterpret_cast<const char*>(pts.xyz), xs, N, part);
-    bbox_final<<<1, 64, 0, s>>>(part, nbb, o.normParams);
+    bbox_final<<<1, kThreads, 0, s>>>(part, nbb, o.normParams);
     // normalise + encode
     float4* posU = c.buf<float4>("pos_unsorted", N);
     float4* nrmU = c.buf<float4>("nrm_unsorted", N);

@@ -82,8 +82,10 @@ __global__ void __launch_bounds__(1024) scan_tile_sums(int* __restrict__ sums, i
     if (threadIdx.x == 0 && total) *total = tot;
 }
 
-__global__ void __launch_bounds__(kScanThreads) scan_downsweep(const int* __restrict__ in, int* __restrict__ out,
-                                                               int n, const int* __restrict__ tileOffsets) {
+// in and out may be the same array: each thread reads its items before it
+// writes them, and the tiles are disjoint.
+__global__ void __launch_bounds__(kScanThreads) scan_downsweep(const int* in, int* out, int n,
+                                                               const int* __restrict__ tileOffsets) {
     // Each thread owns kScanItems consecutive elements of the tile.
     int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
     int v[kScanItems];
@@ -105,7 +107,15 @@ __global__ void __launch_bounds__(kScanThreads) scan_downsweep(const int* __rest
     }
 }
 
-size_t scan_scratch_ints(size_t n) { return div_up(n, kScanTile) + 64; }
+// Up to this many tile sums are scanned by one workgroup; more go through a
+// second level of the same three-pass scan.
+static constexpr int kScanSingleLevel = 1024 * 64;
+
+size_t scan_scratch_ints(size_t n) {
+    size_t tiles = div_up(n, kScanTile);
+    if (tiles <= size_t(kScanSingleLevel)) return tiles + 64;
+    return 2 * tiles + 64 + scan_scratch_ints(tiles);
+}
 
 void scan_exclusive(const int* in, int* out, int n, int* total, int* scratch, hipStream_t s) {
     if (n <= 0) {
@@ -113,10 +123,16 @@ void scan_exclusive(const int* in, int* out, int n, int* total, int* scratch, hi
         return;
     }
     int tiles = int(div_up(n, kScanTile));
-    if (tiles > 1024 * 64) throw std::runtime_error("scan_exclusive: input too large");
     scan_reduce_tiles<<<tiles, kScanThreads, 0, s>>>(in, n, scratch);
-    scan_tile_sums<<<1, 1024, 0, s>>>(scratch, tiles, total);
-    scan_downsweep<<<tiles, kScanThreads, 0, s>>>(in, out, n, scratch);
+    if (tiles <= kScanSingleLevel) {
+        scan_tile_sums<<<1, 1024, 0, s>>>(scratch, tiles, total);
+        scan_downsweep<<<tiles, kScanThreads, 0, s>>>(in, out, n, scratch);
+    } else {
+        // layout: tile sums [tiles] | their exclusive scan [tiles] | inner scratch
+        int* offs = scratch + tiles;
+        scan_exclusive(scratch, offs, tiles, total, offs + tiles + 64, s);
+        scan_downsweep<<<tiles, kScanThreads, 0, s>>>(in, out, n, offs);
+    }
     PSR_HIP_CHECK(hipGetLastError());
 }
 

@@ -2,6 +2,9 @@
 // input points and the iso-value.
 #include "psr_internal.h"
 
+#include <algorithm>
+#include <stdexcept>
+
 namespace psr {
 
 namespace {
@@ -57,8 +60,19 @@ struct CgParams {
     unsigned spinLimit;
 };
 
-constexpr int kCgThreads = 512;
+constexpr int kCgThreads = 256;
 constexpr int kCgWaves = kCgThreads / PSR_WAVE;
+
+// Vectors read by other workgroups (r, p) and the reduction partials are
+// accessed with agent-scope relaxed atomics, which are coherent across the
+// XCDs without flushing or invalidating the L2; the barrier then only has to
+// order them (s_waitcnt) before the arrival.
+__device__ __forceinline__ float ld_shared(const float* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_shared(float* p, float v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 __device__ __forceinline__ double block_sum_double(double v, double* sh) {
     for (int off = PSR_WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, PSR_WAVE);
@@ -75,25 +89,25 @@ __device__ __forceinline__ double block_sum_double(double v, double* sh) {
 // spin limit was hit (then the error flag is set and the caller stops).
 __device__ __forceinline__ bool group_barrier(unsigned* counter, unsigned target, unsigned spinLimit,
                                               int* errFlag, int* shFlag) {
-    __threadfence_block();
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's agent-scope stores have completed
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
     __syncthreads();
     if (threadIdx.x == 0) {
-        __threadfence();  // release the workgroup's stores at device scope
-        atomicAdd(counter, 1u);
+        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         unsigned spins = 0;
         int ok = 1;
         while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
             if (++spins > spinLimit || __hip_atomic_load(errFlag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
                 ok = 0;
-                atomicExch(errFlag, 1);
+                __hip_atomic_store(errFlag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 break;
             }
             __builtin_amdgcn_s_sleep(1);
         }
-        __threadfence();  // acquire: later loads see the other workgroups' stores
         *shFlag = ok;
     }
     __syncthreads();
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
     return *shFlag != 0;
 }
 
@@ -102,51 +116,51 @@ __device__ __forceinline__ float p_next(float r, float beta, float p) { return r
 __global__ void __launch_bounds__(kCgThreads) cg_all_depths(CgParams P, const int* __restrict__ neigh,
                                                             const float* __restrict__ val,
                                                             const float* __restrict__ b, float* __restrict__ x,
-                                                            float* __restrict__ r, float* __restrict__ pA,
-                                                            float* __restrict__ pB, float* __restrict__ Ap,
-                                                            double* __restrict__ partial, unsigned* __restrict__ bar,
-                                                            int* __restrict__ iters, double* __restrict__ resid,
-                                                            int* __restrict__ errFlag) {
+                                                            float* r, float* pA, float* pB,
+                                                            float* __restrict__ Ap, double* partial,
+                                                            unsigned* bar, int* __restrict__ iters,
+                                                            double* __restrict__ resid, int* errFlag) {
     __shared__ double sh[kCgWaves];
     __shared__ int shFlag;
-    // locate the group of this workgroup
     int gi = 0;
     while (gi + 1 < P.ngroups && int(blockIdx.x) >= P.g[gi + 1].firstWg) ++gi;
     const CgGroup G = P.g[gi];
     const int local = blockIdx.x - G.firstWg;
     const int r0 = G.rowBegin + local * G.rowsPerWg;
     const int r1 = min(G.rowBegin + G.rows, r0 + G.rowsPerWg);
-    const int M = P.M, base = G.rowBegin;
+    const int M = P.M;
     double* part = partial + 2 * G.firstWg;
     unsigned* cnt = bar + gi;
     unsigned phase = 0;
     const bool multi = G.wgs > 1;
 
+    // Sum over the group.  Partials are double-buffered by phase parity: a
+    // workgroup that runs ahead writes the other half while slower ones still
+    // read this one.
     auto group_sum = [&](double v, double& out) -> bool {
         double s = block_sum_double(v, sh);
         if (!multi) {
             out = s;
             return true;
         }
-        // partials double-buffered by phase parity: a workgroup that runs ahead
-        // writes the other half while slower ones still read this one
         double* half = part + (phase & 1u) * unsigned(G.wgs);
-        if (threadIdx.x == 0) half[local] = s;
+        if (threadIdx.x == 0) __hip_atomic_store(&half[local], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         ++phase;
         if (!group_barrier(cnt, phase * unsigned(G.wgs), P.spinLimit, errFlag, &shFlag)) return false;
         double t = 0;
-        for (int k = 0; k < G.wgs; ++k) t += __hip_atomic_load(&half[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        out = t;
+        for (int k = threadIdx.x; k < G.wgs; k += kCgThreads)
+            t += __hip_atomic_load(&half[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        out = block_sum_double(t, sh);
         return true;
     };
 
-    // x = 0, r = b, rr = r.r
+    // x = 0, r = b, rr = r.r  (CGAlgorithm.cu:21-33 with x0 = 0)
     double loc = 0;
     for (int i = r0 + threadIdx.x; i < r1; i += kCgThreads) {
         float bi = b[i];
         x[i] = 0.f;
-        r[i] = bi;
-        pA[i] = 0.f;
+        st_shared(&r[i], bi);
+        st_shared(&pA[i], 0.f);
         loc += double(bi * bi);
     }
     double rrD;
@@ -156,33 +170,35 @@ __global__ void __launch_bounds__(kCgThreads) cg_all_depths(CgParams P, const in
     float* pOld = pA;
     float* pNew = pB;
     while (rr > P.tolSq && k <= P.maxIter) {
-        float beta = (k > 1) ? rr / rrOld : 0.f;
-        // p = r + beta p; Ap = A p; pAp
+        const float beta = (k > 1) ? rr / rrOld : 0.f;
+        // p = r + beta p (CGAlgorithm.cu:37-43), Ap = A p (:47), p.Ap (:53).  The
+        // neighbours' p entries are formed from the previous r and p, so no
+        // barrier is needed between the update and the product.
         double lpAp = 0;
         for (int i = r0 + threadIdx.x; i < r1; i += kCgThreads) {
             float s = 0.f;
+#pragma unroll
             for (int q = 0; q < 27; ++q) {
                 float v = val[q * M + i];
-                if (v != 0.f) {
-                    int j = neigh[q * M + i];
-                    float pj = (k > 1) ? p_next(r[j], beta, pOld[j]) : r[j];
-                    s += v * pj;
-                }
+                int j = neigh[q * M + i];
+                j = (v != 0.f) ? j : i;  // dropped or missing entry: safe index, zero weight
+                float pj = p_next(ld_shared(&r[j]), beta, ld_shared(&pOld[j]));
+                s += v * pj;
             }
-            float pi = (k > 1) ? p_next(r[i], beta, pOld[i]) : r[i];
-            pNew[i] = pi;
+            float pi = p_next(ld_shared(&r[i]), beta, ld_shared(&pOld[i]));
+            st_shared(&pNew[i], pi);
             Ap[i] = s;
             lpAp += double(pi * s);
         }
         double pAp;
         if (!group_sum(lpAp, pAp)) return;
-        float alpha = float(double(rr) / pAp);
+        const float alpha = float(double(rr) / pAp);
         double lrr = 0;
         for (int i = r0 + threadIdx.x; i < r1; i += kCgThreads) {
-            float pi = pNew[i];
+            float pi = ld_shared(&pNew[i]);
             x[i] = x[i] + alpha * pi;
-            float ri = r[i] + (-alpha) * Ap[i];
-            r[i] = ri;
+            float ri = ld_shared(&r[i]) + (-alpha) * Ap[i];
+            st_shared(&r[i], ri);
             lrr += double(ri * ri);
         }
         rrOld = rr;
@@ -198,7 +214,6 @@ __global__ void __launch_bounds__(kCgThreads) cg_all_depths(CgParams P, const in
         iters[G.depth] = k - 1;
         resid[G.depth] = sqrt(double(rr));
     }
-    (void)base;
 }
 
 // --- implicit function at points (LaplacianSolver.cu:85-118, 269-305) ------
@@ -263,6 +278,22 @@ __global__ void __launch_bounds__(1024) iso_final(const double* __restrict__ par
     }
 }
 
+// stored (non-dropped) operator entries per depth
+__global__ void __launch_bounds__(kThreads) count_nonzeros(Levels L, int M, const float* __restrict__ val,
+                                                           int* __restrict__ nnz) {
+    int o = blockIdx.x * kThreads + threadIdx.x;
+    int c = 0;
+    if (o < M)
+        for (int q = 0; q < 27; ++q) c += val[q * M + o] != 0.f;
+    int d = o < M ? depth_of(L, o) : 0;
+    // rows of one wave usually share a depth: reduce per depth value in the wave
+    for (int dd = 0; dd <= L.D; ++dd) {
+        int v = (o < M && d == dd) ? c : 0;
+        for (int off = PSR_WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, PSR_WAVE);
+        if (__lane_id() == 0 && v) atomicAdd(&nnz[dd], v);
+    }
+}
+
 __global__ void zero_u32(unsigned* a, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) a[i] = 0u;
@@ -278,6 +309,9 @@ void solve_depths(Ctx& c) {
     float* val = c.buf<float>("lap_val", size_t(27) * M);
     assemble_operator<<<div_up(M, kThreads), kThreads, 0, s>>>(L, M, o.neigh, o.encIdx, c.tables.dot, c.tables.d2dot,
                                                               c.tables.res, val);
+    int* nnz = c.buf<int>("cg_nnz", PSR_MAX_DEPTH + 1);
+    zero_u32<<<1, 64, 0, s>>>(reinterpret_cast<unsigned*>(nnz), PSR_MAX_DEPTH + 1);
+    count_nonzeros<<<div_up(M, kThreads), kThreads, 0, s>>>(L, M, val, nnz);
     c.fld.dx = c.buf<float>("dx", M);
     float* r = c.buf<float>("cg_r", M);
     float* pA = c.buf<float>("cg_pa", M);
@@ -288,7 +322,23 @@ void solve_depths(Ctx& c) {
     P.tolSq = c.cfg.cg_tol * c.cfg.cg_tol;
     P.maxIter = c.cfg.cg_max_iter;
     P.spinLimit = 1u << 26;
-    const int kSingle = 4096, kRowsPerWg = 1024;
+    // workgroup budget: every workgroup of the cooperative launch must be resident
+    if (c.cgMaxWgs <= 0) {
+        int perCu = 0, cus = 0;
+        PSR_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, cg_all_depths, kCgThreads, 0));
+        PSR_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device));
+        c.cgMaxWgs = std::max(1, perCu * cus);
+    }
+    int rowsPerWg = c.cgRowsPerWg > 0 ? c.cgRowsPerWg : 512;
+    {
+        // widen the row blocks until every group fits the resident budget
+        auto need = [&](int rpw) {
+            long long w = 0;
+            for (int d = 0; d <= D; ++d) w += std::max<long long>(1, (L.count[d] + rpw - 1) / rpw);
+            return w;
+        };
+        while (need(rowsPerWg) > c.cgMaxWgs) rowsPerWg *= 2;
+    }
     int wg = 0;
     // largest depths first so their groups start on separate CUs early
     for (int d = D; d >= 0; --d) {
@@ -297,15 +347,12 @@ void solve_depths(Ctx& c) {
         g.rowBegin = L.base[d];
         g.rows = L.count[d];
         g.firstWg = wg;
-        if (g.rows <= kSingle) {
-            g.wgs = 1;
-            g.rowsPerWg = g.rows;
-        } else {
-            g.wgs = int(div_up(g.rows, kRowsPerWg));
-            g.rowsPerWg = kRowsPerWg;
-        }
+        g.rowsPerWg = rowsPerWg;
+        g.wgs = std::max(1, int(div_up(g.rows, rowsPerWg)));
         wg += g.wgs;
     }
+    if (wg > c.cgMaxWgs) throw std::runtime_error("CG launch exceeds the resident workgroup budget");
+    c.stats.cg_workgroups = uint32_t(wg);
     double* partial = c.buf<double>("cg_partial", size_t(2) * wg);
     unsigned* bar = c.buf<unsigned>("cg_bar", PSR_MAX_DEPTH + 1);
     int* iters = c.buf<int>("cg_iters", PSR_MAX_DEPTH + 1);
@@ -314,8 +361,10 @@ void solve_depths(Ctx& c) {
     zero_u32<<<1, 64, 0, s>>>(bar, PSR_MAX_DEPTH + 1);
     zero_u32<<<1, 64, 0, s>>>(reinterpret_cast<unsigned*>(err), 1);
     void* args[] = {&P, &o.neigh, &val, &c.fld.div, &c.fld.dx, &r, &pA, &pB, &Ap, &partial, &bar, &iters, &resid, &err};
+    PSR_HIP_CHECK(hipEventRecord(c.ev[8], s));
     PSR_HIP_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(cg_all_depths), dim3(wg), dim3(kCgThreads),
                                              args, 0, s));
+    PSR_HIP_CHECK(hipEventRecord(c.ev[9], s));
     PSR_HIP_CHECK(hipGetLastError());
 }
 

@@ -208,3 +208,17 @@ def test_cg_iteration_cap_reported_not_raised(pkg, torch):
     assert max(st["cg_iterations"]) <= 1
     assert max(st["cg_updated_residual"]) > 1e-5
     rec.close()
+
+
+def test_subdivision_chunking_does_not_change_the_mesh(pkg, torch, runs):
+    """The subdivision sets run in working sets of bounded size; a tiny bound
+    (many chunks, each set alone) gives the same vertices and triangles."""
+    rec, ref, xyz, nrm = runs["torus"]
+    st = rec.stats()
+    assert st["roots"] > 0
+    tx, tn = _upload(torch, xyz, nrm)
+    small = pkg.PoissonReconstruction(depth=7, subdiv_chunk_nodes=1000)
+    small.SolvePoissionReconstructionMesh(tx, tn)
+    np.testing.assert_array_equal(small.GetRebuildMeshVertices(), rec.GetRebuildMeshVertices())
+    np.testing.assert_array_equal(small.GetRebuildMeshTriangleIndices(), rec.GetRebuildMeshTriangleIndices())
+    small.close()
