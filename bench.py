@@ -21,6 +21,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+PROFILE_TAG = {"torus": "torus_d7", "sphere": "sphere_d6", "displaced": "displaced_d9"}
 WORKLOADS = {"torus": "200,000-point torus (R 0.3, r 0.1), normals rotated <= 2 deg, depth 7",
              "sphere": "20,000-point sphere (r 0.4), jittered, depth 6",
              "displaced": "2,000,000-point displaced sphere, depth 9"}
@@ -51,6 +52,19 @@ def copy_probe(mib):
         torch.add(x, 0.0, out=y)
     torch.cuda.synchronize()
     print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "launches": 3}))
+
+
+def measured_traffic(workload, precision, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary of the same workload (FETCH_SIZE and WRITE_SIZE passes scaled by a
+    copy kernel of known size), or None when there is none."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_%s_%s.md" % (PROFILE_TAG.get(workload, workload), precision))
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.strip().startswith("TRAFFIC "):
+            return json.loads(line.strip()[len("TRAFFIC "):]).get(kernel)
+    return None
 
 
 def cpu_baseline(xyz, nrm, depth, reps, threads):
@@ -124,7 +138,8 @@ def main():
                    "precision": a.precision, "nodes": st["n_nodes"], "cg_iterations": st["cg_iterations"],
                    "mesh_triangles": st["mesh_triangles"]},
         "roofline": {"kernel": "cg_all_depths", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(a.workload, a.precision, "cg_all_depths"),
                      "ms_per_launch": ms_cg, "bytes_per_launch": float(np.mean(cg_bytes))},
         "stage_ms": {k: st[k] for k in ("ms_octree", "ms_field", "ms_solve", "ms_mesh", "ms_total")},
     }
