@@ -54,3 +54,32 @@ def test_batch_sizes(pkg):
     clouds = pkg.synth.batch64()
     sizes = [c[0].shape[0] for c in clouds]
     assert len(clouds) == 64 and min(sizes) >= 5000 and max(sizes) <= 150000
+
+
+def test_cpp_facade_compiles_links_and_layouts_agree(pkg, tmp_path):
+    """The C++ facade builds against psr.h and links libpsr.so; the C struct
+    sizes and field offsets match the ctypes mirror."""
+    import subprocess
+    b = pkg.binding
+    src = tmp_path / "facade.cpp"
+    src.write_text(r'''
+#include <cstddef>
+#include <cstdio>
+#include "psr_facade.hpp"
+int main() {
+    psr_config c; psr_default_config(&c);
+    std::printf("%zu %zu %zu %zu %zu %zu %d %s\n", sizeof(psr_config), sizeof(psr_points), sizeof(psr_mesh),
+                sizeof(psr_stats), offsetof(psr_stats, ms_cg_kernel), offsetof(psr_stats, cg_nnz), c.depth,
+                psr_status_string(PSR_ERR_INVALID));
+    return 0;
+}
+''')
+    exe = tmp_path / "facade"
+    libdir = os.path.dirname(b.LIB_PATH)
+    subprocess.check_call(["g++", "-std=c++17", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe),
+                           "-L", libdir, "-lpsr", "-Wl,-rpath," + libdir, "-Wl,-rpath,/opt/rocm/lib"])
+    out = subprocess.check_output([str(exe)]).decode().split()
+    assert [int(x) for x in out[:4]] == [ctypes.sizeof(b.Config), ctypes.sizeof(b.Points), ctypes.sizeof(b.Mesh),
+                                         ctypes.sizeof(b.Stats)]
+    assert int(out[4]) == b.Stats.ms_cg_kernel.offset and int(out[5]) == b.Stats.cg_nnz.offset
+    assert out[6] == "7" and " ".join(out[7:]) == b.lib().psr_status_string(1).decode()
