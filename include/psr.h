@@ -72,7 +72,7 @@ typedef struct {
     uint32_t nodes_per_depth[PSR_STATS_MAX_DEPTH + 1];
     int cg_iterations[PSR_STATS_MAX_DEPTH + 1];
     double cg_updated_residual[PSR_STATS_MAX_DEPTH + 1];
-    double cg_true_relres[PSR_STATS_MAX_DEPTH + 1]; /* filled when stats_level >= 1 */
+    double cg_true_relres[PSR_STATS_MAX_DEPTH + 1]; /* PSR_PRECISION_MIXED: final FP64 ||b - A x|| / ||b||; -1 otherwise */
     float iso_value;
     uint32_t n_vertex_nodes, n_edge_nodes, n_face_nodes;
     uint32_t n_subdivision_roots, n_coarse_roots;

@@ -76,6 +76,7 @@ class Stats(ctypes.Structure):
             "nodes_per_depth": list(self.nodes_per_depth)[:d + 1],
             "cg_iterations": list(self.cg_iterations)[:d + 1],
             "cg_updated_residual": list(self.cg_updated_residual)[:d + 1],
+            "cg_true_relres": list(self.cg_true_relres)[:d + 1],
             "iso_value": self.iso_value, "V": self.n_vertex_nodes, "E": self.n_edge_nodes, "F": self.n_face_nodes,
             "roots": self.n_subdivision_roots, "coarse_roots": self.n_coarse_roots,
             "mesh_vertices": self.n_mesh_vertices, "mesh_triangles": self.n_mesh_triangles,

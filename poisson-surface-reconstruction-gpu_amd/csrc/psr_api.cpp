@@ -118,9 +118,13 @@ int run(psr_ctx* ctx, const psr_points* pts, psr_mesh* out) {
     int err = 0;
     PSR_HIP_CHECK(hipMemcpy(&err, c.arena.get("cg_err", 4), sizeof(int), hipMemcpyDeviceToHost));
     if (err) return fail(ctx, PSR_ERR_INTERNAL, "CG group barrier timed out");
+    double trueRel[PSR_MAX_DEPTH + 1];
+    PSR_HIP_CHECK(hipMemcpy(trueRel, c.arena.get("cg_truerel", 8 * (PSR_MAX_DEPTH + 1)), sizeof(trueRel),
+                            hipMemcpyDeviceToHost));
     for (int d = 0; d <= o.L.D; ++d) {
         st.cg_iterations[d] = iters[d];
         st.cg_updated_residual[d] = resid[d];
+        st.cg_true_relres[d] = c.cfg.precision == PSR_PRECISION_MIXED ? trueRel[d] : -1.0;
     }
     PSR_HIP_CHECK(hipMemcpy(&st.iso_value, c.arena.get("iso_value", 4), sizeof(float), hipMemcpyDeviceToHost));
     st.n_vertex_nodes = c.geo.V;
@@ -174,6 +178,8 @@ int psr_create(const psr_config* cfg, void* hip_stream, psr_ctx** out) {
     psr_default_config(&def);
     const psr_config& k = cfg ? *cfg : def;
     if (k.depth < 2 || k.depth > 9) return PSR_ERR_INVALID;
+    if (k.precision != PSR_PRECISION_FP32 && k.precision != PSR_PRECISION_MIXED) return PSR_ERR_INVALID;
+    if (k.precision == PSR_PRECISION_MIXED && !(k.refine_rel_tol > 0.f)) return PSR_ERR_INVALID;
     psr_ctx* ctx = new (std::nothrow) psr_ctx();
     if (!ctx) return PSR_ERR_INTERNAL;
     try {
@@ -278,8 +284,14 @@ int psr_export(psr_ctx* ctx, int id, void* dst, uint64_t bytes, uint64_t* needed
             if (!dst) return PSR_OK;
             if (bytes < size_t(12) * nD) return PSR_ERR_CAPACITY;
             std::vector<float> t(size_t(3) * nD);
-            if (hipMemcpy(t.data(), c.fld.vfield, 12 * size_t(nD), hipMemcpyDeviceToHost) != hipSuccess)
+            if (c.fld.vhalf) {  // FP16 storage of V * 2^-(D+1)
+                std::vector<__half> h(size_t(3) * nD);
+                if (hipMemcpy(h.data(), c.fld.vhalf, 6 * size_t(nD), hipMemcpyDeviceToHost) != hipSuccess)
+                    return PSR_ERR_HIP;
+                for (size_t i = 0; i < h.size(); ++i) t[i] = __half2float(h[i]) * float(2 << D);
+            } else if (hipMemcpy(t.data(), c.fld.vfield, 12 * size_t(nD), hipMemcpyDeviceToHost) != hipSuccess) {
                 return PSR_ERR_HIP;
+            }
             float* out = static_cast<float*>(dst);
             for (int i = 0; i < nD; ++i)
                 for (int a = 0; a < 3; ++a) out[3 * size_t(i) + a] = t[size_t(a) * nD + i];

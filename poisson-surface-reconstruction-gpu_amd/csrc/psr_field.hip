@@ -1,6 +1,8 @@
 // Basis inner-product tables, vector-field splat and divergence.
 #include "psr_internal.h"
 
+#include <hip/hip_fp16.h>
+
 namespace psr {
 
 namespace {
@@ -119,11 +121,20 @@ __device__ __forceinline__ int range_of(const NbRanges& r, int t) {
 // --- vector field (ComputeVectorField.cu:61-100) ---------------------------
 // V(o) = sum over the 27 neighbours' points p of F_o(x)F_o(y)F_o(z) n_p,
 // one wave per depth-D node.
+// Field storage: FP32, or FP16 holding V * 2^-(D+1) (the normals' scale, an
+// exact power of two, so the stored value is the FP16 rounding of the sum of
+// weighted unit normals).
+__device__ __forceinline__ void store_field(float* p, float v, float) { *p = v; }
+__device__ __forceinline__ void store_field(__half* p, float v, float invScale) { *p = __float2half_rn(v * invScale); }
+__device__ __forceinline__ float load_field(const float* p, float) { return *p; }
+__device__ __forceinline__ float load_field(const __half* p, float scale) { return __half2float(*p) * scale; }
+
 constexpr int kVfWaves = 4;
+template <class FieldT>
 __global__ void __launch_bounds__(kVfWaves * PSR_WAVE) vector_field_kernel(
     int nD, int offD, int M, int D, const int* __restrict__ neigh, const int* __restrict__ pidx,
     const int* __restrict__ pnum, const int* __restrict__ key, const float4* __restrict__ pos,
-    const float4* __restrict__ nrm, float* __restrict__ vf) {
+    const float4* __restrict__ nrm, FieldT* __restrict__ vf) {
     __shared__ NbRanges R[kVfWaves];
     int wid = threadIdx.x / PSR_WAVE, lane = __lane_id();
     int node = blockIdx.x * kVfWaves + wid;
@@ -157,30 +168,34 @@ __global__ void __launch_bounds__(kVfWaves * PSR_WAVE) vector_field_kernel(
         az += __shfl_xor(az, off, PSR_WAVE);
     }
     if (lane == 0) {
-        vf[node] = ax;
-        vf[nD + node] = ay;
-        vf[2 * nD + node] = az;
+        const float inv = 1.0f / float(2 << D);
+        store_field(&vf[node], ax, inv);
+        store_field(&vf[nD + node], ay, inv);
+        store_field(&vf[2 * nD + node], az, inv);
     }
 }
 
 // --- divergence (ComputeNodesDivergence.cu:25-136) -------------------------
 // b(o) = sum over the 27 neighbours' depth-D descendants j of
 //        V_j . (<F'_ox, F_jx>, <F'_oy, F_jy>, <F'_oz, F_jz>)     (reference Q3)
+template <class FieldT>
 __device__ __forceinline__ double div_term(int j, int offD, int nD, int res, int D, int3 fo,
-                                           const int* __restrict__ enc, const float* __restrict__ vf,
+                                           const int* __restrict__ enc, const FieldT* __restrict__ vf,
                                            const double* __restrict__ ddot) {
+    const float sc = float(2 << D);
     int3 fj = decode_func_index(enc[offD + j], D);
-    double s = double(vf[j]) * ddot[fo.x + res * fj.x];
-    s += double(vf[nD + j]) * ddot[fo.y + res * fj.y];
-    s += double(vf[2 * nD + j]) * ddot[fo.z + res * fj.z];
+    double s = double(load_field(&vf[j], sc)) * ddot[fo.x + res * fj.x];
+    s += double(load_field(&vf[nD + j], sc)) * ddot[fo.y + res * fj.y];
+    s += double(load_field(&vf[2 * nD + j], sc)) * ddot[fo.z + res * fj.z];
     return s;
 }
 
 constexpr int kDivWaves = 4;
+template <class FieldT>
 __global__ void __launch_bounds__(kDivWaves * PSR_WAVE) divergence_fine(
     int begin, int count, int M, int offD, int nD, int res, int D, const int* __restrict__ neigh,
     const int* __restrict__ didx, const int* __restrict__ dnum, const int* __restrict__ enc,
-    const float* __restrict__ vf, const double* __restrict__ ddot, float* __restrict__ div) {
+    const FieldT* __restrict__ vf, const double* __restrict__ ddot, float* __restrict__ div) {
     __shared__ NbRanges R[kDivWaves];
     int wid = threadIdx.x / PSR_WAVE, lane = __lane_id();
     int node = blockIdx.x * kDivWaves + wid;
@@ -204,9 +219,10 @@ __global__ void __launch_bounds__(kDivWaves * PSR_WAVE) divergence_fine(
 
 // Coarse nodes (few, each covering a large share of depth D): a workgroup each.
 constexpr int kDivCoarseThreads = 1024;
+template <class FieldT>
 __global__ void __launch_bounds__(kDivCoarseThreads) divergence_coarse(
     int count, int M, int offD, int nD, int res, int D, const int* __restrict__ neigh, const int* __restrict__ didx,
-    const int* __restrict__ dnum, const int* __restrict__ enc, const float* __restrict__ vf,
+    const int* __restrict__ dnum, const int* __restrict__ enc, const FieldT* __restrict__ vf,
     const double* __restrict__ ddot, float* __restrict__ div) {
     int o = blockIdx.x;
     if (o >= count) return;
@@ -256,26 +272,44 @@ void build_tables(Ctx& c) {
     PSR_HIP_CHECK(hipGetLastError());
 }
 
-void compute_field(Ctx& c) {
+namespace {
+
+template <class FieldT>
+void field_stages(Ctx& c, FieldT* vf) {
     Octree& o = c.oct;
     const Levels& L = o.L;
     const int D = L.D, nD = L.count[D], offD = L.base[D], M = o.M;
     hipStream_t s = c.stream;
-    c.fld.vfield = c.buf<float>("vfield", size_t(3) * nD);
-    c.fld.div = c.buf<float>("divergence", M);
-    vector_field_kernel<<<div_up(nD, kVfWaves), kVfWaves * PSR_WAVE, 0, s>>>(
-        nD, offD, M, D, o.neigh, o.pidx, o.pnum, o.key, o.spos, o.snrm, c.fld.vfield);
+    vector_field_kernel<FieldT><<<div_up(nD, kVfWaves), kVfWaves * PSR_WAVE, 0, s>>>(
+        nD, offD, M, D, o.neigh, o.pidx, o.pnum, o.key, o.spos, o.snrm, vf);
     // depths 0..kCoarse: one workgroup per node; deeper: one wave per node
     const int kCoarse = D >= 5 ? 3 : 1;
     int coarseCount = L.base[kCoarse + 1];
-    divergence_coarse<<<coarseCount, kDivCoarseThreads, 0, s>>>(coarseCount, M, offD, nD, c.tables.res, D, o.neigh,
-                                                                o.didx, o.dnum, o.encIdx, c.fld.vfield,
-                                                                c.tables.ddot, c.fld.div);
+    divergence_coarse<FieldT><<<coarseCount, kDivCoarseThreads, 0, s>>>(coarseCount, M, offD, nD, c.tables.res, D,
+                                                                        o.neigh, o.didx, o.dnum, o.encIdx, vf,
+                                                                        c.tables.ddot, c.fld.div);
     int fineCount = M - coarseCount;
-    divergence_fine<<<div_up(fineCount, kDivWaves), kDivWaves * PSR_WAVE, 0, s>>>(
-        coarseCount, fineCount, M, offD, nD, c.tables.res, D, o.neigh, o.didx, o.dnum, o.encIdx, c.fld.vfield,
-        c.tables.ddot, c.fld.div);
+    divergence_fine<FieldT><<<div_up(fineCount, kDivWaves), kDivWaves * PSR_WAVE, 0, s>>>(
+        coarseCount, fineCount, M, offD, nD, c.tables.res, D, o.neigh, o.didx, o.dnum, o.encIdx, vf, c.tables.ddot,
+        c.fld.div);
     PSR_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace
+
+void compute_field(Ctx& c) {
+    const Levels& L = c.oct.L;
+    const int nD = L.count[L.D];
+    c.fld.div = c.buf<float>("divergence", c.oct.M);
+    if (c.cfg.precision == PSR_PRECISION_MIXED) {
+        c.fld.vfield = nullptr;
+        c.fld.vhalf = c.buf<__half>("vfield_half", size_t(3) * nD);
+        field_stages(c, c.fld.vhalf);
+    } else {
+        c.fld.vhalf = nullptr;
+        c.fld.vfield = c.buf<float>("vfield", size_t(3) * nD);
+        field_stages(c, c.fld.vfield);
+    }
 }
 
 }  // namespace psr

@@ -7,6 +7,8 @@
 #include "../../include/psr.h"
 #include "psr_common.h"
 
+#include <hip/hip_fp16.h>
+
 namespace psr {
 
 // Grow-only device arena: one allocation per named buffer, reused across
@@ -64,7 +66,8 @@ struct Octree {
 };
 
 struct Field {
-    float* vfield = nullptr;  // [3][nD]
+    float* vfield = nullptr;  // [3][nD] (FP32 form)
+    __half* vhalf = nullptr;  // [3][nD], V * 2^-(D+1) (mixed-precision form)
     float* div = nullptr;     // [M]
     float* dx = nullptr;      // [M]
     float* chiPts = nullptr;  // [N]

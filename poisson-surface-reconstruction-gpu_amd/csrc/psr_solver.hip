@@ -58,6 +58,8 @@ struct CgParams {
     float tolSq;     // reference: r1 > tol * tol with float tol (CGAlgorithm.cu:36)
     int maxIter;
     unsigned spinLimit;
+    double relTol;   // mixed form: target ||b - A x|| / ||b|| per depth
+    int maxOuter;    // mixed form: refinement steps
 };
 
 constexpr int kCgThreads = 256;
@@ -216,6 +218,234 @@ __global__ void __launch_bounds__(kCgThreads) cg_all_depths(CgParams P, const in
     }
 }
 
+// --- mixed-precision CG ------------------------------------------------------
+// Same system per depth, solved to a true relative residual:
+//   outer (FP64): r = b - A x accumulated in double, ||r|| / ||b|| in double;
+//   inner (FP32): Chronopoulos-Gear CG on A e = r, one group reduction per
+//                 iteration, dot products as compensated FP32 sums (Dot2:
+//                 TwoProduct by fma, TwoSum accumulation).
+// One barrier per inner iteration is possible because every value a row
+// reads from its neighbours was written before the previous barrier: the
+// neighbour's next residual is formed on the fly as
+//   r_{k+1} = r_k - alpha_k (w_k + beta_k s_{k-1}),  w = A r,  s = A p,
+// with r, w and s double-buffered by iteration parity.
+
+__device__ __forceinline__ void two_sum_acc(float& s, float& c, float v) {
+#pragma clang fp contract(off)
+    float t = s + v;
+    float bv = t - s;
+    float e = (s - (t - bv)) + (v - bv);
+    s = t;
+    c += e;
+}
+__device__ __forceinline__ void dot2_acc(float& s, float& c, float a, float b) {
+#pragma clang fp contract(off)
+    float p = a * b;
+    float e = __fmaf_rn(a, b, -p);
+    two_sum_acc(s, c, p);
+    c += e;
+}
+__device__ __forceinline__ void comp_merge(float& s, float& c, float s2, float c2) {
+#pragma clang fp contract(off)
+    two_sum_acc(s, c, s2);
+    c += c2;
+}
+// s_k = w_k + beta s_{k-1};  r_{k+1} = r_k - alpha s_k (owner and neighbours use the same operations)
+__device__ __forceinline__ float s_next(float w, float beta, float sPrev) { return __fmaf_rn(beta, sPrev, w); }
+__device__ __forceinline__ float r_next(float r, float alpha, float s) { return __fmaf_rn(-alpha, s, r); }
+
+struct Comp2 {
+    float gs, gc, ds, dc;  // gamma = gs + gc, delta = ds + dc
+};
+
+__device__ __forceinline__ void comp2_merge(Comp2& a, const Comp2& b) {
+    comp_merge(a.gs, a.gc, b.gs, b.gc);
+    comp_merge(a.ds, a.dc, b.ds, b.dc);
+}
+
+// Block-wide merge in a fixed order; every thread returns the same value.
+__device__ __forceinline__ Comp2 block_comp2(Comp2 v, Comp2* sh) {
+    for (int off = PSR_WAVE / 2; off > 0; off >>= 1) {
+        Comp2 o{__shfl_xor(v.gs, off, PSR_WAVE), __shfl_xor(v.gc, off, PSR_WAVE), __shfl_xor(v.ds, off, PSR_WAVE),
+                __shfl_xor(v.dc, off, PSR_WAVE)};
+        comp2_merge(v, o);
+    }
+    int w = threadIdx.x / PSR_WAVE;
+    if (__lane_id() == 0) sh[w] = v;
+    __syncthreads();
+    Comp2 t = sh[0];
+    for (int k = 1; k < kCgWaves; ++k) comp2_merge(t, sh[k]);
+    __syncthreads();
+    return t;
+}
+
+__global__ void __launch_bounds__(kCgThreads) cg_mixed_all_depths(
+    CgParams P, const int* __restrict__ neigh, const float* __restrict__ val, const float* __restrict__ b,
+    float* x, float* rOut, float* rbuf, float* wbuf, float* sbuf, float* __restrict__ pvec, double* partialD,
+    float* partialF, unsigned* bar, int* __restrict__ iters, double* __restrict__ resid,
+    double* __restrict__ trueRel, int* errFlag) {
+    __shared__ double shd[kCgWaves];
+    __shared__ Comp2 shc[kCgWaves];
+    __shared__ int shFlag;
+    int gi = 0;
+    while (gi + 1 < P.ngroups && int(blockIdx.x) >= P.g[gi + 1].firstWg) ++gi;
+    const CgGroup G = P.g[gi];
+    const int local = blockIdx.x - G.firstWg;
+    const int r0 = G.rowBegin + local * G.rowsPerWg;
+    const int r1 = min(G.rowBegin + G.rows, r0 + G.rowsPerWg);
+    const int M = P.M;
+    double* partD = partialD + 2 * G.firstWg;
+    float* partF = partialF + 8 * G.firstWg;
+    unsigned* cnt = bar + gi;
+    unsigned phase = 0;
+    const bool multi = G.wgs > 1;
+
+    auto sum_double = [&](double v, double& out) -> bool {
+        double s = block_sum_double(v, shd);
+        if (!multi) {
+            out = s;
+            return true;
+        }
+        double* half = partD + (phase & 1u) * unsigned(G.wgs);
+        if (threadIdx.x == 0) __hip_atomic_store(&half[local], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ++phase;
+        if (!group_barrier(cnt, phase * unsigned(G.wgs), P.spinLimit, errFlag, &shFlag)) return false;
+        double t = 0;
+        for (int k = threadIdx.x; k < G.wgs; k += kCgThreads)
+            t += __hip_atomic_load(&half[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        out = block_sum_double(t, shd);
+        return true;
+    };
+    auto sum_comp = [&](Comp2 v, Comp2& out) -> bool {
+        Comp2 s = block_comp2(v, shc);
+        if (!multi) {
+            out = s;
+            return true;
+        }
+        float* half = partF + (phase & 1u) * unsigned(4 * G.wgs);
+        if (threadIdx.x < 4) {
+            float e = threadIdx.x == 0 ? s.gs : threadIdx.x == 1 ? s.gc : threadIdx.x == 2 ? s.ds : s.dc;
+            __hip_atomic_store(&half[4 * local + threadIdx.x], e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        ++phase;
+        if (!group_barrier(cnt, phase * unsigned(G.wgs), P.spinLimit, errFlag, &shFlag)) return false;
+        Comp2 t{0.f, 0.f, 0.f, 0.f};
+        for (int k = threadIdx.x; k < G.wgs; k += kCgThreads) {
+            Comp2 o{ld_shared(&half[4 * k]), ld_shared(&half[4 * k + 1]), ld_shared(&half[4 * k + 2]),
+                    ld_shared(&half[4 * k + 3])};
+            comp2_merge(t, o);
+        }
+        out = block_comp2(t, shc);
+        return true;
+    };
+
+    // x = 0; ||b||^2 in FP64 (the outer residual of x = 0 is b itself)
+    double loc = 0;
+    for (int i = r0 + threadIdx.x; i < r1; i += kCgThreads) {
+        st_shared(&x[i], 0.f);
+        double bi = b[i];
+        loc += bi * bi;
+    }
+    double bb;
+    if (!sum_double(loc, bb)) return;
+    double rr = bb, rel = bb > 0 ? 1.0 : 0.0;
+    const float* rhs = b;
+    int total = 0, outer = 0;
+    float gamma = 0.f;
+    while (rel > P.relTol && outer < P.maxOuter && total < P.maxIter) {
+        // inner target on the updated residual: half the tolerance left, no tighter than FP32 can follow
+        const double tgt = fmax(0.5 * P.relTol * sqrt(bb), 1e-6 * sqrt(rr));
+        const float tgt2 = float(tgt * tgt);
+        // r_0 = rhs, w_0 = A r_0, s_{-1} = 0, p_{-1} = 0
+        Comp2 d0{0.f, 0.f, 0.f, 0.f};
+        for (int i = r0 + threadIdx.x; i < r1; i += kCgThreads) {
+            float w = 0.f;
+#pragma unroll
+            for (int q = 0; q < 27; ++q) {
+                float v = val[q * M + i];
+                int j = neigh[q * M + i];
+                j = (v != 0.f) ? j : i;
+                w = __fmaf_rn(v, ld_shared(&rhs[j]), w);
+            }
+            float ri = ld_shared(&rhs[i]);
+            st_shared(&rbuf[i], ri);
+            st_shared(&wbuf[i], w);
+            st_shared(&sbuf[M + i], 0.f);
+            pvec[i] = 0.f;
+            dot2_acc(d0.gs, d0.gc, ri, ri);
+            dot2_acc(d0.ds, d0.dc, w, ri);
+        }
+        Comp2 g;
+        if (!sum_comp(d0, g)) return;
+        gamma = g.gs + g.gc;
+        float delta = g.ds + g.dc;
+        float alpha = gamma / delta, beta = 0.f;
+        int k = 0;
+        while (gamma > tgt2 && total + k < P.maxIter) {
+            const float* rk = rbuf + (k & 1) * M;
+            const float* wk = wbuf + (k & 1) * M;
+            const float* sp = sbuf + ((k + 1) & 1) * M;  // s_{k-1}
+            float* rn = rbuf + ((k + 1) & 1) * M;
+            float* wn = wbuf + ((k + 1) & 1) * M;
+            float* sk = sbuf + (k & 1) * M;
+            Comp2 dl{0.f, 0.f, 0.f, 0.f};
+            for (int i = r0 + threadIdx.x; i < r1; i += kCgThreads) {
+                float w = 0.f;
+#pragma unroll
+                for (int q = 0; q < 27; ++q) {
+                    float v = val[q * M + i];
+                    int j = neigh[q * M + i];
+                    j = (v != 0.f) ? j : i;
+                    float rj = r_next(ld_shared(&rk[j]), alpha, s_next(ld_shared(&wk[j]), beta, ld_shared(&sp[j])));
+                    w = __fmaf_rn(v, rj, w);
+                }
+                float ri = ld_shared(&rk[i]);
+                float si = s_next(ld_shared(&wk[i]), beta, ld_shared(&sp[i]));
+                float rni = r_next(ri, alpha, si);
+                float pi = __fmaf_rn(beta, pvec[i], ri);
+                pvec[i] = pi;
+                st_shared(&x[i], __fmaf_rn(alpha, pi, ld_shared(&x[i])));
+                st_shared(&sk[i], si);
+                st_shared(&rn[i], rni);
+                st_shared(&wn[i], w);
+                dot2_acc(dl.gs, dl.gc, rni, rni);
+                dot2_acc(dl.ds, dl.dc, w, rni);
+            }
+            if (!sum_comp(dl, g)) return;
+            const float gNew = g.gs + g.gc, dNew = g.ds + g.dc;
+            const float bNew = gNew / gamma;
+            alpha = gNew / (dNew - bNew * gNew / alpha);
+            beta = bNew;
+            gamma = gNew;
+            ++k;
+        }
+        total += k;
+        ++outer;
+        // outer residual in FP64
+        loc = 0;
+        for (int i = r0 + threadIdx.x; i < r1; i += kCgThreads) {
+            double acc = b[i];
+#pragma unroll
+            for (int q = 0; q < 27; ++q) {
+                float v = val[q * M + i];
+                int j = neigh[q * M + i];
+                j = (v != 0.f) ? j : i;
+                acc -= double(v) * double(ld_shared(&x[j]));
+            }
+            st_shared(&rOut[i], float(acc));
+            loc += acc * acc;
+        }
+        if (!sum_double(loc, rr)) return;
+        rel = bb > 0 ? sqrt(rr / bb) : 0.0;
+        rhs = rOut;
+    }
+    if (local == 0 && threadIdx.x == 0) {
+        iters[G.depth] = total;
+        resid[G.depth] = sqrt(double(gamma));
+        trueRel[G.depth] = rel;
+    }
+}
+
 // --- implicit function at points (LaplacianSolver.cu:85-118, 269-305) ------
 __device__ __forceinline__ float chi_sum_node(int node, float3 p, int M, int D, const int* __restrict__ neigh,
                                               const int* __restrict__ enc, const float* __restrict__ dx) {
@@ -325,7 +555,10 @@ void solve_depths(Ctx& c) {
     // workgroup budget: every workgroup of the cooperative launch must be resident
     if (c.cgMaxWgs <= 0) {
         int perCu = 0, cus = 0;
+        int perCuMixed = 0;
         PSR_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, cg_all_depths, kCgThreads, 0));
+        PSR_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCuMixed, cg_mixed_all_depths, kCgThreads, 0));
+        perCu = std::min(perCu, perCuMixed);
         PSR_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device));
         c.cgMaxWgs = std::max(1, perCu * cus);
     }
@@ -357,13 +590,30 @@ void solve_depths(Ctx& c) {
     unsigned* bar = c.buf<unsigned>("cg_bar", PSR_MAX_DEPTH + 1);
     int* iters = c.buf<int>("cg_iters", PSR_MAX_DEPTH + 1);
     double* resid = c.buf<double>("cg_resid", PSR_MAX_DEPTH + 1);
+    double* trueRel = c.buf<double>("cg_truerel", PSR_MAX_DEPTH + 1);
     int* err = c.buf<int>("cg_err", 1);
     zero_u32<<<1, 64, 0, s>>>(bar, PSR_MAX_DEPTH + 1);
     zero_u32<<<1, 64, 0, s>>>(reinterpret_cast<unsigned*>(err), 1);
-    void* args[] = {&P, &o.neigh, &val, &c.fld.div, &c.fld.dx, &r, &pA, &pB, &Ap, &partial, &bar, &iters, &resid, &err};
+    zero_u32<<<1, 64, 0, s>>>(reinterpret_cast<unsigned*>(trueRel), 2 * (PSR_MAX_DEPTH + 1));
     PSR_HIP_CHECK(hipEventRecord(c.ev[8], s));
-    PSR_HIP_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(cg_all_depths), dim3(wg), dim3(kCgThreads),
-                                             args, 0, s));
+    if (c.cfg.precision == PSR_PRECISION_MIXED) {
+        P.relTol = c.cfg.refine_rel_tol;
+        P.maxOuter = std::max(1, c.cfg.refine_max_outer);
+        float* rOut = c.buf<float>("cg_rout", M);
+        float* rbuf = c.buf<float>("cg_rbuf", size_t(2) * M);
+        float* wbuf = c.buf<float>("cg_wbuf", size_t(2) * M);
+        float* sbuf = c.buf<float>("cg_sbuf", size_t(2) * M);
+        float* partialF = c.buf<float>("cg_partial_f", size_t(8) * wg);
+        void* args[] = {&P,   &o.neigh, &val,  &c.fld.div, &c.fld.dx, &rOut,  &rbuf,   &wbuf,   &sbuf,
+                        &pA,  &partial, &partialF, &bar,   &iters,    &resid, &trueRel, &err};
+        PSR_HIP_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(cg_mixed_all_depths), dim3(wg),
+                                                 dim3(kCgThreads), args, 0, s));
+    } else {
+        void* args[] = {&P, &o.neigh, &val, &c.fld.div, &c.fld.dx, &r, &pA, &pB, &Ap, &partial, &bar, &iters, &resid,
+                        &err};
+        PSR_HIP_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(cg_all_depths), dim3(wg),
+                                                 dim3(kCgThreads), args, 0, s));
+    }
     PSR_HIP_CHECK(hipEventRecord(c.ev[9], s));
     PSR_HIP_CHECK(hipGetLastError());
 }
