@@ -2,6 +2,7 @@
 // mesh/PoissonReconstruction.cpp:195-245): octree -> field -> divergence ->
 // per-depth CG -> implicit function -> geometry -> marching cubes.
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <stdexcept>
@@ -185,6 +186,8 @@ int psr_create(const psr_config* cfg, void* hip_stream, psr_ctx** out) {
     try {
         psr::Ctx& c = ctx->c;
         c.cfg = k;
+        // tuning override of the CG row-block size (rows per workgroup)
+        if (const char* e = std::getenv("PSR_CG_ROWS_PER_WG")) c.cgRowsPerWg = std::max(0, std::atoi(e));
         PSR_HIP_CHECK(hipGetDevice(&c.device));
         if (hip_stream) {
             c.stream = static_cast<hipStream_t>(hip_stream);

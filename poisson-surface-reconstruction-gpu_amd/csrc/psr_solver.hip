@@ -87,18 +87,73 @@ __device__ __forceinline__ double block_sum_double(double v, double* sh) {
     return s;
 }
 
-// Group barrier: monotonic arrival counter per group.  Returns false if the
-// spin limit was hit (then the error flag is set and the caller stops).
-__device__ __forceinline__ bool group_barrier(unsigned* counter, unsigned target, unsigned spinLimit,
-                                              int* errFlag, int* shFlag) {
+// Per-group control block; each group's counters sit on their own cache lines.
+struct GroupCtl {
+    unsigned cnt;         // arrivals, monotonic: phase p is complete at p * workgroups
+    unsigned pad0[31];
+    unsigned gen;         // last phase whose total has been published
+    unsigned pad1[31];
+    unsigned res[2][32];  // published totals by phase parity
+};
+static_assert(sizeof(GroupCtl) == 512, "GroupCtl layout");
+
+template <class T>
+__device__ __forceinline__ void st_agent(T* p, const T& v) {
+    static_assert(sizeof(T) % 4 == 0, "word-sized values");
+    const unsigned* s = reinterpret_cast<const unsigned*>(&v);
+    unsigned* d = reinterpret_cast<unsigned*>(p);
+#pragma unroll
+    for (int k = 0; k < int(sizeof(T) / 4); ++k) __hip_atomic_store(&d[k], s[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ __forceinline__ T ld_agent(const T* p) {
+    T v;
+    unsigned* d = reinterpret_cast<unsigned*>(&v);
+    const unsigned* s = reinterpret_cast<const unsigned*>(p);
+#pragma unroll
+    for (int k = 0; k < int(sizeof(T) / 4); ++k) d[k] = __hip_atomic_load(&s[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return v;
+}
+
+// Group all-reduce and barrier in one.  Every workgroup of the group stores its
+// block total `mine` and arrives; the last to arrive folds the partials in a
+// fixed order (fold(parts, n), called by the whole block, returns the same
+// value in every thread), publishes the total and the phase; the others wait
+// for the phase and read the total.  Every workgroup therefore sees the same
+// bits.  The wait is bounded by spinLimit; on expiry (or when another
+// workgroup has failed) the error flag is set and false returned.
+template <class T, class Fold>
+__device__ __forceinline__ bool group_allreduce(const T& mine, T& out, T* partial, GroupCtl* ctl, unsigned& phase,
+                                                int G, int local, unsigned spinLimit, int* errFlag, int* shInt,
+                                                T* shT, Fold fold) {
+    ++phase;
+    T* half = partial + (phase & 1u) * unsigned(G);
+    if (threadIdx.x == 0) st_agent(&half[local], mine);
     __builtin_amdgcn_s_waitcnt(0);  // this wave's agent-scope stores have completed
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
     __syncthreads();
     if (threadIdx.x == 0) {
-        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned old = __hip_atomic_fetch_add(&ctl->cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        *shInt = (old == phase * unsigned(G) - 1u) ? 1 : 0;
+    }
+    __syncthreads();
+    T* res = reinterpret_cast<T*>(ctl->res[phase & 1u]);
+    if (*shInt) {
+        T t = fold(half, G);
+        if (threadIdx.x == 0) {
+            st_agent(res, t);
+            __builtin_amdgcn_s_waitcnt(0);
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            __hip_atomic_store(&ctl->gen, phase, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        out = t;
+        __syncthreads();
+        return true;
+    }
+    if (threadIdx.x == 0) {
         unsigned spins = 0;
         int ok = 1;
-        while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        while (__hip_atomic_load(&ctl->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < phase) {
             if (++spins > spinLimit || __hip_atomic_load(errFlag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
                 ok = 0;
                 __hip_atomic_store(errFlag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -106,11 +161,21 @@ __device__ __forceinline__ bool group_barrier(unsigned* counter, unsigned target
             }
             __builtin_amdgcn_s_sleep(1);
         }
-        *shFlag = ok;
+        if (ok) *shT = ld_agent(res);
+        *shInt = ok;
     }
     __syncthreads();
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    return *shFlag != 0;
+    const bool ok = *shInt != 0;
+    out = *shT;
+    __syncthreads();
+    return ok;
+}
+
+__device__ __forceinline__ double fold_double(const double* parts, int n, double* sh) {
+    double t = 0;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) t += ld_agent(&parts[k]);
+    return block_sum_double(t, sh);
 }
 
 __device__ __forceinline__ float p_next(float r, float beta, float p) { return r + beta * p; }
@@ -120,10 +185,11 @@ __global__ void __launch_bounds__(kCgThreads) cg_all_depths(CgParams P, const in
                                                             const float* __restrict__ b, float* __restrict__ x,
                                                             float* r, float* pA, float* pB,
                                                             float* __restrict__ Ap, double* partial,
-                                                            unsigned* bar, int* __restrict__ iters,
+                                                            GroupCtl* ctlAll, int* __restrict__ iters,
                                                             double* __restrict__ resid, int* errFlag) {
     __shared__ double sh[kCgWaves];
     __shared__ int shFlag;
+    __shared__ double shRes;
     int gi = 0;
     while (gi + 1 < P.ngroups && int(blockIdx.x) >= P.g[gi + 1].firstWg) ++gi;
     const CgGroup G = P.g[gi];
@@ -132,28 +198,19 @@ __global__ void __launch_bounds__(kCgThreads) cg_all_depths(CgParams P, const in
     const int r1 = min(G.rowBegin + G.rows, r0 + G.rowsPerWg);
     const int M = P.M;
     double* part = partial + 2 * G.firstWg;
-    unsigned* cnt = bar + gi;
+    GroupCtl* ctl = ctlAll + gi;
     unsigned phase = 0;
     const bool multi = G.wgs > 1;
 
-    // Sum over the group.  Partials are double-buffered by phase parity: a
-    // workgroup that runs ahead writes the other half while slower ones still
-    // read this one.
+    // Sum over the group (partials double-buffered by phase parity).
     auto group_sum = [&](double v, double& out) -> bool {
         double s = block_sum_double(v, sh);
         if (!multi) {
             out = s;
             return true;
         }
-        double* half = part + (phase & 1u) * unsigned(G.wgs);
-        if (threadIdx.x == 0) __hip_atomic_store(&half[local], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        ++phase;
-        if (!group_barrier(cnt, phase * unsigned(G.wgs), P.spinLimit, errFlag, &shFlag)) return false;
-        double t = 0;
-        for (int k = threadIdx.x; k < G.wgs; k += kCgThreads)
-            t += __hip_atomic_load(&half[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        out = block_sum_double(t, sh);
-        return true;
+        return group_allreduce(s, out, part, ctl, phase, G.wgs, local, P.spinLimit, errFlag, &shFlag, &shRes,
+                               [&](const double* parts, int n) { return fold_double(parts, n, sh); });
     };
 
     // x = 0, r = b, rr = r.r  (CGAlgorithm.cu:21-33 with x0 = 0)
@@ -282,11 +339,13 @@ __device__ __forceinline__ Comp2 block_comp2(Comp2 v, Comp2* sh) {
 __global__ void __launch_bounds__(kCgThreads) cg_mixed_all_depths(
     CgParams P, const int* __restrict__ neigh, const float* __restrict__ val, const float* __restrict__ b,
     float* x, float* rOut, float* rbuf, float* wbuf, float* sbuf, float* __restrict__ pvec, double* partialD,
-    float* partialF, unsigned* bar, int* __restrict__ iters, double* __restrict__ resid,
+    Comp2* partialC, GroupCtl* ctlAll, int* __restrict__ iters, double* __restrict__ resid,
     double* __restrict__ trueRel, int* errFlag) {
     __shared__ double shd[kCgWaves];
     __shared__ Comp2 shc[kCgWaves];
     __shared__ int shFlag;
+    __shared__ double shResD;
+    __shared__ Comp2 shResC;
     int gi = 0;
     while (gi + 1 < P.ngroups && int(blockIdx.x) >= P.g[gi + 1].firstWg) ++gi;
     const CgGroup G = P.g[gi];
@@ -295,8 +354,8 @@ __global__ void __launch_bounds__(kCgThreads) cg_mixed_all_depths(
     const int r1 = min(G.rowBegin + G.rows, r0 + G.rowsPerWg);
     const int M = P.M;
     double* partD = partialD + 2 * G.firstWg;
-    float* partF = partialF + 8 * G.firstWg;
-    unsigned* cnt = bar + gi;
+    Comp2* partC = partialC + 2 * G.firstWg;
+    GroupCtl* ctl = ctlAll + gi;
     unsigned phase = 0;
     const bool multi = G.wgs > 1;
 
@@ -306,15 +365,8 @@ __global__ void __launch_bounds__(kCgThreads) cg_mixed_all_depths(
             out = s;
             return true;
         }
-        double* half = partD + (phase & 1u) * unsigned(G.wgs);
-        if (threadIdx.x == 0) __hip_atomic_store(&half[local], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        ++phase;
-        if (!group_barrier(cnt, phase * unsigned(G.wgs), P.spinLimit, errFlag, &shFlag)) return false;
-        double t = 0;
-        for (int k = threadIdx.x; k < G.wgs; k += kCgThreads)
-            t += __hip_atomic_load(&half[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        out = block_sum_double(t, shd);
-        return true;
+        return group_allreduce(s, out, partD, ctl, phase, G.wgs, local, P.spinLimit, errFlag, &shFlag, &shResD,
+                               [&](const double* parts, int n) { return fold_double(parts, n, shd); });
     };
     auto sum_comp = [&](Comp2 v, Comp2& out) -> bool {
         Comp2 s = block_comp2(v, shc);
@@ -322,21 +374,12 @@ __global__ void __launch_bounds__(kCgThreads) cg_mixed_all_depths(
             out = s;
             return true;
         }
-        float* half = partF + (phase & 1u) * unsigned(4 * G.wgs);
-        if (threadIdx.x < 4) {
-            float e = threadIdx.x == 0 ? s.gs : threadIdx.x == 1 ? s.gc : threadIdx.x == 2 ? s.ds : s.dc;
-            __hip_atomic_store(&half[4 * local + threadIdx.x], e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        ++phase;
-        if (!group_barrier(cnt, phase * unsigned(G.wgs), P.spinLimit, errFlag, &shFlag)) return false;
-        Comp2 t{0.f, 0.f, 0.f, 0.f};
-        for (int k = threadIdx.x; k < G.wgs; k += kCgThreads) {
-            Comp2 o{ld_shared(&half[4 * k]), ld_shared(&half[4 * k + 1]), ld_shared(&half[4 * k + 2]),
-                    ld_shared(&half[4 * k + 3])};
-            comp2_merge(t, o);
-        }
-        out = block_comp2(t, shc);
-        return true;
+        return group_allreduce(s, out, partC, ctl, phase, G.wgs, local, P.spinLimit, errFlag, &shFlag, &shResC,
+                               [&](const Comp2* parts, int n) {
+                                   Comp2 t{0.f, 0.f, 0.f, 0.f};
+                                   for (int k = threadIdx.x; k < n; k += kCgThreads) comp2_merge(t, ld_agent(&parts[k]));
+                                   return block_comp2(t, shc);
+                               });
     };
 
     // x = 0; ||b||^2 in FP64 (the outer residual of x = 0 is b itself)
@@ -562,7 +605,7 @@ void solve_depths(Ctx& c) {
         PSR_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device));
         c.cgMaxWgs = std::max(1, perCu * cus);
     }
-    int rowsPerWg = c.cgRowsPerWg > 0 ? c.cgRowsPerWg : 512;
+    int rowsPerWg = c.cgRowsPerWg > 0 ? c.cgRowsPerWg : 256;
     {
         // widen the row blocks until every group fits the resident budget
         auto need = [&](int rpw) {
@@ -587,12 +630,13 @@ void solve_depths(Ctx& c) {
     if (wg > c.cgMaxWgs) throw std::runtime_error("CG launch exceeds the resident workgroup budget");
     c.stats.cg_workgroups = uint32_t(wg);
     double* partial = c.buf<double>("cg_partial", size_t(2) * wg);
-    unsigned* bar = c.buf<unsigned>("cg_bar", PSR_MAX_DEPTH + 1);
+    GroupCtl* ctl = c.buf<GroupCtl>("cg_ctl", PSR_MAX_DEPTH + 1);
+    const int ctlWords = int(sizeof(GroupCtl) / 4) * (PSR_MAX_DEPTH + 1);
     int* iters = c.buf<int>("cg_iters", PSR_MAX_DEPTH + 1);
     double* resid = c.buf<double>("cg_resid", PSR_MAX_DEPTH + 1);
     double* trueRel = c.buf<double>("cg_truerel", PSR_MAX_DEPTH + 1);
     int* err = c.buf<int>("cg_err", 1);
-    zero_u32<<<1, 64, 0, s>>>(bar, PSR_MAX_DEPTH + 1);
+    zero_u32<<<div_up(ctlWords, 256), 256, 0, s>>>(reinterpret_cast<unsigned*>(ctl), ctlWords);
     zero_u32<<<1, 64, 0, s>>>(reinterpret_cast<unsigned*>(err), 1);
     zero_u32<<<1, 64, 0, s>>>(reinterpret_cast<unsigned*>(trueRel), 2 * (PSR_MAX_DEPTH + 1));
     PSR_HIP_CHECK(hipEventRecord(c.ev[8], s));
@@ -603,13 +647,13 @@ void solve_depths(Ctx& c) {
         float* rbuf = c.buf<float>("cg_rbuf", size_t(2) * M);
         float* wbuf = c.buf<float>("cg_wbuf", size_t(2) * M);
         float* sbuf = c.buf<float>("cg_sbuf", size_t(2) * M);
-        float* partialF = c.buf<float>("cg_partial_f", size_t(8) * wg);
-        void* args[] = {&P,   &o.neigh, &val,  &c.fld.div, &c.fld.dx, &rOut,  &rbuf,   &wbuf,   &sbuf,
-                        &pA,  &partial, &partialF, &bar,   &iters,    &resid, &trueRel, &err};
+        Comp2* partialC = c.buf<Comp2>("cg_partial_c", size_t(2) * wg);
+        void* args[] = {&P,  &o.neigh, &val,      &c.fld.div, &c.fld.dx, &rOut,  &rbuf,    &wbuf, &sbuf,
+                        &pA, &partial, &partialC, &ctl,       &iters,    &resid, &trueRel, &err};
         PSR_HIP_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(cg_mixed_all_depths), dim3(wg),
                                                  dim3(kCgThreads), args, 0, s));
     } else {
-        void* args[] = {&P, &o.neigh, &val, &c.fld.div, &c.fld.dx, &r, &pA, &pB, &Ap, &partial, &bar, &iters, &resid,
+        void* args[] = {&P, &o.neigh, &val, &c.fld.div, &c.fld.dx, &r, &pA, &pB, &Ap, &partial, &ctl, &iters, &resid,
                         &err};
         PSR_HIP_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(cg_all_depths), dim3(wg),
                                                  dim3(kCgThreads), args, 0, s));
