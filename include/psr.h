@@ -46,8 +46,8 @@ typedef struct {
     int cg_max_iter;       /* 10000 (CGAlgorithm.cu:15) */
     float refine_rel_tol;  /* PSR_PRECISION_MIXED: target true relative residual per depth, 1e-5 */
     int refine_max_outer;  /* PSR_PRECISION_MIXED: outer refinement steps, 4 */
-    uint32_t subdiv_chunk_nodes; /* virtual nodes per subdivision working set; 0 = 16M (about 2 GB of
-                                    neighbour tables); results do not depend on it */
+    uint32_t subdiv_chunk_nodes; /* virtual depth-D cells per subdivision working set; 0 = 16M
+                                    (about 3 GB); results do not depend on it */
 } psr_config;
 
 typedef struct {

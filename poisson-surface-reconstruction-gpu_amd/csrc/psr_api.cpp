@@ -128,6 +128,7 @@ int run(psr_ctx* ctx, const psr_points* pts, psr_mesh* out) {
         st.cg_true_relres[d] = c.cfg.precision == PSR_PRECISION_MIXED ? trueRel[d] : -1.0;
     }
     PSR_HIP_CHECK(hipMemcpy(&st.iso_value, c.arena.get("iso_value", 4), sizeof(float), hipMemcpyDeviceToHost));
+    PSR_HIP_CHECK(hipMemcpy(&st.q1_fired, o.q1Fired, sizeof(int), hipMemcpyDeviceToHost));
     st.n_vertex_nodes = c.geo.V;
     st.n_edge_nodes = c.geo.E;
     st.n_face_nodes = c.geo.F;

@@ -100,6 +100,32 @@ __device__ __forceinline__ float basis_value(int fidx, float x) {
     return bspline((x - c) * invw);
 }
 
+// Basis parameters of a node: per-axis centres of its 1-D functions and the
+// shared inverse width (func_center_invw of each axis index), so that
+// node_term(...) equals dx * basis_value(fx, x) * basis_value(fy, y) * basis_value(fz, z).
+__device__ __forceinline__ float4 node_basis_params(int3 f) {
+    float4 b;
+    func_center_invw(f.x, b.x, b.w);
+    func_center_invw(f.y, b.y, b.w);
+    func_center_invw(f.z, b.z, b.w);
+    return b;
+}
+__device__ __forceinline__ float node_term(float4 b, float dx, float3 p) {
+    return dx * bspline((p.x - b.x) * b.w) * bspline((p.y - b.y) * b.w) * bspline((p.z - b.z) * b.w);
+}
+
+// sum over the 27 neighbours of `node` of dx * F(x) F(y) F(z)
+__device__ __forceinline__ float chi_node_sum(int node, float3 p, int M, const int* __restrict__ neigh,
+                                              const float4* __restrict__ basis, const float* __restrict__ dx) {
+    float v = 0.f;
+    for (int i = 0; i < 27; ++i) {
+        int nb = neigh[i * M + node];
+        if (nb == -1) continue;
+        v += node_term(basis[nb], dx[nb], p);
+    }
+    return v;
+}
+
 // 3x3x3 neighbourhood look-up tables (derived; BuildOctree.cu:20-42 lists the
 // same values): neighbour i of a child with son key s is child LUTchild of the
 // parent's neighbour LUTparent.

@@ -52,6 +52,8 @@ struct Octree {
     int* codes = nullptr;            // sorted Morton codes
     int* order = nullptr;            // sorted -> input index
     int* point2node = nullptr;       // sorted point -> level-local depth-D node
+    float4* basis = nullptr;         // per node: 1-D basis centres (x, y, z) and inverse width
+    int* q1Fired = nullptr;          // device flag: the level-local child 0 parent rule changed a parent
     int* key = nullptr;
     int* pidx = nullptr;
     int* pnum = nullptr;

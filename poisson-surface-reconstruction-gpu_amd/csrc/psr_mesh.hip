@@ -224,7 +224,8 @@ __global__ void __launch_bounds__(kThreads) vertex_values(NodeView v, const int*
                                                           const float4* __restrict__ vPos,
                                                           const int* __restrict__ parent,
                                                           const int* __restrict__ childBase,
-                                                          const int* __restrict__ enc, const float* __restrict__ dx,
+                                                          const float4* __restrict__ basis,
+                                                          const float* __restrict__ dx,
                                                           const float* __restrict__ iso, float* __restrict__ vv) {
     int t = blockIdx.x * kThreads + threadIdx.x;
     if (t >= *count) return;
@@ -236,7 +237,7 @@ __global__ void __launch_bounds__(kThreads) vertex_values(NodeView v, const int*
     if (owner > 0) {
         int now = owner;
         while (now != -1) {
-            val += chi_node_sum(now, p, v.M, v.D, v.neigh, enc, dx);
+            val += chi_node_sum(now, p, v.M, v.neigh, basis, dx);
             now = parent[now];
         }
         // descent into the key-ordered child picked by the Bourke corner id
@@ -247,7 +248,7 @@ __global__ void __launch_bounds__(kThreads) vertex_values(NodeView v, const int*
             int cb = childBase[now];
             if (cb < 0) break;
             now = cb + ex;
-            val += chi_node_sum(now, p, v.M, v.D, v.neigh, enc, dx);
+            val += chi_node_sum(now, p, v.M, v.neigh, basis, dx);
         }
     }
     vv[t] = val - *iso;
@@ -419,7 +420,7 @@ void build_geometry(Ctx& c) {
     float* isoDev = c.buf<float>("iso_value", 1);
     g.vvalue = c.buf<float>("vvalue", size_t(8) * M);
     vertex_values<<<div_up(size_t(8) * M, kThreads), kThreads, 0, s>>>(v, g.counts + 0, g.vtxOwner, g.vtxKind,
-                                                                       g.vtxPos, o.parent, o.childBase, o.encIdx,
+                                                                       g.vtxPos, o.parent, o.childBase, o.basis,
                                                                        c.fld.dx, isoDev, g.vvalue);
     PSR_HIP_CHECK(hipMemcpyAsync(c.hostPinned, g.counts, sizeof(int) * 3, hipMemcpyDeviceToHost, s));
     PSR_HIP_CHECK(hipStreamSynchronize(s));

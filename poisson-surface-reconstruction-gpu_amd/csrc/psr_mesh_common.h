@@ -93,19 +93,6 @@ __device__ __forceinline__ float3 interpolate_point(float3 p1, float3 p2, int di
     return make_float3(a[0], a[1], a[2]);
 }
 
-// sum over the 27 neighbours of `node` of dx * F(x) F(y) F(z)
-__device__ __forceinline__ float chi_node_sum(int node, float3 p, int M, int D, const int* __restrict__ neigh,
-                                              const int* __restrict__ enc, const float* __restrict__ dx) {
-    float v = 0.f;
-    for (int i = 0; i < 27; ++i) {
-        int nb = neigh[i * M + node];
-        if (nb == -1) continue;
-        int3 f = decode_func_index(enc[nb], D);
-        v += dx[nb] * basis_value(f.x, p.x) * basis_value(f.y, p.y) * basis_value(f.z, p.z);
-    }
-    return v;
-}
-
 // Per-call state of the depth-D marching cubes.
 struct MainMc {
     int* vexAddr = nullptr;   // crossing index per depth-D edge

@@ -269,7 +269,7 @@ __global__ void __launch_bounds__(kThreads) fill_occupied(const unsigned* __rest
                                                           int* __restrict__ key, int* __restrict__ pidx,
                                                           int* __restrict__ pnum, int* __restrict__ parent,
                                                           int* __restrict__ childBase, int* __restrict__ didx,
-                                                          int* __restrict__ dnum) {
+                                                          int* __restrict__ dnum, int* __restrict__ q1Fired) {
     int t = blockIdx.x * kThreads + threadIdx.x;
     if (t >= total) return;
     const int D = L.D;
@@ -290,12 +290,15 @@ __global__ void __launch_bounds__(kThreads) fill_occupied(const unsigned* __rest
     int par;
     if (d == 1) {
         par = 0;
-    } else if (local == 0) {
-        par = L.base[d - 1];  // Q1: level-local child 0 reads as "no child" (BuildOctree.cu:384-389, 412)
     } else {
         int sonUp = int((code >> (sh + 3)) & 7u);
         int ppr = (d >= 3) ? parentRank[lo.lo[d - 1] + pr] : 0;
         par = L.base[d - 1] + 8 * ppr + sonUp;
+        if (local == 0) {
+            // Q1: level-local child 0 reads as "no child" (BuildOctree.cu:384-389, 412)
+            if (par != L.base[d - 1]) atomicOr(q1Fired, 1);
+            par = L.base[d - 1];
+        }
     }
     parent[slot] = par;
     childBase[slot] = (d < D) ? L.base[d + 1] + 8 * q : -1;
@@ -394,14 +397,17 @@ __global__ void root_neighbors(int* __restrict__ neigh, int M) {
 }
 
 // Packed 1-D function index (BuildOctree.cu:514-550).
+// Also writes each node's depth and basis parameters (centres, inverse width).
 __global__ void __launch_bounds__(kThreads) encode_function_index(Levels L, int M, const int* __restrict__ key,
-                                                                  int* __restrict__ enc, int* __restrict__ depth) {
+                                                                  int* __restrict__ enc, int* __restrict__ depth,
+                                                                  float4* __restrict__ basis) {
     int o = blockIdx.x * kThreads + threadIdx.x;
     if (o >= M) return;
     int d = depth_of(L, o);
     depth[o] = d;
     int3 f = key_func_index(key[o], d, L.D);
     enc[o] = f.x | (f.y << (L.D + 1)) | (f.z << (2 * (L.D + 1)));
+    basis[o] = node_basis_params(f);
 }
 
 }  // namespace
@@ -481,9 +487,12 @@ void build_octree(Ctx& c, const psr_points& pts) {
     o.neigh = c.buf<int>("node_neigh", size_t(27) * M);
     o.encIdx = c.buf<int>("node_enc", M);
     zero_int<<<div_up(M, kThreads), kThreads, 0, s>>>(o.pnum, M);
+    o.q1Fired = c.buf<int>("q1_fired", 1);
+    zero_int<<<1, kThreads, 0, s>>>(o.q1Fired, 1);
     fill_occupied<<<div_up(totalNodes, kThreads), kThreads, 0, s>>>(keys, N, L, lo, Ud, firstPoint, parentRank,
                                                                      rankDm1, totalNodes, o.key, o.pidx, o.pnum,
-                                                                     o.parent, o.childBase, o.didx, o.dnum);
+                                                                     o.parent, o.childBase, o.didx, o.dnum,
+                                                                     o.q1Fired);
     int totalBlocks = (M - 1) / 8;
     fill_blocks<<<div_up(totalBlocks + 1, kThreads), kThreads, 0, s>>>(L, totalBlocks, N, o.U[D - 1], o.key, o.pidx,
                                                                         o.pnum, o.parent, o.childBase, o.didx,
@@ -495,7 +504,8 @@ void build_octree(Ctx& c, const psr_points& pts) {
         neighbors_level<<<div_up(L.count[d], kThreads), kThreads, 0, s>>>(L.base[d], L.count[d], D, d, M, o.key,
                                                                            o.parent, o.childBase, o.neigh, lut);
     o.depth = c.buf<int>("node_depth", M);
-    encode_function_index<<<div_up(M, kThreads), kThreads, 0, s>>>(L, M, o.key, o.encIdx, o.depth);
+    o.basis = c.buf<float4>("node_basis", M);
+    encode_function_index<<<div_up(M, kThreads), kThreads, 0, s>>>(L, M, o.key, o.encIdx, o.depth, o.basis);
     PSR_HIP_CHECK(hipGetLastError());
 }
 

@@ -490,24 +490,12 @@ __global__ void __launch_bounds__(kCgThreads) cg_mixed_all_depths(
 }
 
 // --- implicit function at points (LaplacianSolver.cu:85-118, 269-305) ------
-__device__ __forceinline__ float chi_sum_node(int node, float3 p, int M, int D, const int* __restrict__ neigh,
-                                              const int* __restrict__ enc, const float* __restrict__ dx) {
-    float v = 0.f;
-    for (int i = 0; i < 27; ++i) {
-        int nb = neigh[i * M + node];
-        if (nb == -1) continue;
-        int3 f = decode_func_index(enc[nb], D);
-        v += dx[nb] * basis_value(f.x, p.x) * basis_value(f.y, p.y) * basis_value(f.z, p.z);
-    }
-    return v;
-}
-
 __global__ void __launch_bounds__(kThreads) points_chi_kernel(int N, int offD, int M, int D,
                                                               const float4* __restrict__ pos,
                                                               const int* __restrict__ p2n,
                                                               const int* __restrict__ parent,
                                                               const int* __restrict__ neigh,
-                                                              const int* __restrict__ enc,
+                                                              const float4* __restrict__ basis,
                                                               const float* __restrict__ dx,
                                                               float* __restrict__ chi, double* __restrict__ part) {
     __shared__ double sh[kThreads / PSR_WAVE];
@@ -519,7 +507,7 @@ __global__ void __launch_bounds__(kThreads) points_chi_kernel(int N, int offD, i
         int now = offD + p2n[i];
         float v = 0.f;
         while (now != -1) {
-            v += chi_sum_node(now, p, M, D, neigh, enc, dx);
+            v += chi_node_sum(now, p, M, neigh, basis, dx);
             now = parent[now];
         }
         chi[i] = v;
@@ -672,7 +660,7 @@ void points_implicit(Ctx& c) {
     c.fld.isoSum = c.buf<double>("iso_sum", 1);
     float* isoDev = c.buf<float>("iso_value", 1);
     points_chi_kernel<<<nb, kThreads, 0, s>>>(N, L.base[L.D], o.M, L.D, o.spos, o.point2node, o.parent, o.neigh,
-                                              o.encIdx, c.fld.dx, c.fld.chiPts, part);
+                                              o.basis, c.fld.dx, c.fld.chiPts, part);
     iso_final<<<1, 1024, 0, s>>>(part, nb, N, c.fld.isoSum, isoDev);
     PSR_HIP_CHECK(hipGetLastError());
 }

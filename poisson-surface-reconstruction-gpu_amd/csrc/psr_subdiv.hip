@@ -4,12 +4,25 @@
 // (CoarserSubdivideNodeAndRebuildMesh, ComputeTriangleIndices.cu:1475-1799)
 // and the finer roots once per root depth (FinerSubdivideNodeAndRebuildMesh,
 // :1801-2194), each as a full virtual subtree down to depth D whose ids start
-// at NodeArraySize.  Here every such rebuild is a "set" with a disjoint virtual
-// id range, and all sets run in the same launches: one launch per absolute
-// depth for the neighbour tables, one per stage for the depth-D work.  A child
-// look-up through a real parent is redirected only to the roots of the same
-// set, which is what the reference's temporary child-pointer rewrite gives
-// (:1525 and :1784 for the coarse path; :1137 for the finer path).
+// at NodeArraySize, with neighbour tables (computeRebuildNeighbor :669-696),
+// vertex and edge ownership (:698-954), vertex values (:956-1038) and marching
+// cubes on the virtual depth-D cells.  Here every such rebuild is a "set", and
+// the virtual subtrees are never materialised: they are full octrees, so a
+// virtual cell is an integer coordinate, its key its Morton code, and all
+// structure follows by arithmetic:
+//   * a depth-D cell n is virtual in set S iff its depth-rd ancestor cell is
+//     the cell of a root of S (the same root, or a neighbour of it in the
+//     real table that is a root of S: child look-ups through a real parent are
+//     redirected only to roots of the same set, :1137 and :1525/:1784);
+//   * the owner of a corner (edge) is the virtual cell of smallest key among
+//     the 8 (4) cells sharing it; real cells never own (:735);
+//   * the value at a corner walks the owner's chain: at virtual depths only
+//     real neighbours contribute (virtual d_x = 0, :978, :1021); they lie in
+//     the root's real neighbours and are found by child descent; at the root
+//     depth the root copy's neighbours are the root's real neighbours with
+//     root copies mapped to neighs[13] of their root; above it the real chain.
+// The order of every floating-point sum and of every emitted vertex and
+// triangle is that of the materialised formulation.
 #include "psr_mesh_common.h"
 #include "psr_prims.h"
 
@@ -22,390 +35,667 @@ namespace {
 constexpr int kThreads = 256;
 
 struct SetDesc {
-    int rd;           // root depth
-    int nr;           // roots in the set
-    int rootOffset;   // first root in the root list
-    int coarse;       // 1: coarse path semantics
-    int vbase;        // first virtual index of the set
-    int dEnum;        // first depth-D enumeration index of the set
-    int levelStart[PSR_MAX_DEPTH + 2];  // relative to vbase
+    int rd;          // root depth
+    int nr;          // roots in the set
+    int rootOffset;  // first root in the root list
+    int dEnum;       // first depth-D cell enumeration index of the set (chunk-local)
 };
 
-struct Span {
-    int set, start, count, prefix;
-};
-
-struct SubView {
-    int NAS, D, M, nSets;
+struct Grid {
+    int D, M, nSets;
     const SetDesc* sets;
-    const int* roots;      // real node ids
+    const int* roots;      // real node ids, ascending
     const int* rootPos;    // real node -> position in roots, or -1 (nodes above depth D)
-    int rootPosSize;
-    const int* key;        // real arrays
-    const int* depth;
+    const int* key;
     const int* parent;
     const int* childBase;
     const int* neigh;
-    const int* enc;
+    const float4* basis;
     const float* dx;
-    int* vkey;             // virtual arrays
-    int* vparent;
-    int* vneigh;           // [27][VN]
-    int VN;
+    const float4* nbrBasis;  // [node][27] basis parameters of the node's neighbours (nodes above depth D-1)
+    const float* nbrDx;      // [node][27] their d_x (0 where absent)
 };
 
-__device__ __forceinline__ int find_set(const SubView& s, int v) {
-    int lo = 0, hi = s.nSets - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (s.sets[mid].vbase <= v) lo = mid; else hi = mid - 1;
+// Bourke corner order: slot -> (x, y, z) offset (BuildMeshGeometry.cu:133-144).
+__device__ __forceinline__ int3 bourke_offset(int b) {
+    int x = (b == 1 || b == 2 || b == 5 || b == 6) ? 1 : 0;
+    int y = (b == 2 || b == 3 || b == 6 || b == 7) ? 1 : 0;
+    return make_int3(x, y, b >> 2);
+}
+__device__ __forceinline__ int bourke_slot(int ox, int oy, int oz) {
+    int i = ox | (oz << 2);
+    if (oy) i += ox ? 1 : 3;
+    return i;
+}
+
+// Morton code of a cell at a depth with `bits` levels (x high in each triplet):
+// the node key format of the reference.
+__device__ __forceinline__ int morton(int3 c, int bits) {
+    int k = 0;
+    for (int i = 0; i < bits; ++i)
+        k |= (((c.x >> i) & 1) << (3 * i + 2)) | (((c.y >> i) & 1) << (3 * i + 1)) | (((c.z >> i) & 1) << (3 * i));
+    return k;
+}
+__device__ __forceinline__ int3 demorton(int k, int bits) {
+    int3 c = make_int3(0, 0, 0);
+    for (int i = 0; i < bits; ++i) {
+        c.x |= ((k >> (3 * i + 2)) & 1) << i;
+        c.y |= ((k >> (3 * i + 1)) & 1) << i;
+        c.z |= ((k >> (3 * i)) & 1) << i;
     }
-    return lo;
+    return c;
 }
 
-__device__ __forceinline__ int find_span(const Span* sp, int n, int t) {
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (sp[mid].prefix <= t) lo = mid; else hi = mid - 1;
+// A virtual depth-D cell and its root.
+struct Leaf {
+    SetDesc S;
+    int si, q, root, sh;  // sh = D - rd
+    int3 Rc;              // root cell at depth rd
+    int3 c;               // cell at depth D
+};
+
+// Work is issued in tiles of at most 256 consecutive cells of one set; a set
+// whose roots hold at least 512 cells is tiled along its roots, so such a tile
+// lies in one root.
+struct Tile {
+    int set, start, count;
+};
+constexpr int kTile = 256;
+
+__device__ __forceinline__ Leaf make_leaf_in(const Grid& g, int si, const SetDesc& S, int e) {
+    Leaf L;
+    L.si = si;
+    L.S = S;
+    L.sh = g.D - L.S.rd;
+    int rel = e - L.S.dEnum;
+    L.q = rel >> (3 * L.sh);
+    int j = rel - (L.q << (3 * L.sh));
+    L.root = g.roots[L.S.rootOffset + L.q];
+    L.Rc = demorton(g.key[L.root] >> (3 * L.sh), L.S.rd);
+    int3 lc = demorton(j, L.sh);
+    L.c = make_int3((L.Rc.x << L.sh) + lc.x, (L.Rc.y << L.sh) + lc.y, (L.Rc.z << L.sh) + lc.z);
+    return L;
+}
+
+__device__ __forceinline__ int nbr_index(int dx, int dy, int dz) { return 9 * (dx + 1) + 3 * (dy + 1) + (dz + 1); }
+
+__device__ __forceinline__ bool same_set_root(const Grid& g, const SetDesc& S, int node, int& q) {
+    int rp = g.rootPos[node];
+    q = rp - S.rootOffset;
+    return rp >= S.rootOffset && rp < S.rootOffset + S.nr;
+}
+
+// Enumeration index of depth-D cell n if it is virtual in L's set, else -1.
+__device__ __forceinline__ int virtual_enum(const Grid& g, const Leaf& L, int3 n) {
+    const int lim = 1 << g.D;
+    if (n.x < 0 || n.y < 0 || n.z < 0 || n.x >= lim || n.y >= lim || n.z >= lim) return -1;
+    int3 A = make_int3(n.x >> L.sh, n.y >> L.sh, n.z >> L.sh);
+    int q = L.q;
+    if (A.x != L.Rc.x || A.y != L.Rc.y || A.z != L.Rc.z) {
+        int N = g.neigh[nbr_index(A.x - L.Rc.x, A.y - L.Rc.y, A.z - L.Rc.z) * g.M + L.root];
+        if (N < 0 || !same_set_root(g, L.S, N, q)) return -1;
     }
-    return lo;
+    int3 lc = make_int3(n.x - (A.x << L.sh), n.y - (A.y << L.sh), n.z - (A.z << L.sh));
+    return L.S.dEnum + (q << (3 * L.sh)) + morton(lc, L.sh);
 }
 
-// Per-root count of a set's level l and the node's (root, offset) coordinates.
-__device__ __forceinline__ void level_coords(const SetDesc& S, int l, int rel, int& q, int& j) {
-    int per = 1 << (3 * (l - S.rd));
-    int local = rel - S.levelStart[l];
-    q = local / per;
-    j = local - q * per;
+__device__ __forceinline__ bool inside_root(const Leaf& L, int lo, int hi, int axisRoot) {
+    return (lo >> L.sh) == axisRoot && (hi >> L.sh) == axisRoot;
 }
 
-__device__ __forceinline__ int level_of(const SetDesc& S, int rel, int D) {
-    int l = S.rd;
-    while (l < D && rel >= S.levelStart[l + 1]) ++l;
-    return l;
+// Owner (enumeration, Bourke slot within the owner) of the grid point c + off(b);
+// ve(n) is the enumeration of depth-D cell n if it is virtual in L's set, else -1.
+template <class VE>
+__device__ __forceinline__ int corner_owner(const Leaf& L, int D, int b, int& slot, VE ve) {
+    int3 o = bourke_offset(b);
+    int3 p = make_int3(L.c.x + o.x, L.c.y + o.y, L.c.z + o.z);
+    if (inside_root(L, p.x - 1, p.x, L.Rc.x) && inside_root(L, p.y - 1, p.y, L.Rc.y) &&
+        inside_root(L, p.z - 1, p.z, L.Rc.z)) {
+        // all 8 sharing cells are in the root's cube; the smallest key is the lowest cell
+        slot = bourke_slot(1, 1, 1);
+        int3 lc = make_int3(p.x - 1 - (L.Rc.x << L.sh), p.y - 1 - (L.Rc.y << L.sh), p.z - 1 - (L.Rc.z << L.sh));
+        return L.S.dEnum + (L.q << (3 * L.sh)) + morton(lc, L.sh);
+    }
+    int best = -1, bestKey = 0x7fffffff;
+    slot = 0;
+    for (int k = 0; k < 8; ++k) {
+        int3 d = make_int3(k & 1, (k >> 1) & 1, (k >> 2) & 1);
+        int3 n = make_int3(p.x - d.x, p.y - d.y, p.z - d.z);
+        int v = ve(n);
+        if (v < 0) continue;
+        int key = morton(n, D);
+        if (key < bestKey) {
+            bestKey = key;
+            best = v;
+            slot = bourke_slot(d.x, d.y, d.z);
+        }
+    }
+    return best;
 }
 
-// key / parent of every virtual node (singleRebuildArray :599-635, wholeRebuildArray :1110-1168)
-__global__ void __launch_bounds__(kThreads) vnode_init(SubView s) {
-    int v = blockIdx.x * kThreads + threadIdx.x;
-    if (v >= s.VN) return;
-    int si = find_set(s, v);
-    const SetDesc& S = s.sets[si];
-    int rel = v - S.vbase;
-    int l = level_of(S, rel, s.D);
-    int q, j;
-    level_coords(S, l, rel, q, j);
-    int root = s.roots[S.rootOffset + q];
-    s.vkey[v] = s.key[root] | (j << (3 * (s.D - l)));
-    if (l == S.rd) {
-        s.vparent[v] = s.parent[root];
+// Owner (enumeration, edge kind within the owner) of edge slot k of L.
+// Edge kind = 4 * axis + offsets of the other two axes (bit set: positive side).
+template <class VE>
+__device__ __forceinline__ int edge_owner(const Leaf& L, int D, int k, int& kind, VE ve) {
+    const int a = k >> 2;
+    const int u0 = a == 0 ? 1 : 0, u1 = a == 2 ? 1 : 2;
+    const int o0 = k & 1, o1 = (k >> 1) & 1;
+    int cc[3] = {L.c.x, L.c.y, L.c.z}, rc[3] = {L.Rc.x, L.Rc.y, L.Rc.z};
+    int lo0 = cc[u0] + o0 - 1, lo1 = cc[u1] + o1 - 1;
+    if (inside_root(L, lo0, lo0 + 1, rc[u0]) && inside_root(L, lo1, lo1 + 1, rc[u1])) {
+        kind = (a << 2) | 3;
+        int n[3];
+        n[a] = cc[a];
+        n[u0] = lo0;
+        n[u1] = lo1;
+        int3 lc = make_int3(n[0] - (L.Rc.x << L.sh), n[1] - (L.Rc.y << L.sh), n[2] - (L.Rc.z << L.sh));
+        return L.S.dEnum + (L.q << (3 * L.sh)) + morton(lc, L.sh);
+    }
+    int best = -1, bestKey = 0x7fffffff;
+    kind = 0;
+    for (int t = 0; t < 4; ++t) {
+        int b0 = t & 1, b1 = (t >> 1) & 1;  // 1: the cell below the edge in that axis
+        int n[3];
+        n[a] = cc[a];
+        n[u0] = b0 ? lo0 : lo0 + 1;
+        n[u1] = b1 ? lo1 : lo1 + 1;
+        int3 nn = make_int3(n[0], n[1], n[2]);
+        int v = ve(nn);
+        if (v < 0) continue;
+        int key = morton(nn, D);
+        if (key < bestKey) {
+            bestKey = key;
+            best = v;
+            kind = (a << 2) | b0 | (b1 << 1);
+        }
+    }
+    return best;
+}
+
+__device__ __forceinline__ float grid_term(const Grid& g, int nb, float3 p) { return node_term(g.basis[nb], g.dx[nb], p); }
+
+// Contributor k of a root's fixed list, given the chain node k / 27 (the
+// root, then its ancestors): k / 27 = 0 the root copy's neighbours (root
+// copies standing for neighs[13] of their root), above it the ancestors'
+// neighbours; -1 where there is none (computeSubdivideVertexImplicitFunctionValue
+// :956-1038, the part of the chain from the root upwards, which all cells of a
+// root share).
+__device__ __forceinline__ int root_list_node(const Grid& g, const SetDesc& S, int chainNode, int k) {
+    if (chainNode < 0) return -1;
+    int nb = g.neigh[(k % 27) * g.M + chainNode];
+    int qq;
+    if (k < 27 && nb >= 0 && same_set_root(g, S, nb, qq)) nb = g.neigh[13 * g.M + nb];
+    return nb < 0 ? -1 : nb;
+}
+
+// Real node for the depth-l cell n next to root (cell Rc at depth rd): -1 when
+// the cell is virtual (inside a root of the set) or absent; otherwise the real
+// child reached from the root's real neighbour (virtual d_x = 0, :978, :1021).
+__device__ __forceinline__ int halo_node(const Grid& g, const SetDesc& S, int root, int3 Rc, int l, int3 n) {
+    const int lim = 1 << l, s = l - S.rd;
+    if (n.x < 0 || n.y < 0 || n.z < 0 || n.x >= lim || n.y >= lim || n.z >= lim) return -1;
+    int3 A = make_int3(n.x >> s, n.y >> s, n.z >> s);
+    if (A.x == Rc.x && A.y == Rc.y && A.z == Rc.z) return -1;
+    int node = g.neigh[nbr_index(A.x - Rc.x, A.y - Rc.y, A.z - Rc.z) * g.M + root];
+    int qq;
+    if (node < 0 || same_set_root(g, S, node, qq)) return -1;
+    for (int k = 1; k <= s && node >= 0; ++k) {  // real children down to depth l
+        int cb = g.childBase[node];
+        int sh = s - k;
+        node = cb < 0 ? -1 : cb + ((((n.x >> sh) & 1) << 2) | (((n.y >> sh) & 1) << 1) | ((n.z >> sh) & 1));
+    }
+    return node;
+}
+
+// Tiles.  A tile holds whole roots (small roots: 8 or 64 cells) or 256 cells
+// of one root.  What its cells share is resolved once into LDS by all threads:
+//   * vbox: per root, the virtual enumeration (or -1) of every depth-D cell of
+//     the box around the tile's cells of that root (all owner candidates);
+//   * per root, the fixed contributor list (27 (rd + 1) nodes);
+//   * halo: per root and virtual depth l, the real node (or -1) of every cell
+//     of the box around the tile's cells at depth l (the chain's neighbours);
+//   * for a tile in one root, d_x of the list and the 1-D B-spline factors of
+//     every list node at the tile's corner coordinates (256 Morton-consecutive
+//     cells span 4 x 8 x 8, their corners 5 x 9 x 9).
+// The terms and their order are those of the chain walk.
+constexpr int kTx = 5, kTy = 9, kTz = 9, kTw = kTx + kTy + kTz;
+constexpr int kMaxRootsPerTile = 32;
+constexpr int kLdsWordsOne = 7168;     // one-root tiles: list, halo, vbox, d_x, factors (<= 6954 words)
+constexpr int kLdsWordsSmall = 5376;   // small-root tiles of 64 cells: halo, vbox, staged blocks (<= 5344 words)
+constexpr int kStagedBlocks = 32;  // contributor blocks of 27 staged in LDS per small-root tile
+constexpr int kSmallTile = 64;     // cells (and threads) of a small-root tile
+constexpr int kVboxWords = 2048;
+
+__device__ __forceinline__ int3 box_ext(int3 extD, int D, int l) {
+    int sh = D - l;
+    return make_int3(max(1, extD.x >> sh), max(1, extD.y >> sh), max(1, extD.z >> sh));
+}
+
+struct TileGeom {
+    int nRoots, q0, K;
+    bool oneRoot;
+    int3 ext;         // cells per root covered by the tile at depth D (per axis)
+    int haloPerRoot;  // sum over virtual depths of the box sizes
+    int vboxPerRoot;  // depth-D box size
+};
+
+__device__ __forceinline__ TileGeom tile_geom(const Tile& T, const SetDesc& S, int D) {
+    TileGeom tg;
+    const int sh = D - S.rd;
+    tg.K = 27 * (S.rd + 1);
+    tg.q0 = (T.start - S.dEnum) >> (3 * sh);
+    tg.oneRoot = sh >= 3;
+    tg.nRoots = tg.oneRoot ? 1 : T.count >> (3 * sh);
+    tg.ext = tg.oneRoot ? make_int3(4, 8, 8) : make_int3(1 << sh, 1 << sh, 1 << sh);
+    tg.haloPerRoot = 0;
+    for (int l = D; l > S.rd; --l) {
+        int3 x = box_ext(tg.ext, D, l);
+        tg.haloPerRoot += (x.x + 2) * (x.y + 2) * (x.z + 2);
+    }
+    tg.vboxPerRoot = (tg.ext.x + 2) * (tg.ext.y + 2) * (tg.ext.z + 2);
+    return tg;
+}
+
+// Lowest depth-D cell of root rr's cells in the tile.
+__device__ __forceinline__ Leaf tile_root_first(const Grid& g, const Tile& T, const SetDesc& S, const TileGeom& tg,
+                                                int rr) {
+    return make_leaf_in(g, T.set, S, tg.oneRoot ? T.start : S.dEnum + ((tg.q0 + rr) << (3 * (g.D - S.rd))));
+}
+
+// Per-root facts of a tile, kept in LDS: root node, root cell, lowest tile cell.
+struct TileRoot {
+    int root, q;
+    int3 Rc, first;
+};
+
+__device__ __forceinline__ void fill_roots(const Grid& g, const Tile& T, const SetDesc& S, const TileGeom& tg,
+                                           TileRoot* tr) {
+    for (int rr = threadIdx.x; rr < tg.nRoots; rr += blockDim.x) {
+        Leaf F = tile_root_first(g, T, S, tg, rr);
+        tr[rr] = TileRoot{F.root, F.q, F.Rc, F.c};
+    }
+}
+
+__device__ __forceinline__ Leaf root_leaf(const Grid& g, const SetDesc& S, int si, const TileRoot& R) {
+    Leaf F;
+    F.S = S;
+    F.si = si;
+    F.q = R.q;
+    F.root = R.root;
+    F.sh = g.D - S.rd;
+    F.Rc = R.Rc;
+    F.c = R.first;
+    return F;
+}
+
+// Virtual enumerations of the depth-D box cells of every root of the tile
+// (needs the roots in LDS).
+__device__ __forceinline__ void fill_vbox(const Grid& g, const Tile& T, const SetDesc& S, const TileGeom& tg,
+                                          const TileRoot* tr, int* vbox) {
+    const int by = tg.ext.y + 2, bz = tg.ext.z + 2;
+#pragma unroll 4
+    for (int t = threadIdx.x; t < tg.nRoots * tg.vboxPerRoot; t += blockDim.x) {
+        int rr = t / tg.vboxPerRoot, u = t - rr * tg.vboxPerRoot;
+        const Leaf F = root_leaf(g, S, T.set, tr[rr]);
+        int dz = u % bz, dy = (u / bz) % by, dx = u / (bz * by);
+        vbox[t] = virtual_enum(g, F, make_int3(F.c.x - 1 + dx, F.c.y - 1 + dy, F.c.z - 1 + dz));
+    }
+}
+
+// Virtual-cell lookup in the tile's depth-D box of this cell's root.
+struct BoxVE {
+    const int* vbox;
+    int3 lo;  // box origin (one below the lowest tile cell of the root)
+    int by, bz;
+    __device__ __forceinline__ int operator()(int3 n) const {
+        return vbox[((n.x - lo.x) * by + (n.y - lo.y)) * bz + (n.z - lo.z)];
+    }
+};
+
+__device__ __forceinline__ BoxVE box_ve(const int* vboxAll, const TileGeom& tg, int r, int3 rlo) {
+    BoxVE v;
+    v.vbox = vboxAll + r * tg.vboxPerRoot;
+    v.lo = make_int3(rlo.x - 1, rlo.y - 1, rlo.z - 1);
+    v.by = tg.ext.y + 2;
+    v.bz = tg.ext.z + 2;
+    return v;
+}
+
+// val += node terms of ids[0..n) in order, with the node loads of a batch in flight together.
+template <int B>
+__device__ __forceinline__ float accumulate_nodes(const Grid& g, const int* ids, int n, float3 p, float val) {
+    for (int k0 = 0; k0 < n; k0 += B) {
+        float4 bb[B];
+        float dd[B];
+        bool ok[B];
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+            int nb = (k0 + j < n) ? ids[k0 + j] : -1;
+            ok[j] = nb >= 0;
+            bb[j] = ok[j] ? g.basis[nb] : make_float4(0.f, 0.f, 0.f, 0.f);
+            dd[j] = ok[j] ? g.dx[nb] : 0.f;
+        }
+#pragma unroll
+        for (int j = 0; j < B; ++j)
+            if (ok[j]) val += node_term(bb[j], dd[j], p);
+    }
+    return val;
+}
+
+// Values of the corners each cell owns: cval[slot][e].  ONE: tiles lying in
+// one root (roots of at least 512 cells); otherwise tiles of whole small roots.
+template <bool ONE, int NT>
+__global__ void __launch_bounds__(NT) sub_corner_values(Grid g, const Tile* __restrict__ tiles, int nVD,
+                                                        const float* __restrict__ iso, float* __restrict__ cval) {
+    constexpr int kWords = ONE ? kLdsWordsOne : kLdsWordsSmall;
+    __shared__ float4 ldsStore[kWords / 4];  // 16-byte aligned; every region below starts at a multiple of 4 words
+    int* lds = reinterpret_cast<int*>(ldsStore);
+    __shared__ int lChain[(ONE ? 1 : kMaxRootsPerTile) * (PSR_MAX_DEPTH + 1)];  // [nRoots][rd + 1]: root, ancestors
+    __shared__ TileRoot lRoots[ONE ? 1 : kMaxRootsPerTile];
+    const Tile T = tiles[blockIdx.x];
+    const SetDesc S = g.sets[T.set];
+    const int D = g.D, sh = D - S.rd;
+    TileGeom tg = tile_geom(T, S, D);
+    tg.oneRoot = ONE;
+    const int K = tg.K;
+    // one root:   list [K] | halo | vbox | dx [K] | B-spline factors [K][kTw]
+    // small roots: halo | vbox | staged block bases [kStagedBlocks * 27] (float4) | staged d_x
+    int* lList = lds;
+    int* lHalo = lList + (tg.oneRoot ? K : 0);
+    int* lVbox = lHalo + tg.nRoots * tg.haloPerRoot;
+    float* lDx = reinterpret_cast<float*>(lVbox + tg.nRoots * tg.vboxPerRoot);
+    float* lTab = lDx + K;
+    float4* sBasis = reinterpret_cast<float4*>(lDx);  // small roots: halo and vbox sizes are multiples of 4 words
+    float* sDx = reinterpret_cast<float*>(sBasis + kStagedBlocks * 27);
+    __shared__ int lBlk[ONE ? 1 : kMaxRootsPerTile * (PSR_MAX_DEPTH + 1)];  // [nRoots][rd + 1] -> staged block
+    __shared__ int lBlkNode[ONE ? 1 : kMaxRootsPerTile * (PSR_MAX_DEPTH + 1)];
+    __shared__ int lNumBlk;
+    const bool active = int(threadIdx.x) < T.count;
+    const int e = T.start + (active ? int(threadIdx.x) : 0);
+    Leaf L = make_leaf_in(g, T.set, S, e);
+    const int r = L.q - tg.q0;  // this cell's root within the tile
+    const float inv = 1.0f / float(1 << D);
+
+    // roots and their chains
+    fill_roots(g, T, S, tg, lRoots);
+    for (int rr = threadIdx.x; rr < tg.nRoots; rr += NT) {
+        int node = g.roots[S.rootOffset + tg.q0 + rr];
+        for (int u = 0; u <= S.rd; ++u) {
+            lChain[rr * (S.rd + 1) + u] = node;
+            node = node < 0 ? -1 : g.parent[node];
+        }
+    }
+    __syncthreads();
+    fill_vbox(g, T, S, tg, lRoots, lVbox);
+    // halo boxes, depth D first
+#pragma unroll 4
+    for (int t = threadIdx.x; t < tg.nRoots * tg.haloPerRoot; t += NT) {
+        int rr = t / tg.haloPerRoot, u = t - rr * tg.haloPerRoot;
+        const Leaf F = root_leaf(g, S, T.set, lRoots[rr]);
+        int l = D;
+        int3 x = box_ext(tg.ext, D, l);
+        while (u >= (x.x + 2) * (x.y + 2) * (x.z + 2)) {
+            u -= (x.x + 2) * (x.y + 2) * (x.z + 2);
+            --l;
+            x = box_ext(tg.ext, D, l);
+        }
+        int dz = u % (x.z + 2), dy = (u / (x.z + 2)) % (x.y + 2), dx = u / ((x.z + 2) * (x.y + 2));
+        int3 n = make_int3((F.c.x >> (D - l)) - 1 + dx, (F.c.y >> (D - l)) - 1 + dy, (F.c.z >> (D - l)) - 1 + dz);
+        lHalo[t] = halo_node(g, S, F.root, F.Rc, l, n);
+    }
+    const int C = S.rd + 1;  // chain length: the root, then its ancestors
+    if (tg.oneRoot) {
+        for (int k = threadIdx.x; k < K; k += NT) {
+            int nb = root_list_node(g, S, lChain[k / 27], k);
+            lList[k] = nb;
+            lDx[k] = nb < 0 ? 0.f : g.dx[nb];
+        }
     } else {
-        int perUp = 1 << (3 * (l - 1 - S.rd));
-        s.vparent[v] = s.NAS + S.vbase + S.levelStart[l - 1] + q * perUp + (j >> 3);
-    }
-}
-
-// neighbours of the virtual nodes of absolute depth l (computeRebuildNeighbor :669-696)
-__global__ void __launch_bounds__(kThreads) vnode_neighbors(SubView s, const Span* __restrict__ spans, int nSpans,
-                                                            int total, int l, NeighborLut lut) {
-    int t = blockIdx.x * kThreads + threadIdx.x;
-    if (t >= total) return;
-    const Span& sp = spans[find_span(spans, nSpans, t)];
-    const SetDesc& S = s.sets[sp.set];
-    int v = sp.start + (t - sp.prefix);
-    int son = (s.vkey[v] >> (3 * (s.D - l))) & 7;
-    int p = s.vparent[v];
-    for (int i = 0; i < 27; ++i) {
-        int pi = lut.parent[son][i], ci = lut.child[son][i];
-        int np = (p < s.NAS) ? s.neigh[pi * s.M + p] : s.vneigh[size_t(pi) * s.VN + (p - s.NAS)];
-        int r = -1;
-        if (np != -1) {
-            if (np < s.NAS) {
-                int cb = s.childBase[np];
-                if (cb >= 0) {
-                    r = cb + ci;
-                    int rp = (r < s.rootPosSize) ? s.rootPos[r] : -1;
-                    if (rp >= S.rootOffset && rp < S.rootOffset + S.nr)
-                        r = s.NAS + S.vbase + S.levelStart[S.rd] + (rp - S.rootOffset);
+        // contributor blocks: the root level of each root, and each distinct
+        // ancestor (roots are in key order, so shared ancestors are adjacent)
+        if (threadIdx.x == 0) {
+            int nblk = 0;
+            for (int rr = 0; rr < tg.nRoots; ++rr)
+                for (int u = 0; u < C; ++u) {
+                    int id;
+                    if (u > 0 && rr > 0 && lChain[rr * C + u] == lChain[(rr - 1) * C + u]) {
+                        id = lBlk[(rr - 1) * C + u];
+                    } else {
+                        id = nblk++;
+                        lBlkNode[id] = u == 0 ? rr : -1 - (rr * C + u);  // root level: root; else chain slot
+                    }
+                    lBlk[rr * C + u] = id;
                 }
-            } else {
-                int q, j;
-                level_coords(S, l - 1, np - s.NAS - S.vbase, q, j);
-                r = s.NAS + S.vbase + S.levelStart[l] + q * (1 << (3 * (l - S.rd))) + 8 * j + ci;
-            }
+            lNumBlk = nblk;
         }
-        s.vneigh[size_t(i) * s.VN + v] = r;
-    }
-}
-
-__device__ __forceinline__ float3 any_center(const SubView& s, int id, int depth) {
-    return id < s.NAS ? key_center(s.key[id], s.depth[id], s.D) : key_center(s.vkey[id - s.NAS], depth, s.D);
-}
-
-// depth-D enumeration e -> virtual index
-__device__ __forceinline__ int venum(const Span* sp, int n, int e, int& set) {
-    const Span& x = sp[find_span(sp, n, e)];
-    set = x.set;
-    return x.start + (e - x.prefix);
-}
-
-__device__ __forceinline__ int enum_of(const SubView& s, const SetDesc& S, int vid) {
-    return S.dEnum + (vid - s.NAS - S.vbase - S.levelStart[s.D]);
-}
-
-// vertex ownership among virtual neighbours only (initSubdivideVertexOwner :698-763)
-__global__ void __launch_bounds__(kThreads) vvertex_owner(SubView s, const Span* __restrict__ sp, int nsp, int nVD,
-                                                          int* __restrict__ flag) {
-    int e = blockIdx.x * kThreads + threadIdx.x;
-    if (e >= nVD) return;
-    int si;
-    int v = venum(sp, nsp, e, si);
-    const int D = s.D;
-    float half = 1.0f / float(1 << (D + 1)), W = 1.0f / float(1 << D), W2 = W * W;
-    int nb[27];
-    for (int i = 0; i < 27; ++i) nb[i] = s.vneigh[size_t(i) * s.VN + v];
-    float3 nc = (nb[13] >= s.NAS) ? key_center(s.vkey[nb[13] - s.NAS], D, D) : make_float3(0.f, 0.f, 0.f);
-    int ownKey[8], own[8];
-    for (int i = 0; i < 8; ++i) {
-        ownKey[i] = 0x7fffffff;
-        own[i] = -1;
-    }
-    for (int j = 0; j < 27; ++j) {
-        if (nb[j] < s.NAS) continue;  // real neighbours never own (:735)
-        float3 c = key_center(s.vkey[nb[j] - s.NAS], D, D);
-        int k = s.vkey[nb[j] - s.NAS];
-        for (int i = 0; i < 8; ++i) {
-            if (sqdist(corner_pos(nc, i, half), c) < double(W2) && ownKey[i] > k) {
-                ownKey[i] = k;
-                own[i] = nb[j];
+        __syncthreads();
+        const int nStage = min(lNumBlk, kStagedBlocks);
+#pragma unroll 4
+        for (int t = threadIdx.x; t < nStage * 27; t += NT) {
+            int j = t / 27, i = t - j * 27;
+            int tag = lBlkNode[j];
+            if (tag >= 0) {  // root level
+                int nb = root_list_node(g, S, lChain[tag * C], i);
+                sBasis[t] = nb < 0 ? make_float4(0.f, 0.f, 0.f, 0.f) : g.basis[nb];
+                sDx[t] = nb < 0 ? 0.f : g.dx[nb];
+            } else {         // ancestor: its precomputed neighbour block
+                int node = lChain[-1 - tag];
+                sBasis[t] = node < 0 ? make_float4(0.f, 0.f, 0.f, 0.f) : g.nbrBasis[27 * size_t(node) + i];
+                sDx[t] = node < 0 ? 0.f : g.nbrDx[27 * size_t(node) + i];
             }
         }
     }
-    for (int i = 0; i < 8; ++i) flag[8 * e + i] = (own[i] == s.NAS + v) ? 1 : 0;
-}
-
-__global__ void __launch_bounds__(kThreads) vvertex_emit(SubView s, const Span* __restrict__ sp, int nsp, int nVD,
-                                                         const int* __restrict__ flag, const int* __restrict__ addr,
-                                                         int* __restrict__ vOwner, int* __restrict__ vKind,
-                                                         float4* __restrict__ vPos) {
-    int t = blockIdx.x * kThreads + threadIdx.x;
-    if (t >= 8 * nVD || !flag[t]) return;
-    int e = t >> 3, i = t & 7, si;
-    int v = venum(sp, nsp, e, si);
-    const int D = s.D;
-    float half = 1.0f / float(1 << (D + 1));
-    int self = s.vneigh[size_t(13) * s.VN + v];
-    float3 nc = (self >= s.NAS) ? key_center(s.vkey[self - s.NAS], D, D) : make_float3(0.f, 0.f, 0.f);
-    float3 p = corner_pos(nc, i, half);
-    int a = addr[t];
-    vOwner[a] = s.NAS + v;
-    vKind[a] = i;
-    vPos[a] = make_float4(p.x, p.y, p.z, __int_as_float(si));
-}
-
-// maintainSubdivideVertexNodePointer (:765-810): vertex slots of virtual nodes only
-__global__ void __launch_bounds__(kThreads) vvertex_maintain(SubView s, int nV, const int* __restrict__ vOwner,
-                                                             const float4* __restrict__ vPos, int nVD,
-                                                             int* __restrict__ vvert) {
-    int t = blockIdx.x * kThreads + threadIdx.x;
-    if (t >= nV) return;
-    int owner = vOwner[t];
-    float4 q = vPos[t];
-    const SetDesc& S = s.sets[__float_as_int(q.w)];
-    const int D = s.D;
-    float W = 1.0f / float(1 << D), W2 = W * W;
-    float3 p = make_float3(q.x, q.y, q.z);
-    for (int i = 0; i < 27; ++i) {
-        int nb = s.vneigh[size_t(i) * s.VN + (owner - s.NAS)];
-        if (nb == -1) continue;
-        float3 c = any_center(s, nb, D);
-        if (!(sqdist(p, c) < double(W2))) continue;
-        if (nb >= s.NAS) vvert[size_t(corner_slot(c, p)) * nVD + enum_of(s, S, nb)] = t + 1;
-    }
-}
-
-__global__ void __launch_bounds__(kThreads) vedge_owner(SubView s, const Span* __restrict__ sp, int nsp, int nVD,
-                                                        int* __restrict__ flag) {
-    int e = blockIdx.x * kThreads + threadIdx.x;
-    if (e >= nVD) return;
-    int si;
-    int v = venum(sp, nsp, e, si);
-    const int D = s.D;
-    float half = 1.0f / float(1 << (D + 1)), W = 1.0f / float(1 << D), W2 = W * W;
-    int nb[27];
-    for (int i = 0; i < 27; ++i) nb[i] = s.vneigh[size_t(i) * s.VN + v];
-    float3 nc = (nb[13] >= s.NAS) ? key_center(s.vkey[nb[13] - s.NAS], D, D) : make_float3(0.f, 0.f, 0.f);
-    int ownKey[12], own[12];
-    for (int k = 0; k < 12; ++k) {
-        ownKey[k] = 0x7fffffff;
-        own[k] = -1;
-    }
-    for (int j = 0; j < 27; ++j) {
-        if (nb[j] < s.NAS) continue;
-        float3 c = key_center(s.vkey[nb[j] - s.NAS], D, D);
-        int kk = s.vkey[nb[j] - s.NAS];
-        for (int k = 0; k < 12; ++k) {
-            if (sqdist(edge_center(nc, k, half), c) < double(W2) && ownKey[k] > kk) {
-                ownKey[k] = kk;
-                own[k] = nb[j];
+    int3 lo = make_int3(0, 0, 0);
+    if (tg.oneRoot) {
+        lo = lRoots[0].first;
+        __syncthreads();
+        // B-spline factors of the list nodes at the tile's corner coordinates
+        for (int t = threadIdx.x; t < K * kTw; t += NT) {
+            int k = t / kTw, c = t - k * kTw;
+            int nb = lList[k];
+            float v = 0.f;
+            if (nb >= 0) {
+                float4 b = g.basis[nb];
+                if (c < kTx)
+                    v = bspline((float(lo.x + c) * inv - b.x) * b.w);
+                else if (c < kTx + kTy)
+                    v = bspline((float(lo.y + c - kTx) * inv - b.y) * b.w);
+                else
+                    v = bspline((float(lo.z + c - kTx - kTy) * inv - b.z) * b.w);
             }
+            lTab[t] = v;
         }
     }
-    for (int k = 0; k < 12; ++k) flag[12 * e + k] = (own[k] == s.NAS + v) ? 1 : 0;
-}
-
-__global__ void __launch_bounds__(kThreads) vedge_emit(SubView s, const Span* __restrict__ sp, int nsp, int nVD,
-                                                       const int* __restrict__ flag, const int* __restrict__ addr,
-                                                       int* __restrict__ eOwner, int* __restrict__ eKind) {
-    int t = blockIdx.x * kThreads + threadIdx.x;
-    if (t >= 12 * nVD || !flag[t]) return;
-    int e = t / 12, si;
-    int v = venum(sp, nsp, e, si);
-    int a = addr[t];
-    eOwner[a] = s.NAS + v;
-    eKind[a] = (t % 12) | (si << 4);
-}
-
-__global__ void __launch_bounds__(kThreads) vedge_maintain(SubView s, int nE, const int* __restrict__ eOwner,
-                                                           const int* __restrict__ eKind, int nVD,
-                                                           int* __restrict__ vedge) {
-    int t = blockIdx.x * kThreads + threadIdx.x;
-    if (t >= nE) return;
-    int owner = eOwner[t], kind = eKind[t] & 15;
-    const SetDesc& S = s.sets[eKind[t] >> 4];
-    const int D = s.D;
-    float half = 1.0f / float(1 << (D + 1)), W = 1.0f / float(1 << D), W2 = W * W;
-    int self = s.vneigh[size_t(13) * s.VN + (owner - s.NAS)];
-    float3 nc = (self != -1) ? any_center(s, self, D) : make_float3(0.f, 0.f, 0.f);
-    float3 p = edge_center(nc, kind, half);
-    for (int i = 0; i < 27; ++i) {
-        int nb = s.vneigh[size_t(i) * s.VN + (owner - s.NAS)];
-        if (nb == -1) continue;
-        float3 c = any_center(s, nb, D);
-        if (!(sqdist(p, c) < double(W2))) continue;
-        if (nb >= s.NAS) vedge[size_t(edge_slot(c, p, kind)) * nVD + enum_of(s, S, nb)] = t + 1;
-    }
-}
-
-// computeSubdivideVertexImplicitFunctionValue (:956-1038)
-__global__ void __launch_bounds__(kThreads) vvertex_values(SubView s, int nV, const int* __restrict__ vOwner,
-                                                           const float4* __restrict__ vPos,
-                                                           const float* __restrict__ iso, float* __restrict__ vv) {
-    int t = blockIdx.x * kThreads + threadIdx.x;
-    if (t >= nV) return;
-    float4 q = vPos[t];
-    const SetDesc& S = s.sets[__float_as_int(q.w)];
-    float3 p = make_float3(q.x, q.y, q.z);
-    const int rootLo = s.NAS + S.vbase + S.levelStart[S.rd];
-    float val = 0.f;
-    int now = vOwner[t];
-    while (now != -1) {
-        for (int i = 0; i < 27; ++i) {
-            int nb = (now < s.NAS) ? s.neigh[i * s.M + now] : s.vneigh[size_t(i) * s.VN + (now - s.NAS)];
-            if (nb == -1) continue;
-            if (nb >= s.NAS) {
-                if (nb >= rootLo && nb < rootLo + S.nr) {
-                    // the root copy stands for the real root node (coarse: rootId, finer: ReplacedNodeId)
-                    int root = s.roots[S.rootOffset + (nb - rootLo)];
-                    nb = s.neigh[13 * s.M + root];
-                    if (nb < 0) continue;
+    __syncthreads();
+    if (!active) return;
+    const float isoV = *iso;
+    // lowest tile cell of this cell's root at depth D
+    const int3 rlo = tg.oneRoot ? lo : make_int3(L.Rc.x << sh, L.Rc.y << sh, L.Rc.z << sh);
+    const BoxVE ve = box_ve(lVbox, tg, r, rlo);
+    const int* halo = lHalo + r * tg.haloPerRoot;
+    for (int b = 0; b < 8; ++b) {
+        int slot;
+        if (corner_owner(L, D, b, slot, ve) != e) continue;
+        int3 o = bourke_offset(b);
+        int3 pt = make_int3(L.c.x + o.x, L.c.y + o.y, L.c.z + o.z);
+        float3 p = make_float3(float(pt.x) * inv, float(pt.y) * inv, float(pt.z) * inv);
+        float val = 0.f;
+        // virtual depths D .. rd+1: real neighbours of the chain node
+        int hb = 0;
+        for (int l = D; l > S.rd; --l) {
+            const int s = l - S.rd;
+            const int3 x = box_ext(tg.ext, D, l);
+            const int by = x.y + 2, bz = x.z + 2;
+            const int3 cl = make_int3(L.c.x >> (D - l), L.c.y >> (D - l), L.c.z >> (D - l));
+            const bool interior = ((cl.x - 1) >> s) == L.Rc.x && ((cl.x + 1) >> s) == L.Rc.x &&
+                                  ((cl.y - 1) >> s) == L.Rc.y && ((cl.y + 1) >> s) == L.Rc.y &&
+                                  ((cl.z - 1) >> s) == L.Rc.z && ((cl.z + 1) >> s) == L.Rc.z;
+            if (!interior) {
+                const int3 bl = make_int3((rlo.x >> (D - l)) - 1, (rlo.y >> (D - l)) - 1, (rlo.z >> (D - l)) - 1);
+                int ids[27];
+#pragma unroll
+                for (int i = 0; i < 27; ++i) {
+                    int3 n = make_int3(cl.x + i / 9 - 1 - bl.x, cl.y + (i / 3) % 3 - 1 - bl.y,
+                                       cl.z + i % 3 - 1 - bl.z);
+                    ids[i] = halo[hb + (n.x * by + n.y) * bz + n.z];
+                }
+                val = accumulate_nodes<9>(g, ids, 27, p, val);
+            }
+            hb += (x.x + 2) * by * bz;
+        }
+        // root depth and real ancestors; absent contributors are zero terms,
+        // which leave the sum unchanged (it never holds -0)
+        if (tg.oneRoot) {
+            const int ix = pt.x - lo.x, iy = kTx + pt.y - lo.y, iz = kTx + kTy + pt.z - lo.z;
+            for (int k = 0; k < K; ++k) {
+                const float* t = lTab + k * kTw;
+                val += lDx[k] * t[ix] * t[iy] * t[iz];
+            }
+        } else {
+            for (int u = 0; u < C; ++u) {
+                const int j = lBlk[r * C + u];
+                if (j < kStagedBlocks) {
+                    const float4* bb = sBasis + j * 27;
+                    const float* dd = sDx + j * 27;
+                    for (int i = 0; i < 27; ++i) val += node_term(bb[i], dd[i], p);
+                } else if (u > 0) {
+                    const int node = lChain[r * C + u];
+                    if (node >= 0) {
+                        const float4* bb = g.nbrBasis + 27 * size_t(node);
+                        const float* dd = g.nbrDx + 27 * size_t(node);
+                        for (int i = 0; i < 27; ++i) val += node_term(bb[i], dd[i], p);
+                    }
                 } else {
-                    continue;  // d_x = 0 in the subdivision space
+                    int ids[27];
+                    const int node = lChain[r * C];
+                    for (int i = 0; i < 27; ++i) ids[i] = root_list_node(g, S, node, i);
+                    val = accumulate_nodes<9>(g, ids, 27, p, val);
                 }
             }
-            int3 f = decode_func_index(s.enc[nb], s.D);
-            val += s.dx[nb] * basis_value(f.x, p.x) * basis_value(f.y, p.y) * basis_value(f.z, p.z);
         }
-        now = (now < s.NAS) ? s.parent[now] : s.vparent[now - s.NAS];
+        cval[size_t(b) * nVD + e] = val - isoV;
     }
-    vv[t] = val - *iso;
 }
 
-__global__ void __launch_bounds__(kThreads) vedge_crossings(SubView s, int nE, const int* __restrict__ eOwner,
-                                                            const int* __restrict__ eKind, int nVD,
-                                                            const int* __restrict__ vvert,
-                                                            const float* __restrict__ vv, int* __restrict__ flag) {
-    int t = blockIdx.x * kThreads + threadIdx.x;
-    if (t >= nE) return;
-    int kind = eKind[t] & 15;
-    const SetDesc& S = s.sets[eKind[t] >> 4];
-    int e = enum_of(s, S, eOwner[t]);
-    int v1 = vvert[size_t(kEdgeVertex[kind][0]) * nVD + e] - 1, v2 = vvert[size_t(kEdgeVertex[kind][1]) * nVD + e] - 1;
-    flag[t] = (vv[v1] * vv[v2] <= 0.f) ? 1 : 0;
-}
-
-__global__ void __launch_bounds__(kThreads) vcrossing_points(SubView s, int nE, const int* __restrict__ eOwner,
-                                                             const int* __restrict__ eKind, int nVD,
-                                                             const int* __restrict__ vvert,
-                                                             const float* __restrict__ vv,
-                                                             const float4* __restrict__ vPos,
-                                                             const int* __restrict__ flag,
-                                                             const int* __restrict__ addr, float* __restrict__ out) {
-    int t = blockIdx.x * kThreads + threadIdx.x;
-    if (t >= nE || !flag[t]) return;
-    int kind = eKind[t] & 15;
-    const SetDesc& S = s.sets[eKind[t] >> 4];
-    int e = enum_of(s, S, eOwner[t]);
-    int v1 = vvert[size_t(kEdgeVertex[kind][0]) * nVD + e] - 1, v2 = vvert[size_t(kEdgeVertex[kind][1]) * nVD + e] - 1;
-    float4 a = vPos[v1], b = vPos[v2];
-    float3 p = interpolate_point(make_float3(a.x, a.y, a.z), make_float3(b.x, b.y, b.z), kind >> 2, vv[v1], vv[v2]);
-    int k = addr[t];
-    out[3 * k] = p.x;
-    out[3 * k + 1] = p.y;
-    out[3 * k + 2] = p.z;
-}
-
-__global__ void __launch_bounds__(kThreads) vcube_classes(int nVD, const int* __restrict__ vvert,
-                                                          const float* __restrict__ vv, int* __restrict__ cube,
-                                                          int* __restrict__ triNum) {
-    int e = blockIdx.x * kThreads + threadIdx.x;
-    if (e >= nVD) return;
+// Corner values of every cell, cube class, triangle count, and the flags of
+// the edges it owns that cross the surface.
+__global__ void __launch_bounds__(kThreads) sub_cells(Grid g, const Tile* __restrict__ tiles, int nVD,
+                                                      const float* __restrict__ cval, float* __restrict__ vals,
+                                                      int* __restrict__ cube, int* __restrict__ triNum,
+                                                      int* __restrict__ cflag) {
+    __shared__ int lVbox[kVboxWords];
+    __shared__ TileRoot lRoots[kMaxRootsPerTile];
+    const Tile T = tiles[blockIdx.x];
+    const SetDesc S = g.sets[T.set];
+    const int D = g.D, sh = D - S.rd;
+    const TileGeom tg = tile_geom(T, S, D);
+    fill_roots(g, T, S, tg, lRoots);
+    __syncthreads();
+    fill_vbox(g, T, S, tg, lRoots, lVbox);
+    __syncthreads();
+    if (int(threadIdx.x) >= T.count) return;
+    const int e = T.start + threadIdx.x;
+    Leaf L = make_leaf_in(g, T.set, S, e);
+    const int3 rlo = tg.oneRoot ? lRoots[0].first : make_int3(L.Rc.x << sh, L.Rc.y << sh, L.Rc.z << sh);
+    const BoxVE ve = box_ve(lVbox, tg, L.q - tg.q0, rlo);
+    float v[8];
     int c = 0;
-    for (int i = 0; i < 8; ++i)
-        if (vv[vvert[size_t(i) * nVD + e] - 1] < 0.f) c |= 1 << i;
+    for (int b = 0; b < 8; ++b) {
+        int slot;
+        int owner = corner_owner(L, D, b, slot, ve);
+        v[b] = cval[size_t(slot) * nVD + owner];
+        vals[size_t(b) * nVD + e] = v[b];
+        if (v[b] < 0.f) c |= 1 << b;
+    }
     cube[e] = c;
     triNum[e] = mc_tri_count(c);
+    for (int k = 0; k < 12; ++k) {
+        int kind;
+        bool own = edge_owner(L, D, k, kind, ve) == e && kind == k;
+        cflag[12 * size_t(e) + k] = (own && v[kEdgeVertex[k][0]] * v[kEdgeVertex[k][1]] <= 0.f) ? 1 : 0;
+    }
 }
 
-// First crossing of each set of a chunk: the crossing scan value at the set's first edge.
-__global__ void set_cross_start(int nSets, const SetDesc* __restrict__ sets, const int* __restrict__ edgeAddr,
-                                int nEflag, const int* __restrict__ vexAddr, int nE, int nCross,
-                                int* __restrict__ crossStart) {
+// Crossing points of the owned crossing edges (generateIntersectionPoint :440-474).
+__global__ void __launch_bounds__(kThreads) sub_crossings(Grid g, const Tile* __restrict__ tiles, int nVD,
+                                                          const float* __restrict__ vals,
+                                                          const int* __restrict__ cflag, const int* __restrict__ cidx,
+                                                          float* __restrict__ out) {
+    const Tile T = tiles[blockIdx.x];
+    if (int(threadIdx.x) >= T.count) return;
+    const int e = T.start + threadIdx.x;
+    int any = 0;
+    for (int k = 0; k < 12; ++k) any |= cflag[12 * size_t(e) + k];
+    if (!any) return;
+    Leaf L = make_leaf_in(g, T.set, g.sets[T.set], e);
+    const float inv = 1.0f / float(1 << g.D);
+    for (int k = 0; k < 12; ++k) {
+        if (!cflag[12 * size_t(e) + k]) continue;
+        int b1 = kEdgeVertex[k][0], b2 = kEdgeVertex[k][1];
+        int3 o1 = bourke_offset(b1), o2 = bourke_offset(b2);
+        float3 p1 = make_float3(float(L.c.x + o1.x) * inv, float(L.c.y + o1.y) * inv, float(L.c.z + o1.z) * inv);
+        float3 p2 = make_float3(float(L.c.x + o2.x) * inv, float(L.c.y + o2.y) * inv, float(L.c.z + o2.z) * inv);
+        float3 p = interpolate_point(p1, p2, k >> 2, vals[size_t(b1) * nVD + e], vals[size_t(b2) * nVD + e]);
+        size_t a = cidx[12 * size_t(e) + k];
+        out[3 * a] = p.x;
+        out[3 * a + 1] = p.y;
+        out[3 * a + 2] = p.z;
+    }
+}
+
+// First crossing of each set: the crossing scan value at the set's first edge flag.
+__global__ void set_cross_start(int nSets, const SetDesc* __restrict__ sets, const int* __restrict__ cidx,
+                                size_t nflag, int nCross, int* __restrict__ crossStart) {
     int si = blockIdx.x * blockDim.x + threadIdx.x;
     if (si >= nSets) return;
-    int f = 12 * sets[si].dEnum;
-    int firstEdge = f < nEflag ? edgeAddr[f] : nE;
-    crossStart[si] = firstEdge < nE ? vexAddr[firstEdge] : nCross;
+    size_t f = 12 * size_t(sets[si].dEnum);
+    crossStart[si] = f < nflag ? cidx[f] : nCross;
 }
 
 // Triangles of the subdivision batches (insertTriangle :1247-1282): an index is
 // the number of vertices kept before the batch plus the raw offset inside the
 // batch; indices are not remapped around dropped vertices.
-__global__ void __launch_bounds__(kThreads) vemit_triangles(const Span* __restrict__ sp, int nsp, int nVD,
-                                                            const int* __restrict__ cube,
-                                                            const int* __restrict__ triAddr,
-                                                            const int* __restrict__ vedge,
-                                                            const int* __restrict__ vexAddr,
-                                                            const int* __restrict__ crossStart,
-                                                            const int* __restrict__ localKept,
-                                                            const int* __restrict__ keptBase, int triBase,
-                                                            int* __restrict__ tri) {
-    int e = blockIdx.x * kThreads + threadIdx.x;
-    if (e >= nVD) return;
+__global__ void __launch_bounds__(kThreads) sub_triangles(Grid g, const Tile* __restrict__ tiles,
+                                                          const int* __restrict__ cube,
+                                                          const int* __restrict__ triAddr,
+                                                          const int* __restrict__ cidx,
+                                                          const int* __restrict__ crossStart,
+                                                          const int* __restrict__ localKept,
+                                                          const int* __restrict__ keptBase, int triBase,
+                                                          int* __restrict__ tri) {
+    const Tile T = tiles[blockIdx.x];
+    if (int(threadIdx.x) >= T.count) return;
+    const int e = T.start + threadIdx.x;
     int c = cube[e];
     int nt = mc_tri_count(c);
     if (!nt) return;
-    int si = sp[find_span(sp, nsp, e)].set;
-    int bs = crossStart[si];  // first raw vertex of this batch within the chunk
+    Leaf L = make_leaf_in(g, T.set, g.sets[T.set], e);
+    int bs = crossStart[L.si];  // first raw vertex of this batch within the chunk
     int keptBefore = *keptBase + localKept[bs];
     uint64_t w = mc_word(c);
     size_t base = size_t(triBase) + triAddr[e];
     for (int k = 0; k < 3 * nt; ++k) {
         int ed = int((w >> (4 * k)) & 0xF);
-        int raw = vexAddr[vedge[size_t(ed) * nVD + e] - 1];
+        int kind;
+        int owner = edge_owner(L, g.D, ed, kind, [&](int3 n) { return virtual_enum(g, L, n); });
+        int raw = cidx[12 * size_t(owner) + kind];
         tri[3 * base + k] = keptBefore + (raw - bs);
     }
+}
+
+// Basis parameters and d_x of the 27 neighbours of every node above depth D-1
+// (the ancestors of subdivision roots); zero where a neighbour is absent.
+__global__ void __launch_bounds__(kThreads) neighbour_blocks(int nNodes, int M, const int* __restrict__ neigh,
+                                                             const float4* __restrict__ basis,
+                                                             const float* __restrict__ dx,
+                                                             float4* __restrict__ nbrBasis,
+                                                             float* __restrict__ nbrDx) {
+    size_t t = size_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (t >= size_t(27) * nNodes) return;
+    int node = int(t / 27), i = int(t - size_t(node) * 27);
+    int nb = neigh[i * M + node];
+    nbrBasis[t] = nb < 0 ? make_float4(0.f, 0.f, 0.f, 0.f) : basis[nb];
+    nbrDx[t] = nb < 0 ? 0.f : dx[nb];
 }
 
 // markValidMeshVertexIndex (:1170-1181): a vertex with |x| < 1e-6 is dropped;
@@ -441,20 +731,20 @@ __global__ void __launch_bounds__(kThreads) fill_int(int* __restrict__ a, size_t
     if (i < n) a[i] = v;
 }
 
-// Default virtual nodes per chunk of subdivision sets; a set larger than the
-// chunk size forms a chunk of its own.
-constexpr size_t kChunkNodes = size_t(1) << 24;
+// Default depth-D virtual cells per chunk of subdivision sets; a set larger
+// than the chunk size forms a chunk of its own.
+constexpr size_t kChunkCells = size_t(1) << 24;
 
 }  // namespace
 
 void extract_mesh(Ctx& c) {
     Octree& o = c.oct;
-    Geometry& g = c.geo;
+    Geometry& geo = c.geo;
     const Levels& L = o.L;
     const int M = o.M, D = L.D, nD = L.count[D], offD = L.base[D];
     hipStream_t st = c.stream;
     // ---- depth-D marching cubes: the first output batch
-    float* mainV = c.buf<float>("raw_main_v", size_t(3) * std::max(g.E, 1));
+    float* mainV = c.buf<float>("raw_main_v", size_t(3) * std::max(geo.E, 1));
     int* mainT = c.buf<int>("raw_main_t", size_t(15) * nD);
     MainMc mm;
     depth_mc(c, mm, mainV, mainT);
@@ -485,35 +775,30 @@ void extract_mesh(Ctx& c) {
         triTotal = mm.nTri;
     }
     // ---- subdivision sets: coarse roots one per set, finer roots one set per depth
+    // (ComputeTriangleIndices.h:371: roots above depth D-4 are coarse)
     const int finerDepth = D - 4;
     std::vector<SetDesc> sets;
-    std::vector<size_t> setNodes;
+    std::vector<size_t> setCells;
     int rootCursor = 0, nCoarse = 0;
     for (int d = 0; d < D; ++d) {
         int cnt = mm.rootsPerDepth[d];
         if (cnt == 0) continue;
-        auto make = [&](int nr, bool coarse) {
+        auto make = [&](int nr) {
             SetDesc S{};
             S.rd = d;
             S.nr = nr;
             S.rootOffset = rootCursor;
-            S.coarse = coarse ? 1 : 0;
-            size_t a = 0;
-            for (int l = d; l <= D; ++l) {
-                S.levelStart[l] = int(a);
-                a += size_t(nr) << (3 * (l - d));
-            }
-            S.levelStart[D + 1] = int(a);
-            if (a + size_t(M) > size_t(INT32_MAX)) throw std::runtime_error("subdivision set exceeds 32-bit node ids");
+            size_t cells = size_t(nr) << (3 * (D - d));
+            if (cells * 12 + 1 > size_t(INT32_MAX)) throw std::runtime_error("subdivision set exceeds 32-bit indexing");
             rootCursor += nr;
             sets.push_back(S);
-            setNodes.push_back(a);
+            setCells.push_back(cells);
         };
         if (d < finerDepth) {
-            for (int k = 0; k < cnt; ++k) make(1, true);
+            for (int k = 0; k < cnt; ++k) make(1);
             nCoarse += cnt;
         } else {
-            make(cnt, false);
+            make(cnt);
         }
     }
     c.stats.n_coarse_roots = nCoarse;
@@ -522,141 +807,90 @@ void extract_mesh(Ctx& c) {
         int* rootPos = c.buf<int>("root_pos", offD);
         fill_int<<<div_up(offD, kThreads), kThreads, 0, st>>>(rootPos, size_t(offD), -1);
         fill_root_pos<<<div_up(mm.nRoots, kThreads), kThreads, 0, st>>>(mm.roots, mm.nRoots, rootPos);
-        NeighborLut lut;
-        make_neighbor_lut(lut);
-        const size_t chunkNodes = c.cfg.subdiv_chunk_nodes ? c.cfg.subdiv_chunk_nodes : kChunkNodes;
+        const size_t chunkCells = std::min<size_t>(c.cfg.subdiv_chunk_nodes ? c.cfg.subdiv_chunk_nodes : kChunkCells,
+                                                   size_t(INT32_MAX - 1) / 12);
+        float* isoDev = c.buf<float>("iso_value", 1);
+        const int nAnc = std::max(1, L.base[D - 1]);  // nodes above depth D-1
+        float4* nbrBasis = c.buf<float4>("sub_nbr_basis", size_t(27) * nAnc);
+        float* nbrDx = c.buf<float>("sub_nbr_dx", size_t(27) * nAnc);
+        neighbour_blocks<<<div_up(size_t(27) * nAnc, kThreads), kThreads, 0, st>>>(nAnc, M, o.neigh, o.basis, c.fld.dx,
+                                                                                  nbrBasis, nbrDx);
         std::vector<SetDesc> cs;
-        std::vector<Span> spans;
+        std::vector<Tile> smallTiles;
+        std::vector<Tile> tiles;
         for (int s0 = 0; s0 < nSets;) {
-            // ---- one chunk of sets [s0, s1), virtual ids local to the chunk
-            size_t vnTotal = 0, vdTotal = 0;
+            // ---- one chunk of sets [s0, s1), cell enumeration local to the chunk
+            size_t cells = 0;
             int s1 = s0;
-            while (s1 < nSets && (s1 == s0 || vnTotal + setNodes[s1] <= chunkNodes)) {
-                vnTotal += setNodes[s1];
-                vdTotal += size_t(sets[s1].nr) << (3 * (D - sets[s1].rd));
-                ++s1;
-            }
-            const int nCs = s1 - s0, VN = int(vnTotal), nVD = int(vdTotal);
+            while (s1 < nSets && (s1 == s0 || cells + setCells[s1] <= chunkCells)) cells += setCells[s1++];
+            const int nCs = s1 - s0, nVD = int(cells);
             cs.assign(sets.begin() + s0, sets.begin() + s1);
-            {
-                int vb = 0, de = 0;
-                for (int k = 0; k < nCs; ++k) {
-                    cs[k].vbase = vb;
-                    cs[k].dEnum = de;
-                    vb += int(setNodes[s0 + k]);
-                    de += cs[k].nr << (3 * (D - cs[k].rd));
-                }
+            for (int k = 0, de = 0; k < nCs; ++k) {
+                cs[k].dEnum = de;
+                de += int(setCells[s0 + k]);
             }
             SetDesc* dsets = c.buf<SetDesc>("sub_sets", nCs);
             PSR_HIP_CHECK(hipMemcpyAsync(dsets, cs.data(), sizeof(SetDesc) * nCs, hipMemcpyHostToDevice, st));
-            // spans per absolute depth
-            spans.clear();
-            std::vector<int> spanStart(D + 2, 0), spanTotal(D + 1, 0);
-            for (int l = 0; l <= D; ++l) {
-                spanStart[l] = int(spans.size());
-                int prefix = 0;
-                for (int si = 0; si < nCs; ++si) {
-                    const SetDesc& S = cs[si];
-                    if (l < S.rd) continue;
-                    int cnt = S.nr << (3 * (l - S.rd));
-                    spans.push_back({si, S.vbase + S.levelStart[l], cnt, prefix});
-                    prefix += cnt;
+            // tiles: those lying in one root first, then those of small roots; the
+            // two groups together cover every cell once
+            tiles.clear();
+            for (int pass = 0; pass < 2; ++pass)
+                for (int k = 0; k < nCs; ++k) {
+                    if ((D - cs[k].rd >= 3) != (pass == 0)) continue;
+                    for (size_t t = 0; t < setCells[s0 + k]; t += kTile)
+                        tiles.push_back({k, cs[k].dEnum + int(t), int(std::min<size_t>(kTile, setCells[s0 + k] - t))});
                 }
-                spanTotal[l] = prefix;
+            const int nTiles = int(tiles.size());
+            int nOne = 0;
+            while (nOne < nTiles && D - cs[tiles[nOne].set].rd >= 3) ++nOne;
+            Tile* dtiles = c.buf<Tile>("sub_tiles", nTiles);
+            PSR_HIP_CHECK(hipMemcpyAsync(dtiles, tiles.data(), sizeof(Tile) * nTiles, hipMemcpyHostToDevice, st));
+            // corner values of small roots run in one-wave tiles of 64 cells
+            smallTiles.clear();
+            for (int k = 0; k < nCs; ++k) {
+                if (D - cs[k].rd >= 3) continue;
+                for (size_t t = 0; t < setCells[s0 + k]; t += kSmallTile)
+                    smallTiles.push_back(
+                        {k, cs[k].dEnum + int(t), int(std::min<size_t>(kSmallTile, setCells[s0 + k] - t))});
             }
-            spanStart[D + 1] = int(spans.size());
-            Span* dspans = c.buf<Span>("sub_spans", spans.size());
-            PSR_HIP_CHECK(hipMemcpyAsync(dspans, spans.data(), sizeof(Span) * spans.size(), hipMemcpyHostToDevice,
-                                         st));
-            SubView sv{};
-            sv.NAS = M;
-            sv.D = D;
-            sv.M = M;
-            sv.nSets = nCs;
-            sv.sets = dsets;
-            sv.roots = mm.roots;
-            sv.rootPos = rootPos;
-            sv.rootPosSize = offD;
-            sv.key = o.key;
-            sv.depth = o.depth;
-            sv.parent = o.parent;
-            sv.childBase = o.childBase;
-            sv.neigh = o.neigh;
-            sv.enc = o.encIdx;
-            sv.dx = c.fld.dx;
-            sv.VN = VN;
-            sv.vkey = c.buf<int>("sub_vkey", VN);
-            sv.vparent = c.buf<int>("sub_vparent", VN);
-            sv.vneigh = c.buf<int>("sub_vneigh", size_t(27) * VN);
-            vnode_init<<<div_up(VN, kThreads), kThreads, 0, st>>>(sv);
-            for (int l = 0; l <= D; ++l) {
-                int n = spanTotal[l];
-                if (n == 0) continue;
-                vnode_neighbors<<<div_up(n, kThreads), kThreads, 0, st>>>(sv, dspans + spanStart[l],
-                                                                         spanStart[l + 1] - spanStart[l], n, l, lut);
-            }
-            Span* dSpans = dspans + spanStart[D];
-            const int nDspans = spanStart[D + 1] - spanStart[D];
-            // vertices of the virtual depth-D nodes
+            const int nSmall = int(smallTiles.size());
+            Tile* dsmall = c.buf<Tile>("sub_small_tiles", std::max(nSmall, 1));
+            if (nSmall)
+                PSR_HIP_CHECK(hipMemcpyAsync(dsmall, smallTiles.data(), sizeof(Tile) * nSmall, hipMemcpyHostToDevice,
+                                             st));
+            Grid gr{D,        M,        nCs,        dsets,     mm.roots, rootPos, o.key,
+                    o.parent, o.childBase, o.neigh, o.basis, c.fld.dx, nbrBasis, nbrDx};
             const size_t nflag = size_t(12) * nVD;
-            int* flag = c.buf<int>("sub_flag", nflag + 1);
-            int* addr = c.buf<int>("sub_addr", nflag + 1);
-            int* cnts = c.buf<int>("sub_counts", 8);
-            int* scratch = c.scratch_for(scan_scratch_ints(nflag + 1) + 64);
-            int* vOwner = c.buf<int>("sub_vowner", size_t(8) * nVD);
-            int* vKind = c.buf<int>("sub_vkind", size_t(8) * nVD);
-            float4* vPos = c.buf<float4>("sub_vpos", size_t(8) * nVD);
-            int* vvert = c.buf<int>("sub_vvert", size_t(8) * nVD);
-            int* vedge = c.buf<int>("sub_vedge", size_t(12) * nVD);
-            fill_int<<<div_up(size_t(8) * nVD, kThreads), kThreads, 0, st>>>(vvert, size_t(8) * nVD, 0);
-            fill_int<<<div_up(size_t(12) * nVD, kThreads), kThreads, 0, st>>>(vedge, size_t(12) * nVD, 0);
-            vvertex_owner<<<div_up(nVD, kThreads), kThreads, 0, st>>>(sv, dSpans, nDspans, nVD, flag);
-            scan_exclusive(flag, addr, 8 * nVD, cnts + 0, scratch, st);
-            vvertex_emit<<<div_up(size_t(8) * nVD, kThreads), kThreads, 0, st>>>(sv, dSpans, nDspans, nVD, flag,
-                                                                                addr, vOwner, vKind, vPos);
-            // edges
-            int* eflag = c.buf<int>("sub_eflag", nflag);
-            int* eflagAddr = c.buf<int>("sub_eaddr", nflag);
-            int* eOwner = c.buf<int>("sub_eowner", nflag);
-            int* eKind = c.buf<int>("sub_ekind", nflag);
-            vedge_owner<<<div_up(nVD, kThreads), kThreads, 0, st>>>(sv, dSpans, nDspans, nVD, eflag);
-            scan_exclusive(eflag, eflagAddr, 12 * nVD, cnts + 1, scratch, st);
-            vedge_emit<<<div_up(nflag, kThreads), kThreads, 0, st>>>(sv, dSpans, nDspans, nVD, eflag, eflagAddr,
-                                                                     eOwner, eKind);
-            PSR_HIP_CHECK(hipMemcpyAsync(c.hostPinned, cnts, sizeof(int) * 2, hipMemcpyDeviceToHost, st));
-            PSR_HIP_CHECK(hipStreamSynchronize(st));
-            const int nVsub = c.hostPinned[0], nEsub = c.hostPinned[1];
-            vvertex_maintain<<<div_up(nVsub, kThreads), kThreads, 0, st>>>(sv, nVsub, vOwner, vPos, nVD, vvert);
-            vedge_maintain<<<div_up(nEsub, kThreads), kThreads, 0, st>>>(sv, nEsub, eOwner, eKind, nVD, vedge);
-            float* vv = c.buf<float>("sub_vvalue", nVsub);
-            float* isoDev = c.buf<float>("iso_value", 1);
-            vvertex_values<<<div_up(nVsub, kThreads), kThreads, 0, st>>>(sv, nVsub, vOwner, vPos, isoDev, vv);
-            int* cflag = c.buf<int>("sub_cflag", nEsub);
-            int* vexAddr = c.buf<int>("sub_vexaddr", nEsub);
-            vedge_crossings<<<div_up(nEsub, kThreads), kThreads, 0, st>>>(sv, nEsub, eOwner, eKind, nVD, vvert, vv,
-                                                                         cflag);
-            scan_exclusive(cflag, vexAddr, nEsub, cnts + 2, scratch, st);
+            float* cval = c.buf<float>("sub_cval", size_t(8) * nVD);
+            float* vals = c.buf<float>("sub_vals", size_t(8) * nVD);
             int* cube = c.buf<int>("sub_cube", nVD);
             int* triNum = c.buf<int>("sub_trinum", nVD);
             int* triAddr = c.buf<int>("sub_triaddr", nVD);
-            vcube_classes<<<div_up(nVD, kThreads), kThreads, 0, st>>>(nVD, vvert, vv, cube, triNum);
-            scan_exclusive(triNum, triAddr, nVD, cnts + 3, scratch, st);
-            PSR_HIP_CHECK(hipMemcpyAsync(c.hostPinned, cnts + 2, sizeof(int) * 2, hipMemcpyDeviceToHost, st));
+            int* cflag = c.buf<int>("sub_cflag", nflag + 1);
+            int* cidx = c.buf<int>("sub_cidx", nflag + 1);
+            int* cnts = c.buf<int>("sub_counts", 8);
+            int* scratch = c.scratch_for(scan_scratch_ints(nflag + 1) + 64);
+            if (nOne > 0) sub_corner_values<true, kThreads><<<nOne, kThreads, 0, st>>>(gr, dtiles, nVD, isoDev, cval);
+            if (nSmall > 0)
+                sub_corner_values<false, kSmallTile><<<nSmall, kSmallTile, 0, st>>>(gr, dsmall, nVD, isoDev, cval);
+            sub_cells<<<nTiles, kThreads, 0, st>>>(gr, dtiles, nVD, cval, vals, cube, triNum, cflag);
+            scan_exclusive(cflag, cidx, int(nflag), cnts + 0, scratch, st);
+            scan_exclusive(triNum, triAddr, nVD, cnts + 1, scratch, st);
+            PSR_HIP_CHECK(hipMemcpyAsync(c.hostPinned, cnts, sizeof(int) * 2, hipMemcpyDeviceToHost, st));
             PSR_HIP_CHECK(hipStreamSynchronize(st));
             const int nCross = c.hostPinned[0], nTri = c.hostPinned[1];
             float* subV = c.buf<float>("raw_sub_v", size_t(3) * std::max(nCross, 1));
-            vcrossing_points<<<div_up(nEsub, kThreads), kThreads, 0, st>>>(sv, nEsub, eOwner, eKind, nVD, vvert, vv,
-                                                                          vPos, cflag, vexAddr, subV);
+            sub_crossings<<<nTiles, kThreads, 0, st>>>(gr, dtiles, nVD, vals, cflag, cidx, subV);
             int* crossStart = c.buf<int>("sub_cross_start", nCs);
-            set_cross_start<<<div_up(nCs, 64), 64, 0, st>>>(nCs, dsets, eflagAddr, int(nflag), vexAddr, nEsub, nCross,
-                                                            crossStart);
+            set_cross_start<<<div_up(nCs, 64), 64, 0, st>>>(nCs, dsets, cidx, nflag, nCross, crossStart);
             // this chunk's batches: vertices, then triangles against the kept count before the chunk
-            append_vertices(subV, nCross, flag, addr, scratch);
+            int* keep = c.buf<int>("sub_keep", size_t(nCross) + 1);
+            int* localKept = c.buf<int>("sub_kept", size_t(nCross) + 1);
+            append_vertices(subV, nCross, keep, localKept, scratch);
             if (nTri > 0) {
                 outT = c.buf_keep<int>("mesh_triangles", 3 * (triTotal + nTri), 3 * triTotal);
-                vemit_triangles<<<div_up(nVD, kThreads), kThreads, 0, st>>>(dSpans, nDspans, nVD, cube, triAddr,
-                                                                            vedge, vexAddr, crossStart, addr, acc,
-                                                                            int(triTotal), outT);
+                sub_triangles<<<nTiles, kThreads, 0, st>>>(gr, dtiles, cube, triAddr, cidx, crossStart, localKept,
+                                                           acc, int(triTotal), outT);
                 triTotal += nTri;
             }
             add_count<<<1, 1, 0, st>>>(acc, acc + 1);
