@@ -41,6 +41,9 @@ def parse():
                     help="both: time the FP32 form and the mixed-precision form; the line's value is the latter")
     ap.add_argument("--cpu-sample", type=int, default=3, help="reconstructions in the CPU-baseline sample (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="OpenMP threads of the CPU baseline (0: all, max 16)")
+    ap.add_argument("--batch-reps", type=int, default=3,
+                    help="timed repetitions of the 64-cloud batch, sequential and batched (0: skip)")
+    ap.add_argument("--batch-workers", type=int, default=8, help="clouds in flight in the batched form")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain elementwise copy of MIB MiB three times (HBM counter calibration)")
     return ap.parse_args()
@@ -114,6 +117,39 @@ def measure(pkg, tx, tn, depth, precision, steps, warmup):
             "cg_bytes": float(np.mean(cg_bytes)), "stats": st}
 
 
+def measure_batch(pkg, reps, workers):
+    """The 64-cloud batch (sizes log-uniform in [5k, 150k], ellipsoids and tori,
+    depth 7): 64 sequential single-cloud calls against one batched call."""
+    import torch
+    clouds = pkg.synth.batch64()
+    dev = [(torch.from_numpy(x).cuda(), torch.from_numpy(n).cuda()) for x, n in clouds]
+    torch.cuda.synchronize()
+    single = pkg.PoissonReconstruction(depth=7)
+    for x, n in dev[:4]:
+        single.SolvePoissionReconstructionMesh(x, n)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for x, n in dev:
+            single.SolvePoissionReconstructionMesh(x, n)
+    torch.cuda.synchronize()
+    seq = (time.perf_counter() - t0) / reps
+    single.close()
+    batch = pkg.PoissonReconstruction(depth=7, batch_workers=workers)
+    batch.ReconstructBatch(dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        batch.ReconstructBatch(dev)
+    torch.cuda.synchronize()
+    bat = (time.perf_counter() - t0) / reps
+    batch.close()
+    return {"workload": "64 clouds, 5k-150k points (%d total), ellipsoids and tori, depth 7, FP32"
+                        % sum(int(x.shape[0]) for x, _ in dev),
+            "sequential_recon_per_s": 64 / seq, "batched_recon_per_s": 64 / bat, "speedup": seq / bat,
+            "batch_workers": workers, "reps": reps}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -169,6 +205,8 @@ def main():
         line["fp32"] = {"value": f["value"], "ms_per_step": f["ms_per_step"], "cg_ms": f["cg_ms"],
                         "cg_iterations": f["stats"]["cg_iterations"]}
         line["mixed_vs_fp32"] = h["value"] / f["value"]
+    if a.batch_reps > 0:
+        line["batch64"] = measure_batch(pkg, a.batch_reps, a.batch_workers)
     if a.cpu_sample > 0:
         line["cpu_baseline"] = cpu_baseline(xyz, nrm, depth, a.cpu_sample, a.cpu_threads)
     else:

@@ -48,6 +48,8 @@ typedef struct {
     int refine_max_outer;  /* PSR_PRECISION_MIXED: outer refinement steps, 4 */
     uint32_t subdiv_chunk_nodes; /* virtual depth-D cells per subdivision working set; 0 = 16M
                                     (about 3 GB); results do not depend on it */
+    uint32_t batch_workers;      /* psr_reconstruct_batch: clouds in flight at once (own stream and
+                                    buffers each); 0 = 8 */
 } psr_config;
 
 typedef struct {
@@ -129,6 +131,17 @@ int psr_create(const psr_config* cfg, void* hip_stream, psr_ctx** out);
 /* Runs the whole pipeline; on return the mesh counts are final and the buffers
  * are ready on the context's stream. */
 int psr_reconstruct(psr_ctx* ctx, const psr_points* points, psr_mesh* out);
+/* Reconstructs count independent clouds.  Up to cfg.batch_workers clouds are
+ * in flight at once, each on a stream and buffers of its own, so that one
+ * cloud's latency-bound stages overlap another's work; the concurrent CG
+ * kernels run one after another on the device.  outs[i] receives cloud i's
+ * mesh (device buffers owned by the context, valid until the next batch call or
+ * psr_destroy); the call returns when every mesh is complete.  No counterpart
+ * in the reference, which reconstructs one cloud per call
+ * (mesh/PoissonReconstruction.cpp:195-245). */
+int psr_reconstruct_batch(psr_ctx* ctx, const psr_points* clouds, uint32_t count, psr_mesh* outs);
+/* Statistics of cloud `index` of the last batch. */
+int psr_get_batch_stats(psr_ctx* ctx, uint32_t index, psr_stats* out);
 int psr_sync(psr_ctx* ctx);
 int psr_get_stats(psr_ctx* ctx, psr_stats* out);
 /* Copies array `array_id` of the last reconstruction to host memory.  With

@@ -1188,7 +1188,7 @@ static void marching_cubes(Result& r) {
     r.batchT.clear();
     // depth-D crossings (:401-422, 440-474)
     int nE = int(r.edges.size());
-    std::vector<int> vexAddr(nE);
+    std::vector<int> vexAddr(nE), crossing(nE, 0);
     std::vector<float> VB;
     int acc = 0;
     for (int k = 0; k < nE; ++k) {
@@ -1196,8 +1196,14 @@ static void marching_cubes(Result& r) {
         int kind = r.edges[k].kind;
         int v1 = ow.vertices[kEdgeVertex[kind][0]] - 1, v2 = ow.vertices[kEdgeVertex[kind][1]] - 1;
         vexAddr[k] = acc;
+        // Where the level-local child 0 parent rule (quirk Q1) has broken the
+        // neighbour tables, slots stay unassigned and the reference reads out of
+        // bounds; here an unassigned slot makes the edge, cube, face or corner
+        // absent (the same rule as the GPU kernels).
+        if (v1 < 0 || v2 < 0) continue;
         float f1 = float(r.vvalue[v1]), f2 = float(r.vvalue[v2]);
         if (f1 * f2 <= 0) {
+            crossing[k] = 1;
             float p[3];
             interp(r.verts[v1].pos, r.verts[v2].pos, kind >> 2, f1, f2, p);
             VB.insert(VB.end(), p, p + 3);
@@ -1211,8 +1217,17 @@ static void marching_cubes(Result& r) {
     for (int o = 0; o < nD; ++o) {
         const OctNode& n = NA[offD + o];
         int cube = 0;
-        for (int i = 0; i < 8; ++i)
-            if (float(r.vvalue[n.vertices[i] - 1]) < 0) cube |= 1 << i;
+        bool ok = true;
+        for (int i = 0; i < 8; ++i) {
+            if (n.vertices[i] == 0) ok = false;
+            else if (float(r.vvalue[n.vertices[i] - 1]) < 0) cube |= 1 << i;
+        }
+        for (int e = 0; e < 12; ++e)
+            if (n.edges[e] == 0) ok = false;
+        if (ok)
+            for (int k = 0; k < 3 * mc_count(cube); ++k)
+                if (!crossing[n.edges[mc_edge(cube, k)] - 1]) ok = false;
+        if (!ok) cube = 0;
         r.cube[o] = cube;
         int nt = mc_count(cube), edgeHas[12] = {0};
         for (int t = 0; t < nt; ++t)
@@ -1226,9 +1241,11 @@ static void marching_cubes(Result& r) {
             for (int j = 0; j < 4; ++j) mark |= edgeHas[kFaceEdges[i][j]];
             if (!mark) continue;
             int pn = n.parent, cf = n.faces[i] - 1;
+            if (cf < 0) continue;
             hasInter[cf] = 1;
-            while (r.faces[cf].hasParentFace != -1) {
+            while (r.faces[cf].hasParentFace != -1 && pn >= 0) {
                 cf = NA[pn].faces[i] - 1;
+                if (cf < 0) break;
                 pn = NA[pn].parent;
                 hasInter[cf] = 1;
             }
@@ -1239,16 +1256,20 @@ static void marching_cubes(Result& r) {
     std::vector<int> roots;
     for (int idx = 0; idx < offD; ++idx) {
         OctNode& n = NA[idx];
-        int sign = float(r.vvalue[n.vertices[0] - 1]) < 0 ? -1 : 1, hasTri = 0;
-        for (int i = 1; i < 8; ++i)
-            if (sign * float(r.vvalue[n.vertices[i] - 1]) < 0) {
-                hasTri = 1;
-                break;
-            }
+        int i0 = 0, hasTri = 0;
+        while (i0 < 8 && n.vertices[i0] == 0) ++i0;
+        if (i0 < 8) {
+            int sign = float(r.vvalue[n.vertices[i0] - 1]) < 0 ? -1 : 1;
+            for (int i = i0 + 1; i < 8; ++i)
+                if (n.vertices[i] != 0 && sign * float(r.vvalue[n.vertices[i] - 1]) < 0) {
+                    hasTri = 1;
+                    break;
+                }
+        }
         n.hasTriangle = hasTri;
         int hi = 0;
         for (int i = 0; i < 6; ++i)
-            if (hasInter[n.faces[i] - 1]) {
+            if (n.faces[i] != 0 && hasInter[n.faces[i] - 1]) {
                 hi = 1;
                 break;
             }
@@ -1256,7 +1277,8 @@ static void marching_cubes(Result& r) {
         bool leaf = true;
         for (int c = 0; c < 8; ++c)
             if (n.children[c] != -1) leaf = false;
-        if (leaf && (hasTri || hi)) roots.push_back(idx);
+        // a root must be its own neighbour: the virtual subtree is built from it
+        if (leaf && (hasTri || hi) && n.neighs[13] == idx) roots.push_back(idx);
     }
     r.rootNodes = roots;
     r.nRoots = int(roots.size());

@@ -33,14 +33,16 @@ _ARRAY_LAYOUT = {
     "MESH_TRIANGLES": (np.int32, 3), "TABLE_DOT": (np.float64, 1), "TABLE_DDOT": (np.float64, 1),
     "TABLE_D2DOT": (np.float64, 1)}
 
-EXPORTED_SYMBOLS = ["psr_default_config", "psr_create", "psr_reconstruct", "psr_sync", "psr_get_stats",
-                    "psr_export", "psr_destroy", "psr_status_string", "psr_last_error"]
+EXPORTED_SYMBOLS = ["psr_default_config", "psr_create", "psr_reconstruct", "psr_reconstruct_batch",
+                    "psr_get_batch_stats", "psr_sync", "psr_get_stats", "psr_export", "psr_destroy",
+                    "psr_status_string", "psr_last_error"]
 
 
 class Config(ctypes.Structure):
     _fields_ = [("depth", ctypes.c_int), ("max_points", ctypes.c_uint32), ("precision", ctypes.c_int),
                 ("cg_tol", ctypes.c_float), ("cg_max_iter", ctypes.c_int), ("refine_rel_tol", ctypes.c_float),
-                ("refine_max_outer", ctypes.c_int), ("subdiv_chunk_nodes", ctypes.c_uint32)]
+                ("refine_max_outer", ctypes.c_int), ("subdiv_chunk_nodes", ctypes.c_uint32),
+                ("batch_workers", ctypes.c_uint32)]
 
 
 class Points(ctypes.Structure):
@@ -82,11 +84,26 @@ class Stats(ctypes.Structure):
             "mesh_vertices": self.n_mesh_vertices, "mesh_triangles": self.n_mesh_triangles,
             "ms_octree": self.ms_octree, "ms_field": self.ms_field, "ms_solve": self.ms_solve,
             "ms_mesh": self.ms_mesh, "ms_total": self.ms_total, "ms_cg_kernel": self.ms_cg_kernel,
-            "cg_workgroups": self.cg_workgroups, "cg_nnz": list(self.cg_nnz)[:d + 1],
+            "cg_workgroups": self.cg_workgroups, "cg_nnz": list(self.cg_nnz)[:d + 1], "q1_fired": self.q1_fired,
         }
 
 
 _lib = None
+_hip = None
+
+
+def _memcpy_d2h(dst, src_ptr):
+    """hipMemcpy of a device pointer into a numpy array (device-to-host), through
+    the HIP runtime libpsr.so is linked against (resolved via its handle)."""
+    global _hip
+    if _hip is None:
+        fn = lib().hipMemcpy
+        fn.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        fn.restype = ctypes.c_int
+        _hip = fn
+    rc = _hip(dst.ctypes.data, ctypes.c_void_p(src_ptr), dst.nbytes, 2)  # hipMemcpyDeviceToHost
+    if rc != 0:
+        raise RuntimeError("hipMemcpy failed: %d" % rc)
 
 
 def lib():
@@ -98,6 +115,9 @@ def lib():
         L.psr_default_config.argtypes = [ctypes.POINTER(Config)]
         L.psr_create.argtypes = [ctypes.POINTER(Config), ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]
         L.psr_reconstruct.argtypes = [ctypes.c_void_p, ctypes.POINTER(Points), ctypes.POINTER(Mesh)]
+        L.psr_reconstruct_batch.argtypes = [ctypes.c_void_p, ctypes.POINTER(Points), ctypes.c_uint32,
+                                            ctypes.POINTER(Mesh)]
+        L.psr_get_batch_stats.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(Stats)]
         L.psr_sync.argtypes = [ctypes.c_void_p]
         L.psr_get_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(Stats)]
         L.psr_export.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64,
@@ -121,7 +141,7 @@ class PoissonReconstruction:
     """Host-side mirror of SparseSurfelFusion::PoissonReconstruction."""
 
     def __init__(self, depth=7, precision=PRECISION_FP32, max_points=0, cg_tol=1e-5, cg_max_iter=10000,
-                 stream=None, subdiv_chunk_nodes=0):
+                 stream=None, subdiv_chunk_nodes=0, batch_workers=0):
         cfg = Config()
         lib().psr_default_config(ctypes.byref(cfg))
         cfg.depth = depth
@@ -130,6 +150,7 @@ class PoissonReconstruction:
         cfg.cg_tol = cg_tol
         cfg.cg_max_iter = cg_max_iter
         cfg.subdiv_chunk_nodes = subdiv_chunk_nodes
+        cfg.batch_workers = batch_workers
         self._ctx = ctypes.c_void_p()
         rc = lib().psr_create(ctypes.byref(cfg), stream, ctypes.byref(self._ctx))
         if rc != PSR_OK:
@@ -163,6 +184,37 @@ class PoissonReconstruction:
             return self.solve_ptr(base + 16, base + 32, xyz.shape[0], 64)
         assert xyz.is_cuda and normals.is_cuda and xyz.is_contiguous() and normals.is_contiguous()
         return self.solve_ptr(xyz.data_ptr(), normals.data_ptr(), xyz.shape[0], 12)
+
+    def ReconstructBatch(self, clouds):
+        """clouds: list of (xyz, normals) contiguous float32 GPU tensor pairs.
+        Returns one psr_mesh per cloud (device buffers valid until the next batch)."""
+        n = len(clouds)
+        pts = (Points * n)(*[Points(x.data_ptr(), nn.data_ptr(), x.shape[0], 12) for x, nn in clouds])
+        self.batch = (Mesh * n)()
+        rc = lib().psr_reconstruct_batch(self._ctx, pts, n, self.batch)
+        if rc != PSR_OK:
+            raise PsrError(rc, lib().psr_last_error(self._ctx).decode())
+        return list(self.batch)
+
+    def batch_stats(self, i):
+        st = Stats()
+        rc = lib().psr_get_batch_stats(self._ctx, i, ctypes.byref(st))
+        if rc != PSR_OK:
+            raise PsrError(rc, "batch stats %d" % i)
+        return st.as_dict()
+
+    @staticmethod
+    def mesh_to_host(mesh):
+        """Copies a psr_mesh (device buffers) to numpy (vertices (V, 3), triangles (T, 3))."""
+        import torch
+        v = np.empty((mesh.n_vertices, 3), np.float32)
+        t = np.empty((mesh.n_triangles, 3), np.int32)
+        torch.cuda.synchronize()
+        if mesh.n_vertices:
+            _memcpy_d2h(v, mesh.vertices)
+        if mesh.n_triangles:
+            _memcpy_d2h(t, mesh.triangles)
+        return v, t
 
     def stats(self):
         st = Stats()

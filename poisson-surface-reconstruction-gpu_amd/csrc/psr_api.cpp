@@ -2,6 +2,9 @@
 // mesh/PoissonReconstruction.cpp:195-245): octree -> field -> divergence ->
 // per-depth CG -> implicit function -> geometry -> marching cubes.
 #include <algorithm>
+#include <atomic>
+#include <mutex>
+#include <thread>
 #include <cstdlib>
 #include <cstring>
 #include <new>
@@ -14,6 +17,10 @@
 struct psr_ctx {
     psr::Ctx c;
     std::string lastError;
+    // batch mode
+    std::vector<psr_ctx*> workers;  // contexts of their own (stream, buffers) for clouds in flight
+    std::vector<psr_stats> batchStats;
+    std::mutex batchMu;             // guards the arena of this context while workers copy out
 };
 
 namespace psr {
@@ -224,6 +231,83 @@ int psr_reconstruct(psr_ctx* ctx, const psr_points* points, psr_mesh* out) {
     }
 }
 
+int psr_reconstruct_batch(psr_ctx* ctx, const psr_points* clouds, uint32_t count, psr_mesh* outs) {
+    if (!ctx || (count && (!clouds || !outs))) return PSR_ERR_INVALID;
+    psr::Ctx& c = ctx->c;
+    const uint32_t W = std::min<uint32_t>(count, c.cfg.batch_workers ? c.cfg.batch_workers : 8);
+    try {
+        while (ctx->workers.size() < W) {
+            psr_ctx* w = nullptr;
+            int rc = psr_create(&c.cfg, nullptr, &w);
+            if (rc != PSR_OK) return fail(ctx, rc, "batch worker creation failed");
+            ctx->workers.push_back(w);
+        }
+    } catch (const std::exception& e) {
+        return fail(ctx, PSR_ERR_INTERNAL, e.what());
+    }
+    ctx->batchStats.assign(count, psr_stats{});
+    std::atomic<uint32_t> next{0};
+    std::atomic<int> status{PSR_OK};
+    std::string message;
+    std::mutex msgMu;
+    auto record = [&](int rc, uint32_t i, const std::string& m) {
+        int expected = PSR_OK;
+        if (status.compare_exchange_strong(expected, rc)) {
+            std::lock_guard<std::mutex> lk(msgMu);
+            message = "cloud " + std::to_string(i) + ": " + m;
+        }
+    };
+    auto work = [&](uint32_t wi) {
+        psr_ctx* wk = ctx->workers[wi];
+        if (hipSetDevice(c.device) != hipSuccess) {
+            record(PSR_ERR_HIP, 0, "hipSetDevice");
+            return;
+        }
+        for (;;) {
+            const uint32_t i = next.fetch_add(1);
+            if (i >= count || status.load() != PSR_OK) return;
+            psr_mesh m{};
+            int rc = psr_reconstruct(wk, &clouds[i], &m);
+            if (rc != PSR_OK) {
+                record(rc, i, wk->lastError);
+                return;
+            }
+            try {
+                float* v;
+                int* t;
+                {
+                    std::lock_guard<std::mutex> lk(ctx->batchMu);
+                    v = c.buf<float>(("batch_v_" + std::to_string(i)).c_str(), size_t(3) * m.n_vertices);
+                    t = c.buf<int>(("batch_t_" + std::to_string(i)).c_str(), size_t(3) * m.n_triangles);
+                }
+                hipStream_t ws = wk->c.stream;
+                PSR_HIP_CHECK(hipMemcpyAsync(v, m.vertices, sizeof(float) * 3 * m.n_vertices,
+                                             hipMemcpyDeviceToDevice, ws));
+                PSR_HIP_CHECK(hipMemcpyAsync(t, m.triangles, sizeof(int) * 3 * m.n_triangles,
+                                             hipMemcpyDeviceToDevice, ws));
+                PSR_HIP_CHECK(hipStreamSynchronize(ws));
+                outs[i] = psr_mesh{v, t, m.n_vertices, m.n_triangles};
+                ctx->batchStats[i] = wk->c.stats;
+            } catch (const std::exception& e) {
+                record(PSR_ERR_HIP, i, e.what());
+                return;
+            }
+        }
+    };
+    std::vector<std::thread> threads;
+    for (uint32_t w = 1; w < W; ++w) threads.emplace_back(work, w);
+    if (W > 0) work(0);
+    for (auto& th : threads) th.join();
+    if (status.load() != PSR_OK) return fail(ctx, status.load(), message);
+    return PSR_OK;
+}
+
+int psr_get_batch_stats(psr_ctx* ctx, uint32_t index, psr_stats* out) {
+    if (!ctx || !out || index >= ctx->batchStats.size()) return PSR_ERR_INVALID;
+    *out = ctx->batchStats[index];
+    return PSR_OK;
+}
+
 int psr_sync(psr_ctx* ctx) {
     if (!ctx) return PSR_ERR_INVALID;
     return hipStreamSynchronize(ctx->c.stream) == hipSuccess ? PSR_OK : PSR_ERR_HIP;
@@ -389,6 +473,8 @@ int psr_export(psr_ctx* ctx, int id, void* dst, uint64_t bytes, uint64_t* needed
 
 void psr_destroy(psr_ctx* ctx) {
     if (!ctx) return;
+    for (psr_ctx* w : ctx->workers) psr_destroy(w);
+    ctx->workers.clear();
     psr::Ctx& c = ctx->c;
     if (c.stream) (void)hipStreamSynchronize(c.stream);
     c.arena.release();

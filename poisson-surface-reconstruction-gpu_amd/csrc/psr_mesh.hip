@@ -265,7 +265,9 @@ __global__ void __launch_bounds__(kThreads) edge_crossings(const int* __restrict
     int owner = eOwner[t], kind = eKind[t];
     int v1 = nodeVert[kEdgeVertex[kind][0] * M + owner] - 1;
     int v2 = nodeVert[kEdgeVertex[kind][1] * M + owner] - 1;
-    flag[t] = (vv[v1] * vv[v2] <= 0.f) ? 1 : 0;
+    // a corner slot left unassigned (the reference's level-local child 0 parent
+    // rule breaks neighbour tables) makes the edge absent, not a crossing
+    flag[t] = (v1 >= 0 && v2 >= 0 && vv[v1] * vv[v2] <= 0.f) ? 1 : 0;
 }
 
 __global__ void __launch_bounds__(kThreads) crossing_points(const int* __restrict__ count, int M,
@@ -290,14 +292,32 @@ __global__ void __launch_bounds__(kThreads) crossing_points(const int* __restric
     out[3 * k + 2] = p.z;
 }
 
+// A cube emits triangles only when all its corners and edges are present and
+// every edge its triangles use crosses; otherwise (only where the reference's
+// level-local child 0 parent rule has broken the tables and the reference
+// reads out of bounds) it emits none.
 __global__ void __launch_bounds__(kThreads) cube_classes(int offD, int nD, int M, const int* __restrict__ nodeVert,
+                                                         const int* __restrict__ nodeEdge,
+                                                         const int* __restrict__ crossFlag,
                                                          const float* __restrict__ vv, int* __restrict__ cube,
                                                          int* __restrict__ triNum) {
     int t = blockIdx.x * kThreads + threadIdx.x;
     if (t >= nD) return;
     int o = offD + t, c = 0;
-    for (int i = 0; i < 8; ++i)
-        if (vv[nodeVert[i * M + o] - 1] < 0.f) c |= 1 << i;
+    bool ok = true;
+    for (int i = 0; i < 8; ++i) {
+        int v = nodeVert[i * M + o];
+        if (v == 0) ok = false;
+        else if (vv[v - 1] < 0.f) c |= 1 << i;
+    }
+    for (int e = 0; e < 12; ++e)
+        if (nodeEdge[e * nD + t] == 0) ok = false;
+    if (ok) {
+        uint64_t w = mc_word(c);
+        for (int k = 0, n = 3 * mc_tri_count(c); k < n; ++k)
+            if (!crossFlag[nodeEdge[int((w >> (4 * k)) & 0xF) * nD + t] - 1]) ok = false;
+    }
+    if (!ok) c = 0;
     cube[t] = c;
     triNum[t] = mc_tri_count(c);
 }
@@ -324,9 +344,12 @@ __global__ void __launch_bounds__(kThreads) emit_triangles_and_faces(
         if (!mark) continue;
         int pn = parent[o];
         int cf = nodeFace[f * M + o] - 1;
+        if (cf < 0) continue;  // face slot left unassigned (broken tables): nothing to mark
         hasInter[cf] = 1;
-        while (faceHasParent[cf] != -1) {
+        // the walk stops where the reference would leave the tree or read an unassigned slot
+        while (faceHasParent[cf] != -1 && pn >= 0) {
             cf = nodeFace[f * M + pn] - 1;
+            if (cf < 0) break;
             pn = parent[pn];
             hasInter[cf] = 1;
         }
@@ -334,28 +357,39 @@ __global__ void __launch_bounds__(kThreads) emit_triangles_and_faces(
 }
 
 // --- subdivision roots (ProcessLeafNodesAtOtherDepth :555-587, copy_if) -----
-__global__ void __launch_bounds__(kThreads) root_flags(int count, int M, const int* __restrict__ nodeVert,
+__global__ void __launch_bounds__(kThreads) root_flags(int count, int M, const int* __restrict__ neigh,
+                                                       const int* __restrict__ nodeVert,
                                                        const int* __restrict__ nodeFace,
                                                        const int* __restrict__ childBase,
                                                        const float* __restrict__ vv,
                                                        const int* __restrict__ hasInter, int* __restrict__ flag) {
     int o = blockIdx.x * kThreads + threadIdx.x;
     if (o >= count) return;
-    float v0 = vv[nodeVert[o] - 1];
-    float sign = v0 < 0.f ? -1.f : 1.f;
+    // unassigned corner and face slots are skipped; the sign reference is the
+    // first present corner (corner 0 in a consistent tree)
+    int i0 = 0;
+    while (i0 < 8 && nodeVert[i0 * M + o] == 0) ++i0;
     int hasTri = 0;
-    for (int i = 1; i < 8; ++i)
-        if (sign * vv[nodeVert[i * M + o] - 1] < 0.f) {
-            hasTri = 1;
-            break;
+    if (i0 < 8) {
+        float sign = vv[nodeVert[i0 * M + o] - 1] < 0.f ? -1.f : 1.f;
+        for (int i = i0 + 1; i < 8; ++i) {
+            int v = nodeVert[i * M + o];
+            if (v != 0 && sign * vv[v - 1] < 0.f) {
+                hasTri = 1;
+                break;
+            }
         }
+    }
     int hi = 0;
-    for (int f = 0; f < 6; ++f)
-        if (hasInter[nodeFace[f * M + o] - 1]) {
+    for (int f = 0; f < 6; ++f) {
+        int fs = nodeFace[f * M + o];
+        if (fs != 0 && hasInter[fs - 1]) {
             hi = 1;
             break;
         }
-    flag[o] = (childBase[o] < 0 && (hasTri || hi)) ? 1 : 0;
+    }
+    // a root must be its own neighbour (the virtual subtree is built from it)
+    flag[o] = (childBase[o] < 0 && (hasTri || hi) && neigh[13 * M + o] == o) ? 1 : 0;
 }
 
 __global__ void __launch_bounds__(kThreads) root_emit(int count, int M, const int* __restrict__ flag,
@@ -450,7 +484,8 @@ void depth_mc(Ctx& c, MainMc& mm, float* rawV, int* rawT) {
     mm.cube = c.buf<int>("mc_cube", nD);
     int* triNum = c.buf<int>("mc_trinum", nD);
     mm.triAddr = c.buf<int>("mc_triaddr", nD);
-    cube_classes<<<div_up(nD, kThreads), kThreads, 0, s>>>(offD, nD, M, g.nodeVert, g.vvalue, mm.cube, triNum);
+    cube_classes<<<div_up(nD, kThreads), kThreads, 0, s>>>(offD, nD, M, g.nodeVert, g.nodeEdge, cflag, g.vvalue,
+                                                           mm.cube, triNum);
     scan_exclusive(triNum, mm.triAddr, nD, mm.counts + 1, scratch, s);
     mm.hasInter = c.buf<int>("mc_hasinter", F);
     zero_ints<<<div_up(F, kThreads), kThreads, 0, s>>>(mm.hasInter, size_t(F));
@@ -461,8 +496,8 @@ void depth_mc(Ctx& c, MainMc& mm, float* rawV, int* rawT) {
     int* rflag = c.buf<int>("root_flag", offD);
     int* raddr = c.buf<int>("root_addr", offD);
     mm.roots = c.buf<int>("roots", offD);
-    root_flags<<<div_up(offD, kThreads), kThreads, 0, s>>>(offD, M, g.nodeVert, g.nodeFace, o.childBase, g.vvalue,
-                                                           mm.hasInter, rflag);
+    root_flags<<<div_up(offD, kThreads), kThreads, 0, s>>>(offD, M, o.neigh, g.nodeVert, g.nodeFace, o.childBase,
+                                                           g.vvalue, mm.hasInter, rflag);
     scan_exclusive(rflag, raddr, offD, mm.counts + 2, scratch, s);
     root_emit<<<div_up(offD, kThreads), kThreads, 0, s>>>(offD, M, rflag, raddr, o.neigh, o.depth, mm.roots,
                                                           mm.counts + 4);

@@ -3,6 +3,7 @@
 #include "psr_internal.h"
 
 #include <algorithm>
+#include <mutex>
 #include <stdexcept>
 
 namespace psr {
@@ -562,6 +563,21 @@ __global__ void zero_u32(unsigned* a, int n) {
 
 }  // namespace
 
+std::mutex& cg_order_lock() {
+    static std::mutex m;
+    return m;
+}
+
+// Completion of the last cooperative CG launched in this process (one GPU per process).
+hipEvent_t cg_order_event() {
+    static hipEvent_t e = [] {
+        hipEvent_t ev = nullptr;
+        PSR_HIP_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        return ev;
+    }();
+    return e;
+}
+
 void solve_depths(Ctx& c) {
     Octree& o = c.oct;
     const Levels& L = o.L;
@@ -627,6 +643,11 @@ void solve_depths(Ctx& c) {
     zero_u32<<<div_up(ctlWords, 256), 256, 0, s>>>(reinterpret_cast<unsigned*>(ctl), ctlWords);
     zero_u32<<<1, 64, 0, s>>>(reinterpret_cast<unsigned*>(err), 1);
     zero_u32<<<1, 64, 0, s>>>(reinterpret_cast<unsigned*>(trueRel), 2 * (PSR_MAX_DEPTH + 1));
+    // Cooperative CG kernels of different contexts (streams) are ordered on the
+    // device: each waits for the previous one to finish, so two of them never
+    // compete for residency while spinning in their group barriers.
+    std::lock_guard<std::mutex> order(cg_order_lock());
+    PSR_HIP_CHECK(hipStreamWaitEvent(s, cg_order_event(), 0));
     PSR_HIP_CHECK(hipEventRecord(c.ev[8], s));
     if (c.cfg.precision == PSR_PRECISION_MIXED) {
         P.relTol = c.cfg.refine_rel_tol;
@@ -647,6 +668,7 @@ void solve_depths(Ctx& c) {
                                                  dim3(kCgThreads), args, 0, s));
     }
     PSR_HIP_CHECK(hipEventRecord(c.ev[9], s));
+    PSR_HIP_CHECK(hipEventRecord(cg_order_event(), s));
     PSR_HIP_CHECK(hipGetLastError());
 }
 

@@ -263,7 +263,7 @@ constexpr int kMaxRootsPerTile = 32;
 constexpr int kLdsWordsOne = 7168;     // one-root tiles: list, halo, vbox, d_x, factors (<= 6954 words)
 constexpr int kLdsWordsSmall = 5376;   // small-root tiles of 64 cells: halo, vbox, staged blocks (<= 5344 words)
 constexpr int kStagedBlocks = 32;  // contributor blocks of 27 staged in LDS per small-root tile
-constexpr int kSmallTile = 64;     // cells (and threads) of a small-root tile
+constexpr int kSmallThreads = 128;  // threads of a small-root tile (4 x 27 or 125 corner points)
 constexpr int kVboxWords = 2048;
 
 __device__ __forceinline__ int3 box_ext(int3 extD, int D, int l) {
@@ -507,18 +507,12 @@ __global__ void __launch_bounds__(NT) sub_corner_values(Grid g, const Tile* __re
         }
     }
     __syncthreads();
-    if (!active) return;
     const float isoV = *iso;
-    // lowest tile cell of this cell's root at depth D
-    const int3 rlo = tg.oneRoot ? lo : make_int3(L.Rc.x << sh, L.Rc.y << sh, L.Rc.z << sh);
-    const BoxVE ve = box_ve(lVbox, tg, r, rlo);
-    const int* halo = lHalo + r * tg.haloPerRoot;
-    for (int b = 0; b < 8; ++b) {
-        int slot;
-        if (corner_owner(L, D, b, slot, ve) != e) continue;
-        int3 o = bourke_offset(b);
-        int3 pt = make_int3(L.c.x + o.x, L.c.y + o.y, L.c.z + o.z);
+    // Value at grid point pt through the chain of owner cell Lo (in tile root rr,
+    // whose lowest tile cell is rlo).
+    auto value_at = [&](const Leaf& Lo, int rr, int3 rlo, int3 pt) -> float {
         float3 p = make_float3(float(pt.x) * inv, float(pt.y) * inv, float(pt.z) * inv);
+        const int* halo = lHalo + rr * tg.haloPerRoot;
         float val = 0.f;
         // virtual depths D .. rd+1: real neighbours of the chain node
         int hb = 0;
@@ -526,10 +520,10 @@ __global__ void __launch_bounds__(NT) sub_corner_values(Grid g, const Tile* __re
             const int s = l - S.rd;
             const int3 x = box_ext(tg.ext, D, l);
             const int by = x.y + 2, bz = x.z + 2;
-            const int3 cl = make_int3(L.c.x >> (D - l), L.c.y >> (D - l), L.c.z >> (D - l));
-            const bool interior = ((cl.x - 1) >> s) == L.Rc.x && ((cl.x + 1) >> s) == L.Rc.x &&
-                                  ((cl.y - 1) >> s) == L.Rc.y && ((cl.y + 1) >> s) == L.Rc.y &&
-                                  ((cl.z - 1) >> s) == L.Rc.z && ((cl.z + 1) >> s) == L.Rc.z;
+            const int3 cl = make_int3(Lo.c.x >> (D - l), Lo.c.y >> (D - l), Lo.c.z >> (D - l));
+            const bool interior = ((cl.x - 1) >> s) == Lo.Rc.x && ((cl.x + 1) >> s) == Lo.Rc.x &&
+                                  ((cl.y - 1) >> s) == Lo.Rc.y && ((cl.y + 1) >> s) == Lo.Rc.y &&
+                                  ((cl.z - 1) >> s) == Lo.Rc.z && ((cl.z + 1) >> s) == Lo.Rc.z;
             if (!interior) {
                 const int3 bl = make_int3((rlo.x >> (D - l)) - 1, (rlo.y >> (D - l)) - 1, (rlo.z >> (D - l)) - 1);
                 int ids[27];
@@ -553,13 +547,13 @@ __global__ void __launch_bounds__(NT) sub_corner_values(Grid g, const Tile* __re
             }
         } else {
             for (int u = 0; u < C; ++u) {
-                const int j = lBlk[r * C + u];
+                const int j = lBlk[rr * C + u];
                 if (j < kStagedBlocks) {
                     const float4* bb = sBasis + j * 27;
                     const float* dd = sDx + j * 27;
                     for (int i = 0; i < 27; ++i) val += node_term(bb[i], dd[i], p);
                 } else if (u > 0) {
-                    const int node = lChain[r * C + u];
+                    const int node = lChain[rr * C + u];
                     if (node >= 0) {
                         const float4* bb = g.nbrBasis + 27 * size_t(node);
                         const float* dd = g.nbrDx + 27 * size_t(node);
@@ -567,13 +561,52 @@ __global__ void __launch_bounds__(NT) sub_corner_values(Grid g, const Tile* __re
                     }
                 } else {
                     int ids[27];
-                    const int node = lChain[r * C];
+                    const int node = lChain[rr * C];
                     for (int i = 0; i < 27; ++i) ids[i] = root_list_node(g, S, node, i);
                     val = accumulate_nodes<9>(g, ids, 27, p, val);
                 }
             }
         }
-        cval[size_t(b) * nVD + e] = val - isoV;
+        return val;
+    };
+    if (ONE) {
+        // one thread per cell: the corners it owns (one, away from the root's faces)
+        if (!active) return;
+        const BoxVE ve = box_ve(lVbox, tg, r, lo);
+        for (int b = 0; b < 8; ++b) {
+            int slot;
+            if (corner_owner(L, D, b, slot, ve) != e) continue;
+            int3 o = bourke_offset(b);
+            cval[size_t(b) * nVD + e] = value_at(L, 0, lo, make_int3(L.c.x + o.x, L.c.y + o.y, L.c.z + o.z)) - isoV;
+        }
+    } else {
+        // one thread per corner point of each small root; the point is evaluated
+        // for the root holding its owner
+        const int n = (1 << sh) + 1, P = n * n * n;
+        const int rr = threadIdx.x / P, pi = threadIdx.x - rr * P;
+        if (rr >= tg.nRoots) return;
+        const TileRoot R = lRoots[rr];
+        const int3 pt = make_int3(R.first.x + pi / (n * n), R.first.y + (pi / n) % n, R.first.z + pi % n);
+        const BoxVE ve = box_ve(lVbox, tg, rr, R.first);
+        int best = -1, bestKey = 0x7fffffff, slot = 0;
+        int3 bestCell = pt;
+        for (int k = 0; k < 8; ++k) {
+            int3 d = make_int3(k & 1, (k >> 1) & 1, (k >> 2) & 1);
+            int3 c = make_int3(pt.x - d.x, pt.y - d.y, pt.z - d.z);
+            int v = ve(c);
+            if (v < 0) continue;
+            int key = morton(c, D);
+            if (key < bestKey) {
+                bestKey = key;
+                best = v;
+                slot = bourke_slot(d.x, d.y, d.z);
+                bestCell = c;
+            }
+        }
+        if (best < 0 || ((best - S.dEnum) >> (3 * sh)) != tg.q0 + rr) return;
+        Leaf Lo = root_leaf(g, S, T.set, R);
+        Lo.c = bestCell;
+        cval[size_t(slot) * nVD + best] = value_at(Lo, rr, R.first, pt) - isoV;
     }
 }
 
@@ -845,13 +878,15 @@ void extract_mesh(Ctx& c) {
             while (nOne < nTiles && D - cs[tiles[nOne].set].rd >= 3) ++nOne;
             Tile* dtiles = c.buf<Tile>("sub_tiles", nTiles);
             PSR_HIP_CHECK(hipMemcpyAsync(dtiles, tiles.data(), sizeof(Tile) * nTiles, hipMemcpyHostToDevice, st));
-            // corner values of small roots run in one-wave tiles of 64 cells
+            // corner values of small roots: a tile per 4 roots of 8 cells or per
+            // root of 64 cells, one thread per corner point
             smallTiles.clear();
             for (int k = 0; k < nCs; ++k) {
-                if (D - cs[k].rd >= 3) continue;
-                for (size_t t = 0; t < setCells[s0 + k]; t += kSmallTile)
-                    smallTiles.push_back(
-                        {k, cs[k].dEnum + int(t), int(std::min<size_t>(kSmallTile, setCells[s0 + k] - t))});
+                const int shk = D - cs[k].rd;
+                if (shk >= 3) continue;
+                const size_t step = shk == 1 ? 32 : 64;
+                for (size_t t = 0; t < setCells[s0 + k]; t += step)
+                    smallTiles.push_back({k, cs[k].dEnum + int(t), int(std::min<size_t>(step, setCells[s0 + k] - t))});
             }
             const int nSmall = int(smallTiles.size());
             Tile* dsmall = c.buf<Tile>("sub_small_tiles", std::max(nSmall, 1));
@@ -872,7 +907,7 @@ void extract_mesh(Ctx& c) {
             int* scratch = c.scratch_for(scan_scratch_ints(nflag + 1) + 64);
             if (nOne > 0) sub_corner_values<true, kThreads><<<nOne, kThreads, 0, st>>>(gr, dtiles, nVD, isoDev, cval);
             if (nSmall > 0)
-                sub_corner_values<false, kSmallTile><<<nSmall, kSmallTile, 0, st>>>(gr, dsmall, nVD, isoDev, cval);
+                sub_corner_values<false, kSmallThreads><<<nSmall, kSmallThreads, 0, st>>>(gr, dsmall, nVD, isoDev, cval);
             sub_cells<<<nTiles, kThreads, 0, st>>>(gr, dtiles, nVD, cval, vals, cube, triNum, cflag);
             scan_exclusive(cflag, cidx, int(nflag), cnts + 0, scratch, st);
             scan_exclusive(triNum, triAddr, nVD, cnts + 1, scratch, st);

@@ -298,3 +298,51 @@ def test_mixed_precision_meets_residual_and_mesh_targets(runs, mixed_runs, case)
     diag = float(np.linalg.norm(hi - lo))
     h = max(cKDTree(ov).query(v)[0].max(), cKDTree(v).query(ov)[0].max())
     assert h <= 1e-4 * diag, h
+
+
+def test_batch_matches_single_reconstructions(pkg, torch):
+    """psr_reconstruct_batch: every cloud's mesh equals its own single-cloud
+    reconstruction (same kernels, deterministic), whatever runs beside it."""
+    s = pkg.synth
+    clouds = [s.sphere(n=6000), s.torus(n=30000)] + [tuple(c) for c in s.batch64(count=6)[:6]]
+    dev = [(torch.from_numpy(x).cuda(), torch.from_numpy(n).cuda()) for x, n in clouds]
+    batch = pkg.PoissonReconstruction(depth=6, batch_workers=3)
+    meshes = batch.ReconstructBatch(dev)
+    assert len(meshes) == len(clouds)
+    single = pkg.PoissonReconstruction(depth=6)
+    for i, (x, n) in enumerate(dev):
+        single.SolvePoissionReconstructionMesh(x, n)
+        v, t = batch.mesh_to_host(meshes[i])
+        np.testing.assert_array_equal(v, single.GetRebuildMeshVertices())
+        np.testing.assert_array_equal(t, single.GetRebuildMeshTriangleIndices())
+        st = batch.batch_stats(i)
+        assert st["n_points"] == x.shape[0] and st["mesh_triangles"] == t.shape[0]
+    # a second, smaller batch reuses the buffers
+    meshes = batch.ReconstructBatch(dev[:2])
+    single.SolvePoissionReconstructionMesh(*dev[1])
+    v, t = batch.mesh_to_host(meshes[1])
+    np.testing.assert_array_equal(t, single.GetRebuildMeshTriangleIndices())
+    single.close()
+    batch.close()
+
+
+def test_level_local_child0_quirk_parity(pkg, orc, torch):
+    """Quirk Q1 (BuildOctree.cu:384-389, 412) on a cloud that fires it: the node
+    arrays and neighbour tables reproduce it bit-exactly, and the mesh stages,
+    where the reference would read unassigned slots, follow the same defined
+    rule as the oracle (absent slots are skipped)."""
+    x, n = pkg.synth.batch64()[36]
+    tx, tn = _upload(torch, x, n)
+    rec = pkg.PoissonReconstruction(depth=7)
+    rec.SolvePoissionReconstructionMesh(tx, tn)
+    st = rec.stats()
+    # FP32 restatement: with the broken tables the reference's FP32 CG rule
+    # decides the solution (and with it the roots and the mesh)
+    ref = orc.Result(x, n, 7, precision=0)
+    assert st["q1_fired"] == 1 and ref.scalar("q1") == 1
+    for g, o in (("NODE_PARENT", "node_parent"), ("NODE_NEIGHS", "node_neighs"), ("NODE_VERTICES", "node_vertices"),
+                 ("NODE_FACES", "node_faces")):
+        np.testing.assert_array_equal(rec.export(g).reshape(-1), ref.array(o).reshape(-1))
+    assert st["roots"] == int(ref.scalar("n_roots"))
+    assert st["mesh_triangles"] == ref.count("mesh_t") // 3
+    rec.close()

@@ -180,3 +180,16 @@ def test_point_to_node_map(small_sphere_ref):
     np.testing.assert_array_equal(key[base[D] + p2n], codes)
     i = np.arange(len(codes))
     assert np.all((pidx[base[D] + p2n] <= i) & (i < pidx[base[D] + p2n] + pnum[base[D] + p2n]))
+
+
+def test_level_local_child0_quirk_detected(pkg, orc):
+    """Cloud 36 of the 64-cloud batch (a rotated torus, depth 7) leaves octant 0
+    empty while block 0 of depth 2 holds an occupied child 0: the reference's
+    parent rule (BuildOctree.cu:384-389, 412) gives that block an empty parent
+    (quirk Q1).  The oracle reports it and stays inside its arrays."""
+    x, n = pkg.synth.batch64()[36]
+    r = orc.Result(x, n, 7, stages=orc.STAGE_OCTREE, precision=0)
+    assert r.scalar("q1") == 1
+    base = r.base
+    par = r.array("node_parent")
+    assert all(par[base[2] + k] == base[1] for k in range(8))
