@@ -91,6 +91,23 @@ def cpu_baseline(xyz, nrm, depth, reps, threads):
                       "OpenMP, geometry and marching cubes run on one thread" % (reps, dt)}
 
 
+def cg_algorithmic_bytes(st, precision):
+    """HBM bytes the CG launch must move, per depth with z stored operator
+    entries (4 B value + 4 B column) and n rows, each array read once per pass.
+    FP32 (CGAlgorithm.cu:10-80): 8z + 48n per iteration (stencil; x, r, p, Ap
+    and the double-buffered p).  Mixed: 8n once (b, x), then per outer step
+    16z + 32n (the start pass: rhs, r, w, s, p; the FP64 residual: b, x, r),
+    and per inner iteration 8z + 40n (r, w, s read; s, r, w written; p and x
+    read and written)."""
+    total = 0
+    for k, z, n, o in zip(st["cg_iterations"], st["cg_nnz"], st["nodes_per_depth"], st["cg_outer"]):
+        if precision == "mixed":
+            total += 8 * n + o * (16 * z + 32 * n) + k * (8 * z + 40 * n)
+        else:
+            total += k * (8 * z + 48 * n)
+    return total
+
+
 def measure(pkg, tx, tn, depth, precision, steps, warmup):
     """Times `steps` whole reconstructions after `warmup` untimed ones; the CG
     kernel's launch time comes from the library's hipEvents on its stream."""
@@ -107,8 +124,7 @@ def measure(pkg, tx, tn, depth, precision, steps, warmup):
         rec.SolvePoissionReconstructionMesh(tx, tn)
         st = rec.stats()
         cg_ms.append(st["ms_cg_kernel"])
-        cg_bytes.append(sum(k * (8 * z + 48 * n) for k, z, n in
-                            zip(st["cg_iterations"], st["cg_nnz"], st["nodes_per_depth"])))
+        cg_bytes.append(cg_algorithmic_bytes(st, precision))
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     st = rec.stats()

@@ -84,6 +84,7 @@ typedef struct {
     float ms_cg_kernel;   /* the concurrent CG kernel alone, hipEvents on the context's stream */
     uint32_t cg_workgroups;
     int cg_nnz[PSR_STATS_MAX_DEPTH + 1]; /* stored operator entries per depth */
+    int cg_outer[PSR_STATS_MAX_DEPTH + 1]; /* PSR_PRECISION_MIXED: outer refinement steps per depth; 1 otherwise */
 } psr_stats;
 
 /* Arrays that psr_export can copy to host memory (for parity checks). */

@@ -115,9 +115,9 @@ int run(psr_ctx* ctx, const psr_points* pts, psr_mesh* out) {
     st.n_points = N;
     st.n_nodes = o.M;
     for (int d = 0; d <= o.L.D; ++d) st.nodes_per_depth[d] = o.L.count[d];
-    int iters[PSR_MAX_DEPTH + 1];
+    int iters[2 * (PSR_MAX_DEPTH + 1)];
     double resid[PSR_MAX_DEPTH + 1];
-    PSR_HIP_CHECK(hipMemcpy(iters, c.arena.get("cg_iters", 4 * (PSR_MAX_DEPTH + 1)), sizeof(iters),
+    PSR_HIP_CHECK(hipMemcpy(iters, c.arena.get("cg_iters", 8 * (PSR_MAX_DEPTH + 1)), sizeof(iters),
                             hipMemcpyDeviceToHost));
     PSR_HIP_CHECK(hipMemcpy(resid, c.arena.get("cg_resid", 8 * (PSR_MAX_DEPTH + 1)), sizeof(resid),
                             hipMemcpyDeviceToHost));
@@ -131,6 +131,7 @@ int run(psr_ctx* ctx, const psr_points* pts, psr_mesh* out) {
                             hipMemcpyDeviceToHost));
     for (int d = 0; d <= o.L.D; ++d) {
         st.cg_iterations[d] = iters[d];
+        st.cg_outer[d] = iters[PSR_MAX_DEPTH + 1 + d];
         st.cg_updated_residual[d] = resid[d];
         st.cg_true_relres[d] = c.cfg.precision == PSR_PRECISION_MIXED ? trueRel[d] : -1.0;
     }

@@ -503,7 +503,9 @@ void depth_mc(Ctx& c, MainMc& mm, float* rawV, int* rawT) {
                                                           mm.counts + 4);
     PSR_HIP_CHECK(hipMemcpyAsync(c.hostPinned, mm.counts, sizeof(int) * (PSR_MAX_DEPTH + 8), hipMemcpyDeviceToHost,
                                  s));
+    PSR_HIP_CHECK(hipMemcpyAsync(c.hostPinned + PSR_MAX_DEPTH + 8, o.q1Fired, sizeof(int), hipMemcpyDeviceToHost, s));
     PSR_HIP_CHECK(hipStreamSynchronize(s));
+    mm.q1Fired = c.hostPinned[PSR_MAX_DEPTH + 8] != 0;
     mm.nCross = c.hostPinned[0];
     mm.nTri = c.hostPinned[1];
     mm.nRoots = c.hostPinned[2];

@@ -103,6 +103,7 @@ struct MainMc {
     int* counts = nullptr;    // device: [0] crossings, [1] triangles, [2] roots, [4..] roots per depth
     int nCross = 0, nTri = 0, nRoots = 0;
     int rootsPerDepth[PSR_MAX_DEPTH + 1] = {0};
+    bool q1Fired = false;  // the level-local child 0 rule changed a parent somewhere
 };
 
 // Depth-D marching cubes into the raw mesh buffers (vertices at [0, nCross),

@@ -272,6 +272,7 @@ __global__ void __launch_bounds__(kCgThreads) cg_all_depths(CgParams P, const in
     }
     if (local == 0 && threadIdx.x == 0) {
         iters[G.depth] = k - 1;
+        iters[PSR_MAX_DEPTH + 1 + G.depth] = 1;
         resid[G.depth] = sqrt(double(rr));
     }
 }
@@ -485,6 +486,7 @@ __global__ void __launch_bounds__(kCgThreads) cg_mixed_all_depths(
     }
     if (local == 0 && threadIdx.x == 0) {
         iters[G.depth] = total;
+        iters[PSR_MAX_DEPTH + 1 + G.depth] = outer;
         resid[G.depth] = sqrt(double(gamma));
         trueRel[G.depth] = rel;
     }
@@ -636,7 +638,7 @@ void solve_depths(Ctx& c) {
     double* partial = c.buf<double>("cg_partial", size_t(2) * wg);
     GroupCtl* ctl = c.buf<GroupCtl>("cg_ctl", PSR_MAX_DEPTH + 1);
     const int ctlWords = int(sizeof(GroupCtl) / 4) * (PSR_MAX_DEPTH + 1);
-    int* iters = c.buf<int>("cg_iters", PSR_MAX_DEPTH + 1);
+    int* iters = c.buf<int>("cg_iters", 2 * (PSR_MAX_DEPTH + 1));  // [inner iterations | outer steps] per depth
     double* resid = c.buf<double>("cg_resid", PSR_MAX_DEPTH + 1);
     double* trueRel = c.buf<double>("cg_truerel", PSR_MAX_DEPTH + 1);
     int* err = c.buf<int>("cg_err", 1);

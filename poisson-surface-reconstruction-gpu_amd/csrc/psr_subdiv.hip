@@ -39,6 +39,7 @@ struct SetDesc {
     int nr;          // roots in the set
     int rootOffset;  // first root in the root list
     int dEnum;       // first depth-D cell enumeration index of the set (chunk-local)
+    int firstTile;   // tile holding the set's first cell (tiles are in enumeration order)
 };
 
 struct Grid {
@@ -54,7 +55,12 @@ struct Grid {
     const float* dx;
     const float4* nbrBasis;  // [node][27] basis parameters of the node's neighbours (nodes above depth D-1)
     const float* nbrDx;      // [node][27] their d_x (0 where absent)
+    const int* rootNbr;      // [root][27] position in roots of the root's real neighbour, or -1
+    const int* rootChain;    // [root][kChain] the root, then its ancestors (-1 past the tree root)
+    const int4* rootCell;    // [root] cell at the root depth (x, y, z) and the root node (w)
+    const float* dxAt;       // per depth l, at (8^l - 1) / 7: d_x of the node at each cell, x-major; 0 where none
 };
+constexpr int kChain = PSR_MAX_DEPTH + 1;
 
 // Bourke corner order: slot -> (x, y, z) offset (BuildMeshGeometry.cu:133-144).
 __device__ __forceinline__ int3 bourke_offset(int b) {
@@ -99,6 +105,7 @@ struct Leaf {
 // lies in one root.
 struct Tile {
     int set, start, count;
+    int root;  // position in roots of the tile's first root
 };
 constexpr int kTile = 256;
 
@@ -316,6 +323,37 @@ __device__ __forceinline__ void fill_roots(const Grid& g, const Tile& T, const S
     }
 }
 
+// fill_roots from the root records.
+__device__ __forceinline__ void fill_roots_rec(const Grid& g, const Tile& T, const SetDesc& S, const TileGeom& tg,
+                                               TileRoot* tr) {
+    const int sh = g.D - S.rd;
+    for (int rr = threadIdx.x; rr < tg.nRoots; rr += blockDim.x) {
+        const int q = tg.q0 + rr;
+        const int4 rc = g.rootCell[T.root + rr];
+        const int3 Rc = make_int3(rc.x, rc.y, rc.z);
+        int3 first = make_int3(Rc.x << sh, Rc.y << sh, Rc.z << sh);
+        if (tg.oneRoot) {
+            const int3 lc = demorton(T.start - S.dEnum - (q << (3 * sh)), sh);
+            first = make_int3(first.x + lc.x, first.y + lc.y, first.z + lc.z);
+        }
+        tr[rr] = TileRoot{rc.w, q, Rc, first};
+    }
+}
+
+// Leaf of cell e, which lies in tile root R.
+__device__ __forceinline__ Leaf leaf_of(const Grid& g, int si, const SetDesc& S, const TileRoot& R, int e) {
+    Leaf L;
+    L.si = si;
+    L.S = S;
+    L.sh = g.D - S.rd;
+    L.q = R.q;
+    L.root = R.root;
+    L.Rc = R.Rc;
+    const int3 lc = demorton(e - S.dEnum - (R.q << (3 * L.sh)), L.sh);
+    L.c = make_int3((R.Rc.x << L.sh) + lc.x, (R.Rc.y << L.sh) + lc.y, (R.Rc.z << L.sh) + lc.z);
+    return L;
+}
+
 __device__ __forceinline__ Leaf root_leaf(const Grid& g, const SetDesc& S, int si, const TileRoot& R) {
     Leaf F;
     F.S = S;
@@ -339,6 +377,36 @@ __device__ __forceinline__ void fill_vbox(const Grid& g, const Tile& T, const Se
         const Leaf F = root_leaf(g, S, T.set, tr[rr]);
         int dz = u % bz, dy = (u / bz) % by, dx = u / (bz * by);
         vbox[t] = virtual_enum(g, F, make_int3(F.c.x - 1 + dx, F.c.y - 1 + dy, F.c.z - 1 + dz));
+    }
+}
+
+// fill_vbox with the roots' neighbour positions in LDS (nbr: [root][27]); the
+// same test as virtual_enum.
+__device__ __forceinline__ void fill_vbox_rec(const Grid& g, const Tile& T, const SetDesc& S, const TileGeom& tg,
+                                              const TileRoot* tr, const int* nbr, int* vbox) {
+    const int by = tg.ext.y + 2, bz = tg.ext.z + 2;
+    const int sh = g.D - S.rd, lim = 1 << g.D;
+    for (int t = threadIdx.x; t < tg.nRoots * tg.vboxPerRoot; t += blockDim.x) {
+        const int rr = t / tg.vboxPerRoot, u = t - rr * tg.vboxPerRoot;
+        const TileRoot R = tr[rr];
+        const int dz = u % bz, dy = (u / bz) % by, dx = u / (bz * by);
+        const int3 n = make_int3(R.first.x - 1 + dx, R.first.y - 1 + dy, R.first.z - 1 + dz);
+        int v = -1;
+        if (n.x >= 0 && n.y >= 0 && n.z >= 0 && n.x < lim && n.y < lim && n.z < lim) {
+            const int3 A = make_int3(n.x >> sh, n.y >> sh, n.z >> sh);
+            int q = R.q;
+            bool ok = true;
+            if (A.x != R.Rc.x || A.y != R.Rc.y || A.z != R.Rc.z) {
+                const int rp = nbr[rr * 27 + nbr_index(A.x - R.Rc.x, A.y - R.Rc.y, A.z - R.Rc.z)];
+                q = rp - S.rootOffset;
+                ok = rp >= S.rootOffset && rp < S.rootOffset + S.nr;
+            }
+            if (ok) {
+                const int3 lc = make_int3(n.x - (A.x << sh), n.y - (A.y << sh), n.z - (A.z << sh));
+                v = S.dEnum + (q << (3 * sh)) + morton(lc, sh);
+            }
+        }
+        vbox[t] = v;
     }
 }
 
@@ -610,110 +678,434 @@ __global__ void __launch_bounds__(NT) sub_corner_values(Grid g, const Tile* __re
     }
 }
 
-// Corner values of every cell, cube class, triangle count, and the flags of
-// the edges it owns that cross the surface.
-__global__ void __launch_bounds__(kThreads) sub_cells(Grid g, const Tile* __restrict__ tiles, int nVD,
-                                                      const float* __restrict__ cval, float* __restrict__ vals,
-                                                      int* __restrict__ cube, int* __restrict__ triNum,
-                                                      int* __restrict__ cflag) {
+// Quotient by a small divisor through a multiply: exact for u * d < 2^32.
+struct SmallDiv {
+    unsigned m;
+    int d;
+    __device__ __forceinline__ explicit SmallDiv(int dd) : m(0xFFFFFFFFu / unsigned(dd) + 1u), d(dd) {}
+    __device__ __forceinline__ int div(int u) const { return int(__umulhi(unsigned(u), m)); }
+};
+
+// One-root tiles when every parent is the true one (the level-local child 0
+// rule changed none).  The octree is then geometric: the real node at depth l
+// and cell n, if any, is the one the halo descent from the root's neighbour
+// reaches, and its basis centre is (n + 0.5) / 2^l with inverse width 2^l.  So
+// the halo is read as d_x from the dense per-depth grid, and the B-spline
+// factors of every halo node and every list node at the tile's corner
+// coordinates come from tables in LDS.  The list tables are stored per corner
+// coordinate with the list index fastest, d_x folded into the x factor (the
+// product d_x F(x) is the first one the term forms), so four terms are read
+// with one 16-byte load per axis.  Terms, their order and their association
+// are those of sub_corner_values.
+constexpr int kLdsWordsGrid = 7168;  // upper bound of grid_tile_words for depths up to 9
+constexpr int kHaloPerThread = 4;     // halo entries per thread (<= 1024 for depths up to 9)
+__global__ void __launch_bounds__(kThreads) sub_corner_values_grid(Grid g, const Tile* __restrict__ tiles, int nVD,
+                                                                   const float* __restrict__ iso,
+                                                                   float* __restrict__ cval) {
+    extern __shared__ float4 ldsDyn[];  // grid_tile_words(D, rd) words
+    __shared__ int lNbr[27];
+    __shared__ int lChain[kChain];
+    __shared__ TileRoot lRoot[1];
+    float* lds = reinterpret_cast<float*>(ldsDyn);
+    const Tile T = tiles[blockIdx.x];
+    if (threadIdx.x < 27) lNbr[threadIdx.x] = g.rootNbr[27 * size_t(T.root) + threadIdx.x];
+    if (threadIdx.x < kChain) lChain[threadIdx.x] = g.rootChain[kChain * size_t(T.root) + threadIdx.x];
+    const SetDesc S = g.sets[T.set];
+    const int D = g.D;
+    const TileGeom tg = tile_geom(T, S, D);
+    const int K = tg.K, Kp = (K + 3) & ~3;
+    // list x (with d_x) [kTx][Kp] | list y [kTy][Kp] | list z [kTz][Kp] | vbox | halo d_x | halo factors
+    float* lLx = lds;
+    float* lLy = lLx + kTx * Kp;
+    float* lLz = lLy + kTy * Kp;
+    int* lVbox = reinterpret_cast<int*>(lLz + kTz * Kp);
+    float* lHdx = reinterpret_cast<float*>(lVbox + tg.vboxPerRoot);
+    float* lFac = lHdx + tg.haloPerRoot;
+    int facWords = 0;
+    for (int l = D; l > S.rd; --l) {
+        const int3 x = box_ext(tg.ext, D, l);
+        facWords += (x.x + 2) * kTx + (x.y + 2) * kTy + (x.z + 2) * kTz;
+    }
+    fill_roots_rec(g, T, S, tg, lRoot);
+    __syncthreads();
+    const TileRoot R = lRoot[0];
+    const int3 lo = R.first;
+    const float inv = 1.0f / float(1 << D);
+    // the root list: the 27 neighbours of the root and of each ancestor; thread k
+    // loads list node k and writes its factors at the tile's corner coordinates
+    if (int(threadIdx.x) < Kp) {
+        const int k = threadIdx.x;
+        const int node = k < K ? lChain[k / 27] : -1;
+        float4 b = make_float4(0.f, 0.f, 0.f, 0.f);
+        float d = 0.f;
+        if (node >= 0) {
+            b = g.nbrBasis[27 * size_t(node) + k % 27];
+            d = g.nbrDx[27 * size_t(node) + k % 27];
+        }
+        for (int cc = 0; cc < kTx; ++cc)
+            lLx[cc * Kp + k] = node < 0 ? 0.f : d * bspline((float(lo.x + cc) * inv - b.x) * b.w);
+        for (int cc = 0; cc < kTy; ++cc)
+            lLy[cc * Kp + k] = node < 0 ? 0.f : bspline((float(lo.y + cc) * inv - b.y) * b.w);
+        for (int cc = 0; cc < kTz; ++cc)
+            lLz[cc * Kp + k] = node < 0 ? 0.f : bspline((float(lo.z + cc) * inv - b.z) * b.w);
+    }
+    // halo d_x, depth D first: 0 inside the roots of the set and where no node is;
+    // a thread's loads are issued together
+#pragma unroll
+    for (int j = 0; j < kHaloPerThread; ++j) {
+        const int t = threadIdx.x + j * kThreads;
+        if (t >= tg.haloPerRoot) break;
+        int u = t, l = D;
+        int3 x = box_ext(tg.ext, D, l);
+        while (u >= (x.x + 2) * (x.y + 2) * (x.z + 2)) {
+            u -= (x.x + 2) * (x.y + 2) * (x.z + 2);
+            --l;
+            x = box_ext(tg.ext, D, l);
+        }
+        const SmallDiv byz((x.y + 2) * (x.z + 2)), bz(x.z + 2);
+        const int dx = byz.div(u), r = u - dx * byz.d, dy = bz.div(r), dz = r - dy * bz.d;
+        const int3 n = make_int3((lo.x >> (D - l)) - 1 + dx, (lo.y >> (D - l)) - 1 + dy, (lo.z >> (D - l)) - 1 + dz);
+        const int lim = 1 << l, s = l - S.rd;
+        float v = 0.f;
+        if (n.x >= 0 && n.y >= 0 && n.z >= 0 && n.x < lim && n.y < lim && n.z < lim) {
+            const int3 A = make_int3(n.x >> s, n.y >> s, n.z >> s);
+            if (A.x != R.Rc.x || A.y != R.Rc.y || A.z != R.Rc.z) {
+                const int rp = lNbr[nbr_index(A.x - R.Rc.x, A.y - R.Rc.y, A.z - R.Rc.z)];
+                if (!(rp >= S.rootOffset && rp < S.rootOffset + S.nr)) {
+                    const size_t cell = (size_t(n.x) << (2 * l)) | (size_t(n.y) << l) | size_t(n.z);
+                    v = g.dxAt[((size_t(1) << (3 * l)) - 1) / 7 + cell];
+                }
+            }
+        }
+        lHdx[t] = v;
+    }
+    fill_vbox_rec(g, T, S, tg, lRoot, lNbr, lVbox);
+    // halo factor tables: per depth, node cell (box-relative) x corner coordinate
+    for (int t = threadIdx.x; t < facWords; t += kThreads) {
+        int u = t, l = D;
+        int3 x = box_ext(tg.ext, D, l);
+        int w = (x.x + 2) * kTx + (x.y + 2) * kTy + (x.z + 2) * kTz;
+        while (u >= w) {
+            u -= w;
+            --l;
+            x = box_ext(tg.ext, D, l);
+            w = (x.x + 2) * kTx + (x.y + 2) * kTy + (x.z + 2) * kTz;
+        }
+        const float invw = float(1 << l);
+        int cell, corner;
+        if (u < (x.x + 2) * kTx) {
+            cell = (lo.x >> (D - l)) - 1 + u / kTx;
+            corner = lo.x + u % kTx;
+        } else if ((u -= (x.x + 2) * kTx) < (x.y + 2) * kTy) {
+            cell = (lo.y >> (D - l)) - 1 + u / kTy;
+            corner = lo.y + u % kTy;
+        } else {
+            u -= (x.y + 2) * kTy;
+            cell = (lo.z >> (D - l)) - 1 + u / kTz;
+            corner = lo.z + u % kTz;
+        }
+        const float c = (0.5f + float(cell)) / invw;
+        lFac[t] = bspline((float(corner) * inv - c) * invw);
+    }
+    __syncthreads();
+    if (int(threadIdx.x) >= T.count) return;
+    const int e = T.start + threadIdx.x;
+    const Leaf L = leaf_of(g, T.set, S, R, e);
+    const BoxVE ve = box_ve(lVbox, tg, 0, lo);
+    const float isoV = *iso;
+    const int sh = D - S.rd, side = 1 << sh;
+    const int3 lc = make_int3(L.c.x - (R.Rc.x << sh), L.c.y - (R.Rc.y << sh), L.c.z - (R.Rc.z << sh));
+    for (int b = 0; b < 8; ++b) {
+        const int3 o = bourke_offset(b);
+        // a corner all of whose cells lie in the root's cube is owned by its lowest cell
+        const bool inner = lc.x + o.x > 0 && lc.x + o.x < side && lc.y + o.y > 0 && lc.y + o.y < side &&
+                           lc.z + o.z > 0 && lc.z + o.z < side;
+        if (inner) {
+            if (o.x + o.y + o.z != 3) continue;
+        } else {
+            int slot;
+            if (corner_owner(L, D, b, slot, ve) != e) continue;
+        }
+        const int3 pt = make_int3(L.c.x + o.x, L.c.y + o.y, L.c.z + o.z);
+        const int px = pt.x - lo.x, py = pt.y - lo.y, pz = pt.z - lo.z;
+        float val = 0.f;
+        int hb = 0, fb = 0;
+        for (int l = D; l > S.rd; --l) {
+            const int s = l - S.rd;
+            const int3 x = box_ext(tg.ext, D, l);
+            const int bx = x.x + 2, by = x.y + 2, bz = x.z + 2;
+            const int3 cl = make_int3(L.c.x >> (D - l), L.c.y >> (D - l), L.c.z >> (D - l));
+            const bool interior = ((cl.x - 1) >> s) == L.Rc.x && ((cl.x + 1) >> s) == L.Rc.x &&
+                                  ((cl.y - 1) >> s) == L.Rc.y && ((cl.y + 1) >> s) == L.Rc.y &&
+                                  ((cl.z - 1) >> s) == L.Rc.z && ((cl.z + 1) >> s) == L.Rc.z;
+            if (!interior) {
+                const int3 bl = make_int3((lo.x >> (D - l)) - 1, (lo.y >> (D - l)) - 1, (lo.z >> (D - l)) - 1);
+                const float* fx = lFac + fb;
+                const float* fy = fx + bx * kTx;
+                const float* fz = fy + by * kTy;
+                const float* hd = lHdx + hb;
+#pragma unroll
+                for (int i = 0; i < 27; ++i) {
+                    const int nx = cl.x + i / 9 - 1 - bl.x, ny = cl.y + (i / 3) % 3 - 1 - bl.y,
+                              nz = cl.z + i % 3 - 1 - bl.z;
+                    val += hd[(nx * by + ny) * bz + nz] * fx[nx * kTx + px] * fy[ny * kTy + py] * fz[nz * kTz + pz];
+                }
+            }
+            hb += bx * by * bz;
+            fb += bx * kTx + by * kTy + bz * kTz;
+        }
+        const float4* X4 = reinterpret_cast<const float4*>(lLx + px * Kp);
+        const float4* Y4 = reinterpret_cast<const float4*>(lLy + py * Kp);
+        const float4* Z4 = reinterpret_cast<const float4*>(lLz + pz * Kp);
+        for (int k4 = 0; k4 < Kp / 4; ++k4) {
+            const float4 a = X4[k4], yb = Y4[k4], zc = Z4[k4];
+            val += a.x * yb.x * zc.x;
+            val += a.y * yb.y * zc.y;
+            val += a.z * yb.z * zc.z;
+            val += a.w * yb.w * zc.w;
+        }
+        cval[size_t(b) * nVD + e] = val - isoV;
+    }
+}
+
+// Exclusive prefix of v over the block's threads and the block total (every
+// thread of the block must call it).
+template <int NT>
+__device__ __forceinline__ int block_exclusive(int v, int* sh, int& total) {
+    constexpr int kW = NT / PSR_WAVE;
+    const int lane = __lane_id(), w = threadIdx.x / PSR_WAVE;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < PSR_WAVE; o <<= 1) {
+        int y = __shfl_up(x, o, PSR_WAVE);
+        if (lane >= o) x += y;
+    }
+    if (lane == PSR_WAVE - 1) sh[w] = x;
+    __syncthreads();
+    int before = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < kW; ++k) {
+        int t = sh[k];
+        before += k < w ? t : 0;
+        tot += t;
+    }
+    __syncthreads();
+    total = tot;
+    return before + x - v;
+}
+
+// Crossing point of edge k of the depth-D cell c with corner values v
+// (generateIntersectionPoint :440-474).
+__device__ __forceinline__ float3 cell_crossing(int3 c, int k, const float* v, float inv) {
+    const int b1 = kEdgeVertex[k][0], b2 = kEdgeVertex[k][1];
+    const int3 o1 = bourke_offset(b1), o2 = bourke_offset(b2);
+    const float3 p1 = make_float3(float(c.x + o1.x) * inv, float(c.y + o1.y) * inv, float(c.z + o1.z) * inv);
+    const float3 p2 = make_float3(float(c.x + o2.x) * inv, float(c.y + o2.y) * inv, float(c.z + o2.z) * inv);
+    return interpolate_point(p1, p2, k >> 2, v[b1], v[b2]);
+}
+
+// Cell word: bits 0-11 the owned edges that cross the surface, 12-23 those
+// whose vertex is kept (markValidMeshVertexIndex :1170-1181 drops |x| < 1e-6),
+// 24-31 the cube class.
+constexpr unsigned kEdgeBits = 0xFFFu;
+
+// Per cell: corner values, cube class, owned crossing edges and kept vertices;
+// per tile: crossing, kept-vertex and triangle counts, and each cell's first
+// crossing within the tile.  Corner values of crossing cells are kept
+// for the emission pass.
+__global__ void __launch_bounds__(kThreads) sub_classify(Grid g, const Tile* __restrict__ tiles, int nVD,
+                                                         const float* __restrict__ cval, float* __restrict__ vals,
+                                                         unsigned* __restrict__ cellWord,
+                                                         int* __restrict__ crossPrefix, int4* __restrict__ tileCnt) {
     __shared__ int lVbox[kVboxWords];
     __shared__ TileRoot lRoots[kMaxRootsPerTile];
+    __shared__ int lNbr[kMaxRootsPerTile * 27];
+    __shared__ int lSum[kThreads / PSR_WAVE];
     const Tile T = tiles[blockIdx.x];
     const SetDesc S = g.sets[T.set];
-    const int D = g.D, sh = D - S.rd;
+    const int D = g.D;
     const TileGeom tg = tile_geom(T, S, D);
-    fill_roots(g, T, S, tg, lRoots);
+    fill_roots_rec(g, T, S, tg, lRoots);
+    for (int t = threadIdx.x; t < tg.nRoots * 27; t += kThreads) lNbr[t] = g.rootNbr[27 * size_t(T.root) + t];
     __syncthreads();
-    fill_vbox(g, T, S, tg, lRoots, lVbox);
+    fill_vbox_rec(g, T, S, tg, lRoots, lNbr, lVbox);
     __syncthreads();
-    if (int(threadIdx.x) >= T.count) return;
-    const int e = T.start + threadIdx.x;
-    Leaf L = make_leaf_in(g, T.set, S, e);
-    const int3 rlo = tg.oneRoot ? lRoots[0].first : make_int3(L.Rc.x << sh, L.Rc.y << sh, L.Rc.z << sh);
-    const BoxVE ve = box_ve(lVbox, tg, L.q - tg.q0, rlo);
-    float v[8];
-    int c = 0;
-    for (int b = 0; b < 8; ++b) {
-        int slot;
-        int owner = corner_owner(L, D, b, slot, ve);
-        v[b] = cval[size_t(slot) * nVD + owner];
-        vals[size_t(b) * nVD + e] = v[b];
-        if (v[b] < 0.f) c |= 1 << b;
+    const bool active = int(threadIdx.x) < T.count;
+    const int e = T.start + (active ? int(threadIdx.x) : 0);
+    unsigned mask = 0, keep = 0;
+    int nt = 0;
+    if (active) {
+        const int rr = tg.oneRoot ? 0 : int(threadIdx.x) >> (3 * (D - S.rd));
+        const Leaf L = leaf_of(g, T.set, S, lRoots[rr], e);
+        const BoxVE ve = box_ve(lVbox, tg, rr, lRoots[rr].first);
+        float v[8];
+        int cls = 0;
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            int slot;
+            const int owner = corner_owner(L, D, b, slot, ve);
+            v[b] = cval[size_t(slot) * nVD + owner];
+            if (v[b] < 0.f) cls |= 1 << b;
+        }
+        const float inv = 1.0f / float(1 << D);
+        for (int k = 0; k < 12; ++k) {
+            if (!(v[kEdgeVertex[k][0]] * v[kEdgeVertex[k][1]] <= 0.f)) continue;
+            int kind;
+            if (edge_owner(L, D, k, kind, ve) != e || kind != k) continue;
+            mask |= 1u << k;
+            if (!(fabsf(cell_crossing(L.c, k, v, inv).x) < 1e-6f)) keep |= 1u << k;
+        }
+        cellWord[e] = mask | (keep << 12) | (unsigned(cls) << 24);
+        if (mask)
+#pragma unroll
+            for (int b = 0; b < 8; ++b) vals[size_t(b) * nVD + e] = v[b];
+        nt = mc_tri_count(cls);
     }
-    cube[e] = c;
-    triNum[e] = mc_tri_count(c);
-    for (int k = 0; k < 12; ++k) {
-        int kind;
-        bool own = edge_owner(L, D, k, kind, ve) == e && kind == k;
-        cflag[12 * size_t(e) + k] = (own && v[kEdgeVertex[k][0]] * v[kEdgeVertex[k][1]] <= 0.f) ? 1 : 0;
+    int tc, tk, tt;
+    const int pre = block_exclusive<kThreads>(__popc(mask), lSum, tc);
+    if (active) crossPrefix[e] = pre;
+    (void)block_exclusive<kThreads>(__popc(keep), lSum, tk);
+    (void)block_exclusive<kThreads>(nt, lSum, tt);
+    if (threadIdx.x == 0) tileCnt[blockIdx.x] = make_int4(tc, tk, tt, 0);
+}
+
+// Exclusive scan of the per-tile counts in one workgroup; totals[0..2] are the
+// chunk's crossings, kept vertices and triangles.
+constexpr int kScanThreads = 1024;
+__global__ void __launch_bounds__(kScanThreads) sub_tile_scan(const int4* __restrict__ cnt, int n,
+                                                              int4* __restrict__ base, int* __restrict__ totals) {
+    __shared__ int lSum[kScanThreads / PSR_WAVE];
+    const int per = (n + kScanThreads - 1) / kScanThreads;
+    const int i0 = min(n, int(threadIdx.x) * per), i1 = min(n, i0 + per);
+    int a = 0, b = 0, c = 0;
+    for (int i = i0; i < i1; ++i) {
+        const int4 t = cnt[i];
+        a += t.x;
+        b += t.y;
+        c += t.z;
+    }
+    int ta, tb, tc;
+    a = block_exclusive<kScanThreads>(a, lSum, ta);
+    b = block_exclusive<kScanThreads>(b, lSum, tb);
+    c = block_exclusive<kScanThreads>(c, lSum, tc);
+    for (int i = i0; i < i1; ++i) {
+        const int4 t = cnt[i];
+        base[i] = make_int4(a, b, c, 0);
+        a += t.x;
+        b += t.y;
+        c += t.z;
+    }
+    if (threadIdx.x == 0) {
+        totals[0] = ta;
+        totals[1] = tb;
+        totals[2] = tc;
     }
 }
 
-// Crossing points of the owned crossing edges (generateIntersectionPoint :440-474).
-__global__ void __launch_bounds__(kThreads) sub_crossings(Grid g, const Tile* __restrict__ tiles, int nVD,
-                                                          const float* __restrict__ vals,
-                                                          const int* __restrict__ cflag, const int* __restrict__ cidx,
-                                                          float* __restrict__ out) {
+// Kept crossing points and the triangles of the subdivision batches.  A
+// triangle index is the number of vertices kept before the batch plus the raw
+// offset inside the batch; indices are not remapped around dropped vertices
+// (insertTriangle :1247-1282).  keptBefore / triBefore: vertices kept and
+// triangles written before this chunk.
+__global__ void __launch_bounds__(kThreads) sub_emit(Grid g, const Tile* __restrict__ tiles, int nVD,
+                                                     const float* __restrict__ vals,
+                                                     const unsigned* __restrict__ cellWord,
+                                                     const int* __restrict__ crossPrefix,
+                                                     const int4* __restrict__ tileCnt,
+                                                     const int4* __restrict__ tileBase, int keptBefore,
+                                                     int triBefore, float* __restrict__ outV, int* __restrict__ outT) {
+    __shared__ int lSum[kThreads / PSR_WAVE];
+    const int4 cnt = tileCnt[blockIdx.x];
+    if (cnt.y == 0 && cnt.z == 0) return;  // the same for the whole block
     const Tile T = tiles[blockIdx.x];
-    if (int(threadIdx.x) >= T.count) return;
-    const int e = T.start + threadIdx.x;
-    int any = 0;
-    for (int k = 0; k < 12; ++k) any |= cflag[12 * size_t(e) + k];
-    if (!any) return;
-    Leaf L = make_leaf_in(g, T.set, g.sets[T.set], e);
-    const float inv = 1.0f / float(1 << g.D);
-    for (int k = 0; k < 12; ++k) {
-        if (!cflag[12 * size_t(e) + k]) continue;
-        int b1 = kEdgeVertex[k][0], b2 = kEdgeVertex[k][1];
-        int3 o1 = bourke_offset(b1), o2 = bourke_offset(b2);
-        float3 p1 = make_float3(float(L.c.x + o1.x) * inv, float(L.c.y + o1.y) * inv, float(L.c.z + o1.z) * inv);
-        float3 p2 = make_float3(float(L.c.x + o2.x) * inv, float(L.c.y + o2.y) * inv, float(L.c.z + o2.z) * inv);
-        float3 p = interpolate_point(p1, p2, k >> 2, vals[size_t(b1) * nVD + e], vals[size_t(b2) * nVD + e]);
-        size_t a = cidx[12 * size_t(e) + k];
-        out[3 * a] = p.x;
-        out[3 * a + 1] = p.y;
-        out[3 * a + 2] = p.z;
+    const bool active = int(threadIdx.x) < T.count;
+    const int e = T.start + (active ? int(threadIdx.x) : 0);
+    const unsigned w = active ? cellWord[e] : 0u;
+    const unsigned keep = (w >> 12) & kEdgeBits;
+    const int nt = mc_tri_count(int(w >> 24));
+    int tk, tt;
+    const int kp = block_exclusive<kThreads>(__popc(keep), lSum, tk);
+    const int tp = block_exclusive<kThreads>(nt, lSum, tt);
+    if (!active || (keep == 0 && nt == 0)) return;
+    const SetDesc S = g.sets[T.set];
+    const Leaf L = make_leaf_in(g, T.set, S, e);
+    const int4 tb = tileBase[blockIdx.x];
+    if (keep) {
+        float v[8];
+#pragma unroll
+        for (int b = 0; b < 8; ++b) v[b] = vals[size_t(b) * nVD + e];
+        const float inv = 1.0f / float(1 << g.D);
+        size_t a = size_t(keptBefore) + tb.y + kp;
+        for (int k = 0; k < 12; ++k) {
+            if (!((keep >> k) & 1u)) continue;
+            const float3 p = cell_crossing(L.c, k, v, inv);
+            outV[3 * a] = p.x;
+            outV[3 * a + 1] = p.y;
+            outV[3 * a + 2] = p.z;
+            ++a;
+        }
+    }
+    if (nt) {
+        const int4 sb = tileBase[S.firstTile];  // the batch's first raw vertex and kept count before it
+        const int batchKept = keptBefore + sb.y;
+        const uint64_t mw = mc_word(int(w >> 24));
+        const size_t base = size_t(triBefore) + tb.z + tp;
+        for (int k = 0; k < 3 * nt; ++k) {
+            const int ed = int((mw >> (4 * k)) & 0xF);
+            int kind;
+            const int owner = edge_owner(L, g.D, ed, kind, [&](int3 n) { return virtual_enum(g, L, n); });
+            const unsigned ow = cellWord[owner];
+            const int raw = tileBase[S.firstTile + (owner - S.dEnum) / kTile].x + crossPrefix[owner] +
+                            __popc(ow & kEdgeBits & ((1u << kind) - 1u));
+            outT[3 * base + k] = batchKept + (raw - sb.x);
+        }
     }
 }
 
-// First crossing of each set: the crossing scan value at the set's first edge flag.
-__global__ void set_cross_start(int nSets, const SetDesc* __restrict__ sets, const int* __restrict__ cidx,
-                                size_t nflag, int nCross, int* __restrict__ crossStart) {
-    int si = blockIdx.x * blockDim.x + threadIdx.x;
-    if (si >= nSets) return;
-    size_t f = 12 * size_t(sets[si].dEnum);
-    crossStart[si] = f < nflag ? cidx[f] : nCross;
+// Per root: the position in roots of each real neighbour (or -1), and the
+// chain of the root and its ancestors.
+__global__ void __launch_bounds__(kThreads) root_records(int nRoots, int M, Levels Lv, const int* __restrict__ roots,
+                                                         const int* __restrict__ rootPos,
+                                                         const int* __restrict__ neigh,
+                                                         const int* __restrict__ parent,
+                                                         const int* __restrict__ key, int* __restrict__ rootNbr,
+                                                         int* __restrict__ rootChain, int4* __restrict__ rootCell) {
+    const size_t t = size_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (t >= size_t(27) * nRoots) return;
+    const int q = int(t / 27), i = int(t - size_t(q) * 27);
+    const int root = roots[q];
+    const int nb = neigh[i * M + root];
+    rootNbr[t] = nb < 0 ? -1 : rootPos[nb];
+    if (i == 1) {
+        int rd = 0;
+        while (rd < Lv.D && root >= Lv.base[rd + 1]) ++rd;
+        const int3 c = demorton(key[root] >> (3 * (Lv.D - rd)), rd);
+        rootCell[q] = make_int4(c.x, c.y, c.z, root);
+    }
+    if (i == 0) {
+        int node = root;
+        for (int u = 0; u < kChain; ++u) {
+            rootChain[kChain * size_t(q) + u] = node;
+            node = node < 0 ? -1 : parent[node];
+        }
+    }
 }
 
-// Triangles of the subdivision batches (insertTriangle :1247-1282): an index is
-// the number of vertices kept before the batch plus the raw offset inside the
-// batch; indices are not remapped around dropped vertices.
-__global__ void __launch_bounds__(kThreads) sub_triangles(Grid g, const Tile* __restrict__ tiles,
-                                                          const int* __restrict__ cube,
-                                                          const int* __restrict__ triAddr,
-                                                          const int* __restrict__ cidx,
-                                                          const int* __restrict__ crossStart,
-                                                          const int* __restrict__ localKept,
-                                                          const int* __restrict__ keptBase, int triBase,
-                                                          int* __restrict__ tri) {
-    const Tile T = tiles[blockIdx.x];
-    if (int(threadIdx.x) >= T.count) return;
-    const int e = T.start + threadIdx.x;
-    int c = cube[e];
-    int nt = mc_tri_count(c);
-    if (!nt) return;
-    Leaf L = make_leaf_in(g, T.set, g.sets[T.set], e);
-    int bs = crossStart[L.si];  // first raw vertex of this batch within the chunk
-    int keptBefore = *keptBase + localKept[bs];
-    uint64_t w = mc_word(c);
-    size_t base = size_t(triBase) + triAddr[e];
-    for (int k = 0; k < 3 * nt; ++k) {
-        int ed = int((w >> (4 * k)) & 0xF);
-        int kind;
-        int owner = edge_owner(L, g.D, ed, kind, [&](int3 n) { return virtual_enum(g, L, n); });
-        int raw = cidx[12 * size_t(owner) + kind];
-        tri[3 * base + k] = keptBefore + (raw - bs);
-    }
+// d_x of every node at its cell of the dense grid of its depth.
+__global__ void __launch_bounds__(kThreads) scatter_dx_grid(int M, Levels L, const int* __restrict__ key,
+                                                            const float* __restrict__ dx, float* __restrict__ grid) {
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= M) return;
+    int l = 0;
+    while (l < L.D && i >= L.base[l + 1]) ++l;
+    const int3 c = demorton(key[i] >> (3 * (L.D - l)), l);
+    const size_t cell = (size_t(c.x) << (2 * l)) | (size_t(c.y) << l) | size_t(c.z);
+    grid[((size_t(1) << (3 * l)) - 1) / 7 + cell] = dx[i];
+}
+
+__global__ void __launch_bounds__(kThreads) zero_floats(float* __restrict__ a, size_t n) {
+    const size_t i = size_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (i < n) a[i] = 0.f;
 }
 
 // Basis parameters and d_x of the 27 neighbours of every node above depth D-1
@@ -764,6 +1156,19 @@ __global__ void __launch_bounds__(kThreads) fill_int(int* __restrict__ a, size_t
     if (i < n) a[i] = v;
 }
 
+// LDS words of a sub_corner_values_grid tile whose root is at depth rd, and
+// its halo entries.
+int grid_tile_words(int D, int rd, int* haloOut = nullptr) {
+    int halo = 0, fac = 0;
+    for (int l = D; l > rd; --l) {
+        const int x = std::max(1, 4 >> (D - l)) + 2, y = std::max(1, 8 >> (D - l)) + 2, z = y;
+        halo += x * y * z;
+        fac += x * kTx + y * kTy + z * kTz;
+    }
+    if (haloOut) *haloOut = halo;
+    return 6 * 10 * 10 + halo + fac + ((27 * (rd + 1) + 3) & ~3) * kTw;
+}
+
 // Default depth-D virtual cells per chunk of subdivision sets; a set larger
 // than the chunk size forms a chunk of its own.
 constexpr size_t kChunkCells = size_t(1) << 24;
@@ -776,17 +1181,17 @@ void extract_mesh(Ctx& c) {
     const Levels& L = o.L;
     const int M = o.M, D = L.D, nD = L.count[D], offD = L.base[D];
     hipStream_t st = c.stream;
-    // ---- depth-D marching cubes: the first output batch
+    // ---- depth-D marching cubes: the first output batch; its triangles are
+    // written in place (raw crossing ids: batch start 0, nothing kept before it)
     float* mainV = c.buf<float>("raw_main_v", size_t(3) * std::max(geo.E, 1));
-    int* mainT = c.buf<int>("raw_main_t", size_t(15) * nD);
+    int* outT = c.buf_keep<int>("mesh_triangles", size_t(15) * nD, 0);
     MainMc mm;
-    depth_mc(c, mm, mainV, mainT);
+    depth_mc(c, mm, mainV, outT);
     c.stats.n_subdivision_roots = mm.nRoots;
     int* acc = c.buf<int>("mesh_counts", 4);  // [0] vertices kept so far, [1] batch kept count
     fill_int<<<1, kThreads, 0, st>>>(acc, 4, 0);
-    size_t rawUpper = 0, triTotal = 0;
+    size_t rawUpper = 0, triTotal = 0, keptTotal = 0;
     float* outV = nullptr;
-    int* outT = nullptr;
     // Appends one batch of raw vertices: keep flags, local scan, compaction at
     // the running kept count.  localKept holds n + 1 entries afterwards.
     auto append_vertices = [&](const float* rawV, int n, int* keep, int* localKept, int* scratch) {
@@ -802,9 +1207,6 @@ void extract_mesh(Ctx& c) {
         int* scratch = c.scratch_for(scan_scratch_ints(size_t(mm.nCross) + 1) + 64);
         append_vertices(mainV, mm.nCross, keep, kept, scratch);
         add_count<<<1, 1, 0, st>>>(acc, acc + 1);
-        // depth-D batch: indices are raw crossing ids (batch start 0, nothing kept before it)
-        outT = c.buf_keep<int>("mesh_triangles", size_t(3) * std::max(mm.nTri, 1), 0);
-        PSR_HIP_CHECK(hipMemcpyAsync(outT, mainT, sizeof(int) * 3 * mm.nTri, hipMemcpyDeviceToDevice, st));
         triTotal = mm.nTri;
     }
     // ---- subdivision sets: coarse roots one per set, finer roots one set per depth
@@ -848,6 +1250,26 @@ void extract_mesh(Ctx& c) {
         float* nbrDx = c.buf<float>("sub_nbr_dx", size_t(27) * nAnc);
         neighbour_blocks<<<div_up(size_t(27) * nAnc, kThreads), kThreads, 0, st>>>(nAnc, M, o.neigh, o.basis, c.fld.dx,
                                                                                   nbrBasis, nbrDx);
+        int* rootNbr = c.buf<int>("sub_root_nbr", size_t(27) * mm.nRoots);
+        int* rootChain = c.buf<int>("sub_root_chain", size_t(kChain) * mm.nRoots);
+        int4* rootCell = c.buf<int4>("sub_root_cell", mm.nRoots);
+        root_records<<<div_up(size_t(27) * mm.nRoots, kThreads), kThreads, 0, st>>>(
+            mm.nRoots, M, L, mm.roots, rootPos, o.neigh, o.parent, o.key, rootNbr, rootChain, rootCell);
+        // one-root tiles read d_x from dense per-depth grids when the octree is geometric
+        const bool denseHalo = !mm.q1Fired && D - sets[0].rd >= 3;
+        for (const SetDesc& S : sets) {
+            int halo = 0;
+            if (denseHalo && D - S.rd >= 3 &&
+                (grid_tile_words(D, S.rd, &halo) > kLdsWordsGrid || halo > kHaloPerThread * kThreads))
+                throw std::runtime_error("subdivision tile exceeds its LDS budget");
+        }
+        float* dxAt = nullptr;
+        if (denseHalo) {
+            const size_t cellsAll = ((size_t(1) << (3 * (D + 1))) - 1) / 7;
+            dxAt = c.buf<float>("sub_dx_grid", cellsAll);
+            zero_floats<<<div_up(cellsAll, kThreads), kThreads, 0, st>>>(dxAt, cellsAll);
+            scatter_dx_grid<<<div_up(M, kThreads), kThreads, 0, st>>>(M, L, o.key, c.fld.dx, dxAt);
+        }
         std::vector<SetDesc> cs;
         std::vector<Tile> smallTiles;
         std::vector<Tile> tiles;
@@ -862,17 +1284,21 @@ void extract_mesh(Ctx& c) {
                 cs[k].dEnum = de;
                 de += int(setCells[s0 + k]);
             }
-            SetDesc* dsets = c.buf<SetDesc>("sub_sets", nCs);
-            PSR_HIP_CHECK(hipMemcpyAsync(dsets, cs.data(), sizeof(SetDesc) * nCs, hipMemcpyHostToDevice, st));
-            // tiles: those lying in one root first, then those of small roots; the
-            // two groups together cover every cell once
+            // tiles: those lying in one root first, then those of small roots; sets
+            // of one-root tiles come first, so the tiles are in enumeration order
             tiles.clear();
             for (int pass = 0; pass < 2; ++pass)
                 for (int k = 0; k < nCs; ++k) {
                     if ((D - cs[k].rd >= 3) != (pass == 0)) continue;
+                    cs[k].firstTile = int(tiles.size());
                     for (size_t t = 0; t < setCells[s0 + k]; t += kTile)
-                        tiles.push_back({k, cs[k].dEnum + int(t), int(std::min<size_t>(kTile, setCells[s0 + k] - t))});
+                        tiles.push_back({k, cs[k].dEnum + int(t), int(std::min<size_t>(kTile, setCells[s0 + k] - t)),
+                                         cs[k].rootOffset + int(t >> (3 * (D - cs[k].rd)))});
                 }
+            for (int k = 1; k < nCs; ++k)
+                if (cs[k].firstTile < cs[k - 1].firstTile) throw std::runtime_error("subdivision tiles out of order");
+            SetDesc* dsets = c.buf<SetDesc>("sub_sets", nCs);
+            PSR_HIP_CHECK(hipMemcpyAsync(dsets, cs.data(), sizeof(SetDesc) * nCs, hipMemcpyHostToDevice, st));
             const int nTiles = int(tiles.size());
             int nOne = 0;
             while (nOne < nTiles && D - cs[tiles[nOne].set].rd >= 3) ++nOne;
@@ -886,58 +1312,61 @@ void extract_mesh(Ctx& c) {
                 if (shk >= 3) continue;
                 const size_t step = shk == 1 ? 32 : 64;
                 for (size_t t = 0; t < setCells[s0 + k]; t += step)
-                    smallTiles.push_back({k, cs[k].dEnum + int(t), int(std::min<size_t>(step, setCells[s0 + k] - t))});
+                    smallTiles.push_back({k, cs[k].dEnum + int(t), int(std::min<size_t>(step, setCells[s0 + k] - t)),
+                                          cs[k].rootOffset + int(t >> (3 * shk))});
             }
             const int nSmall = int(smallTiles.size());
             Tile* dsmall = c.buf<Tile>("sub_small_tiles", std::max(nSmall, 1));
             if (nSmall)
                 PSR_HIP_CHECK(hipMemcpyAsync(dsmall, smallTiles.data(), sizeof(Tile) * nSmall, hipMemcpyHostToDevice,
                                              st));
-            Grid gr{D,        M,        nCs,        dsets,     mm.roots, rootPos, o.key,
-                    o.parent, o.childBase, o.neigh, o.basis, c.fld.dx, nbrBasis, nbrDx};
-            const size_t nflag = size_t(12) * nVD;
+            Grid gr{D,       M,       nCs,      dsets,    mm.roots, rootPos,   o.key,    o.parent, o.childBase,
+                    o.neigh, o.basis, c.fld.dx, nbrBasis, nbrDx,    rootNbr,   rootChain, rootCell, dxAt};
             float* cval = c.buf<float>("sub_cval", size_t(8) * nVD);
             float* vals = c.buf<float>("sub_vals", size_t(8) * nVD);
-            int* cube = c.buf<int>("sub_cube", nVD);
-            int* triNum = c.buf<int>("sub_trinum", nVD);
-            int* triAddr = c.buf<int>("sub_triaddr", nVD);
-            int* cflag = c.buf<int>("sub_cflag", nflag + 1);
-            int* cidx = c.buf<int>("sub_cidx", nflag + 1);
-            int* cnts = c.buf<int>("sub_counts", 8);
-            int* scratch = c.scratch_for(scan_scratch_ints(nflag + 1) + 64);
-            if (nOne > 0) sub_corner_values<true, kThreads><<<nOne, kThreads, 0, st>>>(gr, dtiles, nVD, isoDev, cval);
+            unsigned* cellWord = c.buf<unsigned>("sub_cell_word", nVD);
+            int* crossPre = c.buf<int>("sub_cross_prefix", nVD);
+            int4* tileCnt = c.buf<int4>("sub_tile_cnt", nTiles);
+            int4* tileBase = c.buf<int4>("sub_tile_base", nTiles);
+            int* cnts = c.buf<int>("sub_counts", 4);
+            if (nOne > 0 && denseHalo) {
+                int words = 0;
+                for (int k = 0; k < nCs; ++k)
+                    if (D - cs[k].rd >= 3) words = std::max(words, grid_tile_words(D, cs[k].rd));
+                sub_corner_values_grid<<<nOne, kThreads, sizeof(float) * words, st>>>(gr, dtiles, nVD, isoDev, cval);
+            }
+            else if (nOne > 0)
+                sub_corner_values<true, kThreads><<<nOne, kThreads, 0, st>>>(gr, dtiles, nVD, isoDev, cval);
             if (nSmall > 0)
                 sub_corner_values<false, kSmallThreads><<<nSmall, kSmallThreads, 0, st>>>(gr, dsmall, nVD, isoDev, cval);
-            sub_cells<<<nTiles, kThreads, 0, st>>>(gr, dtiles, nVD, cval, vals, cube, triNum, cflag);
-            scan_exclusive(cflag, cidx, int(nflag), cnts + 0, scratch, st);
-            scan_exclusive(triNum, triAddr, nVD, cnts + 1, scratch, st);
-            PSR_HIP_CHECK(hipMemcpyAsync(c.hostPinned, cnts, sizeof(int) * 2, hipMemcpyDeviceToHost, st));
+            sub_classify<<<nTiles, kThreads, 0, st>>>(gr, dtiles, nVD, cval, vals, cellWord, crossPre, tileCnt);
+            sub_tile_scan<<<1, kScanThreads, 0, st>>>(tileCnt, nTiles, tileBase, cnts);
+            PSR_HIP_CHECK(hipMemcpyAsync(c.hostPinned, cnts, sizeof(int) * 3, hipMemcpyDeviceToHost, st));
+            if (s0 == 0) PSR_HIP_CHECK(hipMemcpyAsync(c.hostPinned + 3, acc, sizeof(int), hipMemcpyDeviceToHost, st));
             PSR_HIP_CHECK(hipStreamSynchronize(st));
-            const int nCross = c.hostPinned[0], nTri = c.hostPinned[1];
-            float* subV = c.buf<float>("raw_sub_v", size_t(3) * std::max(nCross, 1));
-            sub_crossings<<<nTiles, kThreads, 0, st>>>(gr, dtiles, nVD, vals, cflag, cidx, subV);
-            int* crossStart = c.buf<int>("sub_cross_start", nCs);
-            set_cross_start<<<div_up(nCs, 64), 64, 0, st>>>(nCs, dsets, cidx, nflag, nCross, crossStart);
-            // this chunk's batches: vertices, then triangles against the kept count before the chunk
-            int* keep = c.buf<int>("sub_keep", size_t(nCross) + 1);
-            int* localKept = c.buf<int>("sub_kept", size_t(nCross) + 1);
-            append_vertices(subV, nCross, keep, localKept, scratch);
-            if (nTri > 0) {
-                outT = c.buf_keep<int>("mesh_triangles", 3 * (triTotal + nTri), 3 * triTotal);
-                sub_triangles<<<nTiles, kThreads, 0, st>>>(gr, dtiles, cube, triAddr, cidx, crossStart, localKept,
-                                                           acc, int(triTotal), outT);
-                triTotal += nTri;
-            }
-            add_count<<<1, 1, 0, st>>>(acc, acc + 1);
+            const int nKept = c.hostPinned[1], nTri = c.hostPinned[2];
+            if (s0 == 0) keptTotal = size_t(c.hostPinned[3]);
+            if (keptTotal + nKept > size_t(INT32_MAX) || triTotal + nTri > size_t(INT32_MAX) / 3)
+                throw std::runtime_error("mesh exceeds 32-bit indexing");
+            outV = c.buf_keep<float>("mesh_vertices", 3 * (keptTotal + nKept), 3 * keptTotal);
+            outT = c.buf_keep<int>("mesh_triangles", 3 * (triTotal + nTri), 3 * triTotal);
+            if (nKept > 0 || nTri > 0)
+                sub_emit<<<nTiles, kThreads, 0, st>>>(gr, dtiles, nVD, vals, cellWord, crossPre, tileCnt, tileBase,
+                                                      int(keptTotal), int(triTotal), outV, outT);
+            keptTotal += nKept;
+            triTotal += nTri;
             s0 = s1;
         }
     }
-    PSR_HIP_CHECK(hipMemcpyAsync(c.hostPinned, acc, sizeof(int), hipMemcpyDeviceToHost, st));
-    PSR_HIP_CHECK(hipStreamSynchronize(st));
+    if (nSets == 0) {  // only the depth-D batch: read its kept count
+        PSR_HIP_CHECK(hipMemcpyAsync(c.hostPinned, acc, sizeof(int), hipMemcpyDeviceToHost, st));
+        PSR_HIP_CHECK(hipStreamSynchronize(st));
+        keptTotal = size_t(c.hostPinned[0]);
+    }
     if (triTotal > size_t(INT32_MAX) / 3) throw std::runtime_error("mesh exceeds 32-bit triangle indexing");
     c.mesh.vertices = outV;
     c.mesh.triangles = outT;
-    c.mesh.nv = c.hostPinned[0];
+    c.mesh.nv = int(keptTotal);
     c.mesh.nt = int(triTotal);
     c.mesh.counts = acc;
     c.stats.n_mesh_vertices = c.mesh.nv;
