@@ -91,17 +91,30 @@ def cpu_baseline(xyz, nrm, depth, reps, threads):
                       "OpenMP, geometry and marching cubes run on one thread" % (reps, dt)}
 
 
+def cg_kernel_name(st, precision):
+    if st["cg_row_form"]:
+        return "cg_mixed_row" if precision == "mixed" else "cg_fp32_row"
+    return "cg_mixed_all_depths" if precision == "mixed" else "cg_all_depths"
+
+
 def cg_algorithmic_bytes(st, precision):
     """HBM bytes the CG launch must move, per depth with z stored operator
     entries (4 B value + 4 B column) and n rows, each array read once per pass.
-    FP32 (CGAlgorithm.cu:10-80): 8z + 48n per iteration (stencil; x, r, p, Ap
-    and the double-buffered p).  Mixed: 8n once (b, x), then per outer step
-    16z + 32n (the start pass: rhs, r, w, s, p; the FP64 residual: b, x, r),
-    and per inner iteration 8z + 40n (r, w, s read; s, r, w written; p and x
-    read and written)."""
+    One row per thread (stencil held on chip, read once): FP32 8z + 16n (b in,
+    records out, x out) + 16n per iteration (8-byte records {r, p} read and
+    written); mixed 8z + 8n (b, x) + 28n per outer step (rhs, 12-byte records
+    out; b, x, residual) + 28n per inner iteration (records {r, w, s} read and
+    written, x out).  Several rows per thread (stencil read every sweep):
+    FP32 (CGAlgorithm.cu:10-80) 8z + 48n per iteration; mixed 8n, then
+    16z + 32n per outer step and 8z + 40n per inner iteration."""
     total = 0
+    row = bool(st["cg_row_form"])
     for k, z, n, o in zip(st["cg_iterations"], st["cg_nnz"], st["nodes_per_depth"], st["cg_outer"]):
-        if precision == "mixed":
+        if row and precision == "mixed":
+            total += 8 * z + 8 * n + o * 28 * n + k * 28 * n
+        elif row:
+            total += 8 * z + 16 * n + k * 16 * n
+        elif precision == "mixed":
             total += 8 * n + o * (16 * z + 32 * n) + k * (8 * z + 40 * n)
         else:
             total += k * (8 * z + 48 * n)
@@ -189,7 +202,7 @@ def main():
     head = forms[-1]  # the headline form: mixed precision when both run
     h = res[head]
     st = h["stats"]
-    kernel = "cg_mixed_all_depths" if head == "mixed" else "cg_all_depths"
+    kernel = cg_kernel_name(st, head)
     achieved = h["cg_bytes"] / (h["cg_ms"] * 1e-3) / 1e9
     line = {
         "metric": "reconstructions/sec at depth 7, 200k pts; mixed-prec vs FP32 at <=1e-5 residual",

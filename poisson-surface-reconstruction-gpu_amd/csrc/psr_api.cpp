@@ -107,6 +107,7 @@ int run(psr_ctx* ctx, const psr_points* pts, psr_mesh* out) {
     build_geometry(c);
     extract_mesh(c);
     PSR_HIP_CHECK(hipEventRecord(c.ev[4], s));
+    PSR_HIP_CHECK(hipMemcpyAsync(c.hostResults, c.results(), sizeof(DevResults), hipMemcpyDeviceToHost, s));
     PSR_HIP_CHECK(hipStreamSynchronize(s));
     // statistics
     psr_stats& st = c.stats;
@@ -115,28 +116,17 @@ int run(psr_ctx* ctx, const psr_points* pts, psr_mesh* out) {
     st.n_points = N;
     st.n_nodes = o.M;
     for (int d = 0; d <= o.L.D; ++d) st.nodes_per_depth[d] = o.L.count[d];
-    int iters[2 * (PSR_MAX_DEPTH + 1)];
-    double resid[PSR_MAX_DEPTH + 1];
-    PSR_HIP_CHECK(hipMemcpy(iters, c.arena.get("cg_iters", 8 * (PSR_MAX_DEPTH + 1)), sizeof(iters),
-                            hipMemcpyDeviceToHost));
-    PSR_HIP_CHECK(hipMemcpy(resid, c.arena.get("cg_resid", 8 * (PSR_MAX_DEPTH + 1)), sizeof(resid),
-                            hipMemcpyDeviceToHost));
-    PSR_HIP_CHECK(hipMemcpy(st.cg_nnz, c.arena.get("cg_nnz", 4 * (PSR_MAX_DEPTH + 1)), sizeof(st.cg_nnz),
-                            hipMemcpyDeviceToHost));
-    int err = 0;
-    PSR_HIP_CHECK(hipMemcpy(&err, c.arena.get("cg_err", 4), sizeof(int), hipMemcpyDeviceToHost));
-    if (err) return fail(ctx, PSR_ERR_INTERNAL, "CG group barrier timed out");
-    double trueRel[PSR_MAX_DEPTH + 1];
-    PSR_HIP_CHECK(hipMemcpy(trueRel, c.arena.get("cg_truerel", 8 * (PSR_MAX_DEPTH + 1)), sizeof(trueRel),
-                            hipMemcpyDeviceToHost));
+    const DevResults& r = *c.hostResults;
+    if (r.cgErr) return fail(ctx, PSR_ERR_INTERNAL, "CG group barrier timed out");
     for (int d = 0; d <= o.L.D; ++d) {
-        st.cg_iterations[d] = iters[d];
-        st.cg_outer[d] = iters[PSR_MAX_DEPTH + 1 + d];
-        st.cg_updated_residual[d] = resid[d];
-        st.cg_true_relres[d] = c.cfg.precision == PSR_PRECISION_MIXED ? trueRel[d] : -1.0;
+        st.cg_iterations[d] = r.cgIters[d];
+        st.cg_outer[d] = r.cgIters[PSR_MAX_DEPTH + 1 + d];
+        st.cg_nnz[d] = r.cgNnz[d];
+        st.cg_updated_residual[d] = r.cgResid[d];
+        st.cg_true_relres[d] = c.cfg.precision == PSR_PRECISION_MIXED ? r.cgTrueRel[d] : -1.0;
     }
-    PSR_HIP_CHECK(hipMemcpy(&st.iso_value, c.arena.get("iso_value", 4), sizeof(float), hipMemcpyDeviceToHost));
-    PSR_HIP_CHECK(hipMemcpy(&st.q1_fired, o.q1Fired, sizeof(int), hipMemcpyDeviceToHost));
+    st.iso_value = r.iso;
+    st.q1_fired = r.q1Fired;
     st.n_vertex_nodes = c.geo.V;
     st.n_edge_nodes = c.geo.E;
     st.n_face_nodes = c.geo.F;
@@ -206,6 +196,7 @@ int psr_create(const psr_config* cfg, void* hip_stream, psr_ctx** out) {
         }
         for (auto& e : c.ev) PSR_HIP_CHECK(hipEventCreate(&e));
         PSR_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&c.hostPinned), 64 * sizeof(int)));
+        PSR_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&c.hostResults), sizeof(psr::DevResults)));
         psr::build_tables(c);  // one-time, like ComputeVectorField's constructor (ComputeVectorField.cpp:11-15)
         PSR_HIP_CHECK(hipStreamSynchronize(c.stream));
     } catch (const psr::HipError& e) {
@@ -482,6 +473,8 @@ void psr_destroy(psr_ctx* ctx) {
     for (auto& e : c.ev)
         if (e) (void)hipEventDestroy(e);
     if (c.hostPinned) (void)hipHostFree(c.hostPinned);
+    if (c.hostResults) (void)hipHostFree(c.hostResults);
+    if (c.hostStage) (void)hipHostFree(c.hostStage);
     if (c.ownStream && c.stream) (void)hipStreamDestroy(c.stream);
     delete ctx;
 }

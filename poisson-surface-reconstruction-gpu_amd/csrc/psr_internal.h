@@ -101,6 +101,17 @@ struct Mesh {
     int* counts = nullptr;      // device: final vertex / triangle counts
 };
 
+// Small per-call results written by the kernels and read back in one copy.
+struct DevResults {
+    int cgIters[2 * (PSR_MAX_DEPTH + 1)];  // inner iterations | outer steps, per depth
+    int cgNnz[PSR_MAX_DEPTH + 1];          // stored operator entries per depth
+    int cgErr;                             // a CG group barrier timed out
+    int q1Fired;                           // the level-local child 0 rule changed a parent
+    float iso;                             // iso-value
+    double cgResid[PSR_MAX_DEPTH + 1];     // final updated residual norm per depth
+    double cgTrueRel[PSR_MAX_DEPTH + 1];   // mixed form: final true relative residual per depth
+};
+
 struct Ctx {
     psr_config cfg{};
     hipStream_t stream = nullptr;
@@ -117,7 +128,11 @@ struct Ctx {
     size_t scratchInts = 0;
     hipEvent_t ev[16] = {};
     int* hostPinned = nullptr;   // small pinned buffer for size read-backs
-    int cgMaxWgs = 0;            // resident workgroups of the CG kernel
+    DevResults* hostResults = nullptr;  // pinned mirror of results()
+    void* hostStage = nullptr;          // pinned staging for small uploads
+    size_t hostStageBytes = 0;
+    int cgMaxWgs = 0;            // resident workgroups of the CG kernels with several rows per thread
+    int cgMaxWgsRow = 0;         // resident workgroups of the one-row-per-thread CG kernels
     int cgRowsPerWg = 0;         // rows per CG workgroup (0: default)
     template <class T>
     T* buf(const char* name, size_t count) {
@@ -126,6 +141,18 @@ struct Ctx {
     template <class T>
     T* buf_keep(const char* name, size_t count, size_t keepCount) {
         return static_cast<T*>(arena.get_keep(name, sizeof(T) * (count ? count : 1), sizeof(T) * keepCount));
+    }
+    DevResults* results() { return buf<DevResults>("results", 1); }
+    // Pinned staging of at least `bytes`; only reused after a stream synchronisation.
+    void* stage(size_t bytes) {
+        if (bytes > hostStageBytes) {
+            if (hostStage) PSR_HIP_CHECK(hipHostFree(hostStage));
+            hostStage = nullptr;
+            hostStageBytes = 0;
+            PSR_HIP_CHECK(hipHostMalloc(&hostStage, bytes + bytes / 2 + 4096));
+            hostStageBytes = bytes + bytes / 2 + 4096;
+        }
+        return hostStage;
     }
     int* scratch_for(size_t ints) {
         if (ints > scratchInts) {

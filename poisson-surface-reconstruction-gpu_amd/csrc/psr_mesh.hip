@@ -451,7 +451,7 @@ void build_geometry(Ctx& c) {
     face_maintain<<<div_up(size_t(6) * M, kThreads), kThreads, 0, s>>>(v, g.counts + 2, g.faceOwner, g.faceKind,
                                                                       g.nodeFace);
     // vertex implicit values
-    float* isoDev = c.buf<float>("iso_value", 1);
+    float* isoDev = &c.results()->iso;
     g.vvalue = c.buf<float>("vvalue", size_t(8) * M);
     vertex_values<<<div_up(size_t(8) * M, kThreads), kThreads, 0, s>>>(v, g.counts + 0, g.vtxOwner, g.vtxKind,
                                                                        g.vtxPos, o.parent, o.childBase, o.basis,

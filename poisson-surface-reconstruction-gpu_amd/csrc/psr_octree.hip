@@ -487,7 +487,7 @@ void build_octree(Ctx& c, const psr_points& pts) {
     o.neigh = c.buf<int>("node_neigh", size_t(27) * M);
     o.encIdx = c.buf<int>("node_enc", M);
     zero_int<<<div_up(M, kThreads), kThreads, 0, s>>>(o.pnum, M);
-    o.q1Fired = c.buf<int>("q1_fired", 1);
+    o.q1Fired = &c.results()->q1Fired;
     zero_int<<<1, kThreads, 0, s>>>(o.q1Fired, 1);
     fill_occupied<<<div_up(totalNodes, kThreads), kThreads, 0, s>>>(keys, N, L, lo, Ud, firstPoint, parentRank,
                                                                      rankDm1, totalNodes, o.key, o.pidx, o.pnum,

@@ -14,8 +14,9 @@ void scan_exclusive(const int* in, int* out, int n, int* total, int* scratch, hi
 // Scratch needed by radix_sort_pairs for n elements, in int32 units.
 size_t sort_scratch_ints(size_t n);
 // Stable ascending sort of keys (only the low `bits` bits are examined) carrying
-// values.  Sorted data ends in (keys, vals); ktmp/vtmp are n-element buffers.
-void radix_sort_pairs(unsigned* keys, unsigned* vals, unsigned* ktmp, unsigned* vtmp, int n, int bits,
+// values.  The sorted data ends in (keys, vals): the four buffers are swapped
+// in place of a final copy, so keys/vals may come back as the former ktmp/vtmp.
+void radix_sort_pairs(unsigned*& keys, unsigned*& vals, unsigned*& ktmp, unsigned*& vtmp, int n, int bits,
                       int* scratch, hipStream_t s);
 
 }  // namespace psr

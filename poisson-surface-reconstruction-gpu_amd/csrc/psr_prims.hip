@@ -211,8 +211,8 @@ size_t sort_scratch_ints(size_t n) {
     return h + scan_scratch_ints(h) + 64;
 }
 
-void radix_sort_pairs(unsigned* keys, unsigned* vals, unsigned* ktmp, unsigned* vtmp, int n, int bits, int* scratch,
-                      hipStream_t s) {
+void radix_sort_pairs(unsigned*& keys, unsigned*& vals, unsigned*& ktmp, unsigned*& vtmp, int n, int bits,
+                      int* scratch, hipStream_t s) {
     if (n <= 1) return;
     int tiles = int(div_up(n, kSortTile));
     int* hist = scratch;
@@ -228,8 +228,8 @@ void radix_sort_pairs(unsigned* keys, unsigned* vals, unsigned* ktmp, unsigned* 
         std::swap(va, vb);
     }
     if (ka != keys) {
-        PSR_HIP_CHECK(hipMemcpyAsync(keys, ka, sizeof(unsigned) * n, hipMemcpyDeviceToDevice, s));
-        PSR_HIP_CHECK(hipMemcpyAsync(vals, va, sizeof(unsigned) * n, hipMemcpyDeviceToDevice, s));
+        std::swap(keys, ktmp);
+        std::swap(vals, vtmp);
     }
     PSR_HIP_CHECK(hipGetLastError());
 }

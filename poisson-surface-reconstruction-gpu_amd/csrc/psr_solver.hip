@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <mutex>
 #include <stdexcept>
+#include <vector>
 
 namespace psr {
 
@@ -181,6 +182,217 @@ __device__ __forceinline__ double fold_double(const double* parts, int n, double
 
 __device__ __forceinline__ float p_next(float r, float beta, float p) { return r + beta * p; }
 
+// Vectors that other workgroups read are exchanged as packed 8- or 16-byte
+// records through buffer loads and stores with the cache policy of the
+// agent-scope atomics (sc1), so one load brings all of a neighbour's values.
+constexpr int kCoherent = 16;  // cache-policy bits: sc1
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t vec_rsrc(const void* base, size_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), short(0), int(bytes), 0x00020000);
+}
+__device__ __forceinline__ float2 ld_vec2(__amdgpu_buffer_rsrc_t r, int idx) {
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, idx * 8, 0, kCoherent));
+}
+__device__ __forceinline__ void st_vec2(__amdgpu_buffer_rsrc_t r, int idx, float2 v) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v), r,
+                                          idx * 8, 0, kCoherent);
+}
+__device__ __forceinline__ float3 ld_vec3(__amdgpu_buffer_rsrc_t r, int idx) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b96(r, idx * 12, 0, kCoherent);
+    return make_float3(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]));
+}
+__device__ __forceinline__ void st_vec3(__amdgpu_buffer_rsrc_t r, int idx, float3 v) {
+    __attribute__((ext_vector_type(3))) unsigned u;
+    u[0] = __float_as_uint(v.x);
+    u[1] = __float_as_uint(v.y);
+    u[2] = __float_as_uint(v.z);
+    __builtin_amdgcn_raw_buffer_store_b96(u, r, idx * 12, 0, kCoherent);
+}
+__device__ __forceinline__ float4 ld_vec4(__amdgpu_buffer_rsrc_t r, int idx) {
+    return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, idx * 16, 0, kCoherent));
+}
+__device__ __forceinline__ void st_vec4(__amdgpu_buffer_rsrc_t r, int idx, float4 v) {
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(__attribute__((ext_vector_type(4))) unsigned, v), r,
+                                           idx * 16, 0, kCoherent);
+}
+
+// Atomic-free group all-reduce of the mixed one-row kernel (measured faster
+// there than the arrival counter of group_allreduce; the FP32 kernel, with two
+// reductions and little work between them, is faster with the counter).  Each
+// workgroup stores its block total in its own 16-byte slot(s) of the group's
+// region, stamped with the phase (one store per slot, so a reader sees a slot
+// whole); every
+// workgroup polls the group's slots until all carry the phase and then folds
+// them in the same fixed order, so every workgroup holds the same bits.  Slots
+// are double-buffered by phase parity (a workgroup runs at most one phase
+// ahead of the slowest) and zeroed before the launch (phases start at 1).
+// S: slots per workgroup (two payload words each); thread t polls the slots
+// of workgroups t and t + kCgThreads.
+constexpr int kPollPerThread = 2;  // groups of up to kPollPerThread * kCgThreads workgroups
+template <int S>
+__device__ __forceinline__ bool slot_exchange(float4* region, int G, int local, unsigned& phase,
+                                              const unsigned (&mine)[2 * S], float4 (&got)[kPollPerThread][S],
+                                              unsigned spinLimit, int* errFlag) {
+    ++phase;
+    const __amdgpu_buffer_rsrc_t rs =
+        vec_rsrc(region + size_t(phase & 1u) * G * S, sizeof(float4) * size_t(G) * S);
+    const float stamp = __uint_as_float(phase);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int h = 0; h < S; ++h)
+            st_vec4(rs, local * S + h,
+                    make_float4(__uint_as_float(mine[2 * h]), __uint_as_float(mine[2 * h + 1]), stamp, 0.f));
+    }
+    for (unsigned spins = 0;; ++spins) {
+        int ok = 1;
+#pragma unroll
+        for (int j = 0; j < kPollPerThread; ++j) {
+            const int k = int(threadIdx.x) + j * kCgThreads;
+            if (k < G) {
+#pragma unroll
+                for (int h = 0; h < S; ++h) {
+                    got[j][h] = ld_vec4(rs, k * S + h);
+                    ok &= __float_as_uint(got[j][h].z) == phase ? 1 : 0;
+                }
+            }
+        }
+        if (__syncthreads_and(ok)) return true;
+        if (spins >= spinLimit) {
+            if (threadIdx.x == 0) __hip_atomic_store(errFlag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+        if ((spins & 63u) == 63u) {
+            const int e =
+                threadIdx.x == 0 ? __hip_atomic_load(errFlag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+            if (__syncthreads_or(e)) return false;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+// Sum of the group's block totals (v: this workgroup's, the same in every
+// thread), folded as fold_double folds.
+__device__ __forceinline__ bool slot_sum_double(double v, double& out, float4* region, int G, int local,
+                                                unsigned& phase, unsigned spinLimit, int* errFlag, double* sh) {
+    const unsigned long long u = static_cast<unsigned long long>(__double_as_longlong(v));
+    const unsigned mine[2] = {unsigned(u), unsigned(u >> 32)};
+    float4 got[kPollPerThread][1];
+    if (!slot_exchange<1>(region, G, local, phase, mine, got, spinLimit, errFlag)) return false;
+    double t = 0;
+#pragma unroll
+    for (int j = 0; j < kPollPerThread; ++j)
+        if (int(threadIdx.x) + j * kCgThreads < G)
+            t += __hiloint2double(__float_as_int(got[j][0].y), __float_as_int(got[j][0].x));
+    out = block_sum_double(t, sh);
+    return true;
+}
+
+// A row's 27 stencil entries, held in registers for the whole solve; a
+// dropped or missing entry reads the row itself with weight 0.
+struct Stencil {
+    float v[27];
+    int j[27];
+    __device__ __forceinline__ void load(const float* __restrict__ val, const int* __restrict__ neigh, int M, int i) {
+#pragma unroll
+        for (int q = 0; q < 27; ++q) {
+            v[q] = val[q * M + i];
+            const int c = neigh[q * M + i];
+            j[q] = (v[q] != 0.f) ? c : i;
+        }
+    }
+};
+
+// The reference's CG (CGAlgorithm.cu:10-80), one launch for all depths, one
+// row per thread.  Per iteration k the record of row i in buffer k & 1 holds
+// {r_k, p_{k-1}}; a row forms its neighbours' p_k = r_k + beta p_{k-1} on the
+// fly, so the update of p and the product A p need no barrier in between
+// (CGAlgorithm.cu:37-53).  The row's stencil, x, r, p and A p stay in
+// registers; one 8-byte record per neighbour is read and one written per
+// iteration.
+__global__ void __launch_bounds__(kCgThreads) cg_fp32_row(CgParams P, const int* __restrict__ neigh,
+                                                          const float* __restrict__ val, const float* __restrict__ b,
+                                                          float* __restrict__ x, float2* state, double* partial,
+                                                          GroupCtl* ctlAll, int* __restrict__ iters,
+                                                          double* __restrict__ resid, int* errFlag) {
+    __shared__ double sh[kCgWaves];
+    __shared__ int shFlag;
+    __shared__ double shRes;
+    int gi = 0;
+    while (gi + 1 < P.ngroups && int(blockIdx.x) >= P.g[gi + 1].firstWg) ++gi;
+    const CgGroup G = P.g[gi];
+    const int local = blockIdx.x - G.firstWg;
+    const int i = G.rowBegin + local * G.rowsPerWg + int(threadIdx.x);
+    const bool own = int(threadIdx.x) < G.rowsPerWg && i < G.rowBegin + G.rows;
+    const int M = P.M;
+    double* part = partial + 2 * G.firstWg;
+    GroupCtl* ctl = ctlAll + gi;
+    unsigned phase = 0;
+    const bool multi = G.wgs > 1;
+    const __amdgpu_buffer_rsrc_t st = vec_rsrc(state, sizeof(float2) * 2 * size_t(M));
+    auto group_sum = [&](double v, double& out) -> bool {
+        double s = block_sum_double(v, sh);
+        if (!multi) {
+            out = s;
+            return true;
+        }
+        return group_allreduce(s, out, part, ctl, phase, G.wgs, local, P.spinLimit, errFlag, &shFlag, &shRes,
+                               [&](const double* parts, int n) { return fold_double(parts, n, sh); });
+    };
+    Stencil sr;
+    if (own) sr.load(val, neigh, M, i);
+    // x = 0, r = b, rr = r.r  (CGAlgorithm.cu:21-33 with x0 = 0)
+    float xi = 0.f, ri = 0.f, pi = 0.f, api = 0.f;
+    double loc = 0;
+    if (own) {
+        ri = b[i];
+        st_vec2(st, i, make_float2(ri, 0.f));
+        loc = double(ri * ri);
+    }
+    double rrD;
+    if (!group_sum(loc, rrD)) return;
+    float rr = float(rrD), rrOld = 1.f;
+    int k = 1;
+    while (rr > P.tolSq && k <= P.maxIter) {
+        const float beta = (k > 1) ? rr / rrOld : 0.f;
+        const int cur = ((k - 1) & 1) * M, nxt = M - cur;
+        // p = r + beta p (CGAlgorithm.cu:37-43), Ap = A p (:47), p.Ap (:53)
+        double lpAp = 0;
+        if (own) {
+            float s = 0.f;
+#pragma unroll
+            for (int q = 0; q < 27; ++q) {
+                const float2 nb = ld_vec2(st, cur + sr.j[q]);
+                s += sr.v[q] * p_next(nb.x, beta, nb.y);
+            }
+            pi = p_next(ri, beta, pi);
+            api = s;
+            lpAp = double(pi * s);
+        }
+        double pAp;
+        if (!group_sum(lpAp, pAp)) return;
+        const float alpha = float(double(rr) / pAp);
+        double lrr = 0;
+        if (own) {
+            xi = xi + alpha * pi;
+            ri = ri + (-alpha) * api;
+            st_vec2(st, nxt + i, make_float2(ri, pi));
+            lrr = double(ri * ri);
+        }
+        rrOld = rr;
+        double rrNew;
+        if (!group_sum(lrr, rrNew)) return;
+        rr = float(rrNew);
+        ++k;
+    }
+    if (own) x[i] = xi;
+    if (local == 0 && threadIdx.x == 0) {
+        iters[G.depth] = k - 1;
+        iters[PSR_MAX_DEPTH + 1 + G.depth] = 1;
+        resid[G.depth] = sqrt(double(rr));
+    }
+}
+
+// The same iteration with several rows per thread (large depths): vectors in
+// memory, the stencil read every sweep.
 __global__ void __launch_bounds__(kCgThreads) cg_all_depths(CgParams P, const int* __restrict__ neigh,
                                                             const float* __restrict__ val,
                                                             const float* __restrict__ b, float* __restrict__ x,
@@ -338,6 +550,161 @@ __device__ __forceinline__ Comp2 block_comp2(Comp2 v, Comp2* sh) {
     return t;
 }
 
+// Mixed form, one launch for all depths, one row per thread.  Per inner
+// iteration k the 12-byte record of row i in buffer k & 1 holds {r_k, w_k, s_{k-1}};
+// the row writes {r_{k+1}, w_{k+1}, s_k} to the other buffer, keeps its r, w,
+// s, p and x in registers (x is also stored for the outer residual) and its
+// stencil in LDS.
+__global__ void __launch_bounds__(kCgThreads) cg_mixed_row(CgParams P, const int* __restrict__ neigh,
+                                                           const float* __restrict__ val,
+                                                           const float* __restrict__ b, float* x, float* rOut,
+                                                           float3* state, float4* slots, int* __restrict__ iters,
+                                                           double* __restrict__ resid, double* __restrict__ trueRel,
+                                                           int* errFlag) {
+    __shared__ double shd[kCgWaves];
+    __shared__ Comp2 shc[kCgWaves];
+    __shared__ float lV[27][kCgThreads];  // the row's stencil, read from LDS every sweep
+    __shared__ int lJ[27][kCgThreads];
+    int gi = 0;
+    while (gi + 1 < P.ngroups && int(blockIdx.x) >= P.g[gi + 1].firstWg) ++gi;
+    const CgGroup G = P.g[gi];
+    const int local = blockIdx.x - G.firstWg;
+    const int i = G.rowBegin + local * G.rowsPerWg + int(threadIdx.x);
+    const bool own = int(threadIdx.x) < G.rowsPerWg && i < G.rowBegin + G.rows;
+    const int M = P.M;
+    float4* region = slots + size_t(4) * G.firstWg;  // [2][wgs][2]
+    unsigned phase = 0;
+    const bool multi = G.wgs > 1;
+    const __amdgpu_buffer_rsrc_t st = vec_rsrc(state, sizeof(float3) * 2 * size_t(M));
+    auto sum_double = [&](double v, double& out) -> bool {
+        double s = block_sum_double(v, shd);
+        if (!multi) {
+            out = s;
+            return true;
+        }
+        return slot_sum_double(s, out, region, G.wgs, local, phase, P.spinLimit, errFlag, shd);
+    };
+    auto sum_comp = [&](Comp2 v, Comp2& out) -> bool {
+        Comp2 s = block_comp2(v, shc);
+        if (!multi) {
+            out = s;
+            return true;
+        }
+        const unsigned mine[4] = {__float_as_uint(s.gs), __float_as_uint(s.gc), __float_as_uint(s.ds),
+                                  __float_as_uint(s.dc)};
+        float4 got[kPollPerThread][2];
+        if (!slot_exchange<2>(region, G.wgs, local, phase, mine, got, P.spinLimit, errFlag)) return false;
+        Comp2 t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int j = 0; j < kPollPerThread; ++j)
+            if (int(threadIdx.x) + j * kCgThreads < G.wgs)
+                comp2_merge(t, Comp2{got[j][0].x, got[j][0].y, got[j][1].x, got[j][1].y});
+        out = block_comp2(t, shc);
+        return true;
+    };
+    if (own) {
+        Stencil sr;
+        sr.load(val, neigh, M, i);
+#pragma unroll
+        for (int q = 0; q < 27; ++q) {
+            lV[q][threadIdx.x] = sr.v[q];
+            lJ[q][threadIdx.x] = sr.j[q];
+        }
+    }
+    float xi = 0.f, ro = 0.f, wo = 0.f, so = 0.f, po = 0.f;
+    // x = 0; ||b||^2 in FP64 (the outer residual of x = 0 is b itself)
+    double loc = 0;
+    if (own) {
+        st_shared(&x[i], 0.f);
+        const double bi = b[i];
+        loc = bi * bi;
+    }
+    double bb;
+    if (!sum_double(loc, bb)) return;
+    double rr = bb, rel = bb > 0 ? 1.0 : 0.0;
+    const float* rhs = b;
+    int total = 0, outer = 0;
+    float gamma = 0.f;
+    while (rel > P.relTol && outer < P.maxOuter && total < P.maxIter) {
+        // inner target on the updated residual: half the tolerance left, no tighter than FP32 can follow
+        const double tgt = fmax(0.5 * P.relTol * sqrt(bb), 1e-6 * sqrt(rr));
+        const float tgt2 = float(tgt * tgt);
+        // r_0 = rhs, w_0 = A r_0, s_{-1} = 0, p_{-1} = 0
+        Comp2 d0{0.f, 0.f, 0.f, 0.f};
+        if (own) {
+            float w = 0.f;
+#pragma unroll
+            for (int q = 0; q < 27; ++q) w = __fmaf_rn(lV[q][threadIdx.x], ld_shared(&rhs[lJ[q][threadIdx.x]]), w);
+            ro = ld_shared(&rhs[i]);
+            wo = w;
+            so = 0.f;
+            po = 0.f;
+            st_vec3(st, i, make_float3(ro, w, 0.f));
+            dot2_acc(d0.gs, d0.gc, ro, ro);
+            dot2_acc(d0.ds, d0.dc, w, ro);
+        }
+        Comp2 g;
+        if (!sum_comp(d0, g)) return;
+        gamma = g.gs + g.gc;
+        float delta = g.ds + g.dc;
+        float alpha = gamma / delta, beta = 0.f;
+        int k = 0;
+        while (gamma > tgt2 && total + k < P.maxIter) {
+            const int cur = (k & 1) * M, nxt = M - cur;
+            Comp2 dl{0.f, 0.f, 0.f, 0.f};
+            if (own) {
+                float w = 0.f;
+#pragma unroll
+                for (int q = 0; q < 27; ++q) {
+                    const float3 nb = ld_vec3(st, cur + lJ[q][threadIdx.x]);
+                    w = __fmaf_rn(lV[q][threadIdx.x], r_next(nb.x, alpha, s_next(nb.y, beta, nb.z)), w);
+                }
+                const float si = s_next(wo, beta, so);
+                const float rni = r_next(ro, alpha, si);
+                po = __fmaf_rn(beta, po, ro);
+                xi = __fmaf_rn(alpha, po, xi);
+                st_shared(&x[i], xi);
+                st_vec3(st, nxt + i, make_float3(rni, w, si));
+                ro = rni;
+                wo = w;
+                so = si;
+                dot2_acc(dl.gs, dl.gc, rni, rni);
+                dot2_acc(dl.ds, dl.dc, w, rni);
+            }
+            if (!sum_comp(dl, g)) return;
+            const float gNew = g.gs + g.gc, dNew = g.ds + g.dc;
+            const float bNew = gNew / gamma;
+            alpha = gNew / (dNew - bNew * gNew / alpha);
+            beta = bNew;
+            gamma = gNew;
+            ++k;
+        }
+        total += k;
+        ++outer;
+        // outer residual in FP64
+        loc = 0;
+        if (own) {
+            double acc = b[i];
+#pragma unroll
+            for (int q = 0; q < 27; ++q)
+                acc -= double(lV[q][threadIdx.x]) * double(ld_shared(&x[lJ[q][threadIdx.x]]));
+            st_shared(&rOut[i], float(acc));
+            loc = acc * acc;
+        }
+        if (!sum_double(loc, rr)) return;
+        rel = bb > 0 ? sqrt(rr / bb) : 0.0;
+        rhs = rOut;
+    }
+    if (local == 0 && threadIdx.x == 0) {
+        iters[G.depth] = total;
+        iters[PSR_MAX_DEPTH + 1 + G.depth] = outer;
+        resid[G.depth] = sqrt(double(gamma));
+        trueRel[G.depth] = rel;
+    }
+}
+
+// The mixed form with several rows per thread (large depths): vectors in
+// memory, the stencil read every sweep.
 __global__ void __launch_bounds__(kCgThreads) cg_mixed_all_depths(
     CgParams P, const int* __restrict__ neigh, const float* __restrict__ val, const float* __restrict__ b,
     float* x, float* rOut, float* rbuf, float* wbuf, float* sbuf, float* __restrict__ pvec, double* partialD,
@@ -588,39 +955,42 @@ void solve_depths(Ctx& c) {
     float* val = c.buf<float>("lap_val", size_t(27) * M);
     assemble_operator<<<div_up(M, kThreads), kThreads, 0, s>>>(L, M, o.neigh, o.encIdx, c.tables.dot, c.tables.d2dot,
                                                               c.tables.res, val);
-    int* nnz = c.buf<int>("cg_nnz", PSR_MAX_DEPTH + 1);
+    int* nnz = c.results()->cgNnz;
     zero_u32<<<1, 64, 0, s>>>(reinterpret_cast<unsigned*>(nnz), PSR_MAX_DEPTH + 1);
     count_nonzeros<<<div_up(M, kThreads), kThreads, 0, s>>>(L, M, val, nnz);
     c.fld.dx = c.buf<float>("dx", M);
-    float* r = c.buf<float>("cg_r", M);
-    float* pA = c.buf<float>("cg_pa", M);
-    float* pB = c.buf<float>("cg_pb", M);
-    float* Ap = c.buf<float>("cg_ap", M);
+    if (size_t(M) * 2 * sizeof(float4) > size_t(INT32_MAX)) throw std::runtime_error("CG state exceeds 2 GiB");
     CgParams P{};
     P.M = M;
     P.tolSq = c.cfg.cg_tol * c.cfg.cg_tol;
     P.maxIter = c.cfg.cg_max_iter;
-    P.spinLimit = 1u << 26;
-    // workgroup budget: every workgroup of the cooperative launch must be resident
+    P.spinLimit = 1u << 26;  // row form: polls of about a microsecond each, set below
+    const bool mixed = c.cfg.precision == PSR_PRECISION_MIXED;
+    const void* rowKernel = mixed ? reinterpret_cast<const void*>(cg_mixed_row) : reinterpret_cast<const void*>(cg_fp32_row);
+    const void* genKernel =
+        mixed ? reinterpret_cast<const void*>(cg_mixed_all_depths) : reinterpret_cast<const void*>(cg_all_depths);
+    // workgroup budgets: every workgroup of the cooperative launch must be resident
     if (c.cgMaxWgs <= 0) {
-        int perCu = 0, cus = 0;
-        int perCuMixed = 0;
-        PSR_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, cg_all_depths, kCgThreads, 0));
-        PSR_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCuMixed, cg_mixed_all_depths, kCgThreads, 0));
-        perCu = std::min(perCu, perCuMixed);
+        int perRow = 0, perGen = 0, cus = 0;
+        PSR_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perRow, rowKernel, kCgThreads, 0));
+        PSR_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perGen, genKernel, kCgThreads, 0));
         PSR_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device));
-        c.cgMaxWgs = std::max(1, perCu * cus);
+        c.cgMaxWgsRow = std::max(0, perRow * cus);
+        c.cgMaxWgs = std::max(1, perGen * cus);
     }
-    int rowsPerWg = c.cgRowsPerWg > 0 ? c.cgRowsPerWg : 256;
-    {
-        // widen the row blocks until every group fits the resident budget
-        auto need = [&](int rpw) {
-            long long w = 0;
-            for (int d = 0; d <= D; ++d) w += std::max<long long>(1, (L.count[d] + rpw - 1) / rpw);
-            return w;
-        };
+    auto need = [&](int rpw) {
+        long long w = 0;
+        for (int d = 0; d <= D; ++d) w += std::max<long long>(1, (L.count[d] + rpw - 1) / rpw);
+        return w;
+    };
+    // one row per thread (stencil in registers) when every group fits at
+    // kCgThreads rows per workgroup; otherwise widen the row blocks
+    int rowsPerWg = c.cgRowsPerWg > 0 ? c.cgRowsPerWg : kCgThreads;
+    bool rowForm = rowsPerWg <= kCgThreads && need(rowsPerWg) <= c.cgMaxWgsRow;
+    for (int d = 0; d <= D; ++d)
+        if (div_up(L.count[d], kCgThreads) > unsigned(kPollPerThread * kCgThreads)) rowForm = false;
+    if (!rowForm)
         while (need(rowsPerWg) > c.cgMaxWgs) rowsPerWg *= 2;
-    }
     int wg = 0;
     // largest depths first so their groups start on separate CUs early
     for (int d = D; d >= 0; --d) {
@@ -633,15 +1003,17 @@ void solve_depths(Ctx& c) {
         g.wgs = std::max(1, int(div_up(g.rows, rowsPerWg)));
         wg += g.wgs;
     }
-    if (wg > c.cgMaxWgs) throw std::runtime_error("CG launch exceeds the resident workgroup budget");
+    if (wg > (rowForm ? c.cgMaxWgsRow : c.cgMaxWgs))
+        throw std::runtime_error("CG launch exceeds the resident workgroup budget");
     c.stats.cg_workgroups = uint32_t(wg);
     double* partial = c.buf<double>("cg_partial", size_t(2) * wg);
     GroupCtl* ctl = c.buf<GroupCtl>("cg_ctl", PSR_MAX_DEPTH + 1);
     const int ctlWords = int(sizeof(GroupCtl) / 4) * (PSR_MAX_DEPTH + 1);
-    int* iters = c.buf<int>("cg_iters", 2 * (PSR_MAX_DEPTH + 1));  // [inner iterations | outer steps] per depth
-    double* resid = c.buf<double>("cg_resid", PSR_MAX_DEPTH + 1);
-    double* trueRel = c.buf<double>("cg_truerel", PSR_MAX_DEPTH + 1);
-    int* err = c.buf<int>("cg_err", 1);
+    DevResults* res = c.results();
+    int* iters = res->cgIters;  // [inner iterations | outer steps] per depth
+    double* resid = res->cgResid;
+    double* trueRel = res->cgTrueRel;
+    int* err = &res->cgErr;
     zero_u32<<<div_up(ctlWords, 256), 256, 0, s>>>(reinterpret_cast<unsigned*>(ctl), ctlWords);
     zero_u32<<<1, 64, 0, s>>>(reinterpret_cast<unsigned*>(err), 1);
     zero_u32<<<1, 64, 0, s>>>(reinterpret_cast<unsigned*>(trueRel), 2 * (PSR_MAX_DEPTH + 1));
@@ -651,24 +1023,45 @@ void solve_depths(Ctx& c) {
     std::lock_guard<std::mutex> order(cg_order_lock());
     PSR_HIP_CHECK(hipStreamWaitEvent(s, cg_order_event(), 0));
     PSR_HIP_CHECK(hipEventRecord(c.ev[8], s));
-    if (c.cfg.precision == PSR_PRECISION_MIXED) {
+    float4* slots = nullptr;
+    if (rowForm && mixed) {
+        P.spinLimit = 1u << 22;
+        slots = c.buf<float4>("cg_slots", size_t(4) * wg);
+        zero_u32<<<div_up(size_t(16) * wg, 256), 256, 0, s>>>(reinterpret_cast<unsigned*>(slots), 16 * wg);
+    }
+    if (mixed) {
         P.relTol = c.cfg.refine_rel_tol;
         P.maxOuter = std::max(1, c.cfg.refine_max_outer);
         float* rOut = c.buf<float>("cg_rout", M);
-        float* rbuf = c.buf<float>("cg_rbuf", size_t(2) * M);
-        float* wbuf = c.buf<float>("cg_wbuf", size_t(2) * M);
-        float* sbuf = c.buf<float>("cg_sbuf", size_t(2) * M);
         Comp2* partialC = c.buf<Comp2>("cg_partial_c", size_t(2) * wg);
-        void* args[] = {&P,  &o.neigh, &val,      &c.fld.div, &c.fld.dx, &rOut,  &rbuf,    &wbuf, &sbuf,
-                        &pA, &partial, &partialC, &ctl,       &iters,    &resid, &trueRel, &err};
-        PSR_HIP_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(cg_mixed_all_depths), dim3(wg),
-                                                 dim3(kCgThreads), args, 0, s));
+        if (rowForm) {
+            float3* state = c.buf<float3>("cg_state3", size_t(2) * M);
+            void* args[] = {&P,     &o.neigh, &val,  &c.fld.div, &c.fld.dx, &rOut,
+                            &state, &slots,   &iters, &resid,    &trueRel,  &err};
+            PSR_HIP_CHECK(hipLaunchCooperativeKernel(rowKernel, dim3(wg), dim3(kCgThreads), args, 0, s));
+        } else {
+            float* rbuf = c.buf<float>("cg_rbuf", size_t(2) * M);
+            float* wbuf = c.buf<float>("cg_wbuf", size_t(2) * M);
+            float* sbuf = c.buf<float>("cg_sbuf", size_t(2) * M);
+            float* pvec = c.buf<float>("cg_pa", M);
+            void* args[] = {&P,    &o.neigh, &val,     &c.fld.div, &c.fld.dx, &rOut,  &rbuf,    &wbuf, &sbuf,
+                            &pvec, &partial, &partialC, &ctl,      &iters,    &resid, &trueRel, &err};
+            PSR_HIP_CHECK(hipLaunchCooperativeKernel(genKernel, dim3(wg), dim3(kCgThreads), args, 0, s));
+        }
+    } else if (rowForm) {
+        float2* state = c.buf<float2>("cg_state2", size_t(2) * M);
+        void* args[] = {&P, &o.neigh, &val, &c.fld.div, &c.fld.dx, &state, &partial, &ctl, &iters, &resid, &err};
+        PSR_HIP_CHECK(hipLaunchCooperativeKernel(rowKernel, dim3(wg), dim3(kCgThreads), args, 0, s));
     } else {
+        float* r = c.buf<float>("cg_r", M);
+        float* pA = c.buf<float>("cg_pa", M);
+        float* pB = c.buf<float>("cg_pb", M);
+        float* Ap = c.buf<float>("cg_ap", M);
         void* args[] = {&P, &o.neigh, &val, &c.fld.div, &c.fld.dx, &r, &pA, &pB, &Ap, &partial, &ctl, &iters, &resid,
                         &err};
-        PSR_HIP_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(cg_all_depths), dim3(wg),
-                                                 dim3(kCgThreads), args, 0, s));
+        PSR_HIP_CHECK(hipLaunchCooperativeKernel(genKernel, dim3(wg), dim3(kCgThreads), args, 0, s));
     }
+    c.stats.cg_row_form = rowForm ? 1 : 0;
     PSR_HIP_CHECK(hipEventRecord(c.ev[9], s));
     PSR_HIP_CHECK(hipEventRecord(cg_order_event(), s));
     PSR_HIP_CHECK(hipGetLastError());
@@ -682,7 +1075,7 @@ void points_implicit(Ctx& c) {
     c.fld.chiPts = c.buf<float>("chi_points", N);
     double* part = c.buf<double>("iso_part", nb);
     c.fld.isoSum = c.buf<double>("iso_sum", 1);
-    float* isoDev = c.buf<float>("iso_value", 1);
+    float* isoDev = &c.results()->iso;
     points_chi_kernel<<<nb, kThreads, 0, s>>>(N, L.base[L.D], o.M, L.D, o.spos, o.point2node, o.parent, o.neigh,
                                               o.basis, c.fld.dx, c.fld.chiPts, part);
     iso_final<<<1, 1024, 0, s>>>(part, nb, N, c.fld.isoSum, isoDev);

@@ -27,6 +27,7 @@
 #include "psr_prims.h"
 
 #include <algorithm>
+#include <cstring>
 
 namespace psr {
 
@@ -1244,7 +1245,7 @@ void extract_mesh(Ctx& c) {
         fill_root_pos<<<div_up(mm.nRoots, kThreads), kThreads, 0, st>>>(mm.roots, mm.nRoots, rootPos);
         const size_t chunkCells = std::min<size_t>(c.cfg.subdiv_chunk_nodes ? c.cfg.subdiv_chunk_nodes : kChunkCells,
                                                    size_t(INT32_MAX - 1) / 12);
-        float* isoDev = c.buf<float>("iso_value", 1);
+        float* isoDev = &c.results()->iso;
         const int nAnc = std::max(1, L.base[D - 1]);  // nodes above depth D-1
         float4* nbrBasis = c.buf<float4>("sub_nbr_basis", size_t(27) * nAnc);
         float* nbrDx = c.buf<float>("sub_nbr_dx", size_t(27) * nAnc);
@@ -1297,13 +1298,9 @@ void extract_mesh(Ctx& c) {
                 }
             for (int k = 1; k < nCs; ++k)
                 if (cs[k].firstTile < cs[k - 1].firstTile) throw std::runtime_error("subdivision tiles out of order");
-            SetDesc* dsets = c.buf<SetDesc>("sub_sets", nCs);
-            PSR_HIP_CHECK(hipMemcpyAsync(dsets, cs.data(), sizeof(SetDesc) * nCs, hipMemcpyHostToDevice, st));
             const int nTiles = int(tiles.size());
             int nOne = 0;
             while (nOne < nTiles && D - cs[tiles[nOne].set].rd >= 3) ++nOne;
-            Tile* dtiles = c.buf<Tile>("sub_tiles", nTiles);
-            PSR_HIP_CHECK(hipMemcpyAsync(dtiles, tiles.data(), sizeof(Tile) * nTiles, hipMemcpyHostToDevice, st));
             // corner values of small roots: a tile per 4 roots of 8 cells or per
             // root of 64 cells, one thread per corner point
             smallTiles.clear();
@@ -1316,10 +1313,20 @@ void extract_mesh(Ctx& c) {
                                           cs[k].rootOffset + int(t >> (3 * shk))});
             }
             const int nSmall = int(smallTiles.size());
-            Tile* dsmall = c.buf<Tile>("sub_small_tiles", std::max(nSmall, 1));
-            if (nSmall)
-                PSR_HIP_CHECK(hipMemcpyAsync(dsmall, smallTiles.data(), sizeof(Tile) * nSmall, hipMemcpyHostToDevice,
-                                             st));
+            // sets, tiles and small tiles go up in one copy from pinned staging
+            // (reused only after this chunk's synchronisation)
+            const size_t bSets = (sizeof(SetDesc) * nCs + 255) & ~size_t(255);
+            const size_t bTiles = (sizeof(Tile) * nTiles + 255) & ~size_t(255);
+            const size_t bAll = bSets + bTiles + sizeof(Tile) * std::max(nSmall, 1);
+            char* hs = static_cast<char*>(c.stage(bAll));
+            std::memcpy(hs, cs.data(), sizeof(SetDesc) * nCs);
+            std::memcpy(hs + bSets, tiles.data(), sizeof(Tile) * nTiles);
+            if (nSmall) std::memcpy(hs + bSets + bTiles, smallTiles.data(), sizeof(Tile) * nSmall);
+            char* dAll = c.buf<char>("sub_upload", bAll);
+            PSR_HIP_CHECK(hipMemcpyAsync(dAll, hs, bAll, hipMemcpyHostToDevice, st));
+            SetDesc* dsets = reinterpret_cast<SetDesc*>(dAll);
+            Tile* dtiles = reinterpret_cast<Tile*>(dAll + bSets);
+            Tile* dsmall = reinterpret_cast<Tile*>(dAll + bSets + bTiles);
             Grid gr{D,       M,       nCs,      dsets,    mm.roots, rootPos,   o.key,    o.parent, o.childBase,
                     o.neigh, o.basis, c.fld.dx, nbrBasis, nbrDx,    rootNbr,   rootChain, rootCell, dxAt};
             float* cval = c.buf<float>("sub_cval", size_t(8) * nVD);
