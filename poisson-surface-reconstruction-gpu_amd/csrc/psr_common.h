@@ -114,42 +114,23 @@ __device__ __forceinline__ float node_term(float4 b, float dx, float3 p) {
     return dx * bspline((p.x - b.x) * b.w) * bspline((p.y - b.y) * b.w) * bspline((p.z - b.z) * b.w);
 }
 
-// sum over the 27 neighbours of `node` of dx * F(x) F(y) F(z)
-__device__ __forceinline__ float chi_node_sum(int node, float3 p, int M, const int* __restrict__ neigh,
-                                              const float4* __restrict__ basis, const float* __restrict__ dx) {
-    float v = 0.f;
+// sum over the 27 neighbours of `node` of dx * F(x) F(y) F(z), from the
+// neighbour columns ([27][M]: basis parameters and d_x of neighbour i, zero
+// where absent).  An absent neighbour adds +0 or -0, which leaves the sum
+// unchanged: it starts at +0 and can never become -0 by adding.
+__device__ __forceinline__ float chi_node_sum(int node, float3 p, int M, const float4* __restrict__ nbrBasis,
+                                              const float* __restrict__ nbrDx) {
+    float4 b[27];
+    float d[27];
+#pragma unroll
     for (int i = 0; i < 27; ++i) {
-        int nb = neigh[i * M + node];
-        if (nb == -1) continue;
-        v += node_term(basis[nb], dx[nb], p);
+        b[i] = nbrBasis[size_t(i) * M + node];
+        d[i] = nbrDx[size_t(i) * M + node];
     }
+    float v = 0.f;
+#pragma unroll
+    for (int i = 0; i < 27; ++i) v += node_term(b[i], d[i], p);
     return v;
-}
-
-// 3x3x3 neighbourhood look-up tables (derived; BuildOctree.cu:20-42 lists the
-// same values): neighbour i of a child with son key s is child LUTchild of the
-// parent's neighbour LUTparent.
-struct NeighborLut {
-    unsigned char parent[8][27];
-    unsigned char child[8][27];
-};
-
-__host__ __device__ inline void make_neighbor_lut(NeighborLut& t) {
-    for (int s = 0; s < 8; ++s) {
-        int sb[3] = {(s >> 2) & 1, (s >> 1) & 1, s & 1};
-        for (int i = 0; i < 27; ++i) {
-            int dd[3] = {i / 9 - 1, (i / 3) % 3 - 1, i % 3 - 1};
-            int pi = 0, ci = 0;
-            for (int a = 0; a < 3; ++a) {
-                int c = sb[a] + dd[a];
-                int p = (c < 0) ? -1 : (c > 1 ? 1 : 0);
-                pi = pi * 3 + (p + 1);
-                ci = ci * 2 + (c - 2 * p);
-            }
-            t.parent[s][i] = (unsigned char)pi;
-            t.child[s][i] = (unsigned char)ci;
-        }
-    }
 }
 
 }  // namespace psr

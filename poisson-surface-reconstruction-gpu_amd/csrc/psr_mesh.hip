@@ -224,8 +224,8 @@ __global__ void __launch_bounds__(kThreads) vertex_values(NodeView v, const int*
                                                           const float4* __restrict__ vPos,
                                                           const int* __restrict__ parent,
                                                           const int* __restrict__ childBase,
-                                                          const float4* __restrict__ basis,
-                                                          const float* __restrict__ dx,
+                                                          const float4* __restrict__ nbrBasis,
+                                                          const float* __restrict__ nbrDx,
                                                           const float* __restrict__ iso, float* __restrict__ vv) {
     int t = blockIdx.x * kThreads + threadIdx.x;
     if (t >= *count) return;
@@ -237,7 +237,7 @@ __global__ void __launch_bounds__(kThreads) vertex_values(NodeView v, const int*
     if (owner > 0) {
         int now = owner;
         while (now != -1) {
-            val += chi_node_sum(now, p, v.M, v.neigh, basis, dx);
+            val += chi_node_sum(now, p, v.M, nbrBasis, nbrDx);
             now = parent[now];
         }
         // descent into the key-ordered child picked by the Bourke corner id
@@ -248,7 +248,7 @@ __global__ void __launch_bounds__(kThreads) vertex_values(NodeView v, const int*
             int cb = childBase[now];
             if (cb < 0) break;
             now = cb + ex;
-            val += chi_node_sum(now, p, v.M, v.neigh, basis, dx);
+            val += chi_node_sum(now, p, v.M, nbrBasis, nbrDx);
         }
     }
     vv[t] = val - *iso;
@@ -392,15 +392,21 @@ __global__ void __launch_bounds__(kThreads) root_flags(int count, int M, const i
     flag[o] = (childBase[o] < 0 && (hasTri || hi) && neigh[13 * M + o] == o) ? 1 : 0;
 }
 
-__global__ void __launch_bounds__(kThreads) root_emit(int count, int M, const int* __restrict__ flag,
-                                                      const int* __restrict__ addr, const int* __restrict__ neigh,
-                                                      const int* __restrict__ depth, int* __restrict__ roots,
-                                                      int* __restrict__ depthHist) {
+// Roots in ascending node order, and roots per depth.  precomputeSubdivideDepth
+// reads a root's depth through neighs[13]; a root is its own neighbour
+// (root_flags), so that is the root's level, and the count of a depth is the
+// difference of the root scan at the level bounds.
+__global__ void __launch_bounds__(kThreads) root_emit(int count, Levels L, const int* __restrict__ flag,
+                                                      const int* __restrict__ addr, const int* __restrict__ total,
+                                                      int* __restrict__ roots, int* __restrict__ depthHist) {
     int o = blockIdx.x * kThreads + threadIdx.x;
+    if (blockIdx.x == 0 && int(threadIdx.x) < L.D) {
+        const int d = threadIdx.x;
+        const int hi = L.base[d + 1] < count ? addr[L.base[d + 1]] : *total;
+        depthHist[d] = hi - addr[L.base[d]];
+    }
     if (o >= count || !flag[o]) return;
     roots[addr[o]] = o;
-    int self = neigh[13 * M + o];  // precomputeSubdivideDepth reads the depth through neighs[13]
-    atomicAdd(&depthHist[depth[self >= 0 ? self : o]], 1);
 }
 
 }  // namespace
@@ -454,8 +460,8 @@ void build_geometry(Ctx& c) {
     float* isoDev = &c.results()->iso;
     g.vvalue = c.buf<float>("vvalue", size_t(8) * M);
     vertex_values<<<div_up(size_t(8) * M, kThreads), kThreads, 0, s>>>(v, g.counts + 0, g.vtxOwner, g.vtxKind,
-                                                                       g.vtxPos, o.parent, o.childBase, o.basis,
-                                                                       c.fld.dx, isoDev, g.vvalue);
+                                                                       g.vtxPos, o.parent, o.childBase,
+                                                                       c.fld.nbrBasis, c.fld.nbrDx, isoDev, g.vvalue);
     PSR_HIP_CHECK(hipMemcpyAsync(c.hostPinned, g.counts, sizeof(int) * 3, hipMemcpyDeviceToHost, s));
     PSR_HIP_CHECK(hipStreamSynchronize(s));
     g.V = c.hostPinned[0];
@@ -499,7 +505,7 @@ void depth_mc(Ctx& c, MainMc& mm, float* rawV, int* rawT) {
     root_flags<<<div_up(offD, kThreads), kThreads, 0, s>>>(offD, M, o.neigh, g.nodeVert, g.nodeFace, o.childBase,
                                                            g.vvalue, mm.hasInter, rflag);
     scan_exclusive(rflag, raddr, offD, mm.counts + 2, scratch, s);
-    root_emit<<<div_up(offD, kThreads), kThreads, 0, s>>>(offD, M, rflag, raddr, o.neigh, o.depth, mm.roots,
+    root_emit<<<div_up(offD, kThreads), kThreads, 0, s>>>(offD, L, rflag, raddr, mm.counts + 2, mm.roots,
                                                           mm.counts + 4);
     PSR_HIP_CHECK(hipMemcpyAsync(c.hostPinned, mm.counts, sizeof(int) * (PSR_MAX_DEPTH + 8), hipMemcpyDeviceToHost,
                                  s));

@@ -370,24 +370,39 @@ __global__ void __launch_bounds__(kThreads) zero_int(int* __restrict__ a, int n)
 }
 
 // Same-depth neighbours from the parent's neighbours (BuildOctree.cu:490-512).
+// Neighbours of the depth-`depth` nodes from their parents' (the LUT recurrence
+// of BuildOctree.cu:20-42, 413-440): neighbour i of a child with son key s is
+// child LUTchild[s][i] of the parent's neighbour LUTparent[s][i].  Both are
+// derived per axis: the child-level offset c = s_a + d_a lies in parent cell
+// floor(c / 2) at child bit c - 2 floor(c / 2); only the 8 parent neighbours
+// with per-axis offsets in {s_a - 1, s_a} are touched, each read once.
 __global__ void __launch_bounds__(kThreads) neighbors_level(int begin, int count, int D, int depth, int M,
                                                             const int* __restrict__ key,
                                                             const int* __restrict__ parent,
                                                             const int* __restrict__ childBase,
-                                                            int* __restrict__ neigh, NeighborLut lut) {
-    int t = blockIdx.x * kThreads + threadIdx.x;
+                                                            int* __restrict__ neigh) {
+    const int t = blockIdx.x * kThreads + threadIdx.x;
     if (t >= count) return;
-    int o = begin + t;
-    int s = (key[o] >> (3 * (D - depth))) & 7;
-    int p = parent[o];
+    const int o = begin + t;
+    const int s = (key[o] >> (3 * (D - depth))) & 7;
+    const int p = parent[o];
+    const int sx = (s >> 2) & 1, sy = (s >> 1) & 1, sz = s & 1;
+    int cb[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        const int px = sx - 1 + ((u >> 2) & 1), py = sy - 1 + ((u >> 1) & 1), pz = sz - 1 + (u & 1);
+        const int np = neigh[(9 * (px + 1) + 3 * (py + 1) + (pz + 1)) * M + p];
+        cb[u] = np;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) cb[u] = cb[u] == -1 ? -1 : childBase[cb[u]];
+#pragma unroll
     for (int i = 0; i < 27; ++i) {
-        int np = neigh[lut.parent[s][i] * M + p];
-        int r = -1;
-        if (np != -1) {
-            int cb = childBase[np];
-            r = cb < 0 ? -1 : cb + lut.child[s][i];
-        }
-        neigh[i * M + o] = r;
+        const int cx = sx + i / 9 - 1, cy = sy + (i / 3) % 3 - 1, cz = sz + i % 3 - 1;
+        const int fx = cx >> 1, fy = cy >> 1, fz = cz >> 1;  // floor(c / 2) for c in [-1, 2]
+        const int u = ((fx - sx + 1) << 2) | ((fy - sy + 1) << 1) | (fz - sz + 1);
+        const int b = cb[u];
+        neigh[i * M + o] = b < 0 ? -1 : b + (((cx - 2 * fx) << 2) | ((cy - 2 * fy) << 1) | (cz - 2 * fz));
     }
 }
 
@@ -497,12 +512,10 @@ void build_octree(Ctx& c, const psr_points& pts) {
     fill_blocks<<<div_up(totalBlocks + 1, kThreads), kThreads, 0, s>>>(L, totalBlocks, N, o.U[D - 1], o.key, o.pidx,
                                                                         o.pnum, o.parent, o.childBase, o.didx,
                                                                         o.dnum);
-    NeighborLut lut;
-    make_neighbor_lut(lut);
     root_neighbors<<<1, 64, 0, s>>>(o.neigh, M);
     for (int d = 1; d <= D; ++d)
         neighbors_level<<<div_up(L.count[d], kThreads), kThreads, 0, s>>>(L.base[d], L.count[d], D, d, M, o.key,
-                                                                           o.parent, o.childBase, o.neigh, lut);
+                                                                           o.parent, o.childBase, o.neigh);
     o.depth = c.buf<int>("node_depth", M);
     o.basis = c.buf<float4>("node_basis", M);
     encode_function_index<<<div_up(M, kThreads), kThreads, 0, s>>>(L, M, o.key, o.encIdx, o.depth, o.basis);

@@ -860,13 +860,26 @@ __global__ void __launch_bounds__(kCgThreads) cg_mixed_all_depths(
 }
 
 // --- implicit function at points (LaplacianSolver.cu:85-118, 269-305) ------
+// Neighbour columns: basis parameters and d_x of neighbour i of every node, so
+// the 27 terms of a node need no dependent index loads.
+__global__ void __launch_bounds__(kThreads) neighbour_columns(int M, const int* __restrict__ neigh,
+                                                              const float4* __restrict__ basis,
+                                                              const float* __restrict__ dx,
+                                                              float4* __restrict__ nbrBasis,
+                                                              float* __restrict__ nbrDx) {
+    const size_t t = size_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (t >= size_t(27) * M) return;
+    const int nb = neigh[t];
+    nbrBasis[t] = nb < 0 ? make_float4(0.f, 0.f, 0.f, 0.f) : basis[nb];
+    nbrDx[t] = nb < 0 ? 0.f : dx[nb];
+}
+
 __global__ void __launch_bounds__(kThreads) points_chi_kernel(int N, int offD, int M, int D,
                                                               const float4* __restrict__ pos,
                                                               const int* __restrict__ p2n,
                                                               const int* __restrict__ parent,
-                                                              const int* __restrict__ neigh,
-                                                              const float4* __restrict__ basis,
-                                                              const float* __restrict__ dx,
+                                                              const float4* __restrict__ nbrBasis,
+                                                              const float* __restrict__ nbrDx,
                                                               float* __restrict__ chi, double* __restrict__ part) {
     __shared__ double sh[kThreads / PSR_WAVE];
     int i = blockIdx.x * kThreads + threadIdx.x;
@@ -877,7 +890,7 @@ __global__ void __launch_bounds__(kThreads) points_chi_kernel(int N, int offD, i
         int now = offD + p2n[i];
         float v = 0.f;
         while (now != -1) {
-            v += chi_node_sum(now, p, M, neigh, basis, dx);
+            v += chi_node_sum(now, p, M, nbrBasis, nbrDx);
             now = parent[now];
         }
         chi[i] = v;
@@ -1076,8 +1089,12 @@ void points_implicit(Ctx& c) {
     double* part = c.buf<double>("iso_part", nb);
     c.fld.isoSum = c.buf<double>("iso_sum", 1);
     float* isoDev = &c.results()->iso;
-    points_chi_kernel<<<nb, kThreads, 0, s>>>(N, L.base[L.D], o.M, L.D, o.spos, o.point2node, o.parent, o.neigh,
-                                              o.basis, c.fld.dx, c.fld.chiPts, part);
+    c.fld.nbrBasis = c.buf<float4>("nbr_basis", size_t(27) * o.M);
+    c.fld.nbrDx = c.buf<float>("nbr_dx", size_t(27) * o.M);
+    neighbour_columns<<<div_up(size_t(27) * o.M, kThreads), kThreads, 0, s>>>(o.M, o.neigh, o.basis, c.fld.dx,
+                                                                             c.fld.nbrBasis, c.fld.nbrDx);
+    points_chi_kernel<<<nb, kThreads, 0, s>>>(N, L.base[L.D], o.M, L.D, o.spos, o.point2node, o.parent,
+                                              c.fld.nbrBasis, c.fld.nbrDx, c.fld.chiPts, part);
     iso_final<<<1, 1024, 0, s>>>(part, nb, N, c.fld.isoSum, isoDev);
     PSR_HIP_CHECK(hipGetLastError());
 }

@@ -54,8 +54,8 @@ struct Grid {
     const int* neigh;
     const float4* basis;
     const float* dx;
-    const float4* nbrBasis;  // [node][27] basis parameters of the node's neighbours (nodes above depth D-1)
-    const float* nbrDx;      // [node][27] their d_x (0 where absent)
+    const float4* nbrBasis;  // [27][M] basis parameters of neighbour i of each node (0 where absent)
+    const float* nbrDx;      // [27][M] its d_x (0 where absent)
     const int* rootNbr;      // [root][27] position in roots of the root's real neighbour, or -1
     const int* rootChain;    // [root][kChain] the root, then its ancestors (-1 past the tree root)
     const int4* rootCell;    // [root] cell at the root depth (x, y, z) and the root node (w)
@@ -77,20 +77,30 @@ __device__ __forceinline__ int bourke_slot(int ox, int oy, int oz) {
 
 // Morton code of a cell at a depth with `bits` levels (x high in each triplet):
 // the node key format of the reference.
+__device__ __forceinline__ unsigned spread3(unsigned v) {  // bit i -> bit 3i, i < 10
+    v &= 0x3FFu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+__device__ __forceinline__ unsigned compact3(unsigned v) {  // bit 3i -> bit i, i < 10
+    v &= 0x09249249u;
+    v = (v | (v >> 2)) & 0x030C30C3u;
+    v = (v | (v >> 4)) & 0x0300F00Fu;
+    v = (v | (v >> 8)) & 0x030000FFu;
+    v = (v | (v >> 16)) & 0x3FFu;
+    return v;
+}
+// The low `bits` bits of each coordinate, interleaved.
 __device__ __forceinline__ int morton(int3 c, int bits) {
-    int k = 0;
-    for (int i = 0; i < bits; ++i)
-        k |= (((c.x >> i) & 1) << (3 * i + 2)) | (((c.y >> i) & 1) << (3 * i + 1)) | (((c.z >> i) & 1) << (3 * i));
-    return k;
+    const unsigned m = (1u << bits) - 1u;
+    return int((spread3(unsigned(c.x) & m) << 2) | (spread3(unsigned(c.y) & m) << 1) | spread3(unsigned(c.z) & m));
 }
 __device__ __forceinline__ int3 demorton(int k, int bits) {
-    int3 c = make_int3(0, 0, 0);
-    for (int i = 0; i < bits; ++i) {
-        c.x |= ((k >> (3 * i + 2)) & 1) << i;
-        c.y |= ((k >> (3 * i + 1)) & 1) << i;
-        c.z |= ((k >> (3 * i)) & 1) << i;
-    }
-    return c;
+    const unsigned u = unsigned(k) & ((1u << (3 * bits)) - 1u);
+    return make_int3(int(compact3(u >> 2)), int(compact3(u >> 1)), int(compact3(u)));
 }
 
 // A virtual depth-D cell and its root.
@@ -381,34 +391,28 @@ __device__ __forceinline__ void fill_vbox(const Grid& g, const Tile& T, const Se
     }
 }
 
-// fill_vbox with the roots' neighbour positions in LDS (nbr: [root][27]); the
-// same test as virtual_enum.
-__device__ __forceinline__ void fill_vbox_rec(const Grid& g, const Tile& T, const SetDesc& S, const TileGeom& tg,
-                                              const TileRoot* tr, const int* nbr, int* vbox) {
-    const int by = tg.ext.y + 2, bz = tg.ext.z + 2;
-    const int sh = g.D - S.rd, lim = 1 << g.D;
-    for (int t = threadIdx.x; t < tg.nRoots * tg.vboxPerRoot; t += blockDim.x) {
-        const int rr = t / tg.vboxPerRoot, u = t - rr * tg.vboxPerRoot;
-        const TileRoot R = tr[rr];
-        const int dz = u % bz, dy = (u / bz) % by, dx = u / (bz * by);
-        const int3 n = make_int3(R.first.x - 1 + dx, R.first.y - 1 + dy, R.first.z - 1 + dz);
-        int v = -1;
-        if (n.x >= 0 && n.y >= 0 && n.z >= 0 && n.x < lim && n.y < lim && n.z < lim) {
-            const int3 A = make_int3(n.x >> sh, n.y >> sh, n.z >> sh);
-            int q = R.q;
-            bool ok = true;
-            if (A.x != R.Rc.x || A.y != R.Rc.y || A.z != R.Rc.z) {
-                const int rp = nbr[rr * 27 + nbr_index(A.x - R.Rc.x, A.y - R.Rc.y, A.z - R.Rc.z)];
-                q = rp - S.rootOffset;
-                ok = rp >= S.rootOffset && rp < S.rootOffset + S.nr;
-            }
-            if (ok) {
-                const int3 lc = make_int3(n.x - (A.x << sh), n.y - (A.y << sh), n.z - (A.z << sh));
-                v = S.dEnum + (q << (3 * sh)) + morton(lc, sh);
-            }
+// Virtual-cell lookup from the root's neighbour positions (nbr: its 27
+// entries, in LDS): the same test as virtual_enum, without global loads.
+struct RecVE {
+    const int* nbr;
+    SetDesc S;
+    int3 Rc;
+    int q, sh, lim;
+    __device__ __forceinline__ int operator()(int3 n) const {
+        if (n.x < 0 || n.y < 0 || n.z < 0 || n.x >= lim || n.y >= lim || n.z >= lim) return -1;
+        const int3 A = make_int3(n.x >> sh, n.y >> sh, n.z >> sh);
+        int qq = q;
+        if (A.x != Rc.x || A.y != Rc.y || A.z != Rc.z) {
+            const int rp = nbr[nbr_index(A.x - Rc.x, A.y - Rc.y, A.z - Rc.z)];
+            if (!(rp >= S.rootOffset && rp < S.rootOffset + S.nr)) return -1;
+            qq = rp - S.rootOffset;
         }
-        vbox[t] = v;
+        const int3 lc = make_int3(n.x - (A.x << sh), n.y - (A.y << sh), n.z - (A.z << sh));
+        return S.dEnum + (qq << (3 * sh)) + morton(lc, sh);
     }
+};
+__device__ __forceinline__ RecVE rec_ve(const int* nbr, const Leaf& L, int D) {
+    return RecVE{nbr, L.S, L.Rc, L.q, L.sh, 1 << D};
 }
 
 // Virtual-cell lookup in the tile's depth-D box of this cell's root.
@@ -549,8 +553,8 @@ __global__ void __launch_bounds__(NT) sub_corner_values(Grid g, const Tile* __re
                 sDx[t] = nb < 0 ? 0.f : g.dx[nb];
             } else {         // ancestor: its precomputed neighbour block
                 int node = lChain[-1 - tag];
-                sBasis[t] = node < 0 ? make_float4(0.f, 0.f, 0.f, 0.f) : g.nbrBasis[27 * size_t(node) + i];
-                sDx[t] = node < 0 ? 0.f : g.nbrDx[27 * size_t(node) + i];
+                sBasis[t] = node < 0 ? make_float4(0.f, 0.f, 0.f, 0.f) : g.nbrBasis[size_t(i) * g.M + node];
+                sDx[t] = node < 0 ? 0.f : g.nbrDx[size_t(i) * g.M + node];
             }
         }
     }
@@ -624,9 +628,8 @@ __global__ void __launch_bounds__(NT) sub_corner_values(Grid g, const Tile* __re
                 } else if (u > 0) {
                     const int node = lChain[rr * C + u];
                     if (node >= 0) {
-                        const float4* bb = g.nbrBasis + 27 * size_t(node);
-                        const float* dd = g.nbrDx + 27 * size_t(node);
-                        for (int i = 0; i < 27; ++i) val += node_term(bb[i], dd[i], p);
+                        for (int i = 0; i < 27; ++i)
+                            val += node_term(g.nbrBasis[size_t(i) * g.M + node], g.nbrDx[size_t(i) * g.M + node], p);
                     }
                 } else {
                     int ids[27];
@@ -715,12 +718,11 @@ __global__ void __launch_bounds__(kThreads) sub_corner_values_grid(Grid g, const
     const int D = g.D;
     const TileGeom tg = tile_geom(T, S, D);
     const int K = tg.K, Kp = (K + 3) & ~3;
-    // list x (with d_x) [kTx][Kp] | list y [kTy][Kp] | list z [kTz][Kp] | vbox | halo d_x | halo factors
+    // list x (with d_x) [kTx][Kp] | list y [kTy][Kp] | list z [kTz][Kp] | halo d_x | halo factors
     float* lLx = lds;
     float* lLy = lLx + kTx * Kp;
     float* lLz = lLy + kTy * Kp;
-    int* lVbox = reinterpret_cast<int*>(lLz + kTz * Kp);
-    float* lHdx = reinterpret_cast<float*>(lVbox + tg.vboxPerRoot);
+    float* lHdx = lLz + kTz * Kp;
     float* lFac = lHdx + tg.haloPerRoot;
     int facWords = 0;
     for (int l = D; l > S.rd; --l) {
@@ -740,8 +742,8 @@ __global__ void __launch_bounds__(kThreads) sub_corner_values_grid(Grid g, const
         float4 b = make_float4(0.f, 0.f, 0.f, 0.f);
         float d = 0.f;
         if (node >= 0) {
-            b = g.nbrBasis[27 * size_t(node) + k % 27];
-            d = g.nbrDx[27 * size_t(node) + k % 27];
+            b = g.nbrBasis[size_t(k % 27) * g.M + node];
+            d = g.nbrDx[size_t(k % 27) * g.M + node];
         }
         for (int cc = 0; cc < kTx; ++cc)
             lLx[cc * Kp + k] = node < 0 ? 0.f : d * bspline((float(lo.x + cc) * inv - b.x) * b.w);
@@ -780,7 +782,6 @@ __global__ void __launch_bounds__(kThreads) sub_corner_values_grid(Grid g, const
         }
         lHdx[t] = v;
     }
-    fill_vbox_rec(g, T, S, tg, lRoot, lNbr, lVbox);
     // halo factor tables: per depth, node cell (box-relative) x corner coordinate
     for (int t = threadIdx.x; t < facWords; t += kThreads) {
         int u = t, l = D;
@@ -812,7 +813,7 @@ __global__ void __launch_bounds__(kThreads) sub_corner_values_grid(Grid g, const
     if (int(threadIdx.x) >= T.count) return;
     const int e = T.start + threadIdx.x;
     const Leaf L = leaf_of(g, T.set, S, R, e);
-    const BoxVE ve = box_ve(lVbox, tg, 0, lo);
+    const RecVE ve = rec_ve(lNbr, L, D);
     const float isoV = *iso;
     const int sh = D - S.rd, side = 1 << sh;
     const int3 lc = make_int3(L.c.x - (R.Rc.x << sh), L.c.y - (R.Rc.y << sh), L.c.z - (R.Rc.z << sh));
@@ -918,7 +919,6 @@ __global__ void __launch_bounds__(kThreads) sub_classify(Grid g, const Tile* __r
                                                          const float* __restrict__ cval, float* __restrict__ vals,
                                                          unsigned* __restrict__ cellWord,
                                                          int* __restrict__ crossPrefix, int4* __restrict__ tileCnt) {
-    __shared__ int lVbox[kVboxWords];
     __shared__ TileRoot lRoots[kMaxRootsPerTile];
     __shared__ int lNbr[kMaxRootsPerTile * 27];
     __shared__ int lSum[kThreads / PSR_WAVE];
@@ -929,8 +929,6 @@ __global__ void __launch_bounds__(kThreads) sub_classify(Grid g, const Tile* __r
     fill_roots_rec(g, T, S, tg, lRoots);
     for (int t = threadIdx.x; t < tg.nRoots * 27; t += kThreads) lNbr[t] = g.rootNbr[27 * size_t(T.root) + t];
     __syncthreads();
-    fill_vbox_rec(g, T, S, tg, lRoots, lNbr, lVbox);
-    __syncthreads();
     const bool active = int(threadIdx.x) < T.count;
     const int e = T.start + (active ? int(threadIdx.x) : 0);
     unsigned mask = 0, keep = 0;
@@ -938,7 +936,7 @@ __global__ void __launch_bounds__(kThreads) sub_classify(Grid g, const Tile* __r
     if (active) {
         const int rr = tg.oneRoot ? 0 : int(threadIdx.x) >> (3 * (D - S.rd));
         const Leaf L = leaf_of(g, T.set, S, lRoots[rr], e);
-        const BoxVE ve = box_ve(lVbox, tg, rr, lRoots[rr].first);
+        const RecVE ve = rec_ve(lNbr + 27 * rr, L, D);
         float v[8];
         int cls = 0;
 #pragma unroll
@@ -1109,21 +1107,6 @@ __global__ void __launch_bounds__(kThreads) zero_floats(float* __restrict__ a, s
     if (i < n) a[i] = 0.f;
 }
 
-// Basis parameters and d_x of the 27 neighbours of every node above depth D-1
-// (the ancestors of subdivision roots); zero where a neighbour is absent.
-__global__ void __launch_bounds__(kThreads) neighbour_blocks(int nNodes, int M, const int* __restrict__ neigh,
-                                                             const float4* __restrict__ basis,
-                                                             const float* __restrict__ dx,
-                                                             float4* __restrict__ nbrBasis,
-                                                             float* __restrict__ nbrDx) {
-    size_t t = size_t(blockIdx.x) * kThreads + threadIdx.x;
-    if (t >= size_t(27) * nNodes) return;
-    int node = int(t / 27), i = int(t - size_t(node) * 27);
-    int nb = neigh[i * M + node];
-    nbrBasis[t] = nb < 0 ? make_float4(0.f, 0.f, 0.f, 0.f) : basis[nb];
-    nbrDx[t] = nb < 0 ? 0.f : dx[nb];
-}
-
 // markValidMeshVertexIndex (:1170-1181): a vertex with |x| < 1e-6 is dropped;
 // triangle indices are not remapped (insertTriangle :1247-1282).
 __global__ void __launch_bounds__(kThreads) keep_flags(const float* __restrict__ v, int n, int* __restrict__ keep) {
@@ -1167,7 +1150,7 @@ int grid_tile_words(int D, int rd, int* haloOut = nullptr) {
         fac += x * kTx + y * kTy + z * kTz;
     }
     if (haloOut) *haloOut = halo;
-    return 6 * 10 * 10 + halo + fac + ((27 * (rd + 1) + 3) & ~3) * kTw;
+    return halo + fac + ((27 * (rd + 1) + 3) & ~3) * kTw;
 }
 
 // Default depth-D virtual cells per chunk of subdivision sets; a set larger
@@ -1246,11 +1229,8 @@ void extract_mesh(Ctx& c) {
         const size_t chunkCells = std::min<size_t>(c.cfg.subdiv_chunk_nodes ? c.cfg.subdiv_chunk_nodes : kChunkCells,
                                                    size_t(INT32_MAX - 1) / 12);
         float* isoDev = &c.results()->iso;
-        const int nAnc = std::max(1, L.base[D - 1]);  // nodes above depth D-1
-        float4* nbrBasis = c.buf<float4>("sub_nbr_basis", size_t(27) * nAnc);
-        float* nbrDx = c.buf<float>("sub_nbr_dx", size_t(27) * nAnc);
-        neighbour_blocks<<<div_up(size_t(27) * nAnc, kThreads), kThreads, 0, st>>>(nAnc, M, o.neigh, o.basis, c.fld.dx,
-                                                                                  nbrBasis, nbrDx);
+        const float4* nbrBasis = c.fld.nbrBasis;
+        const float* nbrDx = c.fld.nbrDx;
         int* rootNbr = c.buf<int>("sub_root_nbr", size_t(27) * mm.nRoots);
         int* rootChain = c.buf<int>("sub_root_chain", size_t(kChain) * mm.nRoots);
         int4* rootCell = c.buf<int4>("sub_root_cell", mm.nRoots);
