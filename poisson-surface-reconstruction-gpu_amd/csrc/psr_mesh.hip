@@ -18,10 +18,28 @@ struct NodeView {
     const int* key;
     const int* depth;
     const int* neigh;
+    const float4* basis;  // per node: centre (x, y, z), 2^depth
 };
 
+// Node centre (BuildOctree.cu:568-586): the exact dyadic (cell + 1/2) / 2^depth,
+// which the basis parameters hold bit for bit.
 __device__ __forceinline__ float3 node_center(const NodeView& v, int o) {
-    return key_center(v.key[o], v.depth[o], v.D);
+    const float4 b = v.basis[o];
+    return make_float3(b.x, b.y, b.z);
+}
+
+// Centres and keys of the 27 neighbours of o (absent: key INT_MAX), loaded together.
+__device__ __forceinline__ void load_neighbourhood(const NodeView& v, int o, int (&nb)[27], float3 (&c)[27],
+                                                   int (&k)[27]) {
+#pragma unroll
+    for (int i = 0; i < 27; ++i) nb[i] = v.neigh[i * v.M + o];
+#pragma unroll
+    for (int i = 0; i < 27; ++i) {
+        const bool ok = nb[i] != -1;
+        const float4 b = ok ? v.basis[nb[i]] : make_float4(0.f, 0.f, 0.f, 0.f);
+        c[i] = make_float3(b.x, b.y, b.z);
+        k[i] = ok ? v.key[nb[i]] : 0x7fffffff;
+    }
 }
 
 __global__ void __launch_bounds__(kThreads) vertex_owner(NodeView v, int* __restrict__ flag) {
@@ -29,22 +47,23 @@ __global__ void __launch_bounds__(kThreads) vertex_owner(NodeView v, int* __rest
     if (o >= v.M) return;
     int depth = v.depth[o];
     float half = 1.0f / float(1 << (depth + 1)), W = 1.0f / float(1 << depth), W2 = W * W;
-    int nb[27];
-    for (int i = 0; i < 27; ++i) nb[i] = v.neigh[i * v.M + o];
-    float3 nc = nb[13] != -1 ? node_center(v, nb[13]) : make_float3(0.f, 0.f, 0.f);
+    int nb[27], kk[27];
+    float3 cc[27];
+    load_neighbourhood(v, o, nb, cc, kk);
+    const float3 nc = cc[13];  // (0, 0, 0) when the node has no self-neighbour
     int ownKey[8], own[8];
     for (int i = 0; i < 8; ++i) {
         ownKey[i] = 0x7fffffff;
         own[i] = -1;
     }
+#pragma unroll
     for (int j = 0; j < 27; ++j) {
         if (nb[j] == -1) continue;
-        float3 c = node_center(v, nb[j]);
-        int k = v.key[nb[j]];
+#pragma unroll
         for (int i = 0; i < 8; ++i) {
             float3 p = corner_pos(nc, i, half);
-            if (sqdist(p, c) < double(W2) && ownKey[i] > k) {
-                ownKey[i] = k;
+            if (sqdist(p, cc[j]) < double(W2) && ownKey[i] > kk[j]) {
+                ownKey[i] = kk[j];
                 own[i] = nb[j];
             }
         }
@@ -80,12 +99,13 @@ __global__ void __launch_bounds__(kThreads) vertex_maintain(NodeView v, const in
     float W = 1.0f / float(1 << depth), W2 = W * W;
     float4 q = vPos[t];
     float3 p = make_float3(q.x, q.y, q.z);
+    int nb[27], kk[27];
+    float3 cc[27];
+    load_neighbourhood(v, owner, nb, cc, kk);
+#pragma unroll
     for (int i = 0; i < 27; ++i) {
-        int nb = v.neigh[i * v.M + owner];
-        if (nb == -1) continue;
-        float3 c = node_center(v, nb);
-        if (!(sqdist(p, c) < double(W2))) continue;
-        nodeVert[corner_slot(c, p) * v.M + nb] = t + 1;
+        if (nb[i] == -1 || !(sqdist(p, cc[i]) < double(W2))) continue;
+        nodeVert[corner_slot(cc[i], p) * v.M + nb[i]] = t + 1;
     }
 }
 
@@ -94,22 +114,23 @@ __global__ void __launch_bounds__(kThreads) edge_owner(NodeView v, int offD, int
     if (t >= nD) return;
     int o = offD + t;
     float half = 1.0f / float(1 << (v.D + 1)), W = 1.0f / float(1 << v.D), W2 = W * W;
-    int nb[27];
-    for (int i = 0; i < 27; ++i) nb[i] = v.neigh[i * v.M + o];
-    float3 nc = nb[13] != -1 ? node_center(v, nb[13]) : make_float3(0.f, 0.f, 0.f);
+    int nb[27], kk[27];
+    float3 cc[27];
+    load_neighbourhood(v, o, nb, cc, kk);
+    const float3 nc = cc[13];
     int ownKey[12], own[12];
     for (int e = 0; e < 12; ++e) {
         ownKey[e] = 0x7fffffff;
         own[e] = -1;
     }
+#pragma unroll
     for (int j = 0; j < 27; ++j) {
         if (nb[j] == -1) continue;
-        float3 c = node_center(v, nb[j]);
-        int k = v.key[nb[j]];
+#pragma unroll
         for (int e = 0; e < 12; ++e) {
             float3 p = edge_center(nc, e, half);
-            if (sqdist(p, c) < double(W2) && ownKey[e] > k) {
-                ownKey[e] = k;
+            if (sqdist(p, cc[j]) < double(W2) && ownKey[e] > kk[j]) {
+                ownKey[e] = kk[j];
                 own[e] = nb[j];
             }
         }
@@ -135,15 +156,14 @@ __global__ void __launch_bounds__(kThreads) edge_maintain(NodeView v, int offD, 
     if (t >= *count) return;
     int owner = eOwner[t], kind = eKind[t];
     float half = 1.0f / float(1 << (v.D + 1)), W = 1.0f / float(1 << v.D), W2 = W * W;
-    int self = v.neigh[13 * v.M + owner];
-    float3 nc = self != -1 ? node_center(v, self) : make_float3(0.f, 0.f, 0.f);
-    float3 p = edge_center(nc, kind, half);
+    int nb[27], kk[27];
+    float3 cc[27];
+    load_neighbourhood(v, owner, nb, cc, kk);
+    const float3 p = edge_center(cc[13], kind, half);
+#pragma unroll
     for (int i = 0; i < 27; ++i) {
-        int nb = v.neigh[i * v.M + owner];
-        if (nb == -1) continue;
-        float3 c = node_center(v, nb);
-        if (!(sqdist(p, c) < double(W2))) continue;
-        nodeEdge[edge_slot(c, p, kind) * nD + (nb - offD)] = t + 1;
+        if (nb[i] == -1 || !(sqdist(p, cc[i]) < double(W2))) continue;
+        nodeEdge[edge_slot(cc[i], p, kind) * nD + (nb[i] - offD)] = t + 1;
     }
 }
 
@@ -153,22 +173,23 @@ __global__ void __launch_bounds__(kThreads) face_owner(NodeView v, const int* __
     if (o >= v.M) return;
     int depth = v.depth[o];
     float half = 1.0f / float(1 << (depth + 1)), W = 1.0f / float(1 << depth), W2 = W * W;
-    int nb[27];
-    for (int i = 0; i < 27; ++i) nb[i] = v.neigh[i * v.M + o];
-    float3 nc = nb[13] != -1 ? node_center(v, nb[13]) : make_float3(0.f, 0.f, 0.f);
+    int nb[27], kk[27];
+    float3 cc[27];
+    load_neighbourhood(v, o, nb, cc, kk);
+    const float3 nc = cc[13];
     int ownKey[6], own[6];
     for (int f = 0; f < 6; ++f) {
         ownKey[f] = 0x7fffffff;
         own[f] = -1;
     }
+#pragma unroll
     for (int j = 0; j < 27; ++j) {
         if (nb[j] == -1) continue;
-        float3 c = node_center(v, nb[j]);
-        int k = v.key[nb[j]];
+#pragma unroll
         for (int f = 0; f < 6; ++f) {
             float3 p = face_center(nc, f, half);
-            if (sqdist(p, c) < double(W2) && ownKey[f] > k) {
-                ownKey[f] = k;
+            if (sqdist(p, cc[j]) < double(W2) && ownKey[f] > kk[j]) {
+                ownKey[f] = kk[j];
                 own[f] = nb[j];
             }
         }
@@ -198,17 +219,16 @@ __global__ void __launch_bounds__(kThreads) face_maintain(NodeView v, const int*
     int owner = fOwner[t], kind = fKind[t], orient = kind >> 1;
     int depth = v.depth[owner];
     float half = 1.0f / float(1 << (depth + 1)), W = 1.0f / float(1 << depth), W2 = W * W;
-    int self = v.neigh[13 * v.M + owner];
-    float3 nc = self != -1 ? node_center(v, self) : make_float3(0.f, 0.f, 0.f);
-    float3 p = face_center(nc, kind, half);
+    int nb[27], kk[27];
+    float3 cc[27];
+    load_neighbourhood(v, owner, nb, cc, kk);
+    const float3 p = face_center(cc[13], kind, half);
+#pragma unroll
     for (int i = 0; i < 27; ++i) {
-        int nb = v.neigh[i * v.M + owner];
-        if (nb == -1) continue;
-        float3 c = node_center(v, nb);
-        if (!(sqdist(p, c) < double(W2))) continue;
-        float cv = orient == 0 ? c.x : (orient == 1 ? c.y : c.z);
-        float pv = orient == 0 ? p.x : (orient == 1 ? p.y : p.z);
-        nodeFace[((orient << 1) + (cv - pv < 0 ? 1 : 0)) * v.M + nb] = t + 1;
+        if (nb[i] == -1 || !(sqdist(p, cc[i]) < double(W2))) continue;
+        const float cv = orient == 0 ? cc[i].x : (orient == 1 ? cc[i].y : cc[i].z);
+        const float pv = orient == 0 ? p.x : (orient == 1 ? p.y : p.z);
+        nodeFace[((orient << 1) + (cv - pv < 0 ? 1 : 0)) * v.M + nb[i]] = t + 1;
     }
 }
 
@@ -417,7 +437,7 @@ void build_geometry(Ctx& c) {
     const Levels& L = o.L;
     const int M = o.M, D = L.D, nD = L.count[D], offD = L.base[D];
     hipStream_t s = c.stream;
-    NodeView v{M, D, o.key, o.depth, o.neigh};
+    NodeView v{M, D, o.key, o.depth, o.neigh, o.basis};
     size_t nMax = std::max<size_t>(size_t(8) * M, size_t(12) * nD);
     int* flag = c.buf<int>("geo_flag", nMax);
     int* addr = c.buf<int>("geo_addr", nMax);

@@ -59,6 +59,7 @@ struct Grid {
     const int* rootNbr;      // [root][27] position in roots of the root's real neighbour, or -1
     const int* rootChain;    // [root][kChain] the root, then its ancestors (-1 past the tree root)
     const int4* rootCell;    // [root] cell at the root depth (x, y, z) and the root node (w)
+    const unsigned* rootSet; // [root] bit i: neighbour i of the root belongs to the root's set
     const float* dxAt;       // per depth l, at (8^l - 1) / 7: d_x of the node at each cell, x-major; 0 where none
 };
 constexpr int kChain = PSR_MAX_DEPTH + 1;
@@ -161,6 +162,58 @@ __device__ __forceinline__ bool inside_root(const Leaf& L, int lo, int hi, int a
     return (lo >> L.sh) == axisRoot && (hi >> L.sh) == axisRoot;
 }
 
+// Key order of the cells sharing a grid point p: the cells p - d (d in {0,1}
+// per axis) differ in axis a only in bits 0..ctz(p_a), so two of them compare
+// on their differing axis of highest ctz (x before y before z at the same
+// level), where the one with d_a = 1 has the smaller key.  The cells in
+// increasing key order are therefore d in decreasing lexicographic order over
+// the axes ranked by axis_rank; p_a = 0 (no cell below) ranks last.
+__device__ __forceinline__ int axis_rank(int pa, int prio) { return pa > 0 ? ((__builtin_ctz(pa) << 2) | prio) : -1; }
+
+// Axes of p by decreasing rank, as bit positions (0 = x, 1 = y, 2 = z).
+__device__ __forceinline__ void rank_axes(int3 p, int& a0, int& a1, int& a2) {
+    int r0 = axis_rank(p.x, 2), r1 = axis_rank(p.y, 1), r2 = axis_rank(p.z, 0);
+    a0 = 0;
+    a1 = 1;
+    a2 = 2;
+    if (r1 > r0) {
+        int t = r0; r0 = r1; r1 = t;
+        t = a0; a0 = a1; a1 = t;
+    }
+    if (r2 > r1) {
+        int t = r1; r1 = r2; r2 = t;
+        t = a1; a1 = a2; a2 = t;
+    }
+    if (r1 > r0) {
+        int t = a0; a0 = a1; a1 = t;
+    }
+}
+
+// Whether depth-D cell n is virtual (a look-up may answer this more cheaply
+// than the enumeration).
+template <class VE>
+__device__ __forceinline__ bool ve_valid(const VE& ve, int3 n) {
+    return ve(n) >= 0;
+}
+
+// The virtual cell of smallest key among those sharing grid point p (and its
+// offset d as a bit mask, bit a = d_a), or -1.
+template <class VE>
+__device__ __forceinline__ int point_owner(int3 p, int& dmask, const VE& ve) {
+    int a0, a1, a2;
+    rank_axes(p, a0, a1, a2);
+    for (int j = 7; j >= 0; --j) {
+        const int m = (((j >> 2) & 1) << a0) | (((j >> 1) & 1) << a1) | ((j & 1) << a2);
+        const int3 n = make_int3(p.x - (m & 1), p.y - ((m >> 1) & 1), p.z - ((m >> 2) & 1));
+        if (ve_valid(ve, n)) {
+            dmask = m;
+            return ve(n);
+        }
+    }
+    dmask = 0;
+    return -1;
+}
+
 // Owner (enumeration, Bourke slot within the owner) of the grid point c + off(b);
 // ve(n) is the enumeration of depth-D cell n if it is virtual in L's set, else -1.
 template <class VE>
@@ -174,20 +227,9 @@ __device__ __forceinline__ int corner_owner(const Leaf& L, int D, int b, int& sl
         int3 lc = make_int3(p.x - 1 - (L.Rc.x << L.sh), p.y - 1 - (L.Rc.y << L.sh), p.z - 1 - (L.Rc.z << L.sh));
         return L.S.dEnum + (L.q << (3 * L.sh)) + morton(lc, L.sh);
     }
-    int best = -1, bestKey = 0x7fffffff;
-    slot = 0;
-    for (int k = 0; k < 8; ++k) {
-        int3 d = make_int3(k & 1, (k >> 1) & 1, (k >> 2) & 1);
-        int3 n = make_int3(p.x - d.x, p.y - d.y, p.z - d.z);
-        int v = ve(n);
-        if (v < 0) continue;
-        int key = morton(n, D);
-        if (key < bestKey) {
-            bestKey = key;
-            best = v;
-            slot = bourke_slot(d.x, d.y, d.z);
-        }
-    }
+    int m;
+    const int best = point_owner(p, m, ve);
+    slot = best < 0 ? 0 : bourke_slot(m & 1, (m >> 1) & 1, (m >> 2) & 1);
     return best;
 }
 
@@ -209,25 +251,24 @@ __device__ __forceinline__ int edge_owner(const Leaf& L, int D, int k, int& kind
         int3 lc = make_int3(n[0] - (L.Rc.x << L.sh), n[1] - (L.Rc.y << L.sh), n[2] - (L.Rc.z << L.sh));
         return L.S.dEnum + (L.q << (3 * L.sh)) + morton(lc, L.sh);
     }
-    int best = -1, bestKey = 0x7fffffff;
+    // the 4 cells in increasing key order (axis_rank of the two perpendicular
+    // axes at the point lo + 1); b = 1: the cell below the edge in that axis
+    const bool firstU1 = axis_rank(lo1 + 1, 2 - u1) > axis_rank(lo0 + 1, 2 - u0);
     kind = 0;
-    for (int t = 0; t < 4; ++t) {
-        int b0 = t & 1, b1 = (t >> 1) & 1;  // 1: the cell below the edge in that axis
+    for (int j = 3; j >= 0; --j) {
+        const int hi = (j >> 1) & 1, lo = j & 1;
+        const int b0 = firstU1 ? lo : hi, b1 = firstU1 ? hi : lo;
         int n[3];
         n[a] = cc[a];
-        n[u0] = b0 ? lo0 : lo0 + 1;
-        n[u1] = b1 ? lo1 : lo1 + 1;
-        int3 nn = make_int3(n[0], n[1], n[2]);
-        int v = ve(nn);
-        if (v < 0) continue;
-        int key = morton(nn, D);
-        if (key < bestKey) {
-            bestKey = key;
-            best = v;
+        n[u0] = lo0 + 1 - b0;
+        n[u1] = lo1 + 1 - b1;
+        const int3 nn = make_int3(n[0], n[1], n[2]);
+        if (ve_valid(ve, nn)) {
             kind = (a << 2) | b0 | (b1 << 1);
+            return ve(nn);
         }
     }
-    return best;
+    return -1;
 }
 
 __device__ __forceinline__ float grid_term(const Grid& g, int nb, float3 p) { return node_term(g.basis[nb], g.dx[nb], p); }
@@ -398,6 +439,11 @@ struct RecVE {
     SetDesc S;
     int3 Rc;
     int q, sh, lim;
+    unsigned inSet;  // bit i: neighbour cube i of the root belongs to the set (bit 13: the root)
+    __device__ __forceinline__ bool valid(int3 n) const {
+        if (n.x < 0 || n.y < 0 || n.z < 0 || n.x >= lim || n.y >= lim || n.z >= lim) return false;
+        return (inSet >> nbr_index((n.x >> sh) - Rc.x, (n.y >> sh) - Rc.y, (n.z >> sh) - Rc.z)) & 1u;
+    }
     __device__ __forceinline__ int operator()(int3 n) const {
         if (n.x < 0 || n.y < 0 || n.z < 0 || n.x >= lim || n.y >= lim || n.z >= lim) return -1;
         const int3 A = make_int3(n.x >> sh, n.y >> sh, n.z >> sh);
@@ -411,9 +457,10 @@ struct RecVE {
         return S.dEnum + (qq << (3 * sh)) + morton(lc, sh);
     }
 };
-__device__ __forceinline__ RecVE rec_ve(const int* nbr, const Leaf& L, int D) {
-    return RecVE{nbr, L.S, L.Rc, L.q, L.sh, 1 << D};
+__device__ __forceinline__ RecVE rec_ve(const int* nbr, unsigned inSet, const Leaf& L, int D) {
+    return RecVE{nbr, L.S, L.Rc, L.q, L.sh, 1 << D, inSet};
 }
+__device__ __forceinline__ bool ve_valid(const RecVE& ve, int3 n) { return ve.valid(n); }
 
 // Virtual-cell lookup in the tile's depth-D box of this cell's root.
 struct BoxVE {
@@ -660,21 +707,10 @@ __global__ void __launch_bounds__(NT) sub_corner_values(Grid g, const Tile* __re
         const TileRoot R = lRoots[rr];
         const int3 pt = make_int3(R.first.x + pi / (n * n), R.first.y + (pi / n) % n, R.first.z + pi % n);
         const BoxVE ve = box_ve(lVbox, tg, rr, R.first);
-        int best = -1, bestKey = 0x7fffffff, slot = 0;
-        int3 bestCell = pt;
-        for (int k = 0; k < 8; ++k) {
-            int3 d = make_int3(k & 1, (k >> 1) & 1, (k >> 2) & 1);
-            int3 c = make_int3(pt.x - d.x, pt.y - d.y, pt.z - d.z);
-            int v = ve(c);
-            if (v < 0) continue;
-            int key = morton(c, D);
-            if (key < bestKey) {
-                bestKey = key;
-                best = v;
-                slot = bourke_slot(d.x, d.y, d.z);
-                bestCell = c;
-            }
-        }
+        int m;
+        const int best = point_owner(pt, m, ve);
+        const int slot = bourke_slot(m & 1, (m >> 1) & 1, (m >> 2) & 1);
+        const int3 bestCell = make_int3(pt.x - (m & 1), pt.y - ((m >> 1) & 1), pt.z - ((m >> 2) & 1));
         if (best < 0 || ((best - S.dEnum) >> (3 * sh)) != tg.q0 + rr) return;
         Leaf Lo = root_leaf(g, S, T.set, R);
         Lo.c = bestCell;
@@ -710,9 +746,11 @@ __global__ void __launch_bounds__(kThreads) sub_corner_values_grid(Grid g, const
     __shared__ int lNbr[27];
     __shared__ int lChain[kChain];
     __shared__ TileRoot lRoot[1];
+    __shared__ unsigned lSet;
     float* lds = reinterpret_cast<float*>(ldsDyn);
     const Tile T = tiles[blockIdx.x];
     if (threadIdx.x < 27) lNbr[threadIdx.x] = g.rootNbr[27 * size_t(T.root) + threadIdx.x];
+    if (threadIdx.x == 32) lSet = g.rootSet[T.root];
     if (threadIdx.x < kChain) lChain[threadIdx.x] = g.rootChain[kChain * size_t(T.root) + threadIdx.x];
     const SetDesc S = g.sets[T.set];
     const int D = g.D;
@@ -813,7 +851,7 @@ __global__ void __launch_bounds__(kThreads) sub_corner_values_grid(Grid g, const
     if (int(threadIdx.x) >= T.count) return;
     const int e = T.start + threadIdx.x;
     const Leaf L = leaf_of(g, T.set, S, R, e);
-    const RecVE ve = rec_ve(lNbr, L, D);
+    const RecVE ve = rec_ve(lNbr, lSet, L, D);
     const float isoV = *iso;
     const int sh = D - S.rd, side = 1 << sh;
     const int3 lc = make_int3(L.c.x - (R.Rc.x << sh), L.c.y - (R.Rc.y << sh), L.c.z - (R.Rc.z << sh));
@@ -921,6 +959,7 @@ __global__ void __launch_bounds__(kThreads) sub_classify(Grid g, const Tile* __r
                                                          int* __restrict__ crossPrefix, int4* __restrict__ tileCnt) {
     __shared__ TileRoot lRoots[kMaxRootsPerTile];
     __shared__ int lNbr[kMaxRootsPerTile * 27];
+    __shared__ unsigned lSet[kMaxRootsPerTile];
     __shared__ int lSum[kThreads / PSR_WAVE];
     const Tile T = tiles[blockIdx.x];
     const SetDesc S = g.sets[T.set];
@@ -928,6 +967,7 @@ __global__ void __launch_bounds__(kThreads) sub_classify(Grid g, const Tile* __r
     const TileGeom tg = tile_geom(T, S, D);
     fill_roots_rec(g, T, S, tg, lRoots);
     for (int t = threadIdx.x; t < tg.nRoots * 27; t += kThreads) lNbr[t] = g.rootNbr[27 * size_t(T.root) + t];
+    if (int(threadIdx.x) < tg.nRoots) lSet[threadIdx.x] = g.rootSet[T.root + threadIdx.x];
     __syncthreads();
     const bool active = int(threadIdx.x) < T.count;
     const int e = T.start + (active ? int(threadIdx.x) : 0);
@@ -936,7 +976,7 @@ __global__ void __launch_bounds__(kThreads) sub_classify(Grid g, const Tile* __r
     if (active) {
         const int rr = tg.oneRoot ? 0 : int(threadIdx.x) >> (3 * (D - S.rd));
         const Leaf L = leaf_of(g, T.set, S, lRoots[rr], e);
-        const RecVE ve = rec_ve(lNbr + 27 * rr, L, D);
+        const RecVE ve = rec_ve(lNbr + 27 * rr, lSet[rr], L, D);
         float v[8];
         int cls = 0;
 #pragma unroll
@@ -1063,12 +1103,17 @@ __global__ void __launch_bounds__(kThreads) sub_emit(Grid g, const Tile* __restr
 
 // Per root: the position in roots of each real neighbour (or -1), and the
 // chain of the root and its ancestors.
-__global__ void __launch_bounds__(kThreads) root_records(int nRoots, int M, Levels Lv, const int* __restrict__ roots,
+// Its set: coarse roots (depth < finerDepth) form sets of one; the finer roots
+// of a depth form one set, so a neighbour belongs to a finer root's set iff it
+// is a root (it has the root's depth).
+__global__ void __launch_bounds__(kThreads) root_records(int nRoots, int M, Levels Lv, int finerDepth,
+                                                         const int* __restrict__ roots,
                                                          const int* __restrict__ rootPos,
                                                          const int* __restrict__ neigh,
                                                          const int* __restrict__ parent,
                                                          const int* __restrict__ key, int* __restrict__ rootNbr,
-                                                         int* __restrict__ rootChain, int4* __restrict__ rootCell) {
+                                                         int* __restrict__ rootChain, int4* __restrict__ rootCell,
+                                                         unsigned* __restrict__ rootSet) {
     const size_t t = size_t(blockIdx.x) * kThreads + threadIdx.x;
     if (t >= size_t(27) * nRoots) return;
     const int q = int(t / 27), i = int(t - size_t(q) * 27);
@@ -1080,6 +1125,16 @@ __global__ void __launch_bounds__(kThreads) root_records(int nRoots, int M, Leve
         while (rd < Lv.D && root >= Lv.base[rd + 1]) ++rd;
         const int3 c = demorton(key[root] >> (3 * (Lv.D - rd)), rd);
         rootCell[q] = make_int4(c.x, c.y, c.z, root);
+        unsigned m = 1u << 13;
+        if (rd >= finerDepth) {
+            int nbs[27];
+#pragma unroll
+            for (int k = 0; k < 27; ++k) nbs[k] = neigh[k * M + root];
+#pragma unroll
+            for (int k = 0; k < 27; ++k)
+                if (nbs[k] >= 0 && rootPos[nbs[k]] >= 0) m |= 1u << k;
+        }
+        rootSet[q] = m;
     }
     if (i == 0) {
         int node = root;
@@ -1234,8 +1289,10 @@ void extract_mesh(Ctx& c) {
         int* rootNbr = c.buf<int>("sub_root_nbr", size_t(27) * mm.nRoots);
         int* rootChain = c.buf<int>("sub_root_chain", size_t(kChain) * mm.nRoots);
         int4* rootCell = c.buf<int4>("sub_root_cell", mm.nRoots);
+        unsigned* rootSet = c.buf<unsigned>("sub_root_set", mm.nRoots);
         root_records<<<div_up(size_t(27) * mm.nRoots, kThreads), kThreads, 0, st>>>(
-            mm.nRoots, M, L, mm.roots, rootPos, o.neigh, o.parent, o.key, rootNbr, rootChain, rootCell);
+            mm.nRoots, M, L, finerDepth, mm.roots, rootPos, o.neigh, o.parent, o.key, rootNbr, rootChain, rootCell,
+            rootSet);
         // one-root tiles read d_x from dense per-depth grids when the octree is geometric
         const bool denseHalo = !mm.q1Fired && D - sets[0].rd >= 3;
         for (const SetDesc& S : sets) {
@@ -1308,7 +1365,7 @@ void extract_mesh(Ctx& c) {
             Tile* dtiles = reinterpret_cast<Tile*>(dAll + bSets);
             Tile* dsmall = reinterpret_cast<Tile*>(dAll + bSets + bTiles);
             Grid gr{D,       M,       nCs,      dsets,    mm.roots, rootPos,   o.key,    o.parent, o.childBase,
-                    o.neigh, o.basis, c.fld.dx, nbrBasis, nbrDx,    rootNbr,   rootChain, rootCell, dxAt};
+                    o.neigh, o.basis, c.fld.dx, nbrBasis, nbrDx,    rootNbr,   rootChain, rootCell, rootSet, dxAt};
             float* cval = c.buf<float>("sub_cval", size_t(8) * nVD);
             float* vals = c.buf<float>("sub_vals", size_t(8) * nVD);
             unsigned* cellWord = c.buf<unsigned>("sub_cell_word", nVD);
