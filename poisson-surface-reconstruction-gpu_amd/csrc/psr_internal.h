@@ -1,11 +1,15 @@
 // Reconstruction context shared by the stage drivers.
 #pragma once
+#include <cstddef>
 #include <map>
 #include <string>
 #include <vector>
 
 #include "../../include/psr.h"
 #include "psr_common.h"
+#include "psr_prims.h"
+
+#include <algorithm>
 
 #include <hip/hip_fp16.h>
 
@@ -113,6 +117,8 @@ struct DevResults {
     double cgResid[PSR_MAX_DEPTH + 1];     // final updated residual norm per depth
     double cgTrueRel[PSR_MAX_DEPTH + 1];   // mixed form: final true relative residual per depth
 };
+static_assert(offsetof(DevResults, cgErr) == sizeof(int) * (3 * (PSR_MAX_DEPTH + 1)),
+              "iterations, nnz and the error flag are cleared as one range");
 
 struct Ctx {
     psr_config cfg{};
@@ -155,6 +161,21 @@ struct Ctx {
             hostStageBytes = bytes + bytes / 2 + 4096;
         }
         return hostStage;
+    }
+    ScanState scan;
+    // The stream's scan state, able to scan n elements (zeroed when it grows).
+    ScanState& scan_for(size_t n) {
+        const size_t tiles = scan_tiles(n);
+        if (tiles > scan.capacity || scan.epoch + 1 >= kScanEpochMax) {
+            const size_t cap = std::max<size_t>(tiles + tiles / 2 + 64, 1024);
+            scan.status = static_cast<unsigned long long*>(arena.get("scan_status", cap * sizeof(unsigned long long)));
+            scan.counters = static_cast<unsigned*>(arena.get("scan_counters", kScanRing * sizeof(unsigned)));
+            PSR_HIP_CHECK(hipMemsetAsync(scan.status, 0, cap * sizeof(unsigned long long), stream));
+            PSR_HIP_CHECK(hipMemsetAsync(scan.counters, 0, kScanRing * sizeof(unsigned), stream));
+            scan.capacity = cap;
+            scan.epoch = 0;
+        }
+        return scan;
     }
     int* scratch_for(size_t ints) {
         if (ints > scratchInts) {

@@ -441,7 +441,6 @@ void build_geometry(Ctx& c) {
     size_t nMax = std::max<size_t>(size_t(8) * M, size_t(12) * nD);
     int* flag = c.buf<int>("geo_flag", nMax);
     int* addr = c.buf<int>("geo_addr", nMax);
-    int* scratch = c.scratch_for(scan_scratch_ints(nMax) + 64);
     g.counts = c.buf<int>("geo_counts", 4);
     g.vtxOwner = c.buf<int>("vtx_owner", size_t(8) * M);
     g.vtxKind = c.buf<int>("vtx_kind", size_t(8) * M);
@@ -449,12 +448,10 @@ void build_geometry(Ctx& c) {
     g.nodeVert = c.buf<int>("node_vert", size_t(8) * M);
     g.nodeFace = c.buf<int>("node_face", size_t(6) * M);
     g.nodeEdge = c.buf<int>("node_edge", size_t(12) * nD);
-    zero_ints<<<div_up(size_t(8) * M, kThreads), kThreads, 0, s>>>(g.nodeVert, size_t(8) * M);
-    zero_ints<<<div_up(size_t(6) * M, kThreads), kThreads, 0, s>>>(g.nodeFace, size_t(6) * M);
-    zero_ints<<<div_up(size_t(12) * nD, kThreads), kThreads, 0, s>>>(g.nodeEdge, size_t(12) * nD);
+    zero_spans({{g.nodeVert, size_t(8) * M}, {g.nodeFace, size_t(6) * M}, {g.nodeEdge, size_t(12) * nD}}, s);
     // vertices (all depths)
     vertex_owner<<<div_up(M, kThreads), kThreads, 0, s>>>(v, flag);
-    scan_exclusive(flag, addr, 8 * M, g.counts + 0, scratch, s);
+    scan_exclusive(flag, addr, 8 * M, g.counts + 0, c.scan_for(size_t(8) * M), s);
     vertex_emit<<<div_up(size_t(8) * M, kThreads), kThreads, 0, s>>>(v, flag, addr, g.vtxOwner, g.vtxKind, g.vtxPos);
     vertex_maintain<<<div_up(size_t(8) * M, kThreads), kThreads, 0, s>>>(v, g.counts + 0, g.vtxOwner, g.vtxPos,
                                                                         g.nodeVert);
@@ -462,7 +459,7 @@ void build_geometry(Ctx& c) {
     g.edgeOwner = c.buf<int>("edge_owner", size_t(12) * nD);
     g.edgeKind = c.buf<int>("edge_kind", size_t(12) * nD);
     edge_owner<<<div_up(nD, kThreads), kThreads, 0, s>>>(v, offD, nD, flag);
-    scan_exclusive(flag, addr, 12 * nD, g.counts + 1, scratch, s);
+    scan_exclusive(flag, addr, 12 * nD, g.counts + 1, c.scan_for(size_t(12) * nD), s);
     edge_emit<<<div_up(size_t(12) * nD, kThreads), kThreads, 0, s>>>(offD, nD, flag, addr, g.edgeOwner, g.edgeKind);
     edge_maintain<<<div_up(size_t(12) * nD, kThreads), kThreads, 0, s>>>(v, offD, g.counts + 1, g.edgeOwner,
                                                                         g.edgeKind, nD, g.nodeEdge);
@@ -471,7 +468,7 @@ void build_geometry(Ctx& c) {
     g.faceKind = c.buf<int>("face_kind", size_t(6) * M);
     g.faceHasParent = c.buf<int>("face_hasparent", size_t(6) * M);
     face_owner<<<div_up(M, kThreads), kThreads, 0, s>>>(v, o.parent, flag);
-    scan_exclusive(flag, addr, 6 * M, g.counts + 2, scratch, s);
+    scan_exclusive(flag, addr, 6 * M, g.counts + 2, c.scan_for(size_t(6) * M), s);
     face_emit<<<div_up(size_t(6) * M, kThreads), kThreads, 0, s>>>(v, o.parent, flag, addr, g.faceOwner, g.faceKind,
                                                                    g.faceHasParent);
     face_maintain<<<div_up(size_t(6) * M, kThreads), kThreads, 0, s>>>(v, g.counts + 2, g.faceOwner, g.faceKind,
@@ -497,14 +494,14 @@ void depth_mc(Ctx& c, MainMc& mm, float* rawV, int* rawT) {
     const int M = o.M, D = L.D, nD = L.count[D], offD = L.base[D];
     hipStream_t s = c.stream;
     const int E = g.E, F = g.F;
-    int* scratch = c.scratch_for(scan_scratch_ints(std::max<size_t>(size_t(E), size_t(M))) + 64);
     mm.counts = c.buf<int>("mc_counts", PSR_MAX_DEPTH + 8);
-    zero_ints<<<1, kThreads, 0, s>>>(mm.counts, PSR_MAX_DEPTH + 8);
+    mm.hasInter = c.buf<int>("mc_hasinter", F);
+    zero_spans({{mm.counts, size_t(PSR_MAX_DEPTH + 8)}, {mm.hasInter, size_t(F)}}, s);
     int* cflag = c.buf<int>("mc_cross_flag", E);
     mm.vexAddr = c.buf<int>("mc_vex_addr", E);
     edge_crossings<<<div_up(E, kThreads), kThreads, 0, s>>>(g.counts + 1, M, g.edgeOwner, g.edgeKind, g.nodeVert,
                                                             g.vvalue, cflag);
-    scan_exclusive(cflag, mm.vexAddr, E, mm.counts + 0, scratch, s);
+    scan_exclusive(cflag, mm.vexAddr, E, mm.counts + 0, c.scan_for(size_t(E)), s);
     crossing_points<<<div_up(E, kThreads), kThreads, 0, s>>>(g.counts + 1, M, g.edgeOwner, g.edgeKind, g.nodeVert,
                                                              g.vvalue, g.vtxPos, cflag, mm.vexAddr, rawV);
     mm.cube = c.buf<int>("mc_cube", nD);
@@ -512,9 +509,7 @@ void depth_mc(Ctx& c, MainMc& mm, float* rawV, int* rawT) {
     mm.triAddr = c.buf<int>("mc_triaddr", nD);
     cube_classes<<<div_up(nD, kThreads), kThreads, 0, s>>>(offD, nD, M, g.nodeVert, g.nodeEdge, cflag, g.vvalue,
                                                            mm.cube, triNum);
-    scan_exclusive(triNum, mm.triAddr, nD, mm.counts + 1, scratch, s);
-    mm.hasInter = c.buf<int>("mc_hasinter", F);
-    zero_ints<<<div_up(F, kThreads), kThreads, 0, s>>>(mm.hasInter, size_t(F));
+    scan_exclusive(triNum, mm.triAddr, nD, mm.counts + 1, c.scan_for(size_t(nD)), s);
     emit_triangles_and_faces<<<div_up(nD, kThreads), kThreads, 0, s>>>(offD, nD, M, mm.cube, mm.triAddr, g.nodeEdge,
                                                                        mm.vexAddr, g.nodeFace, o.parent,
                                                                        g.faceHasParent, rawT, mm.hasInter);
@@ -524,7 +519,7 @@ void depth_mc(Ctx& c, MainMc& mm, float* rawV, int* rawT) {
     mm.roots = c.buf<int>("roots", offD);
     root_flags<<<div_up(offD, kThreads), kThreads, 0, s>>>(offD, M, o.neigh, g.nodeVert, g.nodeFace, o.childBase,
                                                            g.vvalue, mm.hasInter, rflag);
-    scan_exclusive(rflag, raddr, offD, mm.counts + 2, scratch, s);
+    scan_exclusive(rflag, raddr, offD, mm.counts + 2, c.scan_for(size_t(offD)), s);
     root_emit<<<div_up(offD, kThreads), kThreads, 0, s>>>(offD, L, rflag, raddr, mm.counts + 2, mm.roots,
                                                           mm.counts + 4);
     PSR_HIP_CHECK(hipMemcpyAsync(c.hostPinned, mm.counts, sizeof(int) * (PSR_MAX_DEPTH + 8), hipMemcpyDeviceToHost,

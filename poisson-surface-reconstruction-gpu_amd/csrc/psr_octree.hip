@@ -449,8 +449,8 @@ void build_octree(Ctx& c, const psr_points& pts) {
     normalize_encode<<<div_up(N, kThreads), kThreads, 0, s>>>(
         reinterpret_cast<const char*>(pts.xyz), xs, reinterpret_cast<const char*>(pts.normal), ns, N, D,
         o.normParams, posU, nrmU, keys, vals);
-    int* scratch = c.scratch_for(sort_scratch_ints(N) + scan_scratch_ints(N));
-    radix_sort_pairs(keys, vals, ktmp, vtmp, N, 3 * D, scratch, s);
+    int* scratch = c.scratch_for(sort_scratch_ints(N));
+    radix_sort_pairs(keys, vals, ktmp, vtmp, N, 3 * D, scratch, c.scan_for(sort_scan_elems(N)), s);
     o.spos = c.buf<float4>("spos", N);
     o.snrm = c.buf<float4>("snrm", N);
     gather_points<<<div_up(N, kThreads), kThreads, 0, s>>>(vals, posU, nrmU, N, o.spos, o.snrm);
@@ -501,9 +501,8 @@ void build_octree(Ctx& c, const psr_points& pts) {
     o.dnum = c.buf<int>("node_dnum", M);
     o.neigh = c.buf<int>("node_neigh", size_t(27) * M);
     o.encIdx = c.buf<int>("node_enc", M);
-    zero_int<<<div_up(M, kThreads), kThreads, 0, s>>>(o.pnum, M);
     o.q1Fired = &c.results()->q1Fired;
-    zero_int<<<1, kThreads, 0, s>>>(o.q1Fired, 1);
+    zero_spans({{o.pnum, size_t(M)}, {o.q1Fired, 1}}, s);
     fill_occupied<<<div_up(totalNodes, kThreads), kThreads, 0, s>>>(keys, N, L, lo, Ud, firstPoint, parentRank,
                                                                      rankDm1, totalNodes, o.key, o.pidx, o.pnum,
                                                                      o.parent, o.childBase, o.didx, o.dnum,

@@ -4,19 +4,42 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <initializer_list>
+
 namespace psr {
 
-// Scratch needed by scan_exclusive for n elements, in int32 units.
-size_t scan_scratch_ints(size_t n);
+// State of the single-pass scan of one stream: per-tile status words
+// (epoch | flag | value) and a ring of tile counters.  A call takes the next
+// epoch, so status words of earlier calls never need clearing; the state is
+// zeroed when it is (re)allocated.
+struct ScanState {
+    unsigned long long* status = nullptr;
+    unsigned* counters = nullptr;
+    size_t capacity = 0;  // tiles
+    unsigned epoch = 0;
+};
+constexpr int kScanRing = 64;             // tile counters, one per epoch modulo the ring
+constexpr unsigned kScanEpochMax = 1u << 29;
+size_t scan_tiles(size_t n);
 // out[i] = sum(in[0..i)); *total = sum(in[0..n)) if total != nullptr.  in may equal out.
-void scan_exclusive(const int* in, int* out, int n, int* total, int* scratch, hipStream_t s);
+void scan_exclusive(const int* in, int* out, int n, int* total, ScanState& st, hipStream_t s);
 
-// Scratch needed by radix_sort_pairs for n elements, in int32 units.
+// Sets up to kZeroSpans word ranges to zero with one launch.
+struct ZeroSpan {
+    void* p;
+    size_t words;  // 4-byte words
+};
+constexpr int kZeroSpans = 8;
+void zero_spans(std::initializer_list<ZeroSpan> spans, hipStream_t s);
+
+// Scratch needed by radix_sort_pairs for n elements, in int32 units, and the
+// length of the histogram it scans (for the scan state).
 size_t sort_scratch_ints(size_t n);
+size_t sort_scan_elems(size_t n);
 // Stable ascending sort of keys (only the low `bits` bits are examined) carrying
 // values.  The sorted data ends in (keys, vals): the four buffers are swapped
 // in place of a final copy, so keys/vals may come back as the former ktmp/vtmp.
 void radix_sort_pairs(unsigned*& keys, unsigned*& vals, unsigned*& ktmp, unsigned*& vtmp, int n, int bits,
-                      int* scratch, hipStream_t s);
+                      int* scratch, ScanState& st, hipStream_t s);
 
 }  // namespace psr

@@ -2,11 +2,13 @@
 #include "psr_common.h"
 #include "psr_prims.h"
 
+#include <algorithm>
+#include <stdexcept>
+
 namespace psr {
 
 // ---------------------------------------------------------------------------
-// Exclusive scan: reduce tiles -> scan tile sums (one workgroup) -> downsweep.
-// A tile is 256 threads x 8 items = 2048 elements.
+// Exclusive scan.  A tile is 256 threads x 8 items = 2048 elements.
 // ---------------------------------------------------------------------------
 static constexpr int kScanThreads = 256;
 static constexpr int kScanItems = 8;
@@ -44,95 +46,139 @@ __device__ __forceinline__ int block_excl_scan(int v, int& excl) {
     return total;
 }
 
-__global__ void __launch_bounds__(kScanThreads) scan_reduce_tiles(const int* __restrict__ in, int n,
-                                                                  int* __restrict__ tileSums) {
-    int base = blockIdx.x * kScanTile;
-    int s = 0;
+// Single-pass scan with decoupled look-back.  Tiles take their index from an
+// arrival counter, so every tile a tile waits on has already started; a tile
+// publishes its own sum first (flag 1), then walks back over its predecessors
+// until one has published an inclusive prefix (flag 2), and publishes its own
+// inclusive prefix.  A status word is one 64-bit atomic: epoch (bits 34-63),
+// flag (32-33), value (0-31), so words of earlier calls read as not ready.
+__global__ void __launch_bounds__(kScanThreads) scan_single_pass(const int* in, int* out, int n, int tiles,
+                                                                 int* __restrict__ total,
+                                                                 unsigned long long* status,
+                                                                 unsigned* counters, unsigned epoch) {
+    __shared__ int sTile, sPrefix;
+    if (threadIdx.x == 0) {
+        const int t = int(atomicAdd(&counters[epoch % kScanRing], 1u));
+        if (t == 0) counters[(epoch + 1) % kScanRing] = 0u;  // the next call's counter
+        sTile = t;
+    }
+    __syncthreads();
+    const int tile = sTile;
+    // each thread owns kScanItems consecutive elements of the tile
+    const int base = tile * kScanTile + threadIdx.x * kScanItems;
+    int v[kScanItems];
+    int sum = 0;
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
-        int i = base + k * kScanThreads + threadIdx.x;
-        s += (i < n) ? in[i] : 0;
+        const int i = base + k;
+        v[k] = (i < n) ? in[i] : 0;
+        sum += v[k];
     }
     int excl;
-    int tot = block_excl_scan<kScanThreads>(s, excl);
-    if (threadIdx.x == 0) tileSums[blockIdx.x] = tot;
-}
-
-// Single workgroup scans up to 1024 * 64 tile sums in place; writes the grand total.
-__global__ void __launch_bounds__(1024) scan_tile_sums(int* __restrict__ sums, int m, int* __restrict__ total) {
-    constexpr int NT = 1024;
-    int per = (m + NT - 1) / NT;
-    int start = threadIdx.x * per;
-    int s = 0;
-    for (int k = 0; k < per; ++k) {
-        int i = start + k;
-        if (i < m) s += sums[i];
-    }
-    int excl;
-    int tot = block_excl_scan<NT>(s, excl);
-    int run = excl;
-    for (int k = 0; k < per; ++k) {
-        int i = start + k;
-        if (i < m) {
-            int v = sums[i];
-            sums[i] = run;
-            run += v;
+    const int agg = block_excl_scan<kScanThreads>(sum, excl);
+    if (threadIdx.x < PSR_WAVE) {
+        // the first wave looks back over windows of 64 predecessors at once
+        const int lane = threadIdx.x;
+        const unsigned long long tag = static_cast<unsigned long long>(epoch) << 34;
+        int prefix = 0;
+        if (tile > 0) {
+            if (lane == 0)
+                __hip_atomic_store(&status[tile], tag | (1ull << 32) | unsigned(agg), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            // every predecessor has started and publishes without waiting, so each
+            // wait ends; the spin bound only guards against a broken state
+            for (int j0 = tile - 1;; j0 -= PSR_WAVE) {
+                const int j = j0 - lane;
+                unsigned flag = 2u;  // before the first tile: an inclusive zero
+                int val = 0;
+                if (j >= 0) {
+                    for (unsigned spins = 0;; ++spins) {
+                        const unsigned long long w =
+                            __hip_atomic_load(&status[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        flag = unsigned(w >> 32) & 3u;
+                        if (unsigned(w >> 34) == epoch && flag != 0u) {
+                            val = int(unsigned(w));
+                            break;
+                        }
+                        if (spins > (1u << 24)) {
+                            flag = 2u;
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                }
+                const unsigned long long incl = __ballot(flag == 2u);
+                const int stop = incl ? __builtin_ctzll(incl) : PSR_WAVE;  // closest inclusive prefix
+                int c = lane <= stop ? val : 0;
+#pragma unroll
+                for (int off = PSR_WAVE / 2; off > 0; off >>= 1) c += __shfl_xor(c, off, PSR_WAVE);
+                prefix += c;
+                if (incl) break;
+            }
+        }
+        if (lane == 0) {
+            __hip_atomic_store(&status[tile], tag | (2ull << 32) | unsigned(prefix + agg), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+            if (total && tile == tiles - 1) *total = prefix + agg;
+            sPrefix = prefix;
         }
     }
-    if (threadIdx.x == 0 && total) *total = tot;
-}
-
-// in and out may be the same array: each thread reads its items before it
-// writes them, and the tiles are disjoint.
-__global__ void __launch_bounds__(kScanThreads) scan_downsweep(const int* in, int* out, int n,
-                                                               const int* __restrict__ tileOffsets) {
-    // Each thread owns kScanItems consecutive elements of the tile.
-    int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
-    int v[kScanItems];
-    int s = 0;
+    __syncthreads();
+    int run = excl + sPrefix;
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
-        int i = base + k;
-        v[k] = (i < n) ? in[i] : 0;
-        s += v[k];
-    }
-    int excl;
-    block_excl_scan<kScanThreads>(s, excl);
-    int run = excl + tileOffsets[blockIdx.x];
-#pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        int i = base + k;
+        const int i = base + k;
         if (i < n) out[i] = run;
         run += v[k];
     }
 }
 
-// Up to this many tile sums are scanned by one workgroup; more go through a
-// second level of the same three-pass scan.
-static constexpr int kScanSingleLevel = 1024 * 64;
+__global__ void set_int(int* p, int v) { *p = v; }
 
-size_t scan_scratch_ints(size_t n) {
-    size_t tiles = div_up(n, kScanTile);
-    if (tiles <= size_t(kScanSingleLevel)) return tiles + 64;
-    return 2 * tiles + 64 + scan_scratch_ints(tiles);
+struct ZeroList {
+    unsigned* p[kZeroSpans];
+    size_t end[kZeroSpans];  // cumulative word counts
+    int n;
+};
+
+__global__ void __launch_bounds__(256) zero_list(ZeroList z) {
+    const size_t total = z.end[z.n - 1];
+    for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < total; i += size_t(gridDim.x) * 256) {
+        int k = 0;
+        while (i >= z.end[k]) ++k;
+        z.p[k][i - (k ? z.end[k - 1] : 0)] = 0u;
+    }
 }
 
-void scan_exclusive(const int* in, int* out, int n, int* total, int* scratch, hipStream_t s) {
+void zero_spans(std::initializer_list<ZeroSpan> spans, hipStream_t s) {
+    ZeroList z{};
+    size_t acc = 0;
+    for (const ZeroSpan& sp : spans) {
+        if (sp.words == 0) continue;
+        if (z.n == kZeroSpans) throw std::runtime_error("too many zero spans");
+        acc += sp.words;
+        z.p[z.n] = static_cast<unsigned*>(sp.p);
+        z.end[z.n] = acc;
+        ++z.n;
+    }
+    if (z.n == 0) return;
+    const unsigned blocks = unsigned(std::min<size_t>(div_up(acc, 256), 2048));
+    zero_list<<<blocks, 256, 0, s>>>(z);
+    PSR_HIP_CHECK(hipGetLastError());
+}
+
+size_t scan_tiles(size_t n) { return div_up(n, kScanTile); }
+
+void scan_exclusive(const int* in, int* out, int n, int* total, ScanState& st, hipStream_t s) {
     if (n <= 0) {
-        if (total) PSR_HIP_CHECK(hipMemsetAsync(total, 0, sizeof(int), s));
+        if (total) set_int<<<1, 1, 0, s>>>(total, 0);
         return;
     }
-    int tiles = int(div_up(n, kScanTile));
-    scan_reduce_tiles<<<tiles, kScanThreads, 0, s>>>(in, n, scratch);
-    if (tiles <= kScanSingleLevel) {
-        scan_tile_sums<<<1, 1024, 0, s>>>(scratch, tiles, total);
-        scan_downsweep<<<tiles, kScanThreads, 0, s>>>(in, out, n, scratch);
-    } else {
-        // layout: tile sums [tiles] | their exclusive scan [tiles] | inner scratch
-        int* offs = scratch + tiles;
-        scan_exclusive(scratch, offs, tiles, total, offs + tiles + 64, s);
-        scan_downsweep<<<tiles, kScanThreads, 0, s>>>(in, out, n, offs);
-    }
+    const int tiles = int(scan_tiles(size_t(n)));
+    if (size_t(tiles) > st.capacity || st.epoch + 1 >= kScanEpochMax)
+        throw std::runtime_error("scan state not reserved for this size");
+    ++st.epoch;
+    scan_single_pass<<<tiles, kScanThreads, 0, s>>>(in, out, n, tiles, total, st.status, st.counters, st.epoch);
     PSR_HIP_CHECK(hipGetLastError());
 }
 
@@ -205,24 +251,25 @@ __global__ void __launch_bounds__(kSortThreads) radix_scatter(const unsigned* __
     }
 }
 
+size_t sort_scan_elems(size_t n) { return 256 * size_t(div_up(n, kSortTile)); }
+
 size_t sort_scratch_ints(size_t n) {
     size_t tiles = div_up(n, kSortTile);
     size_t h = 256 * tiles;
-    return h + scan_scratch_ints(h) + 64;
+    return h + 64;
 }
 
 void radix_sort_pairs(unsigned*& keys, unsigned*& vals, unsigned*& ktmp, unsigned*& vtmp, int n, int bits,
-                      int* scratch, hipStream_t s) {
+                      int* scratch, ScanState& st, hipStream_t s) {
     if (n <= 1) return;
     int tiles = int(div_up(n, kSortTile));
     int* hist = scratch;
-    int* scanScratch = scratch + 256 * size_t(tiles);
     int passes = (bits + 7) / 8;
     unsigned *ka = keys, *va = vals, *kb = ktmp, *vb = vtmp;
     for (int p = 0; p < passes; ++p) {
         int shift = 8 * p;
         radix_hist<<<tiles, kSortThreads, 0, s>>>(ka, n, shift, tiles, hist);
-        scan_exclusive(hist, hist, 256 * tiles, nullptr, scanScratch, s);
+        scan_exclusive(hist, hist, 256 * tiles, nullptr, st, s);
         radix_scatter<<<tiles, kSortThreads, 0, s>>>(ka, va, kb, vb, n, shift, tiles, hist);
         std::swap(ka, kb);
         std::swap(va, vb);

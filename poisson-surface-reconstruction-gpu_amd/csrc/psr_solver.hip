@@ -18,26 +18,36 @@ constexpr int kThreads = 256;
 // same-depth neighbours o' of o; |L| <= 1e-6 is dropped (stored as 0); values
 // kept as float.  ELL layout [27][M] over all nodes; the column of slot i is
 // neigh[i][o] - base[depth].
+// Also counts the stored (nonzero) entries per depth into nnz (zeroed before).
 __global__ void __launch_bounds__(kThreads) assemble_operator(Levels L, int M, const int* __restrict__ neigh,
                                                               const int* __restrict__ enc,
                                                               const double* __restrict__ dot,
                                                               const double* __restrict__ d2dot, int res,
-                                                              float* __restrict__ val) {
-    int o = blockIdx.x * kThreads + threadIdx.x;
-    if (o >= M) return;
-    const int D = L.D;
-    int3 f1 = decode_func_index(enc[o], D);
-    for (int i = 0; i < 27; ++i) {
-        int nb = neigh[i * M + o];
-        float v = 0.f;
-        if (nb != -1) {
-            int3 f2 = decode_func_index(enc[nb], D);
-            size_t sx = size_t(f1.x) * res + f2.x, sy = size_t(f1.y) * res + f2.y, sz = size_t(f1.z) * res + f2.z;
-            double e = dot[sx] * dot[sy] * dot[sz] * (d2dot[sx] + d2dot[sy] + d2dot[sz]);
-            if (fabs(e) > double(1e-6f)) v = float(e);
+                                                              float* __restrict__ val, int* __restrict__ nnz) {
+    __shared__ int cnt[PSR_MAX_DEPTH + 1];
+    if (threadIdx.x <= PSR_MAX_DEPTH) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int o = blockIdx.x * kThreads + threadIdx.x;
+    if (o < M) {
+        const int D = L.D;
+        const int3 f1 = decode_func_index(enc[o], D);
+        int c = 0;
+        for (int i = 0; i < 27; ++i) {
+            int nb = neigh[i * M + o];
+            float v = 0.f;
+            if (nb != -1) {
+                int3 f2 = decode_func_index(enc[nb], D);
+                size_t sx = size_t(f1.x) * res + f2.x, sy = size_t(f1.y) * res + f2.y, sz = size_t(f1.z) * res + f2.z;
+                double e = dot[sx] * dot[sy] * dot[sz] * (d2dot[sx] + d2dot[sy] + d2dot[sz]);
+                if (fabs(e) > double(1e-6f)) v = float(e);
+            }
+            val[i * M + o] = v;
+            c += v != 0.f;
         }
-        val[i * M + o] = v;
+        atomicAdd(&cnt[depth_of(L, o)], c);
     }
+    __syncthreads();
+    if (threadIdx.x <= PSR_MAX_DEPTH && cnt[threadIdx.x]) atomicAdd(&nnz[threadIdx.x], cnt[threadIdx.x]);
 }
 
 // --- concurrent CG (CGAlgorithm.cu:10-147, 296-408) -------------------------
@@ -923,21 +933,6 @@ __global__ void __launch_bounds__(1024) iso_final(const double* __restrict__ par
 }
 
 // stored (non-dropped) operator entries per depth
-__global__ void __launch_bounds__(kThreads) count_nonzeros(Levels L, int M, const float* __restrict__ val,
-                                                           int* __restrict__ nnz) {
-    int o = blockIdx.x * kThreads + threadIdx.x;
-    int c = 0;
-    if (o < M)
-        for (int q = 0; q < 27; ++q) c += val[q * M + o] != 0.f;
-    int d = o < M ? depth_of(L, o) : 0;
-    // rows of one wave usually share a depth: reduce per depth value in the wave
-    for (int dd = 0; dd <= L.D; ++dd) {
-        int v = (o < M && d == dd) ? c : 0;
-        for (int off = PSR_WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, PSR_WAVE);
-        if (__lane_id() == 0 && v) atomicAdd(&nnz[dd], v);
-    }
-}
-
 __global__ void zero_u32(unsigned* a, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) a[i] = 0u;
@@ -966,11 +961,8 @@ void solve_depths(Ctx& c) {
     const int D = L.D, M = o.M;
     hipStream_t s = c.stream;
     float* val = c.buf<float>("lap_val", size_t(27) * M);
-    assemble_operator<<<div_up(M, kThreads), kThreads, 0, s>>>(L, M, o.neigh, o.encIdx, c.tables.dot, c.tables.d2dot,
-                                                              c.tables.res, val);
-    int* nnz = c.results()->cgNnz;
-    zero_u32<<<1, 64, 0, s>>>(reinterpret_cast<unsigned*>(nnz), PSR_MAX_DEPTH + 1);
-    count_nonzeros<<<div_up(M, kThreads), kThreads, 0, s>>>(L, M, val, nnz);
+    DevResults* res = c.results();
+    int* nnz = res->cgNnz;
     c.fld.dx = c.buf<float>("dx", M);
     if (size_t(M) * 2 * sizeof(float4) > size_t(INT32_MAX)) throw std::runtime_error("CG state exceeds 2 GiB");
     CgParams P{};
@@ -1022,26 +1014,26 @@ void solve_depths(Ctx& c) {
     double* partial = c.buf<double>("cg_partial", size_t(2) * wg);
     GroupCtl* ctl = c.buf<GroupCtl>("cg_ctl", PSR_MAX_DEPTH + 1);
     const int ctlWords = int(sizeof(GroupCtl) / 4) * (PSR_MAX_DEPTH + 1);
-    DevResults* res = c.results();
     int* iters = res->cgIters;  // [inner iterations | outer steps] per depth
     double* resid = res->cgResid;
     double* trueRel = res->cgTrueRel;
     int* err = &res->cgErr;
-    zero_u32<<<div_up(ctlWords, 256), 256, 0, s>>>(reinterpret_cast<unsigned*>(ctl), ctlWords);
-    zero_u32<<<1, 64, 0, s>>>(reinterpret_cast<unsigned*>(err), 1);
-    zero_u32<<<1, 64, 0, s>>>(reinterpret_cast<unsigned*>(trueRel), 2 * (PSR_MAX_DEPTH + 1));
+    float4* slots = rowForm && mixed ? c.buf<float4>("cg_slots", size_t(4) * wg) : nullptr;
+    // one launch clears the group controls, the slots, the operator counts and the CG results
+    zero_spans({{ctl, size_t(ctlWords)},
+                {slots, slots ? size_t(16) * wg : 0},
+                {res->cgIters, sizeof(res->cgIters) / 4 + sizeof(res->cgNnz) / 4 + 1},  // iterations, nnz, error
+                {trueRel, 2 * (PSR_MAX_DEPTH + 1)}},
+               s);
+    assemble_operator<<<div_up(M, kThreads), kThreads, 0, s>>>(L, M, o.neigh, o.encIdx, c.tables.dot, c.tables.d2dot,
+                                                              c.tables.res, val, nnz);
     // Cooperative CG kernels of different contexts (streams) are ordered on the
     // device: each waits for the previous one to finish, so two of them never
     // compete for residency while spinning in their group barriers.
     std::lock_guard<std::mutex> order(cg_order_lock());
     PSR_HIP_CHECK(hipStreamWaitEvent(s, cg_order_event(), 0));
     PSR_HIP_CHECK(hipEventRecord(c.ev[8], s));
-    float4* slots = nullptr;
-    if (rowForm && mixed) {
-        P.spinLimit = 1u << 22;
-        slots = c.buf<float4>("cg_slots", size_t(4) * wg);
-        zero_u32<<<div_up(size_t(16) * wg, 256), 256, 0, s>>>(reinterpret_cast<unsigned*>(slots), 16 * wg);
-    }
+    if (rowForm && mixed) P.spinLimit = 1u << 22;
     if (mixed) {
         P.relTol = c.cfg.refine_rel_tol;
         P.maxOuter = std::max(1, c.cfg.refine_max_outer);

@@ -1233,18 +1233,17 @@ void extract_mesh(Ctx& c) {
     float* outV = nullptr;
     // Appends one batch of raw vertices: keep flags, local scan, compaction at
     // the running kept count.  localKept holds n + 1 entries afterwards.
-    auto append_vertices = [&](const float* rawV, int n, int* keep, int* localKept, int* scratch) {
+    auto append_vertices = [&](const float* rawV, int n, int* keep, int* localKept) {
         outV = c.buf_keep<float>("mesh_vertices", 3 * (rawUpper + n), 3 * rawUpper);
         keep_flags<<<div_up(size_t(n) + 1, kThreads), kThreads, 0, st>>>(rawV, n, keep);
-        scan_exclusive(keep, localKept, n + 1, acc + 1, scratch, st);
+        scan_exclusive(keep, localKept, n + 1, acc + 1, c.scan_for(size_t(n) + 1), st);
         compact_vertices<<<div_up(std::max(n, 1), kThreads), kThreads, 0, st>>>(rawV, n, keep, localKept, acc, outV);
         rawUpper += n;
     };
     {
         int* keep = c.buf<int>("keep_flag", size_t(mm.nCross) + 1);
         int* kept = c.buf<int>("keep_scan", size_t(mm.nCross) + 1);
-        int* scratch = c.scratch_for(scan_scratch_ints(size_t(mm.nCross) + 1) + 64);
-        append_vertices(mainV, mm.nCross, keep, kept, scratch);
+        append_vertices(mainV, mm.nCross, keep, kept);
         add_count<<<1, 1, 0, st>>>(acc, acc + 1);
         triTotal = mm.nTri;
     }
