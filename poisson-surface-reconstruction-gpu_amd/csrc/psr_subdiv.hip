@@ -227,10 +227,38 @@ __device__ __forceinline__ int corner_owner(const Leaf& L, int D, int b, int& sl
         int3 lc = make_int3(p.x - 1 - (L.Rc.x << L.sh), p.y - 1 - (L.Rc.y << L.sh), p.z - 1 - (L.Rc.z << L.sh));
         return L.S.dEnum + (L.q << (3 * L.sh)) + morton(lc, L.sh);
     }
+    if (L.S.nr == 1) {
+        // a set of one root: the virtual cells sharing p are those in the root's
+        // cube, and the lowest of them takes d_a = 1 unless p is on the cube's
+        // low face in axis a
+        const int3 lp = make_int3(p.x - (L.Rc.x << L.sh), p.y - (L.Rc.y << L.sh), p.z - (L.Rc.z << L.sh));
+        const int3 d = make_int3(lp.x > 0 ? 1 : 0, lp.y > 0 ? 1 : 0, lp.z > 0 ? 1 : 0);
+        slot = bourke_slot(d.x, d.y, d.z);
+        return L.S.dEnum + (L.q << (3 * L.sh)) + morton(make_int3(lp.x - d.x, lp.y - d.y, lp.z - d.z), L.sh);
+    }
     int m;
     const int best = point_owner(p, m, ve);
     slot = best < 0 ? 0 : bourke_slot(m & 1, (m >> 1) & 1, (m >> 2) & 1);
     return best;
+}
+
+// Whether cell p - off(b) owns grid point p (no virtual cell sharing p comes
+// before it in key order).
+template <class VE>
+__device__ __forceinline__ bool owns_corner(const Leaf& L, int b, const VE& ve) {
+    const int3 o = bourke_offset(b);
+    const int3 lc = make_int3(L.c.x - (L.Rc.x << L.sh), L.c.y - (L.Rc.y << L.sh), L.c.z - (L.Rc.z << L.sh));
+    if (L.S.nr == 1) return (o.x || lc.x == 0) && (o.y || lc.y == 0) && (o.z || lc.z == 0);
+    const int3 p = make_int3(L.c.x + o.x, L.c.y + o.y, L.c.z + o.z);
+    const int om = o.x | (o.y << 1) | (o.z << 2);
+    int a0, a1, a2;
+    rank_axes(p, a0, a1, a2);
+    for (int j = 7; j >= 0; --j) {
+        const int m = (((j >> 2) & 1) << a0) | (((j >> 1) & 1) << a1) | ((j & 1) << a2);
+        if (m == om) return true;
+        if (ve_valid(ve, make_int3(p.x - (m & 1), p.y - ((m >> 1) & 1), p.z - ((m >> 2) & 1)))) return false;
+    }
+    return false;
 }
 
 // Owner (enumeration, edge kind within the owner) of edge slot k of L.
@@ -249,6 +277,17 @@ __device__ __forceinline__ int edge_owner(const Leaf& L, int D, int k, int& kind
         n[u0] = lo0;
         n[u1] = lo1;
         int3 lc = make_int3(n[0] - (L.Rc.x << L.sh), n[1] - (L.Rc.y << L.sh), n[2] - (L.Rc.z << L.sh));
+        return L.S.dEnum + (L.q << (3 * L.sh)) + morton(lc, L.sh);
+    }
+    if (L.S.nr == 1) {
+        // a set of one root: the lowest cell of the root's cube sharing the edge
+        const int b0 = lo0 + 1 > (rc[u0] << L.sh) ? 1 : 0, b1 = lo1 + 1 > (rc[u1] << L.sh) ? 1 : 0;
+        kind = (a << 2) | b0 | (b1 << 1);
+        int n[3];
+        n[a] = cc[a];
+        n[u0] = lo0 + 1 - b0;
+        n[u1] = lo1 + 1 - b1;
+        const int3 lc = make_int3(n[0] - (L.Rc.x << L.sh), n[1] - (L.Rc.y << L.sh), n[2] - (L.Rc.z << L.sh));
         return L.S.dEnum + (L.q << (3 * L.sh)) + morton(lc, L.sh);
     }
     // the 4 cells in increasing key order (axis_rank of the two perpendicular
@@ -860,12 +899,7 @@ __global__ void __launch_bounds__(kThreads) sub_corner_values_grid(Grid g, const
         // a corner all of whose cells lie in the root's cube is owned by its lowest cell
         const bool inner = lc.x + o.x > 0 && lc.x + o.x < side && lc.y + o.y > 0 && lc.y + o.y < side &&
                            lc.z + o.z > 0 && lc.z + o.z < side;
-        if (inner) {
-            if (o.x + o.y + o.z != 3) continue;
-        } else {
-            int slot;
-            if (corner_owner(L, D, b, slot, ve) != e) continue;
-        }
+        if (inner ? (o.x + o.y + o.z != 3) : !owns_corner(L, b, ve)) continue;
         const int3 pt = make_int3(L.c.x + o.x, L.c.y + o.y, L.c.z + o.z);
         const int px = pt.x - lo.x, py = pt.y - lo.y, pz = pt.z - lo.z;
         float val = 0.f;
