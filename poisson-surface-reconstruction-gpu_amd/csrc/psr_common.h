@@ -114,22 +114,24 @@ __device__ __forceinline__ float node_term(float4 b, float dx, float3 p) {
     return dx * bspline((p.x - b.x) * b.w) * bspline((p.y - b.y) * b.w) * bspline((p.z - b.z) * b.w);
 }
 
-// sum over the 27 neighbours of `node` of dx * F(x) F(y) F(z), from the
-// neighbour columns ([27][M]: basis parameters and d_x of neighbour i, zero
-// where absent).  An absent neighbour adds +0 or -0, which leaves the sum
-// unchanged: it starts at +0 and can never become -0 by adding.
-__device__ __forceinline__ float chi_node_sum(int node, float3 p, int M, const float4* __restrict__ nbrBasis,
-                                              const float* __restrict__ nbrDx) {
-    float4 b[27];
-    float d[27];
+// node_term of a neighbour-column entry {centre x, y, z, d_x}; invw is the
+// inverse width of the node's depth, which all its neighbours share.
+__device__ __forceinline__ float col_term(float4 c, float invw, float3 p) {
+    return c.w * bspline((p.x - c.x) * invw) * bspline((p.y - c.y) * invw) * bspline((p.z - c.z) * invw);
+}
+
+// sum over the 27 neighbours of `node` (at the depth of inverse width invw) of
+// dx * F(x) F(y) F(z), from the neighbour columns ([27][M] {centre, d_x} of
+// neighbour i, zero where absent).  An absent neighbour adds +0 or -0, which
+// leaves the sum unchanged: it starts at +0 and can never become -0 by adding.
+__device__ __forceinline__ float chi_node_sum(int node, float invw, float3 p, int M,
+                                              const float4* __restrict__ nbrCol) {
+    float4 c[27];
 #pragma unroll
-    for (int i = 0; i < 27; ++i) {
-        b[i] = nbrBasis[size_t(i) * M + node];
-        d[i] = nbrDx[size_t(i) * M + node];
-    }
+    for (int i = 0; i < 27; ++i) c[i] = nbrCol[size_t(i) * M + node];
     float v = 0.f;
 #pragma unroll
-    for (int i = 0; i < 27; ++i) v += node_term(b[i], d[i], p);
+    for (int i = 0; i < 27; ++i) v += col_term(c[i], invw, p);
     return v;
 }
 

@@ -77,8 +77,7 @@ struct Field {
     float* div = nullptr;     // [M]
     float* dx = nullptr;      // [M]
     float* chiPts = nullptr;  // [N]
-    float4* nbrBasis = nullptr;  // [27][M]: basis parameters of neighbour i of each node (0 where absent)
-    float* nbrDx = nullptr;      // [27][M]: its d_x (0 where absent)
+    float4* nbrCol = nullptr;  // [27][M]: centre (x, y, z) and d_x of neighbour i of each node (0 where absent)
     double* isoSum = nullptr; // device scalar
     float iso = 0.f;
 };

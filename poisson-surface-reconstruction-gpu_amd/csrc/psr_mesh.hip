@@ -244,8 +244,7 @@ __global__ void __launch_bounds__(kThreads) vertex_values(NodeView v, const int*
                                                           const float4* __restrict__ vPos,
                                                           const int* __restrict__ parent,
                                                           const int* __restrict__ childBase,
-                                                          const float4* __restrict__ nbrBasis,
-                                                          const float* __restrict__ nbrDx,
+                                                          const float4* __restrict__ nbrCol,
                                                           const float* __restrict__ iso, float* __restrict__ vv) {
     int t = blockIdx.x * kThreads + threadIdx.x;
     if (t >= *count) return;
@@ -256,8 +255,8 @@ __global__ void __launch_bounds__(kThreads) vertex_values(NodeView v, const int*
     float val = 0.f;
     if (owner > 0) {
         int now = owner;
-        while (now != -1) {
-            val += chi_node_sum(now, p, v.M, nbrBasis, nbrDx);
+        for (int d = depth; now != -1; --d) {  // a parent is one level up
+            val += chi_node_sum(now, float(1 << d), p, v.M, nbrCol);
             now = parent[now];
         }
         // descent into the key-ordered child picked by the Bourke corner id
@@ -268,7 +267,7 @@ __global__ void __launch_bounds__(kThreads) vertex_values(NodeView v, const int*
             int cb = childBase[now];
             if (cb < 0) break;
             now = cb + ex;
-            val += chi_node_sum(now, p, v.M, nbrBasis, nbrDx);
+            val += chi_node_sum(now, float(1 << depth), p, v.M, nbrCol);
         }
     }
     vv[t] = val - *iso;
@@ -478,7 +477,7 @@ void build_geometry(Ctx& c) {
     g.vvalue = c.buf<float>("vvalue", size_t(8) * M);
     vertex_values<<<div_up(size_t(8) * M, kThreads), kThreads, 0, s>>>(v, g.counts + 0, g.vtxOwner, g.vtxKind,
                                                                        g.vtxPos, o.parent, o.childBase,
-                                                                       c.fld.nbrBasis, c.fld.nbrDx, isoDev, g.vvalue);
+                                                                       c.fld.nbrCol, isoDev, g.vvalue);
     PSR_HIP_CHECK(hipMemcpyAsync(c.hostPinned, g.counts, sizeof(int) * 3, hipMemcpyDeviceToHost, s));
     PSR_HIP_CHECK(hipStreamSynchronize(s));
     g.V = c.hostPinned[0];

@@ -875,21 +875,23 @@ __global__ void __launch_bounds__(kCgThreads) cg_mixed_all_depths(
 __global__ void __launch_bounds__(kThreads) neighbour_columns(int M, const int* __restrict__ neigh,
                                                               const float4* __restrict__ basis,
                                                               const float* __restrict__ dx,
-                                                              float4* __restrict__ nbrBasis,
-                                                              float* __restrict__ nbrDx) {
+                                                              float4* __restrict__ nbrCol) {
     const size_t t = size_t(blockIdx.x) * kThreads + threadIdx.x;
     if (t >= size_t(27) * M) return;
     const int nb = neigh[t];
-    nbrBasis[t] = nb < 0 ? make_float4(0.f, 0.f, 0.f, 0.f) : basis[nb];
-    nbrDx[t] = nb < 0 ? 0.f : dx[nb];
+    if (nb < 0) {
+        nbrCol[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+    } else {
+        const float4 b = basis[nb];
+        nbrCol[t] = make_float4(b.x, b.y, b.z, dx[nb]);
+    }
 }
 
 __global__ void __launch_bounds__(kThreads) points_chi_kernel(int N, int offD, int M, int D,
                                                               const float4* __restrict__ pos,
                                                               const int* __restrict__ p2n,
                                                               const int* __restrict__ parent,
-                                                              const float4* __restrict__ nbrBasis,
-                                                              const float* __restrict__ nbrDx,
+                                                              const float4* __restrict__ nbrCol,
                                                               float* __restrict__ chi, double* __restrict__ part) {
     __shared__ double sh[kThreads / PSR_WAVE];
     int i = blockIdx.x * kThreads + threadIdx.x;
@@ -899,8 +901,8 @@ __global__ void __launch_bounds__(kThreads) points_chi_kernel(int N, int offD, i
         float3 p = make_float3(q.x, q.y, q.z);
         int now = offD + p2n[i];
         float v = 0.f;
-        while (now != -1) {
-            v += chi_node_sum(now, p, M, nbrBasis, nbrDx);
+        for (int d = D; now != -1; --d) {  // a parent is one level up
+            v += chi_node_sum(now, float(1 << d), p, M, nbrCol);
             now = parent[now];
         }
         chi[i] = v;
@@ -1081,12 +1083,11 @@ void points_implicit(Ctx& c) {
     double* part = c.buf<double>("iso_part", nb);
     c.fld.isoSum = c.buf<double>("iso_sum", 1);
     float* isoDev = &c.results()->iso;
-    c.fld.nbrBasis = c.buf<float4>("nbr_basis", size_t(27) * o.M);
-    c.fld.nbrDx = c.buf<float>("nbr_dx", size_t(27) * o.M);
+    c.fld.nbrCol = c.buf<float4>("nbr_col", size_t(27) * o.M);
     neighbour_columns<<<div_up(size_t(27) * o.M, kThreads), kThreads, 0, s>>>(o.M, o.neigh, o.basis, c.fld.dx,
-                                                                             c.fld.nbrBasis, c.fld.nbrDx);
+                                                                             c.fld.nbrCol);
     points_chi_kernel<<<nb, kThreads, 0, s>>>(N, L.base[L.D], o.M, L.D, o.spos, o.point2node, o.parent,
-                                              c.fld.nbrBasis, c.fld.nbrDx, c.fld.chiPts, part);
+                                              c.fld.nbrCol, c.fld.chiPts, part);
     iso_final<<<1, 1024, 0, s>>>(part, nb, N, c.fld.isoSum, isoDev);
     PSR_HIP_CHECK(hipGetLastError());
 }

@@ -54,8 +54,7 @@ struct Grid {
     const int* neigh;
     const float4* basis;
     const float* dx;
-    const float4* nbrBasis;  // [27][M] basis parameters of neighbour i of each node (0 where absent)
-    const float* nbrDx;      // [27][M] its d_x (0 where absent)
+    const float4* nbrCol;    // [27][M] centre (x, y, z) and d_x of neighbour i of each node (0 where absent)
     const int* rootNbr;      // [root][27] position in roots of the root's real neighbour, or -1
     const int* rootChain;    // [root][kChain] the root, then its ancestors (-1 past the tree root)
     const int4* rootCell;    // [root] cell at the root depth (x, y, z) and the root node (w)
@@ -637,10 +636,12 @@ __global__ void __launch_bounds__(NT) sub_corner_values(Grid g, const Tile* __re
                 int nb = root_list_node(g, S, lChain[tag * C], i);
                 sBasis[t] = nb < 0 ? make_float4(0.f, 0.f, 0.f, 0.f) : g.basis[nb];
                 sDx[t] = nb < 0 ? 0.f : g.dx[nb];
-            } else {         // ancestor: its precomputed neighbour block
-                int node = lChain[-1 - tag];
-                sBasis[t] = node < 0 ? make_float4(0.f, 0.f, 0.f, 0.f) : g.nbrBasis[size_t(i) * g.M + node];
-                sDx[t] = node < 0 ? 0.f : g.nbrDx[size_t(i) * g.M + node];
+            } else {         // ancestor: its neighbour column, at the ancestor's depth
+                const int slotChain = -1 - tag;
+                const int node = lChain[slotChain];
+                const float4 col = node < 0 ? make_float4(0.f, 0.f, 0.f, 0.f) : g.nbrCol[size_t(i) * g.M + node];
+                sBasis[t] = make_float4(col.x, col.y, col.z, node < 0 ? 0.f : float(1 << (S.rd - slotChain % C)));
+                sDx[t] = col.w;
             }
         }
     }
@@ -714,8 +715,8 @@ __global__ void __launch_bounds__(NT) sub_corner_values(Grid g, const Tile* __re
                 } else if (u > 0) {
                     const int node = lChain[rr * C + u];
                     if (node >= 0) {
-                        for (int i = 0; i < 27; ++i)
-                            val += node_term(g.nbrBasis[size_t(i) * g.M + node], g.nbrDx[size_t(i) * g.M + node], p);
+                        const float invw = float(1 << (S.rd - u));
+                        for (int i = 0; i < 27; ++i) val += col_term(g.nbrCol[size_t(i) * g.M + node], invw, p);
                     }
                 } else {
                     int ids[27];
@@ -819,8 +820,9 @@ __global__ void __launch_bounds__(kThreads) sub_corner_values_grid(Grid g, const
         float4 b = make_float4(0.f, 0.f, 0.f, 0.f);
         float d = 0.f;
         if (node >= 0) {
-            b = g.nbrBasis[size_t(k % 27) * g.M + node];
-            d = g.nbrDx[size_t(k % 27) * g.M + node];
+            const float4 col = g.nbrCol[size_t(k % 27) * g.M + node];
+            b = make_float4(col.x, col.y, col.z, float(1 << (S.rd - k / 27)));
+            d = col.w;
         }
         for (int cc = 0; cc < kTx; ++cc)
             lLx[cc * Kp + k] = node < 0 ? 0.f : d * bspline((float(lo.x + cc) * inv - b.x) * b.w);
@@ -1317,8 +1319,7 @@ void extract_mesh(Ctx& c) {
         const size_t chunkCells = std::min<size_t>(c.cfg.subdiv_chunk_nodes ? c.cfg.subdiv_chunk_nodes : kChunkCells,
                                                    size_t(INT32_MAX - 1) / 12);
         float* isoDev = &c.results()->iso;
-        const float4* nbrBasis = c.fld.nbrBasis;
-        const float* nbrDx = c.fld.nbrDx;
+        const float4* nbrCol = c.fld.nbrCol;
         int* rootNbr = c.buf<int>("sub_root_nbr", size_t(27) * mm.nRoots);
         int* rootChain = c.buf<int>("sub_root_chain", size_t(kChain) * mm.nRoots);
         int4* rootCell = c.buf<int4>("sub_root_cell", mm.nRoots);
@@ -1398,7 +1399,7 @@ void extract_mesh(Ctx& c) {
             Tile* dtiles = reinterpret_cast<Tile*>(dAll + bSets);
             Tile* dsmall = reinterpret_cast<Tile*>(dAll + bSets + bTiles);
             Grid gr{D,       M,       nCs,      dsets,    mm.roots, rootPos,   o.key,    o.parent, o.childBase,
-                    o.neigh, o.basis, c.fld.dx, nbrBasis, nbrDx,    rootNbr,   rootChain, rootCell, rootSet, dxAt};
+                    o.neigh, o.basis, c.fld.dx, nbrCol,   rootNbr,  rootChain, rootCell, rootSet,   dxAt};
             float* cval = c.buf<float>("sub_cval", size_t(8) * nVD);
             float* vals = c.buf<float>("sub_vals", size_t(8) * nVD);
             unsigned* cellWord = c.buf<unsigned>("sub_cell_word", nVD);
