@@ -3,6 +3,8 @@
 
 #include <hip/hip_fp16.h>
 
+#include <algorithm>
+
 namespace psr {
 
 namespace {
@@ -217,17 +219,20 @@ __global__ void __launch_bounds__(kDivWaves * PSR_WAVE) divergence_fine(
     if (lane == 0) div[o] = float(acc);
 }
 
-// Coarse nodes (few, each covering a large share of depth D): a workgroup each.
-constexpr int kDivCoarseThreads = 1024;
+// Coarse nodes (few, each covering a large share of depth D): `parts`
+// workgroups each, every one summing a strided share of the node's terms; the
+// shares are folded in a fixed order by divergence_fold.
+constexpr int kDivCoarseThreads = 256;
 template <class FieldT>
 __global__ void __launch_bounds__(kDivCoarseThreads) divergence_coarse(
-    int count, int M, int offD, int nD, int res, int D, const int* __restrict__ neigh, const int* __restrict__ didx,
-    const int* __restrict__ dnum, const int* __restrict__ enc, const FieldT* __restrict__ vf,
-    const double* __restrict__ ddot, float* __restrict__ div) {
-    int o = blockIdx.x;
-    if (o >= count) return;
+    int begin, int count, int parts, int M, int offD, int nD, int res, int D, const int* __restrict__ neigh,
+    const int* __restrict__ didx, const int* __restrict__ dnum, const int* __restrict__ enc,
+    const FieldT* __restrict__ vf, const double* __restrict__ ddot, double* __restrict__ partial) {
+    const int node = blockIdx.x / parts, part = blockIdx.x - node * parts;
+    if (node >= count) return;
+    const int o = begin + node;
     __shared__ int start[28], first[27];
-    __shared__ double part[kDivCoarseThreads / PSR_WAVE];
+    __shared__ double red[kDivCoarseThreads / PSR_WAVE];
     if (threadIdx.x == 0) {
         int s = 0;
         for (int i = 0; i < 27; ++i) {
@@ -239,22 +244,52 @@ __global__ void __launch_bounds__(kDivCoarseThreads) divergence_coarse(
         start[27] = s;
     }
     __syncthreads();
-    int total = start[27];
-    int3 fo = decode_func_index(enc[o], D);
+    const int total = start[27];
+    const int3 fo = decode_func_index(enc[o], D);
     double acc = 0;
     int i = 0;
-    for (int t = threadIdx.x; t < total; t += kDivCoarseThreads) {
+    for (int t = part * kDivCoarseThreads + threadIdx.x; t < total; t += parts * kDivCoarseThreads) {
         while (start[i + 1] <= t) ++i;
         acc += div_term(first[i] + (t - start[i]), offD, nD, res, D, fo, enc, vf, ddot);
     }
     for (int off = PSR_WAVE / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, PSR_WAVE);
-    if (__lane_id() == 0) part[threadIdx.x / PSR_WAVE] = acc;
+    if (__lane_id() == 0) red[threadIdx.x / PSR_WAVE] = acc;
     __syncthreads();
     if (threadIdx.x == 0) {
         double s = 0;
-        for (int w = 0; w < kDivCoarseThreads / PSR_WAVE; ++w) s += part[w];
-        div[o] = float(s);
+        for (int w = 0; w < kDivCoarseThreads / PSR_WAVE; ++w) s += red[w];
+        partial[blockIdx.x] = s;
     }
+}
+
+__global__ void __launch_bounds__(64) divergence_fold(int begin, int count, int parts,
+                                                      const double* __restrict__ partial, float* __restrict__ div) {
+    const int node = blockIdx.x * 64 + threadIdx.x;
+    if (node >= count) return;
+    double s = 0;
+    for (int k = 0; k < parts; ++k) s += partial[size_t(node) * parts + k];
+    div[begin + node] = float(s);
+}
+
+// Depth-D nodes: their neighbours are depth-D nodes, each its own only
+// descendant, so a thread sums the 27 terms of its node.
+template <class FieldT>
+__global__ void __launch_bounds__(256) divergence_leaf(int offD, int nD, int M, int res, int D,
+                                                       const int* __restrict__ neigh, const int* __restrict__ enc,
+                                                       const FieldT* __restrict__ vf,
+                                                       const double* __restrict__ ddot, float* __restrict__ div) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= nD) return;
+    const int o = offD + t;
+    const int3 fo = decode_func_index(enc[o], D);
+    int nb[27];
+#pragma unroll
+    for (int i = 0; i < 27; ++i) nb[i] = neigh[i * M + o];
+    double acc = 0;
+#pragma unroll
+    for (int i = 0; i < 27; ++i)
+        if (nb[i] != -1) acc += div_term(nb[i] - offD, offD, nD, res, D, fo, enc, vf, ddot);
+    div[o] = float(acc);
 }
 
 }  // namespace
@@ -282,16 +317,39 @@ void field_stages(Ctx& c, FieldT* vf) {
     hipStream_t s = c.stream;
     vector_field_kernel<FieldT><<<div_up(nD, kVfWaves), kVfWaves * PSR_WAVE, 0, s>>>(
         nD, offD, M, D, o.neigh, o.pidx, o.pnum, o.key, o.spos, o.snrm, vf);
-    // depths 0..kCoarse: one workgroup per node; deeper: one wave per node
+    // depths 0..kCoarse: several workgroups per node, per depth as many as the
+    // node's expected share of depth D asks for; depth D: a thread per node;
+    // between: a wave per node
     const int kCoarse = D >= 5 ? 3 : 1;
-    int coarseCount = L.base[kCoarse + 1];
-    divergence_coarse<FieldT><<<coarseCount, kDivCoarseThreads, 0, s>>>(coarseCount, M, offD, nD, c.tables.res, D,
-                                                                        o.neigh, o.didx, o.dnum, o.encIdx, vf,
-                                                                        c.tables.ddot, c.fld.div);
-    int fineCount = M - coarseCount;
-    divergence_fine<FieldT><<<div_up(fineCount, kDivWaves), kDivWaves * PSR_WAVE, 0, s>>>(
-        coarseCount, fineCount, M, offD, nD, c.tables.res, D, o.neigh, o.didx, o.dnum, o.encIdx, vf, c.tables.ddot,
-        c.fld.div);
+    const int coarseCount = L.base[kCoarse + 1];
+    double* partial = nullptr;
+    {
+        size_t blocks = 0;
+        int parts[PSR_MAX_DEPTH + 1];
+        for (int d = 0; d <= kCoarse; ++d) {
+            const double share = std::min(double(nD), 27.0 * nD / std::max(1, L.count[d]));
+            parts[d] = std::max(1, std::min(64, int(share / 2048.0)));
+            blocks += size_t(parts[d]) * L.count[d];
+        }
+        partial = c.buf<double>("div_partial", blocks);
+        size_t off = 0;
+        for (int d = 0; d <= kCoarse; ++d) {
+            if (L.count[d] == 0) continue;
+            divergence_coarse<FieldT><<<unsigned(size_t(parts[d]) * L.count[d]), kDivCoarseThreads, 0, s>>>(
+                L.base[d], L.count[d], parts[d], M, offD, nD, c.tables.res, D, o.neigh, o.didx, o.dnum, o.encIdx, vf,
+                c.tables.ddot, partial + off);
+            divergence_fold<<<div_up(L.count[d], 64), 64, 0, s>>>(L.base[d], L.count[d], parts[d], partial + off,
+                                                                c.fld.div);
+            off += size_t(parts[d]) * L.count[d];
+        }
+    }
+    const int midCount = offD - coarseCount;
+    if (midCount > 0)
+        divergence_fine<FieldT><<<div_up(midCount, kDivWaves), kDivWaves * PSR_WAVE, 0, s>>>(
+            coarseCount, midCount, M, offD, nD, c.tables.res, D, o.neigh, o.didx, o.dnum, o.encIdx, vf, c.tables.ddot,
+            c.fld.div);
+    divergence_leaf<FieldT><<<div_up(nD, 256), 256, 0, s>>>(offD, nD, M, c.tables.res, D, o.neigh, o.encIdx, vf,
+                                                            c.tables.ddot, c.fld.div);
     PSR_HIP_CHECK(hipGetLastError());
 }
 

@@ -189,8 +189,8 @@ void scan_exclusive(const int* in, int* out, int n, int* total, ScanState& st, h
 //   scatter: per tile, stable rank by wave ballots (multisplit) + offsets
 // ---------------------------------------------------------------------------
 static constexpr int kSortThreads = 256;
-static constexpr int kSortIters = 16;
-static constexpr int kSortTile = kSortThreads * kSortIters;  // 4096 keys
+static constexpr int kSortIters = 2;  // small tiles: enough workgroups to fill the GPU at 200k keys
+static constexpr int kSortTile = kSortThreads * kSortIters;  // 512 keys
 static constexpr int kSortWaves = kSortThreads / PSR_WAVE;
 
 __global__ void __launch_bounds__(kSortThreads) radix_hist(const unsigned* __restrict__ keys, int n, int shift,
