@@ -223,14 +223,24 @@ __global__ void __launch_bounds__(kDivWaves * PSR_WAVE) divergence_fine(
 // workgroups each, every one summing a strided share of the node's terms; the
 // shares are folded in a fixed order by divergence_fold.
 constexpr int kDivCoarseThreads = 256;
+constexpr int kMaxCoarse = 4;  // coarse depths 0..3
+struct CoarsePlan {
+    int depths;                      // coarse depths 0..depths-1
+    int blockStart[kMaxCoarse + 1];  // first workgroup (and partial) of each depth
+    int parts[kMaxCoarse];           // workgroups per node
+    int base[kMaxCoarse + 1];        // node range of each depth
+};
 template <class FieldT>
 __global__ void __launch_bounds__(kDivCoarseThreads) divergence_coarse(
-    int begin, int count, int parts, int M, int offD, int nD, int res, int D, const int* __restrict__ neigh,
+    CoarsePlan P, int M, int offD, int nD, int res, int D, const int* __restrict__ neigh,
     const int* __restrict__ didx, const int* __restrict__ dnum, const int* __restrict__ enc,
     const FieldT* __restrict__ vf, const double* __restrict__ ddot, double* __restrict__ partial) {
-    const int node = blockIdx.x / parts, part = blockIdx.x - node * parts;
-    if (node >= count) return;
-    const int o = begin + node;
+    int d = 0;
+    while (d + 1 < P.depths && int(blockIdx.x) >= P.blockStart[d + 1]) ++d;
+    const int parts = P.parts[d];
+    const int rel = blockIdx.x - P.blockStart[d];
+    const int node = rel / parts, part = rel - node * parts;
+    const int o = P.base[d] + node;
     __shared__ int start[28], first[27];
     __shared__ double red[kDivCoarseThreads / PSR_WAVE];
     if (threadIdx.x == 0) {
@@ -262,13 +272,17 @@ __global__ void __launch_bounds__(kDivCoarseThreads) divergence_coarse(
     }
 }
 
-__global__ void __launch_bounds__(64) divergence_fold(int begin, int count, int parts,
-                                                      const double* __restrict__ partial, float* __restrict__ div) {
-    const int node = blockIdx.x * 64 + threadIdx.x;
-    if (node >= count) return;
+__global__ void __launch_bounds__(64) divergence_fold(CoarsePlan P, const double* __restrict__ partial,
+                                                      float* __restrict__ div) {
+    const int o = blockIdx.x * 64 + threadIdx.x;
+    if (o >= P.base[P.depths]) return;
+    int d = 0;
+    while (o >= P.base[d + 1]) ++d;
+    const int parts = P.parts[d];
+    const double* pp = partial + P.blockStart[d] + size_t(o - P.base[d]) * parts;
     double s = 0;
-    for (int k = 0; k < parts; ++k) s += partial[size_t(node) * parts + k];
-    div[begin + node] = float(s);
+    for (int k = 0; k < parts; ++k) s += pp[k];
+    div[o] = float(s);
 }
 
 // Depth-D nodes: their neighbours are depth-D nodes, each its own only
@@ -320,28 +334,25 @@ void field_stages(Ctx& c, FieldT* vf) {
     // depths 0..kCoarse: several workgroups per node, per depth as many as the
     // node's expected share of depth D asks for; depth D: a thread per node;
     // between: a wave per node
-    const int kCoarse = D >= 5 ? 3 : 1;
+    const int kCoarse = D >= 5 ? kMaxCoarse - 1 : 1;
     const int coarseCount = L.base[kCoarse + 1];
-    double* partial = nullptr;
     {
-        size_t blocks = 0;
-        int parts[PSR_MAX_DEPTH + 1];
+        CoarsePlan P{};
+        P.depths = kCoarse + 1;
+        int blocks = 0;
         for (int d = 0; d <= kCoarse; ++d) {
             const double share = std::min(double(nD), 27.0 * nD / std::max(1, L.count[d]));
-            parts[d] = std::max(1, std::min(64, int(share / 2048.0)));
-            blocks += size_t(parts[d]) * L.count[d];
+            P.parts[d] = std::max(1, std::min(64, int(share / 2048.0)));
+            P.blockStart[d] = blocks;
+            P.base[d] = L.base[d];
+            blocks += P.parts[d] * L.count[d];
         }
-        partial = c.buf<double>("div_partial", blocks);
-        size_t off = 0;
-        for (int d = 0; d <= kCoarse; ++d) {
-            if (L.count[d] == 0) continue;
-            divergence_coarse<FieldT><<<unsigned(size_t(parts[d]) * L.count[d]), kDivCoarseThreads, 0, s>>>(
-                L.base[d], L.count[d], parts[d], M, offD, nD, c.tables.res, D, o.neigh, o.didx, o.dnum, o.encIdx, vf,
-                c.tables.ddot, partial + off);
-            divergence_fold<<<div_up(L.count[d], 64), 64, 0, s>>>(L.base[d], L.count[d], parts[d], partial + off,
-                                                                c.fld.div);
-            off += size_t(parts[d]) * L.count[d];
-        }
+        P.blockStart[kCoarse + 1] = blocks;
+        P.base[kCoarse + 1] = L.base[kCoarse + 1];
+        double* partial = c.buf<double>("div_partial", blocks);
+        divergence_coarse<FieldT><<<blocks, kDivCoarseThreads, 0, s>>>(P, M, offD, nD, c.tables.res, D, o.neigh, o.didx,
+                                                                      o.dnum, o.encIdx, vf, c.tables.ddot, partial);
+        divergence_fold<<<div_up(coarseCount, 64), 64, 0, s>>>(P, partial, c.fld.div);
     }
     const int midCount = offD - coarseCount;
     if (midCount > 0)
