@@ -103,7 +103,6 @@ struct Mesh {
     float* vertices = nullptr;  // xyz
     int* triangles = nullptr;
     int nv = 0, nt = 0;
-    int* counts = nullptr;      // device: final vertex / triangle counts
 };
 
 // Small per-call results written by the kernels and read back in one copy.
@@ -163,9 +162,10 @@ struct Ctx {
     }
     ScanState scan;
     // The stream's scan state, able to scan n elements (zeroed when it grows).
-    ScanState& scan_for(size_t n) {
-        const size_t tiles = scan_tiles(n);
-        if (tiles > scan.capacity || scan.epoch + 1 >= kScanEpochMax) {
+    ScanState& scan_for(size_t n) { return scan_for_tiles(scan_tiles(n)); }
+    // the scan state for a launch of `tiles` workgroups that scan their own counts
+    ScanState& scan_for_tiles(size_t tiles) {
+        if (tiles > scan.capacity || scan.epoch + kScanRing >= kScanEpochMax) {
             const size_t cap = std::max<size_t>(tiles + tiles / 2 + 64, 1024);
             scan.status = static_cast<unsigned long long*>(arena.get("scan_status", cap * sizeof(unsigned long long)));
             scan.counters = static_cast<unsigned*>(arena.get("scan_counters", kScanRing * sizeof(unsigned)));

@@ -2,6 +2,7 @@
 // depth-D marching cubes, face-intersection propagation and subdivision roots.
 #include "psr_mesh_common.h"
 #include "psr_prims.h"
+#include "psr_tile_scan.h"
 
 namespace psr {
 
@@ -42,50 +43,53 @@ __device__ __forceinline__ void load_neighbourhood(const NodeView& v, int o, int
     }
 }
 
-__global__ void __launch_bounds__(kThreads) vertex_owner(NodeView v, int* __restrict__ flag) {
-    int o = blockIdx.x * kThreads + threadIdx.x;
-    if (o >= v.M) return;
-    int depth = v.depth[o];
-    float half = 1.0f / float(1 << (depth + 1)), W = 1.0f / float(1 << depth), W2 = W * W;
-    int nb[27], kk[27];
-    float3 cc[27];
-    load_neighbourhood(v, o, nb, cc, kk);
-    const float3 nc = cc[13];  // (0, 0, 0) when the node has no self-neighbour
-    int ownKey[8], own[8];
-    for (int i = 0; i < 8; ++i) {
-        ownKey[i] = 0x7fffffff;
-        own[i] = -1;
-    }
-#pragma unroll
-    for (int j = 0; j < 27; ++j) {
-        if (nb[j] == -1) continue;
-#pragma unroll
+// Owned corners of each node, written in (node, corner) order: the owner test
+// and the compaction in one launch (the scan of the owned counts runs across
+// tiles by decoupled look-back); the grand total goes to t.total.
+__global__ void __launch_bounds__(kThreads) vertex_owner_emit(NodeView v, ScanTicket t, int* __restrict__ vOwner,
+                                                              int* __restrict__ vKind, float4* __restrict__ vPos) {
+    const int tile = ticket_tile(t);
+    const int o = tile * kThreads + threadIdx.x;
+    unsigned mask = 0;
+    int depth = 0;
+    float half = 0.f;
+    float3 nc = make_float3(0.f, 0.f, 0.f);
+    if (o < v.M) {
+        depth = v.depth[o];
+        half = 1.0f / float(1 << (depth + 1));
+        const float W = 1.0f / float(1 << depth), W2 = W * W;
+        int nb[27], kk[27];
+        float3 cc[27];
+        load_neighbourhood(v, o, nb, cc, kk);
+        nc = cc[13];  // (0, 0, 0) when the node has no self-neighbour
+        int ownKey[8], own[8];
         for (int i = 0; i < 8; ++i) {
-            float3 p = corner_pos(nc, i, half);
-            if (sqdist(p, cc[j]) < double(W2) && ownKey[i] > kk[j]) {
-                ownKey[i] = kk[j];
-                own[i] = nb[j];
+            ownKey[i] = 0x7fffffff;
+            own[i] = -1;
+        }
+#pragma unroll
+        for (int j = 0; j < 27; ++j) {
+            if (nb[j] == -1) continue;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                float3 p = corner_pos(nc, i, half);
+                if (sqdist(p, cc[j]) < double(W2) && ownKey[i] > kk[j]) {
+                    ownKey[i] = kk[j];
+                    own[i] = nb[j];
+                }
             }
         }
+        for (int i = 0; i < 8; ++i) mask |= (own[i] == o ? 1u : 0u) << i;
     }
-    for (int i = 0; i < 8; ++i) flag[8 * o + i] = (own[i] == o) ? 1 : 0;
-}
-
-__global__ void __launch_bounds__(kThreads) vertex_emit(NodeView v, const int* __restrict__ flag,
-                                                        const int* __restrict__ addr, int* __restrict__ vOwner,
-                                                        int* __restrict__ vKind, float4* __restrict__ vPos) {
-    int t = blockIdx.x * kThreads + threadIdx.x;
-    if (t >= 8 * v.M || !flag[t]) return;
-    int o = t >> 3, i = t & 7;
-    int depth = v.depth[o];
-    float half = 1.0f / float(1 << (depth + 1));
-    int self = v.neigh[13 * v.M + o];
-    float3 nc = self != -1 ? node_center(v, self) : make_float3(0.f, 0.f, 0.f);
-    float3 p = corner_pos(nc, i, half);
-    int a = addr[t];
-    vOwner[a] = o;
-    vKind[a] = i;
-    vPos[a] = make_float4(p.x, p.y, p.z, __int_as_float(depth));
+    int a = ticket_prefix<kThreads>(t, tile, __popc(mask));
+    for (int i = 0; i < 8; ++i) {
+        if (!((mask >> i) & 1u)) continue;
+        const float3 p = corner_pos(nc, i, half);
+        vOwner[a] = o;
+        vKind[a] = i;
+        vPos[a] = make_float4(p.x, p.y, p.z, __int_as_float(depth));
+        ++a;
+    }
 }
 
 __global__ void __launch_bounds__(kThreads) vertex_maintain(NodeView v, const int* __restrict__ count,
@@ -109,43 +113,44 @@ __global__ void __launch_bounds__(kThreads) vertex_maintain(NodeView v, const in
     }
 }
 
-__global__ void __launch_bounds__(kThreads) edge_owner(NodeView v, int offD, int nD, int* __restrict__ flag) {
-    int t = blockIdx.x * kThreads + threadIdx.x;
-    if (t >= nD) return;
-    int o = offD + t;
-    float half = 1.0f / float(1 << (v.D + 1)), W = 1.0f / float(1 << v.D), W2 = W * W;
-    int nb[27], kk[27];
-    float3 cc[27];
-    load_neighbourhood(v, o, nb, cc, kk);
-    const float3 nc = cc[13];
-    int ownKey[12], own[12];
-    for (int e = 0; e < 12; ++e) {
-        ownKey[e] = 0x7fffffff;
-        own[e] = -1;
-    }
-#pragma unroll
-    for (int j = 0; j < 27; ++j) {
-        if (nb[j] == -1) continue;
-#pragma unroll
+// Owned edges of each depth-D node in (node, edge) order, compacted in the same launch.
+__global__ void __launch_bounds__(kThreads) edge_owner_emit(NodeView v, int offD, int nD, ScanTicket t,
+                                                            int* __restrict__ eOwner, int* __restrict__ eKind) {
+    const int tile = ticket_tile(t);
+    const int n = tile * kThreads + threadIdx.x, o = offD + n;
+    unsigned mask = 0;
+    if (n < nD) {
+        const float half = 1.0f / float(1 << (v.D + 1)), W = 1.0f / float(1 << v.D), W2 = W * W;
+        int nb[27], kk[27];
+        float3 cc[27];
+        load_neighbourhood(v, o, nb, cc, kk);
+        const float3 nc = cc[13];
+        int ownKey[12], own[12];
         for (int e = 0; e < 12; ++e) {
-            float3 p = edge_center(nc, e, half);
-            if (sqdist(p, cc[j]) < double(W2) && ownKey[e] > kk[j]) {
-                ownKey[e] = kk[j];
-                own[e] = nb[j];
+            ownKey[e] = 0x7fffffff;
+            own[e] = -1;
+        }
+#pragma unroll
+        for (int j = 0; j < 27; ++j) {
+            if (nb[j] == -1) continue;
+#pragma unroll
+            for (int e = 0; e < 12; ++e) {
+                float3 p = edge_center(nc, e, half);
+                if (sqdist(p, cc[j]) < double(W2) && ownKey[e] > kk[j]) {
+                    ownKey[e] = kk[j];
+                    own[e] = nb[j];
+                }
             }
         }
+        for (int e = 0; e < 12; ++e) mask |= (own[e] == o ? 1u : 0u) << e;
     }
-    for (int e = 0; e < 12; ++e) flag[12 * t + e] = (own[e] == o) ? 1 : 0;
-}
-
-__global__ void __launch_bounds__(kThreads) edge_emit(int offD, int nD, const int* __restrict__ flag,
-                                                      const int* __restrict__ addr, int* __restrict__ eOwner,
-                                                      int* __restrict__ eKind) {
-    int t = blockIdx.x * kThreads + threadIdx.x;
-    if (t >= 12 * nD || !flag[t]) return;
-    int a = addr[t];
-    eOwner[a] = offD + t / 12;
-    eKind[a] = t % 12;
+    int a = ticket_prefix<kThreads>(t, tile, __popc(mask));
+    for (int e = 0; e < 12; ++e) {
+        if (!((mask >> e) & 1u)) continue;
+        eOwner[a] = o;
+        eKind[a] = e;
+        ++a;
+    }
 }
 
 __global__ void __launch_bounds__(kThreads) edge_maintain(NodeView v, int offD, const int* __restrict__ count,
@@ -167,48 +172,53 @@ __global__ void __launch_bounds__(kThreads) edge_maintain(NodeView v, int offD, 
     }
 }
 
-__global__ void __launch_bounds__(kThreads) face_owner(NodeView v, const int* __restrict__ parent,
-                                                       int* __restrict__ flag) {
-    int o = blockIdx.x * kThreads + threadIdx.x;
-    if (o >= v.M) return;
-    int depth = v.depth[o];
-    float half = 1.0f / float(1 << (depth + 1)), W = 1.0f / float(1 << depth), W2 = W * W;
-    int nb[27], kk[27];
-    float3 cc[27];
-    load_neighbourhood(v, o, nb, cc, kk);
-    const float3 nc = cc[13];
-    int ownKey[6], own[6];
-    for (int f = 0; f < 6; ++f) {
-        ownKey[f] = 0x7fffffff;
-        own[f] = -1;
-    }
-#pragma unroll
-    for (int j = 0; j < 27; ++j) {
-        if (nb[j] == -1) continue;
-#pragma unroll
+// Owned faces of each node in (node, face) order, compacted in the same launch;
+// a face has a parent face when the node has a parent and the face lies on the
+// parent's boundary.
+__global__ void __launch_bounds__(kThreads) face_owner_emit(NodeView v, const int* __restrict__ parent, ScanTicket t,
+                                                            int* __restrict__ fOwner, int* __restrict__ fKind,
+                                                            int* __restrict__ fHasParent) {
+    const int tile = ticket_tile(t);
+    const int o = tile * kThreads + threadIdx.x;
+    unsigned mask = 0;
+    int depth = 0;
+    if (o < v.M) {
+        depth = v.depth[o];
+        const float half = 1.0f / float(1 << (depth + 1)), W = 1.0f / float(1 << depth), W2 = W * W;
+        int nb[27], kk[27];
+        float3 cc[27];
+        load_neighbourhood(v, o, nb, cc, kk);
+        const float3 nc = cc[13];
+        int ownKey[6], own[6];
         for (int f = 0; f < 6; ++f) {
-            float3 p = face_center(nc, f, half);
-            if (sqdist(p, cc[j]) < double(W2) && ownKey[f] > kk[j]) {
-                ownKey[f] = kk[j];
-                own[f] = nb[j];
+            ownKey[f] = 0x7fffffff;
+            own[f] = -1;
+        }
+#pragma unroll
+        for (int j = 0; j < 27; ++j) {
+            if (nb[j] == -1) continue;
+#pragma unroll
+            for (int f = 0; f < 6; ++f) {
+                float3 p = face_center(nc, f, half);
+                if (sqdist(p, cc[j]) < double(W2) && ownKey[f] > kk[j]) {
+                    ownKey[f] = kk[j];
+                    own[f] = nb[j];
+                }
             }
         }
+        for (int f = 0; f < 6; ++f) mask |= (own[f] == o ? 1u : 0u) << f;
     }
-    for (int f = 0; f < 6; ++f) flag[6 * o + f] = (own[f] == o) ? 1 : 0;
-}
-
-__global__ void __launch_bounds__(kThreads) face_emit(NodeView v, const int* __restrict__ parent,
-                                                      const int* __restrict__ flag, const int* __restrict__ addr,
-                                                      int* __restrict__ fOwner, int* __restrict__ fKind,
-                                                      int* __restrict__ fHasParent) {
-    int t = blockIdx.x * kThreads + threadIdx.x;
-    if (t >= 6 * v.M || !flag[t]) return;
-    int o = t / 6, f = t % 6;
-    int a = addr[t];
-    fOwner[a] = o;
-    fKind[a] = f;
-    int son = (v.key[o] >> (3 * (v.D - v.depth[o]))) & 7;
-    fHasParent[a] = (parent[o] == -1) ? -1 : (kParentFaceKind[son][f] != -1 ? 1 : -1);
+    int a = ticket_prefix<kThreads>(t, tile, __popc(mask));
+    if (!mask) return;
+    const int son = (v.key[o] >> (3 * (v.D - depth))) & 7;
+    const bool hasParent = parent[o] != -1;
+    for (int f = 0; f < 6; ++f) {
+        if (!((mask >> f) & 1u)) continue;
+        fOwner[a] = o;
+        fKind[a] = f;
+        fHasParent[a] = hasParent && kParentFaceKind[son][f] != -1 ? 1 : -1;
+        ++a;
+    }
 }
 
 __global__ void __launch_bounds__(kThreads) face_maintain(NodeView v, const int* __restrict__ count,
@@ -274,38 +284,33 @@ __global__ void __launch_bounds__(kThreads) vertex_values(NodeView v, const int*
 }
 
 // --- depth-D marching cubes (ComputeTriangleIndices.cu:401-526) ------------
-__global__ void __launch_bounds__(kThreads) edge_crossings(const int* __restrict__ count, int M,
-                                                           const int* __restrict__ eOwner,
+__global__ void __launch_bounds__(kThreads) edge_crossings(int E, int M, const int* __restrict__ eOwner,
                                                            const int* __restrict__ eKind,
                                                            const int* __restrict__ nodeVert,
-                                                           const float* __restrict__ vv, int* __restrict__ flag) {
-    int t = blockIdx.x * kThreads + threadIdx.x;
-    if (t >= *count) return;
-    int owner = eOwner[t], kind = eKind[t];
-    int v1 = nodeVert[kEdgeVertex[kind][0] * M + owner] - 1;
-    int v2 = nodeVert[kEdgeVertex[kind][1] * M + owner] - 1;
-    // a corner slot left unassigned (the reference's level-local child 0 parent
-    // rule breaks neighbour tables) makes the edge absent, not a crossing
-    flag[t] = (v1 >= 0 && v2 >= 0 && vv[v1] * vv[v2] <= 0.f) ? 1 : 0;
-}
-
-__global__ void __launch_bounds__(kThreads) crossing_points(const int* __restrict__ count, int M,
-                                                            const int* __restrict__ eOwner,
-                                                            const int* __restrict__ eKind,
-                                                            const int* __restrict__ nodeVert,
-                                                            const float* __restrict__ vv,
-                                                            const float4* __restrict__ vPos,
-                                                            const int* __restrict__ flag,
-                                                            const int* __restrict__ addr,
-                                                            float* __restrict__ out) {
-    int t = blockIdx.x * kThreads + threadIdx.x;
-    if (t >= *count || !flag[t]) return;
-    int owner = eOwner[t], kind = eKind[t];
-    int v1 = nodeVert[kEdgeVertex[kind][0] * M + owner] - 1;
-    int v2 = nodeVert[kEdgeVertex[kind][1] * M + owner] - 1;
-    float4 a = vPos[v1], b = vPos[v2];
-    float3 p = interpolate_point(make_float3(a.x, a.y, a.z), make_float3(b.x, b.y, b.z), kind >> 2, vv[v1], vv[v2]);
-    int k = addr[t];
+                                                           const float* __restrict__ vv,
+                                                           const float4* __restrict__ vPos, ScanTicket t,
+                                                           int* __restrict__ flag, int* __restrict__ addr,
+                                                           float* __restrict__ out) {
+    const int tile = ticket_tile(t);
+    const int e = tile * kThreads + threadIdx.x;
+    int v1 = -1, v2 = -1, kind = 0, cross = 0;
+    if (e < E) {
+        const int owner = eOwner[e];
+        kind = eKind[e];
+        v1 = nodeVert[kEdgeVertex[kind][0] * M + owner] - 1;
+        v2 = nodeVert[kEdgeVertex[kind][1] * M + owner] - 1;
+        // a corner slot left unassigned (the reference's level-local child 0
+        // parent rule breaks neighbour tables) makes the edge absent, not a crossing
+        cross = (v1 >= 0 && v2 >= 0 && vv[v1] * vv[v2] <= 0.f) ? 1 : 0;
+    }
+    const int k = ticket_prefix<kThreads>(t, tile, cross);
+    if (e >= E) return;
+    flag[e] = cross;
+    addr[e] = k;
+    if (!cross) return;
+    const float4 a = vPos[v1], b = vPos[v2];
+    const float3 p =
+        interpolate_point(make_float3(a.x, a.y, a.z), make_float3(b.x, b.y, b.z), kind >> 2, vv[v1], vv[v2]);
     out[3 * k] = p.x;
     out[3 * k + 1] = p.y;
     out[3 * k + 2] = p.z;
@@ -314,48 +319,43 @@ __global__ void __launch_bounds__(kThreads) crossing_points(const int* __restric
 // A cube emits triangles only when all its corners and edges are present and
 // every edge its triangles use crosses; otherwise (only where the reference's
 // level-local child 0 parent rule has broken the tables and the reference
-// reads out of bounds) it emits none.
-__global__ void __launch_bounds__(kThreads) cube_classes(int offD, int nD, int M, const int* __restrict__ nodeVert,
-                                                         const int* __restrict__ nodeEdge,
-                                                         const int* __restrict__ crossFlag,
-                                                         const float* __restrict__ vv, int* __restrict__ cube,
-                                                         int* __restrict__ triNum) {
-    int t = blockIdx.x * kThreads + threadIdx.x;
-    if (t >= nD) return;
-    int o = offD + t, c = 0;
-    bool ok = true;
-    for (int i = 0; i < 8; ++i) {
-        int v = nodeVert[i * M + o];
-        if (v == 0) ok = false;
-        else if (vv[v - 1] < 0.f) c |= 1 << i;
+// reads out of bounds) it emits none.  The triangles of each node are written
+// in node order, compacted in the same launch, and the faces bounding an
+// emitted triangle's edges are marked up the parent chain.
+__global__ void __launch_bounds__(kThreads) cube_triangles(
+    int offD, int nD, int M, const int* __restrict__ nodeVert, const int* __restrict__ nodeEdge,
+    const int* __restrict__ crossFlag, const float* __restrict__ vv, const int* __restrict__ vexAddr,
+    const int* __restrict__ nodeFace, const int* __restrict__ parent, const int* __restrict__ faceHasParent,
+    ScanTicket t, int* __restrict__ cube, int* __restrict__ tri, int* __restrict__ hasInter) {
+    const int tile = ticket_tile(t);
+    const int n = tile * kThreads + threadIdx.x, o = offD + n;
+    int c = 0;
+    if (n < nD) {
+        bool ok = true;
+        for (int i = 0; i < 8; ++i) {
+            const int v = nodeVert[i * M + o];
+            if (v == 0) ok = false;
+            else if (vv[v - 1] < 0.f) c |= 1 << i;
+        }
+        for (int e = 0; e < 12; ++e)
+            if (nodeEdge[e * nD + n] == 0) ok = false;
+        if (ok) {
+            const uint64_t w = mc_word(c);
+            for (int k = 0, m = 3 * mc_tri_count(c); k < m; ++k)
+                if (!crossFlag[nodeEdge[int((w >> (4 * k)) & 0xF) * nD + n] - 1]) ok = false;
+        }
+        if (!ok) c = 0;
+        cube[n] = c;
     }
-    for (int e = 0; e < 12; ++e)
-        if (nodeEdge[e * nD + t] == 0) ok = false;
-    if (ok) {
-        uint64_t w = mc_word(c);
-        for (int k = 0, n = 3 * mc_tri_count(c); k < n; ++k)
-            if (!crossFlag[nodeEdge[int((w >> (4 * k)) & 0xF) * nD + t] - 1]) ok = false;
-    }
-    if (!ok) c = 0;
-    cube[t] = c;
-    triNum[t] = mc_tri_count(c);
-}
-
-__global__ void __launch_bounds__(kThreads) emit_triangles_and_faces(
-    int offD, int nD, int M, const int* __restrict__ cube, const int* __restrict__ triAddr,
-    const int* __restrict__ nodeEdge, const int* __restrict__ vexAddr, const int* __restrict__ nodeFace,
-    const int* __restrict__ parent, const int* __restrict__ faceHasParent, int* __restrict__ tri,
-    int* __restrict__ hasInter) {
-    int t = blockIdx.x * kThreads + threadIdx.x;
-    if (t >= nD) return;
-    int o = offD + t, c = cube[t];
-    uint64_t w = mc_word(c);
-    int nt = mc_tri_count(c), base = triAddr[t];
+    const int nt = mc_tri_count(c);
+    const int base = ticket_prefix<kThreads>(t, tile, nt);
+    if (nt == 0) return;
+    const uint64_t w = mc_word(c);
     int edgeHas = 0;
     for (int k = 0; k < 3 * nt; ++k) {
-        int e = int((w >> (4 * k)) & 0xF);
+        const int e = int((w >> (4 * k)) & 0xF);
         edgeHas |= 1 << e;
-        tri[3 * base + k] = vexAddr[nodeEdge[e * nD + t] - 1];
+        tri[3 * base + k] = vexAddr[nodeEdge[e * nD + n] - 1];
     }
     for (int f = 0; f < 6; ++f) {
         bool mark = false;
@@ -376,56 +376,46 @@ __global__ void __launch_bounds__(kThreads) emit_triangles_and_faces(
 }
 
 // --- subdivision roots (ProcessLeafNodesAtOtherDepth :555-587, copy_if) -----
-__global__ void __launch_bounds__(kThreads) root_flags(int count, int M, const int* __restrict__ neigh,
-                                                       const int* __restrict__ nodeVert,
-                                                       const int* __restrict__ nodeFace,
-                                                       const int* __restrict__ childBase,
-                                                       const float* __restrict__ vv,
-                                                       const int* __restrict__ hasInter, int* __restrict__ flag) {
-    int o = blockIdx.x * kThreads + threadIdx.x;
-    if (o >= count) return;
-    // unassigned corner and face slots are skipped; the sign reference is the
-    // first present corner (corner 0 in a consistent tree)
-    int i0 = 0;
-    while (i0 < 8 && nodeVert[i0 * M + o] == 0) ++i0;
-    int hasTri = 0;
-    if (i0 < 8) {
-        float sign = vv[nodeVert[i0 * M + o] - 1] < 0.f ? -1.f : 1.f;
-        for (int i = i0 + 1; i < 8; ++i) {
-            int v = nodeVert[i * M + o];
-            if (v != 0 && sign * vv[v - 1] < 0.f) {
-                hasTri = 1;
-                break;
+// Roots in ascending node order, compacted in the same launch.  The first node
+// of each level records how many roots precede it (levelStart), so the roots
+// of a depth are a difference of two starts: precomputeSubdivideDepth reads a
+// root's depth through neighs[13], and a root is its own neighbour here, so
+// that is the root's level.
+__global__ void __launch_bounds__(kThreads) root_select(int count, int M, Levels L, const int* __restrict__ neigh,
+                                                        const int* __restrict__ nodeVert,
+                                                        const int* __restrict__ nodeFace,
+                                                        const int* __restrict__ childBase,
+                                                        const float* __restrict__ vv,
+                                                        const int* __restrict__ hasInter, ScanTicket t,
+                                                        int* __restrict__ roots, int* __restrict__ levelStart) {
+    const int tile = ticket_tile(t);
+    const int o = tile * kThreads + threadIdx.x;
+    int flag = 0;
+    if (o < count && childBase[o] < 0 && neigh[13 * M + o] == o) {
+        // unassigned corner and face slots are skipped; the sign reference is
+        // the first present corner (corner 0 in a consistent tree)
+        int i0 = 0;
+        while (i0 < 8 && nodeVert[i0 * M + o] == 0) ++i0;
+        if (i0 < 8) {
+            const float sign = vv[nodeVert[i0 * M + o] - 1] < 0.f ? -1.f : 1.f;
+            for (int i = i0 + 1; i < 8; ++i) {
+                const int v = nodeVert[i * M + o];
+                if (v != 0 && sign * vv[v - 1] < 0.f) {
+                    flag = 1;
+                    break;
+                }
             }
         }
-    }
-    int hi = 0;
-    for (int f = 0; f < 6; ++f) {
-        int fs = nodeFace[f * M + o];
-        if (fs != 0 && hasInter[fs - 1]) {
-            hi = 1;
-            break;
+        for (int f = 0; f < 6 && !flag; ++f) {
+            const int fs = nodeFace[f * M + o];
+            if (fs != 0 && hasInter[fs - 1]) flag = 1;
         }
     }
-    // a root must be its own neighbour (the virtual subtree is built from it)
-    flag[o] = (childBase[o] < 0 && (hasTri || hi) && neigh[13 * M + o] == o) ? 1 : 0;
-}
-
-// Roots in ascending node order, and roots per depth.  precomputeSubdivideDepth
-// reads a root's depth through neighs[13]; a root is its own neighbour
-// (root_flags), so that is the root's level, and the count of a depth is the
-// difference of the root scan at the level bounds.
-__global__ void __launch_bounds__(kThreads) root_emit(int count, Levels L, const int* __restrict__ flag,
-                                                      const int* __restrict__ addr, const int* __restrict__ total,
-                                                      int* __restrict__ roots, int* __restrict__ depthHist) {
-    int o = blockIdx.x * kThreads + threadIdx.x;
-    if (blockIdx.x == 0 && int(threadIdx.x) < L.D) {
-        const int d = threadIdx.x;
-        const int hi = L.base[d + 1] < count ? addr[L.base[d + 1]] : *total;
-        depthHist[d] = hi - addr[L.base[d]];
-    }
-    if (o >= count || !flag[o]) return;
-    roots[addr[o]] = o;
+    const int a = ticket_prefix<kThreads>(t, tile, flag);
+    if (o >= count) return;
+    for (int d = 0; d < L.D; ++d)
+        if (L.base[d] == o) levelStart[d] = a;
+    if (flag) roots[a] = o;
 }
 
 }  // namespace
@@ -437,9 +427,6 @@ void build_geometry(Ctx& c) {
     const int M = o.M, D = L.D, nD = L.count[D], offD = L.base[D];
     hipStream_t s = c.stream;
     NodeView v{M, D, o.key, o.depth, o.neigh, o.basis};
-    size_t nMax = std::max<size_t>(size_t(8) * M, size_t(12) * nD);
-    int* flag = c.buf<int>("geo_flag", nMax);
-    int* addr = c.buf<int>("geo_addr", nMax);
     g.counts = c.buf<int>("geo_counts", 4);
     g.vtxOwner = c.buf<int>("vtx_owner", size_t(8) * M);
     g.vtxKind = c.buf<int>("vtx_kind", size_t(8) * M);
@@ -449,27 +436,25 @@ void build_geometry(Ctx& c) {
     g.nodeEdge = c.buf<int>("node_edge", size_t(12) * nD);
     zero_spans({{g.nodeVert, size_t(8) * M}, {g.nodeFace, size_t(6) * M}, {g.nodeEdge, size_t(12) * nD}}, s);
     // vertices (all depths)
-    vertex_owner<<<div_up(M, kThreads), kThreads, 0, s>>>(v, flag);
-    scan_exclusive(flag, addr, 8 * M, g.counts + 0, c.scan_for(size_t(8) * M), s);
-    vertex_emit<<<div_up(size_t(8) * M, kThreads), kThreads, 0, s>>>(v, flag, addr, g.vtxOwner, g.vtxKind, g.vtxPos);
+    const int tilesM = div_up(M, kThreads), tilesD = div_up(nD, kThreads);
+    ScanState& scan = c.scan_for_tiles(size_t(tilesM));
+    vertex_owner_emit<<<tilesM, kThreads, 0, s>>>(v, scan_ticket(scan, tilesM, g.counts + 0), g.vtxOwner, g.vtxKind,
+                                                   g.vtxPos);
     vertex_maintain<<<div_up(size_t(8) * M, kThreads), kThreads, 0, s>>>(v, g.counts + 0, g.vtxOwner, g.vtxPos,
                                                                         g.nodeVert);
     // edges (depth D)
     g.edgeOwner = c.buf<int>("edge_owner", size_t(12) * nD);
     g.edgeKind = c.buf<int>("edge_kind", size_t(12) * nD);
-    edge_owner<<<div_up(nD, kThreads), kThreads, 0, s>>>(v, offD, nD, flag);
-    scan_exclusive(flag, addr, 12 * nD, g.counts + 1, c.scan_for(size_t(12) * nD), s);
-    edge_emit<<<div_up(size_t(12) * nD, kThreads), kThreads, 0, s>>>(offD, nD, flag, addr, g.edgeOwner, g.edgeKind);
+    edge_owner_emit<<<tilesD, kThreads, 0, s>>>(v, offD, nD, scan_ticket(scan, tilesD, g.counts + 1), g.edgeOwner,
+                                                 g.edgeKind);
     edge_maintain<<<div_up(size_t(12) * nD, kThreads), kThreads, 0, s>>>(v, offD, g.counts + 1, g.edgeOwner,
                                                                         g.edgeKind, nD, g.nodeEdge);
     // faces (all depths)
     g.faceOwner = c.buf<int>("face_owner", size_t(6) * M);
     g.faceKind = c.buf<int>("face_kind", size_t(6) * M);
     g.faceHasParent = c.buf<int>("face_hasparent", size_t(6) * M);
-    face_owner<<<div_up(M, kThreads), kThreads, 0, s>>>(v, o.parent, flag);
-    scan_exclusive(flag, addr, 6 * M, g.counts + 2, c.scan_for(size_t(6) * M), s);
-    face_emit<<<div_up(size_t(6) * M, kThreads), kThreads, 0, s>>>(v, o.parent, flag, addr, g.faceOwner, g.faceKind,
-                                                                   g.faceHasParent);
+    face_owner_emit<<<tilesM, kThreads, 0, s>>>(v, o.parent, scan_ticket(scan, tilesM, g.counts + 2), g.faceOwner,
+                                                 g.faceKind, g.faceHasParent);
     face_maintain<<<div_up(size_t(6) * M, kThreads), kThreads, 0, s>>>(v, g.counts + 2, g.faceOwner, g.faceKind,
                                                                       g.nodeFace);
     // vertex implicit values
@@ -498,29 +483,22 @@ void depth_mc(Ctx& c, MainMc& mm, float* rawV, int* rawT) {
     zero_spans({{mm.counts, size_t(PSR_MAX_DEPTH + 8)}, {mm.hasInter, size_t(F)}}, s);
     int* cflag = c.buf<int>("mc_cross_flag", E);
     mm.vexAddr = c.buf<int>("mc_vex_addr", E);
-    edge_crossings<<<div_up(E, kThreads), kThreads, 0, s>>>(g.counts + 1, M, g.edgeOwner, g.edgeKind, g.nodeVert,
-                                                            g.vvalue, cflag);
-    scan_exclusive(cflag, mm.vexAddr, E, mm.counts + 0, c.scan_for(size_t(E)), s);
-    crossing_points<<<div_up(E, kThreads), kThreads, 0, s>>>(g.counts + 1, M, g.edgeOwner, g.edgeKind, g.nodeVert,
-                                                             g.vvalue, g.vtxPos, cflag, mm.vexAddr, rawV);
     mm.cube = c.buf<int>("mc_cube", nD);
-    int* triNum = c.buf<int>("mc_trinum", nD);
-    mm.triAddr = c.buf<int>("mc_triaddr", nD);
-    cube_classes<<<div_up(nD, kThreads), kThreads, 0, s>>>(offD, nD, M, g.nodeVert, g.nodeEdge, cflag, g.vvalue,
-                                                           mm.cube, triNum);
-    scan_exclusive(triNum, mm.triAddr, nD, mm.counts + 1, c.scan_for(size_t(nD)), s);
-    emit_triangles_and_faces<<<div_up(nD, kThreads), kThreads, 0, s>>>(offD, nD, M, mm.cube, mm.triAddr, g.nodeEdge,
-                                                                       mm.vexAddr, g.nodeFace, o.parent,
-                                                                       g.faceHasParent, rawT, mm.hasInter);
+    const int tilesE = div_up(E, kThreads), tilesD = div_up(nD, kThreads), tilesR = div_up(offD, kThreads);
+    ScanState& scan = c.scan_for_tiles(size_t(std::max(tilesE, std::max(tilesD, tilesR))));
+    if (E > 0)
+        edge_crossings<<<tilesE, kThreads, 0, s>>>(E, M, g.edgeOwner, g.edgeKind, g.nodeVert, g.vvalue, g.vtxPos,
+                                                   scan_ticket(scan, tilesE, mm.counts + 0), cflag, mm.vexAddr, rawV);
+    cube_triangles<<<tilesD, kThreads, 0, s>>>(offD, nD, M, g.nodeVert, g.nodeEdge, cflag, g.vvalue, mm.vexAddr,
+                                               g.nodeFace, o.parent, g.faceHasParent,
+                                               scan_ticket(scan, tilesD, mm.counts + 1), mm.cube, rawT, mm.hasInter);
     // subdivision roots: leaves above depth D with a sign change or a marked face
-    int* rflag = c.buf<int>("root_flag", offD);
-    int* raddr = c.buf<int>("root_addr", offD);
     mm.roots = c.buf<int>("roots", offD);
-    root_flags<<<div_up(offD, kThreads), kThreads, 0, s>>>(offD, M, o.neigh, g.nodeVert, g.nodeFace, o.childBase,
-                                                           g.vvalue, mm.hasInter, rflag);
-    scan_exclusive(rflag, raddr, offD, mm.counts + 2, c.scan_for(size_t(offD)), s);
-    root_emit<<<div_up(offD, kThreads), kThreads, 0, s>>>(offD, L, rflag, raddr, mm.counts + 2, mm.roots,
-                                                          mm.counts + 4);
+    int* levelStart = mm.counts + 4;
+    if (offD > 0)
+        root_select<<<tilesR, kThreads, 0, s>>>(offD, M, L, o.neigh, g.nodeVert, g.nodeFace, o.childBase, g.vvalue,
+                                                mm.hasInter, scan_ticket(scan, tilesR, mm.counts + 2), mm.roots,
+                                                levelStart);
     PSR_HIP_CHECK(hipMemcpyAsync(c.hostPinned, mm.counts, sizeof(int) * (PSR_MAX_DEPTH + 8), hipMemcpyDeviceToHost,
                                  s));
     PSR_HIP_CHECK(hipMemcpyAsync(c.hostPinned + PSR_MAX_DEPTH + 8, o.q1Fired, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -529,7 +507,10 @@ void depth_mc(Ctx& c, MainMc& mm, float* rawV, int* rawT) {
     mm.nCross = c.hostPinned[0];
     mm.nTri = c.hostPinned[1];
     mm.nRoots = c.hostPinned[2];
-    for (int d = 0; d <= D; ++d) mm.rootsPerDepth[d] = c.hostPinned[4 + d];
+    // roots of depth d: the difference of the level starts (the total past the last level)
+    auto start = [&](int d) { return L.base[d] < offD ? c.hostPinned[4 + d] : mm.nRoots; };
+    for (int d = 0; d < D; ++d) mm.rootsPerDepth[d] = start(d + 1) - start(d);
+    mm.rootsPerDepth[D] = 0;
     PSR_HIP_CHECK(hipGetLastError());
 }
 

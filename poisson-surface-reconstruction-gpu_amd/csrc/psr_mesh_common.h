@@ -96,11 +96,10 @@ __device__ __forceinline__ float3 interpolate_point(float3 p1, float3 p2, int di
 // Per-call state of the depth-D marching cubes.
 struct MainMc {
     int* vexAddr = nullptr;   // crossing index per depth-D edge
-    int* cube = nullptr;      // cube category per depth-D node
-    int* triAddr = nullptr;   // first triangle per depth-D node
+    int* cube = nullptr;      // cube category per depth-D node (exported)
     int* hasInter = nullptr;  // surface-edge intersection flag per face
     int* roots = nullptr;     // subdivision roots, ascending node index
-    int* counts = nullptr;    // device: [0] crossings, [1] triangles, [2] roots, [4..] roots per depth
+    int* counts = nullptr;    // device: [0] crossings, [1] triangles, [2] roots, [4..] roots before each level
     int nCross = 0, nTri = 0, nRoots = 0;
     int rootsPerDepth[PSR_MAX_DEPTH + 1] = {0};
     bool q1Fired = false;  // the level-local child 0 rule changed a parent somewhere

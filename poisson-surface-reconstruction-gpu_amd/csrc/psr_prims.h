@@ -21,6 +21,17 @@ struct ScanState {
 constexpr int kScanRing = 64;             // tile counters, one per epoch modulo the ring
 constexpr unsigned kScanEpochMax = 1u << 29;
 size_t scan_tiles(size_t n);
+// One call's share of the scan state, for kernels that scan their own counts
+// (psr_tile_scan.h): tiles of the launch, and where the grand total goes.
+struct ScanTicket {
+    unsigned long long* status;
+    unsigned* counters;
+    unsigned epoch;
+    int tiles;
+    int* total;
+};
+// Takes the next epoch for a launch of `tiles` workgroups (the state must hold them).
+ScanTicket scan_ticket(ScanState& st, int tiles, int* total);
 // out[i] = sum(in[0..i)); *total = sum(in[0..n)) if total != nullptr.  in may equal out.
 void scan_exclusive(const int* in, int* out, int n, int* total, ScanState& st, hipStream_t s);
 

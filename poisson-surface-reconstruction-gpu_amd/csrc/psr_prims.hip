@@ -1,6 +1,7 @@
 // Wave64 scan and radix sort.
 #include "psr_common.h"
 #include "psr_prims.h"
+#include "psr_tile_scan.h"
 
 #include <algorithm>
 #include <stdexcept>
@@ -14,57 +15,10 @@ static constexpr int kScanThreads = 256;
 static constexpr int kScanItems = 8;
 static constexpr int kScanTile = kScanThreads * kScanItems;
 
-__device__ __forceinline__ int wave_incl_scan(int v) {
-    int lane = __lane_id();
-#pragma unroll
-    for (int off = 1; off < PSR_WAVE; off <<= 1) {
-        int t = __shfl_up(v, off, PSR_WAVE);
-        if (lane >= off) v += t;
-    }
-    return v;
-}
-
-// Block-wide exclusive scan of one value per thread; returns the block total.
-template <int NT>
-__device__ __forceinline__ int block_excl_scan(int v, int& excl) {
-    __shared__ int wsum[NT / PSR_WAVE];
-    __shared__ int wtotal;
-    int lane = __lane_id(), wid = threadIdx.x / PSR_WAVE;
-    int inc = wave_incl_scan(v);
-    if (lane == PSR_WAVE - 1) wsum[wid] = inc;
-    __syncthreads();
-    if (wid == 0) {
-        int w = lane < NT / PSR_WAVE ? wsum[lane] : 0;
-        int wi = wave_incl_scan(w);
-        if (lane < NT / PSR_WAVE) wsum[lane] = wi - w;
-        if (lane == NT / PSR_WAVE - 1) wtotal = wi;
-    }
-    __syncthreads();
-    excl = inc - v + wsum[wid];
-    int total = wtotal;
-    __syncthreads();
-    return total;
-}
-
-// Single-pass scan with decoupled look-back.  Tiles take their index from an
-// arrival counter, so every tile a tile waits on has already started; a tile
-// publishes its own sum first (flag 1), then walks back over its predecessors
-// until one has published an inclusive prefix (flag 2), and publishes its own
-// inclusive prefix.  A status word is one 64-bit atomic: epoch (bits 34-63),
-// flag (32-33), value (0-31), so words of earlier calls read as not ready.
-__global__ void __launch_bounds__(kScanThreads) scan_single_pass(const int* in, int* out, int n, int tiles,
-                                                                 int* __restrict__ total,
-                                                                 unsigned long long* status,
-                                                                 unsigned* counters, unsigned epoch) {
-    __shared__ int sTile, sPrefix;
-    if (threadIdx.x == 0) {
-        const int t = int(atomicAdd(&counters[epoch % kScanRing], 1u));
-        if (t == 0) counters[(epoch + 1) % kScanRing] = 0u;  // the next call's counter
-        sTile = t;
-    }
-    __syncthreads();
-    const int tile = sTile;
-    // each thread owns kScanItems consecutive elements of the tile
+// Single-pass scan with decoupled look-back (psr_tile_scan.h); each thread
+// owns kScanItems consecutive elements of its tile.
+__global__ void __launch_bounds__(kScanThreads) scan_single_pass(const int* in, int* out, int n, ScanTicket t) {
+    const int tile = ticket_tile(t);
     const int base = tile * kScanTile + threadIdx.x * kScanItems;
     int v[kScanItems];
     int sum = 0;
@@ -74,57 +28,7 @@ __global__ void __launch_bounds__(kScanThreads) scan_single_pass(const int* in, 
         v[k] = (i < n) ? in[i] : 0;
         sum += v[k];
     }
-    int excl;
-    const int agg = block_excl_scan<kScanThreads>(sum, excl);
-    if (threadIdx.x < PSR_WAVE) {
-        // the first wave looks back over windows of 64 predecessors at once
-        const int lane = threadIdx.x;
-        const unsigned long long tag = static_cast<unsigned long long>(epoch) << 34;
-        int prefix = 0;
-        if (tile > 0) {
-            if (lane == 0)
-                __hip_atomic_store(&status[tile], tag | (1ull << 32) | unsigned(agg), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-            // every predecessor has started and publishes without waiting, so each
-            // wait ends; the spin bound only guards against a broken state
-            for (int j0 = tile - 1;; j0 -= PSR_WAVE) {
-                const int j = j0 - lane;
-                unsigned flag = 2u;  // before the first tile: an inclusive zero
-                int val = 0;
-                if (j >= 0) {
-                    for (unsigned spins = 0;; ++spins) {
-                        const unsigned long long w =
-                            __hip_atomic_load(&status[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        flag = unsigned(w >> 32) & 3u;
-                        if (unsigned(w >> 34) == epoch && flag != 0u) {
-                            val = int(unsigned(w));
-                            break;
-                        }
-                        if (spins > (1u << 24)) {
-                            flag = 2u;
-                            break;
-                        }
-                        __builtin_amdgcn_s_sleep(1);
-                    }
-                }
-                const unsigned long long incl = __ballot(flag == 2u);
-                const int stop = incl ? __builtin_ctzll(incl) : PSR_WAVE;  // closest inclusive prefix
-                int c = lane <= stop ? val : 0;
-#pragma unroll
-                for (int off = PSR_WAVE / 2; off > 0; off >>= 1) c += __shfl_xor(c, off, PSR_WAVE);
-                prefix += c;
-                if (incl) break;
-            }
-        }
-        if (lane == 0) {
-            __hip_atomic_store(&status[tile], tag | (2ull << 32) | unsigned(prefix + agg), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-            if (total && tile == tiles - 1) *total = prefix + agg;
-            sPrefix = prefix;
-        }
-    }
-    __syncthreads();
-    int run = excl + sPrefix;
+    int run = ticket_prefix<kScanThreads>(t, tile, sum);
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
         const int i = base + k;
@@ -169,16 +73,20 @@ void zero_spans(std::initializer_list<ZeroSpan> spans, hipStream_t s) {
 
 size_t scan_tiles(size_t n) { return div_up(n, kScanTile); }
 
+ScanTicket scan_ticket(ScanState& st, int tiles, int* total) {
+    if (size_t(tiles) > st.capacity || st.epoch + 1 >= kScanEpochMax)
+        throw std::runtime_error("scan state not reserved for this size");
+    ++st.epoch;
+    return ScanTicket{st.status, st.counters, st.epoch, tiles, total};
+}
+
 void scan_exclusive(const int* in, int* out, int n, int* total, ScanState& st, hipStream_t s) {
     if (n <= 0) {
         if (total) set_int<<<1, 1, 0, s>>>(total, 0);
         return;
     }
     const int tiles = int(scan_tiles(size_t(n)));
-    if (size_t(tiles) > st.capacity || st.epoch + 1 >= kScanEpochMax)
-        throw std::runtime_error("scan state not reserved for this size");
-    ++st.epoch;
-    scan_single_pass<<<tiles, kScanThreads, 0, s>>>(in, out, n, tiles, total, st.status, st.counters, st.epoch);
+    scan_single_pass<<<tiles, kScanThreads, 0, s>>>(in, out, n, scan_ticket(st, tiles, total));
     PSR_HIP_CHECK(hipGetLastError());
 }
 

@@ -25,6 +25,7 @@
 // triangle is that of the materialised formulation.
 #include "psr_mesh_common.h"
 #include "psr_prims.h"
+#include "psr_tile_scan.h"
 
 #include <algorithm>
 #include <cstring>
@@ -1199,27 +1200,28 @@ __global__ void __launch_bounds__(kThreads) zero_floats(float* __restrict__ a, s
 }
 
 // markValidMeshVertexIndex (:1170-1181): a vertex with |x| < 1e-6 is dropped;
-// triangle indices are not remapped (insertTriangle :1247-1282).
-__global__ void __launch_bounds__(kThreads) keep_flags(const float* __restrict__ v, int n, int* __restrict__ keep) {
-    int i = blockIdx.x * kThreads + threadIdx.x;
-    if (i < n) keep[i] = fabsf(v[3 * size_t(i)]) < 1e-6f ? 0 : 1;
-    if (i == n) keep[i] = 0;
+// triangle indices are not remapped (insertTriangle :1247-1282).  The kept
+// vertices are compacted in order in the same launch; their count goes to t.total.
+__global__ void __launch_bounds__(kThreads) keep_compact(const float* __restrict__ v, int n, ScanTicket t,
+                                                         float* __restrict__ out) {
+    const int tile = ticket_tile(t);
+    const int i = tile * kThreads + threadIdx.x;
+    float x = 0.f, y = 0.f, z = 0.f;
+    int keep = 0;
+    if (i < n) {
+        x = v[3 * size_t(i)];
+        keep = fabsf(x) < 1e-6f ? 0 : 1;
+        if (keep) {
+            y = v[3 * size_t(i) + 1];
+            z = v[3 * size_t(i) + 2];
+        }
+    }
+    const int a = ticket_prefix<kThreads>(t, tile, keep);
+    if (!keep) return;
+    out[3 * size_t(a)] = x;
+    out[3 * size_t(a) + 1] = y;
+    out[3 * size_t(a) + 2] = z;
 }
-
-__global__ void __launch_bounds__(kThreads) compact_vertices(const float* __restrict__ v, int n,
-                                                             const int* __restrict__ keep,
-                                                             const int* __restrict__ addr,
-                                                             const int* __restrict__ keptBase,
-                                                             float* __restrict__ out) {
-    int i = blockIdx.x * kThreads + threadIdx.x;
-    if (i >= n || !keep[i]) return;
-    size_t a = size_t(*keptBase) + addr[i];
-    out[3 * a] = v[3 * size_t(i)];
-    out[3 * a + 1] = v[3 * size_t(i) + 1];
-    out[3 * a + 2] = v[3 * size_t(i) + 2];
-}
-
-__global__ void add_count(int* __restrict__ acc, const int* __restrict__ add) { *acc += *add; }
 
 __global__ void fill_root_pos(const int* __restrict__ roots, int n, int* __restrict__ rootPos) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1263,25 +1265,14 @@ void extract_mesh(Ctx& c) {
     MainMc mm;
     depth_mc(c, mm, mainV, outT);
     c.stats.n_subdivision_roots = mm.nRoots;
-    int* acc = c.buf<int>("mesh_counts", 4);  // [0] vertices kept so far, [1] batch kept count
+    int* acc = c.buf<int>("mesh_counts", 4);  // [0] vertices kept by the depth-D batch
     fill_int<<<1, kThreads, 0, st>>>(acc, 4, 0);
-    size_t rawUpper = 0, triTotal = 0, keptTotal = 0;
-    float* outV = nullptr;
-    // Appends one batch of raw vertices: keep flags, local scan, compaction at
-    // the running kept count.  localKept holds n + 1 entries afterwards.
-    auto append_vertices = [&](const float* rawV, int n, int* keep, int* localKept) {
-        outV = c.buf_keep<float>("mesh_vertices", 3 * (rawUpper + n), 3 * rawUpper);
-        keep_flags<<<div_up(size_t(n) + 1, kThreads), kThreads, 0, st>>>(rawV, n, keep);
-        scan_exclusive(keep, localKept, n + 1, acc + 1, c.scan_for(size_t(n) + 1), st);
-        compact_vertices<<<div_up(std::max(n, 1), kThreads), kThreads, 0, st>>>(rawV, n, keep, localKept, acc, outV);
-        rawUpper += n;
-    };
-    {
-        int* keep = c.buf<int>("keep_flag", size_t(mm.nCross) + 1);
-        int* kept = c.buf<int>("keep_scan", size_t(mm.nCross) + 1);
-        append_vertices(mainV, mm.nCross, keep, kept);
-        add_count<<<1, 1, 0, st>>>(acc, acc + 1);
-        triTotal = mm.nTri;
+    size_t triTotal = mm.nTri, keptTotal = 0;
+    float* outV = c.buf_keep<float>("mesh_vertices", size_t(3) * std::max(mm.nCross, 1), 0);
+    if (mm.nCross > 0) {
+        const int tiles = div_up(mm.nCross, kThreads);
+        keep_compact<<<tiles, kThreads, 0, st>>>(mainV, mm.nCross,
+                                                 scan_ticket(c.scan_for_tiles(size_t(tiles)), tiles, acc), outV);
     }
     // ---- subdivision sets: coarse roots one per set, finer roots one set per depth
     // (ComputeTriangleIndices.h:371: roots above depth D-4 are coarse)
@@ -1446,7 +1437,6 @@ void extract_mesh(Ctx& c) {
     c.mesh.triangles = outT;
     c.mesh.nv = int(keptTotal);
     c.mesh.nt = int(triTotal);
-    c.mesh.counts = acc;
     c.stats.n_mesh_vertices = c.mesh.nv;
     c.stats.n_mesh_triangles = c.mesh.nt;
     PSR_HIP_CHECK(hipGetLastError());
