@@ -86,6 +86,8 @@ typedef struct {
     int cg_nnz[PSR_STATS_MAX_DEPTH + 1]; /* stored operator entries per depth */
     int cg_outer[PSR_STATS_MAX_DEPTH + 1]; /* PSR_PRECISION_MIXED: outer refinement steps per depth; 1 otherwise */
     int cg_row_form;      /* 1: one row per CG thread, stencil held in registers; 0: several rows per thread */
+    double ms_cg_kernel_sum; /* ms_cg_kernel summed over the context's completed runs */
+    uint32_t n_runs;         /* completed runs of the context */
 } psr_stats;
 
 /* Arrays that psr_export can copy to host memory (for parity checks). */

@@ -70,7 +70,7 @@ class Stats(ctypes.Structure):
                 ("ms_solve", ctypes.c_float), ("ms_mesh", ctypes.c_float), ("ms_total", ctypes.c_float),
                 ("ms_cg_kernel", ctypes.c_float), ("cg_workgroups", ctypes.c_uint32),
                 ("cg_nnz", ctypes.c_int * _MAXD), ("cg_outer", ctypes.c_int * _MAXD),
-                ("cg_row_form", ctypes.c_int)]
+                ("cg_row_form", ctypes.c_int), ("ms_cg_kernel_sum", ctypes.c_double), ("n_runs", ctypes.c_uint32)]
 
     def as_dict(self):
         d = self.depth
@@ -86,7 +86,8 @@ class Stats(ctypes.Structure):
             "ms_octree": self.ms_octree, "ms_field": self.ms_field, "ms_solve": self.ms_solve,
             "ms_mesh": self.ms_mesh, "ms_total": self.ms_total, "ms_cg_kernel": self.ms_cg_kernel,
             "cg_workgroups": self.cg_workgroups, "cg_nnz": list(self.cg_nnz)[:d + 1],
-            "cg_outer": list(self.cg_outer)[:d + 1], "cg_row_form": self.cg_row_form, "q1_fired": self.q1_fired,
+            "cg_outer": list(self.cg_outer)[:d + 1], "cg_row_form": self.cg_row_form,
+            "ms_cg_kernel_sum": self.ms_cg_kernel_sum, "n_runs": self.n_runs, "q1_fired": self.q1_fired,
         }
 
 

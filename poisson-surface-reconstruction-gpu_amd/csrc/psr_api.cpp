@@ -107,8 +107,7 @@ int run(psr_ctx* ctx, const psr_points* pts, psr_mesh* out) {
     build_geometry(c);
     extract_mesh(c);
     PSR_HIP_CHECK(hipEventRecord(c.ev[4], s));
-    PSR_HIP_CHECK(hipMemcpyAsync(c.hostResults, c.results(), sizeof(DevResults), hipMemcpyDeviceToHost, s));
-    PSR_HIP_CHECK(hipStreamSynchronize(s));
+    c.wait();  // the results reached hostResults with the vertex values (build_geometry)
     // statistics
     psr_stats& st = c.stats;
     const Octree& o = c.oct;
@@ -136,6 +135,9 @@ int run(psr_ctx* ctx, const psr_points* pts, psr_mesh* out) {
     st.ms_mesh = elapsed(c.ev[3], c.ev[4]);
     st.ms_total = elapsed(c.ev[0], c.ev[4]);
     st.ms_cg_kernel = elapsed(c.ev[8], c.ev[9]);
+    c.cgMsSum += st.ms_cg_kernel;
+    st.ms_cg_kernel_sum = c.cgMsSum;
+    st.n_runs = ++c.runs;
     out->vertices = c.mesh.vertices;
     out->triangles = c.mesh.triangles;
     out->n_vertices = uint32_t(c.mesh.nv);
@@ -195,8 +197,12 @@ int psr_create(const psr_config* cfg, void* hip_stream, psr_ctx** out) {
             c.ownStream = true;
         }
         for (auto& e : c.ev) PSR_HIP_CHECK(hipEventCreate(&e));
-        PSR_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&c.hostPinned), 64 * sizeof(int)));
-        PSR_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&c.hostResults), sizeof(psr::DevResults)));
+        PSR_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&c.hostPinned), psr::kMailWords * sizeof(int),
+                                    hipHostMallocMapped | hipHostMallocCoherent));
+        PSR_HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&c.mail), c.hostPinned, 0));
+        PSR_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&c.hostResults), sizeof(psr::DevResults),
+                                    hipHostMallocMapped | hipHostMallocCoherent));
+        PSR_HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&c.mailResults), c.hostResults, 0));
         psr::build_tables(c);  // one-time, like ComputeVectorField's constructor (ComputeVectorField.cpp:11-15)
         PSR_HIP_CHECK(hipStreamSynchronize(c.stream));
     } catch (const psr::HipError& e) {

@@ -118,6 +118,15 @@ struct DevResults {
 static_assert(offsetof(DevResults, cgErr) == sizeof(int) * (3 * (PSR_MAX_DEPTH + 1)),
               "iterations, nnz and the error flag are cleared as one range");
 
+// Read-back mailbox slots (Ctx::hostPinned / Ctx::mail).  Each stage's
+// kernels write the values the host needs next; the host reads them after
+// Ctx::wait(), before the next stage's kernels are queued.
+constexpr int kMailWords = 64;
+constexpr int kMailNorm = 32;                // octree: 4 normalisation parameters (float bits)
+constexpr int kMailLevelStart = 4;           // depth-D MC: roots before each level
+constexpr int kMailQ1 = PSR_MAX_DEPTH + 8;   // depth-D MC: the level-local child 0 rule fired
+static_assert(kMailQ1 < kMailNorm, "mailbox slots overlap");
+
 struct Ctx {
     psr_config cfg{};
     hipStream_t stream = nullptr;
@@ -133,8 +142,12 @@ struct Ctx {
     int* scratch = nullptr;
     size_t scratchInts = 0;
     hipEvent_t ev[16] = {};
-    int* hostPinned = nullptr;   // small pinned buffer for size read-backs
-    DevResults* hostResults = nullptr;  // pinned mirror of results()
+    int* hostPinned = nullptr;   // read-back mailbox: pinned, coherent, kMailWords ints
+    int* mail = nullptr;         // device view of hostPinned: kernels write read-back values there
+    DevResults* hostResults = nullptr;  // pinned, coherent mirror of results()
+    DevResults* mailResults = nullptr;  // device view of hostResults: written once the results are final
+    double cgMsSum = 0.0;               // CG kernel time summed over completed runs
+    uint32_t runs = 0;                  // completed runs
     void* hostStage = nullptr;          // pinned staging for small uploads
     size_t hostStageBytes = 0;
     int cgMaxWgs = 0;            // resident workgroups of the CG kernels with several rows per thread
@@ -149,6 +162,14 @@ struct Ctx {
         return static_cast<T*>(arena.get_keep(name, sizeof(T) * (count ? count : 1), sizeof(T) * keepCount));
     }
     DevResults* results() { return buf<DevResults>("results", 1); }
+    // Waits for the stream by polling it: a blocking wait adds the host thread's
+    // wake-up latency to every read-back.  Errors surface through the final
+    // synchronisation.
+    void wait() {
+        while (hipStreamQuery(stream) == hipErrorNotReady) {
+        }
+        PSR_HIP_CHECK(hipStreamSynchronize(stream));
+    }
     // Pinned staging of at least `bytes`; only reused after a stream synchronisation.
     void* stage(size_t bytes) {
         if (bytes > hostStageBytes) {

@@ -255,8 +255,13 @@ __global__ void __launch_bounds__(kThreads) vertex_values(NodeView v, const int*
                                                           const int* __restrict__ parent,
                                                           const int* __restrict__ childBase,
                                                           const float4* __restrict__ nbrCol,
-                                                          const float* __restrict__ iso, float* __restrict__ vv) {
+                                                          const float* __restrict__ iso, float* __restrict__ vv,
+                                                          int* __restrict__ mail, const int* __restrict__ results,
+                                                          int* __restrict__ resultsHost, int resultWords) {
     int t = blockIdx.x * kThreads + threadIdx.x;
+    if (t < 3) mail[t] = count[t];  // vertex, edge and face counts for the host
+    // the run's device results (CG statistics, iso value) are final here
+    if (t < resultWords) resultsHost[t] = results[t];
     if (t >= *count) return;
     int owner = vOwner[t];
     float4 q = vPos[t];
@@ -326,9 +331,11 @@ __global__ void __launch_bounds__(kThreads) cube_triangles(
     int offD, int nD, int M, const int* __restrict__ nodeVert, const int* __restrict__ nodeEdge,
     const int* __restrict__ crossFlag, const float* __restrict__ vv, const int* __restrict__ vexAddr,
     const int* __restrict__ nodeFace, const int* __restrict__ parent, const int* __restrict__ faceHasParent,
-    ScanTicket t, int* __restrict__ cube, int* __restrict__ tri, int* __restrict__ hasInter) {
+    const int* __restrict__ q1Fired, ScanTicket t, int* __restrict__ cube, int* __restrict__ tri,
+    int* __restrict__ hasInter, int* __restrict__ mail) {
     const int tile = ticket_tile(t);
     const int n = tile * kThreads + threadIdx.x, o = offD + n;
+    if (n == 0) mail[kMailQ1] = *q1Fired;
     int c = 0;
     if (n < nD) {
         bool ok = true;
@@ -462,9 +469,11 @@ void build_geometry(Ctx& c) {
     g.vvalue = c.buf<float>("vvalue", size_t(8) * M);
     vertex_values<<<div_up(size_t(8) * M, kThreads), kThreads, 0, s>>>(v, g.counts + 0, g.vtxOwner, g.vtxKind,
                                                                        g.vtxPos, o.parent, o.childBase,
-                                                                       c.fld.nbrCol, isoDev, g.vvalue);
-    PSR_HIP_CHECK(hipMemcpyAsync(c.hostPinned, g.counts, sizeof(int) * 3, hipMemcpyDeviceToHost, s));
-    PSR_HIP_CHECK(hipStreamSynchronize(s));
+                                                                       c.fld.nbrCol, isoDev, g.vvalue, c.mail,
+                                                                       reinterpret_cast<const int*>(c.results()),
+                                                                       reinterpret_cast<int*>(c.mailResults),
+                                                                       int(sizeof(DevResults) / sizeof(int)));
+    c.wait();
     g.V = c.hostPinned[0];
     g.E = c.hostPinned[1];
     g.F = c.hostPinned[2];
@@ -478,9 +487,11 @@ void depth_mc(Ctx& c, MainMc& mm, float* rawV, int* rawT) {
     const int M = o.M, D = L.D, nD = L.count[D], offD = L.base[D];
     hipStream_t s = c.stream;
     const int E = g.E, F = g.F;
-    mm.counts = c.buf<int>("mc_counts", PSR_MAX_DEPTH + 8);
     mm.hasInter = c.buf<int>("mc_hasinter", F);
-    zero_spans({{mm.counts, size_t(PSR_MAX_DEPTH + 8)}, {mm.hasInter, size_t(F)}}, s);
+    zero_spans({{mm.hasInter, size_t(F)}}, s);
+    // crossing, triangle and root totals and the level starts go to the mailbox;
+    // launches that may be skipped leave zeros
+    for (int k = 0; k < kMailLevelStart + PSR_MAX_DEPTH + 1; ++k) c.hostPinned[k] = 0;
     int* cflag = c.buf<int>("mc_cross_flag", E);
     mm.vexAddr = c.buf<int>("mc_vex_addr", E);
     mm.cube = c.buf<int>("mc_cube", nD);
@@ -488,27 +499,24 @@ void depth_mc(Ctx& c, MainMc& mm, float* rawV, int* rawT) {
     ScanState& scan = c.scan_for_tiles(size_t(std::max(tilesE, std::max(tilesD, tilesR))));
     if (E > 0)
         edge_crossings<<<tilesE, kThreads, 0, s>>>(E, M, g.edgeOwner, g.edgeKind, g.nodeVert, g.vvalue, g.vtxPos,
-                                                   scan_ticket(scan, tilesE, mm.counts + 0), cflag, mm.vexAddr, rawV);
+                                                   scan_ticket(scan, tilesE, c.mail + 0), cflag, mm.vexAddr, rawV);
     cube_triangles<<<tilesD, kThreads, 0, s>>>(offD, nD, M, g.nodeVert, g.nodeEdge, cflag, g.vvalue, mm.vexAddr,
-                                               g.nodeFace, o.parent, g.faceHasParent,
-                                               scan_ticket(scan, tilesD, mm.counts + 1), mm.cube, rawT, mm.hasInter);
+                                               g.nodeFace, o.parent, g.faceHasParent, o.q1Fired,
+                                               scan_ticket(scan, tilesD, c.mail + 1), mm.cube, rawT, mm.hasInter,
+                                               c.mail);
     // subdivision roots: leaves above depth D with a sign change or a marked face
     mm.roots = c.buf<int>("roots", offD);
-    int* levelStart = mm.counts + 4;
     if (offD > 0)
         root_select<<<tilesR, kThreads, 0, s>>>(offD, M, L, o.neigh, g.nodeVert, g.nodeFace, o.childBase, g.vvalue,
-                                                mm.hasInter, scan_ticket(scan, tilesR, mm.counts + 2), mm.roots,
-                                                levelStart);
-    PSR_HIP_CHECK(hipMemcpyAsync(c.hostPinned, mm.counts, sizeof(int) * (PSR_MAX_DEPTH + 8), hipMemcpyDeviceToHost,
-                                 s));
-    PSR_HIP_CHECK(hipMemcpyAsync(c.hostPinned + PSR_MAX_DEPTH + 8, o.q1Fired, sizeof(int), hipMemcpyDeviceToHost, s));
-    PSR_HIP_CHECK(hipStreamSynchronize(s));
-    mm.q1Fired = c.hostPinned[PSR_MAX_DEPTH + 8] != 0;
+                                                mm.hasInter, scan_ticket(scan, tilesR, c.mail + 2), mm.roots,
+                                                c.mail + kMailLevelStart);
+    c.wait();
+    mm.q1Fired = c.hostPinned[kMailQ1] != 0;
     mm.nCross = c.hostPinned[0];
     mm.nTri = c.hostPinned[1];
     mm.nRoots = c.hostPinned[2];
     // roots of depth d: the difference of the level starts (the total past the last level)
-    auto start = [&](int d) { return L.base[d] < offD ? c.hostPinned[4 + d] : mm.nRoots; };
+    auto start = [&](int d) { return L.base[d] < offD ? c.hostPinned[kMailLevelStart + d] : mm.nRoots; };
     for (int d = 0; d < D; ++d) mm.rootsPerDepth[d] = start(d + 1) - start(d);
     mm.rootsPerDepth[D] = 0;
     PSR_HIP_CHECK(hipGetLastError());

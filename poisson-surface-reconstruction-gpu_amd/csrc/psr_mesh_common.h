@@ -99,15 +99,14 @@ struct MainMc {
     int* cube = nullptr;      // cube category per depth-D node (exported)
     int* hasInter = nullptr;  // surface-edge intersection flag per face
     int* roots = nullptr;     // subdivision roots, ascending node index
-    int* counts = nullptr;    // device: [0] crossings, [1] triangles, [2] roots, [4..] roots before each level
     int nCross = 0, nTri = 0, nRoots = 0;
     int rootsPerDepth[PSR_MAX_DEPTH + 1] = {0};
     bool q1Fired = false;  // the level-local child 0 rule changed a parent somewhere
 };
 
 // Depth-D marching cubes into the raw mesh buffers (vertices at [0, nCross),
-// triangles at [0, nTri)), then the subdivision roots.  Ends with one
-// read-back of the counts.
+// triangles at [0, nTri)), then the subdivision roots.  Ends with one wait
+// for the counts.
 void depth_mc(Ctx& c, MainMc& mm, float* rawV, int* rawT);
 
 }  // namespace psr

@@ -53,7 +53,7 @@ __global__ void __launch_bounds__(kThreads) bbox_partial(const char* __restrict_
 }
 
 __global__ void __launch_bounds__(kThreads) bbox_final(const float* __restrict__ part, int nb,
-                                                       float* __restrict__ params) {
+                                                       float* __restrict__ params, float* __restrict__ mail) {
 #pragma clang fp contract(off)
     __shared__ float s[6][kThreads];
     float mx[3] = {-1e6f, -1e6f, -1e6f}, mn[3] = {1e6f, 1e6f, 1e6f};
@@ -88,8 +88,8 @@ __global__ void __launch_bounds__(kThreads) bbox_final(const float* __restrict__
         center[a] = __fdiv_rn(__fadd_rn(s[a][0], s[3 + a][0]), 2.0f);
     }
     maxEdge = __fmul_rn(maxEdge, 1.25f);
-    for (int a = 0; a < 3; ++a) params[a] = __fsub_rn(center[a], __fdiv_rn(maxEdge, 2.0f));
-    params[3] = maxEdge;
+    for (int a = 0; a < 3; ++a) params[a] = mail[a] = __fsub_rn(center[a], __fdiv_rn(maxEdge, 2.0f));
+    params[3] = mail[3] = maxEdge;
 }
 
 // --- normalise + Morton code (BuildOctree.cu:144-208) ---
@@ -168,7 +168,8 @@ __global__ void __launch_bounds__(kThreads) rank_tile_hist(const unsigned* __res
 
 // One workgroup per level d: exclusive offsets of the tiles' new-node counts.
 __global__ void __launch_bounds__(1024) rank_tile_scan(const int* __restrict__ hist, int tiles, int D,
-                                                       int* __restrict__ tileOff, int* __restrict__ U) {
+                                                       int* __restrict__ tileOff, int* __restrict__ U,
+                                                       int* __restrict__ mail) {
     int d = blockIdx.x + 1;
     __shared__ int run;
     if (threadIdx.x == 0) run = 0;
@@ -193,7 +194,7 @@ __global__ void __launch_bounds__(1024) rank_tile_scan(const int* __restrict__ h
         if (threadIdx.x == 1023) run += sbuf[1023];
         __syncthreads();
     }
-    if (threadIdx.x == 0) U[d] = run;
+    if (threadIdx.x == 0) U[d] = mail[d] = run;
 }
 
 struct LevelOffsets {
@@ -438,7 +439,7 @@ void build_octree(Ctx& c, const psr_points& pts) {
     float* part = c.buf<float>("bbox_part", size_t(nbb) * 6);
     o.normParams = c.buf<float>("norm_params", 4);
     bbox_partial<<<nbb, kThreads, 0, s>>>(reinterpret_cast<const char*>(pts.xyz), xs, N, part);
-    bbox_final<<<1, kThreads, 0, s>>>(part, nbb, o.normParams);
+    bbox_final<<<1, kThreads, 0, s>>>(part, nbb, o.normParams, reinterpret_cast<float*>(c.mail + kMailNorm));
     // normalise + encode
     float4* posU = c.buf<float4>("pos_unsorted", N);
     float4* nrmU = c.buf<float4>("nrm_unsorted", N);
@@ -462,12 +463,10 @@ void build_octree(Ctx& c, const psr_points& pts) {
     int* tileOff = c.buf<int>("rank_tileoff", size_t(PSR_MAX_DEPTH + 2) * tiles);
     int* Ud = c.buf<int>("level_U", PSR_MAX_DEPTH + 2);
     rank_tile_hist<<<tiles, kThreads, 0, s>>>(keys, N, D, tiles, hist);
-    rank_tile_scan<<<D, 1024, 0, s>>>(hist, tiles, D, tileOff, Ud);
-    PSR_HIP_CHECK(hipMemcpyAsync(c.hostPinned, Ud, sizeof(int) * (D + 1), hipMemcpyDeviceToHost, s));
-    PSR_HIP_CHECK(hipMemcpyAsync(c.hostPinned + 32, o.normParams, sizeof(float) * 4, hipMemcpyDeviceToHost, s));
-    PSR_HIP_CHECK(hipStreamSynchronize(s));
+    rank_tile_scan<<<D, 1024, 0, s>>>(hist, tiles, D, tileOff, Ud, c.mail);
+    c.wait();
     float edge;
-    std::memcpy(&edge, c.hostPinned + 35, sizeof(float));
+    std::memcpy(&edge, c.hostPinned + kMailNorm + 3, sizeof(float));
     if (!(edge > 0.f) || !std::isfinite(edge))  // the reference divides by it (BuildOctree.cu:149)
         throw std::invalid_argument("degenerate input: zero or non-finite bounding box");
     o.U[0] = 1;

@@ -779,11 +779,13 @@ struct SmallDiv {
 // with one 16-byte load per axis.  Terms, their order and their association
 // are those of sub_corner_values.
 constexpr int kLdsWordsGrid = 7168;  // upper bound of grid_tile_words for depths up to 9
+constexpr int kLdsWordsSmallGrid = 12288;  // upper bound of small_tile_words for depths up to 10
 constexpr int kHaloPerThread = 4;     // halo entries per thread (<= 1024 for depths up to 9)
 __global__ void __launch_bounds__(kThreads) sub_corner_values_grid(Grid g, const Tile* __restrict__ tiles, int nVD,
                                                                    const float* __restrict__ iso,
                                                                    float* __restrict__ cval) {
     extern __shared__ float4 ldsDyn[];  // grid_tile_words(D, rd) words
+    __shared__ unsigned short lItem[kThreads * 8];  // owned (cell << 3 | corner), cell order
     __shared__ int lNbr[27];
     __shared__ int lChain[kChain];
     __shared__ TileRoot lRoot[1];
@@ -889,20 +891,33 @@ __global__ void __launch_bounds__(kThreads) sub_corner_values_grid(Grid g, const
         const float c = (0.5f + float(cell)) / invw;
         lFac[t] = bspline((float(corner) * inv - c) * invw);
     }
-    __syncthreads();
-    if (int(threadIdx.x) >= T.count) return;
-    const int e = T.start + threadIdx.x;
-    const Leaf L = leaf_of(g, T.set, S, R, e);
-    const RecVE ve = rec_ve(lNbr, lSet, L, D);
-    const float isoV = *iso;
+    // the corners each cell owns; a cell on a face of the root's cube may own
+    // several, so the (cell, corner) items are compacted and shared out over
+    // the block rather than evaluated by the owning lane
+    unsigned mask = 0;
     const int sh = D - S.rd, side = 1 << sh;
-    const int3 lc = make_int3(L.c.x - (R.Rc.x << sh), L.c.y - (R.Rc.y << sh), L.c.z - (R.Rc.z << sh));
-    for (int b = 0; b < 8; ++b) {
+    if (int(threadIdx.x) < T.count) {
+        const Leaf L = leaf_of(g, T.set, S, R, T.start + threadIdx.x);
+        const RecVE ve = rec_ve(lNbr, lSet, L, D);
+        const int3 lc = make_int3(L.c.x - (R.Rc.x << sh), L.c.y - (R.Rc.y << sh), L.c.z - (R.Rc.z << sh));
+        for (int b = 0; b < 8; ++b) {
+            const int3 o = bourke_offset(b);
+            // a corner all of whose cells lie in the root's cube is owned by its lowest cell
+            const bool inner = lc.x + o.x > 0 && lc.x + o.x < side && lc.y + o.y > 0 && lc.y + o.y < side &&
+                               lc.z + o.z > 0 && lc.z + o.z < side;
+            if (inner ? (o.x + o.y + o.z == 3) : owns_corner(L, b, ve)) mask |= 1u << b;
+        }
+    }
+    int first;
+    const int nItems = block_excl_scan<kThreads>(__popc(mask), first);
+    for (int b = 0; b < 8; ++b)
+        if ((mask >> b) & 1u) lItem[first++] = (threadIdx.x << 3) | b;
+    __syncthreads();
+    const float isoV = *iso;
+    for (int it = threadIdx.x; it < nItems; it += kThreads) {
+        const int b = lItem[it] & 7, e = T.start + (lItem[it] >> 3);
+        const Leaf L = leaf_of(g, T.set, S, R, e);
         const int3 o = bourke_offset(b);
-        // a corner all of whose cells lie in the root's cube is owned by its lowest cell
-        const bool inner = lc.x + o.x > 0 && lc.x + o.x < side && lc.y + o.y > 0 && lc.y + o.y < side &&
-                           lc.z + o.z > 0 && lc.z + o.z < side;
-        if (inner ? (o.x + o.y + o.z != 3) : !owns_corner(L, b, ve)) continue;
         const int3 pt = make_int3(L.c.x + o.x, L.c.y + o.y, L.c.z + o.z);
         const int px = pt.x - lo.x, py = pt.y - lo.y, pz = pt.z - lo.z;
         float val = 0.f;
@@ -943,6 +958,172 @@ __global__ void __launch_bounds__(kThreads) sub_corner_values_grid(Grid g, const
         }
         cval[size_t(b) * nVD + e] = val - isoV;
     }
+}
+
+// LDS words of a small-root tile whose roots (nRoots of them) are at depth rd
+// with D - rd <= 2: per root the list tables [3][n][Kp] (n = 2^(D-rd) + 1
+// corner coordinates per axis), the halo d_x boxes and their factor tables,
+// rounded to 4 words so that every root's tables are 16-byte aligned.
+__host__ __device__ inline int small_tile_words(int D, int rd, int nRoots) {
+    const int sh = D - rd, n = (1 << sh) + 1, Kp = (27 * (rd + 1) + 3) & ~3;
+    int halo = 0, fac = 0;
+    for (int l = D; l > rd; --l) {
+        const int b = max(1, (1 << sh) >> (D - l)) + 2;
+        halo += b * b * b;
+        fac += 3 * b * n;
+    }
+    return nRoots * ((3 * n * Kp + halo + fac + 3) & ~3);
+}
+
+// Small roots (8 or 64 cells) when the octree is geometric: the tables of
+// sub_corner_values_grid for each root of the tile (the root list at the
+// root's corner coordinates, with d_x folded into x; the halo d_x from the
+// dense per-depth grids and the halo factors), one thread per corner point of
+// a root, evaluated for the root holding the point's owner.  Terms, their
+// order and their association are those of sub_corner_values.
+__global__ void __launch_bounds__(kSmallThreads) sub_corner_values_small_grid(Grid g, const Tile* __restrict__ tiles,
+                                                                              int nVD, const float* __restrict__ iso,
+                                                                              float* __restrict__ cval) {
+    extern __shared__ float4 ldsDyn[];  // small_tile_words(D, rd, roots of the tile) words
+    constexpr int kR = 4;               // roots per tile at most (32 cells of 8)
+    __shared__ TileRoot lRoot[kR];
+    __shared__ int lNbr[kR * 27];
+    __shared__ int lChain[kR * kChain];
+    __shared__ unsigned lSet[kR];
+    float* lds = reinterpret_cast<float*>(ldsDyn);
+    const Tile T = tiles[blockIdx.x];
+    const SetDesc S = g.sets[T.set];
+    const int D = g.D, sh = D - S.rd, n = (1 << sh) + 1, P = n * n * n;
+    const TileGeom tg = tile_geom(T, S, D);
+    const int nR = tg.nRoots, K = tg.K, Kp = (K + 3) & ~3;
+    for (int t = threadIdx.x; t < nR * 27; t += kSmallThreads) lNbr[t] = g.rootNbr[27 * size_t(T.root) + t];
+    for (int t = threadIdx.x; t < nR * kChain; t += kSmallThreads)
+        lChain[t] = g.rootChain[kChain * size_t(T.root) + t];
+    if (int(threadIdx.x) < nR) lSet[threadIdx.x] = g.rootSet[T.root + threadIdx.x];
+    fill_roots_rec(g, T, S, tg, lRoot);
+    __syncthreads();
+    // per root: list x (with d_x) [n][Kp] | list y | list z | halo d_x | halo factors
+    const int listWords = 3 * n * Kp, perRoot = small_tile_words(D, S.rd, 1);
+    const float inv = 1.0f / float(1 << D);
+    for (int t = threadIdx.x; t < nR * Kp; t += kSmallThreads) {
+        const int r = t / Kp, k = t - r * Kp;
+        const int node = k < K ? lChain[r * kChain + k / 27] : -1;
+        float4 b = make_float4(0.f, 0.f, 0.f, 0.f);
+        float d = 0.f;
+        if (node >= 0) {
+            const float4 col = g.nbrCol[size_t(k % 27) * g.M + node];
+            b = make_float4(col.x, col.y, col.z, float(1 << (S.rd - k / 27)));
+            d = col.w;
+        }
+        const int3 lo = lRoot[r].first;
+        float* L = lds + r * perRoot;
+        for (int cc = 0; cc < n; ++cc) {
+            L[cc * Kp + k] = node < 0 ? 0.f : d * bspline((float(lo.x + cc) * inv - b.x) * b.w);
+            L[(n + cc) * Kp + k] = node < 0 ? 0.f : bspline((float(lo.y + cc) * inv - b.y) * b.w);
+            L[(2 * n + cc) * Kp + k] = node < 0 ? 0.f : bspline((float(lo.z + cc) * inv - b.z) * b.w);
+        }
+    }
+    // halo d_x, depth D first: 0 inside the roots of the set and where no node is
+    for (int t = threadIdx.x; t < nR * tg.haloPerRoot; t += kSmallThreads) {
+        const int r = t / tg.haloPerRoot;
+        int u = t - r * tg.haloPerRoot, l = D;
+        int3 x = box_ext(tg.ext, D, l);
+        while (u >= (x.x + 2) * (x.y + 2) * (x.z + 2)) {
+            u -= (x.x + 2) * (x.y + 2) * (x.z + 2);
+            --l;
+            x = box_ext(tg.ext, D, l);
+        }
+        const int bz = x.z + 2, by = x.y + 2;
+        const int dx = u / (by * bz), rem = u - dx * by * bz, dy = rem / bz, dz = rem - dy * bz;
+        const TileRoot R = lRoot[r];
+        const int3 c = make_int3((R.first.x >> (D - l)) - 1 + dx, (R.first.y >> (D - l)) - 1 + dy,
+                                 (R.first.z >> (D - l)) - 1 + dz);
+        const int lim = 1 << l, s = l - S.rd;
+        float v = 0.f;
+        if (c.x >= 0 && c.y >= 0 && c.z >= 0 && c.x < lim && c.y < lim && c.z < lim) {
+            const int3 A = make_int3(c.x >> s, c.y >> s, c.z >> s);
+            if (A.x != R.Rc.x || A.y != R.Rc.y || A.z != R.Rc.z) {
+                const int rp = lNbr[r * 27 + nbr_index(A.x - R.Rc.x, A.y - R.Rc.y, A.z - R.Rc.z)];
+                if (!(rp >= S.rootOffset && rp < S.rootOffset + S.nr)) {
+                    const size_t cell = (size_t(c.x) << (2 * l)) | (size_t(c.y) << l) | size_t(c.z);
+                    v = g.dxAt[((size_t(1) << (3 * l)) - 1) / 7 + cell];
+                }
+            }
+        }
+        lds[r * perRoot + listWords + (t - r * tg.haloPerRoot)] = v;
+    }
+    // halo factors: per depth, per axis, node cell (box-relative) x corner coordinate
+    int facPerRoot = 0;
+    for (int l = D; l > S.rd; --l) facPerRoot += 3 * (box_ext(tg.ext, D, l).x + 2) * n;
+    for (int t = threadIdx.x; t < nR * facPerRoot; t += kSmallThreads) {
+        const int r = t / facPerRoot;
+        int u = t - r * facPerRoot, l = D;
+        int3 x = box_ext(tg.ext, D, l);
+        while (u >= 3 * (x.x + 2) * n) {
+            u -= 3 * (x.x + 2) * n;
+            --l;
+            x = box_ext(tg.ext, D, l);
+        }
+        const int b = x.x + 2, axis = u / (b * n), w = u - axis * b * n, ci = w / n, cc = w - ci * n;
+        const int3 f = lRoot[r].first;
+        const int fa = axis == 0 ? f.x : (axis == 1 ? f.y : f.z);
+        const float invw = float(1 << l);
+        const int cell = (fa >> (D - l)) - 1 + ci;
+        const float c = (0.5f + float(cell)) / invw;
+        lds[r * perRoot + listWords + tg.haloPerRoot + (t - r * facPerRoot)] =
+            bspline((float(fa + cc) * inv - c) * invw);
+    }
+    __syncthreads();
+    const int rr = threadIdx.x / P, pi = threadIdx.x - rr * P;
+    if (rr >= nR) return;
+    const TileRoot R = lRoot[rr];
+    const int3 pt = make_int3(R.first.x + pi / (n * n), R.first.y + (pi / n) % n, R.first.z + pi % n);
+    const Leaf Lr = root_leaf(g, S, T.set, R);
+    const RecVE ve = rec_ve(lNbr + 27 * rr, lSet[rr], Lr, D);
+    int m;
+    const int best = point_owner(pt, m, ve);
+    if (best < 0 || ((best - S.dEnum) >> (3 * sh)) != tg.q0 + rr) return;
+    const int slot = bourke_slot(m & 1, (m >> 1) & 1, (m >> 2) & 1);
+    const int3 oc = make_int3(pt.x - (m & 1), pt.y - ((m >> 1) & 1), pt.z - ((m >> 2) & 1));
+    const int px = pt.x - R.first.x, py = pt.y - R.first.y, pz = pt.z - R.first.z;
+    const float* base = lds + rr * perRoot;
+    const float* hd = base + listWords;
+    const float* fac = hd + tg.haloPerRoot;
+    float val = 0.f;
+    for (int l = D; l > S.rd; --l) {
+        const int s = l - S.rd;
+        const int3 x = box_ext(tg.ext, D, l);
+        const int b = x.x + 2;
+        const int3 cl = make_int3(oc.x >> (D - l), oc.y >> (D - l), oc.z >> (D - l));
+        const bool interior = ((cl.x - 1) >> s) == R.Rc.x && ((cl.x + 1) >> s) == R.Rc.x &&
+                              ((cl.y - 1) >> s) == R.Rc.y && ((cl.y + 1) >> s) == R.Rc.y &&
+                              ((cl.z - 1) >> s) == R.Rc.z && ((cl.z + 1) >> s) == R.Rc.z;
+        if (!interior) {
+            const int3 bl = make_int3((R.first.x >> (D - l)) - 1, (R.first.y >> (D - l)) - 1,
+                                      (R.first.z >> (D - l)) - 1);
+            const float* fx = fac;
+            const float* fy = fx + b * n;
+            const float* fz = fy + b * n;
+#pragma unroll
+            for (int i = 0; i < 27; ++i) {
+                const int nx = cl.x + i / 9 - 1 - bl.x, ny = cl.y + (i / 3) % 3 - 1 - bl.y, nz = cl.z + i % 3 - 1 - bl.z;
+                val += hd[(nx * b + ny) * b + nz] * fx[nx * n + px] * fy[ny * n + py] * fz[nz * n + pz];
+            }
+        }
+        hd += b * b * b;
+        fac += 3 * b * n;
+    }
+    const float4* X4 = reinterpret_cast<const float4*>(base + px * Kp);
+    const float4* Y4 = reinterpret_cast<const float4*>(base + (n + py) * Kp);
+    const float4* Z4 = reinterpret_cast<const float4*>(base + (2 * n + pz) * Kp);
+    for (int k4 = 0; k4 < Kp / 4; ++k4) {
+        const float4 a = X4[k4], yb = Y4[k4], zc = Z4[k4];
+        val += a.x * yb.x * zc.x;
+        val += a.y * yb.y * zc.y;
+        val += a.z * yb.z * zc.z;
+        val += a.w * yb.w * zc.w;
+    }
+    cval[size_t(slot) * nVD + best] = val - *iso;
 }
 
 // Exclusive prefix of v over the block's threads and the block total (every
@@ -1265,14 +1446,15 @@ void extract_mesh(Ctx& c) {
     MainMc mm;
     depth_mc(c, mm, mainV, outT);
     c.stats.n_subdivision_roots = mm.nRoots;
-    int* acc = c.buf<int>("mesh_counts", 4);  // [0] vertices kept by the depth-D batch
-    fill_int<<<1, kThreads, 0, st>>>(acc, 4, 0);
+    // vertices kept by the depth-D batch: mailbox slot kMailKept (zero if none)
+    constexpr int kMailKept = 3;
+    c.hostPinned[kMailKept] = 0;
     size_t triTotal = mm.nTri, keptTotal = 0;
     float* outV = c.buf_keep<float>("mesh_vertices", size_t(3) * std::max(mm.nCross, 1), 0);
     if (mm.nCross > 0) {
         const int tiles = div_up(mm.nCross, kThreads);
-        keep_compact<<<tiles, kThreads, 0, st>>>(mainV, mm.nCross,
-                                                 scan_ticket(c.scan_for_tiles(size_t(tiles)), tiles, acc), outV);
+        keep_compact<<<tiles, kThreads, 0, st>>>(
+            mainV, mm.nCross, scan_ticket(c.scan_for_tiles(size_t(tiles)), tiles, c.mail + kMailKept), outV);
     }
     // ---- subdivision sets: coarse roots one per set, finer roots one set per depth
     // (ComputeTriangleIndices.h:371: roots above depth D-4 are coarse)
@@ -1318,12 +1500,14 @@ void extract_mesh(Ctx& c) {
         root_records<<<div_up(size_t(27) * mm.nRoots, kThreads), kThreads, 0, st>>>(
             mm.nRoots, M, L, finerDepth, mm.roots, rootPos, o.neigh, o.parent, o.key, rootNbr, rootChain, rootCell,
             rootSet);
-        // one-root tiles read d_x from dense per-depth grids when the octree is geometric
-        const bool denseHalo = !mm.q1Fired && D - sets[0].rd >= 3;
+        // tiles read d_x from dense per-depth grids when the octree is geometric
+        const bool denseHalo = !mm.q1Fired;
         for (const SetDesc& S : sets) {
             int halo = 0;
             if (denseHalo && D - S.rd >= 3 &&
                 (grid_tile_words(D, S.rd, &halo) > kLdsWordsGrid || halo > kHaloPerThread * kThreads))
+                throw std::runtime_error("subdivision tile exceeds its LDS budget");
+            if (denseHalo && D - S.rd < 3 && small_tile_words(D, S.rd, D - S.rd == 1 ? 4 : 1) > kLdsWordsSmallGrid)
                 throw std::runtime_error("subdivision tile exceeds its LDS budget");
         }
         float* dxAt = nullptr;
@@ -1397,7 +1581,6 @@ void extract_mesh(Ctx& c) {
             int* crossPre = c.buf<int>("sub_cross_prefix", nVD);
             int4* tileCnt = c.buf<int4>("sub_tile_cnt", nTiles);
             int4* tileBase = c.buf<int4>("sub_tile_base", nTiles);
-            int* cnts = c.buf<int>("sub_counts", 4);
             if (nOne > 0 && denseHalo) {
                 int words = 0;
                 for (int k = 0; k < nCs; ++k)
@@ -1406,15 +1589,21 @@ void extract_mesh(Ctx& c) {
             }
             else if (nOne > 0)
                 sub_corner_values<true, kThreads><<<nOne, kThreads, 0, st>>>(gr, dtiles, nVD, isoDev, cval);
-            if (nSmall > 0)
+            if (nSmall > 0 && denseHalo) {
+                int words = 0;
+                for (int k = 0; k < nCs; ++k)
+                    if (D - cs[k].rd < 3)
+                        words = std::max(words, small_tile_words(D, cs[k].rd, D - cs[k].rd == 1 ? 4 : 1));
+                sub_corner_values_small_grid<<<nSmall, kSmallThreads, sizeof(float) * words, st>>>(gr, dsmall, nVD,
+                                                                                                   isoDev, cval);
+            } else if (nSmall > 0) {
                 sub_corner_values<false, kSmallThreads><<<nSmall, kSmallThreads, 0, st>>>(gr, dsmall, nVD, isoDev, cval);
+            }
             sub_classify<<<nTiles, kThreads, 0, st>>>(gr, dtiles, nVD, cval, vals, cellWord, crossPre, tileCnt);
-            sub_tile_scan<<<1, kScanThreads, 0, st>>>(tileCnt, nTiles, tileBase, cnts);
-            PSR_HIP_CHECK(hipMemcpyAsync(c.hostPinned, cnts, sizeof(int) * 3, hipMemcpyDeviceToHost, st));
-            if (s0 == 0) PSR_HIP_CHECK(hipMemcpyAsync(c.hostPinned + 3, acc, sizeof(int), hipMemcpyDeviceToHost, st));
-            PSR_HIP_CHECK(hipStreamSynchronize(st));
+            sub_tile_scan<<<1, kScanThreads, 0, st>>>(tileCnt, nTiles, tileBase, c.mail);  // totals: slots 0-2
+            c.wait();
             const int nKept = c.hostPinned[1], nTri = c.hostPinned[2];
-            if (s0 == 0) keptTotal = size_t(c.hostPinned[3]);
+            if (s0 == 0) keptTotal = size_t(c.hostPinned[kMailKept]);
             if (keptTotal + nKept > size_t(INT32_MAX) || triTotal + nTri > size_t(INT32_MAX) / 3)
                 throw std::runtime_error("mesh exceeds 32-bit indexing");
             outV = c.buf_keep<float>("mesh_vertices", 3 * (keptTotal + nKept), 3 * keptTotal);
@@ -1428,9 +1617,8 @@ void extract_mesh(Ctx& c) {
         }
     }
     if (nSets == 0) {  // only the depth-D batch: read its kept count
-        PSR_HIP_CHECK(hipMemcpyAsync(c.hostPinned, acc, sizeof(int), hipMemcpyDeviceToHost, st));
-        PSR_HIP_CHECK(hipStreamSynchronize(st));
-        keptTotal = size_t(c.hostPinned[0]);
+        c.wait();
+        keptTotal = size_t(c.hostPinned[kMailKept]);
     }
     if (triTotal > size_t(INT32_MAX) / 3) throw std::runtime_error("mesh exceeds 32-bit triangle indexing");
     c.mesh.vertices = outV;
