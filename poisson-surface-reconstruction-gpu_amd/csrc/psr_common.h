@@ -135,4 +135,30 @@ __device__ __forceinline__ float chi_node_sum(int node, float invw, float3 p, in
     return v;
 }
 
+// chi_node_sum when the node's neighbours lie on the regular grid around it
+// (an octree in which the level-local child 0 rule changed no parent).  Node
+// basis b = {centre x, y, z, inverse width}; neighbour i's centre is the
+// node's plus (i / 9 - 1, (i / 3) % 3 - 1, i % 3 - 1) widths, exactly (the
+// centres are dyadic), so the 27 terms take 3 B-spline values per axis.  Each
+// term is formed as col_term forms it, so the sum equals chi_node_sum's.
+__device__ __forceinline__ float chi_node_sum_regular(int node, float4 b, float3 p, int M,
+                                                      const float4* __restrict__ nbrCol) {
+    const float* dcol = reinterpret_cast<const float*>(nbrCol) + 3;  // d_x of each column entry
+    float d[27];
+#pragma unroll
+    for (int i = 0; i < 27; ++i) d[i] = dcol[4 * (size_t(i) * M + node)];
+    const float w = 1.0f / b.w;  // exact: b.w is a power of two
+    float fx[3], fy[3], fz[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        fx[j] = bspline((p.x - (b.x + float(j - 1) * w)) * b.w);
+        fy[j] = bspline((p.y - (b.y + float(j - 1) * w)) * b.w);
+        fz[j] = bspline((p.z - (b.z + float(j - 1) * w)) * b.w);
+    }
+    float v = 0.f;
+#pragma unroll
+    for (int i = 0; i < 27; ++i) v += d[i] * fx[i / 9] * fy[(i / 3) % 3] * fz[i % 3];
+    return v;
+}
+
 }  // namespace psr

@@ -255,6 +255,7 @@ __global__ void __launch_bounds__(kThreads) vertex_values(NodeView v, const int*
                                                           const int* __restrict__ parent,
                                                           const int* __restrict__ childBase,
                                                           const float4* __restrict__ nbrCol,
+                                                          const int* __restrict__ q1Fired,
                                                           const float* __restrict__ iso, float* __restrict__ vv,
                                                           int* __restrict__ mail, const int* __restrict__ results,
                                                           int* __restrict__ resultsHost, int resultWords) {
@@ -268,10 +269,12 @@ __global__ void __launch_bounds__(kThreads) vertex_values(NodeView v, const int*
     float3 p = make_float3(q.x, q.y, q.z);
     int depth = __float_as_int(q.w);
     float val = 0.f;
+    const bool regular = *q1Fired == 0;
     if (owner > 0) {
         int now = owner;
         for (int d = depth; now != -1; --d) {  // a parent is one level up
-            val += chi_node_sum(now, float(1 << d), p, v.M, nbrCol);
+            val += regular ? chi_node_sum_regular(now, v.basis[now], p, v.M, nbrCol)
+                           : chi_node_sum(now, float(1 << d), p, v.M, nbrCol);
             now = parent[now];
         }
         // descent into the key-ordered child picked by the Bourke corner id
@@ -282,7 +285,8 @@ __global__ void __launch_bounds__(kThreads) vertex_values(NodeView v, const int*
             int cb = childBase[now];
             if (cb < 0) break;
             now = cb + ex;
-            val += chi_node_sum(now, float(1 << depth), p, v.M, nbrCol);
+            val += regular ? chi_node_sum_regular(now, v.basis[now], p, v.M, nbrCol)
+                           : chi_node_sum(now, float(1 << depth), p, v.M, nbrCol);
         }
     }
     vv[t] = val - *iso;
@@ -469,7 +473,7 @@ void build_geometry(Ctx& c) {
     g.vvalue = c.buf<float>("vvalue", size_t(8) * M);
     vertex_values<<<div_up(size_t(8) * M, kThreads), kThreads, 0, s>>>(v, g.counts + 0, g.vtxOwner, g.vtxKind,
                                                                        g.vtxPos, o.parent, o.childBase,
-                                                                       c.fld.nbrCol, isoDev, g.vvalue, c.mail,
+                                                                       c.fld.nbrCol, o.q1Fired, isoDev, g.vvalue, c.mail,
                                                                        reinterpret_cast<const int*>(c.results()),
                                                                        reinterpret_cast<int*>(c.mailResults),
                                                                        int(sizeof(DevResults) / sizeof(int)));

@@ -892,17 +892,21 @@ __global__ void __launch_bounds__(kThreads) points_chi_kernel(int N, int offD, i
                                                               const int* __restrict__ p2n,
                                                               const int* __restrict__ parent,
                                                               const float4* __restrict__ nbrCol,
+                                                              const float4* __restrict__ basis,
+                                                              const int* __restrict__ q1Fired,
                                                               float* __restrict__ chi, double* __restrict__ part) {
     __shared__ double sh[kThreads / PSR_WAVE];
     int i = blockIdx.x * kThreads + threadIdx.x;
     double mine = 0;
+    const bool regular = *q1Fired == 0;
     if (i < N) {
         float4 q = pos[i];
         float3 p = make_float3(q.x, q.y, q.z);
         int now = offD + p2n[i];
         float v = 0.f;
         for (int d = D; now != -1; --d) {  // a parent is one level up
-            v += chi_node_sum(now, float(1 << d), p, M, nbrCol);
+            v += regular ? chi_node_sum_regular(now, basis[now], p, M, nbrCol)
+                         : chi_node_sum(now, float(1 << d), p, M, nbrCol);
             now = parent[now];
         }
         chi[i] = v;
@@ -1087,7 +1091,7 @@ void points_implicit(Ctx& c) {
     neighbour_columns<<<div_up(size_t(27) * o.M, kThreads), kThreads, 0, s>>>(o.M, o.neigh, o.basis, c.fld.dx,
                                                                              c.fld.nbrCol);
     points_chi_kernel<<<nb, kThreads, 0, s>>>(N, L.base[L.D], o.M, L.D, o.spos, o.point2node, o.parent,
-                                              c.fld.nbrCol, c.fld.chiPts, part);
+                                              c.fld.nbrCol, o.basis, o.q1Fired, c.fld.chiPts, part);
     iso_final<<<1, 1024, 0, s>>>(part, nb, N, c.fld.isoSum, isoDev);
     PSR_HIP_CHECK(hipGetLastError());
 }
