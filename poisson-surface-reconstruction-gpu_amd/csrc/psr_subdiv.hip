@@ -217,9 +217,17 @@ __device__ __forceinline__ int point_owner(int3 p, int& dmask, const VE& ve) {
 // Owner (enumeration, Bourke slot within the owner) of the grid point c + off(b);
 // ve(n) is the enumeration of depth-D cell n if it is virtual in L's set, else -1.
 template <class VE>
+__device__ __forceinline__ int owner_of_point(const Leaf& L, int3 p, int& slot, VE ve);
+template <class VE>
 __device__ __forceinline__ int corner_owner(const Leaf& L, int D, int b, int& slot, VE ve) {
-    int3 o = bourke_offset(b);
-    int3 p = make_int3(L.c.x + o.x, L.c.y + o.y, L.c.z + o.z);
+    const int3 o = bourke_offset(b);
+    return owner_of_point(L, make_int3(L.c.x + o.x, L.c.y + o.y, L.c.z + o.z), slot, ve);
+}
+
+// Owner of grid point p, a corner of a cell of L's root (only the root, its
+// cell and L's set are read from L).
+template <class VE>
+__device__ __forceinline__ int owner_of_point(const Leaf& L, int3 p, int& slot, VE ve) {
     if (inside_root(L, p.x - 1, p.x, L.Rc.x) && inside_root(L, p.y - 1, p.y, L.Rc.y) &&
         inside_root(L, p.z - 1, p.z, L.Rc.z)) {
         // all 8 sharing cells are in the root's cube; the smallest key is the lowest cell
@@ -358,6 +366,7 @@ __device__ __forceinline__ int halo_node(const Grid& g, const SetDesc& S, int ro
 // The terms and their order are those of the chain walk.
 constexpr int kTx = 5, kTy = 9, kTz = 9, kTw = kTx + kTy + kTz;
 constexpr int kMaxRootsPerTile = 32;
+constexpr int kClassifyPoints = 864;  // corner points of a tile: 32 roots of 27, 4 of 125 or one box of 5 x 9 x 9
 constexpr int kLdsWordsOne = 7168;     // one-root tiles: list, halo, vbox, d_x, factors (<= 6954 words)
 constexpr int kLdsWordsSmall = 5376;   // small-root tiles of 64 cells: halo, vbox, staged blocks (<= 5344 words)
 constexpr int kStagedBlocks = 32;  // contributor blocks of 27 staged in LDS per small-root tile
@@ -1179,6 +1188,7 @@ __global__ void __launch_bounds__(kThreads) sub_classify(Grid g, const Tile* __r
     __shared__ int lNbr[kMaxRootsPerTile * 27];
     __shared__ unsigned lSet[kMaxRootsPerTile];
     __shared__ int lSum[kThreads / PSR_WAVE];
+    __shared__ float lVal[kClassifyPoints];
     const Tile T = tiles[blockIdx.x];
     const SetDesc S = g.sets[T.set];
     const int D = g.D;
@@ -1186,6 +1196,20 @@ __global__ void __launch_bounds__(kThreads) sub_classify(Grid g, const Tile* __r
     fill_roots_rec(g, T, S, tg, lRoots);
     for (int t = threadIdx.x; t < tg.nRoots * 27; t += kThreads) lNbr[t] = g.rootNbr[27 * size_t(T.root) + t];
     if (int(threadIdx.x) < tg.nRoots) lSet[threadIdx.x] = g.rootSet[T.root + threadIdx.x];
+    __syncthreads();
+    // the values at the corner points of each root's part of the tile, read
+    // once per point (its owner is the same whichever cell asks)
+    const int ex = tg.ext.x + 1, ey = tg.ext.y + 1, ez = tg.ext.z + 1, PP = ex * ey * ez;
+    for (int t = threadIdx.x; t < tg.nRoots * PP; t += kThreads) {
+        const int rr = t / PP, u = t - rr * PP;
+        const int ux = u / (ey * ez), uy = (u / ez) % ey, uz = u % ez;
+        const TileRoot R = lRoots[rr];
+        const Leaf Lr = root_leaf(g, S, T.set, R);
+        int slot;
+        const int owner = owner_of_point(Lr, make_int3(R.first.x + ux, R.first.y + uy, R.first.z + uz), slot,
+                                         rec_ve(lNbr + 27 * rr, lSet[rr], Lr, D));
+        lVal[t] = owner >= 0 ? cval[size_t(slot) * nVD + owner] : 0.f;
+    }
     __syncthreads();
     const bool active = int(threadIdx.x) < T.count;
     const int e = T.start + (active ? int(threadIdx.x) : 0);
@@ -1195,13 +1219,14 @@ __global__ void __launch_bounds__(kThreads) sub_classify(Grid g, const Tile* __r
         const int rr = tg.oneRoot ? 0 : int(threadIdx.x) >> (3 * (D - S.rd));
         const Leaf L = leaf_of(g, T.set, S, lRoots[rr], e);
         const RecVE ve = rec_ve(lNbr + 27 * rr, lSet[rr], L, D);
+        const int3 f = lRoots[rr].first;
+        const float* pv = lVal + rr * PP;
         float v[8];
         int cls = 0;
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
-            int slot;
-            const int owner = corner_owner(L, D, b, slot, ve);
-            v[b] = cval[size_t(slot) * nVD + owner];
+            const int3 o = bourke_offset(b);
+            v[b] = pv[((L.c.x - f.x + o.x) * ey + (L.c.y - f.y + o.y)) * ez + (L.c.z - f.z + o.z)];
             if (v[b] < 0.f) cls |= 1 << b;
         }
         const float inv = 1.0f / float(1 << D);
@@ -1272,9 +1297,18 @@ __global__ void __launch_bounds__(kThreads) sub_emit(Grid g, const Tile* __restr
                                                      const int4* __restrict__ tileBase, int keptBefore,
                                                      int triBefore, float* __restrict__ outV, int* __restrict__ outT) {
     __shared__ int lSum[kThreads / PSR_WAVE];
+    __shared__ TileRoot lRoots[kMaxRootsPerTile];
+    __shared__ int lNbr[kMaxRootsPerTile * 27];
+    __shared__ unsigned lSet[kMaxRootsPerTile];
     const int4 cnt = tileCnt[blockIdx.x];
     if (cnt.y == 0 && cnt.z == 0) return;  // the same for the whole block
     const Tile T = tiles[blockIdx.x];
+    const SetDesc S = g.sets[T.set];
+    const TileGeom tg = tile_geom(T, S, g.D);
+    // the tile's roots (made visible by the block scans' barriers)
+    fill_roots_rec(g, T, S, tg, lRoots);
+    for (int t = threadIdx.x; t < tg.nRoots * 27; t += kThreads) lNbr[t] = g.rootNbr[27 * size_t(T.root) + t];
+    if (int(threadIdx.x) < tg.nRoots) lSet[threadIdx.x] = g.rootSet[T.root + threadIdx.x];
     const bool active = int(threadIdx.x) < T.count;
     const int e = T.start + (active ? int(threadIdx.x) : 0);
     const unsigned w = active ? cellWord[e] : 0u;
@@ -1284,8 +1318,9 @@ __global__ void __launch_bounds__(kThreads) sub_emit(Grid g, const Tile* __restr
     const int kp = block_exclusive<kThreads>(__popc(keep), lSum, tk);
     const int tp = block_exclusive<kThreads>(nt, lSum, tt);
     if (!active || (keep == 0 && nt == 0)) return;
-    const SetDesc S = g.sets[T.set];
-    const Leaf L = make_leaf_in(g, T.set, S, e);
+    const int rr = tg.oneRoot ? 0 : int(threadIdx.x) >> (3 * (g.D - S.rd));
+    const Leaf L = leaf_of(g, T.set, S, lRoots[rr], e);
+    const RecVE ve = rec_ve(lNbr + 27 * rr, lSet[rr], L, g.D);
     const int4 tb = tileBase[blockIdx.x];
     if (keep) {
         float v[8];
@@ -1310,7 +1345,7 @@ __global__ void __launch_bounds__(kThreads) sub_emit(Grid g, const Tile* __restr
         for (int k = 0; k < 3 * nt; ++k) {
             const int ed = int((mw >> (4 * k)) & 0xF);
             int kind;
-            const int owner = edge_owner(L, g.D, ed, kind, [&](int3 n) { return virtual_enum(g, L, n); });
+            const int owner = edge_owner(L, g.D, ed, kind, ve);
             const unsigned ow = cellWord[owner];
             const int raw = tileBase[S.firstTile + (owner - S.dEnum) / kTile].x + crossPrefix[owner] +
                             __popc(ow & kEdgeBits & ((1u << kind) - 1u));
