@@ -454,6 +454,13 @@ __device__ __forceinline__ Leaf leaf_of(const Grid& g, int si, const SetDesc& S,
     return L;
 }
 
+// Depth-D cell of enumeration e, which lies in tile root R.
+__device__ __forceinline__ int3 cell_of(const Grid& g, const SetDesc& S, const TileRoot& R, int e) {
+    const int sh = g.D - S.rd;
+    const int3 lc = demorton(e - S.dEnum - (R.q << (3 * sh)), sh);
+    return make_int3((R.Rc.x << sh) + lc.x, (R.Rc.y << sh) + lc.y, (R.Rc.z << sh) + lc.z);
+}
+
 __device__ __forceinline__ Leaf root_leaf(const Grid& g, const SetDesc& S, int si, const TileRoot& R) {
     Leaf F;
     F.S = S;
@@ -1189,6 +1196,8 @@ __global__ void __launch_bounds__(kThreads) sub_classify(Grid g, const Tile* __r
     __shared__ unsigned lSet[kMaxRootsPerTile];
     __shared__ int lSum[kThreads / PSR_WAVE];
     __shared__ float lVal[kClassifyPoints];
+    __shared__ unsigned short lItem[kThreads * 12];  // crossing edges as (cell << 4 | edge), cell order
+    __shared__ unsigned lOwn[kThreads];              // per cell: owned crossing edges, kept ones << 12
     const Tile T = tiles[blockIdx.x];
     const SetDesc S = g.sets[T.set];
     const int D = g.D;
@@ -1213,42 +1222,68 @@ __global__ void __launch_bounds__(kThreads) sub_classify(Grid g, const Tile* __r
     __syncthreads();
     const bool active = int(threadIdx.x) < T.count;
     const int e = T.start + (active ? int(threadIdx.x) : 0);
-    unsigned mask = 0, keep = 0;
-    int nt = 0;
+    const int rr = tg.oneRoot ? 0 : int(threadIdx.x) >> (3 * (D - S.rd));
+    const float inv = 1.0f / float(1 << D);
+    // corner values, cube class and the edges whose ends differ in sign
+    float v[8];
+    int cls = 0;
+    unsigned cross = 0;
     if (active) {
-        const int rr = tg.oneRoot ? 0 : int(threadIdx.x) >> (3 * (D - S.rd));
-        const Leaf L = leaf_of(g, T.set, S, lRoots[rr], e);
-        const RecVE ve = rec_ve(lNbr + 27 * rr, lSet[rr], L, D);
+        const int3 c = cell_of(g, S, lRoots[rr], e);
         const int3 f = lRoots[rr].first;
         const float* pv = lVal + rr * PP;
-        float v[8];
-        int cls = 0;
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
             const int3 o = bourke_offset(b);
-            v[b] = pv[((L.c.x - f.x + o.x) * ey + (L.c.y - f.y + o.y)) * ez + (L.c.z - f.z + o.z)];
+            v[b] = pv[((c.x - f.x + o.x) * ey + (c.y - f.y + o.y)) * ez + (c.z - f.z + o.z)];
             if (v[b] < 0.f) cls |= 1 << b;
         }
-        const float inv = 1.0f / float(1 << D);
-        for (int k = 0; k < 12; ++k) {
-            if (!(v[kEdgeVertex[k][0]] * v[kEdgeVertex[k][1]] <= 0.f)) continue;
-            int kind;
-            if (edge_owner(L, D, k, kind, ve) != e || kind != k) continue;
-            mask |= 1u << k;
-            if (!(fabsf(cell_crossing(L.c, k, v, inv).x) < 1e-6f)) keep |= 1u << k;
+#pragma unroll
+        for (int k = 0; k < 12; ++k)
+            if (v[kEdgeVertex[k][0]] * v[kEdgeVertex[k][1]] <= 0.f) cross |= 1u << k;
+    }
+    // ownership of those edges: (cell, edge) items shared out over the block,
+    // owned and kept edges gathered per cell in LDS
+    lOwn[threadIdx.x] = 0u;
+    int first;
+    const int nItems = block_excl_scan<kThreads>(__popc(cross), first);
+    for (int k = 0; k < 12; ++k)
+        if ((cross >> k) & 1u) lItem[first++] = (threadIdx.x << 4) | k;
+    __syncthreads();
+    for (int it = threadIdx.x; it < nItems; it += kThreads) {
+        const int cell = lItem[it] >> 4, k = lItem[it] & 15, r = tg.oneRoot ? 0 : cell >> (3 * (D - S.rd));
+        const int ec = T.start + cell;
+        const Leaf L = leaf_of(g, T.set, S, lRoots[r], ec);
+        int kind;
+        if (edge_owner(L, D, k, kind, rec_ve(lNbr + 27 * r, lSet[r], L, D)) != ec || kind != k) continue;
+        const int3 f = lRoots[r].first;
+        const float* pv = lVal + r * PP;
+        float w[8];
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const int3 o = bourke_offset(b);
+            w[b] = pv[((L.c.x - f.x + o.x) * ey + (L.c.y - f.y + o.y)) * ez + (L.c.z - f.z + o.z)];
         }
+        unsigned bits = 1u << k;
+        if (!(fabsf(cell_crossing(L.c, k, w, inv).x) < 1e-6f)) bits |= 1u << (12 + k);
+        atomicOr(&lOwn[cell], bits);
+    }
+    __syncthreads();
+    const unsigned mask = lOwn[threadIdx.x] & kEdgeBits, keep = (lOwn[threadIdx.x] >> 12) & kEdgeBits;
+    int nt = 0;
+    if (active) {
         cellWord[e] = mask | (keep << 12) | (unsigned(cls) << 24);
         if (mask)
 #pragma unroll
             for (int b = 0; b < 8; ++b) vals[size_t(b) * nVD + e] = v[b];
         nt = mc_tri_count(cls);
     }
-    int tc, tk, tt;
-    const int pre = block_exclusive<kThreads>(__popc(mask), lSum, tc);
-    if (active) crossPrefix[e] = pre;
+    // crossings (prefix and total) and triangles in one scan, kept vertices in another
+    int tct, tk;
+    const int pre = block_exclusive<kThreads>((__popc(mask) << 16) | nt, lSum, tct);
+    if (active) crossPrefix[e] = pre >> 16;
     (void)block_exclusive<kThreads>(__popc(keep), lSum, tk);
-    (void)block_exclusive<kThreads>(nt, lSum, tt);
-    if (threadIdx.x == 0) tileCnt[blockIdx.x] = make_int4(tc, tk, tt, 0);
+    if (threadIdx.x == 0) tileCnt[blockIdx.x] = make_int4(tct >> 16, tk, tct & 0xFFFF, 0);
 }
 
 // Exclusive scan of the per-tile counts in one workgroup; totals[0..2] are the
