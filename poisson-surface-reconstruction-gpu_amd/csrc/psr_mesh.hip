@@ -8,7 +8,10 @@ namespace psr {
 
 namespace {
 
-constexpr int kThreads = 256;
+constexpr int kThreads = 256;  // a tile of the self-scanning kernels is one workgroup
+// vertex values: one wave per workgroup, so that the chains of dependent loads
+// of a hundred thousand vertices spread over every CU
+constexpr int kValueThreads = PSR_WAVE;
 
 // --- ownership (BuildMeshGeometry.cu:42-400) -------------------------------
 // A corner / edge / face of node o is owned by the smallest-key neighbour whose
@@ -43,144 +46,31 @@ __device__ __forceinline__ void load_neighbourhood(const NodeView& v, int o, int
     }
 }
 
-// Owned corners of each node, written in (node, corner) order: the owner test
-// and the compaction in one launch (the scan of the owned counts runs across
-// tiles by decoupled look-back); the grand total goes to t.total.
-__global__ void __launch_bounds__(kThreads) vertex_owner_emit(NodeView v, ScanTicket t, int* __restrict__ vOwner,
-                                                              int* __restrict__ vKind, float4* __restrict__ vPos) {
-    const int tile = ticket_tile(t);
+// Per node, the corners (all depths), edges (depth D) and faces (all depths)
+// it owns, each kind compacted in (node, slot) order by its own scan (the
+// three scans share the launch's tile order), and the node-side tables
+// nodeVert / nodeEdge / nodeFace written by the owner for every node sharing
+// the element (BuildMeshGeometry.cu:42-400: ownership, then maintenance).
+struct GeoOut {
+    int* vOwner;
+    int* vKind;
+    float4* vPos;
+    int* nodeVert;
+    int* eOwner;
+    int* eKind;
+    int* nodeEdge;
+    int* fOwner;
+    int* fKind;
+    int* fHasParent;
+    int* nodeFace;
+};
+
+__global__ void __launch_bounds__(kThreads) geometry_emit(NodeView v, int offD, int nD,
+                                                          const int* __restrict__ parent, ScanTicket tv,
+                                                          ScanTicket te, ScanTicket tf, GeoOut out) {
+    const int tile = ticket_tile(tv);
     const int o = tile * kThreads + threadIdx.x;
-    unsigned mask = 0;
-    int depth = 0;
-    float half = 0.f;
-    float3 nc = make_float3(0.f, 0.f, 0.f);
-    if (o < v.M) {
-        depth = v.depth[o];
-        half = 1.0f / float(1 << (depth + 1));
-        const float W = 1.0f / float(1 << depth), W2 = W * W;
-        int nb[27], kk[27];
-        float3 cc[27];
-        load_neighbourhood(v, o, nb, cc, kk);
-        nc = cc[13];  // (0, 0, 0) when the node has no self-neighbour
-        int ownKey[8], own[8];
-        for (int i = 0; i < 8; ++i) {
-            ownKey[i] = 0x7fffffff;
-            own[i] = -1;
-        }
-#pragma unroll
-        for (int j = 0; j < 27; ++j) {
-            if (nb[j] == -1) continue;
-#pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                float3 p = corner_pos(nc, i, half);
-                if (sqdist(p, cc[j]) < double(W2) && ownKey[i] > kk[j]) {
-                    ownKey[i] = kk[j];
-                    own[i] = nb[j];
-                }
-            }
-        }
-        for (int i = 0; i < 8; ++i) mask |= (own[i] == o ? 1u : 0u) << i;
-    }
-    int a = ticket_prefix<kThreads>(t, tile, __popc(mask));
-    for (int i = 0; i < 8; ++i) {
-        if (!((mask >> i) & 1u)) continue;
-        const float3 p = corner_pos(nc, i, half);
-        vOwner[a] = o;
-        vKind[a] = i;
-        vPos[a] = make_float4(p.x, p.y, p.z, __int_as_float(depth));
-        ++a;
-    }
-}
-
-__global__ void __launch_bounds__(kThreads) vertex_maintain(NodeView v, const int* __restrict__ count,
-                                                            const int* __restrict__ vOwner,
-                                                            const float4* __restrict__ vPos,
-                                                            int* __restrict__ nodeVert) {
-    int t = blockIdx.x * kThreads + threadIdx.x;
-    if (t >= *count) return;
-    int owner = vOwner[t];
-    int depth = v.depth[owner];
-    float W = 1.0f / float(1 << depth), W2 = W * W;
-    float4 q = vPos[t];
-    float3 p = make_float3(q.x, q.y, q.z);
-    int nb[27], kk[27];
-    float3 cc[27];
-    load_neighbourhood(v, owner, nb, cc, kk);
-#pragma unroll
-    for (int i = 0; i < 27; ++i) {
-        if (nb[i] == -1 || !(sqdist(p, cc[i]) < double(W2))) continue;
-        nodeVert[corner_slot(cc[i], p) * v.M + nb[i]] = t + 1;
-    }
-}
-
-// Owned edges of each depth-D node in (node, edge) order, compacted in the same launch.
-__global__ void __launch_bounds__(kThreads) edge_owner_emit(NodeView v, int offD, int nD, ScanTicket t,
-                                                            int* __restrict__ eOwner, int* __restrict__ eKind) {
-    const int tile = ticket_tile(t);
-    const int n = tile * kThreads + threadIdx.x, o = offD + n;
-    unsigned mask = 0;
-    if (n < nD) {
-        const float half = 1.0f / float(1 << (v.D + 1)), W = 1.0f / float(1 << v.D), W2 = W * W;
-        int nb[27], kk[27];
-        float3 cc[27];
-        load_neighbourhood(v, o, nb, cc, kk);
-        const float3 nc = cc[13];
-        int ownKey[12], own[12];
-        for (int e = 0; e < 12; ++e) {
-            ownKey[e] = 0x7fffffff;
-            own[e] = -1;
-        }
-#pragma unroll
-        for (int j = 0; j < 27; ++j) {
-            if (nb[j] == -1) continue;
-#pragma unroll
-            for (int e = 0; e < 12; ++e) {
-                float3 p = edge_center(nc, e, half);
-                if (sqdist(p, cc[j]) < double(W2) && ownKey[e] > kk[j]) {
-                    ownKey[e] = kk[j];
-                    own[e] = nb[j];
-                }
-            }
-        }
-        for (int e = 0; e < 12; ++e) mask |= (own[e] == o ? 1u : 0u) << e;
-    }
-    int a = ticket_prefix<kThreads>(t, tile, __popc(mask));
-    for (int e = 0; e < 12; ++e) {
-        if (!((mask >> e) & 1u)) continue;
-        eOwner[a] = o;
-        eKind[a] = e;
-        ++a;
-    }
-}
-
-__global__ void __launch_bounds__(kThreads) edge_maintain(NodeView v, int offD, const int* __restrict__ count,
-                                                          const int* __restrict__ eOwner,
-                                                          const int* __restrict__ eKind, int nD,
-                                                          int* __restrict__ nodeEdge) {
-    int t = blockIdx.x * kThreads + threadIdx.x;
-    if (t >= *count) return;
-    int owner = eOwner[t], kind = eKind[t];
-    float half = 1.0f / float(1 << (v.D + 1)), W = 1.0f / float(1 << v.D), W2 = W * W;
-    int nb[27], kk[27];
-    float3 cc[27];
-    load_neighbourhood(v, owner, nb, cc, kk);
-    const float3 p = edge_center(cc[13], kind, half);
-#pragma unroll
-    for (int i = 0; i < 27; ++i) {
-        if (nb[i] == -1 || !(sqdist(p, cc[i]) < double(W2))) continue;
-        nodeEdge[edge_slot(cc[i], p, kind) * nD + (nb[i] - offD)] = t + 1;
-    }
-}
-
-// Owned faces of each node in (node, face) order, compacted in the same launch;
-// a face has a parent face when the node has a parent and the face lies on the
-// parent's boundary.
-__global__ void __launch_bounds__(kThreads) face_owner_emit(NodeView v, const int* __restrict__ parent, ScanTicket t,
-                                                            int* __restrict__ fOwner, int* __restrict__ fKind,
-                                                            int* __restrict__ fHasParent) {
-    const int tile = ticket_tile(t);
-    const int o = tile * kThreads + threadIdx.x;
-    unsigned mask = 0;
+    unsigned vm = 0, em = 0, fm = 0;
     int depth = 0;
     if (o < v.M) {
         depth = v.depth[o];
@@ -188,67 +78,127 @@ __global__ void __launch_bounds__(kThreads) face_owner_emit(NodeView v, const in
         int nb[27], kk[27];
         float3 cc[27];
         load_neighbourhood(v, o, nb, cc, kk);
-        const float3 nc = cc[13];
-        int ownKey[6], own[6];
-        for (int f = 0; f < 6; ++f) {
-            ownKey[f] = 0x7fffffff;
-            own[f] = -1;
-        }
+        const float3 nc = cc[13];  // (0, 0, 0) when the node has no self-neighbour
+        {
+            int ownKey[8], own[8];
+            for (int i = 0; i < 8; ++i) {
+                ownKey[i] = 0x7fffffff;
+                own[i] = -1;
+            }
 #pragma unroll
-        for (int j = 0; j < 27; ++j) {
-            if (nb[j] == -1) continue;
+            for (int j = 0; j < 27; ++j) {
+                if (nb[j] == -1) continue;
 #pragma unroll
-            for (int f = 0; f < 6; ++f) {
-                float3 p = face_center(nc, f, half);
-                if (sqdist(p, cc[j]) < double(W2) && ownKey[f] > kk[j]) {
-                    ownKey[f] = kk[j];
-                    own[f] = nb[j];
+                for (int i = 0; i < 8; ++i) {
+                    float3 p = corner_pos(nc, i, half);
+                    if (sqdist(p, cc[j]) < double(W2) && ownKey[i] > kk[j]) {
+                        ownKey[i] = kk[j];
+                        own[i] = nb[j];
+                    }
                 }
             }
+            for (int i = 0; i < 8; ++i) vm |= (own[i] == o ? 1u : 0u) << i;
         }
-        for (int f = 0; f < 6; ++f) mask |= (own[f] == o ? 1u : 0u) << f;
+        {
+            int ownKey[6], own[6];
+            for (int f = 0; f < 6; ++f) {
+                ownKey[f] = 0x7fffffff;
+                own[f] = -1;
+            }
+#pragma unroll
+            for (int j = 0; j < 27; ++j) {
+                if (nb[j] == -1) continue;
+#pragma unroll
+                for (int f = 0; f < 6; ++f) {
+                    float3 p = face_center(nc, f, half);
+                    if (sqdist(p, cc[j]) < double(W2) && ownKey[f] > kk[j]) {
+                        ownKey[f] = kk[j];
+                        own[f] = nb[j];
+                    }
+                }
+            }
+            for (int f = 0; f < 6; ++f) fm |= (own[f] == o ? 1u : 0u) << f;
+        }
+        if (o >= offD) {  // depth D (the half width and width are those of depth D)
+            int ownKey[12], own[12];
+            for (int e = 0; e < 12; ++e) {
+                ownKey[e] = 0x7fffffff;
+                own[e] = -1;
+            }
+#pragma unroll
+            for (int j = 0; j < 27; ++j) {
+                if (nb[j] == -1) continue;
+#pragma unroll
+                for (int e = 0; e < 12; ++e) {
+                    float3 p = edge_center(nc, e, half);
+                    if (sqdist(p, cc[j]) < double(W2) && ownKey[e] > kk[j]) {
+                        ownKey[e] = kk[j];
+                        own[e] = nb[j];
+                    }
+                }
+            }
+            for (int e = 0; e < 12; ++e) em |= (own[e] == o ? 1u : 0u) << e;
+        }
     }
-    int a = ticket_prefix<kThreads>(t, tile, __popc(mask));
-    if (!mask) return;
-    const int son = (v.key[o] >> (3 * (v.D - depth))) & 7;
-    const bool hasParent = parent[o] != -1;
-    for (int f = 0; f < 6; ++f) {
-        if (!((mask >> f) & 1u)) continue;
-        fOwner[a] = o;
-        fKind[a] = f;
-        fHasParent[a] = hasParent && kParentFaceKind[son][f] != -1 ? 1 : -1;
-        ++a;
-    }
-}
-
-__global__ void __launch_bounds__(kThreads) face_maintain(NodeView v, const int* __restrict__ count,
-                                                          const int* __restrict__ fOwner,
-                                                          const int* __restrict__ fKind, int* __restrict__ nodeFace) {
-    int t = blockIdx.x * kThreads + threadIdx.x;
-    if (t >= *count) return;
-    int owner = fOwner[t], kind = fKind[t], orient = kind >> 1;
-    int depth = v.depth[owner];
-    float half = 1.0f / float(1 << (depth + 1)), W = 1.0f / float(1 << depth), W2 = W * W;
+    int av = ticket_prefix<kThreads>(tv, tile, __popc(vm));
+    int ae = ticket_prefix<kThreads>(te, tile, __popc(em));
+    int af = ticket_prefix<kThreads>(tf, tile, __popc(fm));
+    if (!(vm | em | fm)) return;
+    // emission and maintenance; the neighbourhood is read again (cached)
+    const float half = 1.0f / float(1 << (depth + 1)), W = 1.0f / float(1 << depth), W2 = W * W;
     int nb[27], kk[27];
     float3 cc[27];
-    load_neighbourhood(v, owner, nb, cc, kk);
-    const float3 p = face_center(cc[13], kind, half);
+    load_neighbourhood(v, o, nb, cc, kk);
+    const float3 nc = cc[13];
+    for (int i = 0; i < 8; ++i) {
+        if (!((vm >> i) & 1u)) continue;
+        const float3 p = corner_pos(nc, i, half);
+        out.vOwner[av] = o;
+        out.vKind[av] = i;
+        out.vPos[av] = make_float4(p.x, p.y, p.z, __int_as_float(depth));
+        ++av;
 #pragma unroll
-    for (int i = 0; i < 27; ++i) {
-        if (nb[i] == -1 || !(sqdist(p, cc[i]) < double(W2))) continue;
-        const float cv = orient == 0 ? cc[i].x : (orient == 1 ? cc[i].y : cc[i].z);
-        const float pv = orient == 0 ? p.x : (orient == 1 ? p.y : p.z);
-        nodeFace[((orient << 1) + (cv - pv < 0 ? 1 : 0)) * v.M + nb[i]] = t + 1;
+        for (int j = 0; j < 27; ++j) {
+            if (nb[j] == -1 || !(sqdist(p, cc[j]) < double(W2))) continue;
+            out.nodeVert[corner_slot(cc[j], p) * v.M + nb[j]] = av;  // index + 1
+        }
     }
-}
-
-__global__ void __launch_bounds__(kThreads) zero_ints(int* __restrict__ a, size_t n) {
-    size_t i = size_t(blockIdx.x) * kThreads + threadIdx.x;
-    if (i < n) a[i] = 0;
+    for (int e = 0; e < 12; ++e) {
+        if (!((em >> e) & 1u)) continue;
+        const float3 p = edge_center(nc, e, half);
+        out.eOwner[ae] = o;
+        out.eKind[ae] = e;
+        ++ae;
+#pragma unroll
+        for (int j = 0; j < 27; ++j) {
+            if (nb[j] == -1 || !(sqdist(p, cc[j]) < double(W2))) continue;
+            out.nodeEdge[edge_slot(cc[j], p, e) * nD + (nb[j] - offD)] = ae;
+        }
+    }
+    if (fm) {
+        const int son = (v.key[o] >> (3 * (v.D - depth))) & 7;
+        const bool hasParent = parent[o] != -1;
+        for (int f = 0; f < 6; ++f) {
+            if (!((fm >> f) & 1u)) continue;
+            const float3 p = face_center(nc, f, half);
+            const int orient = f >> 1;
+            out.fOwner[af] = o;
+            out.fKind[af] = f;
+            out.fHasParent[af] = hasParent && kParentFaceKind[son][f] != -1 ? 1 : -1;
+            ++af;
+#pragma unroll
+            for (int j = 0; j < 27; ++j) {
+                if (nb[j] == -1 || !(sqdist(p, cc[j]) < double(W2))) continue;
+                const float cv = orient == 0 ? cc[j].x : (orient == 1 ? cc[j].y : cc[j].z);
+                const float pv = orient == 0 ? p.x : (orient == 1 ? p.y : p.z);
+                out.nodeFace[((orient << 1) + (cv - pv < 0 ? 1 : 0)) * v.M + nb[j]] = af;
+            }
+        }
+    }
 }
 
 // --- vertex implicit values (ComputeTriangleIndices.cu:346-399), quirk Q8 --
-__global__ void __launch_bounds__(kThreads) vertex_values(NodeView v, const int* __restrict__ count,
+__global__ void __launch_bounds__(kValueThreads) vertex_values(NodeView v, const int* __restrict__ count,
                                                           const int* __restrict__ vOwner,
                                                           const int* __restrict__ vKind,
                                                           const float4* __restrict__ vPos,
@@ -259,7 +209,7 @@ __global__ void __launch_bounds__(kThreads) vertex_values(NodeView v, const int*
                                                           const float* __restrict__ iso, float* __restrict__ vv,
                                                           int* __restrict__ mail, const int* __restrict__ results,
                                                           int* __restrict__ resultsHost, int resultWords) {
-    int t = blockIdx.x * kThreads + threadIdx.x;
+    int t = blockIdx.x * kValueThreads + threadIdx.x;
     if (t < 3) mail[t] = count[t];  // vertex, edge and face counts for the host
     // the run's device results (CG statistics, iso value) are final here
     if (t < resultWords) resultsHost[t] = results[t];
@@ -446,32 +396,27 @@ void build_geometry(Ctx& c) {
     g.nodeFace = c.buf<int>("node_face", size_t(6) * M);
     g.nodeEdge = c.buf<int>("node_edge", size_t(12) * nD);
     zero_spans({{g.nodeVert, size_t(8) * M}, {g.nodeFace, size_t(6) * M}, {g.nodeEdge, size_t(12) * nD}}, s);
-    // vertices (all depths)
-    const int tilesM = div_up(M, kThreads), tilesD = div_up(nD, kThreads);
-    ScanState& scan = c.scan_for_tiles(size_t(tilesM));
-    vertex_owner_emit<<<tilesM, kThreads, 0, s>>>(v, scan_ticket(scan, tilesM, g.counts + 0), g.vtxOwner, g.vtxKind,
-                                                   g.vtxPos);
-    vertex_maintain<<<div_up(size_t(8) * M, kThreads), kThreads, 0, s>>>(v, g.counts + 0, g.vtxOwner, g.vtxPos,
-                                                                        g.nodeVert);
-    // edges (depth D)
+    // vertices, edges, faces: one launch, three scans over the same tiles
     g.edgeOwner = c.buf<int>("edge_owner", size_t(12) * nD);
     g.edgeKind = c.buf<int>("edge_kind", size_t(12) * nD);
-    edge_owner_emit<<<tilesD, kThreads, 0, s>>>(v, offD, nD, scan_ticket(scan, tilesD, g.counts + 1), g.edgeOwner,
-                                                 g.edgeKind);
-    edge_maintain<<<div_up(size_t(12) * nD, kThreads), kThreads, 0, s>>>(v, offD, g.counts + 1, g.edgeOwner,
-                                                                        g.edgeKind, nD, g.nodeEdge);
-    // faces (all depths)
     g.faceOwner = c.buf<int>("face_owner", size_t(6) * M);
     g.faceKind = c.buf<int>("face_kind", size_t(6) * M);
     g.faceHasParent = c.buf<int>("face_hasparent", size_t(6) * M);
-    face_owner_emit<<<tilesM, kThreads, 0, s>>>(v, o.parent, scan_ticket(scan, tilesM, g.counts + 2), g.faceOwner,
-                                                 g.faceKind, g.faceHasParent);
-    face_maintain<<<div_up(size_t(6) * M, kThreads), kThreads, 0, s>>>(v, g.counts + 2, g.faceOwner, g.faceKind,
-                                                                      g.nodeFace);
+    const int tilesM = div_up(M, kThreads);
+    ScanState& scan = c.scan_for_tiles(size_t(3) * tilesM);
+    const ScanTicket tv = scan_ticket(scan, tilesM, g.counts + 0);
+    ScanTicket te = tv, tf = tv;  // one epoch; status words of their own
+    te.status += tilesM;
+    te.total = g.counts + 1;
+    tf.status += 2 * size_t(tilesM);
+    tf.total = g.counts + 2;
+    const GeoOut out{g.vtxOwner,  g.vtxKind,  g.vtxPos,   g.nodeVert,      g.edgeOwner, g.edgeKind,
+                     g.nodeEdge,  g.faceOwner, g.faceKind, g.faceHasParent, g.nodeFace};
+    geometry_emit<<<tilesM, kThreads, 0, s>>>(v, offD, nD, o.parent, tv, te, tf, out);
     // vertex implicit values
     float* isoDev = &c.results()->iso;
     g.vvalue = c.buf<float>("vvalue", size_t(8) * M);
-    vertex_values<<<div_up(size_t(8) * M, kThreads), kThreads, 0, s>>>(v, g.counts + 0, g.vtxOwner, g.vtxKind,
+    vertex_values<<<div_up(size_t(8) * M, kValueThreads), kValueThreads, 0, s>>>(v, g.counts + 0, g.vtxOwner, g.vtxKind,
                                                                        g.vtxPos, o.parent, o.childBase,
                                                                        c.fld.nbrCol, o.q1Fired, isoDev, g.vvalue, c.mail,
                                                                        reinterpret_cast<const int*>(c.results()),
