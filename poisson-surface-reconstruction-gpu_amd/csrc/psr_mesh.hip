@@ -65,14 +65,64 @@ struct GeoOut {
     int* nodeFace;
 };
 
+// Ownership on the regular neighbour grid (an octree in which the
+// level-local child 0 rule changed no parent, and o its own neighbour 13):
+// neighbour i lies at offset (i / 9 - 1, (i / 3) % 3 - 1, i % 3 - 1) widths, so
+// the neighbours within one width of a corner (edge, face) centre are exactly
+// the 8 (4, 2) sharing it -- every other one is at least sqrt(5) / 2 widths
+// away -- and the smallest key among them owns it.  kk: neighbour keys
+// (INT_MAX where absent); the result is that of the distance test.
+__device__ __forceinline__ void regular_owned(const int (&kk)[27], bool depthD, unsigned& vm, unsigned& em,
+                                              unsigned& fm) {
+    constexpr int kStride[3] = {9, 3, 1};
+    const int self = kk[13];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {  // corner side per axis: bit a of i
+        int mn = 0x7fffffff;
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            mn = min(mn, kk[9 * ((i & 1) + (q & 1)) + 3 * (((i >> 1) & 1) + ((q >> 1) & 1)) +
+                            (((i >> 2) & 1) + ((q >> 2) & 1))]);
+        vm |= (mn == self ? 1u : 0u) << i;
+    }
+#pragma unroll
+    for (int f = 0; f < 6; ++f)
+        fm |= (kk[13 + (2 * (f & 1) - 1) * kStride[f >> 1]] > self ? 1u : 0u) << f;
+    if (!depthD) return;
+#pragma unroll
+    for (int e = 0; e < 12; ++e) {  // axis e >> 2; sides of the other two axes in bits 0, 1
+        const int a = e >> 2, u0 = a == 0 ? 1 : 0, u1 = a == 2 ? 1 : 2;
+        int mn = 0x7fffffff;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            mn = min(mn, kk[13 + ((e & 1) - 1 + (q & 1)) * kStride[u0] + (((e >> 1) & 1) - 1 + (q >> 1)) * kStride[u1]]);
+        em |= (mn == self ? 1u : 0u) << e;
+    }
+}
+
 __global__ void __launch_bounds__(kThreads) geometry_emit(NodeView v, int offD, int nD,
-                                                          const int* __restrict__ parent, ScanTicket tv,
+                                                          const int* __restrict__ parent,
+                                                          const int* __restrict__ q1Fired, ScanTicket tv,
                                                           ScanTicket te, ScanTicket tf, GeoOut out) {
     const int tile = ticket_tile(tv);
     const int o = tile * kThreads + threadIdx.x;
     unsigned vm = 0, em = 0, fm = 0;
     int depth = 0;
-    if (o < v.M) {
+    bool done = false;
+    if (o < v.M && *q1Fired == 0) {
+        depth = v.depth[o];
+        int kk[27];
+#pragma unroll
+        for (int i = 0; i < 27; ++i) {
+            const int nb = v.neigh[i * v.M + o];
+            kk[i] = nb != -1 ? v.key[nb] : 0x7fffffff;
+        }
+        if (v.neigh[13 * v.M + o] == o) {
+            regular_owned(kk, o >= offD, vm, em, fm);
+            done = true;
+        }
+    }
+    if (o < v.M && !done) {
         depth = v.depth[o];
         const float half = 1.0f / float(1 << (depth + 1)), W = 1.0f / float(1 << depth), W2 = W * W;
         int nb[27], kk[27];
@@ -412,7 +462,7 @@ void build_geometry(Ctx& c) {
     tf.total = g.counts + 2;
     const GeoOut out{g.vtxOwner,  g.vtxKind,  g.vtxPos,   g.nodeVert,      g.edgeOwner, g.edgeKind,
                      g.nodeEdge,  g.faceOwner, g.faceKind, g.faceHasParent, g.nodeFace};
-    geometry_emit<<<tilesM, kThreads, 0, s>>>(v, offD, nD, o.parent, tv, te, tf, out);
+    geometry_emit<<<tilesM, kThreads, 0, s>>>(v, offD, nD, o.parent, o.q1Fired, tv, te, tf, out);
     // vertex implicit values
     float* isoDev = &c.results()->iso;
     g.vvalue = c.buf<float>("vvalue", size_t(8) * M);
