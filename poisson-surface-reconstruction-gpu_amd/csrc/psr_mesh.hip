@@ -194,8 +194,65 @@ __global__ void __launch_bounds__(kThreads) geometry_emit(NodeView v, int offD, 
     int ae = ticket_prefix<kThreads>(te, tile, __popc(em));
     int af = ticket_prefix<kThreads>(tf, tile, __popc(fm));
     if (!(vm | em | fm)) return;
-    // emission and maintenance; the neighbourhood is read again (cached)
     const float half = 1.0f / float(1 << (depth + 1)), W = 1.0f / float(1 << depth), W2 = W * W;
+    if (done) {
+        // regular grid: the sharing neighbours by index, the element's slot in
+        // each of them by its side (what the distance and sign tests give)
+        constexpr int kStride[3] = {9, 3, 1};
+        int nb[27];
+#pragma unroll
+        for (int i = 0; i < 27; ++i) nb[i] = v.neigh[i * v.M + o];
+        const float4 b = v.basis[o];
+        const float3 nc = make_float3(b.x, b.y, b.z);
+        for (int i = 0; i < 8; ++i) {
+            if (!((vm >> i) & 1u)) continue;
+            const float3 p = corner_pos(nc, i, half);
+            out.vOwner[av] = o;
+            out.vKind[av] = i;
+            out.vPos[av] = make_float4(p.x, p.y, p.z, __int_as_float(depth));
+            ++av;
+            const int sx = i & 1, sy = (i >> 1) & 1, sz = (i >> 2) & 1;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int qx = q & 1, qy = (q >> 1) & 1, qz = q >> 2;
+                const int j = 9 * (sx + qx) + 3 * (sy + qy) + (sz + qz);
+                if (nb[j] == -1) continue;
+                // the corner lies on the high side of neighbour j in axis a iff q_a = 0 (Bourke slot)
+                int slot = (1 - qx) | ((1 - qz) << 2);
+                if (!qy) slot += (slot & 1) ? 1 : 3;
+                out.nodeVert[slot * v.M + nb[j]] = av;  // index + 1
+            }
+        }
+        for (int e = 0; e < 12; ++e) {
+            if (!((em >> e) & 1u)) continue;
+            out.eOwner[ae] = o;
+            out.eKind[ae] = e;
+            ++ae;
+            const int a = e >> 2, u0 = a == 0 ? 1 : 0, u1 = a == 2 ? 1 : 2;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int j = 13 + ((e & 1) - 1 + (q & 1)) * kStride[u0] + (((e >> 1) & 1) - 1 + (q >> 1)) * kStride[u1];
+                if (nb[j] == -1) continue;
+                out.nodeEdge[((a << 2) | (1 - (q & 1)) | ((1 - (q >> 1)) << 1)) * nD + (nb[j] - offD)] = ae;
+            }
+        }
+        if (fm) {
+            const int son = (v.key[o] >> (3 * (v.D - depth))) & 7;
+            const bool hasParent = parent[o] != -1;
+            for (int f = 0; f < 6; ++f) {
+                if (!((fm >> f) & 1u)) continue;
+                out.fOwner[af] = o;
+                out.fKind[af] = f;
+                out.fHasParent[af] = hasParent && kParentFaceKind[son][f] != -1 ? 1 : -1;
+                ++af;
+                out.nodeFace[f * v.M + o] = af;
+                const int j = 13 + (2 * (f & 1) - 1) * kStride[f >> 1];
+                if (nb[j] != -1) out.nodeFace[(f ^ 1) * v.M + nb[j]] = af;
+            }
+        }
+        return;
+    }
+    // emission and maintenance; the neighbourhood is read again (cached)
     int nb[27], kk[27];
     float3 cc[27];
     load_neighbourhood(v, o, nb, cc, kk);
