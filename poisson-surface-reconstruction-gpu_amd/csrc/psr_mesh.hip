@@ -417,18 +417,32 @@ __global__ void __launch_bounds__(kThreads) cube_triangles(
     }
     const int nt = mc_tri_count(c);
     const int base = ticket_prefix<kThreads>(t, tile, nt);
-    if (nt == 0) return;
-    const uint64_t w = mc_word(c);
-    int edgeHas = 0;
-    for (int k = 0; k < 3 * nt; ++k) {
-        const int e = int((w >> (4 * k)) & 0xF);
-        edgeHas |= 1 << e;
-        tri[3 * base + k] = vexAddr[nodeEdge[e * nD + n] - 1];
+    unsigned faces = 0;  // faces holding an edge of an emitted triangle
+    if (nt > 0) {
+        const uint64_t w = mc_word(c);
+        int edgeHas = 0;
+        for (int k = 0; k < 3 * nt; ++k) {
+            const int e = int((w >> (4 * k)) & 0xF);
+            edgeHas |= 1 << e;
+            tri[3 * base + k] = vexAddr[nodeEdge[e * nD + n] - 1];
+        }
+        for (int f = 0; f < 6; ++f) {
+            bool mark = false;
+            for (int j = 0; j < 4; ++j) mark |= (edgeHas >> kFaceEdges[f][j]) & 1;
+            faces |= (mark ? 1u : 0u) << f;
+        }
     }
-    for (int f = 0; f < 6; ++f) {
-        bool mark = false;
-        for (int j = 0; j < 4; ++j) mark |= (edgeHas >> kFaceEdges[f][j]) & 1;
-        if (!mark) continue;
+    // each marked face walks up the parent chain, as an item of its own shared
+    // out over the block (the walks are chains of dependent loads; marks are
+    // idempotent, so their order does not matter)
+    __shared__ unsigned short lWalk[kThreads * 6];
+    int first;
+    const int nWalks = block_excl_scan<kThreads>(__popc(faces), first);
+    for (int f = 0; f < 6; ++f)
+        if ((faces >> f) & 1u) lWalk[first++] = (threadIdx.x << 3) | f;
+    __syncthreads();
+    for (int it = threadIdx.x; it < nWalks; it += kThreads) {
+        const int f = lWalk[it] & 7, o = offD + tile * kThreads + (lWalk[it] >> 3);
         int pn = parent[o];
         int cf = nodeFace[f * M + o] - 1;
         if (cf < 0) continue;  // face slot left unassigned (broken tables): nothing to mark
