@@ -29,12 +29,11 @@ void* Arena::get(const std::string& name, size_t bytes) {
     Block& b = blocks_[name];
     if (b.bytes < bytes) {
         if (b.ptr) {
-            PSR_HIP_CHECK(hipDeviceSynchronize());
-            PSR_HIP_CHECK(hipFree(b.ptr));
+            PSR_HIP_CHECK(hipFreeAsync(b.ptr, stream_));  // after the work queued on it
             total_ -= b.bytes;
         }
         size_t want = bytes + bytes / 8 + 256;
-        PSR_HIP_CHECK(hipMalloc(&b.ptr, want));
+        PSR_HIP_CHECK(hipMallocAsync(&b.ptr, want, stream_));
         b.bytes = want;
         total_ += want;
         ++allocs_;
@@ -47,11 +46,11 @@ void* Arena::get_keep(const std::string& name, size_t bytes, size_t keepBytes) {
     if (b.bytes < bytes) {
         size_t want = bytes + bytes / 2 + 256;
         void* p = nullptr;
-        PSR_HIP_CHECK(hipMalloc(&p, want));
+        PSR_HIP_CHECK(hipMallocAsync(&p, want, stream_));
         if (b.ptr) {
-            PSR_HIP_CHECK(hipDeviceSynchronize());
-            if (keepBytes) PSR_HIP_CHECK(hipMemcpy(p, b.ptr, std::min(keepBytes, b.bytes), hipMemcpyDeviceToDevice));
-            PSR_HIP_CHECK(hipFree(b.ptr));
+            if (keepBytes)
+                PSR_HIP_CHECK(hipMemcpyAsync(p, b.ptr, std::min(keepBytes, b.bytes), hipMemcpyDeviceToDevice, stream_));
+            PSR_HIP_CHECK(hipFreeAsync(b.ptr, stream_));
             total_ -= b.bytes;
         }
         b.ptr = p;
@@ -64,7 +63,8 @@ void* Arena::get_keep(const std::string& name, size_t bytes, size_t keepBytes) {
 
 void Arena::release() {
     for (auto& kv : blocks_)
-        if (kv.second.ptr) (void)hipFree(kv.second.ptr);
+        if (kv.second.ptr) (void)hipFreeAsync(kv.second.ptr, stream_);
+    if (!blocks_.empty()) (void)hipStreamSynchronize(stream_);
     blocks_.clear();
     total_ = 0;
 }
@@ -196,6 +196,7 @@ int psr_create(const psr_config* cfg, void* hip_stream, psr_ctx** out) {
             PSR_HIP_CHECK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
             c.ownStream = true;
         }
+        c.arena.set_stream(c.stream);
         for (auto& e : c.ev) PSR_HIP_CHECK(hipEventCreate(&e));
         PSR_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&c.hostPinned), psr::kMailWords * sizeof(int),
                                     hipHostMallocMapped | hipHostMallocCoherent));
@@ -277,6 +278,8 @@ int psr_reconstruct_batch(psr_ctx* ctx, const psr_points* clouds, uint32_t count
                     std::lock_guard<std::mutex> lk(ctx->batchMu);
                     v = c.buf<float>(("batch_v_" + std::to_string(i)).c_str(), size_t(3) * m.n_vertices);
                     t = c.buf<int>(("batch_t_" + std::to_string(i)).c_str(), size_t(3) * m.n_triangles);
+                    // allocated in the batch context's stream order; used on the worker's stream
+                    PSR_HIP_CHECK(hipStreamSynchronize(c.stream));
                 }
                 hipStream_t ws = wk->c.stream;
                 PSR_HIP_CHECK(hipMemcpyAsync(v, m.vertices, sizeof(float) * 3 * m.n_vertices,
@@ -475,6 +478,7 @@ void psr_destroy(psr_ctx* ctx) {
     ctx->workers.clear();
     psr::Ctx& c = ctx->c;
     if (c.stream) (void)hipStreamSynchronize(c.stream);
+    if (c.ev[9]) psr::cg_forget_event(c.ev[9]);
     c.arena.release();
     for (auto& e : c.ev)
         if (e) (void)hipEventDestroy(e);

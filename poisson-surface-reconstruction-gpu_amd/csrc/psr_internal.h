@@ -17,9 +17,13 @@ namespace psr {
 
 // Grow-only device arena: one allocation per named buffer, reused across
 // calls; a buffer is reallocated only when a call needs more than it holds.
+// Named device buffers of one context, grown on demand with stream-ordered
+// allocation on the context's stream (no device-wide synchronisation, so
+// contexts on other streams -- batch workers -- keep running).
 class Arena {
   public:
     ~Arena() { release(); }
+    void set_stream(hipStream_t s) { stream_ = s; }
     void* get(const std::string& name, size_t bytes);
     // As get, but a reallocation carries over the first keepBytes of the old contents.
     void* get_keep(const std::string& name, size_t bytes, size_t keepBytes);
@@ -33,6 +37,7 @@ class Arena {
         size_t bytes = 0;
     };
     std::map<std::string, Block> blocks_;
+    hipStream_t stream_ = nullptr;
     size_t total_ = 0;
     int allocs_ = 0;
 };
@@ -211,6 +216,9 @@ void build_tables(Ctx& c);
 void build_octree(Ctx& c, const psr_points& pts);
 void compute_field(Ctx& c);
 void solve_depths(Ctx& c);
+// Drops a context's CG completion event from the batch-worker set (before the
+// event is destroyed).
+void cg_forget_event(hipEvent_t e);
 void points_implicit(Ctx& c);
 void build_geometry(Ctx& c);
 void extract_mesh(Ctx& c);

@@ -4,7 +4,9 @@
 
 #include <algorithm>
 #include <mutex>
+#include <chrono>
 #include <stdexcept>
+#include <thread>
 #include <vector>
 
 namespace psr {
@@ -951,14 +953,24 @@ std::mutex& cg_order_lock() {
     return m;
 }
 
-// Completion of the last cooperative CG launched in this process (one GPU per process).
-hipEvent_t cg_order_event() {
-    static hipEvent_t e = [] {
-        hipEvent_t ev = nullptr;
-        PSR_HIP_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-        return ev;
-    }();
-    return e;
+// Resident-workgroup accounting of the CG launches in flight on the device
+// (one GPU per process; guarded by cg_order_lock): a launch holds its share of
+// the device's resident workgroups, in units of kCgUnits for the whole device,
+// until its completion event (its context's CG end event) has fired.
+struct CgInflight {
+    hipEvent_t done;
+    long long units;
+};
+constexpr long long kCgUnits = 1ll << 20;
+std::vector<CgInflight>& cg_inflight() {
+    static std::vector<CgInflight> v;
+    return v;
+}
+
+void cg_forget_event(hipEvent_t e) {
+    std::lock_guard<std::mutex> order(cg_order_lock());
+    auto& v = cg_inflight();
+    v.erase(std::remove_if(v.begin(), v.end(), [&](const CgInflight& f) { return f.done == e; }), v.end());
 }
 
 void solve_depths(Ctx& c) {
@@ -980,7 +992,10 @@ void solve_depths(Ctx& c) {
     const void* rowKernel = mixed ? reinterpret_cast<const void*>(cg_mixed_row) : reinterpret_cast<const void*>(cg_fp32_row);
     const void* genKernel =
         mixed ? reinterpret_cast<const void*>(cg_mixed_all_depths) : reinterpret_cast<const void*>(cg_all_depths);
-    // workgroup budgets: every workgroup of the cooperative launch must be resident
+    // workgroup budgets: every workgroup of a CG launch must be resident at once
+    // (the groups meet in barriers).  The launch is an ordinary one: the budget,
+    // with the accounting of concurrent launches below, is the residency
+    // guarantee, and the barriers' spin bound turns a broken one into an error.
     if (c.cgMaxWgs <= 0) {
         int perRow = 0, perGen = 0, cus = 0;
         PSR_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perRow, rowKernel, kCgThreads, 0));
@@ -1033,11 +1048,29 @@ void solve_depths(Ctx& c) {
                s);
     assemble_operator<<<div_up(M, kThreads), kThreads, 0, s>>>(L, M, o.neigh, o.encIdx, c.tables.dot, c.tables.d2dot,
                                                               c.tables.res, val, nnz);
-    // Cooperative CG kernels of different contexts (streams) are ordered on the
-    // device: each waits for the previous one to finish, so two of them never
-    // compete for residency while spinning in their group barriers.
-    std::lock_guard<std::mutex> order(cg_order_lock());
-    PSR_HIP_CHECK(hipStreamWaitEvent(s, cg_order_event(), 0));
+    // CG kernels of different contexts (streams) never compete for residency
+    // while spinning in their group barriers: a launch is queued only once the
+    // launches still in flight leave room for all its workgroups beside them
+    // (the host waits otherwise), so concurrent contexts -- the batch workers --
+    // run their solves side by side whenever they fit.
+    const long long cap = rowForm ? c.cgMaxWgsRow : c.cgMaxWgs;
+    const long long units = (wg * kCgUnits + cap - 1) / cap;
+    std::unique_lock<std::mutex> order(cg_order_lock());
+    for (;;) {
+        auto& v = cg_inflight();
+        // this context's earlier launch is ahead of this one in its stream
+        v.erase(std::remove_if(v.begin(), v.end(),
+                               [&](const CgInflight& f) {
+                                   return f.done == c.ev[9] || hipEventQuery(f.done) == hipSuccess;
+                               }),
+                v.end());
+        long long used = 0;
+        for (const CgInflight& f : v) used += f.units;
+        if (used + units <= kCgUnits) break;
+        order.unlock();
+        std::this_thread::sleep_for(std::chrono::microseconds(20));
+        order.lock();
+    }
     PSR_HIP_CHECK(hipEventRecord(c.ev[8], s));
     if (rowForm && mixed) P.spinLimit = 1u << 22;
     if (mixed) {
@@ -1049,7 +1082,7 @@ void solve_depths(Ctx& c) {
             float3* state = c.buf<float3>("cg_state3", size_t(2) * M);
             void* args[] = {&P,     &o.neigh, &val,  &c.fld.div, &c.fld.dx, &rOut,
                             &state, &slots,   &iters, &resid,    &trueRel,  &err};
-            PSR_HIP_CHECK(hipLaunchCooperativeKernel(rowKernel, dim3(wg), dim3(kCgThreads), args, 0, s));
+            PSR_HIP_CHECK(hipLaunchKernel(rowKernel, dim3(wg), dim3(kCgThreads), args, 0, s));
         } else {
             float* rbuf = c.buf<float>("cg_rbuf", size_t(2) * M);
             float* wbuf = c.buf<float>("cg_wbuf", size_t(2) * M);
@@ -1057,12 +1090,12 @@ void solve_depths(Ctx& c) {
             float* pvec = c.buf<float>("cg_pa", M);
             void* args[] = {&P,    &o.neigh, &val,     &c.fld.div, &c.fld.dx, &rOut,  &rbuf,    &wbuf, &sbuf,
                             &pvec, &partial, &partialC, &ctl,      &iters,    &resid, &trueRel, &err};
-            PSR_HIP_CHECK(hipLaunchCooperativeKernel(genKernel, dim3(wg), dim3(kCgThreads), args, 0, s));
+            PSR_HIP_CHECK(hipLaunchKernel(genKernel, dim3(wg), dim3(kCgThreads), args, 0, s));
         }
     } else if (rowForm) {
         float2* state = c.buf<float2>("cg_state2", size_t(2) * M);
         void* args[] = {&P, &o.neigh, &val, &c.fld.div, &c.fld.dx, &state, &partial, &ctl, &iters, &resid, &err};
-        PSR_HIP_CHECK(hipLaunchCooperativeKernel(rowKernel, dim3(wg), dim3(kCgThreads), args, 0, s));
+        PSR_HIP_CHECK(hipLaunchKernel(rowKernel, dim3(wg), dim3(kCgThreads), args, 0, s));
     } else {
         float* r = c.buf<float>("cg_r", M);
         float* pA = c.buf<float>("cg_pa", M);
@@ -1070,11 +1103,11 @@ void solve_depths(Ctx& c) {
         float* Ap = c.buf<float>("cg_ap", M);
         void* args[] = {&P, &o.neigh, &val, &c.fld.div, &c.fld.dx, &r, &pA, &pB, &Ap, &partial, &ctl, &iters, &resid,
                         &err};
-        PSR_HIP_CHECK(hipLaunchCooperativeKernel(genKernel, dim3(wg), dim3(kCgThreads), args, 0, s));
+        PSR_HIP_CHECK(hipLaunchKernel(genKernel, dim3(wg), dim3(kCgThreads), args, 0, s));
     }
     c.stats.cg_row_form = rowForm ? 1 : 0;
     PSR_HIP_CHECK(hipEventRecord(c.ev[9], s));
-    PSR_HIP_CHECK(hipEventRecord(cg_order_event(), s));
+    cg_inflight().push_back(CgInflight{c.ev[9], units});
     PSR_HIP_CHECK(hipGetLastError());
 }
 
