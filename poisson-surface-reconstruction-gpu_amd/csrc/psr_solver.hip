@@ -26,19 +26,24 @@ __global__ void __launch_bounds__(kThreads) assemble_operator(Levels L, int M, c
                                                               const double* __restrict__ dot,
                                                               const double* __restrict__ d2dot, int res,
                                                               float* __restrict__ val, int* __restrict__ nnz) {
+    // three threads per row, each the 9 neighbours of one x offset (thread
+    // t = j * M + o: neighbours 9 j .. 9 j + 8 of row o), so that three times as
+    // many waves hide the loads; a thread's table rows stay in its cache
     __shared__ int cnt[PSR_MAX_DEPTH + 1];
     if (threadIdx.x <= PSR_MAX_DEPTH) cnt[threadIdx.x] = 0;
     __syncthreads();
-    const int o = blockIdx.x * kThreads + threadIdx.x;
-    if (o < M) {
-        const int D = L.D;
-        const int3 f1 = decode_func_index(enc[o], D);
-        int c = 0;
-        for (int i = 0; i < 27; ++i) {
-            int nb = neigh[i * M + o];
+    const int t = blockIdx.x * kThreads + threadIdx.x;
+    const bool valid = t < 3 * M;
+    int d = -1, c = 0;
+    if (valid) {
+        const int j = t / M, o = t - j * M;
+        const int3 f1 = decode_func_index(enc[o], L.D);
+#pragma unroll 3
+        for (int i = 9 * j; i < 9 * j + 9; ++i) {
+            const int nb = neigh[i * M + o];
             float v = 0.f;
             if (nb != -1) {
-                int3 f2 = decode_func_index(enc[nb], D);
+                const int3 f2 = decode_func_index(enc[nb], L.D);
                 size_t sx = size_t(f1.x) * res + f2.x, sy = size_t(f1.y) * res + f2.y, sz = size_t(f1.z) * res + f2.z;
                 double e = dot[sx] * dot[sy] * dot[sz] * (d2dot[sx] + d2dot[sy] + d2dot[sz]);
                 if (fabs(e) > double(1e-6f)) v = float(e);
@@ -46,7 +51,19 @@ __global__ void __launch_bounds__(kThreads) assemble_operator(Levels L, int M, c
             val[i * M + o] = v;
             c += v != 0.f;
         }
-        atomicAdd(&cnt[depth_of(L, o)], c);
+        d = depth_of(L, o);
+    }
+    // stored entries per depth: one LDS add per depth present in the wave
+    unsigned long long active = __ballot(valid);
+    while (active) {
+        const int leader = __builtin_ctzll(active);
+        const int dl = __shfl(d, leader, PSR_WAVE);
+        const unsigned long long same = __ballot(d == dl) & active;
+        int s = d == dl ? c : 0;
+#pragma unroll
+        for (int off = PSR_WAVE / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, PSR_WAVE);
+        if (__lane_id() == leader) atomicAdd(&cnt[dl], s);
+        active &= ~same;
     }
     __syncthreads();
     if (threadIdx.x <= PSR_MAX_DEPTH && cnt[threadIdx.x]) atomicAdd(&nnz[threadIdx.x], cnt[threadIdx.x]);
@@ -1046,7 +1063,7 @@ void solve_depths(Ctx& c) {
                 {res->cgIters, sizeof(res->cgIters) / 4 + sizeof(res->cgNnz) / 4 + 1},  // iterations, nnz, error
                 {trueRel, 2 * (PSR_MAX_DEPTH + 1)}},
                s);
-    assemble_operator<<<div_up(M, kThreads), kThreads, 0, s>>>(L, M, o.neigh, o.encIdx, c.tables.dot, c.tables.d2dot,
+    assemble_operator<<<div_up(size_t(3) * M, kThreads), kThreads, 0, s>>>(L, M, o.neigh, o.encIdx, c.tables.dot, c.tables.d2dot,
                                                               c.tables.res, val, nnz);
     // CG kernels of different contexts (streams) never compete for residency
     // while spinning in their group barriers: a launch is queued only once the
