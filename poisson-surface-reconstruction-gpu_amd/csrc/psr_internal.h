@@ -153,7 +153,8 @@ struct Ctx {
     DevResults* mailResults = nullptr;  // device view of hostResults: written once the results are final
     double cgMsSum = 0.0;               // CG kernel time summed over completed runs
     uint32_t runs = 0;                  // completed runs
-    void* hostStage = nullptr;          // pinned staging for small uploads
+    void* hostStage = nullptr;          // pinned, mapped staging for small uploads
+    void* stageDev = nullptr;           // device view of hostStage
     size_t hostStageBytes = 0;
     int cgMaxWgs = 0;            // resident workgroups of the CG kernels with several rows per thread
     int cgMaxWgsRow = 0;         // resident workgroups of the one-row-per-thread CG kernels
@@ -175,13 +176,15 @@ struct Ctx {
         }
         PSR_HIP_CHECK(hipStreamSynchronize(stream));
     }
-    // Pinned staging of at least `bytes`; only reused after a stream synchronisation.
+    // Pinned, mapped, coherent staging of at least `bytes` (device view in stageDev); only
+    // reused after a stream synchronisation.
     void* stage(size_t bytes) {
         if (bytes > hostStageBytes) {
             if (hostStage) PSR_HIP_CHECK(hipHostFree(hostStage));
-            hostStage = nullptr;
+            hostStage = stageDev = nullptr;
             hostStageBytes = 0;
-            PSR_HIP_CHECK(hipHostMalloc(&hostStage, bytes + bytes / 2 + 4096));
+            PSR_HIP_CHECK(hipHostMalloc(&hostStage, bytes + bytes / 2 + 4096, hipHostMallocMapped | hipHostMallocCoherent));
+            PSR_HIP_CHECK(hipHostGetDevicePointer(&stageDev, hostStage, 0));
             hostStageBytes = bytes + bytes / 2 + 4096;
         }
         return hostStage;

@@ -1474,6 +1474,14 @@ __global__ void __launch_bounds__(kThreads) keep_compact(const float* __restrict
     out[3 * size_t(a) + 2] = z;
 }
 
+// Copy of 16-byte words from mapped host memory, on the compute queue (a copy
+// engine would add a hand-over between queues before the kernels that read it).
+__global__ void __launch_bounds__(kThreads) upload_words(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                         size_t n) {
+    for (size_t i = size_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += size_t(gridDim.x) * kThreads)
+        dst[i] = src[i];
+}
+
 __global__ void fill_root_pos(const int* __restrict__ roots, int n, int* __restrict__ rootPos) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) rootPos[roots[i]] = i;
@@ -1639,7 +1647,9 @@ void extract_mesh(Ctx& c) {
             std::memcpy(hs + bSets, tiles.data(), sizeof(Tile) * nTiles);
             if (nSmall) std::memcpy(hs + bSets + bTiles, smallTiles.data(), sizeof(Tile) * nSmall);
             char* dAll = c.buf<char>("sub_upload", bAll);
-            PSR_HIP_CHECK(hipMemcpyAsync(dAll, hs, bAll, hipMemcpyHostToDevice, st));
+            const size_t words = bAll / sizeof(uint4);
+            upload_words<<<unsigned(std::min<size_t>(div_up(words, kThreads), 256)), kThreads, 0, st>>>(
+                static_cast<const uint4*>(c.stageDev), reinterpret_cast<uint4*>(dAll), words);
             SetDesc* dsets = reinterpret_cast<SetDesc*>(dAll);
             Tile* dtiles = reinterpret_cast<Tile*>(dAll + bSets);
             Tile* dsmall = reinterpret_cast<Tile*>(dAll + bSets + bTiles);
