@@ -99,8 +99,9 @@ struct Geometry {
     int* nodeVert = nullptr;  // [8][M], 1-based
     int* nodeEdge = nullptr;  // [12][nD] (depth-D nodes, level-local), 1-based
     int* nodeFace = nullptr;  // [6][M], 1-based
+    int2* edgeOf = nullptr;   // per depth-D node: first owned edge, owned-edge mask
     int* counts = nullptr;    // device: V, E, F
-    int V = 0, E = 0, F = 0;
+    int V = 0, E = 0, F = 0;  // host copies, known once the mesh stage has waited
     float* vvalue = nullptr;
 };
 
@@ -130,7 +131,9 @@ constexpr int kMailWords = 64;
 constexpr int kMailNorm = 32;                // octree: 4 normalisation parameters (float bits)
 constexpr int kMailLevelStart = 4;           // depth-D MC: roots before each level
 constexpr int kMailQ1 = PSR_MAX_DEPTH + 8;   // depth-D MC: the level-local child 0 rule fired
-static_assert(kMailQ1 < kMailNorm, "mailbox slots overlap");
+constexpr int kMailGeo = 40;                 // geometry: vertex, edge and face counts
+static_assert(kMailQ1 < kMailNorm && kMailNorm + 4 <= kMailGeo && kMailGeo + 3 <= kMailWords,
+              "mailbox slots overlap");
 
 struct Ctx {
     psr_config cfg{};

@@ -63,6 +63,7 @@ struct GeoOut {
     int* fKind;
     int* fHasParent;
     int* nodeFace;
+    int2* edgeOf;
 };
 
 // Ownership on the regular neighbour grid (an octree in which the
@@ -193,6 +194,7 @@ __global__ void __launch_bounds__(kThreads) geometry_emit(NodeView v, int offD, 
     int av = ticket_prefix<kThreads>(tv, tile, __popc(vm));
     int ae = ticket_prefix<kThreads>(te, tile, __popc(em));
     int af = ticket_prefix<kThreads>(tf, tile, __popc(fm));
+    if (o >= offD && o < v.M) out.edgeOf[o - offD] = make_int2(ae, int(em));
     if (!(vm | em | fm)) return;
     const float half = 1.0f / float(1 << (depth + 1)), W = 1.0f / float(1 << depth), W2 = W * W;
     if (done) {
@@ -317,7 +319,7 @@ __global__ void __launch_bounds__(kValueThreads) vertex_values(NodeView v, const
                                                           int* __restrict__ mail, const int* __restrict__ results,
                                                           int* __restrict__ resultsHost, int resultWords) {
     int t = blockIdx.x * kValueThreads + threadIdx.x;
-    if (t < 3) mail[t] = count[t];  // vertex, edge and face counts for the host
+    if (t < 3) mail[kMailGeo + t] = count[t];  // vertex, edge and face counts for the host
     // the run's device results (CG statistics, iso value) are final here
     if (t < resultWords) resultsHost[t] = results[t];
     if (t >= *count) return;
@@ -350,36 +352,45 @@ __global__ void __launch_bounds__(kValueThreads) vertex_values(NodeView v, const
 }
 
 // --- depth-D marching cubes (ComputeTriangleIndices.cu:401-526) ------------
-__global__ void __launch_bounds__(kThreads) edge_crossings(int E, int M, const int* __restrict__ eOwner,
-                                                           const int* __restrict__ eKind,
+// The owned edges of each depth-D node that cross the surface (their corner
+// values differ in sign), compacted in edge order -- (node, edge kind), the
+// order of the edge records -- with their crossing points: per edge record a
+// crossing flag and its crossing index.
+__global__ void __launch_bounds__(kThreads) edge_crossings(int offD, int nD, int M, const int2* __restrict__ edgeOf,
                                                            const int* __restrict__ nodeVert,
                                                            const float* __restrict__ vv,
                                                            const float4* __restrict__ vPos, ScanTicket t,
                                                            int* __restrict__ flag, int* __restrict__ addr,
                                                            float* __restrict__ out) {
     const int tile = ticket_tile(t);
-    const int e = tile * kThreads + threadIdx.x;
-    int v1 = -1, v2 = -1, kind = 0, cross = 0;
-    if (e < E) {
-        const int owner = eOwner[e];
-        kind = eKind[e];
-        v1 = nodeVert[kEdgeVertex[kind][0] * M + owner] - 1;
-        v2 = nodeVert[kEdgeVertex[kind][1] * M + owner] - 1;
+    const int n = tile * kThreads + threadIdx.x, o = offD + n;
+    const int2 eo = n < nD ? edgeOf[n] : make_int2(0, 0);
+    unsigned cross = 0;
+    for (int k = 0; k < 12; ++k) {
+        if (!((eo.y >> k) & 1)) continue;
+        const int v1 = nodeVert[kEdgeVertex[k][0] * M + o] - 1, v2 = nodeVert[kEdgeVertex[k][1] * M + o] - 1;
         // a corner slot left unassigned (the reference's level-local child 0
         // parent rule breaks neighbour tables) makes the edge absent, not a crossing
-        cross = (v1 >= 0 && v2 >= 0 && vv[v1] * vv[v2] <= 0.f) ? 1 : 0;
+        if (v1 >= 0 && v2 >= 0 && vv[v1] * vv[v2] <= 0.f) cross |= 1u << k;
     }
-    const int k = ticket_prefix<kThreads>(t, tile, cross);
-    if (e >= E) return;
-    flag[e] = cross;
-    addr[e] = k;
-    if (!cross) return;
-    const float4 a = vPos[v1], b = vPos[v2];
-    const float3 p =
-        interpolate_point(make_float3(a.x, a.y, a.z), make_float3(b.x, b.y, b.z), kind >> 2, vv[v1], vv[v2]);
-    out[3 * k] = p.x;
-    out[3 * k + 1] = p.y;
-    out[3 * k + 2] = p.z;
+    int c = ticket_prefix<kThreads>(t, tile, __popc(cross));
+    int a = eo.x;
+    for (int k = 0; k < 12; ++k) {
+        if (!((eo.y >> k) & 1)) continue;
+        const bool x = (cross >> k) & 1u;
+        flag[a] = x ? 1 : 0;
+        addr[a] = c;
+        ++a;
+        if (!x) continue;
+        const int v1 = nodeVert[kEdgeVertex[k][0] * M + o] - 1, v2 = nodeVert[kEdgeVertex[k][1] * M + o] - 1;
+        const float4 p1 = vPos[v1], p2 = vPos[v2];
+        const float3 p =
+            interpolate_point(make_float3(p1.x, p1.y, p1.z), make_float3(p2.x, p2.y, p2.z), k >> 2, vv[v1], vv[v2]);
+        out[3 * c] = p.x;
+        out[3 * c + 1] = p.y;
+        out[3 * c + 2] = p.z;
+        ++c;
+    }
 }
 
 // A cube emits triangles only when all its corners and edges are present and
@@ -531,8 +542,9 @@ void build_geometry(Ctx& c) {
     te.total = g.counts + 1;
     tf.status += 2 * size_t(tilesM);
     tf.total = g.counts + 2;
-    const GeoOut out{g.vtxOwner,  g.vtxKind,  g.vtxPos,   g.nodeVert,      g.edgeOwner, g.edgeKind,
-                     g.nodeEdge,  g.faceOwner, g.faceKind, g.faceHasParent, g.nodeFace};
+    g.edgeOf = c.buf<int2>("edge_of", nD);
+    const GeoOut out{g.vtxOwner, g.vtxKind,  g.vtxPos,   g.nodeVert,      g.edgeOwner, g.edgeKind,
+                     g.nodeEdge, g.faceOwner, g.faceKind, g.faceHasParent, g.nodeFace,  g.edgeOf};
     geometry_emit<<<tilesM, kThreads, 0, s>>>(v, offD, nD, o.parent, o.q1Fired, tv, te, tf, out);
     // vertex implicit values
     float* isoDev = &c.results()->iso;
@@ -543,10 +555,8 @@ void build_geometry(Ctx& c) {
                                                                        reinterpret_cast<const int*>(c.results()),
                                                                        reinterpret_cast<int*>(c.mailResults),
                                                                        int(sizeof(DevResults) / sizeof(int)));
-    c.wait();
-    g.V = c.hostPinned[0];
-    g.E = c.hostPinned[1];
-    g.F = c.hostPinned[2];
+    // no wait: the stages that follow size their buffers by the bounds 8M, 12 n_D
+    // and 6M, and the counts reach the host with the marching cubes' wait
     PSR_HIP_CHECK(hipGetLastError());
 }
 
@@ -556,20 +566,19 @@ void depth_mc(Ctx& c, MainMc& mm, float* rawV, int* rawT) {
     const Levels& L = o.L;
     const int M = o.M, D = L.D, nD = L.count[D], offD = L.base[D];
     hipStream_t s = c.stream;
-    const int E = g.E, F = g.F;
-    mm.hasInter = c.buf<int>("mc_hasinter", F);
-    zero_spans({{mm.hasInter, size_t(F)}}, s);
+    const size_t Emax = size_t(12) * nD, Fmax = size_t(6) * M;  // bounds of the edge and face counts
+    mm.hasInter = c.buf<int>("mc_hasinter", Fmax);
+    zero_spans({{mm.hasInter, Fmax}}, s);
     // crossing, triangle and root totals and the level starts go to the mailbox;
     // launches that may be skipped leave zeros
     for (int k = 0; k < kMailLevelStart + PSR_MAX_DEPTH + 1; ++k) c.hostPinned[k] = 0;
-    int* cflag = c.buf<int>("mc_cross_flag", E);
-    mm.vexAddr = c.buf<int>("mc_vex_addr", E);
+    int* cflag = c.buf<int>("mc_cross_flag", Emax);
+    mm.vexAddr = c.buf<int>("mc_vex_addr", Emax);
     mm.cube = c.buf<int>("mc_cube", nD);
-    const int tilesE = div_up(E, kThreads), tilesD = div_up(nD, kThreads), tilesR = div_up(offD, kThreads);
-    ScanState& scan = c.scan_for_tiles(size_t(std::max(tilesE, std::max(tilesD, tilesR))));
-    if (E > 0)
-        edge_crossings<<<tilesE, kThreads, 0, s>>>(E, M, g.edgeOwner, g.edgeKind, g.nodeVert, g.vvalue, g.vtxPos,
-                                                   scan_ticket(scan, tilesE, c.mail + 0), cflag, mm.vexAddr, rawV);
+    const int tilesD = div_up(nD, kThreads), tilesR = div_up(offD, kThreads);
+    ScanState& scan = c.scan_for_tiles(size_t(std::max(tilesD, tilesR)));
+    edge_crossings<<<tilesD, kThreads, 0, s>>>(offD, nD, M, g.edgeOf, g.nodeVert, g.vvalue, g.vtxPos,
+                                               scan_ticket(scan, tilesD, c.mail + 0), cflag, mm.vexAddr, rawV);
     cube_triangles<<<tilesD, kThreads, 0, s>>>(offD, nD, M, g.nodeVert, g.nodeEdge, cflag, g.vvalue, mm.vexAddr,
                                                g.nodeFace, o.parent, g.faceHasParent, o.q1Fired,
                                                scan_ticket(scan, tilesD, c.mail + 1), mm.cube, rawT, mm.hasInter,
@@ -582,6 +591,9 @@ void depth_mc(Ctx& c, MainMc& mm, float* rawV, int* rawT) {
                                                 c.mail + kMailLevelStart);
     c.wait();
     mm.q1Fired = c.hostPinned[kMailQ1] != 0;
+    g.V = c.hostPinned[kMailGeo];
+    g.E = c.hostPinned[kMailGeo + 1];
+    g.F = c.hostPinned[kMailGeo + 2];
     mm.nCross = c.hostPinned[0];
     mm.nTri = c.hostPinned[1];
     mm.nRoots = c.hostPinned[2];

@@ -1513,13 +1513,12 @@ constexpr size_t kChunkCells = size_t(1) << 24;
 
 void extract_mesh(Ctx& c) {
     Octree& o = c.oct;
-    Geometry& geo = c.geo;
     const Levels& L = o.L;
     const int M = o.M, D = L.D, nD = L.count[D], offD = L.base[D];
     hipStream_t st = c.stream;
     // ---- depth-D marching cubes: the first output batch; its triangles are
     // written in place (raw crossing ids: batch start 0, nothing kept before it)
-    float* mainV = c.buf<float>("raw_main_v", size_t(3) * std::max(geo.E, 1));
+    float* mainV = c.buf<float>("raw_main_v", size_t(36) * nD);  // at most one crossing per depth-D edge
     int* outT = c.buf_keep<int>("mesh_triangles", size_t(15) * nD, 0);
     MainMc mm;
     depth_mc(c, mm, mainV, outT);
