@@ -262,14 +262,15 @@ int psr_reconstruct_batch(psr_ctx* ctx, const psr_points* clouds, uint32_t count
             record(PSR_ERR_HIP, 0, "hipSetDevice");
             return;
         }
+        // every cloud is attempted; the first failure is the call's status
         for (;;) {
             const uint32_t i = next.fetch_add(1);
-            if (i >= count || status.load() != PSR_OK) return;
+            if (i >= count) return;
             psr_mesh m{};
             int rc = psr_reconstruct(wk, &clouds[i], &m);
             if (rc != PSR_OK) {
                 record(rc, i, wk->lastError);
-                return;
+                continue;
             }
             try {
                 float* v;
@@ -291,7 +292,6 @@ int psr_reconstruct_batch(psr_ctx* ctx, const psr_points* clouds, uint32_t count
                 ctx->batchStats[i] = wk->c.stats;
             } catch (const std::exception& e) {
                 record(PSR_ERR_HIP, i, e.what());
-                return;
             }
         }
     };

@@ -326,6 +326,32 @@ def test_batch_matches_single_reconstructions(pkg, torch):
     batch.close()
 
 
+def test_batch_with_a_failing_cloud(pkg, torch):
+    """A cloud that fails (fewer than 8 points) stops no other cloud: the call
+    reports that cloud, and every other cloud's mesh equals its single-cloud
+    reconstruction."""
+    s = pkg.synth
+    good = [s.sphere(n=5000), s.torus(n=8000), s.sphere(n=7000), s.torus(n=6000)]
+    bad = (np.ascontiguousarray(good[0][0][:4]), np.ascontiguousarray(good[0][1][:4]))
+    clouds = [good[0], bad, good[1], good[2], good[3]]
+    dev = [(torch.from_numpy(x).cuda(), torch.from_numpy(n).cuda()) for x, n in clouds]
+    batch = pkg.PoissonReconstruction(depth=6, batch_workers=2)
+    with pytest.raises(pkg.binding.PsrError, match="cloud 1"):
+        batch.ReconstructBatch(dev)
+    single = pkg.PoissonReconstruction(depth=6)
+    for i in (0, 2, 3, 4):
+        single.SolvePoissionReconstructionMesh(*dev[i])
+        v, t = batch.mesh_to_host(batch.batch[i])
+        np.testing.assert_array_equal(v, single.GetRebuildMeshVertices())
+        np.testing.assert_array_equal(t, single.GetRebuildMeshTriangleIndices())
+    # the context stays usable
+    meshes = batch.ReconstructBatch([dev[0], dev[2], dev[3]])
+    single.SolvePoissionReconstructionMesh(*dev[3])
+    np.testing.assert_array_equal(batch.mesh_to_host(meshes[2])[1], single.GetRebuildMeshTriangleIndices())
+    single.close()
+    batch.close()
+
+
 def test_level_local_child0_quirk_parity(pkg, orc, torch):
     """Quirk Q1 (BuildOctree.cu:384-389, 412) on a cloud that fires it: the node
     arrays and neighbour tables reproduce it bit-exactly, and the mesh stages,
