@@ -377,34 +377,57 @@ __global__ void __launch_bounds__(kThreads) zero_int(int* __restrict__ a, int n)
 // derived per axis: the child-level offset c = s_a + d_a lies in parent cell
 // floor(c / 2) at child bit c - 2 floor(c / 2); only the 8 parent neighbours
 // with per-axis offsets in {s_a - 1, s_a} are touched, each read once.
-__global__ void __launch_bounds__(kThreads) neighbors_level(int begin, int count, int D, int depth, int M,
-                                                            const int* __restrict__ key,
+// Neighbour i of a node with son key s, from the child bases of the 8 parent
+// neighbours u with per-axis offsets {s_a - 1, s_a} (cb[u], -1 if none).
+__device__ __forceinline__ int child_neighbour(const int (&cb)[8], int s, int i) {
+    const int sx = (s >> 2) & 1, sy = (s >> 1) & 1, sz = s & 1;
+    const int cx = sx + i / 9 - 1, cy = sy + (i / 3) % 3 - 1, cz = sz + i % 3 - 1;
+    const int fx = cx >> 1, fy = cy >> 1, fz = cz >> 1;  // floor(c / 2) for c in [-1, 2]
+    const int u = ((fx - sx + 1) << 2) | ((fy - sy + 1) << 1) | (fz - sz + 1);
+    const int b = cb[u];
+    return b < 0 ? -1 : b + (((cx - 2 * fx) << 2) | ((cy - 2 * fy) << 1) | (cz - 2 * fz));
+}
+// Index of parent neighbour u of a node with son key s.
+__device__ __forceinline__ int parent_slot(int s, int u) {
+    const int px = ((s >> 2) & 1) - 1 + ((u >> 2) & 1), py = ((s >> 1) & 1) - 1 + ((u >> 1) & 1),
+              pz = (s & 1) - 1 + (u & 1);
+    return 9 * (px + 1) + 3 * (py + 1) + (pz + 1);
+}
+
+// Two depths per launch: the first count0 threads take depth `depth` from
+// their parents' final neighbours; the rest take depth + 1, whose parents'
+// neighbours this launch is still writing, so the 8 each one needs are formed
+// again from the grandparent's by the same recurrence (count1 = 0: one depth).
+__global__ void __launch_bounds__(kThreads) neighbors_level(int begin, int count0, int count1, int D, int depth,
+                                                            int M, const int* __restrict__ key,
                                                             const int* __restrict__ parent,
                                                             const int* __restrict__ childBase,
                                                             int* __restrict__ neigh) {
     const int t = blockIdx.x * kThreads + threadIdx.x;
-    if (t >= count) return;
+    if (t >= count0 + count1) return;
     const int o = begin + t;
-    const int s = (key[o] >> (3 * (D - depth))) & 7;
+    const int d = t < count0 ? depth : depth + 1;
+    const int s = (key[o] >> (3 * (D - d))) & 7;
     const int p = parent[o];
-    const int sx = (s >> 2) & 1, sy = (s >> 1) & 1, sz = s & 1;
+    int np[8];  // the parent's neighbours at slots parent_slot(s, u)
+    if (t < count0) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) np[u] = neigh[parent_slot(s, u) * M + p];
+    } else {
+        const int sp = (key[p] >> (3 * (D - depth))) & 7, gp = parent[p];
+        int cbp[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) cbp[u] = neigh[parent_slot(sp, u) * M + gp];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) cbp[u] = cbp[u] == -1 ? -1 : childBase[cbp[u]];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) np[u] = child_neighbour(cbp, sp, parent_slot(s, u));
+    }
     int cb[8];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-        const int px = sx - 1 + ((u >> 2) & 1), py = sy - 1 + ((u >> 1) & 1), pz = sz - 1 + (u & 1);
-        const int np = neigh[(9 * (px + 1) + 3 * (py + 1) + (pz + 1)) * M + p];
-        cb[u] = np;
-    }
+    for (int u = 0; u < 8; ++u) cb[u] = np[u] == -1 ? -1 : childBase[np[u]];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) cb[u] = cb[u] == -1 ? -1 : childBase[cb[u]];
-#pragma unroll
-    for (int i = 0; i < 27; ++i) {
-        const int cx = sx + i / 9 - 1, cy = sy + (i / 3) % 3 - 1, cz = sz + i % 3 - 1;
-        const int fx = cx >> 1, fy = cy >> 1, fz = cz >> 1;  // floor(c / 2) for c in [-1, 2]
-        const int u = ((fx - sx + 1) << 2) | ((fy - sy + 1) << 1) | (fz - sz + 1);
-        const int b = cb[u];
-        neigh[i * M + o] = b < 0 ? -1 : b + (((cx - 2 * fx) << 2) | ((cy - 2 * fy) << 1) | (cz - 2 * fz));
-    }
+    for (int i = 0; i < 27; ++i) neigh[i * M + o] = child_neighbour(cb, s, i);
 }
 
 __global__ void root_neighbors(int* __restrict__ neigh, int M) {
@@ -511,9 +534,11 @@ void build_octree(Ctx& c, const psr_points& pts) {
                                                                         o.pnum, o.parent, o.childBase, o.didx,
                                                                         o.dnum);
     root_neighbors<<<1, 64, 0, s>>>(o.neigh, M);
-    for (int d = 1; d <= D; ++d)
-        neighbors_level<<<div_up(L.count[d], kThreads), kThreads, 0, s>>>(L.base[d], L.count[d], D, d, M, o.key,
-                                                                           o.parent, o.childBase, o.neigh);
+    for (int d = 1; d <= D; d += 2) {
+        const int c1 = d + 1 <= D ? L.count[d + 1] : 0;
+        neighbors_level<<<div_up(L.count[d] + c1, kThreads), kThreads, 0, s>>>(L.base[d], L.count[d], c1, D, d, M,
+                                                                                o.key, o.parent, o.childBase, o.neigh);
+    }
     o.depth = c.buf<int>("node_depth", M);
     o.basis = c.buf<float4>("node_basis", M);
     encode_function_index<<<div_up(M, kThreads), kThreads, 0, s>>>(L, M, o.key, o.encIdx, o.depth, o.basis);
