@@ -99,6 +99,7 @@ struct Geometry {
     int* nodeVert = nullptr;  // [8][M], 1-based
     int* nodeEdge = nullptr;  // [12][nD] (depth-D nodes, level-local), 1-based
     int* nodeFace = nullptr;  // [6][M], 1-based
+    int* faceMarks = nullptr; // per face: an emitted triangle's edge lies on it (6M, the face bound)
     int2* edgeOf = nullptr;   // per depth-D node: first owned edge, owned-edge mask
     int* counts = nullptr;    // device: V, E, F
     int V = 0, E = 0, F = 0;  // host copies, known once the mesh stage has waited

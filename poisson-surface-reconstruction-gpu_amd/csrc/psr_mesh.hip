@@ -480,7 +480,8 @@ __global__ void __launch_bounds__(kThreads) root_select(int count, int M, Levels
                                                         const int* __restrict__ childBase,
                                                         const float* __restrict__ vv,
                                                         const int* __restrict__ hasInter, ScanTicket t,
-                                                        int* __restrict__ roots, int* __restrict__ levelStart) {
+                                                        int* __restrict__ roots, int* __restrict__ rootPos,
+                                                        int* __restrict__ levelStart) {
     const int tile = ticket_tile(t);
     const int o = tile * kThreads + threadIdx.x;
     int flag = 0;
@@ -509,6 +510,7 @@ __global__ void __launch_bounds__(kThreads) root_select(int count, int M, Levels
     for (int d = 0; d < L.D; ++d)
         if (L.base[d] == o) levelStart[d] = a;
     if (flag) roots[a] = o;
+    rootPos[o] = flag ? a : -1;  // position in roots, -1 for other nodes
 }
 
 }  // namespace
@@ -524,10 +526,7 @@ void build_geometry(Ctx& c) {
     g.vtxOwner = c.buf<int>("vtx_owner", size_t(8) * M);
     g.vtxKind = c.buf<int>("vtx_kind", size_t(8) * M);
     g.vtxPos = c.buf<float4>("vtx_pos", size_t(8) * M);
-    g.nodeVert = c.buf<int>("node_vert", size_t(8) * M);
-    g.nodeFace = c.buf<int>("node_face", size_t(6) * M);
-    g.nodeEdge = c.buf<int>("node_edge", size_t(12) * nD);
-    zero_spans({{g.nodeVert, size_t(8) * M}, {g.nodeFace, size_t(6) * M}, {g.nodeEdge, size_t(12) * nD}}, s);
+    // nodeVert, nodeFace and nodeEdge were cleared with the neighbour columns
     // vertices, edges, faces: one launch, three scans over the same tiles
     g.edgeOwner = c.buf<int>("edge_owner", size_t(12) * nD);
     g.edgeKind = c.buf<int>("edge_kind", size_t(12) * nD);
@@ -566,9 +565,8 @@ void depth_mc(Ctx& c, MainMc& mm, float* rawV, int* rawT) {
     const Levels& L = o.L;
     const int M = o.M, D = L.D, nD = L.count[D], offD = L.base[D];
     hipStream_t s = c.stream;
-    const size_t Emax = size_t(12) * nD, Fmax = size_t(6) * M;  // bounds of the edge and face counts
-    mm.hasInter = c.buf<int>("mc_hasinter", Fmax);
-    zero_spans({{mm.hasInter, Fmax}}, s);
+    const size_t Emax = size_t(12) * nD;  // bound of the edge count
+    mm.hasInter = g.faceMarks;            // cleared with the neighbour columns
     // crossing, triangle and root totals and the level starts go to the mailbox;
     // launches that may be skipped leave zeros
     for (int k = 0; k < kMailLevelStart + PSR_MAX_DEPTH + 1; ++k) c.hostPinned[k] = 0;
@@ -585,10 +583,11 @@ void depth_mc(Ctx& c, MainMc& mm, float* rawV, int* rawT) {
                                                c.mail);
     // subdivision roots: leaves above depth D with a sign change or a marked face
     mm.roots = c.buf<int>("roots", offD);
+    mm.rootPos = c.buf<int>("root_pos", offD);
     if (offD > 0)
         root_select<<<tilesR, kThreads, 0, s>>>(offD, M, L, o.neigh, g.nodeVert, g.nodeFace, o.childBase, g.vvalue,
                                                 mm.hasInter, scan_ticket(scan, tilesR, c.mail + 2), mm.roots,
-                                                c.mail + kMailLevelStart);
+                                                mm.rootPos, c.mail + kMailLevelStart);
     c.wait();
     mm.q1Fired = c.hostPinned[kMailQ1] != 0;
     g.V = c.hostPinned[kMailGeo];

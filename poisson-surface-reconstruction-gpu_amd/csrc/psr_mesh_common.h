@@ -99,6 +99,7 @@ struct MainMc {
     int* cube = nullptr;      // cube category per depth-D node (exported)
     int* hasInter = nullptr;  // surface-edge intersection flag per face
     int* roots = nullptr;     // subdivision roots, ascending node index
+    int* rootPos = nullptr;   // per node above depth D: position in roots, or -1
     int nCross = 0, nTri = 0, nRoots = 0;
     int rootsPerDepth[PSR_MAX_DEPTH + 1] = {0};
     bool q1Fired = false;  // the level-local child 0 rule changed a parent somewhere

@@ -891,11 +891,15 @@ __global__ void __launch_bounds__(kCgThreads) cg_mixed_all_depths(
 // --- implicit function at points (LaplacianSolver.cu:85-118, 269-305) ------
 // Neighbour columns: basis parameters and d_x of neighbour i of every node, so
 // the 27 terms of a node need no dependent index loads.
+// Also clears the nZero words at zero (the mesh stage's node tables and face
+// marks), which saves that stage a launch of its own.
 __global__ void __launch_bounds__(kThreads) neighbour_columns(int M, const int* __restrict__ neigh,
                                                               const float4* __restrict__ basis,
                                                               const float* __restrict__ dx,
-                                                              float4* __restrict__ nbrCol) {
+                                                              float4* __restrict__ nbrCol,
+                                                              unsigned* __restrict__ zero, size_t nZero) {
     const size_t t = size_t(blockIdx.x) * kThreads + threadIdx.x;
+    for (size_t i = t; i < nZero; i += size_t(gridDim.x) * kThreads) zero[i] = 0u;
     if (t >= size_t(27) * M) return;
     const int nb = neigh[t];
     if (nb < 0) {
@@ -1138,8 +1142,16 @@ void points_implicit(Ctx& c) {
     c.fld.isoSum = c.buf<double>("iso_sum", 1);
     float* isoDev = &c.results()->iso;
     c.fld.nbrCol = c.buf<float4>("nbr_col", size_t(27) * o.M);
-    neighbour_columns<<<div_up(size_t(27) * o.M, kThreads), kThreads, 0, s>>>(o.M, o.neigh, o.basis, c.fld.dx,
-                                                                             c.fld.nbrCol);
+    // the mesh stage's node tables and face marks, cleared with the columns:
+    // nodeVert [8][M], nodeFace [6][M], nodeEdge [12][n_D], face marks (6M, the face bound)
+    const size_t M = size_t(o.M), nD = size_t(L.count[L.D]), nZero = 8 * M + 6 * M + 12 * nD + 6 * M;
+    int* zeroed = c.buf<int>("mesh_zeroed", nZero);
+    c.geo.nodeVert = zeroed;
+    c.geo.nodeFace = zeroed + 8 * M;
+    c.geo.nodeEdge = zeroed + 14 * M;
+    c.geo.faceMarks = zeroed + 14 * M + 12 * nD;
+    neighbour_columns<<<div_up(size_t(27) * o.M, kThreads), kThreads, 0, s>>>(
+        o.M, o.neigh, o.basis, c.fld.dx, c.fld.nbrCol, reinterpret_cast<unsigned*>(zeroed), nZero);
     points_chi_kernel<<<nb, kThreads, 0, s>>>(N, L.base[L.D], o.M, L.D, o.spos, o.point2node, o.parent,
                                               c.fld.nbrCol, o.basis, o.q1Fired, c.fld.chiPts, part);
     iso_final<<<1, 1024, 0, s>>>(part, nb, N, c.fld.isoSum, isoDev);

@@ -1482,16 +1482,6 @@ __global__ void __launch_bounds__(kThreads) upload_words(const uint4* __restrict
         dst[i] = src[i];
 }
 
-__global__ void fill_root_pos(const int* __restrict__ roots, int n, int* __restrict__ rootPos) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) rootPos[roots[i]] = i;
-}
-
-__global__ void __launch_bounds__(kThreads) fill_int(int* __restrict__ a, size_t n, int v) {
-    size_t i = size_t(blockIdx.x) * kThreads + threadIdx.x;
-    if (i < n) a[i] = v;
-}
-
 // LDS words of a sub_corner_values_grid tile whose root is at depth rd, and
 // its halo entries.
 int grid_tile_words(int D, int rd, int* haloOut = nullptr) {
@@ -1563,9 +1553,7 @@ void extract_mesh(Ctx& c) {
     c.stats.n_coarse_roots = nCoarse;
     const int nSets = int(sets.size());
     if (nSets > 0) {
-        int* rootPos = c.buf<int>("root_pos", offD);
-        fill_int<<<div_up(offD, kThreads), kThreads, 0, st>>>(rootPos, size_t(offD), -1);
-        fill_root_pos<<<div_up(mm.nRoots, kThreads), kThreads, 0, st>>>(mm.roots, mm.nRoots, rootPos);
+        const int* rootPos = mm.rootPos;
         const size_t chunkCells = std::min<size_t>(c.cfg.subdiv_chunk_nodes ? c.cfg.subdiv_chunk_nodes : kChunkCells,
                                                    size_t(INT32_MAX - 1) / 12);
         float* isoDev = &c.results()->iso;
