@@ -221,7 +221,7 @@ __global__ void __launch_bounds__(kDivWaves * PSR_WAVE) divergence_fine(
 
 // Coarse nodes (few, each covering a large share of depth D): `parts`
 // workgroups each, every one summing a strided share of the node's terms; the
-// shares are folded in a fixed order by divergence_fold.
+// shares are folded in a fixed order by the first threads of divergence_leaf.
 constexpr int kDivCoarseThreads = 256;
 constexpr int kMaxCoarse = 4;  // coarse depths 0..3
 struct CoarsePlan {
@@ -272,27 +272,25 @@ __global__ void __launch_bounds__(kDivCoarseThreads) divergence_coarse(
     }
 }
 
-__global__ void __launch_bounds__(64) divergence_fold(CoarsePlan P, const double* __restrict__ partial,
-                                                      float* __restrict__ div) {
-    const int o = blockIdx.x * 64 + threadIdx.x;
-    if (o >= P.base[P.depths]) return;
-    int d = 0;
-    while (o >= P.base[d + 1]) ++d;
-    const int parts = P.parts[d];
-    const double* pp = partial + P.blockStart[d] + size_t(o - P.base[d]) * parts;
-    double s = 0;
-    for (int k = 0; k < parts; ++k) s += pp[k];
-    div[o] = float(s);
-}
-
 // Depth-D nodes: their neighbours are depth-D nodes, each its own only
-// descendant, so a thread sums the 27 terms of its node.
+// descendant, so a thread sums the 27 terms of its node.  The first threads
+// also fold the coarse nodes' shares (thread t: coarse node t), in order.
 template <class FieldT>
 __global__ void __launch_bounds__(256) divergence_leaf(int offD, int nD, int M, int res, int D,
                                                        const int* __restrict__ neigh, const int* __restrict__ enc,
                                                        const FieldT* __restrict__ vf,
-                                                       const double* __restrict__ ddot, float* __restrict__ div) {
+                                                       const double* __restrict__ ddot, CoarsePlan P,
+                                                       const double* __restrict__ partial, float* __restrict__ div) {
     const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t < P.base[P.depths]) {
+        int d = 0;
+        while (t >= P.base[d + 1]) ++d;
+        const int parts = P.parts[d];
+        const double* pp = partial + P.blockStart[d] + size_t(t - P.base[d]) * parts;
+        double s = 0;
+        for (int k = 0; k < parts; ++k) s += pp[k];
+        div[t] = float(s);
+    }
     if (t >= nD) return;
     const int o = offD + t;
     const int3 fo = decode_func_index(enc[o], D);
@@ -336,8 +334,9 @@ void field_stages(Ctx& c, FieldT* vf) {
     // between: a wave per node
     const int kCoarse = D >= 5 ? kMaxCoarse - 1 : 1;
     const int coarseCount = L.base[kCoarse + 1];
+    CoarsePlan P{};
+    double* partial = nullptr;
     {
-        CoarsePlan P{};
         P.depths = kCoarse + 1;
         int blocks = 0;
         for (int d = 0; d <= kCoarse; ++d) {
@@ -349,18 +348,17 @@ void field_stages(Ctx& c, FieldT* vf) {
         }
         P.blockStart[kCoarse + 1] = blocks;
         P.base[kCoarse + 1] = L.base[kCoarse + 1];
-        double* partial = c.buf<double>("div_partial", blocks);
+        partial = c.buf<double>("div_partial", blocks);
         divergence_coarse<FieldT><<<blocks, kDivCoarseThreads, 0, s>>>(P, M, offD, nD, c.tables.res, D, o.neigh, o.didx,
                                                                       o.dnum, o.encIdx, vf, c.tables.ddot, partial);
-        divergence_fold<<<div_up(coarseCount, 64), 64, 0, s>>>(P, partial, c.fld.div);
     }
     const int midCount = offD - coarseCount;
     if (midCount > 0)
         divergence_fine<FieldT><<<div_up(midCount, kDivWaves), kDivWaves * PSR_WAVE, 0, s>>>(
             coarseCount, midCount, M, offD, nD, c.tables.res, D, o.neigh, o.didx, o.dnum, o.encIdx, vf, c.tables.ddot,
             c.fld.div);
-    divergence_leaf<FieldT><<<div_up(nD, 256), 256, 0, s>>>(offD, nD, M, c.tables.res, D, o.neigh, o.encIdx, vf,
-                                                            c.tables.ddot, c.fld.div);
+    divergence_leaf<FieldT><<<div_up(std::max(nD, coarseCount), 256), 256, 0, s>>>(
+        offD, nD, M, c.tables.res, D, o.neigh, o.encIdx, vf, c.tables.ddot, P, partial, c.fld.div);
     PSR_HIP_CHECK(hipGetLastError());
 }
 

@@ -207,7 +207,11 @@ struct LevelOffsets {
 __global__ void __launch_bounds__(kThreads) rank_points(const unsigned* __restrict__ codes, int n, int D, int tiles,
                                                         const int* __restrict__ tileOff, LevelOffsets lo,
                                                         int* __restrict__ firstPoint, int* __restrict__ parentRank,
-                                                        int* __restrict__ rankDm1, int* __restrict__ point2node) {
+                                                        int* __restrict__ rankDm1, int* __restrict__ point2node,
+                                                        int* __restrict__ pnum, int M, int* __restrict__ q1Fired) {
+    // clears the point counts (only occupied nodes get one) and the Q1 flag
+    for (int i = blockIdx.x * kThreads + threadIdx.x; i < M; i += gridDim.x * kThreads) pnum[i] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *q1Fired = 0;
     int base = blockIdx.x * kRankTile + threadIdx.x * kRankItems;
     int lv[kRankItems];
     int cnt[PSR_MAX_DEPTH + 1];
@@ -394,30 +398,50 @@ __device__ __forceinline__ int parent_slot(int s, int u) {
     return 9 * (px + 1) + 3 * (py + 1) + (pz + 1);
 }
 
+// Packed 1-D function index (BuildOctree.cu:514-550), depth and basis
+// parameters (centres, inverse width) of node o at depth d.
+__device__ __forceinline__ void encode_node(int o, int d, int D, const int* __restrict__ key, int* __restrict__ enc,
+                                            int* __restrict__ depth, float4* __restrict__ basis) {
+    depth[o] = d;
+    const int3 f = key_func_index(key[o], d, D);
+    enc[o] = f.x | (f.y << (D + 1)) | (f.z << (2 * (D + 1)));
+    basis[o] = node_basis_params(f);
+}
+
 // Two depths per launch: the first count0 threads take depth `depth` from
 // their parents' final neighbours; the rest take depth + 1, whose parents'
 // neighbours this launch is still writing, so the 8 each one needs are formed
 // again from the grandparent's by the same recurrence (count1 = 0: one depth).
+// The launch for depth 1 also writes the root's row (itself at 13, nothing
+// else), reading it as such rather than from memory.  Every thread also
+// encodes its node (and the first, the root).
 __global__ void __launch_bounds__(kThreads) neighbors_level(int begin, int count0, int count1, int D, int depth,
                                                             int M, const int* __restrict__ key,
                                                             const int* __restrict__ parent,
                                                             const int* __restrict__ childBase,
-                                                            int* __restrict__ neigh) {
+                                                            int* __restrict__ neigh, int* __restrict__ enc,
+                                                            int* __restrict__ depthOf, float4* __restrict__ basis) {
     const int t = blockIdx.x * kThreads + threadIdx.x;
+    const bool fromRoot = depth == 1;
+    if (fromRoot && t < 27) neigh[t * M] = t == 13 ? 0 : -1;
+    if (fromRoot && t == 0) encode_node(0, 0, D, key, enc, depthOf, basis);
     if (t >= count0 + count1) return;
     const int o = begin + t;
     const int d = t < count0 ? depth : depth + 1;
+    encode_node(o, d, D, key, enc, depthOf, basis);
     const int s = (key[o] >> (3 * (D - d))) & 7;
     const int p = parent[o];
+    // neighbour at `slot` of an ancestor at depth - 1 (the root when fromRoot)
+    auto upper = [&](int slot, int node) { return fromRoot ? (slot == 13 ? 0 : -1) : neigh[slot * M + node]; };
     int np[8];  // the parent's neighbours at slots parent_slot(s, u)
     if (t < count0) {
 #pragma unroll
-        for (int u = 0; u < 8; ++u) np[u] = neigh[parent_slot(s, u) * M + p];
+        for (int u = 0; u < 8; ++u) np[u] = upper(parent_slot(s, u), p);
     } else {
         const int sp = (key[p] >> (3 * (D - depth))) & 7, gp = parent[p];
         int cbp[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) cbp[u] = neigh[parent_slot(sp, u) * M + gp];
+        for (int u = 0; u < 8; ++u) cbp[u] = upper(parent_slot(sp, u), gp);
 #pragma unroll
         for (int u = 0; u < 8; ++u) cbp[u] = cbp[u] == -1 ? -1 : childBase[cbp[u]];
 #pragma unroll
@@ -428,25 +452,6 @@ __global__ void __launch_bounds__(kThreads) neighbors_level(int begin, int count
     for (int u = 0; u < 8; ++u) cb[u] = np[u] == -1 ? -1 : childBase[np[u]];
 #pragma unroll
     for (int i = 0; i < 27; ++i) neigh[i * M + o] = child_neighbour(cb, s, i);
-}
-
-__global__ void root_neighbors(int* __restrict__ neigh, int M) {
-    int i = threadIdx.x;
-    if (i < 27) neigh[i * M] = (i == 13) ? 0 : -1;
-}
-
-// Packed 1-D function index (BuildOctree.cu:514-550).
-// Also writes each node's depth and basis parameters (centres, inverse width).
-__global__ void __launch_bounds__(kThreads) encode_function_index(Levels L, int M, const int* __restrict__ key,
-                                                                  int* __restrict__ enc, int* __restrict__ depth,
-                                                                  float4* __restrict__ basis) {
-    int o = blockIdx.x * kThreads + threadIdx.x;
-    if (o >= M) return;
-    int d = depth_of(L, o);
-    depth[o] = d;
-    int3 f = key_func_index(key[o], d, L.D);
-    enc[o] = f.x | (f.y << (L.D + 1)) | (f.z << (2 * (L.D + 1)));
-    basis[o] = node_basis_params(f);
 }
 
 }  // namespace
@@ -511,8 +516,6 @@ void build_octree(Ctx& c, const psr_points& pts) {
     int* parentRank = c.buf<int>("parent_rank", totalNodes);
     int* rankDm1 = c.buf<int>("rank_dm1", N);
     o.point2node = c.buf<int>("point2node", N);
-    rank_points<<<tiles, kThreads, 0, s>>>(keys, N, D, tiles, tileOff, lo, firstPoint, parentRank, rankDm1,
-                                           o.point2node);
     // node arrays
     o.key = c.buf<int>("node_key", M);
     o.pidx = c.buf<int>("node_pidx", M);
@@ -524,7 +527,8 @@ void build_octree(Ctx& c, const psr_points& pts) {
     o.neigh = c.buf<int>("node_neigh", size_t(27) * M);
     o.encIdx = c.buf<int>("node_enc", M);
     o.q1Fired = &c.results()->q1Fired;
-    zero_spans({{o.pnum, size_t(M)}, {o.q1Fired, 1}}, s);
+    rank_points<<<tiles, kThreads, 0, s>>>(keys, N, D, tiles, tileOff, lo, firstPoint, parentRank, rankDm1,
+                                           o.point2node, o.pnum, M, o.q1Fired);
     fill_occupied<<<div_up(totalNodes, kThreads), kThreads, 0, s>>>(keys, N, L, lo, Ud, firstPoint, parentRank,
                                                                      rankDm1, totalNodes, o.key, o.pidx, o.pnum,
                                                                      o.parent, o.childBase, o.didx, o.dnum,
@@ -533,15 +537,14 @@ void build_octree(Ctx& c, const psr_points& pts) {
     fill_blocks<<<div_up(totalBlocks + 1, kThreads), kThreads, 0, s>>>(L, totalBlocks, N, o.U[D - 1], o.key, o.pidx,
                                                                         o.pnum, o.parent, o.childBase, o.didx,
                                                                         o.dnum);
-    root_neighbors<<<1, 64, 0, s>>>(o.neigh, M);
-    for (int d = 1; d <= D; d += 2) {
-        const int c1 = d + 1 <= D ? L.count[d + 1] : 0;
-        neighbors_level<<<div_up(L.count[d] + c1, kThreads), kThreads, 0, s>>>(L.base[d], L.count[d], c1, D, d, M,
-                                                                                o.key, o.parent, o.childBase, o.neigh);
-    }
+    // neighbours (two depths per launch), packed index, depth and basis of every node
     o.depth = c.buf<int>("node_depth", M);
     o.basis = c.buf<float4>("node_basis", M);
-    encode_function_index<<<div_up(M, kThreads), kThreads, 0, s>>>(L, M, o.key, o.encIdx, o.depth, o.basis);
+    for (int d = 1; d <= D; d += 2) {
+        const int c1 = d + 1 <= D ? L.count[d + 1] : 0;
+        neighbors_level<<<div_up(std::max(L.count[d] + c1, 27), kThreads), kThreads, 0, s>>>(
+            L.base[d], L.count[d], c1, D, d, M, o.key, o.parent, o.childBase, o.neigh, o.encIdx, o.depth, o.basis);
+    }
     PSR_HIP_CHECK(hipGetLastError());
 }
 
