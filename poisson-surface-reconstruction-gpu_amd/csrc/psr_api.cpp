@@ -108,6 +108,7 @@ int run(psr_ctx* ctx, const psr_points* pts, psr_mesh* out) {
     extract_mesh(c);
     PSR_HIP_CHECK(hipEventRecord(c.ev[4], s));
     c.wait();  // the results reached hostResults with the vertex values (build_geometry)
+    finish_mesh(c);
     // statistics
     psr_stats& st = c.stats;
     const Octree& o = c.oct;

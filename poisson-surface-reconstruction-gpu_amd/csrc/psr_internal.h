@@ -110,6 +110,12 @@ struct Mesh {
     float* vertices = nullptr;  // xyz
     int* triangles = nullptr;
     int nv = 0, nt = 0;
+    // Counts still in the mailbox when extraction queued its last kernels without
+    // waiting: kept vertices = slot 3 + slot 1, triangles = triBefore + slot 2,
+    // each within its allocated bound.
+    bool deferred = false;
+    int triBefore = 0;
+    size_t vCap = 0, tCap = 0;
 };
 
 // Small per-call results written by the kernels and read back in one copy.
@@ -229,5 +235,7 @@ void cg_forget_event(hipEvent_t e);
 void points_implicit(Ctx& c);
 void build_geometry(Ctx& c);
 void extract_mesh(Ctx& c);
+// Mesh counts of a deferred extraction, read from the mailbox after the final wait.
+void finish_mesh(Ctx& c);
 
 }  // namespace psr

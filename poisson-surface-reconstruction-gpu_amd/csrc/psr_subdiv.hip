@@ -1287,10 +1287,12 @@ __global__ void __launch_bounds__(kThreads) sub_classify(Grid g, const Tile* __r
 }
 
 // Exclusive scan of the per-tile counts in one workgroup; totals[0..2] are the
-// chunk's crossings, kept vertices and triangles.
+// chunk's crossings, kept vertices and triangles; totals[3] = *keptMain, the
+// vertices kept by the depth-D batch.
 constexpr int kScanThreads = 1024;
 __global__ void __launch_bounds__(kScanThreads) sub_tile_scan(const int4* __restrict__ cnt, int n,
-                                                              int4* __restrict__ base, int* __restrict__ totals) {
+                                                              int4* __restrict__ base, const int* __restrict__ keptMain,
+                                                              int* __restrict__ totals) {
     __shared__ int lSum[kScanThreads / PSR_WAVE];
     const int per = (n + kScanThreads - 1) / kScanThreads;
     const int i0 = min(n, int(threadIdx.x) * per), i1 = min(n, i0 + per);
@@ -1316,21 +1318,23 @@ __global__ void __launch_bounds__(kScanThreads) sub_tile_scan(const int4* __rest
         totals[0] = ta;
         totals[1] = tb;
         totals[2] = tc;
+        totals[3] = *keptMain;
     }
 }
 
 // Kept crossing points and the triangles of the subdivision batches.  A
 // triangle index is the number of vertices kept before the batch plus the raw
 // offset inside the batch; indices are not remapped around dropped vertices
-// (insertTriangle :1247-1282).  keptBefore / triBefore: vertices kept and
-// triangles written before this chunk.
+// (insertTriangle :1247-1282).  *keptMain + keptBefore / triBefore: vertices
+// kept and triangles written before this chunk (*keptMain: those of the
+// depth-D batch).
 __global__ void __launch_bounds__(kThreads) sub_emit(Grid g, const Tile* __restrict__ tiles, int nVD,
                                                      const float* __restrict__ vals,
                                                      const unsigned* __restrict__ cellWord,
                                                      const int* __restrict__ crossPrefix,
                                                      const int4* __restrict__ tileCnt,
-                                                     const int4* __restrict__ tileBase, int keptBefore,
-                                                     int triBefore, float* __restrict__ outV, int* __restrict__ outT) {
+                                                     const int4* __restrict__ tileBase,
+                                                     const int* __restrict__ keptMain, int keptBefore, int triBefore, float* __restrict__ outV, int* __restrict__ outT) {
     __shared__ int lSum[kThreads / PSR_WAVE];
     __shared__ TileRoot lRoots[kMaxRootsPerTile];
     __shared__ int lNbr[kMaxRootsPerTile * 27];
@@ -1357,6 +1361,7 @@ __global__ void __launch_bounds__(kThreads) sub_emit(Grid g, const Tile* __restr
     const Leaf L = leaf_of(g, T.set, S, lRoots[rr], e);
     const RecVE ve = rec_ve(lNbr + 27 * rr, lSet[rr], L, g.D);
     const int4 tb = tileBase[blockIdx.x];
+    keptBefore += *keptMain;
     if (keep) {
         float v[8];
 #pragma unroll
@@ -1511,18 +1516,9 @@ void extract_mesh(Ctx& c) {
     float* mainV = c.buf<float>("raw_main_v", size_t(36) * nD);  // at most one crossing per depth-D edge
     int* outT = c.buf_keep<int>("mesh_triangles", size_t(15) * nD, 0);
     MainMc mm;
-    depth_mc(c, mm, mainV, outT);
+    c.mesh.deferred = false;
+    depth_mc(c, mm, mainV, outT);  // ends with a wait: no kernel is in flight
     c.stats.n_subdivision_roots = mm.nRoots;
-    // vertices kept by the depth-D batch: mailbox slot kMailKept (zero if none)
-    constexpr int kMailKept = 3;
-    c.hostPinned[kMailKept] = 0;
-    size_t triTotal = mm.nTri, keptTotal = 0;
-    float* outV = c.buf_keep<float>("mesh_vertices", size_t(3) * std::max(mm.nCross, 1), 0);
-    if (mm.nCross > 0) {
-        const int tiles = div_up(mm.nCross, kThreads);
-        keep_compact<<<tiles, kThreads, 0, st>>>(
-            mainV, mm.nCross, scan_ticket(c.scan_for_tiles(size_t(tiles)), tiles, c.mail + kMailKept), outV);
-    }
     // ---- subdivision sets: coarse roots one per set, finer roots one set per depth
     // (ComputeTriangleIndices.h:371: roots above depth D-4 are coarse)
     const int finerDepth = D - 4;
@@ -1552,10 +1548,37 @@ void extract_mesh(Ctx& c) {
     }
     c.stats.n_coarse_roots = nCoarse;
     const int nSets = int(sets.size());
+    const size_t chunkCells = std::min<size_t>(c.cfg.subdiv_chunk_nodes ? c.cfg.subdiv_chunk_nodes : kChunkCells,
+                                               size_t(INT32_MAX - 1) / 12);
+    // One chunk: nothing waits before the emission.  Vertices and triangles go
+    // to buffers sized by bound (a root of s^3 cells has 3s(s+1)^2 edges, a cell
+    // at most 5 triangles) and the counts stay in the mailbox until the call's
+    // final wait: slot 3 the vertices kept by the depth-D batch, slots 1 and 2
+    // the subdivision's kept vertices and triangles.
+    size_t cellsAll = 0, vBound = 0;
+    for (int k = 0; k < nSets; ++k) {
+        const size_t sk = size_t(1) << (D - sets[k].rd);
+        cellsAll += setCells[k];
+        vBound += size_t(sets[k].nr) * 3 * sk * (sk + 1) * (sk + 1);
+    }
+    const size_t vCap = size_t(mm.nCross) + vBound, tCap = size_t(mm.nTri) + 5 * cellsAll;
+    const bool deferred =
+        (nSets <= 1 || cellsAll <= chunkCells) && vCap <= size_t(INT32_MAX) && tCap <= size_t(INT32_MAX) / 3;
+    constexpr int kMailKept = 3;
+    c.hostPinned[1] = c.hostPinned[2] = c.hostPinned[kMailKept] = 0;
+    // vertices kept by the depth-D batch: a device int read by the subdivision
+    // kernels, or the mailbox slot when there is no subdivision
+    int* keptMain = nSets > 0 ? c.buf<int>("sub_kept_main", 1) : c.mail + kMailKept;
+    size_t triTotal = mm.nTri, keptSub = 0, keptMainHost = 0;
+    float* outV = c.buf_keep<float>("mesh_vertices", 3 * (deferred ? vCap : size_t(std::max(mm.nCross, 1))), 0);
+    {
+        const int tiles = std::max(1, int(div_up(mm.nCross, kThreads)));
+        keep_compact<<<tiles, kThreads, 0, st>>>(
+            mainV, mm.nCross, scan_ticket(c.scan_for_tiles(size_t(tiles)), tiles, keptMain), outV);
+    }
+    if (deferred && nSets > 0) outT = c.buf_keep<int>("mesh_triangles", 3 * tCap, 3 * size_t(mm.nTri));
     if (nSets > 0) {
         const int* rootPos = mm.rootPos;
-        const size_t chunkCells = std::min<size_t>(c.cfg.subdiv_chunk_nodes ? c.cfg.subdiv_chunk_nodes : kChunkCells,
-                                                   size_t(INT32_MAX - 1) / 12);
         float* isoDev = &c.results()->iso;
         const float4* nbrCol = c.fld.nbrCol;
         int* rootNbr = c.buf<int>("sub_root_nbr", size_t(27) * mm.nRoots);
@@ -1667,34 +1690,57 @@ void extract_mesh(Ctx& c) {
                 sub_corner_values<false, kSmallThreads><<<nSmall, kSmallThreads, 0, st>>>(gr, dsmall, nVD, isoDev, cval);
             }
             sub_classify<<<nTiles, kThreads, 0, st>>>(gr, dtiles, nVD, cval, vals, cellWord, crossPre, tileCnt);
-            sub_tile_scan<<<1, kScanThreads, 0, st>>>(tileCnt, nTiles, tileBase, c.mail);  // totals: slots 0-2
+            // totals: slots 0-3
+            sub_tile_scan<<<1, kScanThreads, 0, st>>>(tileCnt, nTiles, tileBase, keptMain, c.mail);
+            if (deferred) {
+                sub_emit<<<nTiles, kThreads, 0, st>>>(gr, dtiles, nVD, vals, cellWord, crossPre, tileCnt, tileBase,
+                                                      keptMain, 0, mm.nTri, outV, outT);
+                s0 = s1;
+                continue;
+            }
             c.wait();
             const int nKept = c.hostPinned[1], nTri = c.hostPinned[2];
-            if (s0 == 0) keptTotal = size_t(c.hostPinned[kMailKept]);
+            if (s0 == 0) keptMainHost = size_t(c.hostPinned[kMailKept]);
+            const size_t keptTotal = keptMainHost + keptSub;
             if (keptTotal + nKept > size_t(INT32_MAX) || triTotal + nTri > size_t(INT32_MAX) / 3)
                 throw std::runtime_error("mesh exceeds 32-bit indexing");
             outV = c.buf_keep<float>("mesh_vertices", 3 * (keptTotal + nKept), 3 * keptTotal);
             outT = c.buf_keep<int>("mesh_triangles", 3 * (triTotal + nTri), 3 * triTotal);
             if (nKept > 0 || nTri > 0)
                 sub_emit<<<nTiles, kThreads, 0, st>>>(gr, dtiles, nVD, vals, cellWord, crossPre, tileCnt, tileBase,
-                                                      int(keptTotal), int(triTotal), outV, outT);
-            keptTotal += nKept;
+                                                      keptMain, int(keptSub), int(triTotal), outV, outT);
+            keptSub += nKept;
             triTotal += nTri;
             s0 = s1;
         }
     }
-    if (nSets == 0) {  // only the depth-D batch: read its kept count
-        c.wait();
-        keptTotal = size_t(c.hostPinned[kMailKept]);
-    }
-    if (triTotal > size_t(INT32_MAX) / 3) throw std::runtime_error("mesh exceeds 32-bit triangle indexing");
     c.mesh.vertices = outV;
     c.mesh.triangles = outT;
-    c.mesh.nv = int(keptTotal);
+    PSR_HIP_CHECK(hipGetLastError());
+    if (deferred) {
+        c.mesh.deferred = true;
+        c.mesh.triBefore = mm.nTri;
+        c.mesh.vCap = vCap;
+        c.mesh.tCap = nSets > 0 ? tCap : size_t(mm.nTri);
+        return;
+    }
+    c.mesh.nv = int(keptMainHost + keptSub);
     c.mesh.nt = int(triTotal);
     c.stats.n_mesh_vertices = c.mesh.nv;
     c.stats.n_mesh_triangles = c.mesh.nt;
-    PSR_HIP_CHECK(hipGetLastError());
+}
+
+void finish_mesh(Ctx& c) {
+    Mesh& m = c.mesh;
+    if (!m.deferred) return;
+    m.deferred = false;
+    const size_t kept = size_t(c.hostPinned[3]) + size_t(c.hostPinned[1]);
+    const size_t tri = size_t(m.triBefore) + size_t(c.hostPinned[2]);
+    if (kept > m.vCap || tri > m.tCap) throw std::runtime_error("mesh counts exceed their bounds");
+    m.nv = int(kept);
+    m.nt = int(tri);
+    c.stats.n_mesh_vertices = m.nv;
+    c.stats.n_mesh_triangles = m.nt;
 }
 
 }  // namespace psr
