@@ -1479,12 +1479,31 @@ __global__ void __launch_bounds__(kThreads) keep_compact(const float* __restrict
     out[3 * size_t(a) + 2] = z;
 }
 
-// Copy of 16-byte words from mapped host memory, on the compute queue (a copy
-// engine would add a hand-over between queues before the kernels that read it).
-__global__ void __launch_bounds__(kThreads) upload_words(const uint4* __restrict__ src, uint4* __restrict__ dst,
-                                                         size_t n) {
-    for (size_t i = size_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += size_t(gridDim.x) * kThreads)
-        dst[i] = src[i];
+// The chunk's sets, read from mapped host memory on the compute queue (a copy
+// engine would add a hand-over between queues), and their tiles: set k
+// (blocks (k, *)) has cells = nr << 3(D - rd) cells cut into tiles of kTile
+// cells from firstTile on and, for roots less than 3 levels above depth D,
+// into small tiles of 32 (one level) or 64 (two levels) cells from
+// firstSmall[k] on.  A tile's root is the root holding its first cell.
+__global__ void __launch_bounds__(kThreads) make_tiles(const SetDesc* __restrict__ hostSets,
+                                                       const int* __restrict__ firstSmall, int D,
+                                                       SetDesc* __restrict__ sets, Tile* __restrict__ tiles,
+                                                       Tile* __restrict__ small) {
+    const int k = blockIdx.x;
+    const SetDesc S = hostSets[k];
+    if (blockIdx.y == 0 && threadIdx.x == 0) sets[k] = S;
+    const int sh = 3 * (D - S.rd), cells = S.nr << sh;
+    const int j0 = int(blockIdx.y * kThreads + threadIdx.x), stride = int(gridDim.y * kThreads);
+    for (int j = j0; j < (cells + kTile - 1) / kTile; j += stride) {
+        const int t = j * kTile;
+        tiles[S.firstTile + j] = Tile{k, S.dEnum + t, min(kTile, cells - t), S.rootOffset + (t >> sh)};
+    }
+    if (sh >= 9) return;
+    const int step = sh == 3 ? 32 : 64, f = firstSmall[k];
+    for (int j = j0; j < (cells + step - 1) / step; j += stride) {
+        const int t = j * step;
+        small[f + j] = Tile{k, S.dEnum + t, min(step, cells - t), S.rootOffset + (t >> sh)};
+    }
 }
 
 // LDS words of a sub_corner_values_grid tile whose root is at depth rd, and
@@ -1606,8 +1625,7 @@ void extract_mesh(Ctx& c) {
             scatter_dx_grid<<<div_up(M, kThreads), kThreads, 0, st>>>(M, L, o.key, c.fld.dx, dxAt);
         }
         std::vector<SetDesc> cs;
-        std::vector<Tile> smallTiles;
-        std::vector<Tile> tiles;
+        std::vector<int> firstSmall;
         for (int s0 = 0; s0 < nSets;) {
             // ---- one chunk of sets [s0, s1), cell enumeration local to the chunk
             size_t cells = 0;
@@ -1619,48 +1637,48 @@ void extract_mesh(Ctx& c) {
                 cs[k].dEnum = de;
                 de += int(setCells[s0 + k]);
             }
-            // tiles: those lying in one root first, then those of small roots; sets
-            // of one-root tiles come first, so the tiles are in enumeration order
-            tiles.clear();
+            // tiles: those lying in one root first, then those of small roots
+            // (a tile per 4 roots of 8 cells or per root of 64 cells, for their
+            // corner values); sets of one-root tiles come first, so the tiles are
+            // in enumeration order.  Counted here, written by make_tiles.
+            int nTiles = 0, nOne = 0, nSmall = 0, perSet = 1;
+            firstSmall.assign(nCs, 0);
             for (int pass = 0; pass < 2; ++pass)
                 for (int k = 0; k < nCs; ++k) {
-                    if ((D - cs[k].rd >= 3) != (pass == 0)) continue;
-                    cs[k].firstTile = int(tiles.size());
-                    for (size_t t = 0; t < setCells[s0 + k]; t += kTile)
-                        tiles.push_back({k, cs[k].dEnum + int(t), int(std::min<size_t>(kTile, setCells[s0 + k] - t)),
-                                         cs[k].rootOffset + int(t >> (3 * (D - cs[k].rd)))});
+                    const int shk = D - cs[k].rd;
+                    if ((shk >= 3) != (pass == 0)) continue;
+                    const int n = int(div_up(setCells[s0 + k], size_t(kTile)));
+                    cs[k].firstTile = nTiles;
+                    nTiles += n;
+                    perSet = std::max(perSet, n);
+                    if (pass == 0) {
+                        nOne += n;
+                    } else {
+                        const int ns = int(div_up(setCells[s0 + k], size_t(shk == 1 ? 32 : 64)));
+                        firstSmall[k] = nSmall;
+                        nSmall += ns;
+                        perSet = std::max(perSet, ns);
+                    }
                 }
             for (int k = 1; k < nCs; ++k)
                 if (cs[k].firstTile < cs[k - 1].firstTile) throw std::runtime_error("subdivision tiles out of order");
-            const int nTiles = int(tiles.size());
-            int nOne = 0;
-            while (nOne < nTiles && D - cs[tiles[nOne].set].rd >= 3) ++nOne;
-            // corner values of small roots: a tile per 4 roots of 8 cells or per
-            // root of 64 cells, one thread per corner point
-            smallTiles.clear();
-            for (int k = 0; k < nCs; ++k) {
-                const int shk = D - cs[k].rd;
-                if (shk >= 3) continue;
-                const size_t step = shk == 1 ? 32 : 64;
-                for (size_t t = 0; t < setCells[s0 + k]; t += step)
-                    smallTiles.push_back({k, cs[k].dEnum + int(t), int(std::min<size_t>(step, setCells[s0 + k] - t)),
-                                          cs[k].rootOffset + int(t >> (3 * shk))});
-            }
-            const int nSmall = int(smallTiles.size());
-            // sets, tiles and small tiles go up in one copy from pinned staging
-            // (reused only after this chunk's synchronisation)
+            // sets and first small tiles go up through pinned staging (reused
+            // only after a synchronisation)
             const size_t bSets = (sizeof(SetDesc) * nCs + 255) & ~size_t(255);
             const size_t bTiles = (sizeof(Tile) * nTiles + 255) & ~size_t(255);
-            const size_t bAll = bSets + bTiles + sizeof(Tile) * std::max(nSmall, 1);
-            char* hs = static_cast<char*>(c.stage(bAll));
+            char* hs = static_cast<char*>(c.stage(bSets + sizeof(int) * nCs));
             std::memcpy(hs, cs.data(), sizeof(SetDesc) * nCs);
-            std::memcpy(hs + bSets, tiles.data(), sizeof(Tile) * nTiles);
-            if (nSmall) std::memcpy(hs + bSets + bTiles, smallTiles.data(), sizeof(Tile) * nSmall);
-            char* dAll = c.buf<char>("sub_upload", bAll);
-            const size_t words = bAll / sizeof(uint4);
-            upload_words<<<unsigned(std::min<size_t>(div_up(words, kThreads), 256)), kThreads, 0, st>>>(
-                static_cast<const uint4*>(c.stageDev), reinterpret_cast<uint4*>(dAll), words);
+            std::memcpy(hs + bSets, firstSmall.data(), sizeof(int) * nCs);
+            char* dAll = c.buf<char>("sub_upload", bSets + bTiles + sizeof(Tile) * std::max(nSmall, 1));
             SetDesc* dsets = reinterpret_cast<SetDesc*>(dAll);
+            {
+                const char* hsDev = static_cast<const char*>(c.stageDev);
+                const dim3 grid(unsigned(nCs), unsigned(std::min(16, int(div_up(perSet, kThreads)))));
+                make_tiles<<<grid, kThreads, 0, st>>>(reinterpret_cast<const SetDesc*>(hsDev),
+                                                      reinterpret_cast<const int*>(hsDev + bSets), D, dsets,
+                                                      reinterpret_cast<Tile*>(dAll + bSets),
+                                                      reinterpret_cast<Tile*>(dAll + bSets + bTiles));
+            }
             Tile* dtiles = reinterpret_cast<Tile*>(dAll + bSets);
             Tile* dsmall = reinterpret_cast<Tile*>(dAll + bSets + bTiles);
             Grid gr{D,       M,       nCs,      dsets,    mm.roots, rootPos,   o.key,    o.parent, o.childBase,
