@@ -104,7 +104,13 @@ struct Geometry {
     int* counts = nullptr;    // device: V, E, F
     int V = 0, E = 0, F = 0;  // host copies, known once the mesh stage has waited
     float* vvalue = nullptr;
+    float* dxGrid = nullptr;  // dense per-depth d_x grids, filled when the octree is regular (or null)
 };
+
+// Dense per-depth d_x grids are built before the mesh stage's wait up to this
+// depth (9.6 MB at depth 7, 77 MB at depth 8), after it beyond.
+constexpr int kDenseEarlyDepth = 8;
+inline size_t dense_grid_cells(int D) { return ((size_t(1) << (3 * (D + 1))) - 1) / 7; }
 
 struct Mesh {
     float* vertices = nullptr;  // xyz

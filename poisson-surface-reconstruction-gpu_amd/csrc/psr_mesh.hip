@@ -101,10 +101,12 @@ __device__ __forceinline__ void regular_owned(const int (&kk)[27], bool depthD, 
     }
 }
 
+// dense: the dense d_x grids (or null), where a regular octree's nodes put dx.
 __global__ void __launch_bounds__(kThreads) geometry_emit(NodeView v, int offD, int nD,
                                                           const int* __restrict__ parent,
                                                           const int* __restrict__ q1Fired, ScanTicket tv,
-                                                          ScanTicket te, ScanTicket tf, GeoOut out) {
+                                                          ScanTicket te, ScanTicket tf, GeoOut out,
+                                                          const float* __restrict__ dx, float* __restrict__ dense) {
     const int tile = ticket_tile(tv);
     const int o = tile * kThreads + threadIdx.x;
     unsigned vm = 0, em = 0, fm = 0;
@@ -112,6 +114,7 @@ __global__ void __launch_bounds__(kThreads) geometry_emit(NodeView v, int offD, 
     bool done = false;
     if (o < v.M && *q1Fired == 0) {
         depth = v.depth[o];
+        if (dense) dense[dense_grid_index(v.key[o], depth, v.D)] = dx[o];
         int kk[27];
 #pragma unroll
         for (int i = 0; i < 27; ++i) {
@@ -544,7 +547,8 @@ void build_geometry(Ctx& c) {
     g.edgeOf = c.buf<int2>("edge_of", nD);
     const GeoOut out{g.vtxOwner, g.vtxKind,  g.vtxPos,   g.nodeVert,      g.edgeOwner, g.edgeKind,
                      g.nodeEdge, g.faceOwner, g.faceKind, g.faceHasParent, g.nodeFace,  g.edgeOf};
-    geometry_emit<<<tilesM, kThreads, 0, s>>>(v, offD, nD, o.parent, o.q1Fired, tv, te, tf, out);
+    geometry_emit<<<tilesM, kThreads, 0, s>>>(v, offD, nD, o.parent, o.q1Fired, tv, te, tf, out, c.fld.dx,
+                                              c.geo.dxGrid);
     // vertex implicit values
     float* isoDev = &c.results()->iso;
     g.vvalue = c.buf<float>("vvalue", size_t(8) * M);

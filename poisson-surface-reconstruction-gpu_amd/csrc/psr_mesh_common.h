@@ -105,6 +105,41 @@ struct MainMc {
     bool q1Fired = false;  // the level-local child 0 rule changed a parent somewhere
 };
 
+// Morton code of a cell at a depth with `bits` levels (x high in each triplet):
+// the node key format of the reference.
+__device__ __forceinline__ unsigned spread3(unsigned v) {  // bit i -> bit 3i, i < 10
+    v &= 0x3FFu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+__device__ __forceinline__ unsigned compact3(unsigned v) {  // bit 3i -> bit i, i < 10
+    v &= 0x09249249u;
+    v = (v | (v >> 2)) & 0x030C30C3u;
+    v = (v | (v >> 4)) & 0x0300F00Fu;
+    v = (v | (v >> 8)) & 0x030000FFu;
+    v = (v | (v >> 16)) & 0x3FFu;
+    return v;
+}
+// The low `bits` bits of each coordinate, interleaved.
+__device__ __forceinline__ int morton(int3 c, int bits) {
+    const unsigned m = (1u << bits) - 1u;
+    return int((spread3(unsigned(c.x) & m) << 2) | (spread3(unsigned(c.y) & m) << 1) | spread3(unsigned(c.z) & m));
+}
+__device__ __forceinline__ int3 demorton(int k, int bits) {
+    const unsigned u = unsigned(k) & ((1u << (3 * bits)) - 1u);
+    return make_int3(int(compact3(u >> 2)), int(compact3(u >> 1)), int(compact3(u)));
+}
+
+// Position of node (key, depth l) in the dense per-depth grids of d_x: the
+// depth-l grid starts at (8^l - 1) / 7 and holds cell (x, y, z) at x 4^l + y 2^l + z.
+__device__ __forceinline__ size_t dense_grid_index(int key, int l, int D) {
+    const int3 c = demorton(key >> (3 * (D - l)), l);
+    return ((size_t(1) << (3 * l)) - 1) / 7 + ((size_t(c.x) << (2 * l)) | (size_t(c.y) << l) | size_t(c.z));
+}
+
 // Depth-D marching cubes into the raw mesh buffers (vertices at [0, nCross),
 // triangles at [0, nTri)), then the subdivision roots.  Ends with one wait
 // for the counts.

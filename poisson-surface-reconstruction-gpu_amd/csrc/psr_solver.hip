@@ -892,14 +892,19 @@ __global__ void __launch_bounds__(kCgThreads) cg_mixed_all_depths(
 // Neighbour columns: basis parameters and d_x of neighbour i of every node, so
 // the 27 terms of a node need no dependent index loads.
 // Also clears the nZero words at zero (the mesh stage's node tables and face
-// marks), which saves that stage a launch of its own.
+// marks) and, when the octree is regular, the nDense dense d_x grid cells,
+// which saves the mesh stage launches of its own.
 __global__ void __launch_bounds__(kThreads) neighbour_columns(int M, const int* __restrict__ neigh,
                                                               const float4* __restrict__ basis,
                                                               const float* __restrict__ dx,
                                                               float4* __restrict__ nbrCol,
-                                                              unsigned* __restrict__ zero, size_t nZero) {
+                                                              unsigned* __restrict__ zero, size_t nZero,
+                                                              float* __restrict__ dense, size_t nDense,
+                                                              const int* __restrict__ q1Fired) {
     const size_t t = size_t(blockIdx.x) * kThreads + threadIdx.x;
     for (size_t i = t; i < nZero; i += size_t(gridDim.x) * kThreads) zero[i] = 0u;
+    if (nDense && *q1Fired == 0)
+        for (size_t i = t; i < nDense; i += size_t(gridDim.x) * kThreads) dense[i] = 0.f;
     if (t >= size_t(27) * M) return;
     const int nb = neigh[t];
     if (nb < 0) {
@@ -1150,8 +1155,13 @@ void points_implicit(Ctx& c) {
     c.geo.nodeFace = zeroed + 8 * M;
     c.geo.nodeEdge = zeroed + 14 * M;
     c.geo.faceMarks = zeroed + 14 * M + 12 * nD;
+    // dense d_x grids for the subdivision of a regular octree (filled by the
+    // mesh stage's first kernel)
+    const size_t nDense = L.D <= kDenseEarlyDepth ? dense_grid_cells(L.D) : 0;
+    c.geo.dxGrid = nDense ? c.buf<float>("sub_dx_grid", nDense) : nullptr;
     neighbour_columns<<<div_up(size_t(27) * o.M, kThreads), kThreads, 0, s>>>(
-        o.M, o.neigh, o.basis, c.fld.dx, c.fld.nbrCol, reinterpret_cast<unsigned*>(zeroed), nZero);
+        o.M, o.neigh, o.basis, c.fld.dx, c.fld.nbrCol, reinterpret_cast<unsigned*>(zeroed), nZero, c.geo.dxGrid,
+        nDense, o.q1Fired);
     points_chi_kernel<<<nb, kThreads, 0, s>>>(N, L.base[L.D], o.M, L.D, o.spos, o.point2node, o.parent,
                                               c.fld.nbrCol, o.basis, o.q1Fired, c.fld.chiPts, part);
     iso_final<<<1, 1024, 0, s>>>(part, nb, N, c.fld.isoSum, isoDev);

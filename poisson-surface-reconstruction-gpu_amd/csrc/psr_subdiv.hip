@@ -76,34 +76,6 @@ __device__ __forceinline__ int bourke_slot(int ox, int oy, int oz) {
     return i;
 }
 
-// Morton code of a cell at a depth with `bits` levels (x high in each triplet):
-// the node key format of the reference.
-__device__ __forceinline__ unsigned spread3(unsigned v) {  // bit i -> bit 3i, i < 10
-    v &= 0x3FFu;
-    v = (v | (v << 16)) & 0x030000FFu;
-    v = (v | (v << 8)) & 0x0300F00Fu;
-    v = (v | (v << 4)) & 0x030C30C3u;
-    v = (v | (v << 2)) & 0x09249249u;
-    return v;
-}
-__device__ __forceinline__ unsigned compact3(unsigned v) {  // bit 3i -> bit i, i < 10
-    v &= 0x09249249u;
-    v = (v | (v >> 2)) & 0x030C30C3u;
-    v = (v | (v >> 4)) & 0x0300F00Fu;
-    v = (v | (v >> 8)) & 0x030000FFu;
-    v = (v | (v >> 16)) & 0x3FFu;
-    return v;
-}
-// The low `bits` bits of each coordinate, interleaved.
-__device__ __forceinline__ int morton(int3 c, int bits) {
-    const unsigned m = (1u << bits) - 1u;
-    return int((spread3(unsigned(c.x) & m) << 2) | (spread3(unsigned(c.y) & m) << 1) | spread3(unsigned(c.z) & m));
-}
-__device__ __forceinline__ int3 demorton(int k, int bits) {
-    const unsigned u = unsigned(k) & ((1u << (3 * bits)) - 1u);
-    return make_int3(int(compact3(u >> 2)), int(compact3(u >> 1)), int(compact3(u)));
-}
-
 // A virtual depth-D cell and its root.
 struct Leaf {
     SetDesc S;
@@ -1399,25 +1371,39 @@ __global__ void __launch_bounds__(kThreads) sub_emit(Grid g, const Tile* __restr
 // Its set: coarse roots (depth < finerDepth) form sets of one; the finer roots
 // of a depth form one set, so a neighbour belongs to a finer root's set iff it
 // is a root (it has the root's depth).
-__global__ void __launch_bounds__(kThreads) root_records(int nRoots, int M, Levels Lv, int finerDepth,
-                                                         const int* __restrict__ roots,
-                                                         const int* __restrict__ rootPos,
-                                                         const int* __restrict__ neigh,
-                                                         const int* __restrict__ parent,
-                                                         const int* __restrict__ key, int* __restrict__ rootNbr,
-                                                         int* __restrict__ rootChain, int4* __restrict__ rootCell,
-                                                         unsigned* __restrict__ rootSet) {
-    const size_t t = size_t(blockIdx.x) * kThreads + threadIdx.x;
+struct RootRecArgs {
+    int nRoots, M;
+    Levels Lv;
+    int finerDepth;
+    const int* roots;
+    const int* rootPos;
+    const int* neigh;
+    const int* parent;
+    const int* key;
+    int* rootNbr;
+    int* rootChain;
+    int4* rootCell;
+    unsigned* rootSet;
+};
+__device__ __forceinline__ void root_records(const RootRecArgs& a, int block) {
+    const int nRoots = a.nRoots, M = a.M, finerDepth = a.finerDepth;
+    const Levels& Lv = a.Lv;
+    const int* __restrict__ roots = a.roots;
+    const int* __restrict__ rootPos = a.rootPos;
+    const int* __restrict__ neigh = a.neigh;
+    const int* __restrict__ parent = a.parent;
+    const int* __restrict__ key = a.key;
+    const size_t t = size_t(block) * kThreads + threadIdx.x;
     if (t >= size_t(27) * nRoots) return;
     const int q = int(t / 27), i = int(t - size_t(q) * 27);
     const int root = roots[q];
     const int nb = neigh[i * M + root];
-    rootNbr[t] = nb < 0 ? -1 : rootPos[nb];
+    a.rootNbr[t] = nb < 0 ? -1 : rootPos[nb];
     if (i == 1) {
         int rd = 0;
         while (rd < Lv.D && root >= Lv.base[rd + 1]) ++rd;
         const int3 c = demorton(key[root] >> (3 * (Lv.D - rd)), rd);
-        rootCell[q] = make_int4(c.x, c.y, c.z, root);
+        a.rootCell[q] = make_int4(c.x, c.y, c.z, root);
         unsigned m = 1u << 13;
         if (rd >= finerDepth) {
             int nbs[27];
@@ -1427,12 +1413,12 @@ __global__ void __launch_bounds__(kThreads) root_records(int nRoots, int M, Leve
             for (int k = 0; k < 27; ++k)
                 if (nbs[k] >= 0 && rootPos[nbs[k]] >= 0) m |= 1u << k;
         }
-        rootSet[q] = m;
+        a.rootSet[q] = m;
     }
     if (i == 0) {
         int node = root;
         for (int u = 0; u < kChain; ++u) {
-            rootChain[kChain * size_t(q) + u] = node;
+            a.rootChain[kChain * size_t(q) + u] = node;
             node = node < 0 ? -1 : parent[node];
         }
     }
@@ -1445,9 +1431,7 @@ __global__ void __launch_bounds__(kThreads) scatter_dx_grid(int M, Levels L, con
     if (i >= M) return;
     int l = 0;
     while (l < L.D && i >= L.base[l + 1]) ++l;
-    const int3 c = demorton(key[i] >> (3 * (L.D - l)), l);
-    const size_t cell = (size_t(c.x) << (2 * l)) | (size_t(c.y) << l) | size_t(c.z);
-    grid[((size_t(1) << (3 * l)) - 1) / 7 + cell] = dx[i];
+    grid[dense_grid_index(key[i], l, L.D)] = dx[i];
 }
 
 __global__ void __launch_bounds__(kThreads) zero_floats(float* __restrict__ a, size_t n) {
@@ -1458,8 +1442,17 @@ __global__ void __launch_bounds__(kThreads) zero_floats(float* __restrict__ a, s
 // markValidMeshVertexIndex (:1170-1181): a vertex with |x| < 1e-6 is dropped;
 // triangle indices are not remapped (insertTriangle :1247-1282).  The kept
 // vertices are compacted in order in the same launch; their count goes to t.total.
-__global__ void __launch_bounds__(kThreads) keep_compact(const float* __restrict__ v, int n, ScanTicket t,
-                                                         float* __restrict__ out) {
+struct KeepArgs {
+    const float* v;
+    int n;
+    ScanTicket t;
+    float* out;
+};
+__device__ __forceinline__ void keep_compact(const KeepArgs& ka) {
+    const float* __restrict__ v = ka.v;
+    float* __restrict__ out = ka.out;
+    const int n = ka.n;
+    const ScanTicket& t = ka.t;
     const int tile = ticket_tile(t);
     const int i = tile * kThreads + threadIdx.x;
     float x = 0.f, y = 0.f, z = 0.f;
@@ -1485,24 +1478,46 @@ __global__ void __launch_bounds__(kThreads) keep_compact(const float* __restrict
 // cells from firstTile on and, for roots less than 3 levels above depth D,
 // into small tiles of 32 (one level) or 64 (two levels) cells from
 // firstSmall[k] on.  A tile's root is the root holding its first cell.
-__global__ void __launch_bounds__(kThreads) make_tiles(const SetDesc* __restrict__ hostSets,
-                                                       const int* __restrict__ firstSmall, int D,
-                                                       SetDesc* __restrict__ sets, Tile* __restrict__ tiles,
-                                                       Tile* __restrict__ small) {
-    const int k = blockIdx.x;
-    const SetDesc S = hostSets[k];
-    if (blockIdx.y == 0 && threadIdx.x == 0) sets[k] = S;
+struct TileArgs {
+    const SetDesc* hostSets;
+    const int* firstSmall;
+    int D, parts;  // parts: blocks per set
+    SetDesc* sets;
+    Tile* tiles;
+    Tile* small;
+};
+__device__ __forceinline__ void make_tiles(const TileArgs& a, int block) {
+    const int k = block / a.parts, part = block - k * a.parts, D = a.D;
+    const SetDesc S = a.hostSets[k];
+    Tile* __restrict__ tiles = a.tiles;
+    Tile* __restrict__ small = a.small;
+    if (part == 0 && threadIdx.x == 0) a.sets[k] = S;
     const int sh = 3 * (D - S.rd), cells = S.nr << sh;
-    const int j0 = int(blockIdx.y * kThreads + threadIdx.x), stride = int(gridDim.y * kThreads);
+    const int j0 = part * kThreads + int(threadIdx.x), stride = a.parts * kThreads;
     for (int j = j0; j < (cells + kTile - 1) / kTile; j += stride) {
         const int t = j * kTile;
         tiles[S.firstTile + j] = Tile{k, S.dEnum + t, min(kTile, cells - t), S.rootOffset + (t >> sh)};
     }
     if (sh >= 9) return;
-    const int step = sh == 3 ? 32 : 64, f = firstSmall[k];
+    const int step = sh == 3 ? 32 : 64, f = a.firstSmall[k];
     for (int j = j0; j < (cells + step - 1) / step; j += stride) {
         const int t = j * step;
         small[f + j] = Tile{k, S.dEnum + t, min(step, cells - t), S.rootOffset + (t >> sh)};
+    }
+}
+
+// The subdivision's set-up in one launch: blocks [0, keepBlocks) compact the
+// vertices kept by the depth-D batch, the next rootBlocks blocks write the
+// root records, the remaining blocks expand the tiles (parts blocks per set).
+__global__ void __launch_bounds__(kThreads) sub_setup(KeepArgs ka, int keepBlocks, RootRecArgs ra, int rootBlocks,
+                                                      TileArgs ta) {
+    const int b = blockIdx.x;
+    if (b < keepBlocks) {
+        keep_compact(ka);
+    } else if (b < keepBlocks + rootBlocks) {
+        root_records(ra, b - keepBlocks);
+    } else {
+        make_tiles(ta, b - keepBlocks - rootBlocks);
     }
 }
 
@@ -1590,11 +1605,11 @@ void extract_mesh(Ctx& c) {
     int* keptMain = nSets > 0 ? c.buf<int>("sub_kept_main", 1) : c.mail + kMailKept;
     size_t triTotal = mm.nTri, keptSub = 0, keptMainHost = 0;
     float* outV = c.buf_keep<float>("mesh_vertices", 3 * (deferred ? vCap : size_t(std::max(mm.nCross, 1))), 0);
-    {
-        const int tiles = std::max(1, int(div_up(mm.nCross, kThreads)));
-        keep_compact<<<tiles, kThreads, 0, st>>>(
-            mainV, mm.nCross, scan_ticket(c.scan_for_tiles(size_t(tiles)), tiles, keptMain), outV);
-    }
+    // the depth-D batch's kept vertices: compacted by the first set-up launch
+    const int keepBlocks = std::max(1, int(div_up(mm.nCross, kThreads)));
+    const KeepArgs ka{mainV, mm.nCross,
+                      scan_ticket(c.scan_for_tiles(size_t(keepBlocks)), keepBlocks, keptMain), outV};
+    if (nSets == 0) sub_setup<<<keepBlocks, kThreads, 0, st>>>(ka, keepBlocks, RootRecArgs{}, 0, TileArgs{});
     if (deferred && nSets > 0) outT = c.buf_keep<int>("mesh_triangles", 3 * tCap, 3 * size_t(mm.nTri));
     if (nSets > 0) {
         const int* rootPos = mm.rootPos;
@@ -1604,9 +1619,9 @@ void extract_mesh(Ctx& c) {
         int* rootChain = c.buf<int>("sub_root_chain", size_t(kChain) * mm.nRoots);
         int4* rootCell = c.buf<int4>("sub_root_cell", mm.nRoots);
         unsigned* rootSet = c.buf<unsigned>("sub_root_set", mm.nRoots);
-        root_records<<<div_up(size_t(27) * mm.nRoots, kThreads), kThreads, 0, st>>>(
-            mm.nRoots, M, L, finerDepth, mm.roots, rootPos, o.neigh, o.parent, o.key, rootNbr, rootChain, rootCell,
-            rootSet);
+        const RootRecArgs ra{mm.nRoots, M,        L,       finerDepth, mm.roots, rootPos,  o.neigh,
+                             o.parent,  o.key,    rootNbr, rootChain,  rootCell, rootSet};
+        const int rootBlocks = int(div_up(size_t(27) * mm.nRoots, kThreads));
         // tiles read d_x from dense per-depth grids when the octree is geometric
         const bool denseHalo = !mm.q1Fired;
         for (const SetDesc& S : sets) {
@@ -1617,9 +1632,9 @@ void extract_mesh(Ctx& c) {
             if (denseHalo && D - S.rd < 3 && small_tile_words(D, S.rd, D - S.rd == 1 ? 4 : 1) > kLdsWordsSmallGrid)
                 throw std::runtime_error("subdivision tile exceeds its LDS budget");
         }
-        float* dxAt = nullptr;
-        if (denseHalo) {
-            const size_t cellsAll = ((size_t(1) << (3 * (D + 1))) - 1) / 7;
+        float* dxAt = denseHalo ? c.geo.dxGrid : nullptr;  // built before the wait up to kDenseEarlyDepth
+        if (denseHalo && !dxAt) {
+            const size_t cellsAll = dense_grid_cells(D);
             dxAt = c.buf<float>("sub_dx_grid", cellsAll);
             zero_floats<<<div_up(cellsAll, kThreads), kThreads, 0, st>>>(dxAt, cellsAll);
             scatter_dx_grid<<<div_up(M, kThreads), kThreads, 0, st>>>(M, L, o.key, c.fld.dx, dxAt);
@@ -1640,7 +1655,7 @@ void extract_mesh(Ctx& c) {
             // tiles: those lying in one root first, then those of small roots
             // (a tile per 4 roots of 8 cells or per root of 64 cells, for their
             // corner values); sets of one-root tiles come first, so the tiles are
-            // in enumeration order.  Counted here, written by make_tiles.
+            // in enumeration order.  Counted here, written by sub_setup.
             int nTiles = 0, nOne = 0, nSmall = 0, perSet = 1;
             firstSmall.assign(nCs, 0);
             for (int pass = 0; pass < 2; ++pass)
@@ -1672,12 +1687,15 @@ void extract_mesh(Ctx& c) {
             char* dAll = c.buf<char>("sub_upload", bSets + bTiles + sizeof(Tile) * std::max(nSmall, 1));
             SetDesc* dsets = reinterpret_cast<SetDesc*>(dAll);
             {
+                // the first chunk's launch also compacts the depth-D batch and
+                // writes the root records
                 const char* hsDev = static_cast<const char*>(c.stageDev);
-                const dim3 grid(unsigned(nCs), unsigned(std::min(16, int(div_up(perSet, kThreads)))));
-                make_tiles<<<grid, kThreads, 0, st>>>(reinterpret_cast<const SetDesc*>(hsDev),
-                                                      reinterpret_cast<const int*>(hsDev + bSets), D, dsets,
-                                                      reinterpret_cast<Tile*>(dAll + bSets),
-                                                      reinterpret_cast<Tile*>(dAll + bSets + bTiles));
+                const int parts = std::min(16, int(div_up(perSet, kThreads)));
+                const TileArgs ta{reinterpret_cast<const SetDesc*>(hsDev), reinterpret_cast<const int*>(hsDev + bSets),
+                                  D, parts, dsets, reinterpret_cast<Tile*>(dAll + bSets),
+                                  reinterpret_cast<Tile*>(dAll + bSets + bTiles)};
+                const int kb = s0 == 0 ? keepBlocks : 0, rb = s0 == 0 ? rootBlocks : 0;
+                sub_setup<<<unsigned(kb + rb + nCs * parts), kThreads, 0, st>>>(ka, kb, ra, rb, ta);
             }
             Tile* dtiles = reinterpret_cast<Tile*>(dAll + bSets);
             Tile* dsmall = reinterpret_cast<Tile*>(dAll + bSets + bTiles);
