@@ -21,6 +21,58 @@ namespace {
 constexpr int kThreads = 256;
 
 // --- bounding box (BuildOctree.cu:51-133) and host adjustment (:619-631) ---
+// Reduces the nb block partials and returns the normalisation (x0, y0, z0,
+// edge) to every thread of the block; block 0 also stores it to params and mail.
+__device__ __forceinline__ float4 bbox_reduce(const float* __restrict__ part, int nb, float* __restrict__ params,
+                                              float* __restrict__ mail) {
+#pragma clang fp contract(off)
+    __shared__ float s[6][kThreads];
+    __shared__ float4 sOut;
+    float mx[3] = {-1e6f, -1e6f, -1e6f}, mn[3] = {1e6f, 1e6f, 1e6f};
+    for (int b = threadIdx.x; b < nb; b += kThreads)
+        for (int a = 0; a < 3; ++a) {
+            float v = part[b * 6 + a];
+            mx[a] = mx[a] < v ? v : mx[a];
+            v = part[b * 6 + 3 + a];
+            mn[a] = mn[a] > v ? v : mn[a];
+        }
+    for (int a = 0; a < 3; ++a) {
+        s[a][threadIdx.x] = mx[a];
+        s[3 + a][threadIdx.x] = mn[a];
+    }
+    __syncthreads();
+    for (int st = kThreads / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < st)
+            for (int a = 0; a < 3; ++a) {
+                float l = s[a][threadIdx.x], r = s[a][threadIdx.x + st];
+                s[a][threadIdx.x] = l < r ? r : l;
+                l = s[3 + a][threadIdx.x];
+                r = s[3 + a][threadIdx.x + st];
+                s[3 + a][threadIdx.x] = l > r ? r : l;
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        float maxEdge = 1.0f, center[3], lo[3];
+        for (int a = 0; a < 3; ++a) {
+            float ext = __fsub_rn(s[a][0], s[3 + a][0]);
+            if (a == 0 || maxEdge < ext) maxEdge = ext;
+            center[a] = __fdiv_rn(__fadd_rn(s[a][0], s[3 + a][0]), 2.0f);
+        }
+        maxEdge = __fmul_rn(maxEdge, 1.25f);
+        for (int a = 0; a < 3; ++a) lo[a] = __fsub_rn(center[a], __fdiv_rn(maxEdge, 2.0f));
+        sOut = make_float4(lo[0], lo[1], lo[2], maxEdge);
+        if (blockIdx.x == 0) {
+            for (int a = 0; a < 3; ++a) params[a] = mail[a] = lo[a];
+            params[3] = mail[3] = maxEdge;
+        }
+    }
+    __syncthreads();
+    return sOut;
+}
+
+// Block partial bounds (reduced by every block of the encoding, which needs
+// no launch of its own).
 __global__ void __launch_bounds__(kThreads) bbox_partial(const char* __restrict__ xyz, unsigned stride, int n,
                                                          float* __restrict__ part) {
     float mn[3] = {1e6f, 1e6f, 1e6f}, mx[3] = {-1e6f, -1e6f, -1e6f};
@@ -52,63 +104,18 @@ __global__ void __launch_bounds__(kThreads) bbox_partial(const char* __restrict_
     if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = s[threadIdx.x][0];
 }
 
-__global__ void __launch_bounds__(kThreads) bbox_final(const float* __restrict__ part, int nb,
-                                                       float* __restrict__ params, float* __restrict__ mail) {
-#pragma clang fp contract(off)
-    __shared__ float s[6][kThreads];
-    float mx[3] = {-1e6f, -1e6f, -1e6f}, mn[3] = {1e6f, 1e6f, 1e6f};
-    for (int b = threadIdx.x; b < nb; b += kThreads)
-        for (int a = 0; a < 3; ++a) {
-            float v = part[b * 6 + a];
-            mx[a] = mx[a] < v ? v : mx[a];
-            v = part[b * 6 + 3 + a];
-            mn[a] = mn[a] > v ? v : mn[a];
-        }
-    for (int a = 0; a < 3; ++a) {
-        s[a][threadIdx.x] = mx[a];
-        s[3 + a][threadIdx.x] = mn[a];
-    }
-    __syncthreads();
-    for (int st = kThreads / 2; st > 0; st >>= 1) {
-        if (threadIdx.x < st)
-            for (int a = 0; a < 3; ++a) {
-                float l = s[a][threadIdx.x], r = s[a][threadIdx.x + st];
-                s[a][threadIdx.x] = l < r ? r : l;
-                l = s[3 + a][threadIdx.x];
-                r = s[3 + a][threadIdx.x + st];
-                s[3 + a][threadIdx.x] = l > r ? r : l;
-            }
-        __syncthreads();
-    }
-    if (threadIdx.x != 0) return;
-    float maxEdge = 1.0f, center[3];
-    for (int a = 0; a < 3; ++a) {
-        float ext = __fsub_rn(s[a][0], s[3 + a][0]);
-        if (a == 0 || maxEdge < ext) maxEdge = ext;
-        center[a] = __fdiv_rn(__fadd_rn(s[a][0], s[3 + a][0]), 2.0f);
-    }
-    maxEdge = __fmul_rn(maxEdge, 1.25f);
-    for (int a = 0; a < 3; ++a) params[a] = mail[a] = __fsub_rn(center[a], __fdiv_rn(maxEdge, 2.0f));
-    params[3] = mail[3] = maxEdge;
-}
-
 // --- normalise + Morton code (BuildOctree.cu:144-208) ---
-__global__ void __launch_bounds__(kThreads) normalize_encode(const char* __restrict__ xyz, unsigned xs,
-                                                             const char* __restrict__ nrm, unsigned ns, int n,
-                                                             int D, const float* __restrict__ params,
-                                                             float4* __restrict__ posOut,
-                                                             float4* __restrict__ nrmOut,
-                                                             unsigned* __restrict__ keys,
-                                                             unsigned* __restrict__ vals) {
+__device__ __forceinline__ unsigned encode_point(const char* __restrict__ xyz, unsigned xs,
+                                                 const char* __restrict__ nrm, unsigned ns, int i, int D,
+                                                 float4 prm, float4* __restrict__ posOut,
+                                                 float4* __restrict__ nrmOut) {
 #pragma clang fp contract(off)
-    int i = blockIdx.x * kThreads + threadIdx.x;
-    if (i >= n) return;
     const float* p = reinterpret_cast<const float*>(xyz + size_t(i) * xs);
     const float* q = reinterpret_cast<const float*>(nrm + size_t(i) * ns);
-    float edge = params[3];
-    float x = __fdiv_rn(__fsub_rn(p[0], params[0]), edge);
-    float y = __fdiv_rn(__fsub_rn(p[1], params[1]), edge);
-    float z = __fdiv_rn(__fsub_rn(p[2], params[2]), edge);
+    float edge = prm.w;
+    float x = __fdiv_rn(__fsub_rn(p[0], prm.x), edge);
+    float y = __fdiv_rn(__fsub_rn(p[1], prm.y), edge);
+    float z = __fdiv_rn(__fsub_rn(p[2], prm.z), edge);
     float nx = q[0], ny = q[1], nz = q[2];
     float len = __fsqrt_rn(__fadd_rn(__fadd_rn(__fmul_rn(nx, nx), __fmul_rn(ny, ny)), __fmul_rn(nz, nz)));
     if (double(len) > double(1e-6f)) len = __fdiv_rn(1.0f, len);
@@ -123,19 +130,36 @@ __global__ void __launch_bounds__(kThreads) normalize_encode(const char* __restr
         if (z > cz) { code |= 1u << (3 * l); cz += w; } else { cz -= w; }
         w *= 0.5f;
     }
-    keys[i] = code;
-    vals[i] = unsigned(i);
+    return code;
 }
 
-__global__ void __launch_bounds__(kThreads) gather_points(const unsigned* __restrict__ perm,
-                                                          const float4* __restrict__ posIn,
-                                                          const float4* __restrict__ nrmIn, int n,
-                                                          float4* __restrict__ posOut, float4* __restrict__ nrmOut) {
-    int i = blockIdx.x * kThreads + threadIdx.x;
-    if (i >= n) return;
-    unsigned o = perm[i];
-    posOut[i] = posIn[o];
-    nrmOut[i] = nrmIn[o];
+// The normalisation comes from the nb bounding-box partials (block 0 stores
+// it to params and mail).  A block encodes one tile of the sort (two points
+// a thread) and takes the tile's histogram of the sort's first digit
+// (radix_sort_pairs), tiles of them at hist.
+__global__ void __launch_bounds__(kThreads) normalize_encode(const char* __restrict__ xyz, unsigned xs,
+                                                             const char* __restrict__ nrm, unsigned ns, int n,
+                                                             int D, const float* __restrict__ part, int nb,
+                                                             float* __restrict__ params, float* __restrict__ mail,
+                                                             float4* __restrict__ posOut,
+                                                             float4* __restrict__ nrmOut,
+                                                             unsigned* __restrict__ keys,
+                                                             unsigned* __restrict__ vals, int tiles,
+                                                             int* __restrict__ hist) {
+    __shared__ int lh[256];
+    static_assert(kThreads == 256 && kSortTileKeys == 2 * kThreads, "one thread per digit, two keys each");
+    lh[threadIdx.x] = 0;
+    const float4 prm = bbox_reduce(part, nb, params, mail);  // its barriers publish lh
+    for (int k = 0; k < 2; ++k) {
+        const int i = blockIdx.x * kSortTileKeys + k * kThreads + threadIdx.x;
+        if (i >= n) continue;
+        const unsigned code = encode_point(xyz, xs, nrm, ns, i, D, prm, posOut, nrmOut);
+        keys[i] = code;
+        vals[i] = unsigned(i);
+        atomicAdd(&lh[code & 255u], 1);
+    }
+    __syncthreads();
+    hist[threadIdx.x * tiles + blockIdx.x] = lh[threadIdx.x];
 }
 
 // --- level ranks -----------------------------------------------------------
@@ -462,12 +486,11 @@ void build_octree(Ctx& c, const psr_points& pts) {
     hipStream_t s = c.stream;
     o.N = N;
     const unsigned xs = pts.stride_bytes ? pts.stride_bytes : 12, ns = xs;
-    // bbox + normalisation parameters
-    int nbb = std::min<int>(1024, int(div_up(N, kThreads)));
+    // bbox + normalisation parameters (block partials: every encoding block reduces them)
+    const int nbb = std::min<int>(128, int(div_up(N, kThreads)));
     float* part = c.buf<float>("bbox_part", size_t(nbb) * 6);
     o.normParams = c.buf<float>("norm_params", 4);
     bbox_partial<<<nbb, kThreads, 0, s>>>(reinterpret_cast<const char*>(pts.xyz), xs, N, part);
-    bbox_final<<<1, kThreads, 0, s>>>(part, nbb, o.normParams, reinterpret_cast<float*>(c.mail + kMailNorm));
     // normalise + encode
     float4* posU = c.buf<float4>("pos_unsorted", N);
     float4* nrmU = c.buf<float4>("nrm_unsorted", N);
@@ -475,14 +498,17 @@ void build_octree(Ctx& c, const psr_points& pts) {
     unsigned* vals = c.buf<unsigned>("sort_vals", N);
     unsigned* ktmp = c.buf<unsigned>("sort_ktmp", N);
     unsigned* vtmp = c.buf<unsigned>("sort_vtmp", N);
-    normalize_encode<<<div_up(N, kThreads), kThreads, 0, s>>>(
-        reinterpret_cast<const char*>(pts.xyz), xs, reinterpret_cast<const char*>(pts.normal), ns, N, D,
-        o.normParams, posU, nrmU, keys, vals);
-    int* scratch = c.scratch_for(sort_scratch_ints(N));
-    radix_sort_pairs(keys, vals, ktmp, vtmp, N, 3 * D, scratch, c.scan_for(sort_scan_elems(N)), s);
+    // the encoding also takes the sort's first histogram
+    int* sortHist = c.scratch_for(sort_scratch_ints(N));
+    const int sortTiles = int(div_up(N, kSortTileKeys));
+    normalize_encode<<<sortTiles, kThreads, 0, s>>>(
+        reinterpret_cast<const char*>(pts.xyz), xs, reinterpret_cast<const char*>(pts.normal), ns, N, D, part, nbb,
+        o.normParams, reinterpret_cast<float*>(c.mail + kMailNorm), posU, nrmU, keys, vals, sortTiles, sortHist);
+    // sorted by code; the last pass gathers the points into code order
     o.spos = c.buf<float4>("spos", N);
     o.snrm = c.buf<float4>("snrm", N);
-    gather_points<<<div_up(N, kThreads), kThreads, 0, s>>>(vals, posU, nrmU, N, o.spos, o.snrm);
+    radix_sort_pairs(keys, vals, ktmp, vtmp, N, 3 * D, sortHist, true, c.scan_for(sort_scan_elems(N)),
+                     SortGather{posU, nrmU, o.spos, o.snrm}, s);
     o.codes = reinterpret_cast<int*>(keys);
     o.order = reinterpret_cast<int*>(vals);
     // per-level occupied-node counts

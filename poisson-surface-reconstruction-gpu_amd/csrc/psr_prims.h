@@ -47,10 +47,22 @@ void zero_spans(std::initializer_list<ZeroSpan> spans, hipStream_t s);
 // length of the histogram it scans (for the scan state).
 size_t sort_scratch_ints(size_t n);
 size_t sort_scan_elems(size_t n);
+// Tiles of the sort: the first pass's histogram, when its producer takes it
+// (firstHist), holds at scratch[d * tiles + t] the keys of tile t (keys
+// [512 t, 512 t + 512)) whose low 8 bits are d.
+constexpr int kSortTileKeys = 512;
+// Optional gather fused into the last pass: posOut[i] = posIn[v], nrmOut[i] =
+// nrmIn[v] for the value v that lands at i.
+struct SortGather {
+    const float4* posIn;
+    const float4* nrmIn;
+    float4* posOut;
+    float4* nrmOut;
+};
 // Stable ascending sort of keys (only the low `bits` bits are examined) carrying
 // values.  The sorted data ends in (keys, vals): the four buffers are swapped
 // in place of a final copy, so keys/vals may come back as the former ktmp/vtmp.
 void radix_sort_pairs(unsigned*& keys, unsigned*& vals, unsigned*& ktmp, unsigned*& vtmp, int n, int bits,
-                      int* scratch, ScanState& st, hipStream_t s);
+                      int* scratch, bool firstHist, ScanState& st, const SortGather& g, hipStream_t s);
 
 }  // namespace psr

@@ -94,11 +94,13 @@ void scan_exclusive(const int* in, int* out, int n, int* total, ScanState& st, h
 // Stable LSD radix sort, 8-bit digits.
 //   hist:    per tile 256-bin histogram, stored digit-major (digit * tiles + tile)
 //   scan:    exclusive scan of the digit-major histogram -> global offsets
-//   scatter: per tile, stable rank by wave ballots (multisplit) + offsets
+//   scatter: per tile, stable rank by wave ballots (multisplit) + offsets;
+//            the last pass can also gather the points into the sorted order
 // ---------------------------------------------------------------------------
 static constexpr int kSortThreads = 256;
 static constexpr int kSortIters = 2;  // small tiles: enough workgroups to fill the GPU at 200k keys
 static constexpr int kSortTile = kSortThreads * kSortIters;  // 512 keys
+static_assert(kSortTile == kSortTileKeys, "tile of the sort's interface");
 static constexpr int kSortWaves = kSortThreads / PSR_WAVE;
 
 __global__ void __launch_bounds__(kSortThreads) radix_hist(const unsigned* __restrict__ keys, int n, int shift,
@@ -120,7 +122,8 @@ __global__ void __launch_bounds__(kSortThreads) radix_scatter(const unsigned* __
                                                               const unsigned* __restrict__ vin,
                                                               unsigned* __restrict__ kout,
                                                               unsigned* __restrict__ vout, int n, int shift,
-                                                              int tiles, const int* __restrict__ offs) {
+                                                              int tiles, const int* __restrict__ offs,
+                                                              SortGather g) {
     __shared__ int running[256];      // digit count of earlier iterations + global base
     __shared__ int waveCnt[kSortWaves][256];
     const int lane = __lane_id(), wid = threadIdx.x / PSR_WAVE;
@@ -150,6 +153,10 @@ __global__ void __launch_bounds__(kSortThreads) radix_scatter(const unsigned* __
             int pos = running[d] + before + lower;
             kout[pos] = key;
             vout[pos] = val;
+            if (g.posOut) {
+                g.posOut[pos] = g.posIn[val];
+                g.nrmOut[pos] = g.nrmIn[val];
+            }
         }
         __syncthreads();
         int add = 0;
@@ -168,17 +175,19 @@ size_t sort_scratch_ints(size_t n) {
 }
 
 void radix_sort_pairs(unsigned*& keys, unsigned*& vals, unsigned*& ktmp, unsigned*& vtmp, int n, int bits,
-                      int* scratch, ScanState& st, hipStream_t s) {
-    if (n <= 1) return;
+                      int* scratch, bool firstHist, ScanState& st, const SortGather& g, hipStream_t s) {
+    if (n <= 0) return;
     int tiles = int(div_up(n, kSortTile));
     int* hist = scratch;
     int passes = (bits + 7) / 8;
     unsigned *ka = keys, *va = vals, *kb = ktmp, *vb = vtmp;
     for (int p = 0; p < passes; ++p) {
         int shift = 8 * p;
-        radix_hist<<<tiles, kSortThreads, 0, s>>>(ka, n, shift, tiles, hist);
+        if (p > 0 || !firstHist) radix_hist<<<tiles, kSortThreads, 0, s>>>(ka, n, shift, tiles, hist);
         scan_exclusive(hist, hist, 256 * tiles, nullptr, st, s);
-        radix_scatter<<<tiles, kSortThreads, 0, s>>>(ka, va, kb, vb, n, shift, tiles, hist);
+        const SortGather none{};
+        radix_scatter<<<tiles, kSortThreads, 0, s>>>(ka, va, kb, vb, n, shift, tiles, hist,
+                                                     p == passes - 1 ? g : none);
         std::swap(ka, kb);
         std::swap(va, vb);
     }
