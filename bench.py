@@ -44,6 +44,7 @@ def parse():
     ap.add_argument("--batch-reps", type=int, default=3,
                     help="timed repetitions of the 64-cloud batch, sequential and batched (0: skip)")
     ap.add_argument("--batch-workers", type=int, default=8, help="clouds in flight in the batched form")
+    ap.add_argument("--weld-reps", type=int, default=20, help="timed welds of the workload's mesh (0: skip)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain elementwise copy of MIB MiB three times (HBM counter calibration)")
     return ap.parse_args()
@@ -148,6 +149,24 @@ def measure(pkg, tx, tn, depth, precision, steps, warmup):
             "cg_bytes": float(cg_algorithmic_bytes(st, precision)), "stats": st}
 
 
+def measure_weld(pkg, tx, tn, depth, reps):
+    """Times psr_weld_mesh on the workload's mixed-precision mesh (the call
+    returns with the welded counts, so each timing includes its wait)."""
+    import torch
+    rec = pkg.PoissonReconstruction(depth=depth, precision=pkg.binding.PRECISION_MIXED)
+    mesh = rec.SolvePoissionReconstructionMesh(tx, tn)
+    nv, nt = mesh.n_vertices, mesh.n_triangles
+    rec.WeldMesh()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        w = rec.WeldMesh()
+    dt = time.perf_counter() - t0
+    out = {"ms": dt * 1e3 / reps, "vertices": nv, "welded_vertices": w.n_vertices, "triangles": nt, "reps": reps}
+    rec.close()
+    return out
+
+
 def measure_batch(pkg, reps, workers):
     """The 64-cloud batch (sizes log-uniform in [5k, 150k], ellipsoids and tori,
     depth 7): 64 sequential single-cloud calls against one batched call."""
@@ -236,6 +255,8 @@ def main():
         line["fp32"] = {"value": f["value"], "ms_per_step": f["ms_per_step"], "cg_ms": f["cg_ms"],
                         "cg_iterations": f["stats"]["cg_iterations"]}
         line["mixed_vs_fp32"] = h["value"] / f["value"]
+    if a.weld_reps > 0:
+        line["weld"] = measure_weld(pkg, tx, tn, depth, a.weld_reps)
     if a.batch_reps > 0:
         line["batch64"] = measure_batch(pkg, a.batch_reps, a.batch_workers)
     if a.cpu_sample > 0:

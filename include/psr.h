@@ -126,6 +126,10 @@ enum {
     PSR_ARRAY_TABLE_DDOT,
     PSR_ARRAY_TABLE_D2DOT,
     PSR_ARRAY_CUBE,             /* int32 [nD] depth-D cube categories */
+    PSR_ARRAY_MESH_VERTEX_KEYS, /* uint32 [n_vertices]: each vertex's depth-D edge, ((x S + y) S + z) 3 + axis,
+                                   S = 2^D + 1, (x, y, z) the edge's lower grid corner */
+    PSR_ARRAY_WELDED_VERTICES,  /* float [welded n_vertices][3], after psr_weld_mesh */
+    PSR_ARRAY_WELDED_TRIANGLES, /* int32 [n_triangles][3], after psr_weld_mesh */
     PSR_ARRAY_COUNT_
 };
 
@@ -151,6 +155,19 @@ int psr_get_stats(psr_ctx* ctx, psr_stats* out);
 /* Copies array `array_id` of the last reconstruction to host memory.  With
  * host_dst == NULL only *needed (bytes) is written. */
 int psr_export(psr_ctx* ctx, int array_id, void* host_dst, uint64_t bytes, uint64_t* needed);
+/* Welds the mesh of the last psr_reconstruct on ctx: one vertex per depth-D
+ * edge (the first of the vertices carrying that edge's key, in ascending key
+ * order), triangle corners remapped (a corner index outside the mesh becomes
+ * -1).  out receives device buffers owned by the context, valid until the
+ * next reconstruction.  No counterpart in the reference, which leaves the
+ * duplicates of its marching-cubes batches in place
+ * (mesh/ComputeTriangleIndices.cu:1247-1282). */
+int psr_weld_mesh(psr_ctx* ctx, psr_mesh* out);
+/* Writes a binary little-endian PLY (float x y z; uchar-counted int32 face
+ * lists) from HOST arrays.  No counterpart in the reference, which renders
+ * the mesh (mesh/DrawMesh.cpp) and writes no file. */
+int psr_write_ply(const char* path, const float* vertices, uint32_t n_vertices, const int32_t* triangles,
+                  uint32_t n_triangles);
 void psr_destroy(psr_ctx* ctx);
 const char* psr_status_string(int status);
 const char* psr_last_error(const psr_ctx* ctx);

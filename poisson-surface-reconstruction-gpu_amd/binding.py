@@ -22,7 +22,7 @@ ARRAYS = ["SORTED_POS", "SORTED_NRM", "CODES", "ORDER", "POINT2NODE", "NODE_KEY"
           "NODE_PARENT", "NODE_CHILDREN", "NODE_NEIGHS", "NODE_DIDX", "NODE_DNUM", "NODE_ENCIDX", "LEVEL_BASE",
           "LEVEL_COUNT", "NORM_PARAMS", "VFIELD", "DIVERGENCE", "DX", "CHI_POINTS", "VERTS", "EDGES", "FACES",
           "NODE_VERTICES", "NODE_EDGES", "NODE_FACES", "VVALUE", "MESH_VERTICES", "MESH_TRIANGLES", "TABLE_DOT",
-          "TABLE_DDOT", "TABLE_D2DOT", "CUBE"]
+          "TABLE_DDOT", "TABLE_D2DOT", "CUBE", "MESH_VERTEX_KEYS", "WELDED_VERTICES", "WELDED_TRIANGLES"]
 ARRAY_ID = {n: i for i, n in enumerate(ARRAYS)}
 _ARRAY_LAYOUT = {
     "SORTED_POS": (np.float32, 4), "SORTED_NRM": (np.float32, 4), "NODE_CHILDREN": (np.int32, 8),
@@ -31,11 +31,12 @@ _ARRAY_LAYOUT = {
     "EDGES": (np.int32, 2), "FACES": (np.int32, 3), "NODE_VERTICES": (np.int32, 8), "NODE_EDGES": (np.int32, 12),
     "NODE_FACES": (np.int32, 6), "VVALUE": (np.float32, 1), "MESH_VERTICES": (np.float32, 3),
     "MESH_TRIANGLES": (np.int32, 3), "TABLE_DOT": (np.float64, 1), "TABLE_DDOT": (np.float64, 1),
-    "TABLE_D2DOT": (np.float64, 1)}
+    "TABLE_D2DOT": (np.float64, 1), "MESH_VERTEX_KEYS": (np.uint32, 1), "WELDED_VERTICES": (np.float32, 3),
+    "WELDED_TRIANGLES": (np.int32, 3)}
 
 EXPORTED_SYMBOLS = ["psr_default_config", "psr_create", "psr_reconstruct", "psr_reconstruct_batch",
                     "psr_get_batch_stats", "psr_sync", "psr_get_stats", "psr_export", "psr_destroy",
-                    "psr_status_string", "psr_last_error"]
+                    "psr_status_string", "psr_last_error", "psr_weld_mesh", "psr_write_ply"]
 
 
 class Config(ctypes.Structure):
@@ -126,6 +127,9 @@ def lib():
         L.psr_export.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64,
                                  ctypes.POINTER(ctypes.c_uint64)]
         L.psr_destroy.argtypes = [ctypes.c_void_p]
+        L.psr_weld_mesh.argtypes = [ctypes.c_void_p, ctypes.POINTER(Mesh)]
+        L.psr_write_ply.argtypes = [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
+                                    ctypes.c_uint32]
         L.psr_status_string.restype = ctypes.c_char_p
         L.psr_status_string.argtypes = [ctypes.c_int]
         L.psr_last_error.restype = ctypes.c_char_p
@@ -238,8 +242,48 @@ class PoissonReconstruction:
             raise PsrError(rc, "export %s" % name)
         return out.reshape(-1, width) if width > 1 else out
 
+    def WeldMesh(self):
+        """Welds the last mesh (one vertex per depth-D edge) on the device and
+        returns its psr_mesh (device buffers valid until the next reconstruction)."""
+        welded = Mesh()
+        rc = lib().psr_weld_mesh(self._ctx, ctypes.byref(welded))
+        if rc != PSR_OK:
+            raise PsrError(rc, lib().psr_last_error(self._ctx).decode())
+        return welded
+
     def GetRebuildMeshVertices(self):
         return self.export("MESH_VERTICES")
 
     def GetRebuildMeshTriangleIndices(self):
         return self.export("MESH_TRIANGLES")
+
+
+def write_ply(path, vertices, triangles):
+    """Writes host arrays (float32 (V, 3), int32 (T, 3)) as a binary PLY."""
+    v = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, 3)
+    t = np.ascontiguousarray(triangles, dtype=np.int32).reshape(-1, 3)
+    rc = lib().psr_write_ply(os.fsencode(path), v.ctypes.data, v.shape[0], t.ctypes.data, t.shape[0])
+    if rc != PSR_OK:
+        raise PsrError(rc, "write_ply %s" % path)
+
+
+def read_ply(path):
+    """Reads a binary little-endian PLY with float x y z vertices and
+    uchar-counted int32 triangle lists (the layout write_ply produces)."""
+    with open(path, "rb") as f:
+        data = f.read()
+    end = data.index(b"end_header\n") + len(b"end_header\n")
+    header = data[:end].decode("ascii").splitlines()
+    nv = nt = 0
+    for line in header:
+        w = line.split()
+        if w[:2] == ["element", "vertex"]:
+            nv = int(w[2])
+        elif w[:2] == ["element", "face"]:
+            nt = int(w[2])
+    v = np.frombuffer(data, np.float32, 3 * nv, end).reshape(nv, 3)
+    rec = np.frombuffer(data, np.uint8, 13 * nt, end + 12 * nv).reshape(nt, 13)
+    if nt and not np.all(rec[:, 0] == 3):
+        raise ValueError("non-triangle face")
+    t = rec[:, 1:].copy().view(np.int32).reshape(nt, 3)
+    return v, t

@@ -96,6 +96,8 @@ int run(psr_ctx* ctx, const psr_points* pts, psr_mesh* out) {
     if (pts->stride_bytes && (pts->stride_bytes < 12 || pts->stride_bytes % 4))
         return fail(ctx, PSR_ERR_INVALID, "bad stride");
     std::memset(&c.stats, 0, sizeof(c.stats));
+    c.mesh.valid = false;
+    c.mesh.weldedNv = c.mesh.weldedNt = 0;
     PSR_HIP_CHECK(hipEventRecord(c.ev[0], s));
     build_octree(c, *pts);
     PSR_HIP_CHECK(hipEventRecord(c.ev[1], s));
@@ -143,6 +145,7 @@ int run(psr_ctx* ctx, const psr_points* pts, psr_mesh* out) {
     out->triangles = c.mesh.triangles;
     out->n_vertices = uint32_t(c.mesh.nv);
     out->n_triangles = uint32_t(c.mesh.nt);
+    c.mesh.valid = true;
     return PSR_OK;
 }
 
@@ -229,6 +232,24 @@ int psr_reconstruct(psr_ctx* ctx, const psr_points* points, psr_mesh* out) {
     } catch (const std::exception& e) {
         return fail(ctx, PSR_ERR_INTERNAL, e.what());
     }
+}
+
+int psr_weld_mesh(psr_ctx* ctx, psr_mesh* out) {
+    if (!ctx || !out) return PSR_ERR_INVALID;
+    psr::Ctx& c = ctx->c;
+    if (!c.mesh.valid) return fail(ctx, PSR_ERR_INVALID, "no completed reconstruction to weld");
+    try {
+        psr::weld_mesh(c);
+    } catch (const psr::HipError& e) {
+        return fail(ctx, PSR_ERR_HIP, e.what());
+    } catch (const std::exception& e) {
+        return fail(ctx, PSR_ERR_INTERNAL, e.what());
+    }
+    out->vertices = c.mesh.weldedVertices;
+    out->triangles = c.mesh.weldedTriangles;
+    out->n_vertices = uint32_t(c.mesh.weldedNv);
+    out->n_triangles = uint32_t(c.mesh.weldedNt);
+    return PSR_OK;
 }
 
 int psr_reconstruct_batch(psr_ctx* ctx, const psr_points* clouds, uint32_t count, psr_mesh* outs) {
@@ -453,6 +474,9 @@ int psr_export(psr_ctx* ctx, int id, void* dst, uint64_t bytes, uint64_t* needed
         case PSR_ARRAY_VVALUE: src = {c.geo.vvalue, size_t(4) * c.geo.V, 0}; break;
         case PSR_ARRAY_MESH_VERTICES: src = {c.mesh.vertices, size_t(12) * c.mesh.nv, 0}; break;
         case PSR_ARRAY_MESH_TRIANGLES: src = {c.mesh.triangles, size_t(12) * c.mesh.nt, 0}; break;
+        case PSR_ARRAY_MESH_VERTEX_KEYS: src = {c.mesh.keys, size_t(4) * c.mesh.nv, 0}; break;
+        case PSR_ARRAY_WELDED_VERTICES: src = {c.mesh.weldedVertices, size_t(12) * c.mesh.weldedNv, 0}; break;
+        case PSR_ARRAY_WELDED_TRIANGLES: src = {c.mesh.weldedTriangles, size_t(12) * c.mesh.weldedNt, 0}; break;
         case PSR_ARRAY_TABLE_DOT: src = {c.tables.dot, size_t(8) * c.tables.res * c.tables.res, 0}; break;
         case PSR_ARRAY_TABLE_DDOT: src = {c.tables.ddot, size_t(8) * c.tables.res * c.tables.res, 0}; break;
         case PSR_ARRAY_TABLE_D2DOT: src = {c.tables.d2dot, size_t(8) * c.tables.res * c.tables.res, 0}; break;

@@ -115,7 +115,13 @@ inline size_t dense_grid_cells(int D) { return ((size_t(1) << (3 * (D + 1))) - 1
 struct Mesh {
     float* vertices = nullptr;  // xyz
     int* triangles = nullptr;
+    unsigned* keys = nullptr;   // per vertex: the global key of its depth-D edge (segment_key)
     int nv = 0, nt = 0;
+    bool valid = false;         // the last reconstruction on this context completed
+    // the welded form (weld_mesh): one vertex per depth-D edge
+    float* weldedVertices = nullptr;
+    int* weldedTriangles = nullptr;
+    int weldedNv = 0, weldedNt = 0;
     // Counts still in the mailbox when extraction queued its last kernels without
     // waiting: kept vertices = slot 3 + slot 1, triangles = triBefore + slot 2,
     // each within its allocated bound.
@@ -243,5 +249,7 @@ void build_geometry(Ctx& c);
 void extract_mesh(Ctx& c);
 // Mesh counts of a deferred extraction, read from the mailbox after the final wait.
 void finish_mesh(Ctx& c);
+// Welds the last mesh into mesh.welded*; returns once the counts are known.
+void weld_mesh(Ctx& c);
 
 }  // namespace psr

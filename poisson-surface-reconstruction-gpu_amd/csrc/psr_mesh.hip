@@ -364,7 +364,7 @@ __global__ void __launch_bounds__(kThreads) edge_crossings(int offD, int nD, int
                                                            const float* __restrict__ vv,
                                                            const float4* __restrict__ vPos, ScanTicket t,
                                                            int* __restrict__ flag, int* __restrict__ addr,
-                                                           float* __restrict__ out) {
+                                                           float* __restrict__ out, int D, unsigned* __restrict__ key) {
     const int tile = ticket_tile(t);
     const int n = tile * kThreads + threadIdx.x, o = offD + n;
     const int2 eo = n < nD ? edgeOf[n] : make_int2(0, 0);
@@ -392,6 +392,10 @@ __global__ void __launch_bounds__(kThreads) edge_crossings(int offD, int nD, int
         out[3 * c] = p.x;
         out[3 * c + 1] = p.y;
         out[3 * c + 2] = p.z;
+        // corner positions are exact multiples of 2^-D
+        const float sc = float(1 << D);
+        key[c] = segment_key(make_int3(int(p1.x * sc), int(p1.y * sc), int(p1.z * sc)),
+                             make_int3(int(p2.x * sc), int(p2.y * sc), int(p2.z * sc)), D);
         ++c;
     }
 }
@@ -563,7 +567,7 @@ void build_geometry(Ctx& c) {
     PSR_HIP_CHECK(hipGetLastError());
 }
 
-void depth_mc(Ctx& c, MainMc& mm, float* rawV, int* rawT) {
+void depth_mc(Ctx& c, MainMc& mm, float* rawV, unsigned* rawKey, int* rawT) {
     Octree& o = c.oct;
     Geometry& g = c.geo;
     const Levels& L = o.L;
@@ -580,7 +584,8 @@ void depth_mc(Ctx& c, MainMc& mm, float* rawV, int* rawT) {
     const int tilesD = div_up(nD, kThreads), tilesR = div_up(offD, kThreads);
     ScanState& scan = c.scan_for_tiles(size_t(std::max(tilesD, tilesR)));
     edge_crossings<<<tilesD, kThreads, 0, s>>>(offD, nD, M, g.edgeOf, g.nodeVert, g.vvalue, g.vtxPos,
-                                               scan_ticket(scan, tilesD, c.mail + 0), cflag, mm.vexAddr, rawV);
+                                               scan_ticket(scan, tilesD, c.mail + 0), cflag, mm.vexAddr, rawV, L.D,
+                                               rawKey);
     cube_triangles<<<tilesD, kThreads, 0, s>>>(offD, nD, M, g.nodeVert, g.nodeEdge, cflag, g.vvalue, mm.vexAddr,
                                                g.nodeFace, o.parent, g.faceHasParent, o.q1Fired,
                                                scan_ticket(scan, tilesD, c.mail + 1), mm.cube, rawT, mm.hasInter,

@@ -140,9 +140,20 @@ __device__ __forceinline__ size_t dense_grid_index(int key, int l, int D) {
     return ((size_t(1) << (3 * l)) - 1) / 7 + ((size_t(c.x) << (2 * l)) | (size_t(c.y) << l) | size_t(c.z));
 }
 
+// Global key of a depth-D grid edge, carried by every mesh vertex for
+// welding: lower endpoint (x, y, z) in [0, 2^D]^3 and axis a (0 x, 1 y, 2 z)
+// give ((x (2^D + 1) + y) (2^D + 1) + z) 3 + a, below 3 (2^D + 1)^3 < 2^32
+// for D <= 10.
+__device__ __forceinline__ unsigned segment_key(int3 p1, int3 p2, int D) {
+    const unsigned S = (1u << D) + 1u;
+    const int axis = p1.x != p2.x ? 0 : (p1.y != p2.y ? 1 : 2);
+    return ((unsigned(min(p1.x, p2.x)) * S + unsigned(min(p1.y, p2.y))) * S + unsigned(min(p1.z, p2.z))) * 3u +
+           unsigned(axis);
+}
+
 // Depth-D marching cubes into the raw mesh buffers (vertices at [0, nCross),
 // triangles at [0, nTri)), then the subdivision roots.  Ends with one wait
 // for the counts.
-void depth_mc(Ctx& c, MainMc& mm, float* rawV, int* rawT);
+void depth_mc(Ctx& c, MainMc& mm, float* rawV, unsigned* rawKey, int* rawT);
 
 }  // namespace psr

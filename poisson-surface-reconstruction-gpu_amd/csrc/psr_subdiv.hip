@@ -1306,7 +1306,9 @@ __global__ void __launch_bounds__(kThreads) sub_emit(Grid g, const Tile* __restr
                                                      const int* __restrict__ crossPrefix,
                                                      const int4* __restrict__ tileCnt,
                                                      const int4* __restrict__ tileBase,
-                                                     const int* __restrict__ keptMain, int keptBefore, int triBefore, float* __restrict__ outV, int* __restrict__ outT) {
+                                                     const int* __restrict__ keptMain, int keptBefore, int triBefore,
+                                                     float* __restrict__ outV, int* __restrict__ outT,
+                                                     unsigned* __restrict__ outKey) {
     __shared__ int lSum[kThreads / PSR_WAVE];
     __shared__ TileRoot lRoots[kMaxRootsPerTile];
     __shared__ int lNbr[kMaxRootsPerTile * 27];
@@ -1346,6 +1348,9 @@ __global__ void __launch_bounds__(kThreads) sub_emit(Grid g, const Tile* __restr
             outV[3 * a] = p.x;
             outV[3 * a + 1] = p.y;
             outV[3 * a + 2] = p.z;
+            const int3 o1 = bourke_offset(kEdgeVertex[k][0]), o2 = bourke_offset(kEdgeVertex[k][1]);
+            outKey[a] = segment_key(make_int3(L.c.x + o1.x, L.c.y + o1.y, L.c.z + o1.z),
+                                    make_int3(L.c.x + o2.x, L.c.y + o2.y, L.c.z + o2.z), g.D);
             ++a;
         }
     }
@@ -1444,9 +1449,11 @@ __global__ void __launch_bounds__(kThreads) zero_floats(float* __restrict__ a, s
 // vertices are compacted in order in the same launch; their count goes to t.total.
 struct KeepArgs {
     const float* v;
+    const unsigned* key;
     int n;
     ScanTicket t;
     float* out;
+    unsigned* outKey;
 };
 __device__ __forceinline__ void keep_compact(const KeepArgs& ka) {
     const float* __restrict__ v = ka.v;
@@ -1470,6 +1477,7 @@ __device__ __forceinline__ void keep_compact(const KeepArgs& ka) {
     out[3 * size_t(a)] = x;
     out[3 * size_t(a) + 1] = y;
     out[3 * size_t(a) + 2] = z;
+    ka.outKey[a] = ka.key[i];
 }
 
 // The chunk's sets, read from mapped host memory on the compute queue (a copy
@@ -1548,10 +1556,11 @@ void extract_mesh(Ctx& c) {
     // ---- depth-D marching cubes: the first output batch; its triangles are
     // written in place (raw crossing ids: batch start 0, nothing kept before it)
     float* mainV = c.buf<float>("raw_main_v", size_t(36) * nD);  // at most one crossing per depth-D edge
+    unsigned* mainKey = c.buf<unsigned>("raw_main_key", size_t(12) * nD);
     int* outT = c.buf_keep<int>("mesh_triangles", size_t(15) * nD, 0);
     MainMc mm;
     c.mesh.deferred = false;
-    depth_mc(c, mm, mainV, outT);  // ends with a wait: no kernel is in flight
+    depth_mc(c, mm, mainV, mainKey, outT);  // ends with a wait: no kernel is in flight
     c.stats.n_subdivision_roots = mm.nRoots;
     // ---- subdivision sets: coarse roots one per set, finer roots one set per depth
     // (ComputeTriangleIndices.h:371: roots above depth D-4 are coarse)
@@ -1605,10 +1614,11 @@ void extract_mesh(Ctx& c) {
     int* keptMain = nSets > 0 ? c.buf<int>("sub_kept_main", 1) : c.mail + kMailKept;
     size_t triTotal = mm.nTri, keptSub = 0, keptMainHost = 0;
     float* outV = c.buf_keep<float>("mesh_vertices", 3 * (deferred ? vCap : size_t(std::max(mm.nCross, 1))), 0);
+    unsigned* outKey = c.buf_keep<unsigned>("mesh_vertex_keys", deferred ? vCap : size_t(std::max(mm.nCross, 1)), 0);
     // the depth-D batch's kept vertices: compacted by the first set-up launch
     const int keepBlocks = std::max(1, int(div_up(mm.nCross, kThreads)));
-    const KeepArgs ka{mainV, mm.nCross,
-                      scan_ticket(c.scan_for_tiles(size_t(keepBlocks)), keepBlocks, keptMain), outV};
+    const KeepArgs ka{mainV, mainKey, mm.nCross,
+                      scan_ticket(c.scan_for_tiles(size_t(keepBlocks)), keepBlocks, keptMain), outV, outKey};
     if (nSets == 0) sub_setup<<<keepBlocks, kThreads, 0, st>>>(ka, keepBlocks, RootRecArgs{}, 0, TileArgs{});
     if (deferred && nSets > 0) outT = c.buf_keep<int>("mesh_triangles", 3 * tCap, 3 * size_t(mm.nTri));
     if (nSets > 0) {
@@ -1730,7 +1740,7 @@ void extract_mesh(Ctx& c) {
             sub_tile_scan<<<1, kScanThreads, 0, st>>>(tileCnt, nTiles, tileBase, keptMain, c.mail);
             if (deferred) {
                 sub_emit<<<nTiles, kThreads, 0, st>>>(gr, dtiles, nVD, vals, cellWord, crossPre, tileCnt, tileBase,
-                                                      keptMain, 0, mm.nTri, outV, outT);
+                                                      keptMain, 0, mm.nTri, outV, outT, outKey);
                 s0 = s1;
                 continue;
             }
@@ -1741,10 +1751,11 @@ void extract_mesh(Ctx& c) {
             if (keptTotal + nKept > size_t(INT32_MAX) || triTotal + nTri > size_t(INT32_MAX) / 3)
                 throw std::runtime_error("mesh exceeds 32-bit indexing");
             outV = c.buf_keep<float>("mesh_vertices", 3 * (keptTotal + nKept), 3 * keptTotal);
+            outKey = c.buf_keep<unsigned>("mesh_vertex_keys", keptTotal + nKept, keptTotal);
             outT = c.buf_keep<int>("mesh_triangles", 3 * (triTotal + nTri), 3 * triTotal);
             if (nKept > 0 || nTri > 0)
                 sub_emit<<<nTiles, kThreads, 0, st>>>(gr, dtiles, nVD, vals, cellWord, crossPre, tileCnt, tileBase,
-                                                      keptMain, int(keptSub), int(triTotal), outV, outT);
+                                                      keptMain, int(keptSub), int(triTotal), outV, outT, outKey);
             keptSub += nKept;
             triTotal += nTri;
             s0 = s1;
@@ -1752,6 +1763,7 @@ void extract_mesh(Ctx& c) {
     }
     c.mesh.vertices = outV;
     c.mesh.triangles = outT;
+    c.mesh.keys = outKey;
     PSR_HIP_CHECK(hipGetLastError());
     if (deferred) {
         c.mesh.deferred = true;
