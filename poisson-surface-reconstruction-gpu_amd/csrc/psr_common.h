@@ -161,4 +161,48 @@ __device__ __forceinline__ float chi_node_sum_regular(int node, float4 b, float3
     return v;
 }
 
+// Implicit function at the sorted points (LaplacianSolver.cu:269-305): one
+// thread per point sums its leaf's and every ancestor's 27 neighbour terms;
+// each block of kPointsChiThreads points leaves the FP64 sum of its values in
+// part[block] for the iso value.
+constexpr int kPointsChiThreads = 256;
+struct PointsChiArgs {
+    int N, offD, M, D;
+    const float4* pos;
+    const int* p2n;
+    const int* parent;
+    const float4* nbrCol;
+    const float4* basis;
+    const int* q1Fired;
+    float* chi;
+    double* part;
+};
+__device__ __forceinline__ void points_chi_block(const PointsChiArgs& a, int block) {
+    __shared__ double sh[kPointsChiThreads / PSR_WAVE];
+    const int i = block * kPointsChiThreads + int(threadIdx.x);
+    double mine = 0;
+    const bool regular = *a.q1Fired == 0;
+    if (i < a.N) {
+        const float4 q = a.pos[i];
+        const float3 p = make_float3(q.x, q.y, q.z);
+        int now = a.offD + a.p2n[i];
+        float v = 0.f;
+        for (int d = a.D; now != -1; --d) {  // a parent is one level up
+            v += regular ? chi_node_sum_regular(now, a.basis[now], p, a.M, a.nbrCol)
+                         : chi_node_sum(now, float(1 << d), p, a.M, a.nbrCol);
+            now = a.parent[now];
+        }
+        a.chi[i] = v;
+        mine = double(v);
+    }
+    for (int off = PSR_WAVE / 2; off > 0; off >>= 1) mine += __shfl_xor(mine, off, PSR_WAVE);
+    if (__lane_id() == 0) sh[threadIdx.x / PSR_WAVE] = mine;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0;
+        for (int w = 0; w < kPointsChiThreads / PSR_WAVE; ++w) s += sh[w];
+        a.part[block] = s;
+    }
+}
+
 }  // namespace psr

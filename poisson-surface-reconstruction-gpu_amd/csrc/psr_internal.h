@@ -85,6 +85,8 @@ struct Field {
     float4* nbrCol = nullptr;  // [27][M]: centre (x, y, z) and d_x of neighbour i of each node (0 where absent)
     double* isoSum = nullptr; // device scalar
     float iso = 0.f;
+    PointsChiArgs chiArgs{};  // the points' implicit values, launched with the mesh stage's first kernel
+    int chiBlocks = 0;
 };
 
 struct Geometry {

@@ -101,12 +101,39 @@ __device__ __forceinline__ void regular_owned(const int (&kk)[27], bool depthD, 
     }
 }
 
+// The iso value: the mean implicit value over the points (LaplacianSolver.cu:269-305),
+// from the per-block sums of points_chi_block, in one fixed order.
+__global__ void __launch_bounds__(1024) iso_final(const double* __restrict__ part, int nb, int N,
+                                                   double* __restrict__ isoSum, float* __restrict__ iso) {
+    __shared__ double sh[1024 / PSR_WAVE];
+    double s = 0;
+    for (int k = threadIdx.x; k < nb; k += 1024) s += part[k];
+    for (int off = PSR_WAVE / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, PSR_WAVE);
+    if (__lane_id() == 0) sh[threadIdx.x / PSR_WAVE] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int w = 0; w < 1024 / PSR_WAVE; ++w) t += sh[w];
+        *isoSum = t;
+        *iso = float(t / double(N));
+    }
+}
+
 // dense: the dense d_x grids (or null), where a regular octree's nodes put dx.
+// Blocks from geoBlocks on compute the points' implicit values instead
+// (points_chi_block): independent work that fills the GPU beside this
+// kernel's few workgroups.
 __global__ void __launch_bounds__(kThreads) geometry_emit(NodeView v, int offD, int nD,
                                                           const int* __restrict__ parent,
                                                           const int* __restrict__ q1Fired, ScanTicket tv,
                                                           ScanTicket te, ScanTicket tf, GeoOut out,
-                                                          const float* __restrict__ dx, float* __restrict__ dense) {
+                                                          const float* __restrict__ dx, float* __restrict__ dense,
+                                                          int geoBlocks, PointsChiArgs pc) {
+    static_assert(kThreads == kPointsChiThreads, "one block size for both roles");
+    if (int(blockIdx.x) >= geoBlocks) {
+        points_chi_block(pc, int(blockIdx.x) - geoBlocks);
+        return;
+    }
     const int tile = ticket_tile(tv);
     const int o = tile * kThreads + threadIdx.x;
     unsigned vm = 0, em = 0, fm = 0;
@@ -551,8 +578,9 @@ void build_geometry(Ctx& c) {
     g.edgeOf = c.buf<int2>("edge_of", nD);
     const GeoOut out{g.vtxOwner, g.vtxKind,  g.vtxPos,   g.nodeVert,      g.edgeOwner, g.edgeKind,
                      g.nodeEdge, g.faceOwner, g.faceKind, g.faceHasParent, g.nodeFace,  g.edgeOf};
-    geometry_emit<<<tilesM, kThreads, 0, s>>>(v, offD, nD, o.parent, o.q1Fired, tv, te, tf, out, c.fld.dx,
-                                              c.geo.dxGrid);
+    geometry_emit<<<tilesM + c.fld.chiBlocks, kThreads, 0, s>>>(v, offD, nD, o.parent, o.q1Fired, tv, te, tf, out,
+                                                                c.fld.dx, c.geo.dxGrid, tilesM, c.fld.chiArgs);
+    iso_final<<<1, 1024, 0, s>>>(c.fld.chiArgs.part, c.fld.chiBlocks, o.N, c.fld.isoSum, &c.results()->iso);
     // vertex implicit values
     float* isoDev = &c.results()->iso;
     g.vvalue = c.buf<float>("vvalue", size_t(8) * M);

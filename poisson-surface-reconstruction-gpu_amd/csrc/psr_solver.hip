@@ -915,56 +915,6 @@ __global__ void __launch_bounds__(kThreads) neighbour_columns(int M, const int* 
     }
 }
 
-__global__ void __launch_bounds__(kThreads) points_chi_kernel(int N, int offD, int M, int D,
-                                                              const float4* __restrict__ pos,
-                                                              const int* __restrict__ p2n,
-                                                              const int* __restrict__ parent,
-                                                              const float4* __restrict__ nbrCol,
-                                                              const float4* __restrict__ basis,
-                                                              const int* __restrict__ q1Fired,
-                                                              float* __restrict__ chi, double* __restrict__ part) {
-    __shared__ double sh[kThreads / PSR_WAVE];
-    int i = blockIdx.x * kThreads + threadIdx.x;
-    double mine = 0;
-    const bool regular = *q1Fired == 0;
-    if (i < N) {
-        float4 q = pos[i];
-        float3 p = make_float3(q.x, q.y, q.z);
-        int now = offD + p2n[i];
-        float v = 0.f;
-        for (int d = D; now != -1; --d) {  // a parent is one level up
-            v += regular ? chi_node_sum_regular(now, basis[now], p, M, nbrCol)
-                         : chi_node_sum(now, float(1 << d), p, M, nbrCol);
-            now = parent[now];
-        }
-        chi[i] = v;
-        mine = double(v);
-    }
-    for (int off = PSR_WAVE / 2; off > 0; off >>= 1) mine += __shfl_xor(mine, off, PSR_WAVE);
-    if (__lane_id() == 0) sh[threadIdx.x / PSR_WAVE] = mine;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double s = 0;
-        for (int w = 0; w < kThreads / PSR_WAVE; ++w) s += sh[w];
-        part[blockIdx.x] = s;
-    }
-}
-
-__global__ void __launch_bounds__(1024) iso_final(const double* __restrict__ part, int nb, int N,
-                                                   double* __restrict__ isoSum, float* __restrict__ iso) {
-    __shared__ double sh[1024 / PSR_WAVE];
-    double s = 0;
-    for (int k = threadIdx.x; k < nb; k += 1024) s += part[k];
-    for (int off = PSR_WAVE / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, PSR_WAVE);
-    if (__lane_id() == 0) sh[threadIdx.x / PSR_WAVE] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double t = 0;
-        for (int w = 0; w < 1024 / PSR_WAVE; ++w) t += sh[w];
-        *isoSum = t;
-        *iso = float(t / double(N));
-    }
-}
 
 // stored (non-dropped) operator entries per depth
 __global__ void zero_u32(unsigned* a, int n) {
@@ -1141,11 +1091,10 @@ void points_implicit(Ctx& c) {
     Octree& o = c.oct;
     const Levels& L = o.L;
     hipStream_t s = c.stream;
-    int N = o.N, nb = int(div_up(N, kThreads));
+    int N = o.N, nb = int(div_up(N, kPointsChiThreads));
     c.fld.chiPts = c.buf<float>("chi_points", N);
     double* part = c.buf<double>("iso_part", nb);
     c.fld.isoSum = c.buf<double>("iso_sum", 1);
-    float* isoDev = &c.results()->iso;
     c.fld.nbrCol = c.buf<float4>("nbr_col", size_t(27) * o.M);
     // the mesh stage's node tables and face marks, cleared with the columns:
     // nodeVert [8][M], nodeFace [6][M], nodeEdge [12][n_D], face marks (6M, the face bound)
@@ -1162,9 +1111,11 @@ void points_implicit(Ctx& c) {
     neighbour_columns<<<div_up(size_t(27) * o.M, kThreads), kThreads, 0, s>>>(
         o.M, o.neigh, o.basis, c.fld.dx, c.fld.nbrCol, reinterpret_cast<unsigned*>(zeroed), nZero, c.geo.dxGrid,
         nDense, o.q1Fired);
-    points_chi_kernel<<<nb, kThreads, 0, s>>>(N, L.base[L.D], o.M, L.D, o.spos, o.point2node, o.parent,
-                                              c.fld.nbrCol, o.basis, o.q1Fired, c.fld.chiPts, part);
-    iso_final<<<1, 1024, 0, s>>>(part, nb, N, c.fld.isoSum, isoDev);
+    // the implicit function at the points runs beside the mesh stage's first
+    // kernel (build_geometry), then the iso value
+    c.fld.chiArgs = PointsChiArgs{N,        L.base[L.D], o.M,          L.D,      o.spos,        o.point2node,
+                                  o.parent, c.fld.nbrCol, o.basis, o.q1Fired, c.fld.chiPts, part};
+    c.fld.chiBlocks = nb;
     PSR_HIP_CHECK(hipGetLastError());
 }
 
