@@ -205,4 +205,75 @@ __device__ __forceinline__ void points_chi_block(const PointsChiArgs& a, int blo
     }
 }
 
+// --- operator (LaplacianSolver.cu:27-74) -----------------------------------
+// L(o, o') = <F,F>_x <F,F>_y <F,F>_z (<F',F'>_x + <F',F'>_y + <F',F'>_z) for the
+// same-depth neighbours o' of o; |L| <= 1e-6 is dropped (stored as 0); values
+// kept as float.  ELL layout [27][M] over all nodes; the column of slot i is
+// neigh[i][o] - base[depth].
+// Also counts the stored (nonzero) entries per depth into nnz (zeroed before).
+// Blocks of kAssembleThreads threads, launched beside the divergence's last
+// kernel (they need only the octree and the tables).
+constexpr int kAssembleThreads = 256;
+struct AssembleArgs {
+    Levels L;
+    int M;
+    const int* neigh;
+    const int* enc;
+    const double* dot;
+    const double* d2dot;
+    int res;
+    float* val;
+    int* nnz;
+};
+__device__ __forceinline__ void assemble_rows(const AssembleArgs& a, int block) {
+    const Levels& L = a.L;
+    const int M = a.M, res = a.res;
+    const int* __restrict__ neigh = a.neigh;
+    const int* __restrict__ enc = a.enc;
+    const double* __restrict__ dot = a.dot;
+    const double* __restrict__ d2dot = a.d2dot;
+    float* __restrict__ val = a.val;
+    // three threads per row, each the 9 neighbours of one x offset (thread
+    // t = j * M + o: neighbours 9 j .. 9 j + 8 of row o), so that three times as
+    // many waves hide the loads; a thread's table rows stay in its cache
+    __shared__ int cnt[PSR_MAX_DEPTH + 1];
+    if (threadIdx.x <= PSR_MAX_DEPTH) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int t = block * kAssembleThreads + int(threadIdx.x);
+    const bool valid = t < 3 * M;
+    int d = -1, c = 0;
+    if (valid) {
+        const int j = t / M, o = t - j * M;
+        const int3 f1 = decode_func_index(enc[o], L.D);
+#pragma unroll 3
+        for (int i = 9 * j; i < 9 * j + 9; ++i) {
+            const int nb = neigh[i * M + o];
+            float v = 0.f;
+            if (nb != -1) {
+                const int3 f2 = decode_func_index(enc[nb], L.D);
+                size_t sx = size_t(f1.x) * res + f2.x, sy = size_t(f1.y) * res + f2.y, sz = size_t(f1.z) * res + f2.z;
+                double e = dot[sx] * dot[sy] * dot[sz] * (d2dot[sx] + d2dot[sy] + d2dot[sz]);
+                if (fabs(e) > double(1e-6f)) v = float(e);
+            }
+            val[i * M + o] = v;
+            c += v != 0.f;
+        }
+        d = depth_of(L, o);
+    }
+    // stored entries per depth: one LDS add per depth present in the wave
+    unsigned long long active = __ballot(valid);
+    while (active) {
+        const int leader = __builtin_ctzll(active);
+        const int dl = __shfl(d, leader, PSR_WAVE);
+        const unsigned long long same = __ballot(d == dl) & active;
+        int s = d == dl ? c : 0;
+#pragma unroll
+        for (int off = PSR_WAVE / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, PSR_WAVE);
+        if (__lane_id() == leader) atomicAdd(&cnt[dl], s);
+        active &= ~same;
+    }
+    __syncthreads();
+    if (threadIdx.x <= PSR_MAX_DEPTH && cnt[threadIdx.x]) atomicAdd(&a.nnz[threadIdx.x], cnt[threadIdx.x]);
+}
+
 }  // namespace psr

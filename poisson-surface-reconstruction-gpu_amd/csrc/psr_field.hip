@@ -234,7 +234,10 @@ template <class FieldT>
 __global__ void __launch_bounds__(kDivCoarseThreads) divergence_coarse(
     CoarsePlan P, int M, int offD, int nD, int res, int D, const int* __restrict__ neigh,
     const int* __restrict__ didx, const int* __restrict__ dnum, const int* __restrict__ enc,
-    const FieldT* __restrict__ vf, const double* __restrict__ ddot, double* __restrict__ partial) {
+    const FieldT* __restrict__ vf, const double* __restrict__ ddot, double* __restrict__ partial,
+    int* __restrict__ nnz) {
+    // the operator's per-depth entry counts, added to beside the last divergence kernel
+    if (blockIdx.x == 0 && threadIdx.x <= PSR_MAX_DEPTH) nnz[threadIdx.x] = 0;
     int d = 0;
     while (d + 1 < P.depths && int(blockIdx.x) >= P.blockStart[d + 1]) ++d;
     const int parts = P.parts[d];
@@ -276,11 +279,19 @@ __global__ void __launch_bounds__(kDivCoarseThreads) divergence_coarse(
 // descendant, so a thread sums the 27 terms of its node.  The first threads
 // also fold the coarse nodes' shares (thread t: coarse node t), in order.
 template <class FieldT>
+// Blocks from leafBlocks on assemble the operator instead (assemble_rows):
+// independent work that fills the GPU beside this kernel's few workgroups.
 __global__ void __launch_bounds__(256) divergence_leaf(int offD, int nD, int M, int res, int D,
                                                        const int* __restrict__ neigh, const int* __restrict__ enc,
                                                        const FieldT* __restrict__ vf,
                                                        const double* __restrict__ ddot, CoarsePlan P,
-                                                       const double* __restrict__ partial, float* __restrict__ div) {
+                                                       const double* __restrict__ partial, float* __restrict__ div,
+                                                       int leafBlocks, AssembleArgs aa) {
+    static_assert(kAssembleThreads == 256, "one block size for both roles");
+    if (int(blockIdx.x) >= leafBlocks) {
+        assemble_rows(aa, int(blockIdx.x) - leafBlocks);
+        return;
+    }
     const int t = blockIdx.x * 256 + threadIdx.x;
     if (t < P.base[P.depths]) {
         int d = 0;
@@ -350,15 +361,21 @@ void field_stages(Ctx& c, FieldT* vf) {
         P.base[kCoarse + 1] = L.base[kCoarse + 1];
         partial = c.buf<double>("div_partial", blocks);
         divergence_coarse<FieldT><<<blocks, kDivCoarseThreads, 0, s>>>(P, M, offD, nD, c.tables.res, D, o.neigh, o.didx,
-                                                                      o.dnum, o.encIdx, vf, c.tables.ddot, partial);
+                                                                      o.dnum, o.encIdx, vf, c.tables.ddot, partial,
+                                                                      c.results()->cgNnz);
     }
     const int midCount = offD - coarseCount;
     if (midCount > 0)
         divergence_fine<FieldT><<<div_up(midCount, kDivWaves), kDivWaves * PSR_WAVE, 0, s>>>(
             coarseCount, midCount, M, offD, nD, c.tables.res, D, o.neigh, o.didx, o.dnum, o.encIdx, vf, c.tables.ddot,
             c.fld.div);
-    divergence_leaf<FieldT><<<div_up(std::max(nD, coarseCount), 256), 256, 0, s>>>(
-        offD, nD, M, c.tables.res, D, o.neigh, o.encIdx, vf, c.tables.ddot, P, partial, c.fld.div);
+    // the operator (it needs only the octree and the tables) is assembled in
+    // the same launch
+    const AssembleArgs aa{L,     M,     o.neigh, o.encIdx, c.tables.dot, c.tables.d2dot, c.tables.res,
+                          c.buf<float>("lap_val", size_t(27) * M), c.results()->cgNnz};
+    const int leafBlocks = int(div_up(std::max(nD, coarseCount), 256));
+    divergence_leaf<FieldT><<<leafBlocks + int(div_up(size_t(3) * M, kAssembleThreads)), 256, 0, s>>>(
+        offD, nD, M, c.tables.res, D, o.neigh, o.encIdx, vf, c.tables.ddot, P, partial, c.fld.div, leafBlocks, aa);
     PSR_HIP_CHECK(hipGetLastError());
 }
 

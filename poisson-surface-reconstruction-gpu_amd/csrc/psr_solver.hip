@@ -15,60 +15,6 @@ namespace {
 
 constexpr int kThreads = 256;
 
-// --- operator (LaplacianSolver.cu:27-74) -----------------------------------
-// L(o, o') = <F,F>_x <F,F>_y <F,F>_z (<F',F'>_x + <F',F'>_y + <F',F'>_z) for the
-// same-depth neighbours o' of o; |L| <= 1e-6 is dropped (stored as 0); values
-// kept as float.  ELL layout [27][M] over all nodes; the column of slot i is
-// neigh[i][o] - base[depth].
-// Also counts the stored (nonzero) entries per depth into nnz (zeroed before).
-__global__ void __launch_bounds__(kThreads) assemble_operator(Levels L, int M, const int* __restrict__ neigh,
-                                                              const int* __restrict__ enc,
-                                                              const double* __restrict__ dot,
-                                                              const double* __restrict__ d2dot, int res,
-                                                              float* __restrict__ val, int* __restrict__ nnz) {
-    // three threads per row, each the 9 neighbours of one x offset (thread
-    // t = j * M + o: neighbours 9 j .. 9 j + 8 of row o), so that three times as
-    // many waves hide the loads; a thread's table rows stay in its cache
-    __shared__ int cnt[PSR_MAX_DEPTH + 1];
-    if (threadIdx.x <= PSR_MAX_DEPTH) cnt[threadIdx.x] = 0;
-    __syncthreads();
-    const int t = blockIdx.x * kThreads + threadIdx.x;
-    const bool valid = t < 3 * M;
-    int d = -1, c = 0;
-    if (valid) {
-        const int j = t / M, o = t - j * M;
-        const int3 f1 = decode_func_index(enc[o], L.D);
-#pragma unroll 3
-        for (int i = 9 * j; i < 9 * j + 9; ++i) {
-            const int nb = neigh[i * M + o];
-            float v = 0.f;
-            if (nb != -1) {
-                const int3 f2 = decode_func_index(enc[nb], L.D);
-                size_t sx = size_t(f1.x) * res + f2.x, sy = size_t(f1.y) * res + f2.y, sz = size_t(f1.z) * res + f2.z;
-                double e = dot[sx] * dot[sy] * dot[sz] * (d2dot[sx] + d2dot[sy] + d2dot[sz]);
-                if (fabs(e) > double(1e-6f)) v = float(e);
-            }
-            val[i * M + o] = v;
-            c += v != 0.f;
-        }
-        d = depth_of(L, o);
-    }
-    // stored entries per depth: one LDS add per depth present in the wave
-    unsigned long long active = __ballot(valid);
-    while (active) {
-        const int leader = __builtin_ctzll(active);
-        const int dl = __shfl(d, leader, PSR_WAVE);
-        const unsigned long long same = __ballot(d == dl) & active;
-        int s = d == dl ? c : 0;
-#pragma unroll
-        for (int off = PSR_WAVE / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, PSR_WAVE);
-        if (__lane_id() == leader) atomicAdd(&cnt[dl], s);
-        active &= ~same;
-    }
-    __syncthreads();
-    if (threadIdx.x <= PSR_MAX_DEPTH && cnt[threadIdx.x]) atomicAdd(&nnz[threadIdx.x], cnt[threadIdx.x]);
-}
-
 // --- concurrent CG (CGAlgorithm.cu:10-147, 296-408) -------------------------
 // Work item = a range of rows of one depth.  All work items of a depth form a
 // group that iterates in lock step through a bounded-spin group barrier; a
@@ -1016,14 +962,14 @@ void solve_depths(Ctx& c) {
     double* trueRel = res->cgTrueRel;
     int* err = &res->cgErr;
     float4* slots = rowForm && mixed ? c.buf<float4>("cg_slots", size_t(4) * wg) : nullptr;
-    // one launch clears the group controls, the slots, the operator counts and the CG results
+    // one launch clears the group controls, the slots and the CG results (the
+    // operator and its entry counts come from the field stage's last launch)
     zero_spans({{ctl, size_t(ctlWords)},
                 {slots, slots ? size_t(16) * wg : 0},
-                {res->cgIters, sizeof(res->cgIters) / 4 + sizeof(res->cgNnz) / 4 + 1},  // iterations, nnz, error
+                {res->cgIters, sizeof(res->cgIters) / 4},
+                {&res->cgErr, 1},
                 {trueRel, 2 * (PSR_MAX_DEPTH + 1)}},
                s);
-    assemble_operator<<<div_up(size_t(3) * M, kThreads), kThreads, 0, s>>>(L, M, o.neigh, o.encIdx, c.tables.dot, c.tables.d2dot,
-                                                              c.tables.res, val, nnz);
     // CG kernels of different contexts (streams) never compete for residency
     // while spinning in their group barriers: a launch is queued only once the
     // launches still in flight leave room for all its workgroups beside them
