@@ -150,8 +150,9 @@ def measure(pkg, tx, tn, depth, precision, steps, warmup):
 
 
 def measure_weld(pkg, tx, tn, depth, reps):
-    """Times psr_weld_mesh on the workload's mixed-precision mesh (the call
-    returns with the welded counts, so each timing includes its wait)."""
+    """Times psr_weld_mesh on the workload's mixed-precision mesh and
+    psr_mesh_normals on the welded mesh (both calls return with their results
+    final, so each timing includes its wait)."""
     import torch
     rec = pkg.PoissonReconstruction(depth=depth, precision=pkg.binding.PRECISION_MIXED)
     mesh = rec.SolvePoissionReconstructionMesh(tx, tn)
@@ -162,7 +163,17 @@ def measure_weld(pkg, tx, tn, depth, reps):
     for _ in range(reps):
         w = rec.WeldMesh()
     dt = time.perf_counter() - t0
-    out = {"ms": dt * 1e3 / reps, "vertices": nv, "welded_vertices": w.n_vertices, "triangles": nt, "reps": reps}
+    # per-vertex normals of the welded mesh (psr_mesh_normals returns once written)
+    import ctypes
+    nbuf = torch.empty((max(w.n_vertices, 1), 3), dtype=torch.float32, device="cuda")
+    L = pkg.binding.lib()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(reps):
+        assert L.psr_mesh_normals(rec._ctx, ctypes.byref(w), nbuf.data_ptr()) == 0
+    dn = time.perf_counter() - t1
+    out = {"ms": dt * 1e3 / reps, "vertices": nv, "welded_vertices": w.n_vertices, "triangles": nt,
+           "normals_ms": dn * 1e3 / reps, "reps": reps}
     rec.close()
     return out
 

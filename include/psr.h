@@ -163,6 +163,16 @@ int psr_export(psr_ctx* ctx, int array_id, void* host_dst, uint64_t bytes, uint6
  * duplicates of its marching-cubes batches in place
  * (mesh/ComputeTriangleIndices.cu:1247-1282). */
 int psr_weld_mesh(psr_ctx* ctx, psr_mesh* out);
+/* Per-vertex normals of a mesh in device memory (any psr_mesh: the last
+ * reconstruction's or its welded form), written to normals, a DEVICE buffer
+ * of 3 * n_vertices floats; returns once they are written.  Follows
+ * mesh/DrawMesh.cu:80-177: each triangle's unit normal cross(A - O, B - O),
+ * summed over the triangles touching a vertex, divided by their count and
+ * normalised.  Sums run in triangle order (the reference adds with float
+ * atomics, in no fixed order); a vertex no triangle touches gets (0, 0, 0)
+ * (the reference divides 0 by 0 there); a triangle with a corner outside the
+ * mesh contributes nothing. */
+int psr_mesh_normals(psr_ctx* ctx, const psr_mesh* mesh, float* normals);
 /* Writes a binary little-endian PLY (float x y z; uchar-counted int32 face
  * lists) from HOST arrays.  No counterpart in the reference, which renders
  * the mesh (mesh/DrawMesh.cpp) and writes no file. */

@@ -122,3 +122,38 @@ def lut(which, s, i):
 
 def mc_word(cube):
     return lib().orc_mc_word(cube) & ((1 << 64) - 1)
+
+
+def mesh_normals(vertices, triangles):
+    """Per-vertex normals in float32 (mesh/DrawMesh.cu:80-177): each
+    triangle's unit normal cross(A - O, B - O) (CalculateMeshNormalsKernel
+    :102-124, CrossProduct :93-100, VectorNormalize :80-91: zero stays zero),
+    summed per vertex in triangle order (the reference's float atomics add in
+    no fixed order), divided by the vertex's triangle count and normalised
+    (:135-177).  A vertex no triangle touches gets zero (the reference divides
+    0 by 0); a triangle with a corner outside the mesh is skipped."""
+    import numpy as np
+    V = np.asarray(vertices, np.float32).reshape(-1, 3)
+    T = np.asarray(triangles, np.int64).reshape(-1, 3)
+    nv = V.shape[0]
+    T = T[np.all((T >= 0) & (T < nv), axis=1)]
+
+    def unit(x):
+        ln = np.sqrt(x[:, 0] * x[:, 0] + x[:, 1] * x[:, 1] + x[:, 2] * x[:, 2])
+        out = np.zeros_like(x)
+        nz = ln != 0
+        out[nz] = x[nz] / ln[nz, None]
+        return out
+
+    O, A, B = V[T[:, 0]], V[T[:, 1]], V[T[:, 2]]
+    a, b = A - O, B - O
+    cr = np.stack([a[:, 1] * b[:, 2] - a[:, 2] * b[:, 1], a[:, 2] * b[:, 0] - a[:, 0] * b[:, 2],
+                   a[:, 0] * b[:, 1] - a[:, 1] * b[:, 0]], 1).astype(np.float32)
+    fn = unit(cr)
+    S = np.zeros((nv, 3), np.float32)
+    np.add.at(S, T.reshape(-1), np.repeat(fn, 3, axis=0))
+    cnt = np.bincount(T.reshape(-1), minlength=nv).astype(np.float32)
+    avg = np.zeros_like(S)
+    has = cnt > 0
+    avg[has] = S[has] / cnt[has, None]
+    return unit(avg)

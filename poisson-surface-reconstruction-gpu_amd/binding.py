@@ -36,7 +36,7 @@ _ARRAY_LAYOUT = {
 
 EXPORTED_SYMBOLS = ["psr_default_config", "psr_create", "psr_reconstruct", "psr_reconstruct_batch",
                     "psr_get_batch_stats", "psr_sync", "psr_get_stats", "psr_export", "psr_destroy",
-                    "psr_status_string", "psr_last_error", "psr_weld_mesh", "psr_write_ply"]
+                    "psr_status_string", "psr_last_error", "psr_weld_mesh", "psr_write_ply", "psr_mesh_normals"]
 
 
 class Config(ctypes.Structure):
@@ -128,6 +128,7 @@ def lib():
                                  ctypes.POINTER(ctypes.c_uint64)]
         L.psr_destroy.argtypes = [ctypes.c_void_p]
         L.psr_weld_mesh.argtypes = [ctypes.c_void_p, ctypes.POINTER(Mesh)]
+        L.psr_mesh_normals.argtypes = [ctypes.c_void_p, ctypes.POINTER(Mesh), ctypes.c_void_p]
         L.psr_write_ply.argtypes = [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
                                     ctypes.c_uint32]
         L.psr_status_string.restype = ctypes.c_char_p
@@ -250,6 +251,18 @@ class PoissonReconstruction:
         if rc != PSR_OK:
             raise PsrError(rc, lib().psr_last_error(self._ctx).decode())
         return welded
+
+    def MeshNormals(self, mesh=None):
+        """Per-vertex normals (numpy (V, 3) float32) of a psr_mesh in device
+        memory, by default the last reconstruction's."""
+        import torch
+        m = self.mesh if mesh is None else mesh
+        out = torch.empty((max(m.n_vertices, 1), 3), dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()
+        rc = lib().psr_mesh_normals(self._ctx, ctypes.byref(m), out.data_ptr())
+        if rc != PSR_OK:
+            raise PsrError(rc, lib().psr_last_error(self._ctx).decode())
+        return out[:m.n_vertices].cpu().numpy()
 
     def GetRebuildMeshVertices(self):
         return self.export("MESH_VERTICES")

@@ -252,6 +252,22 @@ int psr_weld_mesh(psr_ctx* ctx, psr_mesh* out) {
     return PSR_OK;
 }
 
+int psr_mesh_normals(psr_ctx* ctx, const psr_mesh* mesh, float* normals) {
+    if (!ctx || !mesh || (mesh->n_vertices && (!normals || !mesh->vertices)) ||
+        (mesh->n_triangles && !mesh->triangles) || mesh->n_vertices > uint32_t(INT32_MAX) ||
+        mesh->n_triangles > uint32_t(INT32_MAX / 3))
+        return PSR_ERR_INVALID;
+    try {
+        psr::mesh_normals(ctx->c, mesh->vertices, int(mesh->n_vertices), mesh->triangles, int(mesh->n_triangles),
+                          normals);
+    } catch (const psr::HipError& e) {
+        return fail(ctx, PSR_ERR_HIP, e.what());
+    } catch (const std::exception& e) {
+        return fail(ctx, PSR_ERR_INTERNAL, e.what());
+    }
+    return PSR_OK;
+}
+
 int psr_reconstruct_batch(psr_ctx* ctx, const psr_points* clouds, uint32_t count, psr_mesh* outs) {
     if (!ctx || (count && (!clouds || !outs))) return PSR_ERR_INVALID;
     psr::Ctx& c = ctx->c;

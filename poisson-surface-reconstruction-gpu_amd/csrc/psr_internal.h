@@ -253,5 +253,7 @@ void extract_mesh(Ctx& c);
 void finish_mesh(Ctx& c);
 // Welds the last mesh into mesh.welded*; returns once the counts are known.
 void weld_mesh(Ctx& c);
+// Per-vertex normals of a device mesh into out (3 floats per vertex); returns once written.
+void mesh_normals(Ctx& c, const float* V, int nv, const int* T, int nt, float* out);
 
 }  // namespace psr

@@ -68,7 +68,92 @@ __global__ void __launch_bounds__(kThreads) weld_triangles(const int* __restrict
     outT[i] = a >= 0 && a < nv ? remap[a] : -1;
 }
 
+// --- per-vertex normals (DrawMesh.cu:80-177) ---
+// Unit normal of triangle t, cross(A - O, B - O) with (O, A, B) its corners,
+// or nothing for a triangle with a corner outside the mesh; the (vertex,
+// corner) pairs for the grouping sort (key nv: no vertex).  Also clears the
+// output, which the vertex kernel writes only where a vertex has triangles.
+__device__ __forceinline__ float3 unit_or_zero(float x, float y, float z) {
+    // VectorNormalize (DrawMesh.cu:80-91): zero stays zero
+    const float len = __fsqrt_rn(__fadd_rn(__fadd_rn(__fmul_rn(x, x), __fmul_rn(y, y)), __fmul_rn(z, z)));
+    if (len == 0.f) return make_float3(0.f, 0.f, 0.f);
+    return make_float3(__fdiv_rn(x, len), __fdiv_rn(y, len), __fdiv_rn(z, len));
+}
+
+__global__ void __launch_bounds__(kThreads) normals_faces(const float* __restrict__ V, int nv,
+                                                          const int* __restrict__ T, int nt,
+                                                          float3* __restrict__ fn, unsigned* __restrict__ key,
+                                                          unsigned* __restrict__ val, float* __restrict__ out) {
+    const size_t i = size_t(blockIdx.x) * kThreads + threadIdx.x;
+    for (size_t z = i; z < 3 * size_t(nv); z += size_t(gridDim.x) * kThreads) out[z] = 0.f;
+    if (i >= size_t(nt)) return;
+    const int c0 = T[3 * i], c1 = T[3 * i + 1], c2 = T[3 * i + 2];
+    const bool ok = c0 >= 0 && c0 < nv && c1 >= 0 && c1 < nv && c2 >= 0 && c2 < nv;
+    if (ok) {
+        const float ox = V[3 * size_t(c0)], oy = V[3 * size_t(c0) + 1], oz = V[3 * size_t(c0) + 2];
+        const float ax = __fsub_rn(V[3 * size_t(c1)], ox), ay = __fsub_rn(V[3 * size_t(c1) + 1], oy),
+                    az = __fsub_rn(V[3 * size_t(c1) + 2], oz);
+        const float bx = __fsub_rn(V[3 * size_t(c2)], ox), by = __fsub_rn(V[3 * size_t(c2) + 1], oy),
+                    bz = __fsub_rn(V[3 * size_t(c2) + 2], oz);
+        // CrossProduct (DrawMesh.cu:93-100)
+        fn[i] = unit_or_zero(__fsub_rn(__fmul_rn(ay, bz), __fmul_rn(az, by)),
+                             __fsub_rn(__fmul_rn(az, bx), __fmul_rn(ax, bz)),
+                             __fsub_rn(__fmul_rn(ax, by), __fmul_rn(ay, bx)));
+    }
+    key[3 * i] = ok ? unsigned(c0) : unsigned(nv);
+    key[3 * i + 1] = ok ? unsigned(c1) : unsigned(nv);
+    key[3 * i + 2] = ok ? unsigned(c2) : unsigned(nv);
+    val[3 * i] = val[3 * i + 1] = val[3 * i + 2] = unsigned(i);
+}
+
+// The first pair of each vertex's run (pairs sorted stably by vertex, so in
+// triangle order) sums the run's face normals, divides by the count and
+// normalises (VerticesNormalsSumKernel, CalculateVerticesAverageNormals).
+__global__ void __launch_bounds__(kThreads) normals_vertices(const unsigned* __restrict__ key,
+                                                             const unsigned* __restrict__ val, size_t n, int nv,
+                                                             const float3* __restrict__ fn, float* __restrict__ out) {
+    const size_t i = size_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const unsigned k = key[i];
+    if (k >= unsigned(nv) || (i > 0 && key[i - 1] == k)) return;
+    float sx = 0.f, sy = 0.f, sz = 0.f;
+    int cnt = 0;
+    for (size_t j = i; j < n && key[j] == k; ++j, ++cnt) {
+        const float3 f = fn[val[j]];
+        sx = __fadd_rn(sx, f.x);
+        sy = __fadd_rn(sy, f.y);
+        sz = __fadd_rn(sz, f.z);
+    }
+    const float fc = float(cnt);
+    const float3 u = unit_or_zero(__fdiv_rn(sx, fc), __fdiv_rn(sy, fc), __fdiv_rn(sz, fc));
+    out[3 * size_t(k)] = u.x;
+    out[3 * size_t(k) + 1] = u.y;
+    out[3 * size_t(k) + 2] = u.z;
+}
+
 }  // namespace
+
+void mesh_normals(Ctx& c, const float* V, int nv, const int* T, int nt, float* out) {
+    hipStream_t s = c.stream;
+    if (nv <= 0) return;
+    const size_t n = size_t(3) * std::max(nt, 0);
+    float3* fn = c.buf<float3>("nrm_faces", std::max(nt, 1));
+    unsigned* k = c.buf<unsigned>("nrm_k", std::max<size_t>(n, 1));
+    unsigned* v = c.buf<unsigned>("nrm_v", std::max<size_t>(n, 1));
+    unsigned* kt = c.buf<unsigned>("nrm_kt", std::max<size_t>(n, 1));
+    unsigned* vt = c.buf<unsigned>("nrm_vt", std::max<size_t>(n, 1));
+    const unsigned grid = std::max(div_up(size_t(std::max(nt, 1)), kThreads), div_up(3 * size_t(nv), kThreads * 8));
+    normals_faces<<<grid, kThreads, 0, s>>>(V, nv, T, nt, fn, k, v, out);
+    if (n > 0) {
+        int bits = 1;
+        while (bits < 32 && (unsigned(nv) >> bits) != 0) ++bits;  // keys 0..nv
+        int* scratch = c.scratch_for(sort_scratch_ints(n));
+        radix_sort_pairs(k, v, kt, vt, int(n), bits, scratch, false, c.scan_for(sort_scan_elems(n)), SortGather{}, s);
+        normals_vertices<<<div_up(n, kThreads), kThreads, 0, s>>>(k, v, n, nv, fn, out);
+    }
+    c.wait();
+    PSR_HIP_CHECK(hipGetLastError());
+}
 
 void weld_mesh(Ctx& c) {
     Mesh& m = c.mesh;
