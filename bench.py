@@ -44,7 +44,8 @@ def parse():
     ap.add_argument("--batch-reps", type=int, default=3,
                     help="timed repetitions of the 64-cloud batch, sequential and batched (0: skip)")
     ap.add_argument("--batch-workers", type=int, default=8, help="clouds in flight in the batched form")
-    ap.add_argument("--weld-reps", type=int, default=20, help="timed welds of the workload's mesh (0: skip)")
+    ap.add_argument("--weld-reps", type=int, default=20,
+                    help="timed welds, mesh normals and point-normal estimations of the workload (0: skip)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain elementwise copy of MIB MiB three times (HBM counter calibration)")
     return ap.parse_args()
@@ -178,6 +179,29 @@ def measure_weld(pkg, tx, tn, depth, reps):
     return out
 
 
+def measure_point_normals(pkg, tx, reps):
+    """Times psr_estimate_normals (k = 10, viewpoint at the origin) on the
+    workload's points; the call returns with the normals written."""
+    import ctypes
+    import torch
+    rec = pkg.PoissonReconstruction(depth=7)
+    n = tx.shape[0]
+    nrm = torch.empty((n, 3), dtype=torch.float32, device=tx.device)
+    cur = torch.empty((n,), dtype=torch.float32, device=tx.device)
+    vp = (ctypes.c_float * 3)(0.0, 0.0, 0.0)
+    L = pkg.binding.lib()
+    call = lambda: L.psr_estimate_normals(rec._ctx, tx.data_ptr(), n, 12, 10, vp, nrm.data_ptr(), cur.data_ptr(),
+                                          None)
+    assert call() == 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        assert call() == 0
+    dt = time.perf_counter() - t0
+    rec.close()
+    return {"ms": dt * 1e3 / reps, "points": int(n), "k": 10, "reps": reps}
+
+
 def measure_batch(pkg, reps, workers):
     """The 64-cloud batch (sizes log-uniform in [5k, 150k], ellipsoids and tori,
     depth 7): 64 sequential single-cloud calls against one batched call."""
@@ -268,6 +292,7 @@ def main():
         line["mixed_vs_fp32"] = h["value"] / f["value"]
     if a.weld_reps > 0:
         line["weld"] = measure_weld(pkg, tx, tn, depth, a.weld_reps)
+        line["point_normals"] = measure_point_normals(pkg, tx, a.weld_reps)
     if a.batch_reps > 0:
         line["batch64"] = measure_batch(pkg, a.batch_reps, a.batch_workers)
     if a.cpu_sample > 0:

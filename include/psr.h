@@ -173,6 +173,18 @@ int psr_weld_mesh(psr_ctx* ctx, psr_mesh* out);
  * (the reference divides 0 by 0 there); a triangle with a corner outside the
  * mesh contributes nothing. */
 int psr_mesh_normals(psr_ctx* ctx, const psr_mesh* mesh, float* normals);
+/* Normals of an unoriented point cloud in device memory (xyz at the start of
+ * each stride_bytes record), the step before psr_reconstruct: replaces the
+ * reference's PCL NormalEstimationOMP (mesh/PoissonReconstruction.cpp:137-186,
+ * k = 10 from a kd-tree, viewpoint (0, 0, 0)).  For each point: its k nearest
+ * points (itself included; exact, ties to the lower index), their covariance,
+ * the eigenvector of the smallest eigenvalue, flipped so that
+ * (viewpoint - p) . n >= 0.  normals: DEVICE, 3 floats per point; curvature
+ * (lambda_0 / sum lambda, may be NULL): DEVICE, 1 float per point; knn (may be
+ * NULL): DEVICE, k int32 per point, nearest first.  1 <= k <= 16.  Returns
+ * once written.  No counterpart on the GPU in the reference. */
+int psr_estimate_normals(psr_ctx* ctx, const float* xyz, uint32_t count, uint32_t stride_bytes, int k,
+                         const float viewpoint[3], float* normals, float* curvature, int32_t* knn);
 /* Writes a binary little-endian PLY (float x y z; uchar-counted int32 face
  * lists) from HOST arrays.  No counterpart in the reference, which renders
  * the mesh (mesh/DrawMesh.cpp) and writes no file. */

@@ -157,3 +157,34 @@ def mesh_normals(vertices, triangles):
     has = cnt > 0
     avg[has] = S[has] / cnt[has, None]
     return unit(avg)
+
+
+def estimate_normals(xyz, k=10, viewpoint=(0.0, 0.0, 0.0)):
+    """Point normals by the published algorithm of PCL's NormalEstimation (the
+    reference calls NormalEstimationOMP with k = 10 from a kd-tree and the
+    default viewpoint, mesh/PoissonReconstruction.cpp:137-186; PCL itself is
+    absent): the k nearest points (the point itself included), their mean and
+    covariance (divided by k), the eigenvector of the smallest eigenvalue,
+    flipped so that (viewpoint - p) . n >= 0, and curvature
+    lambda_0 / (lambda_0 + lambda_1 + lambda_2).  FP64 throughout.
+    Returns normals, curvature, neighbour indices (nearest first) and the
+    distances to the k-th and (k+1)-th nearest points (for ties)."""
+    import numpy as np
+    from scipy.spatial import cKDTree
+    P = np.asarray(xyz, np.float64).reshape(-1, 3)
+    kk = min(k, P.shape[0])
+    kq = min(kk + 1, P.shape[0])
+    dist, idx = cKDTree(P).query(P, k=kq)
+    dist, idx = dist.reshape(P.shape[0], -1), idx.reshape(P.shape[0], -1)
+    nb = P[idx[:, :kk]]
+    d = nb - nb.mean(axis=1, keepdims=True)
+    C = np.einsum("nki,nkj->nij", d, d) / kk
+    w, V = np.linalg.eigh(C)
+    n = V[:, :, 0]
+    flip = np.sum((np.asarray(viewpoint, np.float64) - P) * n, axis=1) < 0
+    n[flip] *= -1
+    tr = w.sum(axis=1)
+    curv = np.where(tr != 0, w[:, 0] / np.where(tr != 0, tr, 1), 0.0)
+    gap = (w[:, 1] - w[:, 0]) / np.where(tr != 0, tr, 1)
+    dk1 = dist[:, kk] if kq > kk else np.full(P.shape[0], np.inf)
+    return n, curv, idx[:, :kk], dist[:, kk - 1], dk1, gap

@@ -36,7 +36,7 @@ _ARRAY_LAYOUT = {
 
 EXPORTED_SYMBOLS = ["psr_default_config", "psr_create", "psr_reconstruct", "psr_reconstruct_batch",
                     "psr_get_batch_stats", "psr_sync", "psr_get_stats", "psr_export", "psr_destroy",
-                    "psr_status_string", "psr_last_error", "psr_weld_mesh", "psr_write_ply", "psr_mesh_normals"]
+                    "psr_status_string", "psr_last_error", "psr_weld_mesh", "psr_write_ply", "psr_mesh_normals", "psr_estimate_normals"]
 
 
 class Config(ctypes.Structure):
@@ -129,6 +129,9 @@ def lib():
         L.psr_destroy.argtypes = [ctypes.c_void_p]
         L.psr_weld_mesh.argtypes = [ctypes.c_void_p, ctypes.POINTER(Mesh)]
         L.psr_mesh_normals.argtypes = [ctypes.c_void_p, ctypes.POINTER(Mesh), ctypes.c_void_p]
+        L.psr_estimate_normals.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
+                                           ctypes.c_int, ctypes.POINTER(ctypes.c_float), ctypes.c_void_p,
+                                           ctypes.c_void_p, ctypes.c_void_p]
         L.psr_write_ply.argtypes = [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
                                     ctypes.c_uint32]
         L.psr_status_string.restype = ctypes.c_char_p
@@ -263,6 +266,25 @@ class PoissonReconstruction:
         if rc != PSR_OK:
             raise PsrError(rc, lib().psr_last_error(self._ctx).decode())
         return out[:m.n_vertices].cpu().numpy()
+
+    def EstimateNormals(self, xyz, k=10, viewpoint=(0.0, 0.0, 0.0), with_knn=False):
+        """Normals of an unoriented cloud (contiguous float32 (N, 3) GPU tensor),
+        the step before SolvePoissionReconstructionMesh (the reference's PCL
+        NormalEstimationOMP, mesh/PoissonReconstruction.cpp:137-186).  Returns
+        (normals (N, 3), curvature (N,), knn (N, k) int32 or None), GPU tensors."""
+        import torch
+        assert xyz.is_cuda and xyz.is_contiguous() and xyz.dtype == torch.float32
+        n = xyz.shape[0]
+        nrm = torch.empty((max(n, 1), 3), dtype=torch.float32, device=xyz.device)
+        cur = torch.empty((max(n, 1),), dtype=torch.float32, device=xyz.device)
+        knn = torch.empty((max(n, 1), k), dtype=torch.int32, device=xyz.device) if with_knn else None
+        vp = (ctypes.c_float * 3)(*viewpoint)
+        torch.cuda.synchronize()
+        rc = lib().psr_estimate_normals(self._ctx, xyz.data_ptr(), n, 12, k, vp, nrm.data_ptr(), cur.data_ptr(),
+                                        knn.data_ptr() if with_knn else None)
+        if rc != PSR_OK:
+            raise PsrError(rc, lib().psr_last_error(self._ctx).decode())
+        return nrm[:n], cur[:n], (knn[:n] if with_knn else None)
 
     def GetRebuildMeshVertices(self):
         return self.export("MESH_VERTICES")

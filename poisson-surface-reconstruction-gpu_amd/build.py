@@ -11,7 +11,7 @@ LIB = os.path.join(HERE, "libpsr.so")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
 SOURCES = ["psr_prims.hip", "psr_octree.hip", "psr_field.hip", "psr_solver.hip", "psr_mesh.hip",
-           "psr_subdiv.hip", "psr_weld.hip", "psr_api.cpp"]
+           "psr_subdiv.hip", "psr_weld.hip", "psr_normals.hip", "psr_api.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PSR_OFFLOAD_ARCH", "gfx950")

@@ -268,6 +268,25 @@ int psr_mesh_normals(psr_ctx* ctx, const psr_mesh* mesh, float* normals) {
     return PSR_OK;
 }
 
+int psr_estimate_normals(psr_ctx* ctx, const float* xyz, uint32_t count, uint32_t stride_bytes, int k,
+                         const float viewpoint[3], float* normals, float* curvature, int32_t* knn) {
+    if (!ctx || (count && (!xyz || !normals)) || count > uint32_t(INT32_MAX) / 16 ||
+        (stride_bytes && (stride_bytes < 12 || stride_bytes % 4)))
+        return PSR_ERR_INVALID;
+    const float3 vp = viewpoint ? make_float3(viewpoint[0], viewpoint[1], viewpoint[2]) : make_float3(0.f, 0.f, 0.f);
+    try {
+        psr::estimate_normals(ctx->c, xyz, int(count), stride_bytes ? stride_bytes : 12, k, vp, normals, curvature,
+                              knn);
+    } catch (const psr::HipError& e) {
+        return fail(ctx, PSR_ERR_HIP, e.what());
+    } catch (const std::invalid_argument& e) {
+        return fail(ctx, PSR_ERR_INVALID, e.what());
+    } catch (const std::exception& e) {
+        return fail(ctx, PSR_ERR_INTERNAL, e.what());
+    }
+    return PSR_OK;
+}
+
 int psr_reconstruct_batch(psr_ctx* ctx, const psr_points* clouds, uint32_t count, psr_mesh* outs) {
     if (!ctx || (count && (!clouds || !outs))) return PSR_ERR_INVALID;
     psr::Ctx& c = ctx->c;

@@ -255,5 +255,9 @@ void finish_mesh(Ctx& c);
 void weld_mesh(Ctx& c);
 // Per-vertex normals of a device mesh into out (3 floats per vertex); returns once written.
 void mesh_normals(Ctx& c, const float* V, int nv, const int* T, int nt, float* out);
+// Normals (and curvature, k-nearest-neighbour indices when non-null) of an
+// unoriented point cloud; returns once written.
+void estimate_normals(Ctx& c, const float* xyz, int n, unsigned stride, int k, float3 vp, float* normals,
+                      float* curvature, int* knn);
 
 }  // namespace psr
