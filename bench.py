@@ -173,8 +173,20 @@ def measure_weld(pkg, tx, tn, depth, reps):
     for _ in range(reps):
         assert L.psr_mesh_normals(rec._ctx, ctypes.byref(w), nbuf.data_ptr()) == 0
     dn = time.perf_counter() - t1
+    # colours of the welded vertices (in the points' frame) from the points, with seeded colours
+    norm = rec.export("NORM_PARAMS")
+    wv = torch.from_numpy(rec.mesh_to_host(w)[0] * norm[3] + norm[:3]).float().cuda()
+    cols = torch.rand((tx.shape[0], 3), generator=torch.Generator().manual_seed(11)).cuda()
+    cbuf = torch.empty_like(wv)
+    wm = pkg.binding.Mesh(wv.data_ptr(), None, wv.shape[0], 0)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    for _ in range(reps):
+        assert L.psr_vertex_colors(rec._ctx, ctypes.byref(wm), tx.data_ptr(), cols.data_ptr(), tx.shape[0], 12,
+                                   cbuf.data_ptr()) == 0
+    dc = time.perf_counter() - t2
     out = {"ms": dt * 1e3 / reps, "vertices": nv, "welded_vertices": w.n_vertices, "triangles": nt,
-           "normals_ms": dn * 1e3 / reps, "reps": reps}
+           "normals_ms": dn * 1e3 / reps, "colors_ms": dc * 1e3 / reps, "reps": reps}
     rec.close()
     return out
 

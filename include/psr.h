@@ -185,6 +185,15 @@ int psr_mesh_normals(psr_ctx* ctx, const psr_mesh* mesh, float* normals);
  * once written.  No counterpart on the GPU in the reference. */
 int psr_estimate_normals(psr_ctx* ctx, const float* xyz, uint32_t count, uint32_t stride_bytes, int k,
                          const float viewpoint[3], float* normals, float* curvature, int32_t* knn);
+/* Per-vertex colours of a device mesh from coloured points (mesh/DrawMesh.cu:
+ * 35-79, 179-211): each vertex takes its 4 nearest points (exact; the
+ * reference searches all points by brute force) and weights each point's
+ * colour by its squared distance over the four's sum, as the reference does.
+ * xyz: DEVICE, in the mesh's frame, xyz at the start of each stride_bytes
+ * record; point_colors: DEVICE, 3 floats per point; colors: DEVICE, 3 floats
+ * per vertex.  Returns once written. */
+int psr_vertex_colors(psr_ctx* ctx, const psr_mesh* mesh, const float* xyz, const float* point_colors,
+                      uint32_t count, uint32_t stride_bytes, float* colors);
 /* Writes a binary little-endian PLY (float x y z; uchar-counted int32 face
  * lists) from HOST arrays.  No counterpart in the reference, which renders
  * the mesh (mesh/DrawMesh.cpp) and writes no file. */

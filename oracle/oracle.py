@@ -188,3 +188,34 @@ def estimate_normals(xyz, k=10, viewpoint=(0.0, 0.0, 0.0)):
     gap = (w[:, 1] - w[:, 0]) / np.where(tr != 0, tr, 1)
     dk1 = dist[:, kk] if kq > kk else np.full(P.shape[0], np.inf)
     return n, curv, idx[:, :kk], dist[:, kk - 1], dk1, gap
+
+
+def vertex_colors(vertices, xyz, colors):
+    """Vertex colours as mesh/DrawMesh.cu:179-211 forms them (float32): the 4
+    nearest points, weights = squared distance / sum of the four (the farther
+    point weighs more), colour = sum of weight x colour in neighbour order;
+    equal weights where the four distances are all zero.  Returns the colours
+    and the 4th / 5th nearest distances (for ties)."""
+    import numpy as np
+    from scipy.spatial import cKDTree
+    V = np.asarray(vertices, np.float32).reshape(-1, 3)
+    P = np.asarray(xyz, np.float32).reshape(-1, 3)
+    C = np.asarray(colors, np.float32).reshape(-1, 3)
+    kq = min(5, len(P))
+    dist, idx = cKDTree(P.astype(np.float64)).query(V.astype(np.float64), k=kq)
+    dist, idx = dist.reshape(len(V), -1), idx.reshape(len(V), -1)
+    kk = min(4, len(P))
+    t = V[:, None, :] - P[idx[:, :kk]]
+    d = t[:, :, 0] * t[:, :, 0] + t[:, :, 1] * t[:, :, 1] + t[:, :, 2] * t[:, :, 2]
+    s = np.zeros(len(V), np.float32)
+    for j in range(kk):
+        s = s + d[:, j]
+    pos = s > 0
+    inv = np.zeros_like(s)
+    inv[pos] = np.float32(1) / s[pos]
+    w = np.where(pos[:, None], d * inv[:, None], np.float32(1) / np.float32(kk)).astype(np.float32)
+    out = np.zeros((len(V), 3), np.float32)
+    for j in range(kk):
+        out = out + C[idx[:, j]] * w[:, j:j + 1]
+    d5 = dist[:, kk] if kq > kk else np.full(len(V), np.inf)
+    return out, dist[:, kk - 1], d5

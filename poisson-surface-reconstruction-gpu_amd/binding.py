@@ -36,7 +36,8 @@ _ARRAY_LAYOUT = {
 
 EXPORTED_SYMBOLS = ["psr_default_config", "psr_create", "psr_reconstruct", "psr_reconstruct_batch",
                     "psr_get_batch_stats", "psr_sync", "psr_get_stats", "psr_export", "psr_destroy",
-                    "psr_status_string", "psr_last_error", "psr_weld_mesh", "psr_write_ply", "psr_mesh_normals", "psr_estimate_normals"]
+                    "psr_status_string", "psr_last_error", "psr_weld_mesh", "psr_write_ply", "psr_mesh_normals", "psr_estimate_normals",
+                    "psr_vertex_colors"]
 
 
 class Config(ctypes.Structure):
@@ -132,6 +133,8 @@ def lib():
         L.psr_estimate_normals.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
                                            ctypes.c_int, ctypes.POINTER(ctypes.c_float), ctypes.c_void_p,
                                            ctypes.c_void_p, ctypes.c_void_p]
+        L.psr_vertex_colors.argtypes = [ctypes.c_void_p, ctypes.POINTER(Mesh), ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p]
         L.psr_write_ply.argtypes = [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
                                     ctypes.c_uint32]
         L.psr_status_string.restype = ctypes.c_char_p
@@ -285,6 +288,21 @@ class PoissonReconstruction:
         if rc != PSR_OK:
             raise PsrError(rc, lib().psr_last_error(self._ctx).decode())
         return nrm[:n], cur[:n], (knn[:n] if with_knn else None)
+
+    def VertexColors(self, vertices, xyz, colors):
+        """Colours (numpy (V, 3)) of mesh vertices (float32 (V, 3) GPU tensor, in
+        the points' frame) from coloured points (float32 (N, 3) GPU tensors
+        xyz and colors), as DrawMesh.cu:179-211 weights its 4 nearest points."""
+        import torch
+        v = vertices.contiguous()
+        m = Mesh(v.data_ptr(), None, v.shape[0], 0)
+        out = torch.empty((max(v.shape[0], 1), 3), dtype=torch.float32, device=v.device)
+        torch.cuda.synchronize()
+        rc = lib().psr_vertex_colors(self._ctx, ctypes.byref(m), xyz.data_ptr(), colors.data_ptr(), xyz.shape[0], 12,
+                                     out.data_ptr())
+        if rc != PSR_OK:
+            raise PsrError(rc, lib().psr_last_error(self._ctx).decode())
+        return out[:v.shape[0]].cpu().numpy()
 
     def GetRebuildMeshVertices(self):
         return self.export("MESH_VERTICES")

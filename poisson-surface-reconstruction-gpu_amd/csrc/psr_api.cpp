@@ -287,6 +287,25 @@ int psr_estimate_normals(psr_ctx* ctx, const float* xyz, uint32_t count, uint32_
     return PSR_OK;
 }
 
+int psr_vertex_colors(psr_ctx* ctx, const psr_mesh* mesh, const float* xyz, const float* point_colors,
+                      uint32_t count, uint32_t stride_bytes, float* colors) {
+    if (!ctx || !mesh || (mesh->n_vertices && (!mesh->vertices || !colors || !xyz || !point_colors)) ||
+        count > uint32_t(INT32_MAX) / 16 || mesh->n_vertices > uint32_t(INT32_MAX) / 4 ||
+        (stride_bytes && (stride_bytes < 12 || stride_bytes % 4)))
+        return PSR_ERR_INVALID;
+    try {
+        psr::vertex_colors(ctx->c, mesh->vertices, int(mesh->n_vertices), xyz, stride_bytes ? stride_bytes : 12,
+                           point_colors, int(count), colors);
+    } catch (const psr::HipError& e) {
+        return fail(ctx, PSR_ERR_HIP, e.what());
+    } catch (const std::invalid_argument& e) {
+        return fail(ctx, PSR_ERR_INVALID, e.what());
+    } catch (const std::exception& e) {
+        return fail(ctx, PSR_ERR_INTERNAL, e.what());
+    }
+    return PSR_OK;
+}
+
 int psr_reconstruct_batch(psr_ctx* ctx, const psr_points* clouds, uint32_t count, psr_mesh* outs) {
     if (!ctx || (count && (!clouds || !outs))) return PSR_ERR_INVALID;
     psr::Ctx& c = ctx->c;

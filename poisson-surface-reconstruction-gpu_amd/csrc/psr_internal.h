@@ -259,5 +259,8 @@ void mesh_normals(Ctx& c, const float* V, int nv, const int* T, int nt, float* o
 // unoriented point cloud; returns once written.
 void estimate_normals(Ctx& c, const float* xyz, int n, unsigned stride, int k, float3 vp, float* normals,
                       float* curvature, int* knn);
+// Colours of nv mesh vertices from their 4 nearest of n coloured points; returns once written.
+void vertex_colors(Ctx& c, const float* V, int nv, const float* xyz, unsigned stride, const float* pc, int n,
+                   float* out);
 
 }  // namespace psr

@@ -125,20 +125,14 @@ __device__ void jacobi3(double (&A)[3][3], double (&V)[3][3]) {
     }
 }
 
-// One thread per point: exact k nearest neighbours (top kMaxK kept sorted in
-// registers), then the plane fit.
-__global__ void __launch_bounds__(kThreads) ne_estimate(const char* __restrict__ xyz, unsigned stride, int n, int k,
-                                                        GridSpec g, const unsigned* __restrict__ order,
-                                                        const int* __restrict__ start, float3 vp,
-                                                        float* __restrict__ normals, float* __restrict__ curvature,
-                                                        int* __restrict__ knn) {
-    const int i = blockIdx.x * kThreads + threadIdx.x;
-    if (i >= n) return;
-    const float3 p = load_point(xyz, stride, i);
+// Exact k nearest points of p among the n grid points (top kMaxK kept sorted
+// in registers, ties to the lower index): the (2r+1)^3 cells around p's cell,
+// r growing until the k-th distance lies inside the searched box.  p may lie
+// anywhere (its cell is clamped into the grid).
+__device__ __forceinline__ void knn_search(float3 p, const char* __restrict__ xyz, unsigned stride, int kk,
+                                           const GridSpec& g, const unsigned* __restrict__ order,
+                                           const int* __restrict__ start, float (&dist)[kMaxK], int (&idx)[kMaxK]) {
     const int3 c = cell_of(p, g);
-    float dist[kMaxK];
-    int idx[kMaxK];
-    const int kk = min(k, n);
     for (int r = 1;; ++r) {
 #pragma unroll
         for (int j = 0; j < kMaxK; ++j) dist[j] = INFINITY, idx[j] = 0x7fffffff;
@@ -169,7 +163,8 @@ __global__ void __launch_bounds__(kThreads) ne_estimate(const char* __restrict__
                     }
                 }
             }
-        // the searched box covers every point within `cover` of p
+        // every point outside the box lies at least cover - eps from p (eps:
+        // the rounding of the cell assignment), so the k found are the nearest
         const bool all = x0 == 0 && y0 == 0 && z0 == 0 && x1 == g.G - 1 && y1 == g.G - 1 && z1 == g.G - 1;
         float cover = INFINITY;
         if (x0 > 0) cover = fminf(cover, p.x - (g.lo.x + x0 * g.h));
@@ -182,10 +177,23 @@ __global__ void __launch_bounds__(kThreads) ne_estimate(const char* __restrict__
 #pragma unroll
         for (int j = 0; j < kMaxK; ++j)
             if (j == kk - 1) dk = dist[j];
-        // a point outside the box lies at least cover - eps away (eps: the
-        // rounding of the cell assignment), so the k found are the nearest
-        if (all || (dk < INFINITY && sqrtf(dk) * (1.0f + 1e-6f) + g.eps < cover)) break;
+        if (all || (dk < INFINITY && sqrtf(dk) * (1.0f + 1e-6f) + g.eps < cover)) return;
     }
+}
+
+// One thread per point: its k nearest neighbours, then the plane fit.
+__global__ void __launch_bounds__(kThreads) ne_estimate(const char* __restrict__ xyz, unsigned stride, int n, int k,
+                                                        GridSpec g, const unsigned* __restrict__ order,
+                                                        const int* __restrict__ start, float3 vp,
+                                                        float* __restrict__ normals, float* __restrict__ curvature,
+                                                        int* __restrict__ knn) {
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const float3 p = load_point(xyz, stride, i);
+    float dist[kMaxK];
+    int idx[kMaxK];
+    const int kk = min(k, n);
+    knn_search(p, xyz, stride, kk, g, order, start, dist, idx);
     // mean and covariance of the neighbours in FP64
     double m[3] = {0, 0, 0};
     float3 nb[kMaxK];
@@ -225,14 +233,52 @@ __global__ void __launch_bounds__(kThreads) ne_estimate(const char* __restrict__
     }
 }
 
-}  // namespace
+// --- vertex colours (DrawMesh.cu:35-79, 179-211) ---
+// The colour of a mesh vertex from its 4 nearest points, weighted as the
+// reference weights them: each squared distance over their sum (the farther
+// point weighs more), colour = sum of weight x point colour.  Where all four
+// distances are zero the weights are equal (the reference divides by zero).
+__global__ void __launch_bounds__(kThreads) vc_colors(const float* __restrict__ V, int nv,
+                                                      const char* __restrict__ xyz, unsigned stride, int n,
+                                                      const float* __restrict__ pc, GridSpec g,
+                                                      const unsigned* __restrict__ order,
+                                                      const int* __restrict__ start, float* __restrict__ out) {
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= nv) return;
+    const float3 p = make_float3(V[3 * size_t(i)], V[3 * size_t(i) + 1], V[3 * size_t(i) + 2]);
+    float dist[kMaxK];
+    int idx[kMaxK];
+    const int kk = min(4, n);
+    knn_search(p, xyz, stride, kk, g, order, start, dist, idx);
+    float d[4] = {0.f, 0.f, 0.f, 0.f}, sum = 0.f;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        if (j >= kk) continue;
+        const float3 q = load_point(xyz, stride, idx[j]);
+        const float tx = __fsub_rn(p.x, q.x), ty = __fsub_rn(p.y, q.y), tz = __fsub_rn(p.z, q.z);
+        d[j] = __fadd_rn(__fadd_rn(__fmul_rn(tx, tx), __fmul_rn(ty, ty)), __fmul_rn(tz, tz));
+        sum = __fadd_rn(sum, d[j]);
+    }
+    float w[4];
+    const float inv = sum > 0.f ? __fdiv_rn(1.0f, sum) : 0.f;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) w[j] = j >= kk ? 0.f : (sum > 0.f ? __fmul_rn(d[j], inv) : __fdiv_rn(1.0f, float(kk)));
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float acc = 0.f;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (j < kk) acc = __fadd_rn(acc, __fmul_rn(pc[3 * size_t(idx[j]) + a], w[j]));
+        out[3 * size_t(i) + a] = acc;
+    }
+}
 
-void estimate_normals(Ctx& c, const float* xyz, int n, unsigned stride, int k, float3 vp, float* normals,
-                      float* curvature, int* knn) {
-    if (k < 1 || k > kMaxK) throw std::invalid_argument("k must lie in 1..16");
-    if (n <= 0) return;
+// Grid of the n points for exact nearest-neighbour searches: cell size from
+// about k points per occupied surface cell; points sorted by cell (order) and
+// each cell's first sorted position (start, G^3 + 1 entries).
+void build_knn_grid(Ctx& c, const char* base, unsigned stride, int n, int k, GridSpec& g, unsigned*& order,
+                    int*& start) {
     hipStream_t s = c.stream;
-    const char* base = reinterpret_cast<const char*>(xyz);
     const int nb = std::min<int>(128, int(div_up(n, kThreads)));
     float* part = c.buf<float>("ne_bounds", size_t(6) * nb);
     ne_bounds<<<nb, kThreads, 0, s>>>(base, stride, n, part);
@@ -248,7 +294,6 @@ void estimate_normals(Ctx& c, const float* xyz, int n, unsigned stride, int k, f
     // about k points per occupied cell on a surface: 4^L cells of a level meet it
     int L = int(std::lround(std::log(std::max(1.0, double(n) / double(k))) / std::log(4.0)));
     L = std::min(8, std::max(1, L));
-    GridSpec g;
     g.G = 1 << L;
     g.h = ext * 1.0001f / float(g.G);
     g.invH = 1.0f / g.h;
@@ -261,13 +306,42 @@ void estimate_normals(Ctx& c, const float* xyz, int n, unsigned stride, int k, f
     unsigned* val = c.buf<unsigned>("ne_val", n);
     unsigned* kt = c.buf<unsigned>("ne_kt", n);
     unsigned* vt = c.buf<unsigned>("ne_vt", n);
-    int* start = c.buf<int>("ne_start", size_t(cells) + 1);
+    start = c.buf<int>("ne_start", size_t(cells) + 1);
     ne_cells<<<div_up(n, kThreads), kThreads, 0, s>>>(base, stride, n, g, key, val);
     int* scratch = c.scratch_for(sort_scratch_ints(size_t(n)));
     radix_sort_pairs(key, val, kt, vt, n, 3 * L, scratch, false, c.scan_for(sort_scan_elems(size_t(n))), SortGather{},
                      s);
     ne_cell_starts<<<div_up(size_t(cells) + 1, kThreads), kThreads, 0, s>>>(key, n, cells, start);
-    ne_estimate<<<div_up(n, kThreads), kThreads, 0, s>>>(base, stride, n, k, g, val, start, vp, normals, curvature, knn);
+    order = val;
+}
+
+}  // namespace
+
+void estimate_normals(Ctx& c, const float* xyz, int n, unsigned stride, int k, float3 vp, float* normals,
+                      float* curvature, int* knn) {
+    if (k < 1 || k > kMaxK) throw std::invalid_argument("k must lie in 1..16");
+    if (n <= 0) return;
+    const char* base = reinterpret_cast<const char*>(xyz);
+    GridSpec g;
+    unsigned* order = nullptr;
+    int* start = nullptr;
+    build_knn_grid(c, base, stride, n, k, g, order, start);
+    ne_estimate<<<div_up(n, kThreads), kThreads, 0, c.stream>>>(base, stride, n, k, g, order, start, vp, normals,
+                                                                 curvature, knn);
+    c.wait();
+    PSR_HIP_CHECK(hipGetLastError());
+}
+
+void vertex_colors(Ctx& c, const float* V, int nv, const float* xyz, unsigned stride, const float* pc, int n,
+                   float* out) {
+    if (nv <= 0) return;
+    if (n <= 0) throw std::invalid_argument("no points to colour from");
+    const char* base = reinterpret_cast<const char*>(xyz);
+    GridSpec g;
+    unsigned* order = nullptr;
+    int* start = nullptr;
+    build_knn_grid(c, base, stride, n, 4, g, order, start);
+    vc_colors<<<div_up(nv, kThreads), kThreads, 0, c.stream>>>(V, nv, base, stride, n, pc, g, order, start, out);
     c.wait();
     PSR_HIP_CHECK(hipGetLastError());
 }
