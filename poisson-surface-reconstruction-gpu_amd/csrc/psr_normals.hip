@@ -340,7 +340,7 @@ void vertex_colors(Ctx& c, const float* V, int nv, const float* xyz, unsigned st
     GridSpec g;
     unsigned* order = nullptr;
     int* start = nullptr;
-    build_knn_grid(c, base, stride, n, 4, g, order, start);
+    build_knn_grid(c, base, stride, n, 10, g, order, start);  // cells sized as for 10 (the search is exact for 4)
     vc_colors<<<div_up(nv, kThreads), kThreads, 0, c.stream>>>(V, nv, base, stride, n, pc, g, order, start, out);
     c.wait();
     PSR_HIP_CHECK(hipGetLastError());
