@@ -304,6 +304,31 @@ class PoissonReconstruction:
             raise PsrError(rc, lib().psr_last_error(self._ctx).decode())
         return out[:v.shape[0]].cpu().numpy()
 
+    @staticmethod
+    def mesh_tensors(mesh):
+        """Zero-copy torch views (vertices (V, 3) float32, triangles (T, 3)
+        int32) of a psr_mesh's device buffers, through
+        __cuda_array_interface__; torch.utils.dlpack.to_dlpack hands them to
+        other consumers.  They alias memory owned by the context: valid until
+        its next reconstruction (or weld, for a welded mesh) or its close.
+        Replaces the reference's GL buffer mapping (mesh/DrawMesh.cpp:146-209)."""
+        import torch
+
+        class _View:
+            def __init__(self, ptr, shape, typestr):
+                self.__cuda_array_interface__ = {"shape": shape, "typestr": typestr, "data": (ptr, False),
+                                                 "version": 2, "strides": None}
+
+        if mesh.n_vertices:
+            v = torch.as_tensor(_View(mesh.vertices, (mesh.n_vertices, 3), "<f4"), device="cuda")
+        else:
+            v = torch.empty((0, 3), dtype=torch.float32, device="cuda")
+        if mesh.n_triangles:
+            t = torch.as_tensor(_View(mesh.triangles, (mesh.n_triangles, 3), "<i4"), device="cuda")
+        else:
+            t = torch.empty((0, 3), dtype=torch.int32, device="cuda")
+        return v, t
+
     def GetRebuildMeshVertices(self):
         return self.export("MESH_VERTICES")
 

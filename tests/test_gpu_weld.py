@@ -125,3 +125,19 @@ def test_normals_follow_the_input_orientation(welded, case):
     print(case, "median |cos|", float(np.median(np.abs(cos))), "orientation", sign, flush=True)
     assert np.mean(np.abs(cos) > 0.9) > 0.98
     assert np.mean(np.sign(cos) == sign) > 0.99  # one consistent orientation
+
+
+def test_mesh_tensors_alias_device_buffers(pkg):
+    import torch
+    xyz, nrm, d = pkg.synth.workload("sphere")
+    rec = pkg.PoissonReconstruction(depth=d)
+    mesh = rec.SolvePoissionReconstructionMesh(torch.from_numpy(xyz).cuda(), torch.from_numpy(nrm).cuda())
+    v, t = rec.mesh_tensors(mesh)
+    assert v.data_ptr() == mesh.vertices and t.data_ptr() == mesh.triangles  # no copy
+    assert np.array_equal(v.cpu().numpy(), rec.GetRebuildMeshVertices())
+    assert np.array_equal(t.cpu().numpy(), rec.GetRebuildMeshTriangleIndices())
+    back = torch.utils.dlpack.from_dlpack(torch.utils.dlpack.to_dlpack(v))
+    assert back.data_ptr() == mesh.vertices
+    wv, wt = rec.mesh_tensors(rec.WeldMesh())
+    assert wv.shape[1] == 3 and wt.shape[1] == 3 and int(wt.max()) < wv.shape[0]
+    rec.close()
