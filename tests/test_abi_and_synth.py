@@ -83,3 +83,17 @@ int main() {
                                          ctypes.sizeof(b.Stats)]
     assert int(out[4]) == b.Stats.ms_cg_kernel.offset and int(out[5]) == b.Stats.cg_nnz.offset
     assert out[6] == "7" and " ".join(out[7:]) == b.lib().psr_status_string(1).decode()
+
+
+def test_post_processing_entry_points_reject_bad_arguments(pkg):
+    """Argument checks of the welding, normal, colour and PLY entry points
+    (they return before touching the device)."""
+    b = pkg.binding
+    L = b.lib()
+    mesh = b.Mesh(None, None, 5, 0)  # vertices announced, no buffer
+    assert L.psr_weld_mesh(None, ctypes.byref(b.Mesh())) == b.PSR_ERR_INVALID
+    assert L.psr_mesh_normals(None, ctypes.byref(mesh), None) == b.PSR_ERR_INVALID
+    vp = (ctypes.c_float * 3)(0, 0, 0)
+    assert L.psr_estimate_normals(None, None, 10, 12, 10, vp, None, None, None) == b.PSR_ERR_INVALID
+    assert L.psr_vertex_colors(None, ctypes.byref(mesh), None, None, 10, 12, None) == b.PSR_ERR_INVALID
+    assert L.psr_write_ply(None, None, 0, None, 0) == b.PSR_ERR_INVALID
