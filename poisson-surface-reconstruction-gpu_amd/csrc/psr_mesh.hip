@@ -221,9 +221,8 @@ __global__ void __launch_bounds__(kThreads) geometry_emit(NodeView v, int offD, 
             for (int e = 0; e < 12; ++e) em |= (own[e] == o ? 1u : 0u) << e;
         }
     }
-    int av = ticket_prefix<kThreads>(tv, tile, __popc(vm));
-    int ae = ticket_prefix<kThreads>(te, tile, __popc(em));
-    int af = ticket_prefix<kThreads>(tf, tile, __popc(fm));
+    int av, ae, af;
+    ticket_prefix3<kThreads>(tv, te, tf, tile, __popc(vm), __popc(em), __popc(fm), av, ae, af);
     if (o >= offD && o < v.M) out.edgeOf[o - offD] = make_int2(ae, int(em));
     if (!(vm | em | fm)) return;
     const float half = 1.0f / float(1 << (depth + 1)), W = 1.0f / float(1 << depth), W2 = W * W;
