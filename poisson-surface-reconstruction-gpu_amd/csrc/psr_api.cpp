@@ -146,6 +146,7 @@ int run(psr_ctx* ctx, const psr_points* pts, psr_mesh* out) {
     out->n_vertices = uint32_t(c.mesh.nv);
     out->n_triangles = uint32_t(c.mesh.nt);
     c.mesh.valid = true;
+    c.cgClean = true;  // the neighbour-column kernel cleared the CG's controls after the solve
     return PSR_OK;
 }
 

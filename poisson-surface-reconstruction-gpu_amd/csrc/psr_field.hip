@@ -236,8 +236,9 @@ __global__ void __launch_bounds__(kDivCoarseThreads) divergence_coarse(
     const int* __restrict__ didx, const int* __restrict__ dnum, const int* __restrict__ enc,
     const FieldT* __restrict__ vf, const double* __restrict__ ddot, double* __restrict__ partial,
     int* __restrict__ nnz) {
-    // the operator's per-depth entry counts, added to beside the last divergence kernel
-    if (blockIdx.x == 0 && threadIdx.x <= PSR_MAX_DEPTH) nnz[threadIdx.x] = 0;
+    // the operator's per-depth entry counts, added to beside the last divergence
+    // kernel, and the CG error flag that follows them in the results block
+    if (blockIdx.x == 0 && threadIdx.x <= PSR_MAX_DEPTH + 1) nnz[threadIdx.x] = 0;
     int d = 0;
     while (d + 1 < P.depths && int(blockIdx.x) >= P.blockStart[d + 1]) ++d;
     const int parts = P.parts[d];

@@ -142,8 +142,9 @@ struct DevResults {
     double cgResid[PSR_MAX_DEPTH + 1];     // final updated residual norm per depth
     double cgTrueRel[PSR_MAX_DEPTH + 1];   // mixed form: final true relative residual per depth
 };
-static_assert(offsetof(DevResults, cgErr) == sizeof(int) * (3 * (PSR_MAX_DEPTH + 1)),
-              "iterations, nnz and the error flag are cleared as one range");
+static_assert(offsetof(DevResults, cgErr) == offsetof(DevResults, cgNnz) + sizeof(int) * (PSR_MAX_DEPTH + 1) &&
+                  offsetof(DevResults, cgErr) == sizeof(int) * (3 * (PSR_MAX_DEPTH + 1)),
+              "the operator counts and the CG error flag are cleared as one range");
 
 // Read-back mailbox slots (Ctx::hostPinned / Ctx::mail).  Each stage's
 // kernels write the values the host needs next; the host reads them after
@@ -170,6 +171,11 @@ struct Ctx {
     psr_stats stats{};
     int* scratch = nullptr;
     size_t scratchInts = 0;
+    // CG group controls and slots: zero unless a call left them in use
+    bool cgClean = false;
+    void* cgCtl = nullptr;
+    void* cgSlots = nullptr;
+    size_t cgCtlWords = 0, cgSlotWords = 0;
     hipEvent_t ev[16] = {};
     int* hostPinned = nullptr;   // read-back mailbox: pinned, coherent, kMailWords ints
     int* mail = nullptr;         // device view of hostPinned: kernels write read-back values there

@@ -846,9 +846,14 @@ __global__ void __launch_bounds__(kThreads) neighbour_columns(int M, const int* 
                                                               float4* __restrict__ nbrCol,
                                                               unsigned* __restrict__ zero, size_t nZero,
                                                               float* __restrict__ dense, size_t nDense,
-                                                              const int* __restrict__ q1Fired) {
+                                                              const int* __restrict__ q1Fired,
+                                                              unsigned* __restrict__ ctl, size_t nCtl,
+                                                              unsigned* __restrict__ slots, size_t nSlots) {
     const size_t t = size_t(blockIdx.x) * kThreads + threadIdx.x;
     for (size_t i = t; i < nZero; i += size_t(gridDim.x) * kThreads) zero[i] = 0u;
+    // the CG's group controls and slots, used: zero again for the next solve
+    if (t < nCtl) ctl[t] = 0u;
+    if (t < nSlots) slots[t] = 0u;
     if (nDense && *q1Fired == 0)
         for (size_t i = t; i < nDense; i += size_t(gridDim.x) * kThreads) dense[i] = 0.f;
     if (t >= size_t(27) * M) return;
@@ -962,14 +967,19 @@ void solve_depths(Ctx& c) {
     double* trueRel = res->cgTrueRel;
     int* err = &res->cgErr;
     float4* slots = rowForm && mixed ? c.buf<float4>("cg_slots", size_t(4) * wg) : nullptr;
-    // one launch clears the group controls, the slots and the CG results (the
-    // operator and its entry counts come from the field stage's last launch)
-    zero_spans({{ctl, size_t(ctlWords)},
-                {slots, slots ? size_t(16) * wg : 0},
-                {res->cgIters, sizeof(res->cgIters) / 4},
-                {&res->cgErr, 1},
-                {trueRel, 2 * (PSR_MAX_DEPTH + 1)}},
-               s);
+    // The group controls and slots start at zero: the neighbour-column kernel
+    // clears them after every solve, so they need a launch of their own only
+    // when the last call on this context did not complete or the slots moved.
+    // The CG writes every depth's results itself; its error flag is cleared
+    // with the operator counts (divergence_coarse).
+    const size_t slotWords = slots ? size_t(16) * wg : 0;
+    if (!c.cgClean || slots != c.cgSlots || slotWords != c.cgSlotWords)
+        zero_spans({{ctl, size_t(ctlWords)}, {slots, slotWords}}, s);
+    c.cgClean = false;
+    c.cgCtl = ctl;
+    c.cgCtlWords = size_t(ctlWords);
+    c.cgSlots = slots;
+    c.cgSlotWords = slotWords;
     // CG kernels of different contexts (streams) never compete for residency
     // while spinning in their group barriers: a launch is queued only once the
     // launches still in flight leave room for all its workgroups beside them
@@ -1056,7 +1066,8 @@ void points_implicit(Ctx& c) {
     c.geo.dxGrid = nDense ? c.buf<float>("sub_dx_grid", nDense) : nullptr;
     neighbour_columns<<<div_up(size_t(27) * o.M, kThreads), kThreads, 0, s>>>(
         o.M, o.neigh, o.basis, c.fld.dx, c.fld.nbrCol, reinterpret_cast<unsigned*>(zeroed), nZero, c.geo.dxGrid,
-        nDense, o.q1Fired);
+        nDense, o.q1Fired, reinterpret_cast<unsigned*>(c.cgCtl), c.cgCtlWords,
+        reinterpret_cast<unsigned*>(c.cgSlots), c.cgSlotWords);
     // the implicit function at the points runs beside the mesh stage's first
     // kernel (build_geometry), then the iso value
     c.fld.chiArgs = PointsChiArgs{N,        L.base[L.D], o.M,          L.D,      o.spos,        o.point2node,
