@@ -18,6 +18,7 @@
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "psr.h"
 
@@ -108,6 +109,47 @@ class PoissonReconstruction {
                                               mesh_.n_triangles);
     }
 
+    // One vertex per depth-D edge (psr_weld_mesh; the reference keeps its
+    // batches' duplicates).  The welded views are valid until the next
+    // reconstruction.
+    void WeldMesh() { check(psr_weld_mesh(ctx_, &welded_)); }
+    DeviceArrayView<Point3D<float>> GetWeldedMeshVertices() const {
+        return DeviceArrayView<Point3D<float>>(reinterpret_cast<const Point3D<float>*>(welded_.vertices),
+                                               welded_.n_vertices);
+    }
+    DeviceArrayView<TriangleIndex> GetWeldedMeshTriangleIndices() const {
+        return DeviceArrayView<TriangleIndex>(reinterpret_cast<const TriangleIndex*>(welded_.triangles),
+                                              welded_.n_triangles);
+    }
+
+    // Per-vertex normals of the mesh (or of its welded form) into a device
+    // buffer of 3 floats per vertex (DrawMesh::CalculateMeshNormals,
+    // mesh/DrawMesh.cpp).
+    void CalculateMeshNormals(float* normalsDevice, bool welded = false) {
+        check(psr_mesh_normals(ctx_, welded ? &welded_ : &mesh_, normalsDevice));
+    }
+
+    // Normals of an unoriented cloud in device memory, before a reconstruction
+    // (the reference's host-side PCL step, PoissonReconstruction.cpp:137-186).
+    void CalculatePointCloudNormal(const float* xyzDevice, uint32_t count, float* normalsDevice, int k = 10) {
+        const float vp[3] = {0.f, 0.f, 0.f};
+        check(psr_estimate_normals(ctx_, xyzDevice, count, 12, k, vp, normalsDevice, nullptr, nullptr));
+    }
+
+    // The mesh (or its welded form) as a binary PLY file, in the normalised frame.
+    void WriteMeshPly(const char* path, bool welded = false) {
+        const int vid = welded ? PSR_ARRAY_WELDED_VERTICES : PSR_ARRAY_MESH_VERTICES;
+        const int tid = welded ? PSR_ARRAY_WELDED_TRIANGLES : PSR_ARRAY_MESH_TRIANGLES;
+        uint64_t nvb = 0, ntb = 0;
+        check(psr_export(ctx_, vid, nullptr, 0, &nvb));
+        check(psr_export(ctx_, tid, nullptr, 0, &ntb));
+        std::vector<float> v(nvb / sizeof(float));
+        std::vector<int32_t> t(ntb / sizeof(int32_t));
+        check(psr_export(ctx_, vid, v.data(), nvb, &nvb));
+        check(psr_export(ctx_, tid, t.data(), ntb, &ntb));
+        check(psr_write_ply(path, v.data(), uint32_t(v.size() / 3), t.data(), uint32_t(t.size() / 3)));
+    }
+
     psr_stats Stats() const {
         psr_stats s;
         psr_get_stats(ctx_, &s);
@@ -116,13 +158,13 @@ class PoissonReconstruction {
     psr_ctx* Context() const { return ctx_; }
 
   private:
-    void run(const psr_points& p) {
-        int rc = psr_reconstruct(ctx_, &p, &mesh_);
-        if (rc != PSR_OK)
-            throw std::runtime_error(std::string(psr_status_string(rc)) + ": " + psr_last_error(ctx_));
+    void check(int rc) const {
+        if (rc != PSR_OK) throw std::runtime_error(std::string(psr_status_string(rc)) + ": " + psr_last_error(ctx_));
     }
+    void run(const psr_points& p) { check(psr_reconstruct(ctx_, &p, &mesh_)); }
     psr_ctx* ctx_ = nullptr;
     psr_mesh mesh_{};
+    psr_mesh welded_{};
 };
 
 }  // namespace SparseSurfelFusion

@@ -67,6 +67,10 @@ def test_cpp_facade_compiles_links_and_layouts_agree(pkg, tmp_path):
 #include <cstdio>
 #include "psr_facade.hpp"
 int main() {
+    using R = SparseSurfelFusion::PoissonReconstruction;  // the extra operations compile against psr.h
+    auto m1 = &R::WeldMesh; auto m2 = &R::CalculateMeshNormals; auto m3 = &R::CalculatePointCloudNormal;
+    auto m4 = &R::WriteMeshPly; auto m5 = &R::GetWeldedMeshVertices;
+    (void)m1; (void)m2; (void)m3; (void)m4; (void)m5;
     psr_config c; psr_default_config(&c);
     std::printf("%zu %zu %zu %zu %zu %zu %d %s\n", sizeof(psr_config), sizeof(psr_points), sizeof(psr_mesh),
                 sizeof(psr_stats), offsetof(psr_stats, ms_cg_kernel), offsetof(psr_stats, cg_nnz), c.depth,
