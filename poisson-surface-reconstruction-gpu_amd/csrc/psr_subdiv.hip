@@ -1265,7 +1265,8 @@ constexpr int kScanThreads = 1024;
 __global__ void __launch_bounds__(kScanThreads) sub_tile_scan(const int4* __restrict__ cnt, int n,
                                                               int4* __restrict__ base, const int* __restrict__ keptMain,
                                                               int* __restrict__ totals) {
-    __shared__ int lSum[kScanThreads / PSR_WAVE];
+    constexpr int kW = kScanThreads / PSR_WAVE;
+    __shared__ int3 lSum3[kW];
     const int per = (n + kScanThreads - 1) / kScanThreads;
     const int i0 = min(n, int(threadIdx.x) * per), i1 = min(n, i0 + per);
     int a = 0, b = 0, c = 0;
@@ -1275,10 +1276,26 @@ __global__ void __launch_bounds__(kScanThreads) sub_tile_scan(const int4* __rest
         b += t.y;
         c += t.z;
     }
-    int ta, tb, tc;
-    a = block_exclusive<kScanThreads>(a, lSum, ta);
-    b = block_exclusive<kScanThreads>(b, lSum, tb);
-    c = block_exclusive<kScanThreads>(c, lSum, tc);
+    // the three exclusive scans in one pass over the block (one barrier pair)
+    const int lane = __lane_id(), w = threadIdx.x / PSR_WAVE;
+    int xa = a, xb = b, xc = c;
+#pragma unroll
+    for (int o = 1; o < PSR_WAVE; o <<= 1) {
+        const int ya = __shfl_up(xa, o, PSR_WAVE), yb = __shfl_up(xb, o, PSR_WAVE), yc = __shfl_up(xc, o, PSR_WAVE);
+        if (lane >= o) xa += ya, xb += yb, xc += yc;
+    }
+    if (lane == PSR_WAVE - 1) lSum3[w] = make_int3(xa, xb, xc);
+    __syncthreads();
+    int ta = 0, tb = 0, tc = 0, pa = 0, pb = 0, pc = 0;
+#pragma unroll
+    for (int k = 0; k < kW; ++k) {
+        const int3 t = lSum3[k];
+        if (k < w) pa += t.x, pb += t.y, pc += t.z;
+        ta += t.x, tb += t.y, tc += t.z;
+    }
+    a = pa + xa - a;
+    b = pb + xb - b;
+    c = pc + xc - c;
     for (int i = i0; i < i1; ++i) {
         const int4 t = cnt[i];
         base[i] = make_int4(a, b, c, 0);
