@@ -142,6 +142,13 @@ struct Result {
     std::vector<int> batchV, batchT;    // vertex/triangle count per appended batch
     int nRoots = 0, nCoarseRoots = 0;
     std::vector<int> rootNodes;
+    // subdivision record (for the tests' independent owner and value checks):
+    //   subSets:  per set rd, coarse (0/1), first entry in subRoots, root count
+    //   subRoots: the real node index of every root, set by set
+    //   subV:     per virtual vertex set, owner key, corner kind, position (3 float bits),
+    //             value (float bits; NaN when the set emits no edge)
+    //   subE:     per virtual edge set, owner key, edge kind
+    std::vector<int> subSets, subRoots, subV, subE;
     // timings (ms)
     double tOctree = 0, tField = 0, tSolve = 0, tMesh = 0;
 };
@@ -1055,6 +1062,13 @@ static void mc_on_set(Result& r, SubSet& S, bool coarse) {
             if (own == NAS + o) V.push_back({{vp[0], vp[1], vp[2]}, own, i, D});
         }
     }
+    const int setId = int(r.subSets.size() / 4) - 1;
+    const size_t subV0 = r.subV.size();
+    for (const Vtx& vx : V) {
+        int rec[7] = {setId, S.nodes[vx.owner - NAS].key, vx.kind, 0, 0, 0, 0x7fc00000};
+        memcpy(&rec[3], vx.pos, 12);
+        r.subV.insert(r.subV.end(), rec, rec + 7);
+    }
     if (V.empty()) return;
     for (size_t v = 0; v < V.size(); ++v) {
         VNode& ow = S.nodes[V[v].owner - NAS];
@@ -1100,6 +1114,10 @@ static void mc_on_set(Result& r, SubSet& S, bool coarse) {
             }
             if (own == NAS + o) E.push_back({e, own});
         }
+    }
+    for (const Edge& ed : E) {
+        const int rec[3] = {setId, S.nodes[ed.owner - NAS].key, ed.kind};
+        r.subE.insert(r.subE.end(), rec, rec + 3);
     }
     if (E.empty()) return;
     for (size_t k = 0; k < E.size(); ++k) {
@@ -1147,6 +1165,7 @@ static void mc_on_set(Result& r, SubSet& S, bool coarse) {
             }
         }
         vv[v] = float(double(val) - r.iso);
+        memcpy(&r.subV[subV0 + 7 * v + 6], &vv[v], 4);
     }
     // crossings and triangles (:1040-1096, 528-553)
     std::vector<int> vexAddr(E.size());
@@ -1182,6 +1201,10 @@ static void marching_cubes(Result& r) {
     int D = r.D, offD = r.base[D], nD = r.count[D];
     std::vector<OctNode>& NA = r.nodes;
     vertex_values<Real>(r);
+    r.subSets.clear();
+    r.subRoots.clear();
+    r.subV.clear();
+    r.subE.clear();
     r.meshV.clear();
     r.meshT.clear();
     r.batchV.clear();
@@ -1330,6 +1353,11 @@ static void marching_cubes(Result& r) {
             }
         }
         S.nodes[0].parent = rootParent;
+        {
+            const int desc[4] = {rd, 1, int(r.subRoots.size()), 1};
+            r.subSets.insert(r.subSets.end(), desc, desc + 4);
+            r.subRoots.push_back(roots[i]);
+        }
         NA[rootParent].children[rootSon] = NAS;
         mc_on_set<Real>(r, S, true);
         NA[rootParent].children[rootSon] = rootIndex;
@@ -1394,6 +1422,11 @@ static void marching_cubes(Result& r) {
                 }
                 per *= 8;
             }
+        }
+        {
+            const int desc[4] = {rd, 0, int(r.subRoots.size()), nr};
+            r.subSets.insert(r.subSets.end(), desc, desc + 4);
+            for (int q = 0; q < nr; ++q) r.subRoots.push_back(roots[sel[q]]);
         }
         mc_on_set<Real>(r, S, false);
     }
@@ -1477,6 +1510,7 @@ static bool field(Handle* h, const std::string& nm, const void*& ptr, long long&
     F("cg_iters", r.cgIters, 4) F("cg_relres", r.cgRelRes, 8) F("cg_recres", r.cgRecRes, 8) F("nnz", r.nnz, 4)
     F("vvalue", r.vvalue, 8) F("cube", r.cube, 4) F("mesh_v", r.meshV, 4) F("mesh_t", r.meshT, 4)
     F("batch_v", r.batchV, 4) F("batch_t", r.batchT, 4) F("roots", r.rootNodes, 4)
+    F("sub_sets", r.subSets, 4) F("sub_roots", r.subRoots, 4) F("sub_v", r.subV, 4) F("sub_e", r.subE, 4)
     F("dot", h->t.dot, 8) F("ddot", h->t.ddot, 8) F("d2dot", h->t.d2dot, 8)
 #undef F
     return false;
@@ -1580,4 +1614,15 @@ double orc_inner(int a, int b, int da, int db) { return inner(a, b, da, db); }
 double orc_basis(int idx, double x) { return basis_at<double>(idx, x); }
 int orc_lut(int which, int s, int i) { return which ? kLut.child[s][i] : kLut.parent[s][i]; }
 long long orc_mc_word(int cube) { return (long long)kOracleMcTable[cube]; }
+// Entry i of a small quirk-defining table: 0 childrenVertexKind[8], 1 edgeVertex[12][2]
+// (flattened), 2 parentFaceKind[8][6] (flattened), 3 faceEdges[6][4] (flattened).
+int orc_small_table(int which, int i) {
+    switch (which) {
+        case 0: return (i >= 0 && i < 8) ? kChildrenVertexKind[i] : -99;
+        case 1: return (i >= 0 && i < 24) ? kEdgeVertex[i / 2][i % 2] : -99;
+        case 2: return (i >= 0 && i < 48) ? kParentFaceKind[i / 6][i % 6] : -99;
+        case 3: return (i >= 0 && i < 24) ? kFaceEdges[i / 4][i % 4] : -99;
+        default: return -99;
+    }
+}
 }

@@ -7,8 +7,11 @@ buffers owned by the object, read back with ``GetRebuildMeshVertices`` /
 ``GetRebuildMeshTriangleIndices`` (mesh/ComputeTriangleIndices.h:331-342).
 There is no CPU fallback: a missing library raises.
 """
+import atexit
 import ctypes
 import os
+import sys
+import weakref
 
 import numpy as np
 
@@ -145,6 +148,31 @@ def lib():
     return _lib
 
 
+# Contexts still open; closed at interpreter exit, before the HIP runtime and
+# torch tear down, so that no HIP call runs during finalisation.
+_live = weakref.WeakSet()
+
+
+@atexit.register
+def _close_all():
+    for rec in list(_live):
+        try:
+            rec.close()
+        except Exception:
+            pass
+
+
+def _check_points(t, width, name):
+    """A contiguous float32 (N, width) tensor resident on the GPU."""
+    import torch
+    if not (isinstance(t, torch.Tensor) and t.is_cuda):
+        raise TypeError("%s must be a CUDA tensor" % name)
+    if t.dtype != torch.float32 or t.dim() != 2 or t.shape[1] != width:
+        raise ValueError("%s must be float32 of shape (N, %d), got %s %s" % (name, width, t.dtype, tuple(t.shape)))
+    if not t.is_contiguous():
+        raise ValueError("%s must be contiguous" % name)
+
+
 class PsrError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__("%s (%d): %s" % (lib().psr_status_string(code).decode(), code, msg))
@@ -166,11 +194,18 @@ class PoissonReconstruction:
         cfg.subdiv_chunk_nodes = subdiv_chunk_nodes
         cfg.batch_workers = batch_workers
         self._ctx = ctypes.c_void_p()
-        rc = lib().psr_create(ctypes.byref(cfg), stream, ctypes.byref(self._ctx))
+        # The context works on a stream of its own, which waits for the
+        # caller's current stream before every call (inputs written by torch
+        # kernels still in flight are read only once they are complete).
+        import torch
+        self._stream = torch.cuda.Stream() if stream is None else None
+        handle = self._stream.cuda_stream if stream is None else stream
+        rc = lib().psr_create(ctypes.byref(cfg), ctypes.c_void_p(handle), ctypes.byref(self._ctx))
         if rc != PSR_OK:
             raise PsrError(rc, "psr_create")
         self.depth = depth
         self.mesh = Mesh()
+        _live.add(self)
 
     def close(self):
         if self._ctx:
@@ -178,12 +213,22 @@ class PoissonReconstruction:
             self._ctx = ctypes.c_void_p()
 
     def __del__(self):
+        # during interpreter finalisation the atexit hook has closed the
+        # context already; no HIP call may run then
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
             pass
 
+    def _order_after_caller(self):
+        if self._stream is not None:
+            import torch
+            self._stream.wait_stream(torch.cuda.current_stream())
+
     def solve_ptr(self, xyz_ptr, nrm_ptr, count, stride_bytes=12):
+        self._order_after_caller()
         pts = Points(xyz_ptr, nrm_ptr, count, stride_bytes)
         rc = lib().psr_reconstruct(self._ctx, ctypes.byref(pts), ctypes.byref(self.mesh))
         if rc != PSR_OK:
@@ -194,15 +239,25 @@ class PoissonReconstruction:
         """xyz, normals: contiguous float32 GPU tensors of shape (N, 3), or one
         (N, 16) float32 tensor of DepthSurfel records (xyz at 4..6, normal at 8..10)."""
         if normals is None:
+            _check_points(xyz, 16, "DepthSurfel records")
             base = xyz.data_ptr()
             return self.solve_ptr(base + 16, base + 32, xyz.shape[0], 64)
-        assert xyz.is_cuda and normals.is_cuda and xyz.is_contiguous() and normals.is_contiguous()
+        _check_points(xyz, 3, "xyz")
+        _check_points(normals, 3, "normals")
+        if normals.shape[0] != xyz.shape[0]:
+            raise ValueError("xyz and normals differ in length")
         return self.solve_ptr(xyz.data_ptr(), normals.data_ptr(), xyz.shape[0], 12)
 
     def ReconstructBatch(self, clouds):
         """clouds: list of (xyz, normals) contiguous float32 GPU tensor pairs.
         Returns one psr_mesh per cloud (device buffers valid until the next batch)."""
         n = len(clouds)
+        for x, nn in clouds:
+            _check_points(x, 3, "xyz")
+            _check_points(nn, 3, "normals")
+            if nn.shape[0] != x.shape[0]:
+                raise ValueError("xyz and normals differ in length")
+        self._order_after_caller()
         pts = (Points * n)(*[Points(x.data_ptr(), nn.data_ptr(), x.shape[0], 12) for x, nn in clouds])
         self.batch = (Mesh * n)()
         rc = lib().psr_reconstruct_batch(self._ctx, pts, n, self.batch)

@@ -96,6 +96,7 @@ int run(psr_ctx* ctx, const psr_points* pts, psr_mesh* out) {
     if (pts->stride_bytes && (pts->stride_bytes < 12 || pts->stride_bytes % 4))
         return fail(ctx, PSR_ERR_INVALID, "bad stride");
     std::memset(&c.stats, 0, sizeof(c.stats));
+    c.clear_scan_error();
     c.mesh.valid = false;
     c.mesh.weldedNv = c.mesh.weldedNt = 0;
     PSR_HIP_CHECK(hipEventRecord(c.ev[0], s));
@@ -120,6 +121,7 @@ int run(psr_ctx* ctx, const psr_points* pts, psr_mesh* out) {
     for (int d = 0; d <= o.L.D; ++d) st.nodes_per_depth[d] = o.L.count[d];
     const DevResults& r = *c.hostResults;
     if (r.cgErr) return fail(ctx, PSR_ERR_INTERNAL, "CG group barrier timed out");
+    if (c.scan_failed()) return fail(ctx, PSR_ERR_INTERNAL, "scan look-back timed out");
     for (int d = 0; d <= o.L.D; ++d) {
         st.cg_iterations[d] = r.cgIters[d];
         st.cg_outer[d] = r.cgIters[PSR_MAX_DEPTH + 1 + d];
@@ -240,7 +242,9 @@ int psr_weld_mesh(psr_ctx* ctx, psr_mesh* out) {
     psr::Ctx& c = ctx->c;
     if (!c.mesh.valid) return fail(ctx, PSR_ERR_INVALID, "no completed reconstruction to weld");
     try {
+        c.clear_scan_error();
         psr::weld_mesh(c);
+        if (c.scan_failed()) return fail(ctx, PSR_ERR_INTERNAL, "scan look-back timed out");
     } catch (const psr::HipError& e) {
         return fail(ctx, PSR_ERR_HIP, e.what());
     } catch (const std::exception& e) {
@@ -259,8 +263,10 @@ int psr_mesh_normals(psr_ctx* ctx, const psr_mesh* mesh, float* normals) {
         mesh->n_triangles > uint32_t(INT32_MAX / 3))
         return PSR_ERR_INVALID;
     try {
+        ctx->c.clear_scan_error();
         psr::mesh_normals(ctx->c, mesh->vertices, int(mesh->n_vertices), mesh->triangles, int(mesh->n_triangles),
                           normals);
+        if (ctx->c.scan_failed()) return fail(ctx, PSR_ERR_INTERNAL, "scan look-back timed out");
     } catch (const psr::HipError& e) {
         return fail(ctx, PSR_ERR_HIP, e.what());
     } catch (const std::exception& e) {
@@ -276,8 +282,10 @@ int psr_estimate_normals(psr_ctx* ctx, const float* xyz, uint32_t count, uint32_
         return PSR_ERR_INVALID;
     const float3 vp = viewpoint ? make_float3(viewpoint[0], viewpoint[1], viewpoint[2]) : make_float3(0.f, 0.f, 0.f);
     try {
+        ctx->c.clear_scan_error();
         psr::estimate_normals(ctx->c, xyz, int(count), stride_bytes ? stride_bytes : 12, k, vp, normals, curvature,
                               knn);
+        if (ctx->c.scan_failed()) return fail(ctx, PSR_ERR_INTERNAL, "scan look-back timed out");
     } catch (const psr::HipError& e) {
         return fail(ctx, PSR_ERR_HIP, e.what());
     } catch (const std::invalid_argument& e) {
@@ -295,8 +303,10 @@ int psr_vertex_colors(psr_ctx* ctx, const psr_mesh* mesh, const float* xyz, cons
         (stride_bytes && (stride_bytes < 12 || stride_bytes % 4)))
         return PSR_ERR_INVALID;
     try {
+        ctx->c.clear_scan_error();
         psr::vertex_colors(ctx->c, mesh->vertices, int(mesh->n_vertices), xyz, stride_bytes ? stride_bytes : 12,
                            point_colors, int(count), colors);
+        if (ctx->c.scan_failed()) return fail(ctx, PSR_ERR_INTERNAL, "scan look-back timed out");
     } catch (const psr::HipError& e) {
         return fail(ctx, PSR_ERR_HIP, e.what());
     } catch (const std::invalid_argument& e) {

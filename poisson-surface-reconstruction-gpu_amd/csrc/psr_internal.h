@@ -154,6 +154,7 @@ constexpr int kMailNorm = 32;                // octree: 4 normalisation paramete
 constexpr int kMailLevelStart = 4;           // depth-D MC: roots before each level
 constexpr int kMailQ1 = PSR_MAX_DEPTH + 8;   // depth-D MC: the level-local child 0 rule fired
 constexpr int kMailGeo = 40;                 // geometry: vertex, edge and face counts
+constexpr int kMailScanErr = 63;             // any call: a scan's look-back reached its spin bound
 static_assert(kMailQ1 < kMailNorm && kMailNorm + 4 <= kMailGeo && kMailGeo + 3 <= kMailWords,
               "mailbox slots overlap");
 
@@ -198,6 +199,11 @@ struct Ctx {
         return static_cast<T*>(arena.get_keep(name, sizeof(T) * (count ? count : 1), sizeof(T) * keepCount));
     }
     DevResults* results() { return buf<DevResults>("results", 1); }
+    // The scan-error word of the mailbox: cleared by the host when a call
+    // starts (no kernel of the context is in flight then), read after its
+    // last wait.
+    void clear_scan_error() { hostPinned[kMailScanErr] = 0; }
+    bool scan_failed() const { return __atomic_load_n(&hostPinned[kMailScanErr], __ATOMIC_ACQUIRE) != 0; }
     // Waits for the stream by polling it: a blocking wait adds the host thread's
     // wake-up latency to every read-back.  Errors surface through the final
     // synchronisation.
@@ -233,6 +239,7 @@ struct Ctx {
             scan.capacity = cap;
             scan.epoch = 0;
         }
+        scan.err = mail + kMailScanErr;
         return scan;
     }
     int* scratch_for(size_t ints) {

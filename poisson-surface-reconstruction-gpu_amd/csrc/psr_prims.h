@@ -17,6 +17,7 @@ struct ScanState {
     unsigned* counters = nullptr;
     size_t capacity = 0;  // tiles
     unsigned epoch = 0;
+    int* err = nullptr;   // set to 1 by a look-back whose wait reached its spin bound (may be null)
 };
 constexpr int kScanRing = 64;             // tile counters, one per epoch modulo the ring
 constexpr unsigned kScanEpochMax = 1u << 29;
@@ -29,6 +30,7 @@ struct ScanTicket {
     unsigned epoch;
     int tiles;
     int* total;
+    int* err;  // set to 1 when a look-back wait reaches its spin bound (may be null)
 };
 // Takes the next epoch for a launch of `tiles` workgroups (the state must hold them).
 ScanTicket scan_ticket(ScanState& st, int tiles, int* total);

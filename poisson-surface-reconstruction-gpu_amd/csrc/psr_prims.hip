@@ -77,7 +77,7 @@ ScanTicket scan_ticket(ScanState& st, int tiles, int* total) {
     if (size_t(tiles) > st.capacity || st.epoch + 1 >= kScanEpochMax)
         throw std::runtime_error("scan state not reserved for this size");
     ++st.epoch;
-    return ScanTicket{st.status, st.counters, st.epoch, tiles, total};
+    return ScanTicket{st.status, st.counters, st.epoch, tiles, total, st.err};
 }
 
 void scan_exclusive(const int* in, int* out, int n, int* total, ScanState& st, hipStream_t s) {

@@ -83,6 +83,8 @@ __device__ __forceinline__ int wave_lookback(const ScanTicket& t, int tile, int 
                         break;
                     }
                     if (spins > (1u << 24)) {
+                        // a broken state: the prefix is wrong, so the call must fail
+                        if (t.err) __hip_atomic_store(t.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
                         flag = 2u;
                         break;
                     }

@@ -1,6 +1,8 @@
 """Writes tests/golden/reference_tables.json: the small lookup tables the
 reference declares as data (LUTparent / LUTchild, base/Constants.h:5-24;
-trianglesCount, mesh/ComputeTriangleIndices.cu:41-56).  Run in the build
+trianglesCount and the triangles[256][16] triangulation,
+mesh/ComputeTriangleIndices.cu:41-315; childrenVertexKind and edgeVertex,
+:31-38; parentFaceKind, mesh/BuildMeshGeometry.cu:19-26).  Run in the build
 container, where the reference sources are readable; the output is a fixture
 of numbers only.
 """
@@ -24,6 +26,10 @@ out = {
     "LUTparent": grab("base/Constants.h", r"LUTparent_Host\[8\]\[27\]", 216),
     "LUTchild": grab("base/Constants.h", r"LUTchild_Host\[8\]\[27\]", 216),
     "trianglesCount": grab("mesh/ComputeTriangleIndices.cu", r"trianglesCount\[256\]", 256),
+    "triangles": grab("mesh/ComputeTriangleIndices.cu", r"int triangles\[256\]\[16\]", 4096),
+    "childrenVertexKind": grab("mesh/ComputeTriangleIndices.cu", r"childrenVertexKind\[8\]", 8),
+    "edgeVertex": grab("mesh/ComputeTriangleIndices.cu", r"edgeVertex\[12\]\[2\]", 24),
+    "parentFaceKind": grab("mesh/BuildMeshGeometry.cu", r"__constant__ int parentFaceKind\[8\]\[6\]", 48),
 }
 here = os.path.dirname(os.path.abspath(__file__))
 with open(os.path.join(here, "reference_tables.json"), "w") as f:

@@ -131,6 +131,25 @@ def test_cube_topology(runs, case):
     np.testing.assert_array_equal(cube[safe], want[safe])
 
 
+def triangle_overlap(v, t, ov, ot, tol=1e-6):
+    """Fractions of the triangles of each mesh found in the other, order
+    independent: vertices of both meshes closer than tol are one point
+    (connected components of the pairs within tol), a triangle is the set of
+    its three points."""
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import connected_components
+    allv = np.concatenate([v, ov]).astype(np.float64)
+    pairs = cKDTree(allv).query_pairs(tol, output_type="ndarray")
+    n = len(allv)
+    g = coo_matrix((np.ones(len(pairs)), (pairs[:, 0], pairs[:, 1])), shape=(n, n))
+    _, lab = connected_components(g, directed=False)
+    a = np.sort(lab[t], axis=1)
+    b = np.sort(lab[ot + len(v)], axis=1)
+    sa = set(map(tuple, a))
+    sb = set(map(tuple, b))
+    return len(sa & sb) / max(len(sb), 1), len(sa & sb) / max(len(sa), 1)
+
+
 @pytest.mark.parametrize("case", list(CASES))
 def test_mesh(runs, case):
     rec, ref, *_ = runs[case]
@@ -141,9 +160,26 @@ def test_mesh(runs, case):
     diag = float(np.linalg.norm(hi - lo))
     h = max(cKDTree(ov).query(v)[0].max(), cKDTree(v).query(ov)[0].max())
     assert h <= 1e-4 * diag, h
-    if len(v) == len(ov) and len(t) == len(ot) and np.array_equal(rec.export("CUBE"), ref.array("cube")):
+    # every triangle, order independent: the oracle's triangles (as vertex
+    # sets) are the GPU's, except around corners whose value is near zero
+    found, extra = triangle_overlap(v, t, ov, ot)
+    print("%s: oracle triangles found %.6f, GPU triangles found %.6f" % (case, found, extra))
+    assert found >= 0.998 and extra >= 0.998, (found, extra)
+    exact = len(v) == len(ov) and len(t) == len(ot) and np.array_equal(rec.export("CUBE"), ref.array("cube"))
+    print("%s: exact triangle comparison %s" % (case, "ran" if exact else "skipped (cube classes differ)"))
+    if exact:
         np.testing.assert_array_equal(t, ot)
         assert np.abs(v - ov).max() <= 1e-4 * diag
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_operator_entries_per_depth(runs, case):
+    """Stored operator entries per depth (the |L| <= 1e-6 drop,
+    LaplacianSolver.cu:62) equal the oracle's CSR counts."""
+    rec, ref, *_ = runs[case]
+    nnz = ref.array("nnz")
+    st = rec.stats()
+    np.testing.assert_array_equal(np.array(st["cg_nnz"]), nnz[:len(st["cg_nnz"])])
 
 
 def test_deterministic_and_reusable(pkg, torch, runs):
