@@ -48,8 +48,15 @@ typedef struct {
     int refine_max_outer;  /* PSR_PRECISION_MIXED: outer refinement steps, 4 */
     uint32_t subdiv_chunk_nodes; /* virtual depth-D cells per subdivision working set; 0 = 16M
                                     (about 3 GB); results do not depend on it */
-    uint32_t batch_workers;      /* psr_reconstruct_batch: clouds in flight at once (own stream and
-                                    buffers each); 0 = 8 */
+    uint32_t batch_workers;      /* psr_reconstruct_batch: clouds merged into each launch (a pass);
+                                    0 = 64 */
+    int32_t batch_cg_plan;       /* psr_reconstruct_batch, how the clouds' CG solves are laid out:
+                                    0 = the single-cloud plan (256 rows a workgroup, in as many waves
+                                    of the resident budget as needed: meshes equal psr_reconstruct's
+                                    bit for bit); w >= 1 = every depth's rows split evenly over
+                                    workgroups so that all solves fit w waves; -1 (default) = picked
+                                    per batch shape by the built-in tuner, which times the plans on
+                                    the first calls of each shape */
 } psr_config;
 
 typedef struct {
@@ -139,17 +146,28 @@ int psr_create(const psr_config* cfg, void* hip_stream, psr_ctx** out);
 /* Runs the whole pipeline; on return the mesh counts are final and the buffers
  * are ready on the context's stream. */
 int psr_reconstruct(psr_ctx* ctx, const psr_points* points, psr_mesh* out);
-/* Reconstructs count independent clouds.  Up to cfg.batch_workers clouds are
- * in flight at once, each on a stream and buffers of its own, so that one
- * cloud's latency-bound stages overlap another's work; the concurrent CG
- * kernels run one after another on the device.  outs[i] receives cloud i's
- * mesh (device buffers owned by the context, valid until the next batch call or
- * psr_destroy); the call returns when every mesh is complete.  No counterpart
- * in the reference, which reconstructs one cloud per call
- * (mesh/PoissonReconstruction.cpp:195-245). */
+/* Reconstructs count independent clouds of uneven size with one launch per
+ * stage: the clouds' sorts run as one segmented sort, every stage's kernels
+ * take all clouds' work in one grid, and the per-depth CG solves of all clouds
+ * run as independent groups of one persistent launch (waves within the
+ * resident budget), each group stopping on its own convergence.  A phase's
+ * launches are replayed from a captured HIP graph when their structure
+ * recurs.  cfg.batch_workers clouds are merged per pass (0: 64).  outs[i]
+ * receives cloud i's mesh (device buffers owned by the context, valid until
+ * the next batch call or psr_destroy); the call returns when every mesh is
+ * complete.  A failing cloud stops no other: the call returns the first
+ * failure (its cloud named in psr_last_error) with the others' meshes
+ * written.  No counterpart in the reference, which reconstructs one cloud per
+ * call (mesh/PoissonReconstruction.cpp:195-245). */
 int psr_reconstruct_batch(psr_ctx* ctx, const psr_points* clouds, uint32_t count, psr_mesh* outs);
 /* Statistics of cloud `index` of the last batch. */
 int psr_get_batch_stats(psr_ctx* ctx, uint32_t index, psr_stats* out);
+/* The context that reconstructed cloud `index` of the last batch (for
+ * psr_export and psr_get_stats; owned by ctx, valid until its next batch call
+ * or psr_destroy), or NULL. */
+psr_ctx* psr_batch_cloud(psr_ctx* ctx, uint32_t index);
+/* The CG plan the last batch call ran with (see psr_config.batch_cg_plan). */
+int psr_batch_cg_plan(const psr_ctx* ctx);
 int psr_sync(psr_ctx* ctx);
 int psr_get_stats(psr_ctx* ctx, psr_stats* out);
 /* Copies array `array_id` of the last reconstruction to host memory.  With

@@ -15,6 +15,7 @@ _lib = None
 
 STAGE_OCTREE, STAGE_FIELD, STAGE_SOLVE, STAGE_MESH = 1, 2, 4, 8
 STAGE_ALL = 15
+STAGE_GEOMETRY = 16  # with 1|2|4 and without 8: ownership and vertex values, no marching cubes
 
 
 def build():

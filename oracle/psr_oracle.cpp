@@ -1451,6 +1451,9 @@ static void run_numeric(Result& r, const Tables& t, int stages) {
     if (stages & 8) {
         geometry(r);
         marching_cubes<Real>(r);
+    } else if (stages & 16) {  // ownership and vertex values only, no marching cubes
+        geometry(r);
+        vertex_values<Real>(r);
     }
     double t3 = now_ms();
     r.tField = t1 - t0;
@@ -1470,7 +1473,8 @@ struct Handle {
 
 extern "C" {
 
-// stages: 1 octree, 2 field+divergence, 4 solve+point chi, 8 geometry+MC+subdivision.
+// stages: 1 octree, 2 field+divergence, 4 solve+point chi, 8 geometry+MC+subdivision,
+// 16 (without 8) geometry and vertex values only.
 // precision: 0 = FP32 restatement, 1 = FP64 oracle.
 void* orc_run(const float* xyz, const float* nrm, int n, int depth, int stages, int precision, int nthreads) {
 #ifdef _OPENMP
@@ -1482,7 +1486,7 @@ void* orc_run(const float* xyz, const float* nrm, int n, int depth, int stages, 
     double t0 = now_ms();
     build_octree(h->r, xyz, nrm, n, depth);
     h->r.tOctree = now_ms() - t0;
-    if (stages & 14) {
+    if (stages & 30) {
         build_tables(h->t, depth);
         h->tablesDepth = depth;
         if (precision)

@@ -38,7 +38,7 @@ _ARRAY_LAYOUT = {
     "WELDED_TRIANGLES": (np.int32, 3)}
 
 EXPORTED_SYMBOLS = ["psr_default_config", "psr_create", "psr_reconstruct", "psr_reconstruct_batch",
-                    "psr_get_batch_stats", "psr_sync", "psr_get_stats", "psr_export", "psr_destroy",
+                    "psr_get_batch_stats", "psr_batch_cloud", "psr_batch_cg_plan", "psr_sync", "psr_get_stats", "psr_export", "psr_destroy",
                     "psr_status_string", "psr_last_error", "psr_weld_mesh", "psr_write_ply", "psr_mesh_normals", "psr_estimate_normals",
                     "psr_vertex_colors"]
 
@@ -47,7 +47,7 @@ class Config(ctypes.Structure):
     _fields_ = [("depth", ctypes.c_int), ("max_points", ctypes.c_uint32), ("precision", ctypes.c_int),
                 ("cg_tol", ctypes.c_float), ("cg_max_iter", ctypes.c_int), ("refine_rel_tol", ctypes.c_float),
                 ("refine_max_outer", ctypes.c_int), ("subdiv_chunk_nodes", ctypes.c_uint32),
-                ("batch_workers", ctypes.c_uint32)]
+                ("batch_workers", ctypes.c_uint32), ("batch_cg_plan", ctypes.c_int32)]
 
 
 class Points(ctypes.Structure):
@@ -126,6 +126,9 @@ def lib():
         L.psr_reconstruct_batch.argtypes = [ctypes.c_void_p, ctypes.POINTER(Points), ctypes.c_uint32,
                                             ctypes.POINTER(Mesh)]
         L.psr_get_batch_stats.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(Stats)]
+        L.psr_batch_cloud.restype = ctypes.c_void_p
+        L.psr_batch_cloud.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
+        L.psr_batch_cg_plan.argtypes = [ctypes.c_void_p]
         L.psr_sync.argtypes = [ctypes.c_void_p]
         L.psr_get_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(Stats)]
         L.psr_export.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64,
@@ -183,7 +186,7 @@ class PoissonReconstruction:
     """Host-side mirror of SparseSurfelFusion::PoissonReconstruction."""
 
     def __init__(self, depth=7, precision=PRECISION_FP32, max_points=0, cg_tol=1e-5, cg_max_iter=10000,
-                 stream=None, subdiv_chunk_nodes=0, batch_workers=0):
+                 stream=None, subdiv_chunk_nodes=0, batch_workers=0, batch_cg_plan=-1):
         cfg = Config()
         lib().psr_default_config(ctypes.byref(cfg))
         cfg.depth = depth
@@ -193,6 +196,7 @@ class PoissonReconstruction:
         cfg.cg_max_iter = cg_max_iter
         cfg.subdiv_chunk_nodes = subdiv_chunk_nodes
         cfg.batch_workers = batch_workers
+        cfg.batch_cg_plan = batch_cg_plan
         self._ctx = ctypes.c_void_p()
         # The context works on a stream of its own, which waits for the
         # caller's current stream before every call (inputs written by torch
@@ -212,10 +216,11 @@ class PoissonReconstruction:
             lib().psr_destroy(self._ctx)
             self._ctx = ctypes.c_void_p()
 
-    def __del__(self):
+    def __del__(self, _finalizing=sys.is_finalizing):
         # during interpreter finalisation the atexit hook has closed the
-        # context already; no HIP call may run then
-        if sys.is_finalizing():
+        # context already; no HIP call may run then (module globals may be
+        # gone, so the check is bound at definition)
+        if _finalizing():
             return
         try:
             self.close()
@@ -264,6 +269,25 @@ class PoissonReconstruction:
         if rc != PSR_OK:
             raise PsrError(rc, lib().psr_last_error(self._ctx).decode())
         return list(self.batch)
+
+    def batch_cloud(self, i):
+        """A view of the context that reconstructed cloud i of the last batch
+        (export / stats), valid until the next batch call or close."""
+        h = lib().psr_batch_cloud(self._ctx, i)
+        if not h:
+            raise PsrError(PSR_ERR_INVALID, "no batch cloud %d" % i)
+        view = PoissonReconstruction.__new__(PoissonReconstruction)
+        view._ctx = ctypes.c_void_p(h)
+        view._stream = None
+        view.depth = self.depth
+        view.mesh = Mesh()
+        view.close = lambda: None  # owned by the batch context
+        return view
+
+    def batch_cg_plan(self):
+        """The CG plan the last batch call ran with (0: the single-cloud plan;
+        w: every solve in w waves of the resident budget)."""
+        return lib().psr_batch_cg_plan(self._ctx)
 
     def batch_stats(self, i):
         st = Stats()

@@ -11,7 +11,7 @@ LIB = os.path.join(HERE, "libpsr.so")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
 SOURCES = ["psr_prims.hip", "psr_octree.hip", "psr_field.hip", "psr_solver.hip", "psr_mesh.hip",
-           "psr_subdiv.hip", "psr_weld.hip", "psr_normals.hip", "psr_api.cpp"]
+           "psr_subdiv.hip", "psr_weld.hip", "psr_normals.hip", "psr_api.cpp", "psr_batch.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PSR_OFFLOAD_ARCH", "gfx950")
@@ -24,7 +24,7 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
 def _compile(src):
     obj = os.path.join(BUILD, src + ".o")
     cmd = [HIPCC] + FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
-    if src.endswith(".cpp"):
+    if src.endswith(".cpp"):  # host code that includes the HIP headers
         cmd = [HIPCC, "-x", "hip"] + FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:

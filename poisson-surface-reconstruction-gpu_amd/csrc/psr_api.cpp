@@ -14,22 +14,37 @@
 
 #include "psr_internal.h"
 
-struct psr_ctx {
-    psr::Ctx c;
-    std::string lastError;
-    // batch mode
-    std::vector<psr_ctx*> workers;  // contexts of their own (stream, buffers) for clouds in flight
-    std::vector<psr_stats> batchStats;
-    std::mutex batchMu;             // guards the arena of this context while workers copy out
-};
 
 namespace psr {
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return -1.f;
+    return ms;
+}
+
+Recorder*& current_recorder() {
+    static thread_local Recorder* r = nullptr;
+    return r;
+}
+
+void Arena::free_block(void* p) {
+    if (defer_) deferred_.push_back(p);
+    else PSR_HIP_CHECK(hipFreeAsync(p, stream_));  // after the work queued on it
+}
+
+void Arena::defer_frees(bool on) {
+    defer_ = on;
+    if (on) return;
+    for (void* p : deferred_) PSR_HIP_CHECK(hipFreeAsync(p, stream_));
+    deferred_.clear();
+}
 
 void* Arena::get(const std::string& name, size_t bytes) {
     Block& b = blocks_[name];
     if (b.bytes < bytes) {
         if (b.ptr) {
-            PSR_HIP_CHECK(hipFreeAsync(b.ptr, stream_));  // after the work queued on it
+            free_block(b.ptr);
             total_ -= b.bytes;
         }
         size_t want = bytes + bytes / 8 + 256;
@@ -50,7 +65,7 @@ void* Arena::get_keep(const std::string& name, size_t bytes, size_t keepBytes) {
         if (b.ptr) {
             if (keepBytes)
                 PSR_HIP_CHECK(hipMemcpyAsync(p, b.ptr, std::min(keepBytes, b.bytes), hipMemcpyDeviceToDevice, stream_));
-            PSR_HIP_CHECK(hipFreeAsync(b.ptr, stream_));
+            free_block(b.ptr);
             total_ -= b.bytes;
         }
         b.ptr = p;
@@ -62,6 +77,8 @@ void* Arena::get_keep(const std::string& name, size_t bytes, size_t keepBytes) {
 }
 
 void Arena::release() {
+    for (void* p : deferred_) (void)hipFreeAsync(p, stream_);
+    deferred_.clear();
     for (auto& kv : blocks_)
         if (kv.second.ptr) (void)hipFreeAsync(kv.second.ptr, stream_);
     if (!blocks_.empty()) (void)hipStreamSynchronize(stream_);
@@ -78,50 +95,36 @@ int fail(psr_ctx* ctx, int code, const std::string& msg) {
     return code;
 }
 
-float elapsed(hipEvent_t a, hipEvent_t b) {
-    float ms = 0.f;
-    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return -1.f;
-    return ms;
-}
 
-int run(psr_ctx* ctx, const psr_points* pts, psr_mesh* out) {
-    using namespace psr;
-    Ctx& c = ctx->c;
-    hipStream_t s = c.stream;
-    const int N = int(pts->count);
-    if (N < 8) return fail(ctx, PSR_ERR_INVALID, "need at least 8 points");
-    if (c.cfg.max_points && pts->count > c.cfg.max_points)
-        return fail(ctx, PSR_ERR_CAPACITY, "point count exceeds max_points");
-    if (!pts->xyz || !pts->normal) return fail(ctx, PSR_ERR_INVALID, "null point buffer");
-    if (pts->stride_bytes && (pts->stride_bytes < 12 || pts->stride_bytes % 4))
-        return fail(ctx, PSR_ERR_INVALID, "bad stride");
+}  // namespace
+
+namespace psr {
+
+int call_begin(Ctx& c, const psr_points& pts, std::string& err) {
+    const int N = int(pts.count);
+    if (N < 8) return err = "need at least 8 points", PSR_ERR_INVALID;
+    if (c.cfg.max_points && pts.count > c.cfg.max_points)
+        return err = "point count exceeds max_points", PSR_ERR_CAPACITY;
+    if (!pts.xyz || !pts.normal) return err = "null point buffer", PSR_ERR_INVALID;
+    if (pts.stride_bytes && (pts.stride_bytes < 12 || pts.stride_bytes % 4)) return err = "bad stride", PSR_ERR_INVALID;
     std::memset(&c.stats, 0, sizeof(c.stats));
     c.clear_scan_error();
     c.mesh.valid = false;
     c.mesh.weldedNv = c.mesh.weldedNt = 0;
-    PSR_HIP_CHECK(hipEventRecord(c.ev[0], s));
-    build_octree(c, *pts);
-    PSR_HIP_CHECK(hipEventRecord(c.ev[1], s));
-    compute_field(c);
-    PSR_HIP_CHECK(hipEventRecord(c.ev[2], s));
-    solve_depths(c);
-    points_implicit(c);
-    PSR_HIP_CHECK(hipEventRecord(c.ev[3], s));
-    build_geometry(c);
-    extract_mesh(c);
-    PSR_HIP_CHECK(hipEventRecord(c.ev[4], s));
-    c.wait();  // the results reached hostResults with the vertex values (build_geometry)
+    return PSR_OK;
+}
+
+int call_end(Ctx& c, psr_mesh* out, bool timed, std::string& err) {
     finish_mesh(c);
-    // statistics
     psr_stats& st = c.stats;
     const Octree& o = c.oct;
     st.depth = o.L.D;
-    st.n_points = N;
+    st.n_points = uint32_t(o.N);
     st.n_nodes = o.M;
     for (int d = 0; d <= o.L.D; ++d) st.nodes_per_depth[d] = o.L.count[d];
     const DevResults& r = *c.hostResults;
-    if (r.cgErr) return fail(ctx, PSR_ERR_INTERNAL, "CG group barrier timed out");
-    if (c.scan_failed()) return fail(ctx, PSR_ERR_INTERNAL, "scan look-back timed out");
+    if (r.cgErr) return err = "CG group barrier timed out", PSR_ERR_INTERNAL;
+    if (c.scan_failed()) return err = "scan look-back timed out", PSR_ERR_INTERNAL;
     for (int d = 0; d <= o.L.D; ++d) {
         st.cg_iterations[d] = r.cgIters[d];
         st.cg_outer[d] = r.cgIters[PSR_MAX_DEPTH + 1 + d];
@@ -134,13 +137,15 @@ int run(psr_ctx* ctx, const psr_points* pts, psr_mesh* out) {
     st.n_vertex_nodes = c.geo.V;
     st.n_edge_nodes = c.geo.E;
     st.n_face_nodes = c.geo.F;
-    st.ms_octree = elapsed(c.ev[0], c.ev[1]);
-    st.ms_field = elapsed(c.ev[1], c.ev[2]);
-    st.ms_solve = elapsed(c.ev[2], c.ev[3]);
-    st.ms_mesh = elapsed(c.ev[3], c.ev[4]);
-    st.ms_total = elapsed(c.ev[0], c.ev[4]);
-    st.ms_cg_kernel = elapsed(c.ev[8], c.ev[9]);
-    c.cgMsSum += st.ms_cg_kernel;
+    if (timed) {
+        st.ms_octree = elapsed(c.ev[0], c.ev[1]);
+        st.ms_field = elapsed(c.ev[1], c.ev[2]);
+        st.ms_solve = elapsed(c.ev[2], c.ev[3]);
+        st.ms_mesh = elapsed(c.ev[3], c.ev[4]);
+        st.ms_total = elapsed(c.ev[0], c.ev[4]);
+        st.ms_cg_kernel = elapsed(c.ev[8], c.ev[9]);
+        c.cgMsSum += st.ms_cg_kernel;
+    }
     st.ms_cg_kernel_sum = c.cgMsSum;
     st.n_runs = ++c.runs;
     out->vertices = c.mesh.vertices;
@@ -150,6 +155,31 @@ int run(psr_ctx* ctx, const psr_points* pts, psr_mesh* out) {
     c.mesh.valid = true;
     c.cgClean = true;  // the neighbour-column kernel cleared the CG's controls after the solve
     return PSR_OK;
+}
+
+}  // namespace psr
+
+namespace {
+
+int run(psr_ctx* ctx, const psr_points* pts, psr_mesh* out) {
+    using namespace psr;
+    Ctx& c = ctx->c;
+    hipStream_t s = c.stream;
+    int rc = call_begin(c, *pts, ctx->lastError);
+    if (rc != PSR_OK) return rc;
+    PSR_HIP_CHECK(hipEventRecord(c.ev[0], s));
+    build_octree(c, *pts);
+    PSR_HIP_CHECK(hipEventRecord(c.ev[1], s));
+    compute_field(c);
+    PSR_HIP_CHECK(hipEventRecord(c.ev[2], s));
+    solve_depths(c);
+    points_implicit(c);
+    PSR_HIP_CHECK(hipEventRecord(c.ev[3], s));
+    build_geometry(c);
+    extract_mesh(c);
+    PSR_HIP_CHECK(hipEventRecord(c.ev[4], s));
+    c.wait();  // the results reached hostResults with the vertex values (build_geometry)
+    return call_end(c, out, true, ctx->lastError);
 }
 
 // Host copy of a device int array, transformed column-wise into row-major [M][w].
@@ -178,6 +208,7 @@ void psr_default_config(psr_config* cfg) {
     cfg->refine_rel_tol = 1e-5f;
     cfg->refine_max_outer = 4;
     cfg->subdiv_chunk_nodes = 0;
+    cfg->batch_cg_plan = -1;
 }
 
 int psr_create(const psr_config* cfg, void* hip_stream, psr_ctx** out) {
@@ -319,76 +350,25 @@ int psr_vertex_colors(psr_ctx* ctx, const psr_mesh* mesh, const float* xyz, cons
 
 int psr_reconstruct_batch(psr_ctx* ctx, const psr_points* clouds, uint32_t count, psr_mesh* outs) {
     if (!ctx || (count && (!clouds || !outs))) return PSR_ERR_INVALID;
-    psr::Ctx& c = ctx->c;
-    const uint32_t W = std::min<uint32_t>(count, c.cfg.batch_workers ? c.cfg.batch_workers : 8);
     try {
-        while (ctx->workers.size() < W) {
-            psr_ctx* w = nullptr;
-            int rc = psr_create(&c.cfg, nullptr, &w);
-            if (rc != PSR_OK) return fail(ctx, rc, "batch worker creation failed");
-            ctx->workers.push_back(w);
-        }
+        if (!ctx->batch) ctx->batch = psr::batch_create(ctx);
+        std::string err;
+        const int rc = psr::batch_run(ctx->batch, clouds, count, outs, ctx->batchStats, err);
+        if (rc != PSR_OK) return fail(ctx, rc, err);
+    } catch (const psr::HipError& e) {
+        return fail(ctx, PSR_ERR_HIP, e.what());
     } catch (const std::exception& e) {
         return fail(ctx, PSR_ERR_INTERNAL, e.what());
     }
-    ctx->batchStats.assign(count, psr_stats{});
-    std::atomic<uint32_t> next{0};
-    std::atomic<int> status{PSR_OK};
-    std::string message;
-    std::mutex msgMu;
-    auto record = [&](int rc, uint32_t i, const std::string& m) {
-        int expected = PSR_OK;
-        if (status.compare_exchange_strong(expected, rc)) {
-            std::lock_guard<std::mutex> lk(msgMu);
-            message = "cloud " + std::to_string(i) + ": " + m;
-        }
-    };
-    auto work = [&](uint32_t wi) {
-        psr_ctx* wk = ctx->workers[wi];
-        if (hipSetDevice(c.device) != hipSuccess) {
-            record(PSR_ERR_HIP, 0, "hipSetDevice");
-            return;
-        }
-        // every cloud is attempted; the first failure is the call's status
-        for (;;) {
-            const uint32_t i = next.fetch_add(1);
-            if (i >= count) return;
-            psr_mesh m{};
-            int rc = psr_reconstruct(wk, &clouds[i], &m);
-            if (rc != PSR_OK) {
-                record(rc, i, wk->lastError);
-                continue;
-            }
-            try {
-                float* v;
-                int* t;
-                {
-                    std::lock_guard<std::mutex> lk(ctx->batchMu);
-                    v = c.buf<float>(("batch_v_" + std::to_string(i)).c_str(), size_t(3) * m.n_vertices);
-                    t = c.buf<int>(("batch_t_" + std::to_string(i)).c_str(), size_t(3) * m.n_triangles);
-                    // allocated in the batch context's stream order; used on the worker's stream
-                    PSR_HIP_CHECK(hipStreamSynchronize(c.stream));
-                }
-                hipStream_t ws = wk->c.stream;
-                PSR_HIP_CHECK(hipMemcpyAsync(v, m.vertices, sizeof(float) * 3 * m.n_vertices,
-                                             hipMemcpyDeviceToDevice, ws));
-                PSR_HIP_CHECK(hipMemcpyAsync(t, m.triangles, sizeof(int) * 3 * m.n_triangles,
-                                             hipMemcpyDeviceToDevice, ws));
-                PSR_HIP_CHECK(hipStreamSynchronize(ws));
-                outs[i] = psr_mesh{v, t, m.n_vertices, m.n_triangles};
-                ctx->batchStats[i] = wk->c.stats;
-            } catch (const std::exception& e) {
-                record(PSR_ERR_HIP, i, e.what());
-            }
-        }
-    };
-    std::vector<std::thread> threads;
-    for (uint32_t w = 1; w < W; ++w) threads.emplace_back(work, w);
-    if (W > 0) work(0);
-    for (auto& th : threads) th.join();
-    if (status.load() != PSR_OK) return fail(ctx, status.load(), message);
     return PSR_OK;
 }
+
+psr_ctx* psr_batch_cloud(psr_ctx* ctx, uint32_t index) {
+    if (!ctx || index >= ctx->batchStats.size()) return nullptr;
+    return psr::batch_cloud(ctx->batch, index);
+}
+
+int psr_batch_cg_plan(const psr_ctx* ctx) { return ctx ? psr::batch_last_plan(ctx->batch) : 0; }
 
 int psr_get_batch_stats(psr_ctx* ctx, uint32_t index, psr_stats* out) {
     if (!ctx || !out || index >= ctx->batchStats.size()) return PSR_ERR_INVALID;
@@ -564,8 +544,8 @@ int psr_export(psr_ctx* ctx, int id, void* dst, uint64_t bytes, uint64_t* needed
 
 void psr_destroy(psr_ctx* ctx) {
     if (!ctx) return;
-    for (psr_ctx* w : ctx->workers) psr_destroy(w);
-    ctx->workers.clear();
+    if (ctx->batch) psr::batch_destroy(ctx->batch);
+    ctx->batch = nullptr;
     psr::Ctx& c = ctx->c;
     if (c.stream) (void)hipStreamSynchronize(c.stream);
     if (c.ev[9]) psr::cg_forget_event(c.ev[9]);

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "psr_launch.h"
+
 #include <cstdio>
 #include <stdexcept>
 #include <string>
@@ -25,6 +27,48 @@ struct HipError : std::runtime_error {
 };
 
 inline unsigned div_up(size_t a, unsigned b) { return unsigned((a + b - 1) / b); }
+
+// --- wave reductions through DPP ---------------------------------------------
+// Four DPP butterfly steps (swap neighbours, swap pairs, mirror within 8,
+// mirror within 16) leave every lane of a row of 16 with its row's sum; the
+// four row sums (lanes 0, 16, 32, 48) are then read into scalar registers and
+// added in a fixed order, so every lane gets the same bits.  DPP moves run in
+// the vector ALU; a shuffle (ds_bpermute) takes a round trip through the LDS
+// unit, twice for a double.  The whole wave must be active.
+template <int CTRL>
+__device__ __forceinline__ float dpp_f32(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, int(b), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, int(b >> 32), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    return __hiloint2double(__builtin_amdgcn_readlane(int(b >> 32), lane), __builtin_amdgcn_readlane(int(b), lane));
+}
+__device__ __forceinline__ float readlane_f32(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+constexpr int kDppSwap1 = 0xB1, kDppSwap2 = 0x4E, kDppHalfMirror = 0x141, kDppMirror = 0x140;
+// Sum over the wave, the same in every lane.
+__device__ __forceinline__ double wave_sum_f64(double v) {
+    v += dpp_f64<kDppSwap1>(v);
+    v += dpp_f64<kDppSwap2>(v);
+    v += dpp_f64<kDppHalfMirror>(v);
+    v += dpp_f64<kDppMirror>(v);
+    return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
+}
+__device__ __forceinline__ float wave_sum_f32(float v) {
+    v += dpp_f32<kDppSwap1>(v);
+    v += dpp_f32<kDppSwap2>(v);
+    v += dpp_f32<kDppHalfMirror>(v);
+    v += dpp_f32<kDppMirror>(v);
+    return (readlane_f32(v, 0) + readlane_f32(v, 16)) + (readlane_f32(v, 32) + readlane_f32(v, 48));
+}
 
 // Per-depth layout of the node array: level d occupies [base[d], base[d] + count[d]).
 struct Levels {
@@ -195,7 +239,7 @@ __device__ __forceinline__ void points_chi_block(const PointsChiArgs& a, int blo
         a.chi[i] = v;
         mine = double(v);
     }
-    for (int off = PSR_WAVE / 2; off > 0; off >>= 1) mine += __shfl_xor(mine, off, PSR_WAVE);
+    mine = wave_sum_f64(mine);
     if (__lane_id() == 0) sh[threadIdx.x / PSR_WAVE] = mine;
     __syncthreads();
     if (threadIdx.x == 0) {

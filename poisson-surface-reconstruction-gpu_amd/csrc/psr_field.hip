@@ -133,13 +133,13 @@ __device__ __forceinline__ float load_field(const __half* p, float scale) { retu
 
 constexpr int kVfWaves = 4;
 template <class FieldT>
-__global__ void __launch_bounds__(kVfWaves * PSR_WAVE) vector_field_kernel(
+__device__ __forceinline__ void vector_field_kernel_body(int psr_bid, int psr_nbid, 
     int nD, int offD, int M, int D, const int* __restrict__ neigh, const int* __restrict__ pidx,
     const int* __restrict__ pnum, const int* __restrict__ key, const float4* __restrict__ pos,
     const float4* __restrict__ nrm, FieldT* __restrict__ vf) {
     __shared__ NbRanges R[kVfWaves];
     int wid = threadIdx.x / PSR_WAVE, lane = __lane_id();
-    int node = blockIdx.x * kVfWaves + wid;
+    int node = psr_bid * kVfWaves + wid;
     if (node >= nD) return;
     int o = offD + node;
     NbRanges& r = R[wid];
@@ -164,11 +164,9 @@ __global__ void __launch_bounds__(kVfWaves * PSR_WAVE) vector_field_kernel(
         ay += w * n.y;
         az += w * n.z;
     }
-    for (int off = PSR_WAVE / 2; off > 0; off >>= 1) {
-        ax += __shfl_xor(ax, off, PSR_WAVE);
-        ay += __shfl_xor(ay, off, PSR_WAVE);
-        az += __shfl_xor(az, off, PSR_WAVE);
-    }
+    ax = wave_sum_f32(ax);
+    ay = wave_sum_f32(ay);
+    az = wave_sum_f32(az);
     if (lane == 0) {
         const float inv = 1.0f / float(2 << D);
         store_field(&vf[node], ax, inv);
@@ -176,6 +174,20 @@ __global__ void __launch_bounds__(kVfWaves * PSR_WAVE) vector_field_kernel(
         store_field(&vf[2 * nD + node], az, inv);
     }
 }
+template <class FieldT>
+__global__ void __launch_bounds__(kVfWaves * PSR_WAVE) vector_field_kernel(
+    int nD, int offD, int M, int D, const int* __restrict__ neigh, const int* __restrict__ pidx,
+    const int* __restrict__ pnum, const int* __restrict__ key, const float4* __restrict__ pos,
+    const float4* __restrict__ nrm, FieldT* __restrict__ vf) {
+    vector_field_kernel_body<FieldT>(blockIdx.x, gridDim.x, nD, offD, M, D, neigh, pidx, pnum, key, pos, nrm, vf);
+}
+template <class FieldT>
+struct vector_field_kernel_k {
+    static constexpr int kBlock = kVfWaves * PSR_WAVE;
+    template <class... A>
+    __device__ __forceinline__ static void run(int bid, int nbid, const A&... a) { vector_field_kernel_body<FieldT>(bid, nbid, a...); }
+};
+
 
 // --- divergence (ComputeNodesDivergence.cu:25-136) -------------------------
 // b(o) = sum over the 27 neighbours' depth-D descendants j of
@@ -194,13 +206,13 @@ __device__ __forceinline__ double div_term(int j, int offD, int nD, int res, int
 
 constexpr int kDivWaves = 4;
 template <class FieldT>
-__global__ void __launch_bounds__(kDivWaves * PSR_WAVE) divergence_fine(
+__device__ __forceinline__ void divergence_fine_body(int psr_bid, int psr_nbid, 
     int begin, int count, int M, int offD, int nD, int res, int D, const int* __restrict__ neigh,
     const int* __restrict__ didx, const int* __restrict__ dnum, const int* __restrict__ enc,
     const FieldT* __restrict__ vf, const double* __restrict__ ddot, float* __restrict__ div) {
     __shared__ NbRanges R[kDivWaves];
     int wid = threadIdx.x / PSR_WAVE, lane = __lane_id();
-    int node = blockIdx.x * kDivWaves + wid;
+    int node = psr_bid * kDivWaves + wid;
     if (node >= count) return;
     int o = begin + node;
     NbRanges& r = R[wid];
@@ -215,9 +227,23 @@ __global__ void __launch_bounds__(kDivWaves * PSR_WAVE) divergence_fine(
         int i = range_of(r, t);
         acc += div_term(r.first[i] + (t - r.start[i]), offD, nD, res, D, fo, enc, vf, ddot);
     }
-    for (int off = PSR_WAVE / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, PSR_WAVE);
+    acc = wave_sum_f64(acc);
     if (lane == 0) div[o] = float(acc);
 }
+template <class FieldT>
+__global__ void __launch_bounds__(kDivWaves * PSR_WAVE) divergence_fine(
+    int begin, int count, int M, int offD, int nD, int res, int D, const int* __restrict__ neigh,
+    const int* __restrict__ didx, const int* __restrict__ dnum, const int* __restrict__ enc,
+    const FieldT* __restrict__ vf, const double* __restrict__ ddot, float* __restrict__ div) {
+    divergence_fine_body<FieldT>(blockIdx.x, gridDim.x, begin, count, M, offD, nD, res, D, neigh, didx, dnum, enc, vf, ddot, div);
+}
+template <class FieldT>
+struct divergence_fine_k {
+    static constexpr int kBlock = kDivWaves * PSR_WAVE;
+    template <class... A>
+    __device__ __forceinline__ static void run(int bid, int nbid, const A&... a) { divergence_fine_body<FieldT>(bid, nbid, a...); }
+};
+
 
 // Coarse nodes (few, each covering a large share of depth D): `parts`
 // workgroups each, every one summing a strided share of the node's terms; the
@@ -231,18 +257,18 @@ struct CoarsePlan {
     int base[kMaxCoarse + 1];        // node range of each depth
 };
 template <class FieldT>
-__global__ void __launch_bounds__(kDivCoarseThreads) divergence_coarse(
-    CoarsePlan P, int M, int offD, int nD, int res, int D, const int* __restrict__ neigh,
+__device__ __forceinline__ void divergence_coarse_body(int psr_bid, int psr_nbid, 
+    const CoarsePlan& P, int M, int offD, int nD, int res, int D, const int* __restrict__ neigh,
     const int* __restrict__ didx, const int* __restrict__ dnum, const int* __restrict__ enc,
     const FieldT* __restrict__ vf, const double* __restrict__ ddot, double* __restrict__ partial,
     int* __restrict__ nnz) {
     // the operator's per-depth entry counts, added to beside the last divergence
     // kernel, and the CG error flag that follows them in the results block
-    if (blockIdx.x == 0 && threadIdx.x <= PSR_MAX_DEPTH + 1) nnz[threadIdx.x] = 0;
+    if (psr_bid == 0 && threadIdx.x <= PSR_MAX_DEPTH + 1) nnz[threadIdx.x] = 0;
     int d = 0;
-    while (d + 1 < P.depths && int(blockIdx.x) >= P.blockStart[d + 1]) ++d;
+    while (d + 1 < P.depths && int(psr_bid) >= P.blockStart[d + 1]) ++d;
     const int parts = P.parts[d];
-    const int rel = blockIdx.x - P.blockStart[d];
+    const int rel = psr_bid - P.blockStart[d];
     const int node = rel / parts, part = rel - node * parts;
     const int o = P.base[d] + node;
     __shared__ int start[28], first[27];
@@ -266,34 +292,49 @@ __global__ void __launch_bounds__(kDivCoarseThreads) divergence_coarse(
         while (start[i + 1] <= t) ++i;
         acc += div_term(first[i] + (t - start[i]), offD, nD, res, D, fo, enc, vf, ddot);
     }
-    for (int off = PSR_WAVE / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, PSR_WAVE);
+    acc = wave_sum_f64(acc);
     if (__lane_id() == 0) red[threadIdx.x / PSR_WAVE] = acc;
     __syncthreads();
     if (threadIdx.x == 0) {
         double s = 0;
         for (int w = 0; w < kDivCoarseThreads / PSR_WAVE; ++w) s += red[w];
-        partial[blockIdx.x] = s;
+        partial[psr_bid] = s;
     }
 }
+template <class FieldT>
+__global__ void __launch_bounds__(kDivCoarseThreads) divergence_coarse(
+    CoarsePlan P, int M, int offD, int nD, int res, int D, const int* __restrict__ neigh,
+    const int* __restrict__ didx, const int* __restrict__ dnum, const int* __restrict__ enc,
+    const FieldT* __restrict__ vf, const double* __restrict__ ddot, double* __restrict__ partial,
+    int* __restrict__ nnz) {
+    divergence_coarse_body<FieldT>(blockIdx.x, gridDim.x, P, M, offD, nD, res, D, neigh, didx, dnum, enc, vf, ddot, partial, nnz);
+}
+template <class FieldT>
+struct divergence_coarse_k {
+    static constexpr int kBlock = kDivCoarseThreads;
+    template <class... A>
+    __device__ __forceinline__ static void run(int bid, int nbid, const A&... a) { divergence_coarse_body<FieldT>(bid, nbid, a...); }
+};
+
 
 // Depth-D nodes: their neighbours are depth-D nodes, each its own only
 // descendant, so a thread sums the 27 terms of its node.  The first threads
 // also fold the coarse nodes' shares (thread t: coarse node t), in order.
-template <class FieldT>
 // Blocks from leafBlocks on assemble the operator instead (assemble_rows):
 // independent work that fills the GPU beside this kernel's few workgroups.
-__global__ void __launch_bounds__(256) divergence_leaf(int offD, int nD, int M, int res, int D,
+template <class FieldT>
+__device__ __forceinline__ void divergence_leaf_body(int psr_bid, int psr_nbid, int offD, int nD, int M, int res, int D,
                                                        const int* __restrict__ neigh, const int* __restrict__ enc,
                                                        const FieldT* __restrict__ vf,
-                                                       const double* __restrict__ ddot, CoarsePlan P,
+                                                       const double* __restrict__ ddot, const CoarsePlan& P,
                                                        const double* __restrict__ partial, float* __restrict__ div,
-                                                       int leafBlocks, AssembleArgs aa) {
+                                                       int leafBlocks, const AssembleArgs& aa) {
     static_assert(kAssembleThreads == 256, "one block size for both roles");
-    if (int(blockIdx.x) >= leafBlocks) {
-        assemble_rows(aa, int(blockIdx.x) - leafBlocks);
+    if (int(psr_bid) >= leafBlocks) {
+        assemble_rows(aa, int(psr_bid) - leafBlocks);
         return;
     }
-    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int t = psr_bid * 256 + threadIdx.x;
     if (t < P.base[P.depths]) {
         int d = 0;
         while (t >= P.base[d + 1]) ++d;
@@ -315,6 +356,22 @@ __global__ void __launch_bounds__(256) divergence_leaf(int offD, int nD, int M, 
         if (nb[i] != -1) acc += div_term(nb[i] - offD, offD, nD, res, D, fo, enc, vf, ddot);
     div[o] = float(acc);
 }
+template <class FieldT>
+__global__ void __launch_bounds__(256) divergence_leaf(int offD, int nD, int M, int res, int D,
+                                                       const int* __restrict__ neigh, const int* __restrict__ enc,
+                                                       const FieldT* __restrict__ vf,
+                                                       const double* __restrict__ ddot, CoarsePlan P,
+                                                       const double* __restrict__ partial, float* __restrict__ div,
+                                                       int leafBlocks, AssembleArgs aa) {
+    divergence_leaf_body<FieldT>(blockIdx.x, gridDim.x, offD, nD, M, res, D, neigh, enc, vf, ddot, P, partial, div, leafBlocks, aa);
+}
+template <class FieldT>
+struct divergence_leaf_k {
+    static constexpr int kBlock = 256;
+    template <class... A>
+    __device__ __forceinline__ static void run(int bid, int nbid, const A&... a) { divergence_leaf_body<FieldT>(bid, nbid, a...); }
+};
+
 
 }  // namespace
 
@@ -339,7 +396,7 @@ void field_stages(Ctx& c, FieldT* vf) {
     const Levels& L = o.L;
     const int D = L.D, nD = L.count[D], offD = L.base[D], M = o.M;
     hipStream_t s = c.stream;
-    vector_field_kernel<FieldT><<<div_up(nD, kVfWaves), kVfWaves * PSR_WAVE, 0, s>>>(
+    PSR_LAUNCH_K(vector_field_kernel<FieldT>, vector_field_kernel_k<FieldT>, div_up(nD, kVfWaves), 0, s,
         nD, offD, M, D, o.neigh, o.pidx, o.pnum, o.key, o.spos, o.snrm, vf);
     // depths 0..kCoarse: several workgroups per node, per depth as many as the
     // node's expected share of depth D asks for; depth D: a thread per node;
@@ -361,13 +418,13 @@ void field_stages(Ctx& c, FieldT* vf) {
         P.blockStart[kCoarse + 1] = blocks;
         P.base[kCoarse + 1] = L.base[kCoarse + 1];
         partial = c.buf<double>("div_partial", blocks);
-        divergence_coarse<FieldT><<<blocks, kDivCoarseThreads, 0, s>>>(P, M, offD, nD, c.tables.res, D, o.neigh, o.didx,
+        PSR_LAUNCH_K(divergence_coarse<FieldT>, divergence_coarse_k<FieldT>, blocks, 0, s, P, M, offD, nD, c.tables.res, D, o.neigh, o.didx,
                                                                       o.dnum, o.encIdx, vf, c.tables.ddot, partial,
                                                                       c.results()->cgNnz);
     }
     const int midCount = offD - coarseCount;
     if (midCount > 0)
-        divergence_fine<FieldT><<<div_up(midCount, kDivWaves), kDivWaves * PSR_WAVE, 0, s>>>(
+        PSR_LAUNCH_K(divergence_fine<FieldT>, divergence_fine_k<FieldT>, div_up(midCount, kDivWaves), 0, s,
             coarseCount, midCount, M, offD, nD, c.tables.res, D, o.neigh, o.didx, o.dnum, o.encIdx, vf, c.tables.ddot,
             c.fld.div);
     // the operator (it needs only the octree and the tables) is assembled in
@@ -375,7 +432,7 @@ void field_stages(Ctx& c, FieldT* vf) {
     const AssembleArgs aa{L,     M,     o.neigh, o.encIdx, c.tables.dot, c.tables.d2dot, c.tables.res,
                           c.buf<float>("lap_val", size_t(27) * M), c.results()->cgNnz};
     const int leafBlocks = int(div_up(std::max(nD, coarseCount), 256));
-    divergence_leaf<FieldT><<<leafBlocks + int(div_up(size_t(3) * M, kAssembleThreads)), 256, 0, s>>>(
+    PSR_LAUNCH_K(divergence_leaf<FieldT>, divergence_leaf_k<FieldT>, leafBlocks + int(div_up(size_t(3) * M, kAssembleThreads)), 0, s,
         offD, nD, M, c.tables.res, D, o.neigh, o.encIdx, vf, c.tables.ddot, P, partial, c.fld.div, leafBlocks, aa);
     PSR_HIP_CHECK(hipGetLastError());
 }

@@ -30,13 +30,20 @@ class Arena {
     void release();
     size_t bytes_reserved() const { return total_; }
     int allocations() const { return allocs_; }
+    // While on, a buffer replaced by a larger one is freed only when deferral
+    // ends (a batch records launches that still name it); turning it off
+    // queues those frees on the stream.
+    void defer_frees(bool on);
 
   private:
+    void free_block(void* p);
     struct Block {
         void* ptr = nullptr;
         size_t bytes = 0;
     };
     std::map<std::string, Block> blocks_;
+    bool defer_ = false;
+    std::vector<void*> deferred_;
     hipStream_t stream_ = nullptr;
     size_t total_ = 0;
     int allocs_ = 0;
@@ -132,6 +139,18 @@ struct Mesh {
     size_t vCap = 0, tCap = 0;
 };
 
+// The depth-D marching cubes' outputs and the counts read after its wait.
+struct MainMc {
+    int* vexAddr = nullptr;   // crossing index per depth-D edge
+    int* cube = nullptr;      // cube category per depth-D node (exported)
+    int* hasInter = nullptr;  // surface-edge intersection flag per face
+    int* roots = nullptr;     // subdivision roots, ascending node index
+    int* rootPos = nullptr;   // per node above depth D: position in roots, or -1
+    int nCross = 0, nTri = 0, nRoots = 0;
+    int rootsPerDepth[PSR_MAX_DEPTH + 1] = {0};
+    bool q1Fired = false;  // the level-local child 0 rule changed a parent somewhere
+};
+
 // Small per-call results written by the kernels and read back in one copy.
 struct DevResults {
     int cgIters[2 * (PSR_MAX_DEPTH + 1)];  // inner iterations | outer steps, per depth
@@ -169,6 +188,7 @@ struct Ctx {
     Field fld;
     Geometry geo;
     Mesh mesh;
+    MainMc mc;  // the depth-D marching cubes of the call in progress
     psr_stats stats{};
     int* scratch = nullptr;
     size_t scratchInts = 0;
@@ -190,6 +210,7 @@ struct Ctx {
     int cgMaxWgs = 0;            // resident workgroups of the CG kernels with several rows per thread
     int cgMaxWgsRow = 0;         // resident workgroups of the one-row-per-thread CG kernels
     int cgRowsPerWg = 0;         // rows per CG workgroup (0: default)
+    int cgRowTarget = 0;         // batch: rows a CG workgroup takes (0: the single-cloud plan)
     template <class T>
     T* buf(const char* name, size_t count) {
         return static_cast<T*>(arena.get(name, sizeof(T) * (count ? count : 1)));
@@ -251,19 +272,44 @@ struct Ctx {
     }
 };
 
+// Thrown by a stage driver that cannot be recorded for a batch (it waits
+// for the host in the middle); the batch runs that cloud's stage on its own.
+struct NotRecordable {};
+
 // Stage drivers.
 void build_tables(Ctx& c);
 void build_octree(Ctx& c, const psr_points& pts);
+// build_octree in two parts: up to the level counts, and after their wait
+void build_octree_a(Ctx& c, const psr_points& pts);
+void build_octree_b(Ctx& c);
 void compute_field(Ctx& c);
 void solve_depths(Ctx& c);
 // Drops a context's CG completion event from the batch-worker set (before the
 // event is destroyed).
 void cg_forget_event(hipEvent_t e);
+// A batch's merged CG launches take the whole device: cg_admit_device waits
+// until no CG launch of another context is in flight; cg_hold_device then
+// holds the device until `done` (recorded after the last of them) fires.
+void cg_admit_device(hipEvent_t self);
+// Resident workgroups of the batched several-rows CG kernel of c's precision.
+int cg_batch_budget(Ctx& c);
+void cg_hold_device(hipEvent_t done);
 void points_implicit(Ctx& c);
 void build_geometry(Ctx& c);
 void extract_mesh(Ctx& c);
+// extract_mesh in two parts: the depth-D marching cubes, and after their wait
+// the subdivision and the output
+void extract_mesh_a(Ctx& c);
+void extract_mesh_b(Ctx& c);
 // Mesh counts of a deferred extraction, read from the mailbox after the final wait.
 void finish_mesh(Ctx& c);
+// Checks a cloud and resets a context's per-call state (PSR_OK, or a status
+// with the message in err).
+int call_begin(Ctx& c, const psr_points& pts, std::string& err);
+// After a call's last wait: statistics and the mesh (timed: the stage events
+// were recorded on the context's stream).
+int call_end(Ctx& c, psr_mesh* out, bool timed, std::string& err);
+float elapsed(hipEvent_t a, hipEvent_t b);
 // Welds the last mesh into mesh.welded*; returns once the counts are known.
 void weld_mesh(Ctx& c);
 // Per-vertex normals of a device mesh into out (3 floats per vertex); returns once written.
@@ -277,3 +323,28 @@ void vertex_colors(Ctx& c, const float* V, int nv, const float* xyz, unsigned st
                    float* out);
 
 }  // namespace psr
+
+struct psr_ctx;
+namespace psr {
+class Batch;
+Batch* batch_create(psr_ctx* owner);
+void batch_destroy(Batch* b);
+// Reconstructs count clouds with one launch per stage (psr_batch.cpp); outs[i]
+// and stats[i] are written for every cloud that succeeds, the first failure is
+// the status (its message in err).
+int batch_run(Batch* b, const psr_points* pts, uint32_t count, psr_mesh* outs, std::vector<psr_stats>& stats,
+              std::string& err);
+// The CG plan of the batch's last call (psr_config.batch_cg_plan).
+int batch_last_plan(const Batch* b);
+// The context of cloud i of the last batch call (null if none).
+psr_ctx* batch_cloud(const Batch* b, uint32_t i);
+}
+
+// The C ABI's context: one reconstruction context, and the batched form's
+// state (its clouds' contexts share this context's stream).
+struct psr_ctx {
+    psr::Ctx c;
+    std::string lastError;
+    psr::Batch* batch = nullptr;
+    std::vector<psr_stats> batchStats;
+};

@@ -103,7 +103,7 @@ __device__ __forceinline__ void regular_owned(const int (&kk)[27], bool depthD, 
 
 // The iso value: the mean implicit value over the points (LaplacianSolver.cu:269-305),
 // from the per-block sums of points_chi_block, in one fixed order.
-__global__ void __launch_bounds__(1024) iso_final(const double* __restrict__ part, int nb, int N,
+__device__ __forceinline__ void iso_final_body(int psr_bid, int psr_nbid, const double* __restrict__ part, int nb, int N,
                                                    double* __restrict__ isoSum, float* __restrict__ iso) {
     __shared__ double sh[1024 / PSR_WAVE];
     double s = 0;
@@ -118,20 +118,26 @@ __global__ void __launch_bounds__(1024) iso_final(const double* __restrict__ par
         *iso = float(t / double(N));
     }
 }
+__global__ void __launch_bounds__(1024) iso_final(const double* __restrict__ part, int nb, int N,
+                                                   double* __restrict__ isoSum, float* __restrict__ iso) {
+    iso_final_body(blockIdx.x, gridDim.x, part, nb, N, isoSum, iso);
+}
+PSR_BATCHABLE(iso_final, 1024)
+
 
 // dense: the dense d_x grids (or null), where a regular octree's nodes put dx.
 // Blocks from geoBlocks on compute the points' implicit values instead
 // (points_chi_block): independent work that fills the GPU beside this
 // kernel's few workgroups.
-__global__ void __launch_bounds__(kThreads) geometry_emit(NodeView v, int offD, int nD,
+__device__ __forceinline__ void geometry_emit_body(int psr_bid, int psr_nbid, const NodeView& v, int offD, int nD,
                                                           const int* __restrict__ parent,
-                                                          const int* __restrict__ q1Fired, ScanTicket tv,
-                                                          ScanTicket te, ScanTicket tf, GeoOut out,
+                                                          const int* __restrict__ q1Fired, const ScanTicket& tv,
+                                                          const ScanTicket& te, const ScanTicket& tf, const GeoOut& out,
                                                           const float* __restrict__ dx, float* __restrict__ dense,
-                                                          int geoBlocks, PointsChiArgs pc) {
+                                                          int geoBlocks, const PointsChiArgs& pc) {
     static_assert(kThreads == kPointsChiThreads, "one block size for both roles");
-    if (int(blockIdx.x) >= geoBlocks) {
-        points_chi_block(pc, int(blockIdx.x) - geoBlocks);
+    if (int(psr_bid) >= geoBlocks) {
+        points_chi_block(pc, int(psr_bid) - geoBlocks);
         return;
     }
     const int tile = ticket_tile(tv);
@@ -334,9 +340,19 @@ __global__ void __launch_bounds__(kThreads) geometry_emit(NodeView v, int offD, 
         }
     }
 }
+__global__ void __launch_bounds__(kThreads) geometry_emit(NodeView v, int offD, int nD,
+                                                          const int* __restrict__ parent,
+                                                          const int* __restrict__ q1Fired, ScanTicket tv,
+                                                          ScanTicket te, ScanTicket tf, GeoOut out,
+                                                          const float* __restrict__ dx, float* __restrict__ dense,
+                                                          int geoBlocks, PointsChiArgs pc) {
+    geometry_emit_body(blockIdx.x, gridDim.x, v, offD, nD, parent, q1Fired, tv, te, tf, out, dx, dense, geoBlocks, pc);
+}
+PSR_BATCHABLE(geometry_emit, kThreads)
+
 
 // --- vertex implicit values (ComputeTriangleIndices.cu:346-399), quirk Q8 --
-__global__ void __launch_bounds__(kValueThreads) vertex_values(NodeView v, const int* __restrict__ count,
+__device__ __forceinline__ void vertex_values_body(int psr_bid, int psr_nbid, const NodeView& v, const int* __restrict__ count,
                                                           const int* __restrict__ vOwner,
                                                           const int* __restrict__ vKind,
                                                           const float4* __restrict__ vPos,
@@ -347,7 +363,7 @@ __global__ void __launch_bounds__(kValueThreads) vertex_values(NodeView v, const
                                                           const float* __restrict__ iso, float* __restrict__ vv,
                                                           int* __restrict__ mail, const int* __restrict__ results,
                                                           int* __restrict__ resultsHost, int resultWords) {
-    int t = blockIdx.x * kValueThreads + threadIdx.x;
+    int t = psr_bid * kValueThreads + threadIdx.x;
     if (t < 3) mail[kMailGeo + t] = count[t];  // vertex, edge and face counts for the host
     // the run's device results (CG statistics, iso value) are final here
     if (t < resultWords) resultsHost[t] = results[t];
@@ -379,16 +395,31 @@ __global__ void __launch_bounds__(kValueThreads) vertex_values(NodeView v, const
     }
     vv[t] = val - *iso;
 }
+__global__ void __launch_bounds__(kValueThreads) vertex_values(NodeView v, const int* __restrict__ count,
+                                                          const int* __restrict__ vOwner,
+                                                          const int* __restrict__ vKind,
+                                                          const float4* __restrict__ vPos,
+                                                          const int* __restrict__ parent,
+                                                          const int* __restrict__ childBase,
+                                                          const float4* __restrict__ nbrCol,
+                                                          const int* __restrict__ q1Fired,
+                                                          const float* __restrict__ iso, float* __restrict__ vv,
+                                                          int* __restrict__ mail, const int* __restrict__ results,
+                                                          int* __restrict__ resultsHost, int resultWords) {
+    vertex_values_body(blockIdx.x, gridDim.x, v, count, vOwner, vKind, vPos, parent, childBase, nbrCol, q1Fired, iso, vv, mail, results, resultsHost, resultWords);
+}
+PSR_BATCHABLE(vertex_values, kValueThreads)
+
 
 // --- depth-D marching cubes (ComputeTriangleIndices.cu:401-526) ------------
 // The owned edges of each depth-D node that cross the surface (their corner
 // values differ in sign), compacted in edge order -- (node, edge kind), the
 // order of the edge records -- with their crossing points: per edge record a
 // crossing flag and its crossing index.
-__global__ void __launch_bounds__(kThreads) edge_crossings(int offD, int nD, int M, const int2* __restrict__ edgeOf,
+__device__ __forceinline__ void edge_crossings_body(int psr_bid, int psr_nbid, int offD, int nD, int M, const int2* __restrict__ edgeOf,
                                                            const int* __restrict__ nodeVert,
                                                            const float* __restrict__ vv,
-                                                           const float4* __restrict__ vPos, ScanTicket t,
+                                                           const float4* __restrict__ vPos, const ScanTicket& t,
                                                            int* __restrict__ flag, int* __restrict__ addr,
                                                            float* __restrict__ out, int D, unsigned* __restrict__ key) {
     const int tile = ticket_tile(t);
@@ -425,6 +456,16 @@ __global__ void __launch_bounds__(kThreads) edge_crossings(int offD, int nD, int
         ++c;
     }
 }
+__global__ void __launch_bounds__(kThreads) edge_crossings(int offD, int nD, int M, const int2* __restrict__ edgeOf,
+                                                           const int* __restrict__ nodeVert,
+                                                           const float* __restrict__ vv,
+                                                           const float4* __restrict__ vPos, ScanTicket t,
+                                                           int* __restrict__ flag, int* __restrict__ addr,
+                                                           float* __restrict__ out, int D, unsigned* __restrict__ key) {
+    edge_crossings_body(blockIdx.x, gridDim.x, offD, nD, M, edgeOf, nodeVert, vv, vPos, t, flag, addr, out, D, key);
+}
+PSR_BATCHABLE(edge_crossings, kThreads)
+
 
 // A cube emits triangles only when all its corners and edges are present and
 // every edge its triangles use crosses; otherwise (only where the reference's
@@ -432,11 +473,11 @@ __global__ void __launch_bounds__(kThreads) edge_crossings(int offD, int nD, int
 // reads out of bounds) it emits none.  The triangles of each node are written
 // in node order, compacted in the same launch, and the faces bounding an
 // emitted triangle's edges are marked up the parent chain.
-__global__ void __launch_bounds__(kThreads) cube_triangles(
+__device__ __forceinline__ void cube_triangles_body(int psr_bid, int psr_nbid, 
     int offD, int nD, int M, const int* __restrict__ nodeVert, const int* __restrict__ nodeEdge,
     const int* __restrict__ crossFlag, const float* __restrict__ vv, const int* __restrict__ vexAddr,
     const int* __restrict__ nodeFace, const int* __restrict__ parent, const int* __restrict__ faceHasParent,
-    const int* __restrict__ q1Fired, ScanTicket t, int* __restrict__ cube, int* __restrict__ tri,
+    const int* __restrict__ q1Fired, const ScanTicket& t, int* __restrict__ cube, int* __restrict__ tri,
     int* __restrict__ hasInter, int* __restrict__ mail) {
     const int tile = ticket_tile(t);
     const int n = tile * kThreads + threadIdx.x, o = offD + n;
@@ -500,6 +541,16 @@ __global__ void __launch_bounds__(kThreads) cube_triangles(
         }
     }
 }
+__global__ void __launch_bounds__(kThreads) cube_triangles(
+    int offD, int nD, int M, const int* __restrict__ nodeVert, const int* __restrict__ nodeEdge,
+    const int* __restrict__ crossFlag, const float* __restrict__ vv, const int* __restrict__ vexAddr,
+    const int* __restrict__ nodeFace, const int* __restrict__ parent, const int* __restrict__ faceHasParent,
+    const int* __restrict__ q1Fired, ScanTicket t, int* __restrict__ cube, int* __restrict__ tri,
+    int* __restrict__ hasInter, int* __restrict__ mail) {
+    cube_triangles_body(blockIdx.x, gridDim.x, offD, nD, M, nodeVert, nodeEdge, crossFlag, vv, vexAddr, nodeFace, parent, faceHasParent, q1Fired, t, cube, tri, hasInter, mail);
+}
+PSR_BATCHABLE(cube_triangles, kThreads)
+
 
 // --- subdivision roots (ProcessLeafNodesAtOtherDepth :555-587, copy_if) -----
 // Roots in ascending node order, compacted in the same launch.  The first node
@@ -507,12 +558,12 @@ __global__ void __launch_bounds__(kThreads) cube_triangles(
 // of a depth are a difference of two starts: precomputeSubdivideDepth reads a
 // root's depth through neighs[13], and a root is its own neighbour here, so
 // that is the root's level.
-__global__ void __launch_bounds__(kThreads) root_select(int count, int M, Levels L, const int* __restrict__ neigh,
+__device__ __forceinline__ void root_select_body(int psr_bid, int psr_nbid, int count, int M, const Levels& L, const int* __restrict__ neigh,
                                                         const int* __restrict__ nodeVert,
                                                         const int* __restrict__ nodeFace,
                                                         const int* __restrict__ childBase,
                                                         const float* __restrict__ vv,
-                                                        const int* __restrict__ hasInter, ScanTicket t,
+                                                        const int* __restrict__ hasInter, const ScanTicket& t,
                                                         int* __restrict__ roots, int* __restrict__ rootPos,
                                                         int* __restrict__ levelStart) {
     const int tile = ticket_tile(t);
@@ -545,6 +596,18 @@ __global__ void __launch_bounds__(kThreads) root_select(int count, int M, Levels
     if (flag) roots[a] = o;
     rootPos[o] = flag ? a : -1;  // position in roots, -1 for other nodes
 }
+__global__ void __launch_bounds__(kThreads) root_select(int count, int M, Levels L, const int* __restrict__ neigh,
+                                                        const int* __restrict__ nodeVert,
+                                                        const int* __restrict__ nodeFace,
+                                                        const int* __restrict__ childBase,
+                                                        const float* __restrict__ vv,
+                                                        const int* __restrict__ hasInter, ScanTicket t,
+                                                        int* __restrict__ roots, int* __restrict__ rootPos,
+                                                        int* __restrict__ levelStart) {
+    root_select_body(blockIdx.x, gridDim.x, count, M, L, neigh, nodeVert, nodeFace, childBase, vv, hasInter, t, roots, rootPos, levelStart);
+}
+PSR_BATCHABLE(root_select, kThreads)
+
 
 }  // namespace
 
@@ -577,13 +640,13 @@ void build_geometry(Ctx& c) {
     g.edgeOf = c.buf<int2>("edge_of", nD);
     const GeoOut out{g.vtxOwner, g.vtxKind,  g.vtxPos,   g.nodeVert,      g.edgeOwner, g.edgeKind,
                      g.nodeEdge, g.faceOwner, g.faceKind, g.faceHasParent, g.nodeFace,  g.edgeOf};
-    geometry_emit<<<tilesM + c.fld.chiBlocks, kThreads, 0, s>>>(v, offD, nD, o.parent, o.q1Fired, tv, te, tf, out,
+    PSR_LAUNCH(geometry_emit, tilesM + c.fld.chiBlocks, 0, s, v, offD, nD, o.parent, o.q1Fired, tv, te, tf, out,
                                                                 c.fld.dx, c.geo.dxGrid, tilesM, c.fld.chiArgs);
-    iso_final<<<1, 1024, 0, s>>>(c.fld.chiArgs.part, c.fld.chiBlocks, o.N, c.fld.isoSum, &c.results()->iso);
+    PSR_LAUNCH(iso_final, 1, 0, s, c.fld.chiArgs.part, c.fld.chiBlocks, o.N, c.fld.isoSum, &c.results()->iso);
     // vertex implicit values
     float* isoDev = &c.results()->iso;
     g.vvalue = c.buf<float>("vvalue", size_t(8) * M);
-    vertex_values<<<div_up(size_t(8) * M, kValueThreads), kValueThreads, 0, s>>>(v, g.counts + 0, g.vtxOwner, g.vtxKind,
+    PSR_LAUNCH(vertex_values, div_up(size_t(8) * M, kValueThreads), 0, s, v, g.counts + 0, g.vtxOwner, g.vtxKind,
                                                                        g.vtxPos, o.parent, o.childBase,
                                                                        c.fld.nbrCol, o.q1Fired, isoDev, g.vvalue, c.mail,
                                                                        reinterpret_cast<const int*>(c.results()),
@@ -594,7 +657,7 @@ void build_geometry(Ctx& c) {
     PSR_HIP_CHECK(hipGetLastError());
 }
 
-void depth_mc(Ctx& c, MainMc& mm, float* rawV, unsigned* rawKey, int* rawT) {
+void depth_mc_launch(Ctx& c, MainMc& mm, float* rawV, unsigned* rawKey, int* rawT) {
     Octree& o = c.oct;
     Geometry& g = c.geo;
     const Levels& L = o.L;
@@ -610,10 +673,10 @@ void depth_mc(Ctx& c, MainMc& mm, float* rawV, unsigned* rawKey, int* rawT) {
     mm.cube = c.buf<int>("mc_cube", nD);
     const int tilesD = div_up(nD, kThreads), tilesR = div_up(offD, kThreads);
     ScanState& scan = c.scan_for_tiles(size_t(std::max(tilesD, tilesR)));
-    edge_crossings<<<tilesD, kThreads, 0, s>>>(offD, nD, M, g.edgeOf, g.nodeVert, g.vvalue, g.vtxPos,
+    PSR_LAUNCH(edge_crossings, tilesD, 0, s, offD, nD, M, g.edgeOf, g.nodeVert, g.vvalue, g.vtxPos,
                                                scan_ticket(scan, tilesD, c.mail + 0), cflag, mm.vexAddr, rawV, L.D,
                                                rawKey);
-    cube_triangles<<<tilesD, kThreads, 0, s>>>(offD, nD, M, g.nodeVert, g.nodeEdge, cflag, g.vvalue, mm.vexAddr,
+    PSR_LAUNCH(cube_triangles, tilesD, 0, s, offD, nD, M, g.nodeVert, g.nodeEdge, cflag, g.vvalue, mm.vexAddr,
                                                g.nodeFace, o.parent, g.faceHasParent, o.q1Fired,
                                                scan_ticket(scan, tilesD, c.mail + 1), mm.cube, rawT, mm.hasInter,
                                                c.mail);
@@ -621,10 +684,17 @@ void depth_mc(Ctx& c, MainMc& mm, float* rawV, unsigned* rawKey, int* rawT) {
     mm.roots = c.buf<int>("roots", offD);
     mm.rootPos = c.buf<int>("root_pos", offD);
     if (offD > 0)
-        root_select<<<tilesR, kThreads, 0, s>>>(offD, M, L, o.neigh, g.nodeVert, g.nodeFace, o.childBase, g.vvalue,
+        PSR_LAUNCH(root_select, tilesR, 0, s, offD, M, L, o.neigh, g.nodeVert, g.nodeFace, o.childBase, g.vvalue,
                                                 mm.hasInter, scan_ticket(scan, tilesR, c.mail + 2), mm.roots,
                                                 mm.rootPos, c.mail + kMailLevelStart);
-    c.wait();
+    PSR_HIP_CHECK(hipGetLastError());
+}
+
+// After the wait: the counts the depth-D marching cubes left in the mailbox.
+void depth_mc_read(Ctx& c, MainMc& mm) {
+    Geometry& g = c.geo;
+    const Levels& L = c.oct.L;
+    const int D = L.D, offD = L.base[D];
     mm.q1Fired = c.hostPinned[kMailQ1] != 0;
     g.V = c.hostPinned[kMailGeo];
     g.E = c.hostPinned[kMailGeo + 1];

@@ -94,16 +94,7 @@ __device__ __forceinline__ float3 interpolate_point(float3 p1, float3 p2, int di
 }
 
 // Per-call state of the depth-D marching cubes.
-struct MainMc {
-    int* vexAddr = nullptr;   // crossing index per depth-D edge
-    int* cube = nullptr;      // cube category per depth-D node (exported)
-    int* hasInter = nullptr;  // surface-edge intersection flag per face
-    int* roots = nullptr;     // subdivision roots, ascending node index
-    int* rootPos = nullptr;   // per node above depth D: position in roots, or -1
-    int nCross = 0, nTri = 0, nRoots = 0;
-    int rootsPerDepth[PSR_MAX_DEPTH + 1] = {0};
-    bool q1Fired = false;  // the level-local child 0 rule changed a parent somewhere
-};
+
 
 // Morton code of a cell at a depth with `bits` levels (x high in each triplet):
 // the node key format of the reference.
@@ -154,6 +145,7 @@ __device__ __forceinline__ unsigned segment_key(int3 p1, int3 p2, int D) {
 // Depth-D marching cubes into the raw mesh buffers (vertices at [0, nCross),
 // triangles at [0, nTri)), then the subdivision roots.  Ends with one wait
 // for the counts.
-void depth_mc(Ctx& c, MainMc& mm, float* rawV, unsigned* rawKey, int* rawT);
+void depth_mc_launch(Ctx& c, MainMc& mm, float* rawV, unsigned* rawKey, int* rawT);
+void depth_mc_read(Ctx& c, MainMc& mm);
 
 }  // namespace psr

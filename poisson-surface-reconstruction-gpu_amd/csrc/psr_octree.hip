@@ -24,7 +24,7 @@ constexpr int kThreads = 256;
 // Reduces the nb block partials and returns the normalisation (x0, y0, z0,
 // edge) to every thread of the block; block 0 also stores it to params and mail.
 __device__ __forceinline__ float4 bbox_reduce(const float* __restrict__ part, int nb, float* __restrict__ params,
-                                              float* __restrict__ mail) {
+                                              float* __restrict__ mail, int bid) {
 #pragma clang fp contract(off)
     __shared__ float s[6][kThreads];
     __shared__ float4 sOut;
@@ -62,7 +62,7 @@ __device__ __forceinline__ float4 bbox_reduce(const float* __restrict__ part, in
         maxEdge = __fmul_rn(maxEdge, 1.25f);
         for (int a = 0; a < 3; ++a) lo[a] = __fsub_rn(center[a], __fdiv_rn(maxEdge, 2.0f));
         sOut = make_float4(lo[0], lo[1], lo[2], maxEdge);
-        if (blockIdx.x == 0) {
+        if (bid == 0) {
             for (int a = 0; a < 3; ++a) params[a] = mail[a] = lo[a];
             params[3] = mail[3] = maxEdge;
         }
@@ -73,10 +73,10 @@ __device__ __forceinline__ float4 bbox_reduce(const float* __restrict__ part, in
 
 // Block partial bounds (reduced by every block of the encoding, which needs
 // no launch of its own).
-__global__ void __launch_bounds__(kThreads) bbox_partial(const char* __restrict__ xyz, unsigned stride, int n,
+__device__ __forceinline__ void bbox_partial_body(int psr_bid, int psr_nbid, const char* __restrict__ xyz, unsigned stride, int n,
                                                          float* __restrict__ part) {
     float mn[3] = {1e6f, 1e6f, 1e6f}, mx[3] = {-1e6f, -1e6f, -1e6f};
-    for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
+    for (int i = psr_bid * kThreads + threadIdx.x; i < n; i += psr_nbid * kThreads) {
         const float* p = reinterpret_cast<const float*>(xyz + size_t(i) * stride);
         for (int a = 0; a < 3; ++a) {
             float v = p[a];
@@ -101,8 +101,14 @@ __global__ void __launch_bounds__(kThreads) bbox_partial(const char* __restrict_
             }
         __syncthreads();
     }
-    if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = s[threadIdx.x][0];
+    if (threadIdx.x < 6) part[psr_bid * 6 + threadIdx.x] = s[threadIdx.x][0];
 }
+__global__ void __launch_bounds__(kThreads) bbox_partial(const char* __restrict__ xyz, unsigned stride, int n,
+                                                         float* __restrict__ part) {
+    bbox_partial_body(blockIdx.x, gridDim.x, xyz, stride, n, part);
+}
+PSR_BATCHABLE(bbox_partial, kThreads)
+
 
 // --- normalise + Morton code (BuildOctree.cu:144-208) ---
 __device__ __forceinline__ unsigned encode_point(const char* __restrict__ xyz, unsigned xs,
@@ -137,7 +143,7 @@ __device__ __forceinline__ unsigned encode_point(const char* __restrict__ xyz, u
 // it to params and mail).  A block encodes one tile of the sort (two points
 // a thread) and takes the tile's histogram of the sort's first digit
 // (radix_sort_pairs), tiles of them at hist.
-__global__ void __launch_bounds__(kThreads) normalize_encode(const char* __restrict__ xyz, unsigned xs,
+__device__ __forceinline__ void normalize_encode_body(int psr_bid, int psr_nbid, const char* __restrict__ xyz, unsigned xs,
                                                              const char* __restrict__ nrm, unsigned ns, int n,
                                                              int D, const float* __restrict__ part, int nb,
                                                              float* __restrict__ params, float* __restrict__ mail,
@@ -149,9 +155,9 @@ __global__ void __launch_bounds__(kThreads) normalize_encode(const char* __restr
     __shared__ int lh[256];
     static_assert(kThreads == 256 && kSortTileKeys == 2 * kThreads, "one thread per digit, two keys each");
     lh[threadIdx.x] = 0;
-    const float4 prm = bbox_reduce(part, nb, params, mail);  // its barriers publish lh
+    const float4 prm = bbox_reduce(part, nb, params, mail, psr_bid);  // its barriers publish lh
     for (int k = 0; k < 2; ++k) {
-        const int i = blockIdx.x * kSortTileKeys + k * kThreads + threadIdx.x;
+        const int i = psr_bid * kSortTileKeys + k * kThreads + threadIdx.x;
         if (i >= n) continue;
         const unsigned code = encode_point(xyz, xs, nrm, ns, i, D, prm, posOut, nrmOut);
         keys[i] = code;
@@ -159,8 +165,21 @@ __global__ void __launch_bounds__(kThreads) normalize_encode(const char* __restr
         atomicAdd(&lh[code & 255u], 1);
     }
     __syncthreads();
-    hist[threadIdx.x * tiles + blockIdx.x] = lh[threadIdx.x];
+    hist[threadIdx.x * tiles + psr_bid] = lh[threadIdx.x];
 }
+__global__ void __launch_bounds__(kThreads) normalize_encode(const char* __restrict__ xyz, unsigned xs,
+                                                             const char* __restrict__ nrm, unsigned ns, int n,
+                                                             int D, const float* __restrict__ part, int nb,
+                                                             float* __restrict__ params, float* __restrict__ mail,
+                                                             float4* __restrict__ posOut,
+                                                             float4* __restrict__ nrmOut,
+                                                             unsigned* __restrict__ keys,
+                                                             unsigned* __restrict__ vals, int tiles,
+                                                             int* __restrict__ hist) {
+    normalize_encode_body(blockIdx.x, gridDim.x, xyz, xs, nrm, ns, n, D, part, nb, params, mail, posOut, nrmOut, keys, vals, tiles, hist);
+}
+PSR_BATCHABLE(normalize_encode, kThreads)
+
 
 // --- level ranks -----------------------------------------------------------
 // lvl(i): first level 1..D at which the code of sorted point i differs from
@@ -176,25 +195,31 @@ __device__ __forceinline__ int first_level(const unsigned* codes, int i, int D) 
 constexpr int kRankItems = 8;
 constexpr int kRankTile = kThreads * kRankItems;
 
-__global__ void __launch_bounds__(kThreads) rank_tile_hist(const unsigned* __restrict__ codes, int n, int D,
+__device__ __forceinline__ void rank_tile_hist_body(int psr_bid, int psr_nbid, const unsigned* __restrict__ codes, int n, int D,
                                                            int tiles, int* __restrict__ hist) {
     __shared__ int h[PSR_MAX_DEPTH + 2];
     if (threadIdx.x < PSR_MAX_DEPTH + 2) h[threadIdx.x] = 0;
     __syncthreads();
-    int base = blockIdx.x * kRankTile;
+    int base = psr_bid * kRankTile;
     for (int k = 0; k < kRankItems; ++k) {
         int i = base + k * kThreads + threadIdx.x;
         if (i < n) atomicAdd(&h[first_level(codes, i, D)], 1);
     }
     __syncthreads();
-    if (threadIdx.x <= D) hist[threadIdx.x * tiles + blockIdx.x] = h[threadIdx.x];
+    if (threadIdx.x <= D) hist[threadIdx.x * tiles + psr_bid] = h[threadIdx.x];
 }
+__global__ void __launch_bounds__(kThreads) rank_tile_hist(const unsigned* __restrict__ codes, int n, int D,
+                                                           int tiles, int* __restrict__ hist) {
+    rank_tile_hist_body(blockIdx.x, gridDim.x, codes, n, D, tiles, hist);
+}
+PSR_BATCHABLE(rank_tile_hist, kThreads)
+
 
 // One workgroup per level d: exclusive offsets of the tiles' new-node counts.
-__global__ void __launch_bounds__(1024) rank_tile_scan(const int* __restrict__ hist, int tiles, int D,
+__device__ __forceinline__ void rank_tile_scan_body(int psr_bid, int psr_nbid, const int* __restrict__ hist, int tiles, int D,
                                                        int* __restrict__ tileOff, int* __restrict__ U,
                                                        int* __restrict__ mail) {
-    int d = blockIdx.x + 1;
+    int d = psr_bid + 1;
     __shared__ int run;
     if (threadIdx.x == 0) run = 0;
     __syncthreads();
@@ -220,6 +245,13 @@ __global__ void __launch_bounds__(1024) rank_tile_scan(const int* __restrict__ h
     }
     if (threadIdx.x == 0) U[d] = mail[d] = run;
 }
+__global__ void __launch_bounds__(1024) rank_tile_scan(const int* __restrict__ hist, int tiles, int D,
+                                                       int* __restrict__ tileOff, int* __restrict__ U,
+                                                       int* __restrict__ mail) {
+    rank_tile_scan_body(blockIdx.x, gridDim.x, hist, tiles, D, tileOff, U, mail);
+}
+PSR_BATCHABLE(rank_tile_scan, 1024)
+
 
 struct LevelOffsets {
     int lo[PSR_MAX_DEPTH + 2];  // start of level d in the per-level node lists
@@ -228,15 +260,15 @@ struct LevelOffsets {
 // Per point: rank of its node at every level.  New nodes record their first
 // point and their parent's rank; every point records its level-(D-1) rank and
 // its level-local depth-D slot.
-__global__ void __launch_bounds__(kThreads) rank_points(const unsigned* __restrict__ codes, int n, int D, int tiles,
-                                                        const int* __restrict__ tileOff, LevelOffsets lo,
+__device__ __forceinline__ void rank_points_body(int psr_bid, int psr_nbid, const unsigned* __restrict__ codes, int n, int D, int tiles,
+                                                        const int* __restrict__ tileOff, const LevelOffsets& lo,
                                                         int* __restrict__ firstPoint, int* __restrict__ parentRank,
                                                         int* __restrict__ rankDm1, int* __restrict__ point2node,
                                                         int* __restrict__ pnum, int M, int* __restrict__ q1Fired) {
     // clears the point counts (only occupied nodes get one) and the Q1 flag
-    for (int i = blockIdx.x * kThreads + threadIdx.x; i < M; i += gridDim.x * kThreads) pnum[i] = 0;
-    if (blockIdx.x == 0 && threadIdx.x == 0) *q1Fired = 0;
-    int base = blockIdx.x * kRankTile + threadIdx.x * kRankItems;
+    for (int i = psr_bid * kThreads + threadIdx.x; i < M; i += psr_nbid * kThreads) pnum[i] = 0;
+    if (psr_bid == 0 && threadIdx.x == 0) *q1Fired = 0;
+    int base = psr_bid * kRankTile + threadIdx.x * kRankItems;
     int lv[kRankItems];
     int cnt[PSR_MAX_DEPTH + 1];
     for (int d = 1; d <= D; ++d) cnt[d] = 0;
@@ -266,7 +298,7 @@ __global__ void __launch_bounds__(kThreads) rank_points(const unsigned* __restri
             }
         }
         __syncthreads();
-        excl[d] = tileOff[d * tiles + blockIdx.x] + wsum[wid] + inc - v;
+        excl[d] = tileOff[d * tiles + psr_bid] + wsum[wid] + inc - v;
         __syncthreads();
     }
     for (int k = 0; k < kRankItems; ++k) {
@@ -288,10 +320,19 @@ __global__ void __launch_bounds__(kThreads) rank_points(const unsigned* __restri
         point2node[i] = 8 * rank[D - 1] + int(code & 7u);
     }
 }
+__global__ void __launch_bounds__(kThreads) rank_points(const unsigned* __restrict__ codes, int n, int D, int tiles,
+                                                        const int* __restrict__ tileOff, LevelOffsets lo,
+                                                        int* __restrict__ firstPoint, int* __restrict__ parentRank,
+                                                        int* __restrict__ rankDm1, int* __restrict__ point2node,
+                                                        int* __restrict__ pnum, int M, int* __restrict__ q1Fired) {
+    rank_points_body(blockIdx.x, gridDim.x, codes, n, D, tiles, tileOff, lo, firstPoint, parentRank, rankDm1, point2node, pnum, M, q1Fired);
+}
+PSR_BATCHABLE(rank_points, kThreads)
+
 
 // Writes the record of every occupied node (levels 1..D).
-__global__ void __launch_bounds__(kThreads) fill_occupied(const unsigned* __restrict__ codes, int n, Levels L,
-                                                          LevelOffsets lo, const int* __restrict__ U,
+__device__ __forceinline__ void fill_occupied_body(int psr_bid, int psr_nbid, const unsigned* __restrict__ codes, int n, const Levels& L,
+                                                          const LevelOffsets& lo, const int* __restrict__ U,
                                                           const int* __restrict__ firstPoint,
                                                           const int* __restrict__ parentRank,
                                                           const int* __restrict__ rankDm1, int total,
@@ -299,7 +340,7 @@ __global__ void __launch_bounds__(kThreads) fill_occupied(const unsigned* __rest
                                                           int* __restrict__ pnum, int* __restrict__ parent,
                                                           int* __restrict__ childBase, int* __restrict__ didx,
                                                           int* __restrict__ dnum, int* __restrict__ q1Fired) {
-    int t = blockIdx.x * kThreads + threadIdx.x;
+    int t = psr_bid * kThreads + threadIdx.x;
     if (t >= total) return;
     const int D = L.D;
     int d = 1;
@@ -339,16 +380,29 @@ __global__ void __launch_bounds__(kThreads) fill_occupied(const unsigned* __rest
         dnum[slot] = 8 * (rankDm1[next - 1] - rankDm1[first] + 1);
     }
 }
+__global__ void __launch_bounds__(kThreads) fill_occupied(const unsigned* __restrict__ codes, int n, Levels L,
+                                                          LevelOffsets lo, const int* __restrict__ U,
+                                                          const int* __restrict__ firstPoint,
+                                                          const int* __restrict__ parentRank,
+                                                          const int* __restrict__ rankDm1, int total,
+                                                          int* __restrict__ key, int* __restrict__ pidx,
+                                                          int* __restrict__ pnum, int* __restrict__ parent,
+                                                          int* __restrict__ childBase, int* __restrict__ didx,
+                                                          int* __restrict__ dnum, int* __restrict__ q1Fired) {
+    fill_occupied_body(blockIdx.x, gridDim.x, codes, n, L, lo, U, firstPoint, parentRank, rankDm1, total, key, pidx, pnum, parent, childBase, didx, dnum, q1Fired);
+}
+PSR_BATCHABLE(fill_occupied, kThreads)
+
 
 // Completes each block of 8 siblings: empty slots and the shared parent
 // (updateEmptyNodeInfo, BuildOctree.cu:426-488).  Block b of level d holds the
 // children of the b-th occupied node of level d-1.
-__global__ void __launch_bounds__(kThreads) fill_blocks(Levels L, int totalBlocks, int N, int nDm1,
+__device__ __forceinline__ void fill_blocks_body(int psr_bid, int psr_nbid, const Levels& L, int totalBlocks, int N, int nDm1,
                                                         int* __restrict__ key, int* __restrict__ pidx,
                                                         int* __restrict__ pnum, int* __restrict__ parent,
                                                         int* __restrict__ childBase, int* __restrict__ didx,
                                                         int* __restrict__ dnum) {
-    int t = blockIdx.x * kThreads + threadIdx.x;
+    int t = psr_bid * kThreads + threadIdx.x;
     if (t > totalBlocks) return;
     const int D = L.D;
     if (t == totalBlocks) {  // root
@@ -392,11 +446,25 @@ __global__ void __launch_bounds__(kThreads) fill_blocks(Levels L, int totalBlock
         parent[o] = common;
     }
 }
+__global__ void __launch_bounds__(kThreads) fill_blocks(Levels L, int totalBlocks, int N, int nDm1,
+                                                        int* __restrict__ key, int* __restrict__ pidx,
+                                                        int* __restrict__ pnum, int* __restrict__ parent,
+                                                        int* __restrict__ childBase, int* __restrict__ didx,
+                                                        int* __restrict__ dnum) {
+    fill_blocks_body(blockIdx.x, gridDim.x, L, totalBlocks, N, nDm1, key, pidx, pnum, parent, childBase, didx, dnum);
+}
+PSR_BATCHABLE(fill_blocks, kThreads)
 
-__global__ void __launch_bounds__(kThreads) zero_int(int* __restrict__ a, int n) {
-    int i = blockIdx.x * kThreads + threadIdx.x;
+
+__device__ __forceinline__ void zero_int_body(int psr_bid, int psr_nbid, int* __restrict__ a, int n) {
+    int i = psr_bid * kThreads + threadIdx.x;
     if (i < n) a[i] = 0;
 }
+__global__ void __launch_bounds__(kThreads) zero_int(int* __restrict__ a, int n) {
+    zero_int_body(blockIdx.x, gridDim.x, a, n);
+}
+PSR_BATCHABLE(zero_int, kThreads)
+
 
 // Same-depth neighbours from the parent's neighbours (BuildOctree.cu:490-512).
 // Neighbours of the depth-`depth` nodes from their parents' (the LUT recurrence
@@ -439,13 +507,13 @@ __device__ __forceinline__ void encode_node(int o, int d, int D, const int* __re
 // The launch for depth 1 also writes the root's row (itself at 13, nothing
 // else), reading it as such rather than from memory.  Every thread also
 // encodes its node (and the first, the root).
-__global__ void __launch_bounds__(kThreads) neighbors_level(int begin, int count0, int count1, int D, int depth,
+__device__ __forceinline__ void neighbors_level_body(int psr_bid, int psr_nbid, int begin, int count0, int count1, int D, int depth,
                                                             int M, const int* __restrict__ key,
                                                             const int* __restrict__ parent,
                                                             const int* __restrict__ childBase,
                                                             int* __restrict__ neigh, int* __restrict__ enc,
                                                             int* __restrict__ depthOf, float4* __restrict__ basis) {
-    const int t = blockIdx.x * kThreads + threadIdx.x;
+    const int t = psr_bid * kThreads + threadIdx.x;
     const bool fromRoot = depth == 1;
     if (fromRoot && t < 27) neigh[t * M] = t == 13 ? 0 : -1;
     if (fromRoot && t == 0) encode_node(0, 0, D, key, enc, depthOf, basis);
@@ -477,10 +545,20 @@ __global__ void __launch_bounds__(kThreads) neighbors_level(int begin, int count
 #pragma unroll
     for (int i = 0; i < 27; ++i) neigh[i * M + o] = child_neighbour(cb, s, i);
 }
+__global__ void __launch_bounds__(kThreads) neighbors_level(int begin, int count0, int count1, int D, int depth,
+                                                            int M, const int* __restrict__ key,
+                                                            const int* __restrict__ parent,
+                                                            const int* __restrict__ childBase,
+                                                            int* __restrict__ neigh, int* __restrict__ enc,
+                                                            int* __restrict__ depthOf, float4* __restrict__ basis) {
+    neighbors_level_body(blockIdx.x, gridDim.x, begin, count0, count1, D, depth, M, key, parent, childBase, neigh, enc, depthOf, basis);
+}
+PSR_BATCHABLE(neighbors_level, kThreads)
+
 
 }  // namespace
 
-void build_octree(Ctx& c, const psr_points& pts) {
+void build_octree_a(Ctx& c, const psr_points& pts) {
     Octree& o = c.oct;
     const int N = int(pts.count), D = c.cfg.depth;
     hipStream_t s = c.stream;
@@ -490,7 +568,7 @@ void build_octree(Ctx& c, const psr_points& pts) {
     const int nbb = std::min<int>(128, int(div_up(N, kThreads)));
     float* part = c.buf<float>("bbox_part", size_t(nbb) * 6);
     o.normParams = c.buf<float>("norm_params", 4);
-    bbox_partial<<<nbb, kThreads, 0, s>>>(reinterpret_cast<const char*>(pts.xyz), xs, N, part);
+    PSR_LAUNCH(bbox_partial, nbb, 0, s, reinterpret_cast<const char*>(pts.xyz), xs, N, part);
     // normalise + encode
     float4* posU = c.buf<float4>("pos_unsorted", N);
     float4* nrmU = c.buf<float4>("nrm_unsorted", N);
@@ -501,7 +579,7 @@ void build_octree(Ctx& c, const psr_points& pts) {
     // the encoding also takes the sort's first histogram
     int* sortHist = c.scratch_for(sort_scratch_ints(N));
     const int sortTiles = int(div_up(N, kSortTileKeys));
-    normalize_encode<<<sortTiles, kThreads, 0, s>>>(
+    PSR_LAUNCH(normalize_encode, sortTiles, 0, s,
         reinterpret_cast<const char*>(pts.xyz), xs, reinterpret_cast<const char*>(pts.normal), ns, N, D, part, nbb,
         o.normParams, reinterpret_cast<float*>(c.mail + kMailNorm), posU, nrmU, keys, vals, sortTiles, sortHist);
     // sorted by code; the last pass gathers the points into code order
@@ -516,9 +594,19 @@ void build_octree(Ctx& c, const psr_points& pts) {
     int* hist = c.buf<int>("rank_hist", size_t(PSR_MAX_DEPTH + 2) * tiles);
     int* tileOff = c.buf<int>("rank_tileoff", size_t(PSR_MAX_DEPTH + 2) * tiles);
     int* Ud = c.buf<int>("level_U", PSR_MAX_DEPTH + 2);
-    rank_tile_hist<<<tiles, kThreads, 0, s>>>(keys, N, D, tiles, hist);
-    rank_tile_scan<<<D, 1024, 0, s>>>(hist, tiles, D, tileOff, Ud, c.mail);
-    c.wait();
+    PSR_LAUNCH(rank_tile_hist, tiles, 0, s, keys, N, D, tiles, hist);
+    PSR_LAUNCH(rank_tile_scan, D, 0, s, hist, tiles, D, tileOff, Ud, c.mail);
+}
+
+// After the wait for the level counts: the node arrays and neighbour tables.
+void build_octree_b(Ctx& c) {
+    Octree& o = c.oct;
+    const int N = o.N, D = c.cfg.depth;
+    hipStream_t s = c.stream;
+    const unsigned* keys = reinterpret_cast<const unsigned*>(o.codes);
+    const int tiles = int(div_up(N, kRankTile));
+    int* tileOff = c.buf<int>("rank_tileoff", size_t(PSR_MAX_DEPTH + 2) * tiles);
+    int* Ud = c.buf<int>("level_U", PSR_MAX_DEPTH + 2);
     float edge;
     std::memcpy(&edge, c.hostPinned + kMailNorm + 3, sizeof(float));
     if (!(edge > 0.f) || !std::isfinite(edge))  // the reference divides by it (BuildOctree.cu:149)
@@ -553,14 +641,14 @@ void build_octree(Ctx& c, const psr_points& pts) {
     o.neigh = c.buf<int>("node_neigh", size_t(27) * M);
     o.encIdx = c.buf<int>("node_enc", M);
     o.q1Fired = &c.results()->q1Fired;
-    rank_points<<<tiles, kThreads, 0, s>>>(keys, N, D, tiles, tileOff, lo, firstPoint, parentRank, rankDm1,
+    PSR_LAUNCH(rank_points, tiles, 0, s, keys, N, D, tiles, tileOff, lo, firstPoint, parentRank, rankDm1,
                                            o.point2node, o.pnum, M, o.q1Fired);
-    fill_occupied<<<div_up(totalNodes, kThreads), kThreads, 0, s>>>(keys, N, L, lo, Ud, firstPoint, parentRank,
+    PSR_LAUNCH(fill_occupied, div_up(totalNodes, kThreads), 0, s, keys, N, L, lo, Ud, firstPoint, parentRank,
                                                                      rankDm1, totalNodes, o.key, o.pidx, o.pnum,
                                                                      o.parent, o.childBase, o.didx, o.dnum,
                                                                      o.q1Fired);
     int totalBlocks = (M - 1) / 8;
-    fill_blocks<<<div_up(totalBlocks + 1, kThreads), kThreads, 0, s>>>(L, totalBlocks, N, o.U[D - 1], o.key, o.pidx,
+    PSR_LAUNCH(fill_blocks, div_up(totalBlocks + 1, kThreads), 0, s, L, totalBlocks, N, o.U[D - 1], o.key, o.pidx,
                                                                         o.pnum, o.parent, o.childBase, o.didx,
                                                                         o.dnum);
     // neighbours (two depths per launch), packed index, depth and basis of every node
@@ -568,10 +656,16 @@ void build_octree(Ctx& c, const psr_points& pts) {
     o.basis = c.buf<float4>("node_basis", M);
     for (int d = 1; d <= D; d += 2) {
         const int c1 = d + 1 <= D ? L.count[d + 1] : 0;
-        neighbors_level<<<div_up(std::max(L.count[d] + c1, 27), kThreads), kThreads, 0, s>>>(
+        PSR_LAUNCH(neighbors_level, div_up(std::max(L.count[d] + c1, 27), kThreads), 0, s,
             L.base[d], L.count[d], c1, D, d, M, o.key, o.parent, o.childBase, o.neigh, o.encIdx, o.depth, o.basis);
     }
     PSR_HIP_CHECK(hipGetLastError());
+}
+
+void build_octree(Ctx& c, const psr_points& pts) {
+    build_octree_a(c, pts);
+    c.wait();
+    build_octree_b(c);
 }
 
 }  // namespace psr

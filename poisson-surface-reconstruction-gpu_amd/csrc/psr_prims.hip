@@ -17,7 +17,7 @@ static constexpr int kScanTile = kScanThreads * kScanItems;
 
 // Single-pass scan with decoupled look-back (psr_tile_scan.h); each thread
 // owns kScanItems consecutive elements of its tile.
-__global__ void __launch_bounds__(kScanThreads) scan_single_pass(const int* in, int* out, int n, ScanTicket t) {
+__device__ __forceinline__ void scan_single_pass_body(int psr_bid, int psr_nbid, const int* in, int* out, int n, const ScanTicket& t) {
     const int tile = ticket_tile(t);
     const int base = tile * kScanTile + threadIdx.x * kScanItems;
     int v[kScanItems];
@@ -36,8 +36,18 @@ __global__ void __launch_bounds__(kScanThreads) scan_single_pass(const int* in, 
         run += v[k];
     }
 }
+__global__ void __launch_bounds__(kScanThreads) scan_single_pass(const int* in, int* out, int n, ScanTicket t) {
+    scan_single_pass_body(blockIdx.x, gridDim.x, in, out, n, t);
+}
+PSR_BATCHABLE(scan_single_pass, kScanThreads)
 
-__global__ void set_int(int* p, int v) { *p = v; }
+
+__device__ __forceinline__ void set_int_body(int psr_bid, int psr_nbid, int* p, int v) { *p = v; }
+__global__ void __launch_bounds__(1) set_int(int* p, int v) {
+    set_int_body(blockIdx.x, gridDim.x, p, v);
+}
+PSR_BATCHABLE(set_int, 1)
+
 
 struct ZeroList {
     unsigned* p[kZeroSpans];
@@ -45,14 +55,19 @@ struct ZeroList {
     int n;
 };
 
-__global__ void __launch_bounds__(256) zero_list(ZeroList z) {
+__device__ __forceinline__ void zero_list_body(int psr_bid, int psr_nbid, const ZeroList& z) {
     const size_t total = z.end[z.n - 1];
-    for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < total; i += size_t(gridDim.x) * 256) {
+    for (size_t i = size_t(psr_bid) * 256 + threadIdx.x; i < total; i += size_t(psr_nbid) * 256) {
         int k = 0;
         while (i >= z.end[k]) ++k;
         z.p[k][i - (k ? z.end[k - 1] : 0)] = 0u;
     }
 }
+__global__ void __launch_bounds__(256) zero_list(ZeroList z) {
+    zero_list_body(blockIdx.x, gridDim.x, z);
+}
+PSR_BATCHABLE(zero_list, 256)
+
 
 void zero_spans(std::initializer_list<ZeroSpan> spans, hipStream_t s) {
     ZeroList z{};
@@ -67,7 +82,7 @@ void zero_spans(std::initializer_list<ZeroSpan> spans, hipStream_t s) {
     }
     if (z.n == 0) return;
     const unsigned blocks = unsigned(std::min<size_t>(div_up(acc, 256), 2048));
-    zero_list<<<blocks, 256, 0, s>>>(z);
+    PSR_LAUNCH(zero_list, blocks, 0, s, z);
     PSR_HIP_CHECK(hipGetLastError());
 }
 
@@ -82,11 +97,11 @@ ScanTicket scan_ticket(ScanState& st, int tiles, int* total) {
 
 void scan_exclusive(const int* in, int* out, int n, int* total, ScanState& st, hipStream_t s) {
     if (n <= 0) {
-        if (total) set_int<<<1, 1, 0, s>>>(total, 0);
+        if (total) PSR_LAUNCH(set_int, 1, 0, s, total, 0);
         return;
     }
     const int tiles = int(scan_tiles(size_t(n)));
-    scan_single_pass<<<tiles, kScanThreads, 0, s>>>(in, out, n, scan_ticket(st, tiles, total));
+    PSR_LAUNCH(scan_single_pass, tiles, 0, s, in, out, n, scan_ticket(st, tiles, total));
     PSR_HIP_CHECK(hipGetLastError());
 }
 
@@ -103,33 +118,39 @@ static constexpr int kSortTile = kSortThreads * kSortIters;  // 512 keys
 static_assert(kSortTile == kSortTileKeys, "tile of the sort's interface");
 static constexpr int kSortWaves = kSortThreads / PSR_WAVE;
 
-__global__ void __launch_bounds__(kSortThreads) radix_hist(const unsigned* __restrict__ keys, int n, int shift,
+__device__ __forceinline__ void radix_hist_body(int psr_bid, int psr_nbid, const unsigned* __restrict__ keys, int n, int shift,
                                                            int tiles, int* __restrict__ hist) {
     __shared__ int h[256];
     h[threadIdx.x] = 0;
     __syncthreads();
-    int base = blockIdx.x * kSortTile;
+    int base = psr_bid * kSortTile;
 #pragma unroll 4
     for (int k = 0; k < kSortIters; ++k) {
         int i = base + k * kSortThreads + threadIdx.x;
         if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255], 1);
     }
     __syncthreads();
-    hist[threadIdx.x * tiles + blockIdx.x] = h[threadIdx.x];
+    hist[threadIdx.x * tiles + psr_bid] = h[threadIdx.x];
 }
+__global__ void __launch_bounds__(kSortThreads) radix_hist(const unsigned* __restrict__ keys, int n, int shift,
+                                                           int tiles, int* __restrict__ hist) {
+    radix_hist_body(blockIdx.x, gridDim.x, keys, n, shift, tiles, hist);
+}
+PSR_BATCHABLE(radix_hist, kSortThreads)
 
-__global__ void __launch_bounds__(kSortThreads) radix_scatter(const unsigned* __restrict__ kin,
+
+__device__ __forceinline__ void radix_scatter_body(int psr_bid, int psr_nbid, const unsigned* __restrict__ kin,
                                                               const unsigned* __restrict__ vin,
                                                               unsigned* __restrict__ kout,
                                                               unsigned* __restrict__ vout, int n, int shift,
                                                               int tiles, const int* __restrict__ offs,
-                                                              SortGather g) {
+                                                              const SortGather& g) {
     __shared__ int running[256];      // digit count of earlier iterations + global base
     __shared__ int waveCnt[kSortWaves][256];
     const int lane = __lane_id(), wid = threadIdx.x / PSR_WAVE;
-    running[threadIdx.x] = offs[threadIdx.x * tiles + blockIdx.x];
+    running[threadIdx.x] = offs[threadIdx.x * tiles + psr_bid];
     const unsigned long long ltmask = (1ull << lane) - 1ull;
-    int base = blockIdx.x * kSortTile;
+    int base = psr_bid * kSortTile;
     for (int it = 0; it < kSortIters; ++it) {
         for (int w = 0; w < kSortWaves; ++w) waveCnt[w][threadIdx.x] = 0;
         __syncthreads();
@@ -165,6 +186,16 @@ __global__ void __launch_bounds__(kSortThreads) radix_scatter(const unsigned* __
         __syncthreads();
     }
 }
+__global__ void __launch_bounds__(kSortThreads) radix_scatter(const unsigned* __restrict__ kin,
+                                                              const unsigned* __restrict__ vin,
+                                                              unsigned* __restrict__ kout,
+                                                              unsigned* __restrict__ vout, int n, int shift,
+                                                              int tiles, const int* __restrict__ offs,
+                                                              SortGather g) {
+    radix_scatter_body(blockIdx.x, gridDim.x, kin, vin, kout, vout, n, shift, tiles, offs, g);
+}
+PSR_BATCHABLE(radix_scatter, kSortThreads)
+
 
 size_t sort_scan_elems(size_t n) { return 256 * size_t(div_up(n, kSortTile)); }
 
@@ -183,11 +214,10 @@ void radix_sort_pairs(unsigned*& keys, unsigned*& vals, unsigned*& ktmp, unsigne
     unsigned *ka = keys, *va = vals, *kb = ktmp, *vb = vtmp;
     for (int p = 0; p < passes; ++p) {
         int shift = 8 * p;
-        if (p > 0 || !firstHist) radix_hist<<<tiles, kSortThreads, 0, s>>>(ka, n, shift, tiles, hist);
+        if (p > 0 || !firstHist) PSR_LAUNCH(radix_hist, tiles, 0, s, ka, n, shift, tiles, hist);
         scan_exclusive(hist, hist, 256 * tiles, nullptr, st, s);
         const SortGather none{};
-        radix_scatter<<<tiles, kSortThreads, 0, s>>>(ka, va, kb, vb, n, shift, tiles, hist,
-                                                     p == passes - 1 ? g : none);
+        PSR_LAUNCH(radix_scatter, tiles, 0, s, ka, va, kb, vb, n, shift, tiles, hist, p == passes - 1 ? g : none);
         std::swap(ka, kb);
         std::swap(va, vb);
     }

@@ -54,7 +54,7 @@ __device__ __forceinline__ void st_shared(float* p, float v) {
 }
 
 __device__ __forceinline__ double block_sum_double(double v, double* sh) {
-    for (int off = PSR_WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, PSR_WAVE);
+    v = wave_sum_f64(v);
     int w = threadIdx.x / PSR_WAVE;
     if (__lane_id() == 0) sh[w] = v;
     __syncthreads();
@@ -283,7 +283,7 @@ struct Stencil {
 // (CGAlgorithm.cu:37-53).  The row's stencil, x, r, p and A p stay in
 // registers; one 8-byte record per neighbour is read and one written per
 // iteration.
-__global__ void __launch_bounds__(kCgThreads) cg_fp32_row(CgParams P, const int* __restrict__ neigh,
+__device__ __forceinline__ void cg_fp32_row_body(int psr_bid, int psr_nbid, const CgParams& P, const int* __restrict__ neigh,
                                                           const float* __restrict__ val, const float* __restrict__ b,
                                                           float* __restrict__ x, float2* state, double* partial,
                                                           GroupCtl* ctlAll, int* __restrict__ iters,
@@ -292,15 +292,18 @@ __global__ void __launch_bounds__(kCgThreads) cg_fp32_row(CgParams P, const int*
     __shared__ int shFlag;
     __shared__ double shRes;
     int gi = 0;
-    while (gi + 1 < P.ngroups && int(blockIdx.x) >= P.g[gi + 1].firstWg) ++gi;
+    while (gi + 1 < P.ngroups && int(psr_bid) >= P.g[gi + 1].firstWg) ++gi;
     const CgGroup G = P.g[gi];
-    const int local = blockIdx.x - G.firstWg;
+    const int local = psr_bid - G.firstWg;
     const int i = G.rowBegin + local * G.rowsPerWg + int(threadIdx.x);
     const bool own = int(threadIdx.x) < G.rowsPerWg && i < G.rowBegin + G.rows;
     const int M = P.M;
     double* part = partial + 2 * G.firstWg;
     GroupCtl* ctl = ctlAll + gi;
     unsigned phase = 0;
+    // a group of one workgroup exchanges its records through LDS (its rows
+    // are the depth's, all in this workgroup): no memory round trip per sweep
+    __shared__ float2 lrec2[2][kCgThreads];
     const bool multi = G.wgs > 1;
     const __amdgpu_buffer_rsrc_t st = vec_rsrc(state, sizeof(float2) * 2 * size_t(M));
     auto group_sum = [&](double v, double& out) -> bool {
@@ -319,7 +322,8 @@ __global__ void __launch_bounds__(kCgThreads) cg_fp32_row(CgParams P, const int*
     double loc = 0;
     if (own) {
         ri = b[i];
-        st_vec2(st, i, make_float2(ri, 0.f));
+        if (multi) st_vec2(st, i, make_float2(ri, 0.f));
+        else lrec2[0][threadIdx.x] = make_float2(ri, 0.f);
         loc = double(ri * ri);
     }
     double rrD;
@@ -328,14 +332,14 @@ __global__ void __launch_bounds__(kCgThreads) cg_fp32_row(CgParams P, const int*
     int k = 1;
     while (rr > P.tolSq && k <= P.maxIter) {
         const float beta = (k > 1) ? rr / rrOld : 0.f;
-        const int cur = ((k - 1) & 1) * M, nxt = M - cur;
+        const int cur = ((k - 1) & 1) * M, nxt = M - cur, curL = (k - 1) & 1;
         // p = r + beta p (CGAlgorithm.cu:37-43), Ap = A p (:47), p.Ap (:53)
         double lpAp = 0;
         if (own) {
             float s = 0.f;
 #pragma unroll
             for (int q = 0; q < 27; ++q) {
-                const float2 nb = ld_vec2(st, cur + sr.j[q]);
+                const float2 nb = multi ? ld_vec2(st, cur + sr.j[q]) : lrec2[curL][sr.j[q] - G.rowBegin];
                 s += sr.v[q] * p_next(nb.x, beta, nb.y);
             }
             pi = p_next(ri, beta, pi);
@@ -349,7 +353,8 @@ __global__ void __launch_bounds__(kCgThreads) cg_fp32_row(CgParams P, const int*
         if (own) {
             xi = xi + alpha * pi;
             ri = ri + (-alpha) * api;
-            st_vec2(st, nxt + i, make_float2(ri, pi));
+            if (multi) st_vec2(st, nxt + i, make_float2(ri, pi));
+            else lrec2[curL ^ 1][threadIdx.x] = make_float2(ri, pi);
             lrr = double(ri * ri);
         }
         rrOld = rr;
@@ -365,10 +370,78 @@ __global__ void __launch_bounds__(kCgThreads) cg_fp32_row(CgParams P, const int*
         resid[G.depth] = sqrt(double(rr));
     }
 }
+__global__ void __launch_bounds__(kCgThreads) cg_fp32_row(CgParams P, const int* __restrict__ neigh,
+                                                          const float* __restrict__ val, const float* __restrict__ b,
+                                                          float* __restrict__ x, float2* state, double* partial,
+                                                          GroupCtl* ctlAll, int* __restrict__ iters,
+                                                          double* __restrict__ resid, int* errFlag) {
+    cg_fp32_row_body(blockIdx.x, gridDim.x, P, neigh, val, b, x, state, partial, ctlAll, iters, resid, errFlag);
+}
+PSR_BATCHABLE(cg_fp32_row, kCgThreads)
+
+
+// A group of one workgroup with at most one row a thread (the coarse depths,
+// and a batch's small clouds), solved by that workgroup alone with its
+// vectors in LDS: cg_all_depths' arithmetic without a memory round trip per
+// sweep.
+__device__ __forceinline__ void cg_small_fp32(const CgParams& P, const CgGroup& G, const int* __restrict__ neigh,
+                                              const float* __restrict__ val, const float* __restrict__ b,
+                                              float* __restrict__ x, int* __restrict__ iters,
+                                              double* __restrict__ resid, double* sh) {
+    __shared__ float lr[kCgThreads], lp[2][kCgThreads];
+    const int M = P.M, t = int(threadIdx.x), i = G.rowBegin + t, rb = G.rowBegin;
+    const bool own = t < G.rows;
+    float xi = 0.f, ri = 0.f, pi = 0.f, api = 0.f;
+    double loc = 0;
+    if (own) {
+        ri = b[i];
+        lr[t] = ri;
+        lp[0][t] = 0.f;
+        loc = double(ri * ri);
+    }
+    float rr = float(block_sum_double(loc, sh)), rrOld = 1.f;
+    int k = 1;
+    while (rr > P.tolSq && k <= P.maxIter) {
+        const float beta = (k > 1) ? rr / rrOld : 0.f;
+        const int cur = (k - 1) & 1;
+        double lpAp = 0;
+        if (own) {
+            float s = 0.f;
+#pragma unroll
+            for (int q = 0; q < 27; ++q) {
+                const float v = val[q * M + i];
+                const int j = ((v != 0.f) ? neigh[q * M + i] : i) - rb;
+                s += v * p_next(lr[j], beta, lp[cur][j]);
+            }
+            pi = p_next(ri, beta, lp[cur][t]);
+            lp[cur ^ 1][t] = pi;
+            api = s;
+            lpAp = double(pi * s);
+        }
+        const double pAp = block_sum_double(lpAp, sh);
+        const float alpha = float(double(rr) / pAp);
+        double lrr = 0;
+        if (own) {
+            xi = xi + alpha * pi;
+            ri = ri + (-alpha) * api;
+            lr[t] = ri;
+            lrr = double(ri * ri);
+        }
+        rrOld = rr;
+        rr = float(block_sum_double(lrr, sh));
+        ++k;
+    }
+    if (own) x[i] = xi;
+    if (threadIdx.x == 0) {
+        iters[G.depth] = k - 1;
+        iters[PSR_MAX_DEPTH + 1 + G.depth] = 1;
+        resid[G.depth] = sqrt(double(rr));
+    }
+}
 
 // The same iteration with several rows per thread (large depths): vectors in
 // memory, the stencil read every sweep.
-__global__ void __launch_bounds__(kCgThreads) cg_all_depths(CgParams P, const int* __restrict__ neigh,
+__device__ __forceinline__ void cg_all_depths_body(int psr_bid, int psr_nbid, const CgParams& P, const int* __restrict__ neigh,
                                                             const float* __restrict__ val,
                                                             const float* __restrict__ b, float* __restrict__ x,
                                                             float* r, float* pA, float* pB,
@@ -379,9 +452,9 @@ __global__ void __launch_bounds__(kCgThreads) cg_all_depths(CgParams P, const in
     __shared__ int shFlag;
     __shared__ double shRes;
     int gi = 0;
-    while (gi + 1 < P.ngroups && int(blockIdx.x) >= P.g[gi + 1].firstWg) ++gi;
+    while (gi + 1 < P.ngroups && int(psr_bid) >= P.g[gi + 1].firstWg) ++gi;
     const CgGroup G = P.g[gi];
-    const int local = blockIdx.x - G.firstWg;
+    const int local = psr_bid - G.firstWg;
     const int r0 = G.rowBegin + local * G.rowsPerWg;
     const int r1 = min(G.rowBegin + G.rows, r0 + G.rowsPerWg);
     const int M = P.M;
@@ -389,6 +462,10 @@ __global__ void __launch_bounds__(kCgThreads) cg_all_depths(CgParams P, const in
     GroupCtl* ctl = ctlAll + gi;
     unsigned phase = 0;
     const bool multi = G.wgs > 1;
+    if (!multi && G.rows <= kCgThreads) {
+        cg_small_fp32(P, G, neigh, val, b, x, iters, resid, sh);
+        return;
+    }
 
     // Sum over the group (partials double-buffered by phase parity).
     auto group_sum = [&](double v, double& out) -> bool {
@@ -463,6 +540,17 @@ __global__ void __launch_bounds__(kCgThreads) cg_all_depths(CgParams P, const in
         resid[G.depth] = sqrt(double(rr));
     }
 }
+__global__ void __launch_bounds__(kCgThreads) cg_all_depths(CgParams P, const int* __restrict__ neigh,
+                                                            const float* __restrict__ val,
+                                                            const float* __restrict__ b, float* __restrict__ x,
+                                                            float* r, float* pA, float* pB,
+                                                            float* __restrict__ Ap, double* partial,
+                                                            GroupCtl* ctlAll, int* __restrict__ iters,
+                                                            double* __restrict__ resid, int* errFlag) {
+    cg_all_depths_body(blockIdx.x, gridDim.x, P, neigh, val, b, x, r, pA, pB, Ap, partial, ctlAll, iters, resid, errFlag);
+}
+PSR_BATCHABLE(cg_all_depths, kCgThreads)
+
 
 // --- mixed-precision CG ------------------------------------------------------
 // Same system per depth, solved to a true relative residual:
@@ -510,11 +598,27 @@ __device__ __forceinline__ void comp2_merge(Comp2& a, const Comp2& b) {
 }
 
 // Block-wide merge in a fixed order; every thread returns the same value.
+template <int CTRL>
+__device__ __forceinline__ void comp2_dpp_step(Comp2& v) {
+    const Comp2 o{dpp_f32<CTRL>(v.gs), dpp_f32<CTRL>(v.gc), dpp_f32<CTRL>(v.ds), dpp_f32<CTRL>(v.dc)};
+    comp2_merge(v, o);
+}
+__device__ __forceinline__ Comp2 readlane_comp2(const Comp2& v, int lane) {
+    return Comp2{readlane_f32(v.gs, lane), readlane_f32(v.gc, lane), readlane_f32(v.ds, lane), readlane_f32(v.dc, lane)};
+}
 __device__ __forceinline__ Comp2 block_comp2(Comp2 v, Comp2* sh) {
-    for (int off = PSR_WAVE / 2; off > 0; off >>= 1) {
-        Comp2 o{__shfl_xor(v.gs, off, PSR_WAVE), __shfl_xor(v.gc, off, PSR_WAVE), __shfl_xor(v.ds, off, PSR_WAVE),
-                __shfl_xor(v.dc, off, PSR_WAVE)};
-        comp2_merge(v, o);
+    // the wave's merge through DPP (psr_common.h: rows of 16, then the four
+    // rows in a fixed order)
+    comp2_dpp_step<kDppSwap1>(v);
+    comp2_dpp_step<kDppSwap2>(v);
+    comp2_dpp_step<kDppHalfMirror>(v);
+    comp2_dpp_step<kDppMirror>(v);
+    {
+        Comp2 r = readlane_comp2(v, 0);
+        comp2_merge(r, readlane_comp2(v, 16));
+        comp2_merge(r, readlane_comp2(v, 32));
+        comp2_merge(r, readlane_comp2(v, 48));
+        v = r;
     }
     int w = threadIdx.x / PSR_WAVE;
     if (__lane_id() == 0) sh[w] = v;
@@ -530,7 +634,7 @@ __device__ __forceinline__ Comp2 block_comp2(Comp2 v, Comp2* sh) {
 // the row writes {r_{k+1}, w_{k+1}, s_k} to the other buffer, keeps its r, w,
 // s, p and x in registers (x is also stored for the outer residual) and its
 // stencil in LDS.
-__global__ void __launch_bounds__(kCgThreads) cg_mixed_row(CgParams P, const int* __restrict__ neigh,
+__device__ __forceinline__ void cg_mixed_row_body(int psr_bid, int psr_nbid, const CgParams& P, const int* __restrict__ neigh,
                                                            const float* __restrict__ val,
                                                            const float* __restrict__ b, float* x, float* rOut,
                                                            float3* state, float4* slots, int* __restrict__ iters,
@@ -541,14 +645,16 @@ __global__ void __launch_bounds__(kCgThreads) cg_mixed_row(CgParams P, const int
     __shared__ float lV[27][kCgThreads];  // the row's stencil, read from LDS every sweep
     __shared__ int lJ[27][kCgThreads];
     int gi = 0;
-    while (gi + 1 < P.ngroups && int(blockIdx.x) >= P.g[gi + 1].firstWg) ++gi;
+    while (gi + 1 < P.ngroups && int(psr_bid) >= P.g[gi + 1].firstWg) ++gi;
     const CgGroup G = P.g[gi];
-    const int local = blockIdx.x - G.firstWg;
+    const int local = psr_bid - G.firstWg;
     const int i = G.rowBegin + local * G.rowsPerWg + int(threadIdx.x);
     const bool own = int(threadIdx.x) < G.rowsPerWg && i < G.rowBegin + G.rows;
     const int M = P.M;
     float4* region = slots + size_t(4) * G.firstWg;  // [2][wgs][2]
     unsigned phase = 0;
+    // a group of one workgroup exchanges its records through LDS
+    __shared__ float3 lrec3[2][kCgThreads];
     const bool multi = G.wgs > 1;
     const __amdgpu_buffer_rsrc_t st = vec_rsrc(state, sizeof(float3) * 2 * size_t(M));
     auto sum_double = [&](double v, double& out) -> bool {
@@ -614,7 +720,8 @@ __global__ void __launch_bounds__(kCgThreads) cg_mixed_row(CgParams P, const int
             wo = w;
             so = 0.f;
             po = 0.f;
-            st_vec3(st, i, make_float3(ro, w, 0.f));
+            if (multi) st_vec3(st, i, make_float3(ro, w, 0.f));
+            else lrec3[0][threadIdx.x] = make_float3(ro, w, 0.f);
             dot2_acc(d0.gs, d0.gc, ro, ro);
             dot2_acc(d0.ds, d0.dc, w, ro);
         }
@@ -631,7 +738,8 @@ __global__ void __launch_bounds__(kCgThreads) cg_mixed_row(CgParams P, const int
                 float w = 0.f;
 #pragma unroll
                 for (int q = 0; q < 27; ++q) {
-                    const float3 nb = ld_vec3(st, cur + lJ[q][threadIdx.x]);
+                    const int jq = lJ[q][threadIdx.x];
+                    const float3 nb = multi ? ld_vec3(st, cur + jq) : lrec3[k & 1][jq - G.rowBegin];
                     w = __fmaf_rn(lV[q][threadIdx.x], r_next(nb.x, alpha, s_next(nb.y, beta, nb.z)), w);
                 }
                 const float si = s_next(wo, beta, so);
@@ -639,7 +747,8 @@ __global__ void __launch_bounds__(kCgThreads) cg_mixed_row(CgParams P, const int
                 po = __fmaf_rn(beta, po, ro);
                 xi = __fmaf_rn(alpha, po, xi);
                 st_shared(&x[i], xi);
-                st_vec3(st, nxt + i, make_float3(rni, w, si));
+                if (multi) st_vec3(st, nxt + i, make_float3(rni, w, si));
+                else lrec3[(k & 1) ^ 1][threadIdx.x] = make_float3(rni, w, si);
                 ro = rni;
                 wo = w;
                 so = si;
@@ -677,11 +786,129 @@ __global__ void __launch_bounds__(kCgThreads) cg_mixed_row(CgParams P, const int
         trueRel[G.depth] = rel;
     }
 }
+__global__ void __launch_bounds__(kCgThreads) cg_mixed_row(CgParams P, const int* __restrict__ neigh,
+                                                           const float* __restrict__ val,
+                                                           const float* __restrict__ b, float* x, float* rOut,
+                                                           float3* state, float4* slots, int* __restrict__ iters,
+                                                           double* __restrict__ resid, double* __restrict__ trueRel,
+                                                           int* errFlag) {
+    cg_mixed_row_body(blockIdx.x, gridDim.x, P, neigh, val, b, x, rOut, state, slots, iters, resid, trueRel, errFlag);
+}
+PSR_BATCHABLE(cg_mixed_row, kCgThreads)
+
+
+// The mixed form's group of one workgroup with at most one row a thread,
+// vectors in LDS: cg_mixed_all_depths' arithmetic without a memory round trip
+// per sweep.
+__device__ __forceinline__ void cg_small_mixed(const CgParams& P, const CgGroup& G, const int* __restrict__ neigh,
+                                               const float* __restrict__ val, const float* __restrict__ b,
+                                               float* __restrict__ x, int* __restrict__ iters,
+                                               double* __restrict__ resid, double* __restrict__ trueRel,
+                                               double* shd, Comp2* shc) {
+    __shared__ float lx[kCgThreads], lro[kCgThreads], lr[2][kCgThreads], lw[2][kCgThreads], ls[2][kCgThreads];
+    const int M = P.M, t = int(threadIdx.x), i = G.rowBegin + t, rb = G.rowBegin;
+    const bool own = t < G.rows;
+    // x = 0; ||b||^2 in FP64 (the outer residual of x = 0 is b itself)
+    double loc = 0;
+    if (own) {
+        lx[t] = 0.f;
+        const double bi = b[i];
+        loc = bi * bi;
+    }
+    const double bb = block_sum_double(loc, shd);
+    double rr = bb, rel = bb > 0 ? 1.0 : 0.0;
+    bool fromB = true;  // the first outer step solves for b, the later ones for the residual in lro
+    int total = 0, outer = 0;
+    float gamma = 0.f, pv = 0.f;
+    while (rel > P.relTol && outer < P.maxOuter && total < P.maxIter) {
+        const double tgt = fmax(0.5 * P.relTol * sqrt(bb), 1e-6 * sqrt(rr));
+        const float tgt2 = float(tgt * tgt);
+        // r_0 = rhs, w_0 = A r_0, s_{-1} = 0, p_{-1} = 0
+        Comp2 d0{0.f, 0.f, 0.f, 0.f};
+        if (own) {
+            float w = 0.f;
+#pragma unroll
+            for (int q = 0; q < 27; ++q) {
+                const float v = val[q * M + i];
+                const int j = (v != 0.f) ? neigh[q * M + i] : i;
+                w = __fmaf_rn(v, fromB ? b[j] : lro[j - rb], w);
+            }
+            const float ri = fromB ? b[i] : lro[t];
+            lr[0][t] = ri;
+            lw[0][t] = w;
+            ls[1][t] = 0.f;
+            pv = 0.f;
+            dot2_acc(d0.gs, d0.gc, ri, ri);
+            dot2_acc(d0.ds, d0.dc, w, ri);
+        }
+        Comp2 g = block_comp2(d0, shc);
+        gamma = g.gs + g.gc;
+        const float delta = g.ds + g.dc;
+        float alpha = gamma / delta, beta = 0.f;
+        int k = 0;
+        while (gamma > tgt2 && total + k < P.maxIter) {
+            const int c = k & 1, n = c ^ 1;  // r_k, w_k in c; s_{k-1} in n
+            Comp2 dl{0.f, 0.f, 0.f, 0.f};
+            if (own) {
+                float w = 0.f;
+#pragma unroll
+                for (int q = 0; q < 27; ++q) {
+                    const float v = val[q * M + i];
+                    const int j = ((v != 0.f) ? neigh[q * M + i] : i) - rb;
+                    const float rj = r_next(lr[c][j], alpha, s_next(lw[c][j], beta, ls[n][j]));
+                    w = __fmaf_rn(v, rj, w);
+                }
+                const float ri = lr[c][t];
+                const float si = s_next(lw[c][t], beta, ls[n][t]);
+                const float rni = r_next(ri, alpha, si);
+                pv = __fmaf_rn(beta, pv, ri);
+                lx[t] = __fmaf_rn(alpha, pv, lx[t]);
+                ls[c][t] = si;
+                lr[n][t] = rni;
+                lw[n][t] = w;
+                dot2_acc(dl.gs, dl.gc, rni, rni);
+                dot2_acc(dl.ds, dl.dc, w, rni);
+            }
+            g = block_comp2(dl, shc);
+            const float gNew = g.gs + g.gc, dNew = g.ds + g.dc;
+            const float bNew = gNew / gamma;
+            alpha = gNew / (dNew - bNew * gNew / alpha);
+            beta = bNew;
+            gamma = gNew;
+            ++k;
+        }
+        total += k;
+        ++outer;
+        // outer residual in FP64
+        loc = 0;
+        if (own) {
+            double acc = b[i];
+#pragma unroll
+            for (int q = 0; q < 27; ++q) {
+                const float v = val[q * M + i];
+                const int j = ((v != 0.f) ? neigh[q * M + i] : i) - rb;
+                acc -= double(v) * double(lx[j]);
+            }
+            lro[t] = float(acc);
+            loc = acc * acc;
+        }
+        rr = block_sum_double(loc, shd);
+        rel = bb > 0 ? sqrt(rr / bb) : 0.0;
+        fromB = false;
+    }
+    if (own) x[i] = lx[t];
+    if (threadIdx.x == 0) {
+        iters[G.depth] = total;
+        iters[PSR_MAX_DEPTH + 1 + G.depth] = outer;
+        resid[G.depth] = sqrt(double(gamma));
+        trueRel[G.depth] = rel;
+    }
+}
 
 // The mixed form with several rows per thread (large depths): vectors in
 // memory, the stencil read every sweep.
-__global__ void __launch_bounds__(kCgThreads) cg_mixed_all_depths(
-    CgParams P, const int* __restrict__ neigh, const float* __restrict__ val, const float* __restrict__ b,
+__device__ __forceinline__ void cg_mixed_all_depths_body(int psr_bid, int psr_nbid, 
+    const CgParams& P, const int* __restrict__ neigh, const float* __restrict__ val, const float* __restrict__ b,
     float* x, float* rOut, float* rbuf, float* wbuf, float* sbuf, float* __restrict__ pvec, double* partialD,
     Comp2* partialC, GroupCtl* ctlAll, int* __restrict__ iters, double* __restrict__ resid,
     double* __restrict__ trueRel, int* errFlag) {
@@ -691,9 +918,9 @@ __global__ void __launch_bounds__(kCgThreads) cg_mixed_all_depths(
     __shared__ double shResD;
     __shared__ Comp2 shResC;
     int gi = 0;
-    while (gi + 1 < P.ngroups && int(blockIdx.x) >= P.g[gi + 1].firstWg) ++gi;
+    while (gi + 1 < P.ngroups && int(psr_bid) >= P.g[gi + 1].firstWg) ++gi;
     const CgGroup G = P.g[gi];
-    const int local = blockIdx.x - G.firstWg;
+    const int local = psr_bid - G.firstWg;
     const int r0 = G.rowBegin + local * G.rowsPerWg;
     const int r1 = min(G.rowBegin + G.rows, r0 + G.rowsPerWg);
     const int M = P.M;
@@ -702,6 +929,10 @@ __global__ void __launch_bounds__(kCgThreads) cg_mixed_all_depths(
     GroupCtl* ctl = ctlAll + gi;
     unsigned phase = 0;
     const bool multi = G.wgs > 1;
+    if (!multi && G.rows <= kCgThreads) {
+        cg_small_mixed(P, G, neigh, val, b, x, iters, resid, trueRel, shd, shc);
+        return;
+    }
 
     auto sum_double = [&](double v, double& out) -> bool {
         double s = block_sum_double(v, shd);
@@ -833,6 +1064,15 @@ __global__ void __launch_bounds__(kCgThreads) cg_mixed_all_depths(
         trueRel[G.depth] = rel;
     }
 }
+__global__ void __launch_bounds__(kCgThreads) cg_mixed_all_depths(
+    CgParams P, const int* __restrict__ neigh, const float* __restrict__ val, const float* __restrict__ b,
+    float* x, float* rOut, float* rbuf, float* wbuf, float* sbuf, float* __restrict__ pvec, double* partialD,
+    Comp2* partialC, GroupCtl* ctlAll, int* __restrict__ iters, double* __restrict__ resid,
+    double* __restrict__ trueRel, int* errFlag) {
+    cg_mixed_all_depths_body(blockIdx.x, gridDim.x, P, neigh, val, b, x, rOut, rbuf, wbuf, sbuf, pvec, partialD, partialC, ctlAll, iters, resid, trueRel, errFlag);
+}
+PSR_BATCHABLE(cg_mixed_all_depths, kCgThreads)
+
 
 // --- implicit function at points (LaplacianSolver.cu:85-118, 269-305) ------
 // Neighbour columns: basis parameters and d_x of neighbour i of every node, so
@@ -840,7 +1080,7 @@ __global__ void __launch_bounds__(kCgThreads) cg_mixed_all_depths(
 // Also clears the nZero words at zero (the mesh stage's node tables and face
 // marks) and, when the octree is regular, the nDense dense d_x grid cells,
 // which saves the mesh stage launches of its own.
-__global__ void __launch_bounds__(kThreads) neighbour_columns(int M, const int* __restrict__ neigh,
+__device__ __forceinline__ void neighbour_columns_body(int psr_bid, int psr_nbid, int M, const int* __restrict__ neigh,
                                                               const float4* __restrict__ basis,
                                                               const float* __restrict__ dx,
                                                               float4* __restrict__ nbrCol,
@@ -849,13 +1089,13 @@ __global__ void __launch_bounds__(kThreads) neighbour_columns(int M, const int* 
                                                               const int* __restrict__ q1Fired,
                                                               unsigned* __restrict__ ctl, size_t nCtl,
                                                               unsigned* __restrict__ slots, size_t nSlots) {
-    const size_t t = size_t(blockIdx.x) * kThreads + threadIdx.x;
-    for (size_t i = t; i < nZero; i += size_t(gridDim.x) * kThreads) zero[i] = 0u;
+    const size_t t = size_t(psr_bid) * kThreads + threadIdx.x;
+    for (size_t i = t; i < nZero; i += size_t(psr_nbid) * kThreads) zero[i] = 0u;
     // the CG's group controls and slots, used: zero again for the next solve
     if (t < nCtl) ctl[t] = 0u;
     if (t < nSlots) slots[t] = 0u;
     if (nDense && *q1Fired == 0)
-        for (size_t i = t; i < nDense; i += size_t(gridDim.x) * kThreads) dense[i] = 0.f;
+        for (size_t i = t; i < nDense; i += size_t(psr_nbid) * kThreads) dense[i] = 0.f;
     if (t >= size_t(27) * M) return;
     const int nb = neigh[t];
     if (nb < 0) {
@@ -865,13 +1105,31 @@ __global__ void __launch_bounds__(kThreads) neighbour_columns(int M, const int* 
         nbrCol[t] = make_float4(b.x, b.y, b.z, dx[nb]);
     }
 }
+__global__ void __launch_bounds__(kThreads) neighbour_columns(int M, const int* __restrict__ neigh,
+                                                              const float4* __restrict__ basis,
+                                                              const float* __restrict__ dx,
+                                                              float4* __restrict__ nbrCol,
+                                                              unsigned* __restrict__ zero, size_t nZero,
+                                                              float* __restrict__ dense, size_t nDense,
+                                                              const int* __restrict__ q1Fired,
+                                                              unsigned* __restrict__ ctl, size_t nCtl,
+                                                              unsigned* __restrict__ slots, size_t nSlots) {
+    neighbour_columns_body(blockIdx.x, gridDim.x, M, neigh, basis, dx, nbrCol, zero, nZero, dense, nDense, q1Fired, ctl, nCtl, slots, nSlots);
+}
+PSR_BATCHABLE(neighbour_columns, kThreads)
+
 
 
 // stored (non-dropped) operator entries per depth
-__global__ void zero_u32(unsigned* a, int n) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void zero_u32_body(int psr_bid, int psr_nbid, unsigned* a, int n) {
+    int i = psr_bid * blockDim.x + threadIdx.x;
     if (i < n) a[i] = 0u;
 }
+__global__ void __launch_bounds__(256) zero_u32(unsigned* a, int n) {
+    zero_u32_body(blockIdx.x, gridDim.x, a, n);
+}
+PSR_BATCHABLE(zero_u32, 256)
+
 
 }  // namespace
 
@@ -892,6 +1150,35 @@ constexpr long long kCgUnits = 1ll << 20;
 std::vector<CgInflight>& cg_inflight() {
     static std::vector<CgInflight> v;
     return v;
+}
+
+int cg_batch_budget(Ctx& c) {
+    const void* fn = c.cfg.precision == PSR_PRECISION_MIXED
+                         ? multi_queue_kernel_of<cg_mixed_all_depths_k>(cg_mixed_all_depths)
+                         : multi_queue_kernel_of<cg_all_depths_k>(cg_all_depths);
+    int per = 0, cus = 0;
+    PSR_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, kCgThreads, 0));
+    PSR_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device));
+    return std::max(1, per * cus);
+}
+
+void cg_admit_device(hipEvent_t self) {
+    std::unique_lock<std::mutex> order(cg_order_lock());
+    for (;;) {
+        auto& v = cg_inflight();
+        v.erase(std::remove_if(v.begin(), v.end(),
+                               [&](const CgInflight& f) { return f.done == self || hipEventQuery(f.done) == hipSuccess; }),
+                v.end());
+        if (v.empty()) return;
+        order.unlock();
+        std::this_thread::sleep_for(std::chrono::microseconds(20));
+        order.lock();
+    }
+}
+
+void cg_hold_device(hipEvent_t done) {
+    std::lock_guard<std::mutex> order(cg_order_lock());
+    cg_inflight().push_back(CgInflight{done, kCgUnits});
 }
 
 void cg_forget_event(hipEvent_t e) {
@@ -944,6 +1231,19 @@ void solve_depths(Ctx& c) {
         if (div_up(L.count[d], kCgThreads) > unsigned(kPollPerThread * kCgThreads)) rowForm = false;
     if (!rowForm)
         while (need(rowsPerWg) > c.cgMaxWgs) rowsPerWg *= 2;
+    // A batch sets the rows a workgroup takes instead (cgRowTarget, from the
+    // batch's total rows and the resident budget): a depth of n rows gets
+    // ceil(n / target) workgroups of equal shares; one row per thread when
+    // every share fits 256 rows.
+    int capRows[PSR_MAX_DEPTH + 1];
+    if (c.cgRowTarget > 0) {
+        rowForm = true;
+        for (int d = 0; d <= D; ++d) {
+            const int w = std::max(1, int(div_up(L.count[d], unsigned(c.cgRowTarget))));
+            capRows[d] = std::max(1, int(div_up(L.count[d], unsigned(w))));
+            if (capRows[d] > kCgThreads) rowForm = false;
+        }
+    }
     int wg = 0;
     // largest depths first so their groups start on separate CUs early
     for (int d = D; d >= 0; --d) {
@@ -952,8 +1252,8 @@ void solve_depths(Ctx& c) {
         g.rowBegin = L.base[d];
         g.rows = L.count[d];
         g.firstWg = wg;
-        g.rowsPerWg = rowsPerWg;
-        g.wgs = std::max(1, int(div_up(g.rows, rowsPerWg)));
+        g.rowsPerWg = c.cgRowTarget > 0 ? capRows[d] : rowsPerWg;
+        g.wgs = std::max(1, int(div_up(g.rows, g.rowsPerWg)));
         wg += g.wgs;
     }
     if (wg > (rowForm ? c.cgMaxWgsRow : c.cgMaxWgs))
@@ -987,24 +1287,30 @@ void solve_depths(Ctx& c) {
     // run their solves side by side whenever they fit.
     const long long cap = rowForm ? c.cgMaxWgsRow : c.cgMaxWgs;
     const long long units = (wg * kCgUnits + cap - 1) / cap;
-    std::unique_lock<std::mutex> order(cg_order_lock());
-    for (;;) {
-        auto& v = cg_inflight();
-        // this context's earlier launch is ahead of this one in its stream
-        v.erase(std::remove_if(v.begin(), v.end(),
-                               [&](const CgInflight& f) {
-                                   return f.done == c.ev[9] || hipEventQuery(f.done) == hipSuccess;
-                               }),
-                v.end());
-        long long used = 0;
-        for (const CgInflight& f : v) used += f.units;
-        if (used + units <= kCgUnits) break;
-        order.unlock();
-        std::this_thread::sleep_for(std::chrono::microseconds(20));
-        order.lock();
+    // While a batch records, the batch merges every cloud's solve into launches
+    // that fit the resident budget and accounts for them itself.
+    const bool recording = current_recorder() != nullptr;
+    if (!recording) {
+        std::unique_lock<std::mutex> order(cg_order_lock());
+        for (;;) {
+            auto& v = cg_inflight();
+            // this context's earlier launch is ahead of this one in its stream
+            v.erase(std::remove_if(v.begin(), v.end(),
+                                   [&](const CgInflight& f) {
+                                       return f.done == c.ev[9] || hipEventQuery(f.done) == hipSuccess;
+                                   }),
+                    v.end());
+            long long used = 0;
+            for (const CgInflight& f : v) used += f.units;
+            if (used + units <= kCgUnits) break;
+            order.unlock();
+            std::this_thread::sleep_for(std::chrono::microseconds(20));
+            order.lock();
+        }
+        PSR_HIP_CHECK(hipEventRecord(c.ev[8], s));
     }
-    PSR_HIP_CHECK(hipEventRecord(c.ev[8], s));
     if (rowForm && mixed) P.spinLimit = 1u << 22;
+    static const char siteMixedRow = 0, siteMixedGen = 0, siteRow = 0, siteGen = 0;
     if (mixed) {
         P.relTol = c.cfg.refine_rel_tol;
         P.maxOuter = std::max(1, c.cfg.refine_max_outer);
@@ -1012,34 +1318,35 @@ void solve_depths(Ctx& c) {
         Comp2* partialC = c.buf<Comp2>("cg_partial_c", size_t(2) * wg);
         if (rowForm) {
             float3* state = c.buf<float3>("cg_state3", size_t(2) * M);
-            void* args[] = {&P,     &o.neigh, &val,  &c.fld.div, &c.fld.dx, &rOut,
-                            &state, &slots,   &iters, &resid,    &trueRel,  &err};
-            PSR_HIP_CHECK(hipLaunchKernel(rowKernel, dim3(wg), dim3(kCgThreads), args, 0, s));
+            launch_kernel<cg_mixed_row_k>(&siteMixedRow, cg_mixed_row, wg, 0, s, true, P, o.neigh, val, c.fld.div,
+                                          c.fld.dx, rOut, state, slots, iters, resid, trueRel, err);
         } else {
             float* rbuf = c.buf<float>("cg_rbuf", size_t(2) * M);
             float* wbuf = c.buf<float>("cg_wbuf", size_t(2) * M);
             float* sbuf = c.buf<float>("cg_sbuf", size_t(2) * M);
             float* pvec = c.buf<float>("cg_pa", M);
-            void* args[] = {&P,    &o.neigh, &val,     &c.fld.div, &c.fld.dx, &rOut,  &rbuf,    &wbuf, &sbuf,
-                            &pvec, &partial, &partialC, &ctl,      &iters,    &resid, &trueRel, &err};
-            PSR_HIP_CHECK(hipLaunchKernel(genKernel, dim3(wg), dim3(kCgThreads), args, 0, s));
+            launch_kernel<cg_mixed_all_depths_k>(&siteMixedGen, cg_mixed_all_depths, wg, 0, s, true, P, o.neigh, val,
+                                                 c.fld.div, c.fld.dx, rOut, rbuf, wbuf, sbuf, pvec, partial, partialC,
+                                                 ctl, iters, resid, trueRel, err);
         }
     } else if (rowForm) {
         float2* state = c.buf<float2>("cg_state2", size_t(2) * M);
-        void* args[] = {&P, &o.neigh, &val, &c.fld.div, &c.fld.dx, &state, &partial, &ctl, &iters, &resid, &err};
-        PSR_HIP_CHECK(hipLaunchKernel(rowKernel, dim3(wg), dim3(kCgThreads), args, 0, s));
+        launch_kernel<cg_fp32_row_k>(&siteRow, cg_fp32_row, wg, 0, s, true, P, o.neigh, val, c.fld.div, c.fld.dx,
+                                     state, partial, ctl, iters, resid, err);
     } else {
         float* r = c.buf<float>("cg_r", M);
         float* pA = c.buf<float>("cg_pa", M);
         float* pB = c.buf<float>("cg_pb", M);
         float* Ap = c.buf<float>("cg_ap", M);
-        void* args[] = {&P, &o.neigh, &val, &c.fld.div, &c.fld.dx, &r, &pA, &pB, &Ap, &partial, &ctl, &iters, &resid,
-                        &err};
-        PSR_HIP_CHECK(hipLaunchKernel(genKernel, dim3(wg), dim3(kCgThreads), args, 0, s));
+        launch_kernel<cg_all_depths_k>(&siteGen, cg_all_depths, wg, 0, s, true, P, o.neigh, val, c.fld.div, c.fld.dx, r,
+                                       pA, pB, Ap, partial, ctl, iters, resid, err);
     }
     c.stats.cg_row_form = rowForm ? 1 : 0;
-    PSR_HIP_CHECK(hipEventRecord(c.ev[9], s));
-    cg_inflight().push_back(CgInflight{c.ev[9], units});
+    if (!recording) {
+        PSR_HIP_CHECK(hipEventRecord(c.ev[9], s));
+        std::lock_guard<std::mutex> order(cg_order_lock());
+        cg_inflight().push_back(CgInflight{c.ev[9], units});
+    }
     PSR_HIP_CHECK(hipGetLastError());
 }
 
@@ -1064,7 +1371,7 @@ void points_implicit(Ctx& c) {
     // mesh stage's first kernel)
     const size_t nDense = L.D <= kDenseEarlyDepth ? dense_grid_cells(L.D) : 0;
     c.geo.dxGrid = nDense ? c.buf<float>("sub_dx_grid", nDense) : nullptr;
-    neighbour_columns<<<div_up(size_t(27) * o.M, kThreads), kThreads, 0, s>>>(
+    PSR_LAUNCH(neighbour_columns, div_up(size_t(27) * o.M, kThreads), 0, s,
         o.M, o.neigh, o.basis, c.fld.dx, c.fld.nbrCol, reinterpret_cast<unsigned*>(zeroed), nZero, c.geo.dxGrid,
         nDense, o.q1Fired, reinterpret_cast<unsigned*>(c.cgCtl), c.cgCtlWords,
         reinterpret_cast<unsigned*>(c.cgSlots), c.cgSlotWords);

@@ -532,14 +532,14 @@ __device__ __forceinline__ float accumulate_nodes(const Grid& g, const int* ids,
 // Values of the corners each cell owns: cval[slot][e].  ONE: tiles lying in
 // one root (roots of at least 512 cells); otherwise tiles of whole small roots.
 template <bool ONE, int NT>
-__global__ void __launch_bounds__(NT) sub_corner_values(Grid g, const Tile* __restrict__ tiles, int nVD,
+__device__ __forceinline__ void sub_corner_values_body(int psr_bid, int psr_nbid, const Grid& g, const Tile* __restrict__ tiles, int nVD,
                                                         const float* __restrict__ iso, float* __restrict__ cval) {
     constexpr int kWords = ONE ? kLdsWordsOne : kLdsWordsSmall;
     __shared__ float4 ldsStore[kWords / 4];  // 16-byte aligned; every region below starts at a multiple of 4 words
     int* lds = reinterpret_cast<int*>(ldsStore);
     __shared__ int lChain[(ONE ? 1 : kMaxRootsPerTile) * (PSR_MAX_DEPTH + 1)];  // [nRoots][rd + 1]: root, ancestors
     __shared__ TileRoot lRoots[ONE ? 1 : kMaxRootsPerTile];
-    const Tile T = tiles[blockIdx.x];
+    const Tile T = tiles[psr_bid];
     const SetDesc S = g.sets[T.set];
     const int D = g.D, sh = D - S.rd;
     TileGeom tg = tile_geom(T, S, D);
@@ -746,6 +746,24 @@ __global__ void __launch_bounds__(NT) sub_corner_values(Grid g, const Tile* __re
         cval[size_t(slot) * nVD + best] = value_at(Lo, rr, R.first, pt) - isoV;
     }
 }
+template <bool ONE, int NT>
+__global__ void __launch_bounds__(NT) sub_corner_values(Grid g, const Tile* __restrict__ tiles, int nVD,
+                                                        const float* __restrict__ iso, float* __restrict__ cval) {
+    sub_corner_values_body<ONE, NT>(blockIdx.x, gridDim.x, g, tiles, nVD, iso, cval);
+}
+template <bool ONE, int NT>
+struct sub_corner_values_k {
+    static constexpr int kBlock = NT;
+    template <class... A>
+    __device__ __forceinline__ static void run(int bid, int nbid, const A&... a) { sub_corner_values_body<ONE, NT>(bid, nbid, a...); }
+};
+// the two instantiations launched: one-root tiles and small roots
+using sub_corner_values_one_k = sub_corner_values_k<true, kThreads>;
+using sub_corner_values_small_k = sub_corner_values_k<false, kSmallThreads>;
+constexpr auto sub_corner_values_one = sub_corner_values<true, kThreads>;
+constexpr auto sub_corner_values_small = sub_corner_values<false, kSmallThreads>;
+
+
 
 // Quotient by a small divisor through a multiply: exact for u * d < 2^32.
 struct SmallDiv {
@@ -769,7 +787,7 @@ struct SmallDiv {
 constexpr int kLdsWordsGrid = 7168;  // upper bound of grid_tile_words for depths up to 9
 constexpr int kLdsWordsSmallGrid = 12288;  // upper bound of small_tile_words for depths up to 10
 constexpr int kHaloPerThread = 4;     // halo entries per thread (<= 1024 for depths up to 9)
-__global__ void __launch_bounds__(kThreads) sub_corner_values_grid(Grid g, const Tile* __restrict__ tiles, int nVD,
+__device__ __forceinline__ void sub_corner_values_grid_body(int psr_bid, int psr_nbid, const Grid& g, const Tile* __restrict__ tiles, int nVD,
                                                                    const float* __restrict__ iso,
                                                                    float* __restrict__ cval) {
     extern __shared__ float4 ldsDyn[];  // grid_tile_words(D, rd) words
@@ -779,7 +797,7 @@ __global__ void __launch_bounds__(kThreads) sub_corner_values_grid(Grid g, const
     __shared__ TileRoot lRoot[1];
     __shared__ unsigned lSet;
     float* lds = reinterpret_cast<float*>(ldsDyn);
-    const Tile T = tiles[blockIdx.x];
+    const Tile T = tiles[psr_bid];
     if (threadIdx.x < 27) lNbr[threadIdx.x] = g.rootNbr[27 * size_t(T.root) + threadIdx.x];
     if (threadIdx.x == 32) lSet = g.rootSet[T.root];
     if (threadIdx.x < kChain) lChain[threadIdx.x] = g.rootChain[kChain * size_t(T.root) + threadIdx.x];
@@ -947,6 +965,13 @@ __global__ void __launch_bounds__(kThreads) sub_corner_values_grid(Grid g, const
         cval[size_t(b) * nVD + e] = val - isoV;
     }
 }
+__global__ void __launch_bounds__(kThreads) sub_corner_values_grid(Grid g, const Tile* __restrict__ tiles, int nVD,
+                                                                   const float* __restrict__ iso,
+                                                                   float* __restrict__ cval) {
+    sub_corner_values_grid_body(blockIdx.x, gridDim.x, g, tiles, nVD, iso, cval);
+}
+PSR_BATCHABLE(sub_corner_values_grid, kThreads)
+
 
 // LDS words of a small-root tile whose roots (nRoots of them) are at depth rd
 // with D - rd <= 2: per root the list tables [3][n][Kp] (n = 2^(D-rd) + 1
@@ -969,7 +994,7 @@ __host__ __device__ inline int small_tile_words(int D, int rd, int nRoots) {
 // dense per-depth grids and the halo factors), one thread per corner point of
 // a root, evaluated for the root holding the point's owner.  Terms, their
 // order and their association are those of sub_corner_values.
-__global__ void __launch_bounds__(kSmallThreads) sub_corner_values_small_grid(Grid g, const Tile* __restrict__ tiles,
+__device__ __forceinline__ void sub_corner_values_small_grid_body(int psr_bid, int psr_nbid, const Grid& g, const Tile* __restrict__ tiles,
                                                                               int nVD, const float* __restrict__ iso,
                                                                               float* __restrict__ cval) {
     extern __shared__ float4 ldsDyn[];  // small_tile_words(D, rd, roots of the tile) words
@@ -979,7 +1004,7 @@ __global__ void __launch_bounds__(kSmallThreads) sub_corner_values_small_grid(Gr
     __shared__ int lChain[kR * kChain];
     __shared__ unsigned lSet[kR];
     float* lds = reinterpret_cast<float*>(ldsDyn);
-    const Tile T = tiles[blockIdx.x];
+    const Tile T = tiles[psr_bid];
     const SetDesc S = g.sets[T.set];
     const int D = g.D, sh = D - S.rd, n = (1 << sh) + 1, P = n * n * n;
     const TileGeom tg = tile_geom(T, S, D);
@@ -1113,6 +1138,13 @@ __global__ void __launch_bounds__(kSmallThreads) sub_corner_values_small_grid(Gr
     }
     cval[size_t(slot) * nVD + best] = val - *iso;
 }
+__global__ void __launch_bounds__(kSmallThreads) sub_corner_values_small_grid(Grid g, const Tile* __restrict__ tiles,
+                                                                              int nVD, const float* __restrict__ iso,
+                                                                              float* __restrict__ cval) {
+    sub_corner_values_small_grid_body(blockIdx.x, gridDim.x, g, tiles, nVD, iso, cval);
+}
+PSR_BATCHABLE(sub_corner_values_small_grid, kSmallThreads)
+
 
 // Exclusive prefix of v over the block's threads and the block total (every
 // thread of the block must call it).
@@ -1159,7 +1191,7 @@ constexpr unsigned kEdgeBits = 0xFFFu;
 // per tile: crossing, kept-vertex and triangle counts, and each cell's first
 // crossing within the tile.  Corner values of crossing cells are kept
 // for the emission pass.
-__global__ void __launch_bounds__(kThreads) sub_classify(Grid g, const Tile* __restrict__ tiles, int nVD,
+__device__ __forceinline__ void sub_classify_body(int psr_bid, int psr_nbid, const Grid& g, const Tile* __restrict__ tiles, int nVD,
                                                          const float* __restrict__ cval, float* __restrict__ vals,
                                                          unsigned* __restrict__ cellWord,
                                                          int* __restrict__ crossPrefix, int4* __restrict__ tileCnt) {
@@ -1170,7 +1202,7 @@ __global__ void __launch_bounds__(kThreads) sub_classify(Grid g, const Tile* __r
     __shared__ float lVal[kClassifyPoints];
     __shared__ unsigned short lItem[kThreads * 12];  // crossing edges as (cell << 4 | edge), cell order
     __shared__ unsigned lOwn[kThreads];              // per cell: owned crossing edges, kept ones << 12
-    const Tile T = tiles[blockIdx.x];
+    const Tile T = tiles[psr_bid];
     const SetDesc S = g.sets[T.set];
     const int D = g.D;
     const TileGeom tg = tile_geom(T, S, D);
@@ -1255,14 +1287,22 @@ __global__ void __launch_bounds__(kThreads) sub_classify(Grid g, const Tile* __r
     const int pre = block_exclusive<kThreads>((__popc(mask) << 16) | nt, lSum, tct);
     if (active) crossPrefix[e] = pre >> 16;
     (void)block_exclusive<kThreads>(__popc(keep), lSum, tk);
-    if (threadIdx.x == 0) tileCnt[blockIdx.x] = make_int4(tct >> 16, tk, tct & 0xFFFF, 0);
+    if (threadIdx.x == 0) tileCnt[psr_bid] = make_int4(tct >> 16, tk, tct & 0xFFFF, 0);
 }
+__global__ void __launch_bounds__(kThreads) sub_classify(Grid g, const Tile* __restrict__ tiles, int nVD,
+                                                         const float* __restrict__ cval, float* __restrict__ vals,
+                                                         unsigned* __restrict__ cellWord,
+                                                         int* __restrict__ crossPrefix, int4* __restrict__ tileCnt) {
+    sub_classify_body(blockIdx.x, gridDim.x, g, tiles, nVD, cval, vals, cellWord, crossPrefix, tileCnt);
+}
+PSR_BATCHABLE(sub_classify, kThreads)
+
 
 // Exclusive scan of the per-tile counts in one workgroup; totals[0..2] are the
 // chunk's crossings, kept vertices and triangles; totals[3] = *keptMain, the
 // vertices kept by the depth-D batch.
 constexpr int kScanThreads = 1024;
-__global__ void __launch_bounds__(kScanThreads) sub_tile_scan(const int4* __restrict__ cnt, int n,
+__device__ __forceinline__ void sub_tile_scan_body(int psr_bid, int psr_nbid, const int4* __restrict__ cnt, int n,
                                                               int4* __restrict__ base, const int* __restrict__ keptMain,
                                                               int* __restrict__ totals) {
     constexpr int kW = kScanThreads / PSR_WAVE;
@@ -1310,6 +1350,13 @@ __global__ void __launch_bounds__(kScanThreads) sub_tile_scan(const int4* __rest
         totals[3] = *keptMain;
     }
 }
+__global__ void __launch_bounds__(kScanThreads) sub_tile_scan(const int4* __restrict__ cnt, int n,
+                                                              int4* __restrict__ base, const int* __restrict__ keptMain,
+                                                              int* __restrict__ totals) {
+    sub_tile_scan_body(blockIdx.x, gridDim.x, cnt, n, base, keptMain, totals);
+}
+PSR_BATCHABLE(sub_tile_scan, kScanThreads)
+
 
 // Kept crossing points and the triangles of the subdivision batches.  A
 // triangle index is the number of vertices kept before the batch plus the raw
@@ -1317,7 +1364,7 @@ __global__ void __launch_bounds__(kScanThreads) sub_tile_scan(const int4* __rest
 // (insertTriangle :1247-1282).  *keptMain + keptBefore / triBefore: vertices
 // kept and triangles written before this chunk (*keptMain: those of the
 // depth-D batch).
-__global__ void __launch_bounds__(kThreads) sub_emit(Grid g, const Tile* __restrict__ tiles, int nVD,
+__device__ __forceinline__ void sub_emit_body(int psr_bid, int psr_nbid, const Grid& g, const Tile* __restrict__ tiles, int nVD,
                                                      const float* __restrict__ vals,
                                                      const unsigned* __restrict__ cellWord,
                                                      const int* __restrict__ crossPrefix,
@@ -1330,9 +1377,9 @@ __global__ void __launch_bounds__(kThreads) sub_emit(Grid g, const Tile* __restr
     __shared__ TileRoot lRoots[kMaxRootsPerTile];
     __shared__ int lNbr[kMaxRootsPerTile * 27];
     __shared__ unsigned lSet[kMaxRootsPerTile];
-    const int4 cnt = tileCnt[blockIdx.x];
+    const int4 cnt = tileCnt[psr_bid];
     if (cnt.y == 0 && cnt.z == 0) return;  // the same for the whole block
-    const Tile T = tiles[blockIdx.x];
+    const Tile T = tiles[psr_bid];
     const SetDesc S = g.sets[T.set];
     const TileGeom tg = tile_geom(T, S, g.D);
     // the tile's roots (made visible by the block scans' barriers)
@@ -1351,7 +1398,7 @@ __global__ void __launch_bounds__(kThreads) sub_emit(Grid g, const Tile* __restr
     const int rr = tg.oneRoot ? 0 : int(threadIdx.x) >> (3 * (g.D - S.rd));
     const Leaf L = leaf_of(g, T.set, S, lRoots[rr], e);
     const RecVE ve = rec_ve(lNbr + 27 * rr, lSet[rr], L, g.D);
-    const int4 tb = tileBase[blockIdx.x];
+    const int4 tb = tileBase[psr_bid];
     keptBefore += *keptMain;
     if (keep) {
         float v[8];
@@ -1387,6 +1434,19 @@ __global__ void __launch_bounds__(kThreads) sub_emit(Grid g, const Tile* __restr
         }
     }
 }
+__global__ void __launch_bounds__(kThreads) sub_emit(Grid g, const Tile* __restrict__ tiles, int nVD,
+                                                     const float* __restrict__ vals,
+                                                     const unsigned* __restrict__ cellWord,
+                                                     const int* __restrict__ crossPrefix,
+                                                     const int4* __restrict__ tileCnt,
+                                                     const int4* __restrict__ tileBase,
+                                                     const int* __restrict__ keptMain, int keptBefore, int triBefore,
+                                                     float* __restrict__ outV, int* __restrict__ outT,
+                                                     unsigned* __restrict__ outKey) {
+    sub_emit_body(blockIdx.x, gridDim.x, g, tiles, nVD, vals, cellWord, crossPrefix, tileCnt, tileBase, keptMain, keptBefore, triBefore, outV, outT, outKey);
+}
+PSR_BATCHABLE(sub_emit, kThreads)
+
 
 // Per root: the position in roots of each real neighbour (or -1), and the
 // chain of the root and its ancestors.
@@ -1447,19 +1507,30 @@ __device__ __forceinline__ void root_records(const RootRecArgs& a, int block) {
 }
 
 // d_x of every node at its cell of the dense grid of its depth.
-__global__ void __launch_bounds__(kThreads) scatter_dx_grid(int M, Levels L, const int* __restrict__ key,
+__device__ __forceinline__ void scatter_dx_grid_body(int psr_bid, int psr_nbid, int M, const Levels& L, const int* __restrict__ key,
                                                             const float* __restrict__ dx, float* __restrict__ grid) {
-    const int i = blockIdx.x * kThreads + threadIdx.x;
+    const int i = psr_bid * kThreads + threadIdx.x;
     if (i >= M) return;
     int l = 0;
     while (l < L.D && i >= L.base[l + 1]) ++l;
     grid[dense_grid_index(key[i], l, L.D)] = dx[i];
 }
+__global__ void __launch_bounds__(kThreads) scatter_dx_grid(int M, Levels L, const int* __restrict__ key,
+                                                            const float* __restrict__ dx, float* __restrict__ grid) {
+    scatter_dx_grid_body(blockIdx.x, gridDim.x, M, L, key, dx, grid);
+}
+PSR_BATCHABLE(scatter_dx_grid, kThreads)
 
-__global__ void __launch_bounds__(kThreads) zero_floats(float* __restrict__ a, size_t n) {
-    const size_t i = size_t(blockIdx.x) * kThreads + threadIdx.x;
+
+__device__ __forceinline__ void zero_floats_body(int psr_bid, int psr_nbid, float* __restrict__ a, size_t n) {
+    const size_t i = size_t(psr_bid) * kThreads + threadIdx.x;
     if (i < n) a[i] = 0.f;
 }
+__global__ void __launch_bounds__(kThreads) zero_floats(float* __restrict__ a, size_t n) {
+    zero_floats_body(blockIdx.x, gridDim.x, a, n);
+}
+PSR_BATCHABLE(zero_floats, kThreads)
+
 
 // markValidMeshVertexIndex (:1170-1181): a vertex with |x| < 1e-6 is dropped;
 // triangle indices are not remapped (insertTriangle :1247-1282).  The kept
@@ -1534,9 +1605,9 @@ __device__ __forceinline__ void make_tiles(const TileArgs& a, int block) {
 // The subdivision's set-up in one launch: blocks [0, keepBlocks) compact the
 // vertices kept by the depth-D batch, the next rootBlocks blocks write the
 // root records, the remaining blocks expand the tiles (parts blocks per set).
-__global__ void __launch_bounds__(kThreads) sub_setup(KeepArgs ka, int keepBlocks, RootRecArgs ra, int rootBlocks,
-                                                      TileArgs ta) {
-    const int b = blockIdx.x;
+__device__ __forceinline__ void sub_setup_body(int psr_bid, int psr_nbid, const KeepArgs& ka, int keepBlocks, const RootRecArgs& ra, int rootBlocks,
+                                                      const TileArgs& ta) {
+    const int b = psr_bid;
     if (b < keepBlocks) {
         keep_compact(ka);
     } else if (b < keepBlocks + rootBlocks) {
@@ -1545,6 +1616,12 @@ __global__ void __launch_bounds__(kThreads) sub_setup(KeepArgs ka, int keepBlock
         make_tiles(ta, b - keepBlocks - rootBlocks);
     }
 }
+__global__ void __launch_bounds__(kThreads) sub_setup(KeepArgs ka, int keepBlocks, RootRecArgs ra, int rootBlocks,
+                                                      TileArgs ta) {
+    sub_setup_body(blockIdx.x, gridDim.x, ka, keepBlocks, ra, rootBlocks, ta);
+}
+PSR_BATCHABLE(sub_setup, kThreads)
+
 
 // LDS words of a sub_corner_values_grid tile whose root is at depth rd, and
 // its halo entries.
@@ -1565,19 +1642,30 @@ constexpr size_t kChunkCells = size_t(1) << 24;
 
 }  // namespace
 
-void extract_mesh(Ctx& c) {
+// ---- depth-D marching cubes: the first output batch; its triangles are
+// written in place (raw crossing ids: batch start 0, nothing kept before it)
+void extract_mesh_a(Ctx& c) {
+    const Levels& L = c.oct.L;
+    const int nD = L.count[L.D];
+    float* mainV = c.buf<float>("raw_main_v", size_t(36) * nD);  // at most one crossing per depth-D edge
+    unsigned* mainKey = c.buf<unsigned>("raw_main_key", size_t(12) * nD);
+    int* outT = c.buf_keep<int>("mesh_triangles", size_t(15) * nD, 0);
+    c.mc = MainMc{};
+    c.mesh.deferred = false;
+    depth_mc_launch(c, c.mc, mainV, mainKey, outT);
+}
+
+// After the wait for the depth-D counts: the subdivision and the output.
+void extract_mesh_b(Ctx& c) {
     Octree& o = c.oct;
     const Levels& L = o.L;
     const int M = o.M, D = L.D, nD = L.count[D], offD = L.base[D];
     hipStream_t st = c.stream;
-    // ---- depth-D marching cubes: the first output batch; its triangles are
-    // written in place (raw crossing ids: batch start 0, nothing kept before it)
-    float* mainV = c.buf<float>("raw_main_v", size_t(36) * nD);  // at most one crossing per depth-D edge
+    MainMc& mm = c.mc;
+    depth_mc_read(c, mm);
+    float* mainV = c.buf<float>("raw_main_v", size_t(36) * nD);
     unsigned* mainKey = c.buf<unsigned>("raw_main_key", size_t(12) * nD);
     int* outT = c.buf_keep<int>("mesh_triangles", size_t(15) * nD, 0);
-    MainMc mm;
-    c.mesh.deferred = false;
-    depth_mc(c, mm, mainV, mainKey, outT);  // ends with a wait: no kernel is in flight
     c.stats.n_subdivision_roots = mm.nRoots;
     // ---- subdivision sets: coarse roots one per set, finer roots one set per depth
     // (ComputeTriangleIndices.h:371: roots above depth D-4 are coarse)
@@ -1624,6 +1712,7 @@ void extract_mesh(Ctx& c) {
     const size_t vCap = size_t(mm.nCross) + vBound, tCap = size_t(mm.nTri) + 5 * cellsAll;
     const bool deferred =
         (nSets <= 1 || cellsAll <= chunkCells) && vCap <= size_t(INT32_MAX) && tCap <= size_t(INT32_MAX) / 3;
+    if (!deferred && current_recorder()) throw NotRecordable{};  // it waits between working sets
     constexpr int kMailKept = 3;
     c.hostPinned[1] = c.hostPinned[2] = c.hostPinned[kMailKept] = 0;
     // vertices kept by the depth-D batch: a device int read by the subdivision
@@ -1636,7 +1725,7 @@ void extract_mesh(Ctx& c) {
     const int keepBlocks = std::max(1, int(div_up(mm.nCross, kThreads)));
     const KeepArgs ka{mainV, mainKey, mm.nCross,
                       scan_ticket(c.scan_for_tiles(size_t(keepBlocks)), keepBlocks, keptMain), outV, outKey};
-    if (nSets == 0) sub_setup<<<keepBlocks, kThreads, 0, st>>>(ka, keepBlocks, RootRecArgs{}, 0, TileArgs{});
+    if (nSets == 0) PSR_LAUNCH(sub_setup, keepBlocks, 0, st, ka, keepBlocks, RootRecArgs{}, 0, TileArgs{});
     if (deferred && nSets > 0) outT = c.buf_keep<int>("mesh_triangles", 3 * tCap, 3 * size_t(mm.nTri));
     if (nSets > 0) {
         const int* rootPos = mm.rootPos;
@@ -1663,8 +1752,8 @@ void extract_mesh(Ctx& c) {
         if (denseHalo && !dxAt) {
             const size_t cellsAll = dense_grid_cells(D);
             dxAt = c.buf<float>("sub_dx_grid", cellsAll);
-            zero_floats<<<div_up(cellsAll, kThreads), kThreads, 0, st>>>(dxAt, cellsAll);
-            scatter_dx_grid<<<div_up(M, kThreads), kThreads, 0, st>>>(M, L, o.key, c.fld.dx, dxAt);
+            PSR_LAUNCH(zero_floats, div_up(cellsAll, kThreads), 0, st, dxAt, cellsAll);
+            PSR_LAUNCH(scatter_dx_grid, div_up(M, kThreads), 0, st, M, L, o.key, c.fld.dx, dxAt);
         }
         std::vector<SetDesc> cs;
         std::vector<int> firstSmall;
@@ -1722,7 +1811,7 @@ void extract_mesh(Ctx& c) {
                                   D, parts, dsets, reinterpret_cast<Tile*>(dAll + bSets),
                                   reinterpret_cast<Tile*>(dAll + bSets + bTiles)};
                 const int kb = s0 == 0 ? keepBlocks : 0, rb = s0 == 0 ? rootBlocks : 0;
-                sub_setup<<<unsigned(kb + rb + nCs * parts), kThreads, 0, st>>>(ka, kb, ra, rb, ta);
+                PSR_LAUNCH(sub_setup, unsigned(kb + rb + nCs * parts), 0, st, ka, kb, ra, rb, ta);
             }
             Tile* dtiles = reinterpret_cast<Tile*>(dAll + bSets);
             Tile* dsmall = reinterpret_cast<Tile*>(dAll + bSets + bTiles);
@@ -1738,25 +1827,24 @@ void extract_mesh(Ctx& c) {
                 int words = 0;
                 for (int k = 0; k < nCs; ++k)
                     if (D - cs[k].rd >= 3) words = std::max(words, grid_tile_words(D, cs[k].rd));
-                sub_corner_values_grid<<<nOne, kThreads, sizeof(float) * words, st>>>(gr, dtiles, nVD, isoDev, cval);
+                PSR_LAUNCH(sub_corner_values_grid, nOne, sizeof(float) * words, st, gr, dtiles, nVD, isoDev, cval);
             }
             else if (nOne > 0)
-                sub_corner_values<true, kThreads><<<nOne, kThreads, 0, st>>>(gr, dtiles, nVD, isoDev, cval);
+                PSR_LAUNCH_K(sub_corner_values_one, sub_corner_values_one_k, nOne, 0, st, gr, dtiles, nVD, isoDev, cval);
             if (nSmall > 0 && denseHalo) {
                 int words = 0;
                 for (int k = 0; k < nCs; ++k)
                     if (D - cs[k].rd < 3)
                         words = std::max(words, small_tile_words(D, cs[k].rd, D - cs[k].rd == 1 ? 4 : 1));
-                sub_corner_values_small_grid<<<nSmall, kSmallThreads, sizeof(float) * words, st>>>(gr, dsmall, nVD,
-                                                                                                   isoDev, cval);
+                PSR_LAUNCH(sub_corner_values_small_grid, nSmall, sizeof(float) * words, st, gr, dsmall, nVD, isoDev, cval);
             } else if (nSmall > 0) {
-                sub_corner_values<false, kSmallThreads><<<nSmall, kSmallThreads, 0, st>>>(gr, dsmall, nVD, isoDev, cval);
+                PSR_LAUNCH_K(sub_corner_values_small, sub_corner_values_small_k, nSmall, 0, st, gr, dsmall, nVD, isoDev, cval);
             }
-            sub_classify<<<nTiles, kThreads, 0, st>>>(gr, dtiles, nVD, cval, vals, cellWord, crossPre, tileCnt);
+            PSR_LAUNCH(sub_classify, nTiles, 0, st, gr, dtiles, nVD, cval, vals, cellWord, crossPre, tileCnt);
             // totals: slots 0-3
-            sub_tile_scan<<<1, kScanThreads, 0, st>>>(tileCnt, nTiles, tileBase, keptMain, c.mail);
+            PSR_LAUNCH(sub_tile_scan, 1, 0, st, tileCnt, nTiles, tileBase, keptMain, c.mail);
             if (deferred) {
-                sub_emit<<<nTiles, kThreads, 0, st>>>(gr, dtiles, nVD, vals, cellWord, crossPre, tileCnt, tileBase,
+                PSR_LAUNCH(sub_emit, nTiles, 0, st, gr, dtiles, nVD, vals, cellWord, crossPre, tileCnt, tileBase,
                                                       keptMain, 0, mm.nTri, outV, outT, outKey);
                 s0 = s1;
                 continue;
@@ -1771,7 +1859,7 @@ void extract_mesh(Ctx& c) {
             outKey = c.buf_keep<unsigned>("mesh_vertex_keys", keptTotal + nKept, keptTotal);
             outT = c.buf_keep<int>("mesh_triangles", 3 * (triTotal + nTri), 3 * triTotal);
             if (nKept > 0 || nTri > 0)
-                sub_emit<<<nTiles, kThreads, 0, st>>>(gr, dtiles, nVD, vals, cellWord, crossPre, tileCnt, tileBase,
+                PSR_LAUNCH(sub_emit, nTiles, 0, st, gr, dtiles, nVD, vals, cellWord, crossPre, tileCnt, tileBase,
                                                       keptMain, int(keptSub), int(triTotal), outV, outT, outKey);
             keptSub += nKept;
             triTotal += nTri;
@@ -1793,6 +1881,12 @@ void extract_mesh(Ctx& c) {
     c.mesh.nt = int(triTotal);
     c.stats.n_mesh_vertices = c.mesh.nv;
     c.stats.n_mesh_triangles = c.mesh.nt;
+}
+
+void extract_mesh(Ctx& c) {
+    extract_mesh_a(c);
+    c.wait();
+    extract_mesh_b(c);
 }
 
 void finish_mesh(Ctx& c) {

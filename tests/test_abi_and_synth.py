@@ -22,7 +22,7 @@ def test_struct_layouts_match_header(pkg):
     b = pkg.binding
     assert ctypes.sizeof(b.Points) == 24
     assert ctypes.sizeof(b.Mesh) == 24
-    assert ctypes.sizeof(b.Config) == 36
+    assert ctypes.sizeof(b.Config) == 40
     cfg = b.Config()
     b.lib().psr_default_config(ctypes.byref(cfg))
     assert cfg.depth == 7 and abs(cfg.cg_tol - 1e-5) < 1e-12 and cfg.cg_max_iter == 10000

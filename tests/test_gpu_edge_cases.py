@@ -153,14 +153,19 @@ def test_generic_cg_forms(generic_form, oracle_runs, name, prec):
 
 
 def test_depth8_against_oracle(pkg, orc, torch):
-    """Depth 8: node arrays and neighbours bit-exact, per-depth solution
-    within 1e-5 of the FP64 CG, the mesh on the sphere."""
+    """Depth 8: node arrays, neighbours and vertex / edge / face records
+    bit-exact, per-depth solution within 1e-5 of the FP64 CG, vertex values
+    within 1e-4, the mesh on the sphere (the reference's algorithm also leaves
+    an inner sheet at this density, which the oracle's mesh shows too, so the
+    median radius is checked)."""
     xyz, nrm = pkg.synth.sphere(n=100000)
     rec = pkg.PoissonReconstruction(depth=8)
     rec.SolvePoissionReconstructionMesh(_up(torch, xyz), _up(torch, nrm))
-    ref = orc.Result(xyz, nrm, 8, stages=7, precision=1)
-    for g, o in INT_ARRAYS:
+    ref = orc.Result(xyz, nrm, 8, stages=7 | orc.STAGE_GEOMETRY, precision=1)
+    for g, o in INT_ARRAYS + [("VERTS", "verts"), ("EDGES", "edges"), ("FACES", "faces"),
+                              ("NODE_VERTICES", "node_vertices"), ("NODE_FACES", "node_faces")]:
         np.testing.assert_array_equal(rec.export(g).reshape(-1), ref.array(o).reshape(-1))
+    assert _relmax(rec.export("VVALUE"), ref.array("vvalue")) <= 1e-4
     dx, dxo = rec.export("DX").astype(np.float64), ref.array("dx")
     base = ref.base
     for d in range(len(base) - 1):
@@ -171,7 +176,7 @@ def test_depth8_against_oracle(pkg, orc, torch):
     edge = rec.export("NORM_PARAMS")[3]
     r = np.linalg.norm(v - 0.5, axis=1)
     r0 = 0.4 / edge
-    assert abs(np.median(r) - r0) <= 0.5 / 256 and np.abs(r - r0).max() <= 2.0 / 256
+    assert abs(np.median(r) - r0) <= 0.5 / 256
     rec.close()
 
 
