@@ -191,6 +191,18 @@ int psr_weld_mesh(psr_ctx* ctx, psr_mesh* out);
  * (the reference divides 0 by 0 there); a triangle with a corner outside the
  * mesh contributes nothing. */
 int psr_mesh_normals(psr_ctx* ctx, const psr_mesh* mesh, float* normals);
+/* psr_mesh_normals without a context and without allocating: no hipMalloc,
+ * hipFree or allocating library call.  Its scratch (face normals, sort
+ * buffers, histograms, scan state) lives in one file-scope device array of
+ * about 17 MB that each GPU holds until the process ends.  No counterpart in
+ * the reference; for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid it).  Meshes of up to
+ * psr_mesh_normals_static_max_triangles() triangles and 2^24 - 1 vertices
+ * (PSR_ERR_CAPACITY beyond).  Stream-ordered on hip_stream (NULL: the null
+ * stream); returns once the normals are written.  Every call on a device
+ * shares that device's array: calls that overlap on one device race. */
+int psr_mesh_normals_static(const psr_mesh* mesh, float* normals, void* hip_stream);
+uint32_t psr_mesh_normals_static_max_triangles(void);
 /* Normals of an unoriented point cloud in device memory (xyz at the start of
  * each stride_bytes record), the step before psr_reconstruct: replaces the
  * reference's PCL NormalEstimationOMP (mesh/PoissonReconstruction.cpp:137-186,

@@ -40,7 +40,7 @@ _ARRAY_LAYOUT = {
 EXPORTED_SYMBOLS = ["psr_default_config", "psr_create", "psr_reconstruct", "psr_reconstruct_batch",
                     "psr_get_batch_stats", "psr_batch_cloud", "psr_batch_cg_plan", "psr_sync", "psr_get_stats", "psr_export", "psr_destroy",
                     "psr_status_string", "psr_last_error", "psr_weld_mesh", "psr_write_ply", "psr_mesh_normals", "psr_estimate_normals",
-                    "psr_vertex_colors"]
+                    "psr_vertex_colors", "psr_mesh_normals_static", "psr_mesh_normals_static_max_triangles"]
 
 
 class Config(ctypes.Structure):
@@ -136,6 +136,9 @@ def lib():
         L.psr_destroy.argtypes = [ctypes.c_void_p]
         L.psr_weld_mesh.argtypes = [ctypes.c_void_p, ctypes.POINTER(Mesh)]
         L.psr_mesh_normals.argtypes = [ctypes.c_void_p, ctypes.POINTER(Mesh), ctypes.c_void_p]
+        L.psr_mesh_normals_static.argtypes = [ctypes.POINTER(Mesh), ctypes.c_void_p, ctypes.c_void_p]
+        L.psr_mesh_normals_static_max_triangles.restype = ctypes.c_uint32
+        L.psr_mesh_normals_static_max_triangles.argtypes = []
         L.psr_estimate_normals.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
                                            ctypes.c_int, ctypes.POINTER(ctypes.c_float), ctypes.c_void_p,
                                            ctypes.c_void_p, ctypes.c_void_p]

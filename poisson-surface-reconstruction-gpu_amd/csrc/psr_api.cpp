@@ -306,6 +306,23 @@ int psr_mesh_normals(psr_ctx* ctx, const psr_mesh* mesh, float* normals) {
     return PSR_OK;
 }
 
+int psr_mesh_normals_static(const psr_mesh* mesh, float* normals, void* hip_stream) {
+    if (!mesh || (mesh->n_vertices && (!normals || !mesh->vertices)) || (mesh->n_triangles && !mesh->triangles) ||
+        mesh->n_vertices > uint32_t(INT32_MAX) || mesh->n_triangles > uint32_t(INT32_MAX / 3))
+        return PSR_ERR_INVALID;
+    std::string err;
+    try {
+        return psr::mesh_normals_static(mesh->vertices, int(mesh->n_vertices), mesh->triangles,
+                                        int(mesh->n_triangles), normals, static_cast<hipStream_t>(hip_stream), err);
+    } catch (const psr::HipError&) {
+        return PSR_ERR_HIP;
+    } catch (const std::exception&) {
+        return PSR_ERR_INTERNAL;
+    }
+}
+
+uint32_t psr_mesh_normals_static_max_triangles(void) { return uint32_t(psr::mesh_normals_static_max_triangles()); }
+
 int psr_estimate_normals(psr_ctx* ctx, const float* xyz, uint32_t count, uint32_t stride_bytes, int k,
                          const float viewpoint[3], float* normals, float* curvature, int32_t* knn) {
     if (!ctx || (count && (!xyz || !normals)) || count > uint32_t(INT32_MAX) / 16 ||

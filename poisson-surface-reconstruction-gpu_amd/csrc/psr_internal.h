@@ -314,6 +314,10 @@ float elapsed(hipEvent_t a, hipEvent_t b);
 void weld_mesh(Ctx& c);
 // Per-vertex normals of a device mesh into out (3 floats per vertex); returns once written.
 void mesh_normals(Ctx& c, const float* V, int nv, const int* T, int nt, float* out);
+// The same with every scratch buffer in a file-scope device array and no
+// context: allocates nothing; returns a status (err: its message).
+int mesh_normals_static(const float* V, int nv, const int* T, int nt, float* out, hipStream_t s, std::string& err);
+size_t mesh_normals_static_max_triangles();
 // Normals (and curvature, k-nearest-neighbour indices when non-null) of an
 // unoriented point cloud; returns once written.
 void estimate_normals(Ctx& c, const float* xyz, int n, unsigned stride, int k, float3 vp, float* normals,

@@ -17,7 +17,9 @@
 
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <stdexcept>
+#include <string>
 #include <vector>
 
 namespace psr {
@@ -153,6 +155,100 @@ void mesh_normals(Ctx& c, const float* V, int nv, const int* T, int nt, float* o
     }
     c.wait();
     PSR_HIP_CHECK(hipGetLastError());
+}
+
+// ---- per-vertex normals with every scratch buffer in one file-scope device
+// array (psr_mesh_normals_static): for callers that cannot allocate.  The
+// array holds the face normals, the four sort buffers, the sort's histograms,
+// the scan state and an error word for meshes of up to kStaticTriangles
+// triangles; the call clears what must start at zero.  Every call on a device
+// shares that device's array: calls that overlap on one device race.
+namespace {
+
+constexpr size_t kStaticTriangles = size_t(1) << 18;
+constexpr size_t kStaticVertices = size_t(1) << 24;
+constexpr size_t kStaticCorners = 3 * kStaticTriangles;
+constexpr size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+constexpr size_t kStaticSortTiles = (kStaticCorners + kSortTileKeys - 1) / kSortTileKeys;
+constexpr size_t kStaticHistInts = 256 * kStaticSortTiles + 64;      // sort_scratch_ints(kStaticCorners)
+constexpr size_t kStaticScanTiles = (256 * kStaticSortTiles + 2047) / 2048;  // tiles of the histogram scan
+constexpr size_t kOffFaces = 0;
+constexpr size_t kOffKeys = kOffFaces + align256(sizeof(float3) * kStaticTriangles);
+constexpr size_t kOffVals = kOffKeys + align256(4 * kStaticCorners);
+constexpr size_t kOffKeysTmp = kOffVals + align256(4 * kStaticCorners);
+constexpr size_t kOffValsTmp = kOffKeysTmp + align256(4 * kStaticCorners);
+constexpr size_t kOffHist = kOffValsTmp + align256(4 * kStaticCorners);
+constexpr size_t kOffScanStatus = kOffHist + align256(4 * kStaticHistInts);
+constexpr size_t kOffScanCounters = kOffScanStatus + align256(8 * kStaticScanTiles);
+constexpr size_t kOffError = kOffScanCounters + align256(4 * kScanRing);
+constexpr size_t kStaticBytes = kOffError + 256;
+
+static __device__ __align__(256) unsigned char scratch[kStaticBytes];
+
+// The device address of `scratch` on each device, looked up once per device
+// with that device current (the array's name is only a host-side stand-in).
+constexpr int kMaxDevices = 64;
+std::mutex& static_mutex() {
+    static std::mutex m;
+    return m;
+}
+void* g_static_scratch[kMaxDevices] = {};
+
+void* static_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) throw std::invalid_argument("device index out of range");
+    std::lock_guard<std::mutex> lk(static_mutex());
+    if (!g_static_scratch[device]) {
+        void* p = nullptr;
+        PSR_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_static_scratch[device] = p;
+    }
+    return g_static_scratch[device];
+}
+
+}  // namespace
+
+size_t mesh_normals_static_max_triangles() { return kStaticTriangles; }
+
+int mesh_normals_static(const float* V, int nv, const int* T, int nt, float* out, hipStream_t s, std::string& err) {
+    if (size_t(nt) > kStaticTriangles) return err = "more triangles than the static scratch holds", PSR_ERR_CAPACITY;
+    if (size_t(nv) >= kStaticVertices) return err = "more vertices than the static scratch holds", PSR_ERR_CAPACITY;
+    if (nv <= 0) return PSR_OK;
+    int device = 0;
+    PSR_HIP_CHECK(hipGetDevice(&device));
+    char* base = static_cast<char*>(static_scratch(device));
+    const size_t n = size_t(3) * std::max(nt, 0);
+    if (n > 0 && (sort_scratch_ints(n) > kStaticHistInts || scan_tiles(sort_scan_elems(n)) > kStaticScanTiles))
+        return err = "static scratch layout does not fit the sort", PSR_ERR_INTERNAL;
+    float3* fn = reinterpret_cast<float3*>(base + kOffFaces);
+    unsigned* k = reinterpret_cast<unsigned*>(base + kOffKeys);
+    unsigned* v = reinterpret_cast<unsigned*>(base + kOffVals);
+    unsigned* kt = reinterpret_cast<unsigned*>(base + kOffKeysTmp);
+    unsigned* vt = reinterpret_cast<unsigned*>(base + kOffValsTmp);
+    ScanState st;
+    st.status = reinterpret_cast<unsigned long long*>(base + kOffScanStatus);
+    st.counters = reinterpret_cast<unsigned*>(base + kOffScanCounters);
+    st.capacity = kStaticScanTiles;
+    st.epoch = 0;
+    int* errWord = reinterpret_cast<int*>(base + kOffError);
+    st.err = errWord;
+    // the array keeps the last call's contents: the scan's status words, tile
+    // counters and the error word start at zero (a fill kernel, no memset node)
+    zero_spans({{st.status, 2 * kStaticScanTiles}, {st.counters, size_t(kScanRing)}, {errWord, 1}}, s);
+    const unsigned grid = std::max(div_up(size_t(std::max(nt, 1)), kThreads), div_up(3 * size_t(nv), kThreads * 8));
+    normals_faces<<<grid, kThreads, 0, s>>>(V, nv, T, nt, fn, k, v, out);
+    if (n > 0) {
+        int bits = 1;
+        while (bits < 32 && (unsigned(nv) >> bits) != 0) ++bits;  // keys 0..nv
+        radix_sort_pairs(k, v, kt, vt, int(n), bits, reinterpret_cast<int*>(base + kOffHist), false, st, SortGather{},
+                         s);
+        normals_vertices<<<div_up(n, kThreads), kThreads, 0, s>>>(k, v, n, nv, fn, out);
+    }
+    PSR_HIP_CHECK(hipGetLastError());
+    int failed = 0;
+    PSR_HIP_CHECK(hipMemcpyAsync(&failed, errWord, sizeof(int), hipMemcpyDeviceToHost, s));
+    PSR_HIP_CHECK(hipStreamSynchronize(s));
+    if (failed) return err = "scan look-back timed out", PSR_ERR_INTERNAL;
+    return PSR_OK;
 }
 
 void weld_mesh(Ctx& c) {
