@@ -731,6 +731,88 @@ static double true_relres(const Csr& A, const std::vector<double>& b, const std:
     return den > 0 ? std::sqrt(num / den) : std::sqrt(num);
 }
 
+// Whether the depth's operator, as the 1e-6 drop leaves it
+// (LaplacianSolver.cu:62), is indefinite: its entries depend only on the
+// neighbour offset (the inner products are shift-invariant), so it is a
+// constant 27-point stencil whose symbol sum_o L_o cos(theta . o) is sampled
+// over [0, pi]^3; a negative value means eigenvalues of both signs (at depth
+// 9 only the diagonal and the six face entries survive the drop, and
+// 4.99 + 2.18 (cos + cos + cos) changes sign).
+static bool stencil_indefinite(const Tables& t, int depth) {
+    if (depth < 2) return false;
+    const int f = (1 << depth) - 1 + (1 << (depth - 1));  // an interior 1-D function index
+    double L[3][3][3];
+    for (int a = -1; a <= 1; ++a)
+        for (int b = -1; b <= 1; ++b)
+            for (int c = -1; c <= 1; ++c) {
+                const int o[3] = {a, b, c};
+                size_t s[3];
+                for (int k = 0; k < 3; ++k) s[k] = size_t(f) * t.res + size_t(f + o[k]);
+                double v = t.dot[s[0]] * t.dot[s[1]] * t.dot[s[2]] * (t.d2dot[s[0]] + t.d2dot[s[1]] + t.d2dot[s[2]]);
+                L[a + 1][b + 1][c + 1] = std::fabs(v) > double(float(1e-6)) ? v : 0.0;
+            }
+    const int S = 32;
+    double lo = 1e300;
+    for (int i = 0; i <= S; ++i)
+        for (int j = 0; j <= S; ++j)
+            for (int k = 0; k <= S; ++k) {
+                const double th[3] = {M_PI * i / S, M_PI * j / S, M_PI * k / S};
+                double sym = 0;
+                for (int a = -1; a <= 1; ++a)
+                    for (int b = -1; b <= 1; ++b)
+                        for (int c = -1; c <= 1; ++c)
+                            sym += L[a + 1][b + 1][c + 1] * std::cos(th[0] * a) * std::cos(th[1] * b) *
+                                   std::cos(th[2] * c);
+                lo = std::min(lo, sym);
+            }
+    return lo < 0;
+}
+
+// MINRES (Paige and Saunders), FP64, x0 = 0, for the symmetric indefinite
+// depths: Lanczos vectors, Givens rotations on the tridiagonal, stopping when
+// the estimated residual norm falls to rel_tol ||b||.
+static int minres(const Csr& A, const std::vector<double>& b, std::vector<double>& x, double rel_tol, int max_iter,
+                  double& recres) {
+    const int n = A.n;
+    std::vector<double> r1(b), r2(b), y(b), v(n), w(n, 0), w1(n, 0), w2(n, 0);
+    x.assign(n, 0.0);
+    const double beta1 = std::sqrt(dotp(b, b));
+    double beta = beta1, oldb = 0, dbar = 0, epsln = 0, phibar = beta1, cs = -1, sn = 0;
+    int itn = 0;
+    while (phibar > rel_tol * beta1 && itn < max_iter && beta > 0) {
+        ++itn;
+        const double s = 1.0 / beta;
+        for (int i = 0; i < n; ++i) v[i] = s * y[i];
+        spmv(A, A.val, v, y);
+        if (itn >= 2)
+            for (int i = 0; i < n; ++i) y[i] -= (beta / oldb) * r1[i];
+        const double alfa = dotp(v, y);
+        for (int i = 0; i < n; ++i) y[i] -= (alfa / beta) * r2[i];
+        r1.swap(r2);
+        r2 = y;
+        oldb = beta;
+        beta = std::sqrt(dotp(y, y));
+        const double oldeps = epsln;
+        const double delta = cs * dbar + sn * alfa;
+        const double gbar = sn * dbar - cs * alfa;
+        epsln = sn * beta;
+        dbar = -cs * beta;
+        const double gamma = std::max(std::sqrt(gbar * gbar + beta * beta), 1e-300);
+        cs = gbar / gamma;
+        sn = beta / gamma;
+        const double phi = cs * phibar;
+        phibar = sn * phibar;
+        for (int i = 0; i < n; ++i) {
+            w1[i] = w2[i];
+            w2[i] = w[i];
+            w[i] = (v[i] - oldeps * w1[i] - delta * w2[i]) / gamma;
+            x[i] += phi * w[i];
+        }
+    }
+    recres = phibar;
+    return itn;
+}
+
 template <class Real>
 static void solve_all(Result& r, const Tables& t) {
     int D = r.D;
@@ -747,7 +829,9 @@ static void solve_all(Result& r, const Tables& t) {
         double rec = 0;
         int it;
         if (sizeof(Real) == 4)
-            it = cg<float>(A, b, x, 1e-5, 0, 10000, rec);
+            it = cg<float>(A, b, x, 1e-5, 0, 10000, rec);  // the reference's rule, also where it cannot converge
+        else if (stencil_indefinite(t, d))
+            it = minres(A, b, x, 1e-12, 200000, rec);
         else
             it = cg<double>(A, b, x, 0, 1e-12, 100000, rec);
         r.cgIters[d] = it;
@@ -1618,6 +1702,26 @@ double orc_inner(int a, int b, int da, int db) { return inner(a, b, da, db); }
 double orc_basis(int idx, double x) { return basis_at<double>(idx, x); }
 int orc_lut(int which, int s, int i) { return which ? kLut.child[s][i] : kLut.parent[s][i]; }
 long long orc_mc_word(int cube) { return (long long)kOracleMcTable[cube]; }
+// Whether depth `depth` of a depth-D octree gets MINRES (its dropped stencil is indefinite).
+int orc_depth_indefinite(int D, int depth) {
+    Tables t;
+    build_tables(t, D);
+    return stencil_indefinite(t, depth) ? 1 : 0;
+}
+// MINRES on a CSR matrix (the tests' check against a dense solve); returns the iterations.
+int orc_minres(int n, const int* rp, const int* col, const double* val, const double* b, double* x, double rel_tol,
+               int max_iter) {
+    Csr A;
+    A.n = n;
+    A.rp.assign(rp, rp + n + 1);
+    A.col.assign(col, col + rp[n]);
+    A.val.assign(val, val + rp[n]);
+    std::vector<double> bb(b, b + n), xx;
+    double rec = 0;
+    const int it = minres(A, bb, xx, rel_tol, max_iter, rec);
+    std::copy(xx.begin(), xx.end(), x);
+    return it;
+}
 // Entry i of a small quirk-defining table: 0 childrenVertexKind[8], 1 edgeVertex[12][2]
 // (flattened), 2 parentFaceKind[8][6] (flattened), 3 faceEdges[6][4] (flattened).
 int orc_small_table(int which, int i) {
