@@ -16,6 +16,7 @@ sample on the host cores.
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -39,8 +40,8 @@ def parse():
     ap.add_argument("--workload", default="torus", choices=sorted(WORKLOADS))
     ap.add_argument("--precision", default="both", choices=["both", "fp32", "mixed"],
                     help="both: time the FP32 form and the mixed-precision form; the line's value is the latter")
-    ap.add_argument("--cpu-sample", type=int, default=3, help="reconstructions in the CPU-baseline sample (0: skip)")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="OpenMP threads of the CPU baseline (0: all, max 16)")
+    ap.add_argument("--cpu-sample", type=int, default=5, help="timed reconstructions in the CPU-baseline median (0: skip)")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="OpenMP threads of the CPU baseline (0: all usable, capped by OMP_NUM_THREADS)")
     ap.add_argument("--batch-reps", type=int, default=3,
                     help="timed repetitions of the 64-cloud batch, sequential and batched (0: skip)")
     ap.add_argument("--batch-pass", type=int, default=0,
@@ -78,20 +79,43 @@ def measured_traffic(workload, precision, kernel):
     return None
 
 
+def _cpu_model():
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for ln in out.splitlines():
+            if ln.startswith("Model name:"):
+                return ln.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return "unknown"
+
+
 def cpu_baseline(xyz, nrm, depth, reps, threads):
-    """The oracle's FP32 restatement (same stages and CG rule) on host cores."""
+    """The FP32 port of the reference algorithm (same stages, quirks and CG
+    rule: absolute 1e-5 on the recursive residual, at most 10000 iterations),
+    compiled here with -march=native and OpenMP: one full-size warm-up, then
+    the median of `reps` timed reconstructions on all usable cores and one on
+    a single thread."""
+    import statistics
+    import tempfile
     import __graft_entry__ as ge
     orc = ge.load_oracle()
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
     if threads <= 0:
-        threads = min(16, os.cpu_count() or 1)
-    orc.Result(xyz[:20000], nrm[:20000], min(depth, 6), precision=0, threads=threads)  # warm the library
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        orc.Result(xyz, nrm, depth, precision=0, threads=threads)
-    dt = time.perf_counter() - t0
-    return {"value": reps / dt, "unit": "reconstructions/sec", "cores": threads, "kind": "port",
-            "sample": "%d whole FP32 reconstructions of the same input (%.1f s); octree, field and CG use "
-                      "OpenMP, geometry and marching cubes run on one thread" % (reps, dt)}
+        threads = min(usable, int(os.environ.get("OMP_NUM_THREADS", usable) or usable))
+    with tempfile.TemporaryDirectory() as tmp:
+        L = orc.native_port(tmp)
+        orc.time_port(L, xyz, nrm, depth, threads)  # warm-up at full size
+        ms = [orc.time_port(L, xyz, nrm, depth, threads) for _ in range(reps)]
+        one = orc.time_port(L, xyz, nrm, depth, 1)
+    med = statistics.median(ms)
+    return {"value": 1000.0 / med, "unit": "reconstructions/sec", "cores": threads, "kind": "port",
+            "one_thread_value": 1000.0 / one, "nproc": os.cpu_count(), "cpu_model": _cpu_model(),
+            "build": "g++ -O3 -march=native -fopenmp -ffp-contract=off",
+            "sample": "median of %d whole FP32 reconstructions of the same input after a full-size warm-up "
+                      "(%s ms), each stage timed from normalisation to the end of subdivision, basis tables "
+                      "excluded; one_thread_value is one run on a single thread (%.0f ms)"
+                      % (reps, ", ".join("%.0f" % m for m in ms), one)}
 
 
 def cg_kernel_name(st, precision):

@@ -22,22 +22,47 @@ def build():
     subprocess.check_call(["make", "-s", "-C", _HERE])
 
 
+def _declare(L):
+    L.orc_run.restype = ctypes.c_void_p
+    L.orc_run.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                          ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.orc_free.argtypes = [ctypes.c_void_p]
+    L.orc_scalar.restype = ctypes.c_double
+    L.orc_scalar.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
+    return L
+
+
+def native_port(out_dir):
+    """The FP32 port compiled for the host it runs on (-march=native) into
+    out_dir: the timed CPU baseline, built where it is timed."""
+    so = os.path.join(out_dir, "liboracle_native.so")
+    subprocess.check_call(["make", "-s", "-B", "-C", _HERE, "native", "NATIVE_SO=" + so])
+    return _declare(ctypes.CDLL(so))
+
+
+def time_port(L, xyz, nrm, depth, threads, precision=0):
+    """Milliseconds of one whole reconstruction by library L, summed over its
+    stage timers (normalisation and octree through marching cubes and
+    subdivision; the one-time basis tables are excluded)."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float32)
+    nrm = np.ascontiguousarray(nrm, dtype=np.float32)
+    h = L.orc_run(xyz.ctypes.data, nrm.ctypes.data, xyz.shape[0], depth, STAGE_ALL, precision, threads)
+    try:
+        return sum(L.orc_scalar(h, k.encode()) for k in ("t_octree", "t_field", "t_solve", "t_mesh"))
+    finally:
+        L.orc_free(h)
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             build()
-        L = ctypes.CDLL(_LIB_PATH)
-        L.orc_run.restype = ctypes.c_void_p
-        L.orc_run.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
-                              ctypes.c_int, ctypes.c_int, ctypes.c_int]
-        L.orc_free.argtypes = [ctypes.c_void_p]
+        L = _declare(ctypes.CDLL(_LIB_PATH))
         L.orc_count.restype = ctypes.c_longlong
         L.orc_count.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
         L.orc_copy.restype = ctypes.c_int
         L.orc_copy.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p, ctypes.c_longlong]
-        L.orc_scalar.restype = ctypes.c_double
-        L.orc_scalar.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
         L.orc_inner.restype = ctypes.c_double
         L.orc_inner.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.orc_basis.restype = ctypes.c_double

@@ -883,32 +883,68 @@ static inline double sqdist(const float* a, const float* b) {
     return double(dx * dx + dy * dy + dz * dz);
 }
 
+// The largest value stored to *p.  The pointer loops below run their index
+// upward, so in a sequential run the last store is the largest: taking the
+// maximum gives that result under any thread schedule.
+static inline void store_max(int* p, int v) {
+    int cur = __atomic_load_n(p, __ATOMIC_RELAXED);
+    while (cur < v && !__atomic_compare_exchange_n(p, &cur, v, true, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
+    }
+}
+
+// Ownership runs per node in parallel and records a bit per corner / edge /
+// face; the owned elements are then appended in (node, slot) order, the order
+// of the reference's stream compaction, so the arrays do not depend on the
+// thread count.
 static void geometry(Result& r) {
     int M = r.M, D = r.D;
     std::vector<OctNode>& NA = r.nodes;
     auto ctr = [&](int i) { return &r.center[3 * size_t(i)]; };
     static const float kZero[3] = {0, 0, 0};
-    // vertices
-    r.verts.clear();
-    for (int idx = 0; idx < M; ++idx) {
-        int depth = r.depthOf[idx];
-        float half = 1.0f / float(1 << (depth + 1)), W = 1.0f / float(1 << depth), W2 = W * W;
-        const float* nc = NA[idx].neighs[13] != -1 ? ctr(NA[idx].neighs[13]) : kZero;
-        for (int i = 0; i < 8; ++i) {
-            float vp[3] = {nc[0] + (2 * (i & 1) - 1) * half, nc[1] + (2 * ((i & 2) >> 1) - 1) * half,
-                           nc[2] + (2 * ((i & 4) >> 2) - 1) * half};
-            int ownKey = 0x7fffffff, own = -1;
-            for (int j = 0; j < 27; ++j) {
-                int nb = NA[idx].neighs[j];
-                if (nb != -1 && sqdist(vp, ctr(nb)) < double(W2) && ownKey > NA[nb].key) {
-                    ownKey = NA[nb].key;
-                    own = nb;
-                }
+    auto centre_of = [&](int idx) -> const float* { return NA[idx].neighs[13] != -1 ? ctr(NA[idx].neighs[13]) : kZero; };
+    auto corner = [&](int idx, int i, float vp[3]) {
+        float half = 1.0f / float(1 << (r.depthOf[idx] + 1));
+        const float* nc = centre_of(idx);
+        vp[0] = nc[0] + (2 * (i & 1) - 1) * half;
+        vp[1] = nc[1] + (2 * ((i & 2) >> 1) - 1) * half;
+        vp[2] = nc[2] + (2 * ((i & 4) >> 2) - 1) * half;
+    };
+    // the node among idx's 27 neighbours closer than W to p with the smallest key
+    auto owner_near = [&](int idx, const float* p, float W2) {
+        int ownKey = 0x7fffffff, own = -1;
+        for (int j = 0; j < 27; ++j) {
+            int nb = NA[idx].neighs[j];
+            if (nb != -1 && sqdist(p, ctr(nb)) < double(W2) && ownKey > NA[nb].key) {
+                ownKey = NA[nb].key;
+                own = nb;
             }
-            if (own == idx) r.verts.push_back({{vp[0], vp[1], vp[2]}, idx, i, depth});
         }
+        return own;
+    };
+    std::vector<uint16_t> mask(size_t(M), 0);
+    // vertices
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int idx = 0; idx < M; ++idx) {
+        float W = 1.0f / float(1 << r.depthOf[idx]);
+        uint16_t m = 0;
+        for (int i = 0; i < 8; ++i) {
+            float vp[3];
+            corner(idx, i, vp);
+            if (owner_near(idx, vp, W * W) == idx) m |= uint16_t(1u << i);
+        }
+        mask[idx] = m;
     }
-    for (size_t v = 0; v < r.verts.size(); ++v) {
+    r.verts.clear();
+    for (int idx = 0; idx < M; ++idx)
+        for (int i = 0; i < 8; ++i)
+            if (mask[idx] >> i & 1) {
+                float vp[3];
+                corner(idx, i, vp);
+                r.verts.push_back({{vp[0], vp[1], vp[2]}, idx, i, r.depthOf[idx]});
+            }
+    const int nv = int(r.verts.size());
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int v = 0; v < nv; ++v) {
         const Vtx& vx = r.verts[v];
         int owner = vx.owner, depth = r.depthOf[owner];
         float W = 1.0f / float(1 << depth), W2 = W * W;
@@ -920,11 +956,10 @@ static void geometry(Result& r) {
             if (c[0] - vx.pos[0] < 0) index |= 1;
             if (c[2] - vx.pos[2] < 0) index |= 4;
             if (c[1] - vx.pos[1] < 0) index += (index & 1) ? 1 : 3;
-            NA[nb].vertices[index] = int(v) + 1;
+            store_max(&NA[nb].vertices[index], v + 1);
         }
     }
     // edges (depth D only)
-    r.edges.clear();
     float halfD = 1.0f / float(1 << (D + 1)), WD = 1.0f / float(1 << D), WD2 = WD * WD;
     auto edge_center = [&](const float* nc, int kind, float half, float out[3]) {
         int orient = kind >> 2, off[2] = {kind & 1, (kind & 2) >> 1}, dim = 0;
@@ -934,27 +969,27 @@ static void geometry(Result& r) {
             out[a] = nc[a] + m * half;
         }
     };
-    for (int o = r.base[D]; o < r.base[D] + r.count[D]; ++o) {
-        const float* nc = NA[o].neighs[13] != -1 ? ctr(NA[o].neighs[13]) : kZero;
+    const int oD = r.base[D], nD = r.count[D];
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int o = oD; o < oD + nD; ++o) {
+        uint16_t m = 0;
         for (int e = 0; e < 12; ++e) {
             float ec[3];
-            edge_center(nc, e, halfD, ec);
-            int ownKey = 0x7fffffff, own = -1;
-            for (int j = 0; j < 27; ++j) {
-                int nb = NA[o].neighs[j];
-                if (nb != -1 && sqdist(ec, ctr(nb)) < double(WD2) && ownKey > NA[nb].key) {
-                    ownKey = NA[nb].key;
-                    own = nb;
-                }
-            }
-            if (own == o) r.edges.push_back({e, o});
+            edge_center(centre_of(o), e, halfD, ec);
+            if (owner_near(o, ec, WD2) == o) m |= uint16_t(1u << e);
         }
+        mask[o] = m;
     }
-    for (size_t k = 0; k < r.edges.size(); ++k) {
+    r.edges.clear();
+    for (int o = oD; o < oD + nD; ++o)
+        for (int e = 0; e < 12; ++e)
+            if (mask[o] >> e & 1) r.edges.push_back({e, o});
+    const int ne = int(r.edges.size());
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int k = 0; k < ne; ++k) {
         int owner = r.edges[k].owner, kind = r.edges[k].kind, orient = kind >> 2;
-        const float* nc = NA[owner].neighs[13] != -1 ? ctr(NA[owner].neighs[13]) : kZero;
         float ec[3];
-        edge_center(nc, kind, halfD, ec);
+        edge_center(centre_of(owner), kind, halfD, ec);
         for (int i = 0; i < 27; ++i) {
             int nb = NA[owner].neighs[i];
             if (nb == -1 || !(sqdist(ec, ctr(nb)) < double(WD2))) continue;
@@ -964,46 +999,52 @@ static void geometry(Result& r) {
                     if (ctr(nb)[a] - ec[a] < 0) index |= (1 << dim);
                     ++dim;
                 }
-            NA[nb].edges[index] = int(k) + 1;
+            store_max(&NA[nb].edges[index], k + 1);
         }
     }
     // faces
+    auto face_center = [&](int idx, int i, float fc[3]) {
+        float half = 1.0f / float(1 << (r.depthOf[idx] + 1));
+        const float* nc = centre_of(idx);
+        fc[0] = nc[0];
+        fc[1] = nc[1];
+        fc[2] = nc[2];
+        fc[i >> 1] += (2 * (i & 1) - 1) * half;
+    };
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int idx = 0; idx < M; ++idx) {
+        float W = 1.0f / float(1 << r.depthOf[idx]);
+        uint16_t m = 0;
+        for (int i = 0; i < 6; ++i) {
+            float fc[3];
+            face_center(idx, i, fc);
+            if (owner_near(idx, fc, W * W) == idx) m |= uint16_t(1u << i);
+        }
+        mask[idx] = m;
+    }
     r.faces.clear();
     for (int idx = 0; idx < M; ++idx) {
-        int depth = r.depthOf[idx];
-        float half = 1.0f / float(1 << (depth + 1)), W = 1.0f / float(1 << depth), W2 = W * W;
-        const float* nc = NA[idx].neighs[13] != -1 ? ctr(NA[idx].neighs[13]) : kZero;
-        int parent = NA[idx].parent, son = (NA[idx].key >> (3 * (D - depth))) & 7;
-        for (int i = 0; i < 6; ++i) {
-            float fc[3] = {nc[0], nc[1], nc[2]};
-            fc[i >> 1] += (2 * (i & 1) - 1) * half;
-            int ownKey = 0x7fffffff, own = -1;
-            for (int j = 0; j < 27; ++j) {
-                int nb = NA[idx].neighs[j];
-                if (nb != -1 && sqdist(fc, ctr(nb)) < double(W2) && ownKey > NA[nb].key) {
-                    ownKey = NA[nb].key;
-                    own = nb;
-                }
-            }
-            if (own == idx) {
+        if (!mask[idx]) continue;
+        int parent = NA[idx].parent, son = (NA[idx].key >> (3 * (D - r.depthOf[idx]))) & 7;
+        for (int i = 0; i < 6; ++i)
+            if (mask[idx] >> i & 1) {
                 int hp = (parent == -1) ? -1 : (kParentFaceKind[son][i] != -1 ? 1 : -1);
                 r.faces.push_back({i, idx, hp});
             }
-        }
     }
-    for (size_t k = 0; k < r.faces.size(); ++k) {
+    const int nf = int(r.faces.size());
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int k = 0; k < nf; ++k) {
         int owner = r.faces[k].owner, kind = r.faces[k].kind, orient = kind >> 1;
-        int depth = r.depthOf[owner];
-        float half = 1.0f / float(1 << (depth + 1)), W = 1.0f / float(1 << depth), W2 = W * W;
-        const float* nc = NA[owner].neighs[13] != -1 ? ctr(NA[owner].neighs[13]) : kZero;
-        float fc[3] = {nc[0], nc[1], nc[2]};
-        fc[orient] += (2 * (kind & 1) - 1) * half;
+        float W = 1.0f / float(1 << r.depthOf[owner]), W2 = W * W;
+        float fc[3];
+        face_center(owner, kind, fc);
         for (int i = 0; i < 27; ++i) {
             int nb = NA[owner].neighs[i];
             if (nb == -1 || !(sqdist(fc, ctr(nb)) < double(W2))) continue;
             int index = orient << 1;
             if (ctr(nb)[orient] - fc[orient] < 0) ++index;
-            NA[nb].faces[index] = int(k) + 1;
+            store_max(&NA[nb].faces[index], k + 1);
         }
     }
 }
@@ -1105,6 +1146,8 @@ static void mc_on_set(Result& r, SubSet& S, bool coarse) {
     int firstDepth = coarse ? S.depth[0] : 0;
     for (int depth = firstDepth; depth <= D; ++depth) {
         int sh = 3 * (D - depth);
+        // a level reads only the level above it
+#pragma omp parallel for schedule(static)
         for (int o = S.levelStart[depth]; o < S.levelStart[depth] + S.levelCount[depth]; ++o) {
             VNode& n = S.nodes[o];
             int s = (n.key >> sh) & 7, p = n.parent;
@@ -1123,29 +1166,49 @@ static void mc_on_set(Result& r, SubSet& S, bool coarse) {
     static const float kZero[3] = {0, 0, 0};
     auto vcenter_or_zero = [&](int nb) -> const float* { return (nb != -1 && nb >= NAS) ? vctr(nb) : kZero; };
     auto any_center = [&](int nb) -> const float* { return nb < NAS ? &r.center[3 * size_t(nb)] : vctr(nb); };
-    // vertex owners (initSubdivideVertexOwner, :698-763)
-    std::vector<Vtx> V;
-    for (int o = dStart; o < dStart + dCount; ++o) {
-        VNode& n = S.nodes[o];
-        const float* nc = vcenter_or_zero(n.neighs[13]);
-        for (int i = 0; i < 8; ++i) {
-            float vp[3] = {nc[0] + (2 * (i & 1) - 1) * half, nc[1] + (2 * ((i & 2) >> 1) - 1) * half,
-                           nc[2] + (2 * ((i & 4) >> 2) - 1) * half};
-            int ownKey = 0x7fffffff, own = -1;
-            for (int j = 0; j < 27; ++j) {
-                int nb = n.neighs[j];
-                if (nb != -1 && sqdist(vp, vcenter_or_zero(nb)) < double(W2)) {
-                    if (nb < NAS) continue;
-                    int k = S.nodes[nb - NAS].key;
-                    if (ownKey > k) {
-                        ownKey = k;
-                        own = nb;
-                    }
+    // the virtual node among o's 27 neighbours closer than W to p with the smallest key
+    auto owner_near = [&](int o, const float* p) {
+        int ownKey = 0x7fffffff, own = -1;
+        for (int j = 0; j < 27; ++j) {
+            int nb = S.nodes[o].neighs[j];
+            if (nb != -1 && sqdist(p, vcenter_or_zero(nb)) < double(W2)) {
+                if (nb < NAS) continue;
+                int k = S.nodes[nb - NAS].key;
+                if (ownKey > k) {
+                    ownKey = k;
+                    own = nb;
                 }
             }
-            if (own == NAS + o) V.push_back({{vp[0], vp[1], vp[2]}, own, i, D});
         }
+        return own;
+    };
+    auto corner = [&](int o, int i, float vp[3]) {
+        const float* nc = vcenter_or_zero(S.nodes[o].neighs[13]);
+        vp[0] = nc[0] + (2 * (i & 1) - 1) * half;
+        vp[1] = nc[1] + (2 * ((i & 2) >> 1) - 1) * half;
+        vp[2] = nc[2] + (2 * ((i & 4) >> 2) - 1) * half;
+    };
+    // ownership per node in parallel, appended in (node, slot) order as in geometry()
+    std::vector<uint16_t> mask(size_t(dCount), 0);
+    // vertex owners (initSubdivideVertexOwner, :698-763)
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int o = dStart; o < dStart + dCount; ++o) {
+        uint16_t m = 0;
+        for (int i = 0; i < 8; ++i) {
+            float vp[3];
+            corner(o, i, vp);
+            if (owner_near(o, vp) == NAS + o) m |= uint16_t(1u << i);
+        }
+        mask[o - dStart] = m;
     }
+    std::vector<Vtx> V;
+    for (int o = dStart; o < dStart + dCount; ++o)
+        for (int i = 0; i < 8; ++i)
+            if (mask[o - dStart] >> i & 1) {
+                float vp[3];
+                corner(o, i, vp);
+                V.push_back({{vp[0], vp[1], vp[2]}, NAS + o, i, D});
+            }
     const int setId = int(r.subSets.size() / 4) - 1;
     const size_t subV0 = r.subV.size();
     for (const Vtx& vx : V) {
@@ -1154,7 +1217,10 @@ static void mc_on_set(Result& r, SubSet& S, bool coarse) {
         r.subV.insert(r.subV.end(), rec, rec + 7);
     }
     if (V.empty()) return;
-    for (size_t v = 0; v < V.size(); ++v) {
+    // pointers land on depth-D virtual nodes only, whose slots start at 0
+    const int nv = int(V.size());
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int v = 0; v < nv; ++v) {
         VNode& ow = S.nodes[V[v].owner - NAS];
         for (int i = 0; i < 27; ++i) {
             int nb = ow.neighs[i];
@@ -1165,11 +1231,10 @@ static void mc_on_set(Result& r, SubSet& S, bool coarse) {
             if (c[0] - V[v].pos[0] < 0) index |= 1;
             if (c[2] - V[v].pos[2] < 0) index |= 4;
             if (c[1] - V[v].pos[1] < 0) index += (index & 1) ? 1 : 3;
-            if (nb >= NAS) S.nodes[nb - NAS].vertices[index] = int(v) + 1;
+            if (nb >= NAS) store_max(&S.nodes[nb - NAS].vertices[index], v + 1);
         }
     }
     // edges (initSubdivideEdgeArray / maintainSubdivideEdgeNodePointer, :812-949)
-    std::vector<Edge> E;
     auto edge_center = [&](const float* nc, int kind, float out[3]) {
         int orient = kind >> 2, off[2] = {kind & 1, (kind & 2) >> 1}, dim = 0;
         for (int a = 0; a < 3; ++a) {
@@ -1178,33 +1243,29 @@ static void mc_on_set(Result& r, SubSet& S, bool coarse) {
             out[a] = nc[a] + m * half;
         }
     };
+#pragma omp parallel for schedule(dynamic, 1024)
     for (int o = dStart; o < dStart + dCount; ++o) {
-        VNode& n = S.nodes[o];
-        const float* nc = vcenter_or_zero(n.neighs[13]);
+        const float* nc = vcenter_or_zero(S.nodes[o].neighs[13]);
+        uint16_t m = 0;
         for (int e = 0; e < 12; ++e) {
             float ec[3];
             edge_center(nc, e, ec);
-            int ownKey = 0x7fffffff, own = -1;
-            for (int j = 0; j < 27; ++j) {
-                int nb = n.neighs[j];
-                if (nb != -1 && sqdist(ec, vcenter_or_zero(nb)) < double(W2)) {
-                    if (nb < NAS) continue;
-                    int k = S.nodes[nb - NAS].key;
-                    if (ownKey > k) {
-                        ownKey = k;
-                        own = nb;
-                    }
-                }
-            }
-            if (own == NAS + o) E.push_back({e, own});
+            if (owner_near(o, ec) == NAS + o) m |= uint16_t(1u << e);
         }
+        mask[o - dStart] = m;
     }
+    std::vector<Edge> E;
+    for (int o = dStart; o < dStart + dCount; ++o)
+        for (int e = 0; e < 12; ++e)
+            if (mask[o - dStart] >> e & 1) E.push_back({e, NAS + o});
     for (const Edge& ed : E) {
         const int rec[3] = {setId, S.nodes[ed.owner - NAS].key, ed.kind};
         r.subE.insert(r.subE.end(), rec, rec + 3);
     }
     if (E.empty()) return;
-    for (size_t k = 0; k < E.size(); ++k) {
+    const int ne = int(E.size());
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int k = 0; k < ne; ++k) {
         VNode& ow = S.nodes[E[k].owner - NAS];
         int orient = E[k].kind >> 2;
         const float* nc = any_center(ow.neighs[13]);
@@ -1221,12 +1282,13 @@ static void mc_on_set(Result& r, SubSet& S, bool coarse) {
                     if (c[a] - ec[a] < 0) index |= (1 << dim);
                     ++dim;
                 }
-            if (nb >= NAS) S.nodes[nb - NAS].edges[index] = int(k) + 1;
+            if (nb >= NAS) store_max(&S.nodes[nb - NAS].edges[index], k + 1);
         }
     }
     // vertex values (computeSubdivideVertexImplicitFunctionValue, :956-1038)
     std::vector<float> vv(V.size());
-    for (size_t v = 0; v < V.size(); ++v) {
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int v = 0; v < nv; ++v) {
         Real val = 0;
         int now = V[v].owner;
         if (now > 0) {
