@@ -95,6 +95,8 @@ typedef struct {
     int cg_row_form;      /* 1: one row per CG thread, stencil held in registers; 0: several rows per thread */
     double ms_cg_kernel_sum; /* ms_cg_kernel summed over the context's completed runs */
     uint32_t n_runs;         /* completed runs of the context */
+    uint64_t device_bytes;   /* device memory the run's buffers needed (the context's fixed tables and
+                                workspace excluded); the context's grow-only arena holds at least this */
 } psr_stats;
 
 /* Arrays that psr_export can copy to host memory (for parity checks). */

@@ -3,5 +3,5 @@
 The directory name is not a Python identifier; load it with ``load_package``
 from ``__graft_entry__`` (it registers the package as ``psr_gpu_amd``).
 """
-from . import binding, synth  # noqa: F401
+from . import binding, serve, synth  # noqa: F401
 from .binding import PoissonReconstruction  # noqa: F401

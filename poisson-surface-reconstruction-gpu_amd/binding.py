@@ -75,7 +75,8 @@ class Stats(ctypes.Structure):
                 ("ms_solve", ctypes.c_float), ("ms_mesh", ctypes.c_float), ("ms_total", ctypes.c_float),
                 ("ms_cg_kernel", ctypes.c_float), ("cg_workgroups", ctypes.c_uint32),
                 ("cg_nnz", ctypes.c_int * _MAXD), ("cg_outer", ctypes.c_int * _MAXD),
-                ("cg_row_form", ctypes.c_int), ("ms_cg_kernel_sum", ctypes.c_double), ("n_runs", ctypes.c_uint32)]
+                ("cg_row_form", ctypes.c_int), ("ms_cg_kernel_sum", ctypes.c_double), ("n_runs", ctypes.c_uint32),
+                ("device_bytes", ctypes.c_uint64)]
 
     def as_dict(self):
         d = self.depth
@@ -93,6 +94,7 @@ class Stats(ctypes.Structure):
             "cg_workgroups": self.cg_workgroups, "cg_nnz": list(self.cg_nnz)[:d + 1],
             "cg_outer": list(self.cg_outer)[:d + 1], "cg_row_form": self.cg_row_form,
             "ms_cg_kernel_sum": self.ms_cg_kernel_sum, "n_runs": self.n_runs, "q1_fired": self.q1_fired,
+            "device_bytes": self.device_bytes,
         }
 
 

@@ -42,6 +42,7 @@ void Arena::defer_frees(bool on) {
 
 void* Arena::get(const std::string& name, size_t bytes) {
     Block& b = blocks_[name];
+    b.need = std::max(b.need, bytes);
     if (b.bytes < bytes) {
         if (b.ptr) {
             free_block(b.ptr);
@@ -58,6 +59,7 @@ void* Arena::get(const std::string& name, size_t bytes) {
 
 void* Arena::get_keep(const std::string& name, size_t bytes, size_t keepBytes) {
     Block& b = blocks_[name];
+    b.need = std::max(b.need, bytes);
     if (b.bytes < bytes) {
         size_t want = bytes + bytes / 2 + 256;
         void* p = nullptr;
@@ -108,6 +110,7 @@ int call_begin(Ctx& c, const psr_points& pts, std::string& err) {
     if (!pts.xyz || !pts.normal) return err = "null point buffer", PSR_ERR_INVALID;
     if (pts.stride_bytes && (pts.stride_bytes < 12 || pts.stride_bytes % 4)) return err = "bad stride", PSR_ERR_INVALID;
     std::memset(&c.stats, 0, sizeof(c.stats));
+    c.arena.begin_run();
     c.clear_scan_error();
     c.mesh.valid = false;
     c.mesh.weldedNv = c.mesh.weldedNt = 0;
@@ -148,6 +151,7 @@ int call_end(Ctx& c, psr_mesh* out, bool timed, std::string& err) {
     }
     st.ms_cg_kernel_sum = c.cgMsSum;
     st.n_runs = ++c.runs;
+    st.device_bytes = c.arena.bytes_needed();
     out->vertices = c.mesh.vertices;
     out->triangles = c.mesh.triangles;
     out->n_vertices = uint32_t(c.mesh.nv);

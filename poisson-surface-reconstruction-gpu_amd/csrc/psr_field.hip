@@ -4,6 +4,7 @@
 #include <hip/hip_fp16.h>
 
 #include <algorithm>
+#include <cmath>
 
 namespace psr {
 
@@ -386,6 +387,47 @@ void build_tables(Ctx& c) {
     t.d2dot = c.buf<double>("table_d2dot", n);
     build_tables_kernel<<<div_up(n, 256), 256, 0, c.stream>>>(res, t.dot, t.ddot, t.d2dot);
     PSR_HIP_CHECK(hipGetLastError());
+    // Which depths' operators are indefinite once entries with |L| <= 1e-6 are
+    // dropped (LaplacianSolver.cu:62).  An entry depends only on the neighbour
+    // offset (the inner products are shift-invariant inside the domain), so a
+    // depth's operator is a constant 27-point stencil; its symbol
+    // sum_o L_o cos(theta . o), sampled over [0, pi]^3, takes both signs
+    // exactly when the operator has eigenvalues of both signs on large
+    // interiors.  At depth 9 only the diagonal and the six face entries
+    // survive and the symbol at (pi, pi, pi) is negative.
+    for (int d = 2; d <= D; ++d) {
+        const int f = (1 << d) - 1 + (1 << (d - 1));  // an interior function of depth d
+        double dt[2], d2[2];
+        for (int k = 0; k < 2; ++k) {
+            const size_t at = size_t(f) * res + size_t(f + k);
+            PSR_HIP_CHECK(hipMemcpyAsync(&dt[k], t.dot + at, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+            PSR_HIP_CHECK(hipMemcpyAsync(&d2[k], t.d2dot + at, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        }
+        PSR_HIP_CHECK(hipStreamSynchronize(c.stream));
+        double L[2][2][2];  // by |offset| per axis
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b)
+                for (int e = 0; e < 2; ++e) {
+                    const double v = dt[a] * dt[b] * dt[e] * (d2[a] + d2[b] + d2[e]);
+                    L[a][b][e] = std::fabs(v) > double(1e-6f) ? v : 0.0;
+                }
+        constexpr int S = 32;
+        double lo = 1e300;
+        for (int i = 0; i <= S; ++i)
+            for (int j = 0; j <= S; ++j)
+                for (int k = 0; k <= S; ++k) {
+                    // cos(theta o) summed over o = -1, 0, 1 per axis: 1 for |o| = 0, 2 cos theta for |o| = 1
+                    const double cx[2] = {1.0, 2.0 * std::cos(M_PI * i / S)};
+                    const double cy[2] = {1.0, 2.0 * std::cos(M_PI * j / S)};
+                    const double cz[2] = {1.0, 2.0 * std::cos(M_PI * k / S)};
+                    double sym = 0;
+                    for (int a = 0; a < 2; ++a)
+                        for (int b = 0; b < 2; ++b)
+                            for (int e = 0; e < 2; ++e) sym += L[a][b][e] * cx[a] * cy[b] * cz[e];
+                    lo = std::min(lo, sym);
+                }
+        c.indefinite[d] = lo < 0;
+    }
 }
 
 namespace {

@@ -29,6 +29,16 @@ class Arena {
     void* get_keep(const std::string& name, size_t bytes, size_t keepBytes);
     void release();
     size_t bytes_reserved() const { return total_; }
+    // Starts a run's count of the bytes its buffers need (the largest request
+    // of each buffer name); bytes_needed() is that sum.
+    void begin_run() {
+        for (auto& kv : blocks_) kv.second.need = 0;
+    }
+    size_t bytes_needed() const {
+        size_t s = 0;
+        for (const auto& kv : blocks_) s += kv.second.need;
+        return s;
+    }
     int allocations() const { return allocs_; }
     // While on, a buffer replaced by a larger one is freed only when deferral
     // ends (a batch records launches that still name it); turning it off
@@ -40,6 +50,7 @@ class Arena {
     struct Block {
         void* ptr = nullptr;
         size_t bytes = 0;
+        size_t need = 0;
     };
     std::map<std::string, Block> blocks_;
     bool defer_ = false;
@@ -211,6 +222,7 @@ struct Ctx {
     int cgMaxWgsRow = 0;         // resident workgroups of the one-row-per-thread CG kernels
     int cgRowsPerWg = 0;         // rows per CG workgroup (0: default)
     int cgRowTarget = 0;         // batch: rows a CG workgroup takes (0: the single-cloud plan)
+    bool indefinite[PSR_MAX_DEPTH + 1] = {};  // the depth's operator after the 1e-6 drop is indefinite
     template <class T>
     T* buf(const char* name, size_t count) {
         return static_cast<T*>(arena.get(name, sizeof(T) * (count ? count : 1)));

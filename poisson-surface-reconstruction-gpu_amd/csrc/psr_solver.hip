@@ -26,6 +26,7 @@ struct CgGroup {
     int firstWg;    // first workgroup of the group
     int wgs;        // workgroups in the group
     int rowsPerWg;
+    int indef;      // mixed form: the depth's operator is indefinite, solved with MINRES
 };
 
 struct CgParams {
@@ -503,11 +504,12 @@ __device__ __forceinline__ void cg_all_depths_body(int psr_bid, int psr_nbid, co
             float s = 0.f;
 #pragma unroll
             for (int q = 0; q < 27; ++q) {
-                float v = val[q * M + i];
-                int j = neigh[q * M + i];
-                j = (v != 0.f) ? j : i;  // dropped or missing entry: safe index, zero weight
-                float pj = p_next(ld_shared(&r[j]), beta, ld_shared(&pOld[j]));
-                s += v * pj;
+                const float v = val[q * M + i];
+                if (v != 0.f) {  // a dropped or missing entry adds exactly nothing
+                    const int j = neigh[q * M + i];
+                    float pj = p_next(ld_shared(&r[j]), beta, ld_shared(&pOld[j]));
+                    s += v * pj;
+                }
             }
             float pi = p_next(ld_shared(&r[i]), beta, ld_shared(&pOld[i]));
             st_shared(&pNew[i], pi);
@@ -929,7 +931,7 @@ __device__ __forceinline__ void cg_mixed_all_depths_body(int psr_bid, int psr_nb
     GroupCtl* ctl = ctlAll + gi;
     unsigned phase = 0;
     const bool multi = G.wgs > 1;
-    if (!multi && G.rows <= kCgThreads) {
+    if (!multi && G.rows <= kCgThreads && !G.indef) {
         cg_small_mixed(P, G, neigh, val, b, x, iters, resid, trueRel, shd, shc);
         return;
     }
@@ -974,81 +976,193 @@ __device__ __forceinline__ void cg_mixed_all_depths_body(int psr_bid, int psr_nb
         // inner target on the updated residual: half the tolerance left, no tighter than FP32 can follow
         const double tgt = fmax(0.5 * P.relTol * sqrt(bb), 1e-6 * sqrt(rr));
         const float tgt2 = float(tgt * tgt);
-        // r_0 = rhs, w_0 = A r_0, s_{-1} = 0, p_{-1} = 0
-        Comp2 d0{0.f, 0.f, 0.f, 0.f};
-        for (int i = r0 + threadIdx.x; i < r1; i += kCgThreads) {
-            float w = 0.f;
+        if (G.indef) {
+            // MINRES (Paige and Saunders) on A e = rhs, e_0 = 0, FP32 with
+            // compensated sums.  Sweep k forms y_{k-1} = y'_{k-1} - c_{k-1} y_{k-2}
+            // on the fly (c = alpha / beta), stores it, and computes
+            // y'_k = A y_{k-1} / beta_k - (beta_k / beta_{k-1}) y_{k-2} with the
+            // sums v_k . y'_k (= alpha_k) and ||y'_k||^2; v_k is a unit vector,
+            // so beta_{k+1}^2 = ||y'_k||^2 - alpha_k^2 and one group reduction
+            // per iteration suffices.  Iteration k's rotation is known only
+            // after its reduction: its direction w_k = (v_k - eps w_{k-2} -
+            // delta w_{k-1}) / gamma and the update e += phi w_k are applied in
+            // the next sweep, and after the last one.
+            float* Pb0 = rbuf;      // y'_k, k even
+            float* Pb1 = rbuf + M;  // y'_k, k odd
+            float* Qb0 = wbuf;      // y_k by k mod 3
+            float* Qb1 = wbuf + M;
+            float* Qb2 = sbuf;
+            float* W1 = sbuf + M;   // w_{k-2}, own rows only
+            float* W2 = pvec;       // w_{k-1}, own rows only
+            auto Qsel = [&](int k) -> float* { const int m = k % 3; return m == 0 ? Qb0 : (m == 1 ? Qb1 : Qb2); };
+            const float beta1 = float(sqrt(rr));
+            const float tgtf = float(tgt);
+            float beta = beta1, oldb = 1.f, cPrev = 0.f;
+            float cs = -1.f, sn = 0.f, dbar = 0.f, epsln = 0.f, phibar = beta1;
+            float lagEps = 0.f, lagDelta = 0.f, lagGamma = 1.f, lagPhi = 0.f;
+            int it = 1, kdone = 0;
+            while (beta1 > tgtf && total < P.maxIter) {
+                const float* Pprev = (it & 1) ? Pb0 : Pb1;  // y'_{it-1}
+                float* Pnew = (it & 1) ? Pb1 : Pb0;         // y'_it
+                const float* Qold = Qsel(it + 1);           // y_{it-2}
+                float* Qcur = Qsel(it + 2);                 // y_{it-1}
+                const float invB = 1.f / beta;
+                const float r1c = it >= 2 ? beta / oldb : 0.f;
+                const float invOld = 1.f / oldb;
+                Comp2 dl{0.f, 0.f, 0.f, 0.f};
+                for (int i = r0 + threadIdx.x; i < r1; i += kCgThreads) {
+                    float av = 0.f;
 #pragma unroll
-            for (int q = 0; q < 27; ++q) {
-                float v = val[q * M + i];
-                int j = neigh[q * M + i];
-                j = (v != 0.f) ? j : i;
-                w = __fmaf_rn(v, ld_shared(&rhs[j]), w);
+                    for (int q = 0; q < 27; ++q) {
+                        const float v = val[q * M + i];
+                        if (v != 0.f) {  // a dropped or missing entry adds exactly nothing
+                            const int j = neigh[q * M + i];
+                            const float yj = it == 1 ? ld_shared(&rhs[j])
+                                                     : __fmaf_rn(-cPrev, ld_shared(&Qold[j]), ld_shared(&Pprev[j]));
+                            av = __fmaf_rn(v, yj, av);
+                        }
+                    }
+                    const float r1i = it >= 2 ? ld_shared(&Qold[i]) : 0.f;  // y_{it-2}
+                    const float yi = it == 1 ? ld_shared(&rhs[i]) : __fmaf_rn(-cPrev, r1i, ld_shared(&Pprev[i]));
+                    st_shared(&Qcur[i], yi);
+                    const float ypi = __fmaf_rn(-r1c, r1i, av * invB);
+                    st_shared(&Pnew[i], ypi);
+                    dot2_acc(dl.gs, dl.gc, yi * invB, ypi);
+                    dot2_acc(dl.ds, dl.dc, ypi, ypi);
+                    float w1 = 0.f, w2 = 0.f;
+                    if (it >= 2) {
+                        w1 = W1[i];
+                        w2 = W2[i];
+                        const float wk = (r1i * invOld - lagEps * w1 - lagDelta * w2) / lagGamma;
+                        st_shared(&x[i], __fmaf_rn(lagPhi, wk, ld_shared(&x[i])));
+                        w1 = w2;
+                        w2 = wk;
+                    }
+                    W1[i] = w1;
+                    W2[i] = w2;
+                }
+                Comp2 g;
+                if (!sum_comp(dl, g)) return;
+                const float alfa = g.gs + g.gc, yy = g.ds + g.dc;
+                const float betaN = sqrtf(fmaxf(yy - alfa * alfa, 0.f));
+                const float oldeps = epsln;
+                const float delta = cs * dbar + sn * alfa;
+                const float gbar = sn * dbar - cs * alfa;
+                epsln = sn * betaN;
+                dbar = -cs * betaN;
+                const float gam = fmaxf(sqrtf(gbar * gbar + betaN * betaN), 1e-30f);
+                cs = gbar / gam;
+                sn = betaN / gam;
+                const float phi = cs * phibar;
+                phibar = sn * phibar;
+                lagEps = oldeps;
+                lagDelta = delta;
+                lagGamma = gam;
+                lagPhi = phi;
+                cPrev = alfa / beta;
+                oldb = beta;
+                beta = betaN;
+                ++kdone;
+                gamma = phibar * phibar;
+                if (!(phibar > tgtf) || total + kdone >= P.maxIter || !(betaN > 0.f)) break;
+                ++it;
             }
-            float ri = ld_shared(&rhs[i]);
-            st_shared(&rbuf[i], ri);
-            st_shared(&wbuf[i], w);
-            st_shared(&sbuf[M + i], 0.f);
-            pvec[i] = 0.f;
-            dot2_acc(d0.gs, d0.gc, ri, ri);
-            dot2_acc(d0.ds, d0.dc, w, ri);
-        }
-        Comp2 g;
-        if (!sum_comp(d0, g)) return;
-        gamma = g.gs + g.gc;
-        float delta = g.ds + g.dc;
-        float alpha = gamma / delta, beta = 0.f;
-        int k = 0;
-        while (gamma > tgt2 && total + k < P.maxIter) {
-            const float* rk = rbuf + (k & 1) * M;
-            const float* wk = wbuf + (k & 1) * M;
-            const float* sp = sbuf + ((k + 1) & 1) * M;  // s_{k-1}
-            float* rn = rbuf + ((k + 1) & 1) * M;
-            float* wn = wbuf + ((k + 1) & 1) * M;
-            float* sk = sbuf + (k & 1) * M;
-            Comp2 dl{0.f, 0.f, 0.f, 0.f};
+            if (kdone > 0) {
+                // the last iteration's update: v = y_{it-1} / beta_it
+                const float* Qcur = Qsel(it + 2);
+                const float invOld = 1.f / oldb;
+                for (int i = r0 + threadIdx.x; i < r1; i += kCgThreads) {
+                    const float wk = (ld_shared(&Qcur[i]) * invOld - lagEps * W1[i] - lagDelta * W2[i]) / lagGamma;
+                    st_shared(&x[i], __fmaf_rn(lagPhi, wk, ld_shared(&x[i])));
+                }
+            }
+            total += kdone;
+            ++outer;
+            if (kdone == 0) break;
+            // the last update wrote x after the last group reduction: one more
+            // reduction orders it before the outer residual reads neighbours' x
+            double fence;
+            if (!sum_double(0.0, fence)) return;
+        } else {
+            // r_0 = rhs, w_0 = A r_0, s_{-1} = 0, p_{-1} = 0
+            Comp2 d0{0.f, 0.f, 0.f, 0.f};
             for (int i = r0 + threadIdx.x; i < r1; i += kCgThreads) {
                 float w = 0.f;
 #pragma unroll
                 for (int q = 0; q < 27; ++q) {
-                    float v = val[q * M + i];
-                    int j = neigh[q * M + i];
-                    j = (v != 0.f) ? j : i;
-                    float rj = r_next(ld_shared(&rk[j]), alpha, s_next(ld_shared(&wk[j]), beta, ld_shared(&sp[j])));
-                    w = __fmaf_rn(v, rj, w);
+                    const float v = val[q * M + i];
+                    if (v != 0.f) {  // a dropped or missing entry adds exactly nothing
+                        const int j = neigh[q * M + i];
+                        w = __fmaf_rn(v, ld_shared(&rhs[j]), w);
+                    }
                 }
-                float ri = ld_shared(&rk[i]);
-                float si = s_next(ld_shared(&wk[i]), beta, ld_shared(&sp[i]));
-                float rni = r_next(ri, alpha, si);
-                float pi = __fmaf_rn(beta, pvec[i], ri);
-                pvec[i] = pi;
-                st_shared(&x[i], __fmaf_rn(alpha, pi, ld_shared(&x[i])));
-                st_shared(&sk[i], si);
-                st_shared(&rn[i], rni);
-                st_shared(&wn[i], w);
-                dot2_acc(dl.gs, dl.gc, rni, rni);
-                dot2_acc(dl.ds, dl.dc, w, rni);
+                float ri = ld_shared(&rhs[i]);
+                st_shared(&rbuf[i], ri);
+                st_shared(&wbuf[i], w);
+                st_shared(&sbuf[M + i], 0.f);
+                pvec[i] = 0.f;
+                dot2_acc(d0.gs, d0.gc, ri, ri);
+                dot2_acc(d0.ds, d0.dc, w, ri);
             }
-            if (!sum_comp(dl, g)) return;
-            const float gNew = g.gs + g.gc, dNew = g.ds + g.dc;
-            const float bNew = gNew / gamma;
-            alpha = gNew / (dNew - bNew * gNew / alpha);
-            beta = bNew;
-            gamma = gNew;
-            ++k;
+            Comp2 g;
+            if (!sum_comp(d0, g)) return;
+            gamma = g.gs + g.gc;
+            float delta = g.ds + g.dc;
+            float alpha = gamma / delta, beta = 0.f;
+            int k = 0;
+            while (gamma > tgt2 && total + k < P.maxIter) {
+                const float* rk = rbuf + (k & 1) * M;
+                const float* wk = wbuf + (k & 1) * M;
+                const float* sp = sbuf + ((k + 1) & 1) * M;  // s_{k-1}
+                float* rn = rbuf + ((k + 1) & 1) * M;
+                float* wn = wbuf + ((k + 1) & 1) * M;
+                float* sk = sbuf + (k & 1) * M;
+                Comp2 dl{0.f, 0.f, 0.f, 0.f};
+                for (int i = r0 + threadIdx.x; i < r1; i += kCgThreads) {
+                    float w = 0.f;
+#pragma unroll
+                    for (int q = 0; q < 27; ++q) {
+                        const float v = val[q * M + i];
+                        if (v != 0.f) {  // a dropped or missing entry adds exactly nothing
+                            const int j = neigh[q * M + i];
+                            float rj = r_next(ld_shared(&rk[j]), alpha, s_next(ld_shared(&wk[j]), beta, ld_shared(&sp[j])));
+                            w = __fmaf_rn(v, rj, w);
+                        }
+                    }
+                    float ri = ld_shared(&rk[i]);
+                    float si = s_next(ld_shared(&wk[i]), beta, ld_shared(&sp[i]));
+                    float rni = r_next(ri, alpha, si);
+                    float pi = __fmaf_rn(beta, pvec[i], ri);
+                    pvec[i] = pi;
+                    st_shared(&x[i], __fmaf_rn(alpha, pi, ld_shared(&x[i])));
+                    st_shared(&sk[i], si);
+                    st_shared(&rn[i], rni);
+                    st_shared(&wn[i], w);
+                    dot2_acc(dl.gs, dl.gc, rni, rni);
+                    dot2_acc(dl.ds, dl.dc, w, rni);
+                }
+                if (!sum_comp(dl, g)) return;
+                const float gNew = g.gs + g.gc, dNew = g.ds + g.dc;
+                const float bNew = gNew / gamma;
+                alpha = gNew / (dNew - bNew * gNew / alpha);
+                beta = bNew;
+                gamma = gNew;
+                ++k;
+            }
+            total += k;
+            ++outer;
         }
-        total += k;
-        ++outer;
         // outer residual in FP64
         loc = 0;
         for (int i = r0 + threadIdx.x; i < r1; i += kCgThreads) {
             double acc = b[i];
 #pragma unroll
             for (int q = 0; q < 27; ++q) {
-                float v = val[q * M + i];
-                int j = neigh[q * M + i];
-                j = (v != 0.f) ? j : i;
-                acc -= double(v) * double(ld_shared(&x[j]));
+                const float v = val[q * M + i];
+                if (v != 0.f) {  // a dropped or missing entry adds exactly nothing
+                    const int j = neigh[q * M + i];
+                    acc -= double(v) * double(ld_shared(&x[j]));
+                }
             }
             st_shared(&rOut[i], float(acc));
             loc += acc * acc;
@@ -1229,6 +1343,11 @@ void solve_depths(Ctx& c) {
     bool rowForm = rowsPerWg <= kCgThreads && need(rowsPerWg) <= c.cgMaxWgsRow;
     for (int d = 0; d <= D; ++d)
         if (div_up(L.count[d], kCgThreads) > unsigned(kPollPerThread * kCgThreads)) rowForm = false;
+    // the mixed form solves indefinite depths with MINRES, which only the
+    // several-rows-per-thread kernel carries
+    bool anyIndef = false;
+    for (int d = 0; d <= D; ++d) anyIndef |= mixed && c.indefinite[d];
+    if (anyIndef) rowForm = false;
     if (!rowForm)
         while (need(rowsPerWg) > c.cgMaxWgs) rowsPerWg *= 2;
     // A batch sets the rows a workgroup takes instead (cgRowTarget, from the
@@ -1243,6 +1362,7 @@ void solve_depths(Ctx& c) {
             capRows[d] = std::max(1, int(div_up(L.count[d], unsigned(w))));
             if (capRows[d] > kCgThreads) rowForm = false;
         }
+        if (anyIndef) rowForm = false;
     }
     int wg = 0;
     // largest depths first so their groups start on separate CUs early
@@ -1254,6 +1374,7 @@ void solve_depths(Ctx& c) {
         g.firstWg = wg;
         g.rowsPerWg = c.cgRowTarget > 0 ? capRows[d] : rowsPerWg;
         g.wgs = std::max(1, int(div_up(g.rows, g.rowsPerWg)));
+        g.indef = mixed && c.indefinite[d] ? 1 : 0;
         wg += g.wgs;
     }
     if (wg > (rowForm ? c.cgMaxWgsRow : c.cgMaxWgs))

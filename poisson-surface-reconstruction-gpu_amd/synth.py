@@ -221,6 +221,35 @@ def batch64(count=64, seed=SEED_BASE + 4, lo=5000, hi=150000):
     return out
 
 
+def depth_for_size(n):
+    """Octree depth of a request by its point count (depth 6 to 9)."""
+    return 6 if n < 40000 else 7 if n < 150000 else 8 if n < 450000 else 9
+
+
+def request_stream(count=120, rate_hz=20.0, seed=SEED_BASE + 5, lo=10000, hi=1000000):
+    """A seeded stream of reconstruction requests: Poisson arrivals at rate_hz
+    (exponential gaps), sizes log-uniform in [lo, hi], ellipsoids and tori as
+    in batch64, depth by size.  Returns a list of (arrival_s, xyz, normals,
+    depth), arrivals increasing from 0."""
+    meta = SplitMix64(seed, 0)
+    t = 0.0
+    out = []
+    for i in range(count):
+        u = meta.uniform(8)
+        if i:
+            t += -np.log1p(-u[7]) / rate_hz
+        n = int(np.rint(np.exp(np.log(lo) + u[0] * (np.log(hi) - np.log(lo)))))
+        rng = SplitMix64(seed, 100 + i)
+        rot = _random_rotation(rng)
+        if u[1] < 0.5:
+            p, nrm = ellipsoid(n, 0.15 + 0.30 * u[2:5], rot, rng)
+        else:
+            p, nrm = _torus_points(rng, n, 0.2 + 0.1 * u[2], 0.05 + 0.05 * u[3])
+            p, nrm = 0.5 + p @ rot.T, nrm @ rot.T
+        out.append((t, _f32(p), _f32(nrm), depth_for_size(n)))
+    return out
+
+
 WORKLOADS = {
     "sphere": (sphere, 6),
     "torus": (torus, 7),

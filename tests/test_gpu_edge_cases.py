@@ -198,6 +198,51 @@ def test_depth9_displaced_sphere(pkg, orc, torch):
     rec.close()
 
 
+def test_depth9_displaced_sphere_mixed(pkg, orc, torch):
+    """The 2M-point displaced sphere at depth 9 in mixed precision.  After the
+    reference's 1e-6 drop (LaplacianSolver.cu:62) depth 9 keeps only the
+    diagonal and the six face entries and is indefinite (the oracle's stencil
+    test flags depth 9 alone); the solver takes MINRES there and CG elsewhere.
+      * octree and neighbour arrays bit-exact against the oracle;
+      * the true relative residual per depth, recomputed here in FP64 from the
+        exported tables, function indices, neighbour table, divergence and
+        solution, equals the solver's own figure; depths 0-8 meet 1e-5;
+      * depth 9 cannot reach 1e-5 in the 10000-iteration cap: its spectrum
+        straddles zero with eigenvalues near it (the oracle's FP64 MINRES on
+        this operator falls like k^-1/2: 1.1e-3 at 100 iterations, 2.7e-4 at
+        1600), so the bound is 5e-4, against 7.6e-2 for the reference's FP32
+        CG rule at its cap;
+      * the mesh lies on the analytic displaced sphere r = 0.35 (1 + 0.12
+        sin 5t cos 3p): median radial distance within half a depth-9 cell."""
+    from residual_host import true_relres
+    xyz, nrm, d = pkg.synth.workload("displaced")
+    print("displaced depth 9 mixed", flush=True)
+    rec = pkg.PoissonReconstruction(depth=d, precision=pkg.binding.PRECISION_MIXED)
+    rec.SolvePoissionReconstructionMesh(_up(torch, xyz), _up(torch, nrm))
+    st = rec.stats()
+    ref = orc.Result(xyz, nrm, d, stages=1, precision=1)
+    np.testing.assert_array_equal(rec.export("LEVEL_COUNT"), ref.counts)
+    for g, o in INT_ARRAYS:
+        np.testing.assert_array_equal(rec.export(g).reshape(-1), ref.array(o).reshape(-1))
+    host = true_relres(rec, d)
+    kern = st["cg_true_relres"]
+    print("true relres host", ["%.2e" % v for v in host], "iterations", st["cg_iterations"], flush=True)
+    for k in range(d + 1):
+        assert abs(host[k] - kern[k]) <= 0.01 * host[k] + 1e-9, (k, host[k], kern[k])
+    assert max(host[:9]) <= 1e-5
+    assert host[9] <= 5e-4
+    v = rec.GetRebuildMeshVertices().astype(np.float64)
+    npar = rec.export("NORM_PARAMS").astype(np.float64)
+    p = npar[:3] + v * npar[3] - 0.5
+    r = np.linalg.norm(p, axis=1)
+    th = np.arccos(np.clip(p[:, 2] / r, -1, 1))
+    ph = np.arctan2(p[:, 1], p[:, 0])
+    err = np.abs(r - 0.35 * (1 + 0.12 * np.sin(5 * th) * np.cos(3 * ph)))
+    print("radial distance median %.2e, within 0.005: %.3f" % (np.median(err), np.mean(err < 0.005)), flush=True)
+    assert np.median(err) <= 0.5 * npar[3] / 512
+    rec.close()
+
+
 def test_inputs_from_kernels_in_flight(pkg, torch):
     """Points produced by torch kernels on the caller's stream right before the
     call are read only once written: the mesh equals that of resident inputs."""

@@ -263,31 +263,9 @@ def test_subdivision_chunking_does_not_change_the_mesh(pkg, torch, runs):
 
 def _true_relres(rec):
     """||b - A x|| / ||b|| per depth in FP64, with A rebuilt on the host from
-    exported arrays (neighbours, packed function index, inner-product tables)
-    by the reference's entry rule (LaplacianSolver.cu:27-74)."""
-    D = rec.depth
-    res = (1 << (D + 1)) - 1
-    mask = (1 << (D + 1)) - 1
-    neigh = rec.export("NODE_NEIGHS").astype(np.int64)
-    enc = rec.export("NODE_ENCIDX").astype(np.int64)
-    dot = rec.export("TABLE_DOT").reshape(res, res)
-    d2 = rec.export("TABLE_D2DOT").reshape(res, res)
-    b = rec.export("DIVERGENCE").astype(np.float64)
-    x = rec.export("DX").astype(np.float64)
-    f = np.stack([enc & mask, (enc >> (D + 1)) & mask, enc >> (2 * (D + 1))], 1)
-    ok = neigh >= 0
-    nb = np.where(ok, neigh, 0)
-    fo, fn = f[:, None, :], f[nb]
-    e = (dot[fo[..., 0], fn[..., 0]] * dot[fo[..., 1], fn[..., 1]] * dot[fo[..., 2], fn[..., 2]] *
-         (d2[fo[..., 0], fn[..., 0]] + d2[fo[..., 1], fn[..., 1]] + d2[fo[..., 2], fn[..., 2]]))
-    val = np.where(ok & (np.abs(e) > np.float64(np.float32(1e-6))), e.astype(np.float32), 0).astype(np.float64)
-    r = b - (val * x[nb]).sum(1)
-    base = rec.export("LEVEL_BASE")
-    out = []
-    for d in range(D + 1):
-        nbb = np.linalg.norm(b[base[d]:base[d + 1]])
-        out.append(np.linalg.norm(r[base[d]:base[d + 1]]) / nbb if nbb > 0 else 0.0)
-    return np.array(out)
+    exported arrays by the reference's entry rule (LaplacianSolver.cu:27-74)."""
+    from residual_host import true_relres
+    return np.array(true_relres(rec, rec.depth))
 
 
 @pytest.mark.parametrize("case", list(CASES))
