@@ -32,7 +32,7 @@ def test_depth_by_size(pkg):
 
 
 def test_admission_by_depth_budget_and_order(pkg):
-    srv = _server(pkg, budget=10 * 3000 * 100000)
+    srv = _server(pkg, budget=10 * pkg.serve.PRIOR_BYTES_PER_POINT[7] * 100000)
     w = [{"depth": 7, "points": 100000, "i": 0}, {"depth": 6, "points": 20000, "i": 1},
          {"depth": 7, "points": 100000, "i": 2}] + [{"depth": 7, "points": 100000, "i": 3 + k} for k in range(20)]
     depth, batch = srv._admit(w)
