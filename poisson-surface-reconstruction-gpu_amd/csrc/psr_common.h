@@ -267,7 +267,8 @@ struct AssembleArgs {
     const double* d2dot;
     int res;
     float* val;
-    int* nnz;
+    int* nnz;        // per depth; the slot masks follow the CG error flag after it
+    unsigned* slots;
 };
 __device__ __forceinline__ void assemble_rows(const AssembleArgs& a, int block) {
     const Levels& L = a.L;
@@ -281,11 +282,13 @@ __device__ __forceinline__ void assemble_rows(const AssembleArgs& a, int block) 
     // t = j * M + o: neighbours 9 j .. 9 j + 8 of row o), so that three times as
     // many waves hide the loads; a thread's table rows stay in its cache
     __shared__ int cnt[PSR_MAX_DEPTH + 1];
-    if (threadIdx.x <= PSR_MAX_DEPTH) cnt[threadIdx.x] = 0;
+    __shared__ unsigned msk[PSR_MAX_DEPTH + 1];
+    if (threadIdx.x <= PSR_MAX_DEPTH) cnt[threadIdx.x] = 0, msk[threadIdx.x] = 0u;
     __syncthreads();
     const int t = block * kAssembleThreads + int(threadIdx.x);
     const bool valid = t < 3 * M;
     int d = -1, c = 0;
+    unsigned bits = 0u;
     if (valid) {
         const int j = t / M, o = t - j * M;
         const int3 f1 = decode_func_index(enc[o], L.D);
@@ -301,6 +304,7 @@ __device__ __forceinline__ void assemble_rows(const AssembleArgs& a, int block) 
             }
             val[i * M + o] = v;
             c += v != 0.f;
+            bits |= v != 0.f ? 1u << i : 0u;
         }
         d = depth_of(L, o);
     }
@@ -311,13 +315,21 @@ __device__ __forceinline__ void assemble_rows(const AssembleArgs& a, int block) 
         const int dl = __shfl(d, leader, PSR_WAVE);
         const unsigned long long same = __ballot(d == dl) & active;
         int s = d == dl ? c : 0;
+        unsigned b = d == dl ? bits : 0u;
 #pragma unroll
-        for (int off = PSR_WAVE / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, PSR_WAVE);
-        if (__lane_id() == leader) atomicAdd(&cnt[dl], s);
+        for (int off = PSR_WAVE / 2; off > 0; off >>= 1) {
+            s += __shfl_xor(s, off, PSR_WAVE);
+            b |= unsigned(__shfl_xor(int(b), off, PSR_WAVE));
+        }
+        if (__lane_id() == leader) {
+            atomicAdd(&cnt[dl], s);
+            atomicOr(&msk[dl], b);
+        }
         active &= ~same;
     }
     __syncthreads();
     if (threadIdx.x <= PSR_MAX_DEPTH && cnt[threadIdx.x]) atomicAdd(&a.nnz[threadIdx.x], cnt[threadIdx.x]);
+    if (threadIdx.x <= PSR_MAX_DEPTH && msk[threadIdx.x]) atomicOr(&a.slots[threadIdx.x], msk[threadIdx.x]);
 }
 
 }  // namespace psr

@@ -263,9 +263,10 @@ __device__ __forceinline__ void divergence_coarse_body(int psr_bid, int psr_nbid
     const int* __restrict__ didx, const int* __restrict__ dnum, const int* __restrict__ enc,
     const FieldT* __restrict__ vf, const double* __restrict__ ddot, double* __restrict__ partial,
     int* __restrict__ nnz) {
-    // the operator's per-depth entry counts, added to beside the last divergence
-    // kernel, and the CG error flag that follows them in the results block
-    if (psr_bid == 0 && threadIdx.x <= PSR_MAX_DEPTH + 1) nnz[threadIdx.x] = 0;
+    // the operator's per-depth entry counts and slot masks, added to beside
+    // the last divergence kernel, and the CG error flag between them in the
+    // results block
+    if (psr_bid == 0 && threadIdx.x < kResultsClearWords) nnz[threadIdx.x] = 0;
     int d = 0;
     while (d + 1 < P.depths && int(psr_bid) >= P.blockStart[d + 1]) ++d;
     const int parts = P.parts[d];
@@ -472,7 +473,7 @@ void field_stages(Ctx& c, FieldT* vf) {
     // the operator (it needs only the octree and the tables) is assembled in
     // the same launch
     const AssembleArgs aa{L,     M,     o.neigh, o.encIdx, c.tables.dot, c.tables.d2dot, c.tables.res,
-                          c.buf<float>("lap_val", size_t(27) * M), c.results()->cgNnz};
+                          c.buf<float>("lap_val", size_t(27) * M), c.results()->cgNnz, c.results()->cgSlotMask};
     const int leafBlocks = int(div_up(std::max(nD, coarseCount), 256));
     PSR_LAUNCH_K(divergence_leaf<FieldT>, divergence_leaf_k<FieldT>, leafBlocks + int(div_up(size_t(3) * M, kAssembleThreads)), 0, s,
         offD, nD, M, c.tables.res, D, o.neigh, o.encIdx, vf, c.tables.ddot, P, partial, c.fld.div, leafBlocks, aa);

@@ -167,14 +167,17 @@ struct DevResults {
     int cgIters[2 * (PSR_MAX_DEPTH + 1)];  // inner iterations | outer steps, per depth
     int cgNnz[PSR_MAX_DEPTH + 1];          // stored operator entries per depth
     int cgErr;                             // a CG group barrier timed out
+    unsigned cgSlotMask[PSR_MAX_DEPTH + 1];  // per depth: the neighbour slots some row stores
     int q1Fired;                           // the level-local child 0 rule changed a parent
     float iso;                             // iso-value
     double cgResid[PSR_MAX_DEPTH + 1];     // final updated residual norm per depth
     double cgTrueRel[PSR_MAX_DEPTH + 1];   // mixed form: final true relative residual per depth
 };
 static_assert(offsetof(DevResults, cgErr) == offsetof(DevResults, cgNnz) + sizeof(int) * (PSR_MAX_DEPTH + 1) &&
-                  offsetof(DevResults, cgErr) == sizeof(int) * (3 * (PSR_MAX_DEPTH + 1)),
-              "the operator counts and the CG error flag are cleared as one range");
+                  offsetof(DevResults, cgErr) == sizeof(int) * (3 * (PSR_MAX_DEPTH + 1)) &&
+                  offsetof(DevResults, cgSlotMask) == offsetof(DevResults, cgErr) + sizeof(int),
+              "the operator counts, the CG error flag and the slot masks are cleared as one range");
+constexpr int kResultsClearWords = 2 * (PSR_MAX_DEPTH + 1) + 1;  // from cgNnz
 
 // Read-back mailbox slots (Ctx::hostPinned / Ctx::mail).  Each stage's
 // kernels write the values the host needs next; the host reads them after

@@ -158,6 +158,39 @@ __device__ __forceinline__ double fold_double(const double* parts, int n, double
 
 __device__ __forceinline__ float p_next(float r, float beta, float p) { return r + beta * p; }
 
+// The stored entries of row i in slot order, acc = term(acc, v, j) for each:
+// the N slots of `mask` (the slots some row of the depth stores, assembled
+// with the operator).  Every weight and column load of the row is issued
+// before the gathers that term makes; an entry this row does not store
+// (v = 0: dropped or missing) gathers row i itself with weight zero, which
+// adds exactly nothing.
+template <int N, class T, class F>
+__device__ __forceinline__ T row_product_n(unsigned mask, int i, int M, const float* __restrict__ val,
+                                           const int* __restrict__ neigh, T acc, F term) {
+    float vv[N];
+    int jj[N];
+    unsigned m = mask;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        const int q = N == 27 ? k : __builtin_ctz(m);
+        m &= m - 1;
+        vv[k] = val[q * M + i];
+        jj[k] = neigh[q * M + i];
+    }
+#pragma unroll
+    for (int k = 0; k < N; ++k) acc = term(acc, vv[k], vv[k] != 0.f ? jj[k] : i);
+    return acc;
+}
+// Depths whose operator keeps only the diagonal and the six face entries
+// (seven slots: depth 9 and finer after the 1e-6 drop) read those slots only;
+// every other depth reads all 27.
+template <class T, class F>
+__device__ __forceinline__ T row_product(unsigned mask, int i, int M, const float* __restrict__ val,
+                                         const int* __restrict__ neigh, T acc, F term) {
+    if (__popc(mask) == 7) return row_product_n<7>(mask, i, M, val, neigh, acc, term);
+    return row_product_n<27>(0x7ffffffu, i, M, val, neigh, acc, term);
+}
+
 // Vectors that other workgroups read are exchanged as packed 8- or 16-byte
 // records through buffer loads and stores with the cache policy of the
 // agent-scope atomics (sc1), so one load brings all of a neighbour's values.
@@ -448,7 +481,8 @@ __device__ __forceinline__ void cg_all_depths_body(int psr_bid, int psr_nbid, co
                                                             float* r, float* pA, float* pB,
                                                             float* __restrict__ Ap, double* partial,
                                                             GroupCtl* ctlAll, int* __restrict__ iters,
-                                                            double* __restrict__ resid, int* errFlag) {
+                                                            double* __restrict__ resid, int* errFlag,
+                                                            const unsigned* __restrict__ slotMasks) {
     __shared__ double sh[kCgWaves];
     __shared__ int shFlag;
     __shared__ double shRes;
@@ -467,6 +501,7 @@ __device__ __forceinline__ void cg_all_depths_body(int psr_bid, int psr_nbid, co
         cg_small_fp32(P, G, neigh, val, b, x, iters, resid, sh);
         return;
     }
+    const unsigned slotMask = slotMasks[G.depth];
 
     // Sum over the group (partials double-buffered by phase parity).
     auto group_sum = [&](double v, double& out) -> bool {
@@ -502,15 +537,9 @@ __device__ __forceinline__ void cg_all_depths_body(int psr_bid, int psr_nbid, co
         double lpAp = 0;
         for (int i = r0 + threadIdx.x; i < r1; i += kCgThreads) {
             float s = 0.f;
-#pragma unroll
-            for (int q = 0; q < 27; ++q) {
-                const float v = val[q * M + i];
-                if (v != 0.f) {  // a dropped or missing entry adds exactly nothing
-                    const int j = neigh[q * M + i];
-                    float pj = p_next(ld_shared(&r[j]), beta, ld_shared(&pOld[j]));
-                    s += v * pj;
-                }
-            }
+            s = row_product(slotMask, i, M, val, neigh, s, [&](float a, float v, int j) {
+                return a + v * p_next(ld_shared(&r[j]), beta, ld_shared(&pOld[j]));
+            });
             float pi = p_next(ld_shared(&r[i]), beta, ld_shared(&pOld[i]));
             st_shared(&pNew[i], pi);
             Ap[i] = s;
@@ -548,8 +577,10 @@ __global__ void __launch_bounds__(kCgThreads) cg_all_depths(CgParams P, const in
                                                             float* r, float* pA, float* pB,
                                                             float* __restrict__ Ap, double* partial,
                                                             GroupCtl* ctlAll, int* __restrict__ iters,
-                                                            double* __restrict__ resid, int* errFlag) {
-    cg_all_depths_body(blockIdx.x, gridDim.x, P, neigh, val, b, x, r, pA, pB, Ap, partial, ctlAll, iters, resid, errFlag);
+                                                            double* __restrict__ resid, int* errFlag,
+                                                            const unsigned* __restrict__ slotMasks) {
+    cg_all_depths_body(blockIdx.x, gridDim.x, P, neigh, val, b, x, r, pA, pB, Ap, partial, ctlAll, iters, resid, errFlag,
+                       slotMasks);
 }
 PSR_BATCHABLE(cg_all_depths, kCgThreads)
 
@@ -913,7 +944,7 @@ __device__ __forceinline__ void cg_mixed_all_depths_body(int psr_bid, int psr_nb
     const CgParams& P, const int* __restrict__ neigh, const float* __restrict__ val, const float* __restrict__ b,
     float* x, float* rOut, float* rbuf, float* wbuf, float* sbuf, float* __restrict__ pvec, double* partialD,
     Comp2* partialC, GroupCtl* ctlAll, int* __restrict__ iters, double* __restrict__ resid,
-    double* __restrict__ trueRel, int* errFlag) {
+    double* __restrict__ trueRel, int* errFlag, const unsigned* __restrict__ slotMasks) {
     __shared__ double shd[kCgWaves];
     __shared__ Comp2 shc[kCgWaves];
     __shared__ int shFlag;
@@ -935,6 +966,7 @@ __device__ __forceinline__ void cg_mixed_all_depths_body(int psr_bid, int psr_nb
         cg_small_mixed(P, G, neigh, val, b, x, iters, resid, trueRel, shd, shc);
         return;
     }
+    const unsigned slotMask = slotMasks[G.depth];
 
     auto sum_double = [&](double v, double& out) -> bool {
         double s = block_sum_double(v, shd);
@@ -1012,16 +1044,14 @@ __device__ __forceinline__ void cg_mixed_all_depths_body(int psr_bid, int psr_nb
                 Comp2 dl{0.f, 0.f, 0.f, 0.f};
                 for (int i = r0 + threadIdx.x; i < r1; i += kCgThreads) {
                     float av = 0.f;
-#pragma unroll
-                    for (int q = 0; q < 27; ++q) {
-                        const float v = val[q * M + i];
-                        if (v != 0.f) {  // a dropped or missing entry adds exactly nothing
-                            const int j = neigh[q * M + i];
-                            const float yj = it == 1 ? ld_shared(&rhs[j])
-                                                     : __fmaf_rn(-cPrev, ld_shared(&Qold[j]), ld_shared(&Pprev[j]));
-                            av = __fmaf_rn(v, yj, av);
-                        }
-                    }
+                    if (it == 1)
+                        av = row_product(slotMask, i, M, val, neigh, av, [&](float a, float v, int j) {
+                            return __fmaf_rn(v, ld_shared(&rhs[j]), a);
+                        });
+                    else
+                        av = row_product(slotMask, i, M, val, neigh, av, [&](float a, float v, int j) {
+                            return __fmaf_rn(v, __fmaf_rn(-cPrev, ld_shared(&Qold[j]), ld_shared(&Pprev[j])), a);
+                        });
                     const float r1i = it >= 2 ? ld_shared(&Qold[i]) : 0.f;  // y_{it-2}
                     const float yi = it == 1 ? ld_shared(&rhs[i]) : __fmaf_rn(-cPrev, r1i, ld_shared(&Pprev[i]));
                     st_shared(&Qcur[i], yi);
@@ -1088,14 +1118,9 @@ __device__ __forceinline__ void cg_mixed_all_depths_body(int psr_bid, int psr_nb
             Comp2 d0{0.f, 0.f, 0.f, 0.f};
             for (int i = r0 + threadIdx.x; i < r1; i += kCgThreads) {
                 float w = 0.f;
-#pragma unroll
-                for (int q = 0; q < 27; ++q) {
-                    const float v = val[q * M + i];
-                    if (v != 0.f) {  // a dropped or missing entry adds exactly nothing
-                        const int j = neigh[q * M + i];
-                        w = __fmaf_rn(v, ld_shared(&rhs[j]), w);
-                    }
-                }
+                w = row_product(slotMask, i, M, val, neigh, w, [&](float a, float v, int j) {
+                    return __fmaf_rn(v, ld_shared(&rhs[j]), a);
+                });
                 float ri = ld_shared(&rhs[i]);
                 st_shared(&rbuf[i], ri);
                 st_shared(&wbuf[i], w);
@@ -1120,15 +1145,10 @@ __device__ __forceinline__ void cg_mixed_all_depths_body(int psr_bid, int psr_nb
                 Comp2 dl{0.f, 0.f, 0.f, 0.f};
                 for (int i = r0 + threadIdx.x; i < r1; i += kCgThreads) {
                     float w = 0.f;
-#pragma unroll
-                    for (int q = 0; q < 27; ++q) {
-                        const float v = val[q * M + i];
-                        if (v != 0.f) {  // a dropped or missing entry adds exactly nothing
-                            const int j = neigh[q * M + i];
-                            float rj = r_next(ld_shared(&rk[j]), alpha, s_next(ld_shared(&wk[j]), beta, ld_shared(&sp[j])));
-                            w = __fmaf_rn(v, rj, w);
-                        }
-                    }
+                    w = row_product(slotMask, i, M, val, neigh, w, [&](float a, float v, int j) {
+                        const float rj = r_next(ld_shared(&rk[j]), alpha, s_next(ld_shared(&wk[j]), beta, ld_shared(&sp[j])));
+                        return __fmaf_rn(v, rj, a);
+                    });
                     float ri = ld_shared(&rk[i]);
                     float si = s_next(ld_shared(&wk[i]), beta, ld_shared(&sp[i]));
                     float rni = r_next(ri, alpha, si);
@@ -1156,14 +1176,9 @@ __device__ __forceinline__ void cg_mixed_all_depths_body(int psr_bid, int psr_nb
         loc = 0;
         for (int i = r0 + threadIdx.x; i < r1; i += kCgThreads) {
             double acc = b[i];
-#pragma unroll
-            for (int q = 0; q < 27; ++q) {
-                const float v = val[q * M + i];
-                if (v != 0.f) {  // a dropped or missing entry adds exactly nothing
-                    const int j = neigh[q * M + i];
-                    acc -= double(v) * double(ld_shared(&x[j]));
-                }
-            }
+            acc = row_product(slotMask, i, M, val, neigh, acc, [&](double a, float v, int j) {
+                return a - double(v) * double(ld_shared(&x[j]));
+            });
             st_shared(&rOut[i], float(acc));
             loc += acc * acc;
         }
@@ -1182,8 +1197,8 @@ __global__ void __launch_bounds__(kCgThreads) cg_mixed_all_depths(
     CgParams P, const int* __restrict__ neigh, const float* __restrict__ val, const float* __restrict__ b,
     float* x, float* rOut, float* rbuf, float* wbuf, float* sbuf, float* __restrict__ pvec, double* partialD,
     Comp2* partialC, GroupCtl* ctlAll, int* __restrict__ iters, double* __restrict__ resid,
-    double* __restrict__ trueRel, int* errFlag) {
-    cg_mixed_all_depths_body(blockIdx.x, gridDim.x, P, neigh, val, b, x, rOut, rbuf, wbuf, sbuf, pvec, partialD, partialC, ctlAll, iters, resid, trueRel, errFlag);
+    double* __restrict__ trueRel, int* errFlag, const unsigned* __restrict__ slotMasks) {
+    cg_mixed_all_depths_body(blockIdx.x, gridDim.x, P, neigh, val, b, x, rOut, rbuf, wbuf, sbuf, pvec, partialD, partialC, ctlAll, iters, resid, trueRel, errFlag, slotMasks);
 }
 PSR_BATCHABLE(cg_mixed_all_depths, kCgThreads)
 
@@ -1448,7 +1463,7 @@ void solve_depths(Ctx& c) {
             float* pvec = c.buf<float>("cg_pa", M);
             launch_kernel<cg_mixed_all_depths_k>(&siteMixedGen, cg_mixed_all_depths, wg, 0, s, true, P, o.neigh, val,
                                                  c.fld.div, c.fld.dx, rOut, rbuf, wbuf, sbuf, pvec, partial, partialC,
-                                                 ctl, iters, resid, trueRel, err);
+                                                 ctl, iters, resid, trueRel, err, res->cgSlotMask);
         }
     } else if (rowForm) {
         float2* state = c.buf<float2>("cg_state2", size_t(2) * M);
@@ -1460,7 +1475,7 @@ void solve_depths(Ctx& c) {
         float* pB = c.buf<float>("cg_pb", M);
         float* Ap = c.buf<float>("cg_ap", M);
         launch_kernel<cg_all_depths_k>(&siteGen, cg_all_depths, wg, 0, s, true, P, o.neigh, val, c.fld.div, c.fld.dx, r,
-                                       pA, pB, Ap, partial, ctl, iters, resid, err);
+                                       pA, pB, Ap, partial, ctl, iters, resid, err, res->cgSlotMask);
     }
     c.stats.cg_row_form = rowForm ? 1 : 0;
     if (!recording) {
