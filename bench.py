@@ -48,6 +48,11 @@ def parse():
                     help="clouds merged into each launch of the batched form (0: all 64)")
     ap.add_argument("--weld-reps", type=int, default=20,
                     help="timed welds, mesh normals and point-normal estimations of the workload (0: skip)")
+    ap.add_argument("--d9-steps", type=int, default=3,
+                    help="timed reconstructions of the 2M-point depth-9 displaced sphere, each precision (0: skip)")
+    ap.add_argument("--serve-requests", type=int, default=80,
+                    help="requests in the served Poisson-arrival stream (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=8.0, help="arrival rate of the served stream, per second")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain elementwise copy of MIB MiB three times (HBM counter calibration)")
     return ap.parse_args()
@@ -173,6 +178,60 @@ def measure(pkg, tx, tn, depth, precision, steps, warmup):
     cg_ms = (st["ms_cg_kernel_sum"] - before["ms_cg_kernel_sum"]) / steps
     return {"value": steps / dt, "ms_per_step": dt * 1e3 / steps, "cg_ms": cg_ms,
             "cg_bytes": float(cg_algorithmic_bytes(st, precision)), "stats": st}
+
+
+# Non-CG algorithmic bytes of one reconstruction of the 200k-point torus at
+# depth 7 (octree build, splat, divergence, point values, marching cubes and
+# subdivision, each array read or written once), and the per-iteration CG
+# bytes 8 nnz + 48 n at fixed iteration counts: the whole-reconstruction
+# roofline figure taken with the FP32 form's iteration counts.
+NON_CG_BYTES_TORUS_D7 = 156e6
+
+
+def fixed_k_roofline(st_fp32, ms_per_step):
+    total = NON_CG_BYTES_TORUS_D7
+    for k, z, n in zip(st_fp32["cg_iterations"], st_fp32["cg_nnz"], st_fp32["nodes_per_depth"]):
+        total += k * (8 * z + 48 * n)
+    achieved = total / (ms_per_step * 1e-3) / 1e9
+    return {"bytes_per_reconstruction": total, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS}
+
+
+def measure_depth9(pkg, steps):
+    """The 2,000,000-point displaced sphere at depth 9, mixed precision (FP16
+    vector field, MINRES at the indefinite depth 9, CG elsewhere) and FP32 (the
+    reference's CG rule, which runs into its 10000-iteration cap at depth 9)."""
+    import torch
+    xyz, nrm, depth = pkg.synth.workload("displaced")
+    tx, tn = torch.from_numpy(xyz).cuda(), torch.from_numpy(nrm).cuda()
+    out = {"workload": WORKLOADS["displaced"]}
+    for name in ("mixed", "fp32"):
+        r = measure(pkg, tx, tn, depth, name, steps, 1)
+        st = r["stats"]
+        out[name] = {"recon_per_s": r["value"], "ms_per_step": r["ms_per_step"], "cg_ms": r["cg_ms"],
+                     "cg_iterations": st["cg_iterations"], "ms_solve": st["ms_solve"],
+                     "true_relres": st["cg_true_relres"] if name == "mixed" else None}
+    return out
+
+
+def measure_serve(pkg, count, rate):
+    """A seeded Poisson-arrival stream of `count` requests (10k-1M points,
+    depth 6-9 by size, ellipsoids and tori) served with HBM-budget admission
+    into batched calls, mixed precision: once with every request waiting from
+    the start (the saturated rate; also the warm-up), then on the wall clock
+    at `rate` arrivals per second (latency = completion - arrival)."""
+    import torch
+    reqs = pkg.synth.request_stream(count, rate)
+    dev = [(a, torch.from_numpy(x).cuda(), torch.from_numpy(n).cuda(), d) for a, x, n, d in reqs]
+    srv = pkg.serve.Server(precision=pkg.binding.PRECISION_MIXED)
+    drain = pkg.serve.summary(srv.serve(dev, realtime=False), rate)
+    live = pkg.serve.summary(srv.serve(dev, realtime=True), rate)
+    budget = srv.budget
+    srv.close()
+    return {"stream": "%d requests, log-uniform 10k-1M points, depth 6-9 by size" % count,
+            "arrival_rate_hz": rate, "recon_per_s": live["recon_per_s"], "p50_ms": live["p50_ms"],
+            "p99_ms": live["p99_ms"], "mean_batch": live["mean_batch"], "saturated_recon_per_s": drain["recon_per_s"],
+            "per_depth": live["per_depth"], "points": live["points"], "hbm_budget_gb": budget / 1e9}
 
 
 def measure_weld(pkg, tx, tn, depth, reps):
@@ -312,7 +371,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "mixed" if head == "mixed" else "f32",
         "data": "synthetic",
         "config": {"workload": WORKLOADS[a.workload], "points": int(len(xyz)), "depth": depth,
                    "precision": head, "nodes": st["n_nodes"], "cg_iterations": st["cg_iterations"],
@@ -331,11 +390,19 @@ def main():
         line["fp32"] = {"value": f["value"], "ms_per_step": f["ms_per_step"], "cg_ms": f["cg_ms"],
                         "cg_iterations": f["stats"]["cg_iterations"]}
         line["mixed_vs_fp32"] = h["value"] / f["value"]
+        if a.workload == "torus":
+            # the same whole-reconstruction figure with the FP32 form's iteration counts
+            line["roofline_fixed_k"] = fixed_k_roofline(f["stats"], h["ms_per_step"])
+            line["roofline_fixed_k"]["frac_fp32_form"] = fixed_k_roofline(f["stats"], f["ms_per_step"])["frac"]
     if a.weld_reps > 0:
         line["weld"] = measure_weld(pkg, tx, tn, depth, a.weld_reps)
         line["point_normals"] = measure_point_normals(pkg, tx, a.weld_reps)
     if a.batch_reps > 0:
         line["batch64"] = measure_batch(pkg, a.batch_reps, a.batch_pass)
+    if a.d9_steps > 0 and a.workload != "displaced":
+        line["depth9"] = measure_depth9(pkg, a.d9_steps)
+    if a.serve_requests > 0:
+        line["serve"] = measure_serve(pkg, a.serve_requests, a.serve_rate)
     if a.cpu_sample > 0:
         line["cpu_baseline"] = cpu_baseline(xyz, nrm, depth, a.cpu_sample, a.cpu_threads)
     else:
