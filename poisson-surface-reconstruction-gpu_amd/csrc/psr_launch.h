@@ -59,6 +59,17 @@ __device__ __forceinline__ void apply_pack(F&& f, const Pack<H, T...>& p, const 
     apply_pack(f, p.t, a..., p.h);
 }
 
+// The waves per SIMD a body's kernels are compiled for (K::kMinWaves where the
+// descriptor names it, else no bound): the register budget of that occupancy.
+template <class K, class = void>
+struct min_waves {
+    static constexpr int value = 1;
+};
+template <class K>
+struct min_waves<K, std::void_t<decltype(K::kMinWaves)>> {
+    static constexpr int value = K::kMinWaves;
+};
+
 // The grid starts of the clouds of one batched launch, passed by value (the
 // search reads them from the kernel arguments, not from memory).
 constexpr int kMaxMultiClouds = 64;
@@ -69,7 +80,7 @@ struct MultiStarts {
 
 // The batched form of a kernel body K::run (K::kBlock threads a block).
 template <class K, class... T>
-__global__ void __launch_bounds__(K::kBlock)
+__global__ void __launch_bounds__(K::kBlock) __attribute__((amdgpu_waves_per_eu(min_waves<K>::value)))
     multi_kernel(const Pack<T...>* __restrict__ packs, MultiStarts st) {
     const unsigned b = blockIdx.x;
     int lo = 0;  // the last c with s[c] <= b
@@ -91,7 +102,7 @@ __global__ void __launch_bounds__(K::kBlock)
 // small group) takes the next one instead of idling until the slowest group
 // of a wave is done.
 template <class K, class... T>
-__global__ void __launch_bounds__(K::kBlock)
+__global__ void __launch_bounds__(K::kBlock) __attribute__((amdgpu_waves_per_eu(min_waves<K>::value)))
     multi_queue_kernel(const Pack<T...>* __restrict__ packs, MultiStarts st, unsigned* __restrict__ counter) {
     __shared__ unsigned sTask;
     const unsigned total = st.s[st.n];
@@ -169,6 +180,17 @@ inline void launch_kernel(const void* site, void (*fn)(T...), unsigned grid, uns
 #define PSR_BATCHABLE(NAME, NT)                                                         \
     struct NAME##_k {                                                                   \
         static constexpr int kBlock = NT;                                                \
+        template <class... A>                                                           \
+        __device__ __forceinline__ static void run(int bid, int nbid, const A&... a) { \
+            NAME##_body(bid, nbid, a...);                                               \
+        }                                                                               \
+    };
+
+// As PSR_BATCHABLE, for a body whose kernels are compiled for WAVES waves per SIMD.
+#define PSR_BATCHABLE_WAVES(NAME, NT, WAVES)                                            \
+    struct NAME##_k {                                                                   \
+        static constexpr int kBlock = NT;                                                \
+        static constexpr int kMinWaves = WAVES;                                          \
         template <class... A>                                                           \
         __device__ __forceinline__ static void run(int bid, int nbid, const A&... a) { \
             NAME##_body(bid, nbid, a...);                                               \

@@ -42,6 +42,11 @@ struct CgParams {
 
 constexpr int kCgThreads = 256;
 constexpr int kCgWaves = kCgThreads / PSR_WAVE;
+// The FP32 generic form is compiled for 4 waves per SIMD (its persistent
+// launch's resident budget follows from that occupancy; 21% faster sweeps at
+// depth 9 than at 2).  The mixed generic form keeps the compiler's choice
+// (2): bounded to 3 its MINRES sweeps ran 46% slower.
+constexpr int kCgFp32GenericWaves = 4;
 
 // Vectors read by other workgroups (r, p) and the reduction partials are
 // accessed with agent-scope relaxed atomics, which are coherent across the
@@ -52,6 +57,15 @@ __device__ __forceinline__ float ld_shared(const float* p) {
 }
 __device__ __forceinline__ void st_shared(float* p, float v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Orders a wave's LDS and memory writes before its later reads of other
+// lanes' values: the steps of a group that fits one wave need no workgroup
+// barrier.
+__device__ __forceinline__ void wave_sync_memory() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
 __device__ __forceinline__ double block_sum_double(double v, double* sh) {
@@ -167,18 +181,25 @@ __device__ __forceinline__ float p_next(float r, float beta, float p) { return r
 template <int N, class T, class F>
 __device__ __forceinline__ T row_product_n(unsigned mask, int i, int M, const float* __restrict__ val,
                                            const int* __restrict__ neigh, T acc, F term) {
-    float vv[N];
-    int jj[N];
+    // in chunks of at most 9 slots, so that the loads in flight stay within
+    // the register budget of the occupancy the CG launch is sized for
+    constexpr int C = N < 9 ? N : 9;
+    static_assert(N % C == 0, "whole chunks");
     unsigned m = mask;
 #pragma unroll
-    for (int k = 0; k < N; ++k) {
-        const int q = N == 27 ? k : __builtin_ctz(m);
-        m &= m - 1;
-        vv[k] = val[q * M + i];
-        jj[k] = neigh[q * M + i];
-    }
+    for (int c0 = 0; c0 < N; c0 += C) {
+        float vv[C];
+        int jj[C];
 #pragma unroll
-    for (int k = 0; k < N; ++k) acc = term(acc, vv[k], vv[k] != 0.f ? jj[k] : i);
+        for (int k = 0; k < C; ++k) {
+            const int q = N == 27 ? c0 + k : __builtin_ctz(m);
+            m &= m - 1;
+            vv[k] = val[q * M + i];
+            jj[k] = neigh[q * M + i];
+        }
+#pragma unroll
+        for (int k = 0; k < C; ++k) acc = term(acc, vv[k], vv[k] != 0.f ? jj[k] : i);
+    }
     return acc;
 }
 // Depths whose operator keeps only the diagonal and the six face entries
@@ -339,8 +360,18 @@ __device__ __forceinline__ void cg_fp32_row_body(int psr_bid, int psr_nbid, cons
     // are the depth's, all in this workgroup): no memory round trip per sweep
     __shared__ float2 lrec2[2][kCgThreads];
     const bool multi = G.wgs > 1;
+    // a group of at most one wave of rows (the coarsest depths) iterates in
+    // wave 0 alone, with wave reductions and no workgroup barrier per step
+    // (a wave's sum equals the block's, whose other waves add exact zeros)
+    const bool oneWave = !multi && G.rows <= PSR_WAVE;
+    if (oneWave && int(threadIdx.x) >= PSR_WAVE) return;
     const __amdgpu_buffer_rsrc_t st = vec_rsrc(state, sizeof(float2) * 2 * size_t(M));
     auto group_sum = [&](double v, double& out) -> bool {
+        if (oneWave) {
+            out = wave_sum_f64(v);
+            wave_sync_memory();
+            return true;
+        }
         double s = block_sum_double(v, sh);
         if (!multi) {
             out = s;
@@ -418,10 +449,21 @@ PSR_BATCHABLE(cg_fp32_row, kCgThreads)
 // and a batch's small clouds), solved by that workgroup alone with its
 // vectors in LDS: cg_all_depths' arithmetic without a memory round trip per
 // sweep.
+// Makes p opaque to the compiler: loads through it are repeated after this
+// point instead of being kept in registers from an earlier sweep.
+template <class T>
+__device__ __forceinline__ void opaque(const T*& p) {
+    asm volatile("" : "+v"(p));
+}
+
 __device__ __forceinline__ void cg_small_fp32(const CgParams& P, const CgGroup& G, const int* __restrict__ neigh,
                                               const float* __restrict__ val, const float* __restrict__ b,
                                               float* __restrict__ x, int* __restrict__ iters,
                                               double* __restrict__ resid, double* sh) {
+    // the small groups' stencil is re-read every sweep (it stays in cache) rather than
+    // held in registers: this path shares its kernel with the large-depth sweeps
+    const float* vs = val;
+    const int* ns = neigh;
     __shared__ float lr[kCgThreads], lp[2][kCgThreads];
     const int M = P.M, t = int(threadIdx.x), i = G.rowBegin + t, rb = G.rowBegin;
     const bool own = t < G.rows;
@@ -441,12 +483,11 @@ __device__ __forceinline__ void cg_small_fp32(const CgParams& P, const CgGroup& 
         double lpAp = 0;
         if (own) {
             float s = 0.f;
-#pragma unroll
-            for (int q = 0; q < 27; ++q) {
-                const float v = val[q * M + i];
-                const int j = ((v != 0.f) ? neigh[q * M + i] : i) - rb;
-                s += v * p_next(lr[j], beta, lp[cur][j]);
-            }
+            opaque(vs);
+            opaque(ns);
+            s = row_product_n<27>(0x7ffffffu, i, M, vs, ns, s, [&](float a, float v, int j) {
+                return a + v * p_next(lr[j - rb], beta, lp[cur][j - rb]);
+            });
             pi = p_next(ri, beta, lp[cur][t]);
             lp[cur ^ 1][t] = pi;
             api = s;
@@ -571,7 +612,7 @@ __device__ __forceinline__ void cg_all_depths_body(int psr_bid, int psr_nbid, co
         resid[G.depth] = sqrt(double(rr));
     }
 }
-__global__ void __launch_bounds__(kCgThreads) cg_all_depths(CgParams P, const int* __restrict__ neigh,
+__global__ void __launch_bounds__(kCgThreads) __attribute__((amdgpu_waves_per_eu(kCgFp32GenericWaves))) cg_all_depths(CgParams P, const int* __restrict__ neigh,
                                                             const float* __restrict__ val,
                                                             const float* __restrict__ b, float* __restrict__ x,
                                                             float* r, float* pA, float* pB,
@@ -582,7 +623,7 @@ __global__ void __launch_bounds__(kCgThreads) cg_all_depths(CgParams P, const in
     cg_all_depths_body(blockIdx.x, gridDim.x, P, neigh, val, b, x, r, pA, pB, Ap, partial, ctlAll, iters, resid, errFlag,
                        slotMasks);
 }
-PSR_BATCHABLE(cg_all_depths, kCgThreads)
+PSR_BATCHABLE_WAVES(cg_all_depths, kCgThreads, kCgFp32GenericWaves)
 
 
 // --- mixed-precision CG ------------------------------------------------------
@@ -639,20 +680,21 @@ __device__ __forceinline__ void comp2_dpp_step(Comp2& v) {
 __device__ __forceinline__ Comp2 readlane_comp2(const Comp2& v, int lane) {
     return Comp2{readlane_f32(v.gs, lane), readlane_f32(v.gc, lane), readlane_f32(v.ds, lane), readlane_f32(v.dc, lane)};
 }
-__device__ __forceinline__ Comp2 block_comp2(Comp2 v, Comp2* sh) {
-    // the wave's merge through DPP (psr_common.h: rows of 16, then the four
-    // rows in a fixed order)
+// The wave's merge through DPP (psr_common.h: rows of 16, then the four rows
+// in a fixed order); every lane returns the same value.
+__device__ __forceinline__ Comp2 wave_comp2(Comp2 v) {
     comp2_dpp_step<kDppSwap1>(v);
     comp2_dpp_step<kDppSwap2>(v);
     comp2_dpp_step<kDppHalfMirror>(v);
     comp2_dpp_step<kDppMirror>(v);
-    {
-        Comp2 r = readlane_comp2(v, 0);
-        comp2_merge(r, readlane_comp2(v, 16));
-        comp2_merge(r, readlane_comp2(v, 32));
-        comp2_merge(r, readlane_comp2(v, 48));
-        v = r;
-    }
+    Comp2 r = readlane_comp2(v, 0);
+    comp2_merge(r, readlane_comp2(v, 16));
+    comp2_merge(r, readlane_comp2(v, 32));
+    comp2_merge(r, readlane_comp2(v, 48));
+    return r;
+}
+__device__ __forceinline__ Comp2 block_comp2(Comp2 v, Comp2* sh) {
+    v = wave_comp2(v);
     int w = threadIdx.x / PSR_WAVE;
     if (__lane_id() == 0) sh[w] = v;
     __syncthreads();
@@ -689,8 +731,17 @@ __device__ __forceinline__ void cg_mixed_row_body(int psr_bid, int psr_nbid, con
     // a group of one workgroup exchanges its records through LDS
     __shared__ float3 lrec3[2][kCgThreads];
     const bool multi = G.wgs > 1;
+    // a group of at most one wave of rows iterates in wave 0 alone (as in the
+    // FP32 row form)
+    const bool oneWave = !multi && G.rows <= PSR_WAVE;
+    if (oneWave && int(threadIdx.x) >= PSR_WAVE) return;
     const __amdgpu_buffer_rsrc_t st = vec_rsrc(state, sizeof(float3) * 2 * size_t(M));
     auto sum_double = [&](double v, double& out) -> bool {
+        if (oneWave) {
+            out = wave_sum_f64(v);
+            wave_sync_memory();
+            return true;
+        }
         double s = block_sum_double(v, shd);
         if (!multi) {
             out = s;
@@ -699,6 +750,11 @@ __device__ __forceinline__ void cg_mixed_row_body(int psr_bid, int psr_nbid, con
         return slot_sum_double(s, out, region, G.wgs, local, phase, P.spinLimit, errFlag, shd);
     };
     auto sum_comp = [&](Comp2 v, Comp2& out) -> bool {
+        if (oneWave) {
+            out = wave_comp2(v);
+            wave_sync_memory();
+            return true;
+        }
         Comp2 s = block_comp2(v, shc);
         if (!multi) {
             out = s;
@@ -838,6 +894,10 @@ __device__ __forceinline__ void cg_small_mixed(const CgParams& P, const CgGroup&
                                                float* __restrict__ x, int* __restrict__ iters,
                                                double* __restrict__ resid, double* __restrict__ trueRel,
                                                double* shd, Comp2* shc) {
+    // the small groups' stencil is re-read every sweep (it stays in cache) rather than
+    // held in registers: this path shares its kernel with the large-depth sweeps
+    const float* vs = val;
+    const int* ns = neigh;
     __shared__ float lx[kCgThreads], lro[kCgThreads], lr[2][kCgThreads], lw[2][kCgThreads], ls[2][kCgThreads];
     const int M = P.M, t = int(threadIdx.x), i = G.rowBegin + t, rb = G.rowBegin;
     const bool own = t < G.rows;
@@ -860,12 +920,11 @@ __device__ __forceinline__ void cg_small_mixed(const CgParams& P, const CgGroup&
         Comp2 d0{0.f, 0.f, 0.f, 0.f};
         if (own) {
             float w = 0.f;
-#pragma unroll
-            for (int q = 0; q < 27; ++q) {
-                const float v = val[q * M + i];
-                const int j = (v != 0.f) ? neigh[q * M + i] : i;
-                w = __fmaf_rn(v, fromB ? b[j] : lro[j - rb], w);
-            }
+            opaque(vs);
+            opaque(ns);
+            w = row_product_n<27>(0x7ffffffu, i, M, vs, ns, w, [&](float a, float v, int j) {
+                return __fmaf_rn(v, fromB ? b[j] : lro[j - rb], a);
+            });
             const float ri = fromB ? b[i] : lro[t];
             lr[0][t] = ri;
             lw[0][t] = w;
@@ -884,13 +943,12 @@ __device__ __forceinline__ void cg_small_mixed(const CgParams& P, const CgGroup&
             Comp2 dl{0.f, 0.f, 0.f, 0.f};
             if (own) {
                 float w = 0.f;
-#pragma unroll
-                for (int q = 0; q < 27; ++q) {
-                    const float v = val[q * M + i];
-                    const int j = ((v != 0.f) ? neigh[q * M + i] : i) - rb;
-                    const float rj = r_next(lr[c][j], alpha, s_next(lw[c][j], beta, ls[n][j]));
-                    w = __fmaf_rn(v, rj, w);
-                }
+                opaque(vs);
+                opaque(ns);
+                w = row_product_n<27>(0x7ffffffu, i, M, vs, ns, w, [&](float a, float v, int j) {
+                    j -= rb;
+                    return __fmaf_rn(v, r_next(lr[c][j], alpha, s_next(lw[c][j], beta, ls[n][j])), a);
+                });
                 const float ri = lr[c][t];
                 const float si = s_next(lw[c][t], beta, ls[n][t]);
                 const float rni = r_next(ri, alpha, si);
@@ -916,12 +974,11 @@ __device__ __forceinline__ void cg_small_mixed(const CgParams& P, const CgGroup&
         loc = 0;
         if (own) {
             double acc = b[i];
-#pragma unroll
-            for (int q = 0; q < 27; ++q) {
-                const float v = val[q * M + i];
-                const int j = ((v != 0.f) ? neigh[q * M + i] : i) - rb;
-                acc -= double(v) * double(lx[j]);
-            }
+            opaque(vs);
+            opaque(ns);
+            acc = row_product_n<27>(0x7ffffffu, i, M, vs, ns, acc, [&](double a, float v, int j) {
+                return a - double(v) * double(lx[j - rb]);
+            });
             lro[t] = float(acc);
             loc = acc * acc;
         }
