@@ -180,7 +180,7 @@ __global__ void __launch_bounds__(kVfWaves * PSR_WAVE) vector_field_kernel(
     int nD, int offD, int M, int D, const int* __restrict__ neigh, const int* __restrict__ pidx,
     const int* __restrict__ pnum, const int* __restrict__ key, const float4* __restrict__ pos,
     const float4* __restrict__ nrm, FieldT* __restrict__ vf) {
-    vector_field_kernel_body<FieldT>(blockIdx.x, gridDim.x, nD, offD, M, D, neigh, pidx, pnum, key, pos, nrm, vf);
+    vector_field_kernel_body<FieldT>(xcd_block(blockIdx.x, gridDim.x), gridDim.x, nD, offD, M, D, neigh, pidx, pnum, key, pos, nrm, vf);
 }
 template <class FieldT>
 struct vector_field_kernel_k {
