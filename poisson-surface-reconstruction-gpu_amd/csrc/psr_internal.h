@@ -223,6 +223,7 @@ struct Ctx {
     size_t hostStageBytes = 0;
     int cgMaxWgs = 0;            // resident workgroups of the CG kernels with several rows per thread
     int cgMaxWgsRow = 0;         // resident workgroups of the one-row-per-thread CG kernels
+    int cgMaxWgsMinres = 0;      // resident workgroups of the MINRES kernel (indefinite depths, mixed form)
     int cgRowsPerWg = 0;         // rows per CG workgroup (0: default)
     int cgRowTarget = 0;         // batch: rows a CG workgroup takes (0: the single-cloud plan)
     bool indefinite[PSR_MAX_DEPTH + 1] = {};  // the depth's operator after the 1e-6 drop is indefinite

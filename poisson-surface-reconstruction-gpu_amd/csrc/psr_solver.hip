@@ -44,9 +44,11 @@ constexpr int kCgThreads = 256;
 constexpr int kCgWaves = kCgThreads / PSR_WAVE;
 // The FP32 generic form is compiled for 4 waves per SIMD (its persistent
 // launch's resident budget follows from that occupancy; 21% faster sweeps at
-// depth 9 than at 2).  The mixed generic form keeps the compiler's choice
-// (2): bounded to 3 its MINRES sweeps ran 46% slower.
-constexpr int kCgFp32GenericWaves = 4;
+// depth 9 than at 2), the mixed generic CG for 3.
+constexpr int kCgFp32GenericWaves = 4, kCgMixedGenericWaves = 3;
+// The MINRES kernel (indefinite depths of the mixed form) carries only its
+// own path and is compiled for 4 waves per SIMD.
+constexpr int kCgMinresWaves = 4;
 
 // Vectors read by other workgroups (r, p) and the reduction partials are
 // accessed with agent-scope relaxed atomics, which are coherent across the
@@ -996,8 +998,12 @@ __device__ __forceinline__ void cg_small_mixed(const CgParams& P, const CgGroup&
 }
 
 // The mixed form with several rows per thread (large depths): vectors in
-// memory, the stencil read every sweep.
-__device__ __forceinline__ void cg_mixed_all_depths_body(int psr_bid, int psr_nbid, 
+// memory, the stencil read every sweep.  Instantiated twice: the CG kernel
+// for definite depths (kMinres false) and the MINRES kernel for the depths
+// whose operator is indefinite (kMinres true), each compiled without the
+// other's path so that each keeps its own register budget.
+template <bool kMinres>
+__device__ __forceinline__ void cg_mixed_generic_body(int psr_bid, int psr_nbid, 
     const CgParams& P, const int* __restrict__ neigh, const float* __restrict__ val, const float* __restrict__ b,
     float* x, float* rOut, float* rbuf, float* wbuf, float* sbuf, float* __restrict__ pvec, double* partialD,
     Comp2* partialC, GroupCtl* ctlAll, int* __restrict__ iters, double* __restrict__ resid,
@@ -1019,9 +1025,11 @@ __device__ __forceinline__ void cg_mixed_all_depths_body(int psr_bid, int psr_nb
     GroupCtl* ctl = ctlAll + gi;
     unsigned phase = 0;
     const bool multi = G.wgs > 1;
-    if (!multi && G.rows <= kCgThreads && !G.indef) {
-        cg_small_mixed(P, G, neigh, val, b, x, iters, resid, trueRel, shd, shc);
-        return;
+    if constexpr (!kMinres) {
+        if (!multi && G.rows <= kCgThreads) {
+            cg_small_mixed(P, G, neigh, val, b, x, iters, resid, trueRel, shd, shc);
+            return;
+        }
     }
     const unsigned slotMask = slotMasks[G.depth];
 
@@ -1065,7 +1073,7 @@ __device__ __forceinline__ void cg_mixed_all_depths_body(int psr_bid, int psr_nb
         // inner target on the updated residual: half the tolerance left, no tighter than FP32 can follow
         const double tgt = fmax(0.5 * P.relTol * sqrt(bb), 1e-6 * sqrt(rr));
         const float tgt2 = float(tgt * tgt);
-        if (G.indef) {
+        if constexpr (kMinres) {
             // MINRES (Paige and Saunders) on A e = rhs, e_0 = 0, FP32 with
             // compensated sums.  Sweep k forms y_{k-1} = y'_{k-1} - c_{k-1} y_{k-2}
             // on the fly (c = alpha / beta), stores it, and computes
@@ -1250,14 +1258,30 @@ __device__ __forceinline__ void cg_mixed_all_depths_body(int psr_bid, int psr_nb
         trueRel[G.depth] = rel;
     }
 }
-__global__ void __launch_bounds__(kCgThreads) cg_mixed_all_depths(
+template <class... A>
+__device__ __forceinline__ void cg_mixed_all_depths_body(int bid, int nbid, const A&... a) {
+    cg_mixed_generic_body<false>(bid, nbid, a...);
+}
+template <class... A>
+__device__ __forceinline__ void cg_mixed_minres_body(int bid, int nbid, const A&... a) {
+    cg_mixed_generic_body<true>(bid, nbid, a...);
+}
+__global__ void __launch_bounds__(kCgThreads) __attribute__((amdgpu_waves_per_eu(kCgMixedGenericWaves))) cg_mixed_all_depths(
     CgParams P, const int* __restrict__ neigh, const float* __restrict__ val, const float* __restrict__ b,
     float* x, float* rOut, float* rbuf, float* wbuf, float* sbuf, float* __restrict__ pvec, double* partialD,
     Comp2* partialC, GroupCtl* ctlAll, int* __restrict__ iters, double* __restrict__ resid,
     double* __restrict__ trueRel, int* errFlag, const unsigned* __restrict__ slotMasks) {
     cg_mixed_all_depths_body(blockIdx.x, gridDim.x, P, neigh, val, b, x, rOut, rbuf, wbuf, sbuf, pvec, partialD, partialC, ctlAll, iters, resid, trueRel, errFlag, slotMasks);
 }
-PSR_BATCHABLE(cg_mixed_all_depths, kCgThreads)
+PSR_BATCHABLE_WAVES(cg_mixed_all_depths, kCgThreads, kCgMixedGenericWaves)
+__global__ void __launch_bounds__(kCgThreads) __attribute__((amdgpu_waves_per_eu(kCgMinresWaves))) cg_mixed_minres(
+    CgParams P, const int* __restrict__ neigh, const float* __restrict__ val, const float* __restrict__ b,
+    float* x, float* rOut, float* rbuf, float* wbuf, float* sbuf, float* __restrict__ pvec, double* partialD,
+    Comp2* partialC, GroupCtl* ctlAll, int* __restrict__ iters, double* __restrict__ resid,
+    double* __restrict__ trueRel, int* errFlag, const unsigned* __restrict__ slotMasks) {
+    cg_mixed_minres_body(blockIdx.x, gridDim.x, P, neigh, val, b, x, rOut, rbuf, wbuf, sbuf, pvec, partialD, partialC, ctlAll, iters, resid, trueRel, errFlag, slotMasks);
+}
+PSR_BATCHABLE_WAVES(cg_mixed_minres, kCgThreads, kCgMinresWaves)
 
 
 // --- implicit function at points (LaplacianSolver.cu:85-118, 269-305) ------
@@ -1396,32 +1420,55 @@ void solve_depths(Ctx& c) {
     // (the groups meet in barriers).  The launch is an ordinary one: the budget,
     // with the accounting of concurrent launches below, is the residency
     // guarantee, and the barriers' spin bound turns a broken one into an error.
+    // A batch merges the clouds' launches into the kernels' batched forms,
+    // which the compiler may give other register counts: a budget is the
+    // smaller occupancy of the two forms, so that a plan made for one cloud
+    // also fits the merged launch.
     if (c.cgMaxWgs <= 0) {
-        int perRow = 0, perGen = 0, cus = 0;
-        PSR_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perRow, rowKernel, kCgThreads, 0));
-        PSR_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perGen, genKernel, kCgThreads, 0));
+        auto occupancy = [](const void* single, const void* batched) {
+            int a = 0, b = 0;
+            PSR_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, single, kCgThreads, 0));
+            PSR_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, batched, kCgThreads, 0));
+            return std::min(a, b);
+        };
+        const void* rowQueue = mixed ? multi_queue_kernel_of<cg_mixed_row_k>(cg_mixed_row)
+                                     : multi_queue_kernel_of<cg_fp32_row_k>(cg_fp32_row);
+        const void* genQueue = mixed ? multi_queue_kernel_of<cg_mixed_all_depths_k>(cg_mixed_all_depths)
+                                     : multi_queue_kernel_of<cg_all_depths_k>(cg_all_depths);
+        int cus = 0;
+        const int perRow = occupancy(rowKernel, rowQueue), perGen = occupancy(genKernel, genQueue);
+        const int perMinres = occupancy(reinterpret_cast<const void*>(cg_mixed_minres),
+                                        multi_queue_kernel_of<cg_mixed_minres_k>(cg_mixed_minres));
         PSR_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device));
         c.cgMaxWgsRow = std::max(0, perRow * cus);
         c.cgMaxWgs = std::max(1, perGen * cus);
+        c.cgMaxWgsMinres = std::max(1, perMinres * cus);
     }
-    auto need = [&](int rpw) {
+    // the mixed form solves indefinite depths with MINRES, in a launch of its
+    // own (the MINRES kernel) after the CG launch of the other depths
+    bool isMinres[PSR_MAX_DEPTH + 1];
+    bool anyIndef = false;
+    for (int d = 0; d <= D; ++d) anyIndef |= (isMinres[d] = mixed && c.indefinite[d]);
+    auto need_of = [&](int rpw, bool minresSet) {
         long long w = 0;
-        for (int d = 0; d <= D; ++d) w += std::max<long long>(1, (L.count[d] + rpw - 1) / rpw);
+        for (int d = 0; d <= D; ++d)
+            if (isMinres[d] == minresSet) w += std::max<long long>(1, (L.count[d] + rpw - 1) / rpw);
         return w;
     };
+    auto need = [&](int rpw) { return need_of(rpw, false) + need_of(rpw, true); };
     // one row per thread (stencil in registers) when every group fits at
     // kCgThreads rows per workgroup; otherwise widen the row blocks
     int rowsPerWg = c.cgRowsPerWg > 0 ? c.cgRowsPerWg : kCgThreads;
     bool rowForm = rowsPerWg <= kCgThreads && need(rowsPerWg) <= c.cgMaxWgsRow;
     for (int d = 0; d <= D; ++d)
         if (div_up(L.count[d], kCgThreads) > unsigned(kPollPerThread * kCgThreads)) rowForm = false;
-    // the mixed form solves indefinite depths with MINRES, which only the
-    // several-rows-per-thread kernel carries
-    bool anyIndef = false;
-    for (int d = 0; d <= D; ++d) anyIndef |= mixed && c.indefinite[d];
+    // MINRES runs in the several-rows-per-thread form only
     if (anyIndef) rowForm = false;
-    if (!rowForm)
-        while (need(rowsPerWg) > c.cgMaxWgs) rowsPerWg *= 2;
+    int rowsPerWgMinres = rowsPerWg;
+    if (!rowForm) {
+        while (need_of(rowsPerWg, false) > c.cgMaxWgs) rowsPerWg *= 2;
+        while (need_of(rowsPerWgMinres, true) > c.cgMaxWgsMinres) rowsPerWgMinres *= 2;
+    }
     // A batch sets the rows a workgroup takes instead (cgRowTarget, from the
     // batch's total rows and the resident budget): a depth of n rows gets
     // ceil(n / target) workgroups of equal shares; one row per thread when
@@ -1436,23 +1483,27 @@ void solve_depths(Ctx& c) {
         }
         if (anyIndef) rowForm = false;
     }
-    int wg = 0;
-    // largest depths first so their groups start on separate CUs early
+    // largest depths first so their groups start on separate CUs early; the
+    // MINRES depths form a second launch (PM) with its own workgroup numbering
+    CgParams PM = P;
+    int wg = 0, wgM = 0;
     for (int d = D; d >= 0; --d) {
-        CgGroup& g = P.g[P.ngroups++];
+        CgParams& Q = isMinres[d] ? PM : P;
+        int& w = isMinres[d] ? wgM : wg;
+        CgGroup& g = Q.g[Q.ngroups++];
         g.depth = d;
         g.rowBegin = L.base[d];
         g.rows = L.count[d];
-        g.firstWg = wg;
-        g.rowsPerWg = c.cgRowTarget > 0 ? capRows[d] : rowsPerWg;
+        g.firstWg = w;
+        g.rowsPerWg = c.cgRowTarget > 0 ? capRows[d] : (isMinres[d] ? rowsPerWgMinres : rowsPerWg);
         g.wgs = std::max(1, int(div_up(g.rows, g.rowsPerWg)));
-        g.indef = mixed && c.indefinite[d] ? 1 : 0;
-        wg += g.wgs;
+        g.indef = isMinres[d] ? 1 : 0;
+        w += g.wgs;
     }
-    if (wg > (rowForm ? c.cgMaxWgsRow : c.cgMaxWgs))
+    if (wg > (rowForm ? c.cgMaxWgsRow : c.cgMaxWgs) || wgM > c.cgMaxWgsMinres)
         throw std::runtime_error("CG launch exceeds the resident workgroup budget");
-    c.stats.cg_workgroups = uint32_t(wg);
-    double* partial = c.buf<double>("cg_partial", size_t(2) * wg);
+    c.stats.cg_workgroups = uint32_t(wg + wgM);
+    double* partial = c.buf<double>("cg_partial", size_t(2) * (wg + wgM));
     GroupCtl* ctl = c.buf<GroupCtl>("cg_ctl", PSR_MAX_DEPTH + 1);
     const int ctlWords = int(sizeof(GroupCtl) / 4) * (PSR_MAX_DEPTH + 1);
     int* iters = res->cgIters;  // [inner iterations | outer steps] per depth
@@ -1478,8 +1529,11 @@ void solve_depths(Ctx& c) {
     // launches still in flight leave room for all its workgroups beside them
     // (the host waits otherwise), so concurrent contexts -- the batch workers --
     // run their solves side by side whenever they fit.
+    // the two launches of a split solve run one after the other on the stream:
+    // the account holds the larger
     const long long cap = rowForm ? c.cgMaxWgsRow : c.cgMaxWgs;
-    const long long units = (wg * kCgUnits + cap - 1) / cap;
+    const long long units = std::max((wg * kCgUnits + cap - 1) / cap,
+                                     (wgM * kCgUnits + c.cgMaxWgsMinres - 1) / c.cgMaxWgsMinres);
     // While a batch records, the batch merges every cloud's solve into launches
     // that fit the resident budget and accounts for them itself.
     const bool recording = current_recorder() != nullptr;
@@ -1503,12 +1557,12 @@ void solve_depths(Ctx& c) {
         PSR_HIP_CHECK(hipEventRecord(c.ev[8], s));
     }
     if (rowForm && mixed) P.spinLimit = 1u << 22;
-    static const char siteMixedRow = 0, siteMixedGen = 0, siteRow = 0, siteGen = 0;
+    static const char siteMixedRow = 0, siteMixedGen = 0, siteMinres = 0, siteRow = 0, siteGen = 0;
     if (mixed) {
-        P.relTol = c.cfg.refine_rel_tol;
-        P.maxOuter = std::max(1, c.cfg.refine_max_outer);
+        P.relTol = PM.relTol = c.cfg.refine_rel_tol;
+        P.maxOuter = PM.maxOuter = std::max(1, c.cfg.refine_max_outer);
         float* rOut = c.buf<float>("cg_rout", M);
-        Comp2* partialC = c.buf<Comp2>("cg_partial_c", size_t(2) * wg);
+        Comp2* partialC = c.buf<Comp2>("cg_partial_c", size_t(2) * (wg + wgM));
         if (rowForm) {
             float3* state = c.buf<float3>("cg_state3", size_t(2) * M);
             launch_kernel<cg_mixed_row_k>(&siteMixedRow, cg_mixed_row, wg, 0, s, true, P, o.neigh, val, c.fld.div,
@@ -1518,9 +1572,16 @@ void solve_depths(Ctx& c) {
             float* wbuf = c.buf<float>("cg_wbuf", size_t(2) * M);
             float* sbuf = c.buf<float>("cg_sbuf", size_t(2) * M);
             float* pvec = c.buf<float>("cg_pa", M);
-            launch_kernel<cg_mixed_all_depths_k>(&siteMixedGen, cg_mixed_all_depths, wg, 0, s, true, P, o.neigh, val,
-                                                 c.fld.div, c.fld.dx, rOut, rbuf, wbuf, sbuf, pvec, partial, partialC,
-                                                 ctl, iters, resid, trueRel, err, res->cgSlotMask);
+            if (P.ngroups > 0)
+                launch_kernel<cg_mixed_all_depths_k>(&siteMixedGen, cg_mixed_all_depths, wg, 0, s, true, P, o.neigh,
+                                                     val, c.fld.div, c.fld.dx, rOut, rbuf, wbuf, sbuf, pvec, partial,
+                                                     partialC, ctl, iters, resid, trueRel, err, res->cgSlotMask);
+            // the MINRES launch uses the group controls and partials after the CG launch's
+            if (PM.ngroups > 0)
+                launch_kernel<cg_mixed_minres_k>(&siteMinres, cg_mixed_minres, wgM, 0, s, true, PM, o.neigh, val,
+                                                 c.fld.div, c.fld.dx, rOut, rbuf, wbuf, sbuf, pvec,
+                                                 partial + size_t(2) * wg, partialC + size_t(2) * wg,
+                                                 ctl + P.ngroups, iters, resid, trueRel, err, res->cgSlotMask);
         }
     } else if (rowForm) {
         float2* state = c.buf<float2>("cg_state2", size_t(2) * M);
