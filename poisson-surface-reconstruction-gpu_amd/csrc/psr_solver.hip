@@ -404,10 +404,20 @@ __device__ __forceinline__ void cg_fp32_row_body(int psr_bid, int psr_nbid, cons
         double lpAp = 0;
         if (own) {
             float s = 0.f;
+            if (oneWave) {
+                // a straight-line LDS loop for the one-wave groups (the choice
+                // per slot below costs them two branches a slot)
 #pragma unroll
-            for (int q = 0; q < 27; ++q) {
-                const float2 nb = multi ? ld_vec2(st, cur + sr.j[q]) : lrec2[curL][sr.j[q] - G.rowBegin];
-                s += sr.v[q] * p_next(nb.x, beta, nb.y);
+                for (int q = 0; q < 27; ++q) {
+                    const float2 nb = lrec2[curL][sr.j[q] - G.rowBegin];
+                    s += sr.v[q] * p_next(nb.x, beta, nb.y);
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < 27; ++q) {
+                    const float2 nb = multi ? ld_vec2(st, cur + sr.j[q]) : lrec2[curL][sr.j[q] - G.rowBegin];
+                    s += sr.v[q] * p_next(nb.x, beta, nb.y);
+                }
             }
             pi = p_next(ri, beta, pi);
             api = s;
@@ -827,11 +837,20 @@ __device__ __forceinline__ void cg_mixed_row_body(int psr_bid, int psr_nbid, con
             Comp2 dl{0.f, 0.f, 0.f, 0.f};
             if (own) {
                 float w = 0.f;
+                if (oneWave) {
+                    // a straight-line LDS loop for the one-wave groups
 #pragma unroll
-                for (int q = 0; q < 27; ++q) {
-                    const int jq = lJ[q][threadIdx.x];
-                    const float3 nb = multi ? ld_vec3(st, cur + jq) : lrec3[k & 1][jq - G.rowBegin];
-                    w = __fmaf_rn(lV[q][threadIdx.x], r_next(nb.x, alpha, s_next(nb.y, beta, nb.z)), w);
+                    for (int q = 0; q < 27; ++q) {
+                        const float3 nb = lrec3[k & 1][lJ[q][threadIdx.x] - G.rowBegin];
+                        w = __fmaf_rn(lV[q][threadIdx.x], r_next(nb.x, alpha, s_next(nb.y, beta, nb.z)), w);
+                    }
+                } else {
+#pragma unroll
+                    for (int q = 0; q < 27; ++q) {
+                        const int jq = lJ[q][threadIdx.x];
+                        const float3 nb = multi ? ld_vec3(st, cur + jq) : lrec3[k & 1][jq - G.rowBegin];
+                        w = __fmaf_rn(lV[q][threadIdx.x], r_next(nb.x, alpha, s_next(nb.y, beta, nb.z)), w);
+                    }
                 }
                 const float si = s_next(wo, beta, so);
                 const float rni = r_next(ro, alpha, si);
