@@ -260,18 +260,27 @@ __device__ __forceinline__ void st_vec4(__amdgpu_buffer_rsrc_t r, int idx, float
 // S: slots per workgroup (two payload words each); thread t polls the slots
 // of workgroups t and t + kCgThreads.
 constexpr int kPollPerThread = 2;  // groups of up to kPollPerThread * kCgThreads workgroups
+// Every workgroup owns kSlotStride slots per parity whatever S an exchange
+// uses: the layout of a phase never overlaps that of the other parity, so a
+// workgroup that has passed phase p and writes phase p + 1 cannot overwrite
+// a phase-p slot that a slower workgroup has yet to read.  (With a stride of
+// S, an S = 2 exchange right after an S = 1 one wrote into the S = 1 region
+// of the other parity, and a slow reader could wait for a stamp that was
+// already gone.)
+constexpr int kSlotStride = 2;
 template <int S>
 __device__ __forceinline__ bool slot_exchange(float4* region, int G, int local, unsigned& phase,
                                               const unsigned (&mine)[2 * S], float4 (&got)[kPollPerThread][S],
                                               unsigned spinLimit, int* errFlag) {
+    static_assert(S <= kSlotStride, "slots per workgroup");
     ++phase;
     const __amdgpu_buffer_rsrc_t rs =
-        vec_rsrc(region + size_t(phase & 1u) * G * S, sizeof(float4) * size_t(G) * S);
+        vec_rsrc(region + size_t(phase & 1u) * G * kSlotStride, sizeof(float4) * size_t(G) * kSlotStride);
     const float stamp = __uint_as_float(phase);
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int h = 0; h < S; ++h)
-            st_vec4(rs, local * S + h,
+            st_vec4(rs, local * kSlotStride + h,
                     make_float4(__uint_as_float(mine[2 * h]), __uint_as_float(mine[2 * h + 1]), stamp, 0.f));
     }
     for (unsigned spins = 0;; ++spins) {
@@ -282,7 +291,7 @@ __device__ __forceinline__ bool slot_exchange(float4* region, int G, int local, 
             if (k < G) {
 #pragma unroll
                 for (int h = 0; h < S; ++h) {
-                    got[j][h] = ld_vec4(rs, k * S + h);
+                    got[j][h] = ld_vec4(rs, k * kSlotStride + h);
                     ok &= __float_as_uint(got[j][h].z) == phase ? 1 : 0;
                 }
             }
