@@ -72,15 +72,18 @@ def copy_probe(mib):
 
 
 def measured_traffic(workload, precision, kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
-    summary of the same workload (FETCH_SIZE and WRITE_SIZE passes scaled by a
-    copy kernel of known size), or None when there is none."""
+    """HBM bytes per solve of `kernel` ("a+b": the kernels of a solve that runs
+    as two launches, summed) from the committed rocprofv3 counter summary of the
+    same workload (FETCH_SIZE and WRITE_SIZE passes scaled by a copy kernel of
+    known size), or None when there is none."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic_%s_%s.md" % (PROFILE_TAG.get(workload, workload), precision))
     if not os.path.exists(path):
         return None
     for line in open(path):
         if line.strip().startswith("TRAFFIC "):
-            return json.loads(line.strip()[len("TRAFFIC "):]).get(kernel)
+            t = json.loads(line.strip()[len("TRAFFIC "):])
+            parts = [t.get(k) for k in kernel.split("+")]
+            return None if any(p is None for p in parts) else sum(parts)
     return None
 
 
@@ -123,10 +126,18 @@ def cpu_baseline(xyz, nrm, depth, reps, threads):
                       % (reps, ", ".join("%.0f" % m for m in ms), one)}
 
 
+
+
+
 def cg_kernel_name(st, precision):
+    """The CG kernel(s) of the solve: a mixed solve with several rows per
+    thread and a depth 9 or deeper (the indefinite depth solved with MINRES,
+    in a launch of its own) runs two kernels, named "a+b"."""
     if st["cg_row_form"]:
         return "cg_mixed_row" if precision == "mixed" else "cg_fp32_row"
-    return "cg_mixed_all_depths" if precision == "mixed" else "cg_all_depths"
+    if precision == "mixed":
+        return "cg_mixed_all_depths+cg_mixed_minres" if st["depth"] >= 9 else "cg_mixed_all_depths"
+    return "cg_all_depths"
 
 
 def cg_algorithmic_bytes(st, precision):
