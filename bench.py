@@ -346,6 +346,18 @@ def measure_batch(pkg, reps, per_pass):
             "clouds_per_pass": per_pass or 64, "cg_plan": plan, "reps": reps}
 
 
+def leg(fn, *args):
+    """Runs one of the line's secondary measurements; a library error there
+    is recorded in its place rather than losing the line (the headline
+    measurement is not wrapped)."""
+    import torch
+    try:
+        return fn(*args)
+    except Exception as e:  # noqa: BLE001 - reported in the line
+        torch.cuda.synchronize()
+        return {"error": "%s: %s" % (type(e).__name__, e)}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -408,14 +420,14 @@ def main():
             line["roofline_fixed_k"] = fixed_k_roofline(f["stats"], h["ms_per_step"])
             line["roofline_fixed_k"]["frac_fp32_form"] = fixed_k_roofline(f["stats"], f["ms_per_step"])["frac"]
     if a.weld_reps > 0:
-        line["weld"] = measure_weld(pkg, tx, tn, depth, a.weld_reps)
-        line["point_normals"] = measure_point_normals(pkg, tx, a.weld_reps)
+        line["weld"] = leg(measure_weld, pkg, tx, tn, depth, a.weld_reps)
+        line["point_normals"] = leg(measure_point_normals, pkg, tx, a.weld_reps)
     if a.batch_reps > 0:
-        line["batch64"] = measure_batch(pkg, a.batch_reps, a.batch_pass)
+        line["batch64"] = leg(measure_batch, pkg, a.batch_reps, a.batch_pass)
     if a.d9_steps > 0 and a.workload != "displaced":
-        line["depth9"] = measure_depth9(pkg, a.d9_steps)
+        line["depth9"] = leg(measure_depth9, pkg, a.d9_steps)
     if a.serve_requests > 0:
-        line["serve"] = measure_serve(pkg, a.serve_requests, a.serve_rate)
+        line["serve"] = leg(measure_serve, pkg, a.serve_requests, a.serve_rate)
     if a.cpu_sample > 0:
         line["cpu_baseline"] = cpu_baseline(xyz, nrm, depth, a.cpu_sample, a.cpu_threads)
     else:
