@@ -725,6 +725,43 @@ __device__ __forceinline__ Comp2 block_comp2(Comp2 v, Comp2* sh) {
     return t;
 }
 
+// The row form's two dot products, gamma = r.r and delta = w.r, summed in
+// FP64: a lane's product of two floats is exact in FP64, and the wave, block
+// and group sums take the order of wave_sum_f64, block_sum_double and
+// slot_sum_double, so gamma of a right-hand side b is ||b||^2 to the bit.
+struct Dots {
+    double g, d;
+};
+__device__ __forceinline__ Dots wave_sum_dots(Dots v) {
+    v.g += dpp_f64<kDppSwap1>(v.g);
+    v.d += dpp_f64<kDppSwap1>(v.d);
+    v.g += dpp_f64<kDppSwap2>(v.g);
+    v.d += dpp_f64<kDppSwap2>(v.d);
+    v.g += dpp_f64<kDppHalfMirror>(v.g);
+    v.d += dpp_f64<kDppHalfMirror>(v.d);
+    v.g += dpp_f64<kDppMirror>(v.g);
+    v.d += dpp_f64<kDppMirror>(v.d);
+    return Dots{(readlane_f64(v.g, 0) + readlane_f64(v.g, 16)) + (readlane_f64(v.g, 32) + readlane_f64(v.g, 48)),
+                (readlane_f64(v.d, 0) + readlane_f64(v.d, 16)) + (readlane_f64(v.d, 32) + readlane_f64(v.d, 48))};
+}
+// fenceStores: the block's earlier stores complete before its barrier (a
+// workgroup barrier alone does not wait for them); the wave's sum is formed
+// while they are in flight.
+__device__ __forceinline__ Dots block_sum_dots(Dots v, double2* sh, bool fenceStores = false) {
+    v = wave_sum_dots(v);
+    if (fenceStores) __builtin_amdgcn_s_waitcnt(0);
+    const int w = threadIdx.x / PSR_WAVE;
+    if (__lane_id() == 0) sh[w] = make_double2(v.g, v.d);
+    __syncthreads();
+    Dots s{0.0, 0.0};
+    for (int k = 0; k < kCgWaves; ++k) {
+        s.g += sh[k].x;
+        s.d += sh[k].y;
+    }
+    __syncthreads();
+    return s;
+}
+
 // Mixed form, one launch for all depths, one row per thread.  Per inner
 // iteration k the 12-byte record of row i in buffer k & 1 holds {r_k, w_k, s_{k-1}};
 // the row writes {r_{k+1}, w_{k+1}, s_k} to the other buffer, keeps its r, w,
@@ -737,7 +774,7 @@ __device__ __forceinline__ void cg_mixed_row_body(int psr_bid, int psr_nbid, con
                                                            double* __restrict__ resid, double* __restrict__ trueRel,
                                                            int* errFlag) {
     __shared__ double shd[kCgWaves];
-    __shared__ Comp2 shc[kCgWaves];
+    __shared__ double2 shp[kCgWaves];
     __shared__ float lV[27][kCgThreads];  // the row's stencil, read from LDS every sweep
     __shared__ int lJ[27][kCgThreads];
     int gi = 0;
@@ -763,6 +800,9 @@ __device__ __forceinline__ void cg_mixed_row_body(int psr_bid, int psr_nbid, con
             wave_sync_memory();
             return true;
         }
+        // the stores of x and of the outer residual complete before the slot
+        // (as the records before the dots' slot)
+        if (multi) __builtin_amdgcn_s_waitcnt(0);
         double s = block_sum_double(v, shd);
         if (!multi) {
             out = s;
@@ -770,27 +810,33 @@ __device__ __forceinline__ void cg_mixed_row_body(int psr_bid, int psr_nbid, con
         }
         return slot_sum_double(s, out, region, G.wgs, local, phase, P.spinLimit, errFlag, shd);
     };
-    auto sum_comp = [&](Comp2 v, Comp2& out) -> bool {
+    auto sum_dots = [&](Dots v, Dots& out) -> bool {
         if (oneWave) {
-            out = wave_comp2(v);
+            out = wave_sum_dots(v);
             wave_sync_memory();
             return true;
         }
-        Comp2 s = block_comp2(v, shc);
+        // Every wave's record stores complete before the barrier, so the slot
+        // stored after it is never seen before them: a workgroup that has seen
+        // every slot of a phase reads the records of that phase.
+        const Dots s = block_sum_dots(v, shp, multi);
         if (!multi) {
             out = s;
             return true;
         }
-        const unsigned mine[4] = {__float_as_uint(s.gs), __float_as_uint(s.gc), __float_as_uint(s.ds),
-                                  __float_as_uint(s.dc)};
+        const unsigned long long ug = static_cast<unsigned long long>(__double_as_longlong(s.g));
+        const unsigned long long ud = static_cast<unsigned long long>(__double_as_longlong(s.d));
+        const unsigned mine[4] = {unsigned(ug), unsigned(ug >> 32), unsigned(ud), unsigned(ud >> 32)};
         float4 got[kPollPerThread][2];
         if (!slot_exchange<2>(region, G.wgs, local, phase, mine, got, P.spinLimit, errFlag)) return false;
-        Comp2 t{0.f, 0.f, 0.f, 0.f};
+        Dots t{0.0, 0.0};
 #pragma unroll
         for (int j = 0; j < kPollPerThread; ++j)
-            if (int(threadIdx.x) + j * kCgThreads < G.wgs)
-                comp2_merge(t, Comp2{got[j][0].x, got[j][0].y, got[j][1].x, got[j][1].y});
-        out = block_comp2(t, shc);
+            if (int(threadIdx.x) + j * kCgThreads < G.wgs) {
+                t.g += __hiloint2double(__float_as_int(got[j][0].y), __float_as_int(got[j][0].x));
+                t.d += __hiloint2double(__float_as_int(got[j][1].y), __float_as_int(got[j][1].x));
+            }
+        out = block_sum_dots(t, shp);
         return true;
     };
     if (own) {
@@ -803,25 +849,27 @@ __device__ __forceinline__ void cg_mixed_row_body(int psr_bid, int psr_nbid, con
         }
     }
     float xi = 0.f, ro = 0.f, wo = 0.f, so = 0.f, po = 0.f;
-    // x = 0; ||b||^2 in FP64 (the outer residual of x = 0 is b itself)
-    double loc = 0;
-    if (own) {
-        st_shared(&x[i], 0.f);
-        const double bi = b[i];
-        loc = bi * bi;
+    // x = 0.  ||b||^2 in FP64 (the outer residual of x = 0 is b itself) is
+    // gamma of the first outer step, whose sum is bit for bit that of b.b;
+    // a solve that takes no step sums it alone.
+    if (own) st_shared(&x[i], 0.f);
+    double bb = 0;
+    const bool stepping = P.maxOuter > 0 && P.maxIter > 0;
+    if (!stepping) {
+        double loc = 0;
+        if (own) {
+            const double bi = b[i];
+            loc = bi * bi;
+        }
+        if (!sum_double(loc, bb)) return;
     }
-    double bb;
-    if (!sum_double(loc, bb)) return;
-    double rr = bb, rel = bb > 0 ? 1.0 : 0.0;
+    double rr = bb, rel = (bb > 0 || stepping) ? 1.0 : 0.0;
     const float* rhs = b;
     int total = 0, outer = 0;
     float gamma = 0.f;
     while (rel > P.relTol && outer < P.maxOuter && total < P.maxIter) {
-        // inner target on the updated residual: half the tolerance left, no tighter than FP32 can follow
-        const double tgt = fmax(0.5 * P.relTol * sqrt(bb), 1e-6 * sqrt(rr));
-        const float tgt2 = float(tgt * tgt);
         // r_0 = rhs, w_0 = A r_0, s_{-1} = 0, p_{-1} = 0
-        Comp2 d0{0.f, 0.f, 0.f, 0.f};
+        Dots d0{0.0, 0.0};
         if (own) {
             float w = 0.f;
 #pragma unroll
@@ -832,18 +880,28 @@ __device__ __forceinline__ void cg_mixed_row_body(int psr_bid, int psr_nbid, con
             po = 0.f;
             if (multi) st_vec3(st, i, make_float3(ro, w, 0.f));
             else lrec3[0][threadIdx.x] = make_float3(ro, w, 0.f);
-            dot2_acc(d0.gs, d0.gc, ro, ro);
-            dot2_acc(d0.ds, d0.dc, w, ro);
+            d0.g = double(ro) * double(ro);
+            d0.d = double(w) * double(ro);
         }
-        Comp2 g;
-        if (!sum_comp(d0, g)) return;
-        gamma = g.gs + g.gc;
-        float delta = g.ds + g.dc;
+        Dots g;
+        if (!sum_dots(d0, g)) return;
+        if (outer == 0) {
+            bb = rr = g.g;
+            if (!(bb > 0)) {
+                rel = 0.0;
+                break;
+            }
+        }
+        // inner target on the updated residual: half the tolerance left, no tighter than FP32 can follow
+        const double tgt = fmax(0.5 * P.relTol * sqrt(bb), 1e-6 * sqrt(rr));
+        const float tgt2 = float(tgt * tgt);
+        gamma = float(g.g);
+        float delta = float(g.d);
         float alpha = gamma / delta, beta = 0.f;
         int k = 0;
         while (gamma > tgt2 && total + k < P.maxIter) {
             const int cur = (k & 1) * M, nxt = M - cur;
-            Comp2 dl{0.f, 0.f, 0.f, 0.f};
+            Dots dl{0.0, 0.0};
             if (own) {
                 float w = 0.f;
                 if (oneWave) {
@@ -863,19 +921,20 @@ __device__ __forceinline__ void cg_mixed_row_body(int psr_bid, int psr_nbid, con
                 }
                 const float si = s_next(wo, beta, so);
                 const float rni = r_next(ro, alpha, si);
+                // the record first: the exchange waits for it
+                if (multi) st_vec3(st, nxt + i, make_float3(rni, w, si));
+                else lrec3[(k & 1) ^ 1][threadIdx.x] = make_float3(rni, w, si);
                 po = __fmaf_rn(beta, po, ro);
                 xi = __fmaf_rn(alpha, po, xi);
                 st_shared(&x[i], xi);
-                if (multi) st_vec3(st, nxt + i, make_float3(rni, w, si));
-                else lrec3[(k & 1) ^ 1][threadIdx.x] = make_float3(rni, w, si);
                 ro = rni;
                 wo = w;
                 so = si;
-                dot2_acc(dl.gs, dl.gc, rni, rni);
-                dot2_acc(dl.ds, dl.dc, w, rni);
+                dl.g = double(rni) * double(rni);
+                dl.d = double(w) * double(rni);
             }
-            if (!sum_comp(dl, g)) return;
-            const float gNew = g.gs + g.gc, dNew = g.ds + g.dc;
+            if (!sum_dots(dl, g)) return;
+            const float gNew = float(g.g), dNew = float(g.d);
             const float bNew = gNew / gamma;
             alpha = gNew / (dNew - bNew * gNew / alpha);
             beta = bNew;
@@ -885,7 +944,7 @@ __device__ __forceinline__ void cg_mixed_row_body(int psr_bid, int psr_nbid, con
         total += k;
         ++outer;
         // outer residual in FP64
-        loc = 0;
+        double loc = 0;
         if (own) {
             double acc = b[i];
 #pragma unroll
