@@ -309,16 +309,18 @@ def measure_point_normals(pkg, tx, reps):
     return {"ms": dt * 1e3 / reps, "points": int(n), "k": 10, "reps": reps}
 
 
-def measure_batch(pkg, reps, per_pass):
+def measure_batch(pkg, reps, per_pass, precision="fp32"):
     """The 64-cloud batch (sizes log-uniform in [5k, 150k], ellipsoids and tori,
     depth 7): 64 sequential single-cloud calls against one batched call (one
-    launch per stage for all clouds).  The batch's warm-up calls let its tuner
-    try its CG plans (one per call) and settle on the fastest."""
+    launch per stage for all clouds), both in the given precision form.  The
+    batch's warm-up calls let its tuner try its CG plans (one per call after an
+    untimed first) and settle on the fastest."""
     import torch
+    prec = pkg.binding.PRECISION_MIXED if precision == "mixed" else pkg.binding.PRECISION_FP32
     clouds = pkg.synth.batch64()
     dev = [(torch.from_numpy(x).cuda(), torch.from_numpy(n).cuda()) for x, n in clouds]
     torch.cuda.synchronize()
-    single = pkg.PoissonReconstruction(depth=7)
+    single = pkg.PoissonReconstruction(depth=7, precision=prec)
     for x, n in dev[:4]:
         single.SolvePoissionReconstructionMesh(x, n)
     torch.cuda.synchronize()
@@ -329,7 +331,7 @@ def measure_batch(pkg, reps, per_pass):
     torch.cuda.synchronize()
     seq = (time.perf_counter() - t0) / reps
     single.close()
-    batch = pkg.PoissonReconstruction(depth=7, batch_workers=per_pass)
+    batch = pkg.PoissonReconstruction(depth=7, precision=prec, batch_workers=per_pass)
     for _ in range(10):
         batch.ReconstructBatch(dev)
     torch.cuda.synchronize()
@@ -340,8 +342,8 @@ def measure_batch(pkg, reps, per_pass):
     bat = (time.perf_counter() - t0) / reps
     plan = batch.batch_cg_plan()
     batch.close()
-    return {"workload": "64 clouds, 5k-150k points (%d total), ellipsoids and tori, depth 7, FP32"
-                        % sum(int(x.shape[0]) for x, _ in dev),
+    return {"workload": "64 clouds, 5k-150k points (%d total), ellipsoids and tori, depth 7, %s"
+                        % (sum(int(x.shape[0]) for x, _ in dev), "mixed" if precision == "mixed" else "FP32"),
             "sequential_recon_per_s": 64 / seq, "batched_recon_per_s": 64 / bat, "speedup": seq / bat,
             "clouds_per_pass": per_pass or 64, "cg_plan": plan, "reps": reps}
 
@@ -424,6 +426,7 @@ def main():
         line["point_normals"] = leg(measure_point_normals, pkg, tx, a.weld_reps)
     if a.batch_reps > 0:
         line["batch64"] = leg(measure_batch, pkg, a.batch_reps, a.batch_pass)
+        line["batch64_mixed"] = leg(measure_batch, pkg, a.batch_reps, a.batch_pass, "mixed")
     if a.d9_steps > 0 and a.workload != "displaced":
         line["depth9"] = leg(measure_depth9, pkg, a.d9_steps)
     if a.serve_requests > 0:

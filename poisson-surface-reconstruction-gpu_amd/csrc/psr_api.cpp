@@ -246,6 +246,7 @@ int psr_create(const psr_config* cfg, void* hip_stream, psr_ctx** out) {
         PSR_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&c.hostResults), sizeof(psr::DevResults),
                                     hipHostMallocMapped | hipHostMallocCoherent));
         PSR_HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&c.mailResults), c.hostResults, 0));
+        std::memset(c.hostResults, 0, sizeof(psr::DevResults));
         psr::build_tables(c);  // one-time, like ComputeVectorField's constructor (ComputeVectorField.cpp:11-15)
         PSR_HIP_CHECK(hipStreamSynchronize(c.stream));
     } catch (const psr::HipError& e) {

@@ -37,7 +37,8 @@ namespace {
 
 // Online tuner of the CG plan: for each batch shape it runs the candidate
 // plans once each, in order, timing the phase that holds the solves (GPU time
-// between events), then keeps the fastest.
+// between events), then keeps the fastest.  The shape's first call runs the
+// first candidate untimed.
 struct BatchTuner {
     // 0: the single-cloud plan (256 rows a workgroup); w: all solves in w
     // waves of the resident budget
@@ -47,6 +48,7 @@ struct BatchTuner {
         float ms[kCount];
         int tried = 0;
         int best = -1;
+        bool warm = false;  // the shape's first call, untimed: its CG queue order had no history
     };
     std::map<unsigned long long, Entry> table;
     static unsigned long long key(int precision, int depth, size_t clouds, size_t points) {
@@ -68,6 +70,10 @@ struct BatchTuner {
     }
     void report(unsigned long long k, int idx, float ms) {
         Entry& e = table[k];
+        if (!e.warm) {
+            e.warm = true;
+            return;
+        }
         if (e.best >= 0 || idx != e.tried || !(ms > 0.f)) return;
         e.ms[idx] = ms;
         if (++e.tried == kCount) e.best = int(std::min_element(e.ms, e.ms + kCount) - e.ms);
@@ -140,7 +146,18 @@ int Batch::resident(const void* fn, unsigned block, unsigned shmem) {
 // Merges the clouds' recordings (slot[k]: the cloud of recording k) into
 // launches of the batched kernels and queues them.
 void Batch::flush(Recorder& rec, const std::vector<int>& slot, bool timeCg) {
-    (void)slot;
+    // The CG's queue takes the clouds' groups in launch order, and the launch
+    // lasts as long as the group that starts last plus its solve: the clouds
+    // whose previous solve in this context took the most iterations go first
+    // (longest first; the same order while the batch repeats).  A context's
+    // first call has no history and keeps the batch order.
+    std::vector<long long> cgWeight(rec.clouds.size(), 0);
+    for (size_t k = 0; k < rec.clouds.size() && k < slot.size(); ++k) {
+        const Ctx& c = clouds_[size_t(slot[k])]->c;
+        if (c.runs == 0) continue;
+        for (int d = 0; d <= std::min(c.cfg.depth, PSR_MAX_DEPTH); ++d)
+            cgWeight[k] = std::max<long long>(cgWeight[k], c.hostResults->cgIters[d]);
+    }
     // launch order: every cloud's launches in its own order; a launch site
     // (and occurrence) a cloud skipped is placed right after that cloud's
     // previous launch
@@ -168,12 +185,18 @@ void Batch::flush(Recorder& rec, const std::vector<int>& slot, bool timeCg) {
     auto align = [&](size_t a) { blob.resize((blob.size() + a - 1) / a * a); };
     std::vector<Merged> merged;
     std::vector<const LaunchRec*> items;
+    std::vector<std::pair<long long, const LaunchRec*>> weighted;
     for (const auto& key : order) {
         items.clear();
-        for (const auto& v : rec.clouds)
-            for (const LaunchRec& r : v)
-                if (r.site == key.first && r.occurrence == key.second && r.grid > 0) items.push_back(&r);
-        if (items.empty()) continue;
+        weighted.clear();
+        for (size_t k = 0; k < rec.clouds.size(); ++k)
+            for (const LaunchRec& r : rec.clouds[k])
+                if (r.site == key.first && r.occurrence == key.second && r.grid > 0) weighted.emplace_back(cgWeight[k], &r);
+        if (weighted.empty()) continue;
+        if (weighted[0].second->coResident)
+            std::stable_sort(weighted.begin(), weighted.end(),
+                             [](const auto& a, const auto& b) { return a.first > b.first; });
+        for (const auto& w : weighted) items.push_back(w.second);
         const LaunchRec& r0 = *items[0];
         unsigned shmem = 0;
         for (const LaunchRec* r : items) {

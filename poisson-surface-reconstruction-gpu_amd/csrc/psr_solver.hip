@@ -49,6 +49,14 @@ constexpr int kCgFp32GenericWaves = 4, kCgMixedGenericWaves = 3;
 // The MINRES kernel (indefinite depths of the mixed form) carries only its
 // own path and is compiled for 4 waves per SIMD.
 constexpr int kCgMinresWaves = 4;
+// The mixed form's inner CG ends when its residual has not reached a new
+// minimum for this many iterations, and its solution goes back to the
+// iterate of the smallest residual.  A definite depth converges long before
+// (20-70 iterations here); an operator that the reference's neighbour rules
+// leave nonsymmetric or singular at a coarse depth (seen on ellipsoids of the
+// 64-cloud batch) lets CG wander without converging, and the FP64 oracle's
+// own CG diverges there.
+constexpr int kCgStall = 128;
 
 // Vectors read by other workgroups (r, p) and the reduction partials are
 // accessed with agent-scope relaxed atomics, which are coherent across the
@@ -898,8 +906,9 @@ __device__ __forceinline__ void cg_mixed_row_body(int psr_bid, int psr_nbid, con
         gamma = float(g.g);
         float delta = float(g.d);
         float alpha = gamma / delta, beta = 0.f;
-        int k = 0;
-        while (gamma > tgt2 && total + k < P.maxIter) {
+        int k = 0, kBest = 0;
+        float gBest = gamma, xBest = xi;
+        while (gamma > tgt2 && total + k < P.maxIter && k - kBest < kCgStall) {
             const int cur = (k & 1) * M, nxt = M - cur;
             Dots dl{0.0, 0.0};
             if (own) {
@@ -940,9 +949,23 @@ __device__ __forceinline__ void cg_mixed_row_body(int psr_bid, int psr_nbid, con
             beta = bNew;
             gamma = gNew;
             ++k;
+            if (gamma < gBest) {  // the same in every workgroup of the group
+                gBest = gamma;
+                kBest = k;
+                xBest = xi;
+            }
         }
         total += k;
         ++outer;
+        if (kBest < k) {
+            // back to the iterate of the smallest residual; one more exchange
+            // orders the restored x before the outer residual reads it
+            gamma = gBest;
+            xi = xBest;
+            if (own) st_shared(&x[i], xi);
+            double fence;
+            if (!sum_double(0.0, fence)) return;
+        }
         // outer residual in FP64
         double loc = 0;
         if (own) {
@@ -956,6 +979,9 @@ __device__ __forceinline__ void cg_mixed_row_body(int psr_bid, int psr_nbid, con
         if (!sum_double(loc, rr)) return;
         rel = bb > 0 ? sqrt(rr / bb) : 0.0;
         rhs = rOut;
+        // an inner solve that found no smaller residual leaves x as it was:
+        // another outer step would repeat it
+        if (kBest == 0) break;
     }
     if (local == 0 && threadIdx.x == 0) {
         iters[G.depth] = total;
@@ -1026,8 +1052,9 @@ __device__ __forceinline__ void cg_small_mixed(const CgParams& P, const CgGroup&
         gamma = g.gs + g.gc;
         const float delta = g.ds + g.dc;
         float alpha = gamma / delta, beta = 0.f;
-        int k = 0;
-        while (gamma > tgt2 && total + k < P.maxIter) {
+        int k = 0, kBest = 0;
+        float gBest = gamma, xBest = own ? lx[t] : 0.f;
+        while (gamma > tgt2 && total + k < P.maxIter && k - kBest < kCgStall) {
             const int c = k & 1, n = c ^ 1;  // r_k, w_k in c; s_{k-1} in n
             Comp2 dl{0.f, 0.f, 0.f, 0.f};
             if (own) {
@@ -1056,9 +1083,19 @@ __device__ __forceinline__ void cg_small_mixed(const CgParams& P, const CgGroup&
             beta = bNew;
             gamma = gNew;
             ++k;
+            if (gamma < gBest) {
+                gBest = gamma;
+                kBest = k;
+                if (own) xBest = lx[t];
+            }
         }
         total += k;
         ++outer;
+        if (kBest < k) {  // back to the iterate of the smallest residual
+            gamma = gBest;
+            if (own) lx[t] = xBest;
+            __syncthreads();
+        }
         // outer residual in FP64
         loc = 0;
         if (own) {
@@ -1074,6 +1111,7 @@ __device__ __forceinline__ void cg_small_mixed(const CgParams& P, const CgGroup&
         rr = block_sum_double(loc, shd);
         rel = bb > 0 ? sqrt(rr / bb) : 0.0;
         fromB = false;
+        if (kBest == 0) break;  // no smaller residual: another outer step would repeat this one
     }
     if (own) x[i] = lx[t];
     if (threadIdx.x == 0) {
@@ -1156,6 +1194,7 @@ __device__ __forceinline__ void cg_mixed_generic_body(int psr_bid, int psr_nbid,
     const float* rhs = b;
     int total = 0, outer = 0;
     float gamma = 0.f;
+    bool stalled = false;
     while (rel > P.relTol && outer < P.maxOuter && total < P.maxIter) {
         // inner target on the updated residual: half the tolerance left, no tighter than FP32 can follow
         const double tgt = fmax(0.5 * P.relTol * sqrt(bb), 1e-6 * sqrt(rr));
@@ -1286,8 +1325,12 @@ __device__ __forceinline__ void cg_mixed_generic_body(int psr_bid, int psr_nbid,
             gamma = g.gs + g.gc;
             float delta = g.ds + g.dc;
             float alpha = gamma / delta, beta = 0.f;
-            int k = 0;
-            while (gamma > tgt2 && total + k < P.maxIter) {
+            // the iterate of the smallest residual goes to rOut (free until
+            // the outer residual) in the sweep after its reduction found it
+            int k = 0, kBest = 0;
+            float gBest = gamma;
+            bool curBest = true;
+            while (gamma > tgt2 && total + k < P.maxIter && k - kBest < kCgStall) {
                 const float* rk = rbuf + (k & 1) * M;
                 const float* wk = wbuf + (k & 1) * M;
                 const float* sp = sbuf + ((k + 1) & 1) * M;  // s_{k-1}
@@ -1306,7 +1349,9 @@ __device__ __forceinline__ void cg_mixed_generic_body(int psr_bid, int psr_nbid,
                     float rni = r_next(ri, alpha, si);
                     float pi = __fmaf_rn(beta, pvec[i], ri);
                     pvec[i] = pi;
-                    st_shared(&x[i], __fmaf_rn(alpha, pi, ld_shared(&x[i])));
+                    const float xo = ld_shared(&x[i]);
+                    if (curBest) st_shared(&rOut[i], xo);
+                    st_shared(&x[i], __fmaf_rn(alpha, pi, xo));
                     st_shared(&sk[i], si);
                     st_shared(&rn[i], rni);
                     st_shared(&wn[i], w);
@@ -1320,9 +1365,23 @@ __device__ __forceinline__ void cg_mixed_generic_body(int psr_bid, int psr_nbid,
                 beta = bNew;
                 gamma = gNew;
                 ++k;
+                curBest = gamma < gBest;
+                if (curBest) {
+                    gBest = gamma;
+                    kBest = k;
+                }
             }
             total += k;
             ++outer;
+            if (!curBest) {
+                // back to the iterate of the smallest residual (kept in rOut);
+                // one more reduction orders it before the outer residual
+                gamma = gBest;
+                for (int i = r0 + threadIdx.x; i < r1; i += kCgThreads) st_shared(&x[i], ld_shared(&rOut[i]));
+                double fence;
+                if (!sum_double(0.0, fence)) return;
+            }
+            stalled = kBest == 0;
         }
         // outer residual in FP64
         loc = 0;
@@ -1337,6 +1396,7 @@ __device__ __forceinline__ void cg_mixed_generic_body(int psr_bid, int psr_nbid,
         if (!sum_double(loc, rr)) return;
         rel = bb > 0 ? sqrt(rr / bb) : 0.0;
         rhs = rOut;
+        if (stalled) break;  // no smaller residual: another outer step would repeat this one
     }
     if (local == 0 && threadIdx.x == 0) {
         iters[G.depth] = total;

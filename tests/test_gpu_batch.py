@@ -78,11 +78,13 @@ def test_batch64_tuned_plans_within_tolerance(pkg, torch, clouds64, singles):
     host, dev = clouds64
     batch = pkg.PoissonReconstruction(depth=7)
     plans = []
-    for call in range(7):  # the tuner tries its 5 plans, then keeps the fastest
+    # the shape's first call is untimed (plan 0), then the tuner tries its 5
+    # plans and keeps the fastest
+    for call in range(8):
         meshes = batch.ReconstructBatch(dev)
         plans.append(batch.batch_cg_plan())
         print("batch call", call, "plan", plans[-1], flush=True)
-        check = range(64) if call in (1, 6) else range(0, 64, 9)
+        check = range(64) if call in (2, 7) else range(0, 64, 9)
         for i in check:
             view = batch.batch_cloud(i)
             np.testing.assert_array_equal(view.export("NODE_NEIGHS"), singles[i]["NODE_NEIGHS"])
@@ -99,7 +101,7 @@ def test_batch64_tuned_plans_within_tolerance(pkg, torch, clouds64, singles):
                 diag = float(np.linalg.norm(np.ptp(sv, axis=0)))
                 h = max(cKDTree(sv).query(v)[0].max(), cKDTree(v).query(sv)[0].max())
                 assert h <= 1e-4 * diag, (i, h)
-    assert plans[:5] == [0, 1, 2, 4, 8] and plans[5] == plans[6]
+    assert plans[:6] == [0, 0, 1, 2, 4, 8] and plans[6] == plans[7]
     batch.close()
 
 

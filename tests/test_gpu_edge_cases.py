@@ -10,6 +10,8 @@ headline configurations do not reach, through the C ABI.
     vertex per edge and no triangle, as in the oracle;
   * the CG forms with several rows per thread (cg_all_depths,
     cg_mixed_all_depths), forced by a wide row block;
+  * the mixed CG's stall rule on a depth whose operator the reference's
+    neighbour rules leave singular (cloud 36 of the 64-cloud batch);
   * depth 8 (the dense d_x grids built after the mesh wait) against the oracle;
   * the 2M-point depth-9 displaced sphere: integer stages bit-exact against
     the oracle, and the reference's CG rule reported, not raised, at depth 9
@@ -150,6 +152,42 @@ def test_generic_cg_forms(generic_form, oracle_runs, name, prec):
     diag = float(np.linalg.norm(np.ptp(ref.array("spos").reshape(-1, 3), axis=0)))
     h = max(cKDTree(ov).query(v)[0].max(), cKDTree(v).query(ov)[0].max())
     assert h <= 1e-4 * diag, h
+
+
+@pytest.mark.parametrize("rows", [None, "128", "512"])
+def test_mixed_cg_stall_keeps_the_best_iterate(pkg, torch, rows):
+    """Cloud 36 of the 64-cloud batch: the level-local child 0 rule leaves its
+    depth-2 and depth-3 operators nonsymmetric and singular (the FP64 oracle's
+    CG diverges there).  The mixed form's inner CG stops once its residual has
+    not reached a new minimum for 128 iterations and keeps the iterate of the
+    smallest residual, instead of running to the 10000-iteration cap.  Row
+    form (one-wave group at depth 2, one workgroup at depth 3), 128 rows a
+    workgroup (two workgroups at depth 3) and the generic kernel (512 rows a
+    workgroup: its one-workgroup path at depths 2 and 3)."""
+    import residual_host
+    xyz, nrm = pkg.synth.batch64()[36]
+    if rows:
+        os.environ["PSR_CG_ROWS_PER_WG"] = rows
+    try:
+        rec = pkg.PoissonReconstruction(depth=7, precision=pkg.binding.PRECISION_MIXED)
+        rec.SolvePoissionReconstructionMesh(_up(torch, xyz), _up(torch, nrm))
+    finally:
+        os.environ.pop("PSR_CG_ROWS_PER_WG", None)
+    st = rec.stats()
+    it, outer = list(st["cg_iterations"])[:8], list(st["cg_outer"])[:8]
+    rep = np.array(list(st["cg_true_relres"])[:8])
+    host = np.array(residual_host.true_relres(rec, 7))
+    print("rows", rows, "row form", st["cg_row_form"], "iterations", it, "outer", outer, flush=True)
+    print("reported relres", rep, "host relres", host, flush=True)
+    assert st["cg_row_form"] == (0 if rows == "512" else 1)
+    for d in (2, 3):
+        assert it[d] < 10000 and outer[d] <= 4, (d, it[d], outer[d])
+        # the kept iterate is no worse than x = 0 and the report is its residual
+        assert np.isfinite(rep[d]) and rep[d] <= 1.0 + 1e-3, (d, rep[d])
+        assert abs(host[d] - rep[d]) <= 1e-3 * max(rep[d], 1e-6), (d, host[d], rep[d])
+    for d in (0, 1, 4, 5, 6, 7):
+        assert host[d] <= 1e-5, (d, host[d])
+    rec.close()
 
 
 def test_depth8_against_oracle(pkg, orc, torch):
