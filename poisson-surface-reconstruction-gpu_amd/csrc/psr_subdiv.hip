@@ -787,6 +787,10 @@ struct SmallDiv {
 constexpr int kLdsWordsGrid = 7168;  // upper bound of grid_tile_words for depths up to 9
 constexpr int kLdsWordsSmallGrid = 12288;  // upper bound of small_tile_words for depths up to 10
 constexpr int kHaloPerThread = 4;     // halo entries per thread (<= 1024 for depths up to 9)
+// The single-cloud kernel at 7 waves per SIMD (72 registers, 3 spilled): 8%
+// faster than at 5 (82 registers).  The batched form, which would spill 25
+// there, keeps the compiler's choice (the same batch time).
+constexpr int kCornerGridWaves = 7;
 __device__ __forceinline__ void sub_corner_values_grid_body(int psr_bid, int psr_nbid, const Grid& g, const Tile* __restrict__ tiles, int nVD,
                                                                    const float* __restrict__ iso,
                                                                    float* __restrict__ cval) {
@@ -796,10 +800,12 @@ __device__ __forceinline__ void sub_corner_values_grid_body(int psr_bid, int psr
     __shared__ int lChain[kChain];
     __shared__ TileRoot lRoot[1];
     __shared__ unsigned lSet;
+    __shared__ unsigned lNz;  // bit l: some halo d_x of depth l is nonzero
     float* lds = reinterpret_cast<float*>(ldsDyn);
     const Tile T = tiles[psr_bid];
     if (threadIdx.x < 27) lNbr[threadIdx.x] = g.rootNbr[27 * size_t(T.root) + threadIdx.x];
     if (threadIdx.x == 32) lSet = g.rootSet[T.root];
+    if (threadIdx.x == 33) lNz = 0u;
     if (threadIdx.x < kChain) lChain[threadIdx.x] = g.rootChain[kChain * size_t(T.root) + threadIdx.x];
     const SetDesc S = g.sets[T.set];
     const int D = g.D;
@@ -842,6 +848,7 @@ __device__ __forceinline__ void sub_corner_values_grid_body(int psr_bid, int psr
     }
     // halo d_x, depth D first: 0 inside the roots of the set and where no node is;
     // a thread's loads are issued together
+    unsigned nz = 0;
 #pragma unroll
     for (int j = 0; j < kHaloPerThread; ++j) {
         const int t = threadIdx.x + j * kThreads;
@@ -869,7 +876,9 @@ __device__ __forceinline__ void sub_corner_values_grid_body(int psr_bid, int psr
             }
         }
         lHdx[t] = v;
+        if (v != 0.f) nz |= 1u << l;
     }
+    if (nz) atomicOr(&lNz, nz);
     // halo factor tables: per depth, node cell (box-relative) x corner coordinate
     for (int t = threadIdx.x; t < facWords; t += kThreads) {
         int u = t, l = D;
@@ -897,27 +906,37 @@ __device__ __forceinline__ void sub_corner_values_grid_body(int psr_bid, int psr
         const float c = (0.5f + float(cell)) / invw;
         lFac[t] = bspline((float(corner) * inv - c) * invw);
     }
-    // the corners each cell owns; a cell on a face of the root's cube may own
-    // several, so the (cell, corner) items are compacted and shared out over
-    // the block rather than evaluated by the owning lane
-    unsigned mask = 0;
-    const int sh = D - S.rd, side = 1 << sh;
-    if (int(threadIdx.x) < T.count) {
-        const Leaf L = leaf_of(g, T.set, S, R, T.start + threadIdx.x);
-        const RecVE ve = rec_ve(lNbr, lSet, L, D);
-        const int3 lc = make_int3(L.c.x - (R.Rc.x << sh), L.c.y - (R.Rc.y << sh), L.c.z - (R.Rc.z << sh));
-        for (int b = 0; b < 8; ++b) {
-            const int3 o = bourke_offset(b);
-            // a corner all of whose cells lie in the root's cube is owned by its lowest cell
-            const bool inner = lc.x + o.x > 0 && lc.x + o.x < side && lc.y + o.y > 0 && lc.y + o.y < side &&
-                               lc.z + o.z > 0 && lc.z + o.z < side;
-            if (inner ? (o.x + o.y + o.z == 3) : owns_corner(L, b, ve)) mask |= 1u << b;
+    // The (cell, corner) items of the tile, from its corner points: a point is
+    // owned by the virtual cell of smallest key among the 8 sharing it
+    // (owner_of_point; owns_corner asks the same of one cell), and kept when
+    // that cell is one of the tile's.  One owner search per point rather than
+    // per cell and corner: a cell on a face of the root's cube may own several
+    // corners, and the search for the points on those faces diverges.  The
+    // items are compacted and shared out over the block.
+    const Leaf F = root_leaf(g, S, T.set, R);
+    const RecVE ve = rec_ve(lNbr, lSet, F, D);
+    constexpr int kPoints = kTx * kTy * kTz, kPointsPerThread = (kPoints + kThreads - 1) / kThreads;
+    int item[kPointsPerThread];
+    int nOwn = 0;
+#pragma unroll
+    for (int j = 0; j < kPointsPerThread; ++j) {
+        const int pt = int(threadIdx.x) + j * kThreads;
+        item[j] = -1;
+        if (pt < kPoints) {
+            const int3 p = make_int3(lo.x + pt / (kTy * kTz), lo.y + (pt / kTz) % kTy, lo.z + pt % kTz);
+            int slot;
+            const int e = owner_of_point(F, p, slot, ve);
+            if (e >= T.start && e < T.start + T.count) {
+                item[j] = ((e - T.start) << 3) | slot;
+                ++nOwn;
+            }
         }
     }
     int first;
-    const int nItems = block_excl_scan<kThreads>(__popc(mask), first);
-    for (int b = 0; b < 8; ++b)
-        if ((mask >> b) & 1u) lItem[first++] = (threadIdx.x << 3) | b;
+    const int nItems = block_excl_scan<kThreads>(nOwn, first);
+#pragma unroll
+    for (int j = 0; j < kPointsPerThread; ++j)
+        if (item[j] >= 0) lItem[first++] = (unsigned short)item[j];
     __syncthreads();
     const float isoV = *iso;
     for (int it = threadIdx.x; it < nItems; it += kThreads) {
@@ -936,7 +955,9 @@ __device__ __forceinline__ void sub_corner_values_grid_body(int psr_bid, int psr
             const bool interior = ((cl.x - 1) >> s) == L.Rc.x && ((cl.x + 1) >> s) == L.Rc.x &&
                                   ((cl.y - 1) >> s) == L.Rc.y && ((cl.y + 1) >> s) == L.Rc.y &&
                                   ((cl.z - 1) >> s) == L.Rc.z && ((cl.z + 1) >> s) == L.Rc.z;
-            if (!interior) {
+            // a depth whose halo d_x are all zero adds only zero terms, which
+            // leave the sum as it is (it never holds -0): skipped
+            if (!interior && ((lNz >> l) & 1u)) {
                 const int3 bl = make_int3((lo.x >> (D - l)) - 1, (lo.y >> (D - l)) - 1, (lo.z >> (D - l)) - 1);
                 const float* fx = lFac + fb;
                 const float* fy = fx + bx * kTx;
@@ -965,7 +986,7 @@ __device__ __forceinline__ void sub_corner_values_grid_body(int psr_bid, int psr
         cval[size_t(b) * nVD + e] = val - isoV;
     }
 }
-__global__ void __launch_bounds__(kThreads) sub_corner_values_grid(Grid g, const Tile* __restrict__ tiles, int nVD,
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kCornerGridWaves))) sub_corner_values_grid(Grid g, const Tile* __restrict__ tiles, int nVD,
                                                                    const float* __restrict__ iso,
                                                                    float* __restrict__ cval) {
     sub_corner_values_grid_body(blockIdx.x, gridDim.x, g, tiles, nVD, iso, cval);
