@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0, help="OpenMP threads of the CPU baseline (0: all usable, capped by OMP_NUM_THREADS)")
     ap.add_argument("--batch-reps", type=int, default=3,
                     help="timed repetitions of the 64-cloud batch, sequential and batched (0: skip)")
+    ap.add_argument("--batch-forms", default="fp32,mixed",
+                    help="precision forms of the 64-cloud batch leg, comma-separated (fp32, mixed)")
     ap.add_argument("--batch-pass", type=int, default=0,
                     help="clouds merged into each launch of the batched form (0: all 64)")
     ap.add_argument("--weld-reps", type=int, default=20,
@@ -425,8 +427,11 @@ def main():
         line["weld"] = leg(measure_weld, pkg, tx, tn, depth, a.weld_reps)
         line["point_normals"] = leg(measure_point_normals, pkg, tx, a.weld_reps)
     if a.batch_reps > 0:
-        line["batch64"] = leg(measure_batch, pkg, a.batch_reps, a.batch_pass)
-        line["batch64_mixed"] = leg(measure_batch, pkg, a.batch_reps, a.batch_pass, "mixed")
+        forms = [f.strip() for f in a.batch_forms.split(",") if f.strip()]
+        if "fp32" in forms:
+            line["batch64"] = leg(measure_batch, pkg, a.batch_reps, a.batch_pass)
+        if "mixed" in forms:
+            line["batch64_mixed"] = leg(measure_batch, pkg, a.batch_reps, a.batch_pass, "mixed")
     if a.d9_steps > 0 and a.workload != "displaced":
         line["depth9"] = leg(measure_depth9, pkg, a.d9_steps)
     if a.serve_requests > 0:
