@@ -204,10 +204,14 @@ class PoissonReconstruction:
         cfg.batch_cg_plan = batch_cg_plan
         self._ctx = ctypes.c_void_p()
         # The context works on a stream of its own, which waits for the
-        # caller's current stream before every call (inputs written by torch
-        # kernels still in flight are read only once they are complete).
+        # caller's current stream before a call while that stream has work in
+        # flight (inputs written by torch kernels still running are read only
+        # once they are complete).
         import torch
         self._stream = torch.cuda.Stream() if stream is None else None
+        # one event, recorded on the caller's stream when the context must wait
+        # for it (an event created per call costs more than the rest of the facade)
+        self._ready = torch.cuda.Event() if stream is None else None
         handle = self._stream.cuda_stream if stream is None else stream
         rc = lib().psr_create(ctypes.byref(cfg), ctypes.c_void_p(handle), ctypes.byref(self._ctx))
         if rc != PSR_OK:
@@ -233,9 +237,16 @@ class PoissonReconstruction:
             pass
 
     def _order_after_caller(self):
+        # The context's stream waits for the caller's current stream only when
+        # that stream still has work in flight: once it is idle, everything
+        # queued on it before this call (the kernels writing the inputs) is
+        # complete, and a wait would only delay the call's first kernel.
         if self._stream is not None:
             import torch
-            self._stream.wait_stream(torch.cuda.current_stream())
+            cur = torch.cuda.current_stream()
+            if not cur.query():
+                self._ready.record(cur)
+                self._stream.wait_event(self._ready)
 
     def solve_ptr(self, xyz_ptr, nrm_ptr, count, stride_bytes=12):
         self._order_after_caller()
