@@ -231,6 +231,8 @@ int psr_create(const psr_config* cfg, void* hip_stream, psr_ctx** out) {
         c.cfg = k;
         // tuning override of the CG row-block size (rows per workgroup)
         if (const char* e = std::getenv("PSR_CG_ROWS_PER_WG")) c.cgRowsPerWg = std::max(0, std::atoi(e));
+        // test override of the CG barriers' spin bound (forces their time-out path)
+        if (const char* e = std::getenv("PSR_CG_SPIN_LIMIT")) c.cgSpinLimit = std::max(0, std::atoi(e));
         PSR_HIP_CHECK(hipGetDevice(&c.device));
         if (hip_stream) {
             c.stream = static_cast<hipStream_t>(hip_stream);

@@ -226,6 +226,7 @@ struct Ctx {
     int cgMaxWgsMinres = 0;      // resident workgroups of the MINRES kernel (indefinite depths, mixed form)
     int cgRowsPerWg = 0;         // rows per CG workgroup (0: default)
     int cgRowTarget = 0;         // batch: rows a CG workgroup takes (0: the single-cloud plan)
+    int cgSpinLimit = -1;        // spin bound of the CG barriers (-1: the default)
     bool indefinite[PSR_MAX_DEPTH + 1] = {};  // the depth's operator after the 1e-6 drop is indefinite
     template <class T>
     T* buf(const char* name, size_t count) {

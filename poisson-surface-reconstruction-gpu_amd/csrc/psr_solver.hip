@@ -1704,6 +1704,7 @@ void solve_depths(Ctx& c) {
         PSR_HIP_CHECK(hipEventRecord(c.ev[8], s));
     }
     if (rowForm && mixed) P.spinLimit = 1u << 22;
+    if (c.cgSpinLimit >= 0) P.spinLimit = PM.spinLimit = unsigned(c.cgSpinLimit);
     static const char siteMixedRow = 0, siteMixedGen = 0, siteMinres = 0, siteRow = 0, siteGen = 0;
     if (mixed) {
         P.relTol = PM.relTol = c.cfg.refine_rel_tol;

@@ -12,6 +12,7 @@ headline configurations do not reach, through the C ABI.
     cg_mixed_all_depths), forced by a wide row block;
   * the mixed CG's stall rule on a depth whose operator the reference's
     neighbour rules leave singular (cloud 36 of the 64-cloud batch);
+  * the CG barriers' time-out path, forced from the host;
   * depth 8 (the dense d_x grids built after the mesh wait) against the oracle;
   * the 2M-point depth-9 displaced sphere: integer stages bit-exact against
     the oracle, and the reference's CG rule reported, not raised, at depth 9
@@ -188,6 +189,30 @@ def test_mixed_cg_stall_keeps_the_best_iterate(pkg, torch, rows):
     for d in (0, 1, 4, 5, 6, 7):
         assert host[d] <= 1e-5, (d, host[d])
     rec.close()
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_cg_barrier_time_out_is_reported(pkg, torch, prec):
+    """A CG group barrier that reaches its spin bound (forced from the host: a
+    bound of zero polls) ends every workgroup of the launch, so the call
+    completes and fails with the time-out error instead of hanging; the next
+    call on the same context, with the barriers' bound as forced, fails the
+    same way, and a context with the default bound reconstructs the input."""
+    xyz, nrm = pkg.synth.sphere(n=20000)
+    tx, tn = _up(torch, xyz), _up(torch, nrm)
+    os.environ["PSR_CG_SPIN_LIMIT"] = "0"
+    try:
+        rec = pkg.PoissonReconstruction(depth=6, precision=prec)
+    finally:
+        os.environ.pop("PSR_CG_SPIN_LIMIT", None)
+    for _ in range(2):
+        with pytest.raises(pkg.binding.PsrError, match="CG group barrier timed out"):
+            rec.SolvePoissionReconstructionMesh(tx, tn)
+    rec.close()
+    ok = pkg.PoissonReconstruction(depth=6, precision=prec)
+    mesh = ok.SolvePoissionReconstructionMesh(tx, tn)
+    assert mesh.n_triangles > 0
+    ok.close()
 
 
 def test_depth8_against_oracle(pkg, orc, torch):
