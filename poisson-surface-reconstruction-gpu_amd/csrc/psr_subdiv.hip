@@ -337,6 +337,7 @@ __device__ __forceinline__ int halo_node(const Grid& g, const SetDesc& S, int ro
 //     cells span 4 x 8 x 8, their corners 5 x 9 x 9).
 // The terms and their order are those of the chain walk.
 constexpr int kTx = 5, kTy = 9, kTz = 9, kTw = kTx + kTy + kTz;
+constexpr int kTilePoints = kTx * kTy * kTz;  // corner points of a one-root tile
 constexpr int kMaxRootsPerTile = 32;
 constexpr int kClassifyPoints = 864;  // corner points of a tile: 32 roots of 27, 4 of 125 or one box of 5 x 9 x 9
 constexpr int kLdsWordsOne = 7168;     // one-root tiles: list, halo, vbox, d_x, factors (<= 6954 words)
@@ -793,7 +794,8 @@ constexpr int kHaloPerThread = 4;     // halo entries per thread (<= 1024 for de
 constexpr int kCornerGridWaves = 7;
 __device__ __forceinline__ void sub_corner_values_grid_body(int psr_bid, int psr_nbid, const Grid& g, const Tile* __restrict__ tiles, int nVD,
                                                                    const float* __restrict__ iso,
-                                                                   float* __restrict__ cval) {
+                                                                   float* __restrict__ cval,
+                                                                   unsigned* __restrict__ ptOwner) {
     extern __shared__ float4 ldsDyn[];  // grid_tile_words(D, rd) words
     __shared__ unsigned short lItem[kThreads * 8];  // owned (cell << 3 | corner), cell order
     __shared__ int lNbr[27];
@@ -915,7 +917,7 @@ __device__ __forceinline__ void sub_corner_values_grid_body(int psr_bid, int psr
     // items are compacted and shared out over the block.
     const Leaf F = root_leaf(g, S, T.set, R);
     const RecVE ve = rec_ve(lNbr, lSet, F, D);
-    constexpr int kPoints = kTx * kTy * kTz, kPointsPerThread = (kPoints + kThreads - 1) / kThreads;
+    constexpr int kPoints = kTilePoints, kPointsPerThread = (kPoints + kThreads - 1) / kThreads;
     int item[kPointsPerThread];
     int nOwn = 0;
 #pragma unroll
@@ -926,6 +928,9 @@ __device__ __forceinline__ void sub_corner_values_grid_body(int psr_bid, int psr
             const int3 p = make_int3(lo.x + pt / (kTy * kTz), lo.y + (pt / kTz) % kTy, lo.z + pt % kTz);
             int slot;
             const int e = owner_of_point(F, p, slot, ve);
+            // every point's value index, for sub_classify (when it has the map)
+            if (ptOwner)
+                ptOwner[size_t(psr_bid) * kPoints + pt] = e >= 0 ? unsigned(slot) * unsigned(nVD) + unsigned(e) : ~0u;
             if (e >= T.start && e < T.start + T.count) {
                 item[j] = ((e - T.start) << 3) | slot;
                 ++nOwn;
@@ -988,8 +993,9 @@ __device__ __forceinline__ void sub_corner_values_grid_body(int psr_bid, int psr
 }
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kCornerGridWaves))) sub_corner_values_grid(Grid g, const Tile* __restrict__ tiles, int nVD,
                                                                    const float* __restrict__ iso,
-                                                                   float* __restrict__ cval) {
-    sub_corner_values_grid_body(blockIdx.x, gridDim.x, g, tiles, nVD, iso, cval);
+                                                                   float* __restrict__ cval,
+                                                                   unsigned* __restrict__ ptOwner) {
+    sub_corner_values_grid_body(blockIdx.x, gridDim.x, g, tiles, nVD, iso, cval, ptOwner);
 }
 PSR_BATCHABLE(sub_corner_values_grid, kThreads)
 
@@ -1215,7 +1221,8 @@ constexpr unsigned kEdgeBits = 0xFFFu;
 __device__ __forceinline__ void sub_classify_body(int psr_bid, int psr_nbid, const Grid& g, const Tile* __restrict__ tiles, int nVD,
                                                          const float* __restrict__ cval, float* __restrict__ vals,
                                                          unsigned* __restrict__ cellWord,
-                                                         int* __restrict__ crossPrefix, int4* __restrict__ tileCnt) {
+                                                         int* __restrict__ crossPrefix, int4* __restrict__ tileCnt,
+                                                         const unsigned* __restrict__ ptOwner, int nMapped) {
     __shared__ TileRoot lRoots[kMaxRootsPerTile];
     __shared__ int lNbr[kMaxRootsPerTile * 27];
     __shared__ unsigned lSet[kMaxRootsPerTile];
@@ -1232,8 +1239,16 @@ __device__ __forceinline__ void sub_classify_body(int psr_bid, int psr_nbid, con
     if (int(threadIdx.x) < tg.nRoots) lSet[threadIdx.x] = g.rootSet[T.root + threadIdx.x];
     __syncthreads();
     // the values at the corner points of each root's part of the tile, read
-    // once per point (its owner is the same whichever cell asks)
+    // once per point (its owner is the same whichever cell asks); the first
+    // nMapped tiles (one-root tiles) take each point's value index from the
+    // owners that sub_corner_values_grid found for the same points
     const int ex = tg.ext.x + 1, ey = tg.ext.y + 1, ez = tg.ext.z + 1, PP = ex * ey * ez;
+    if (psr_bid < nMapped) {
+        for (int t = threadIdx.x; t < kTilePoints; t += kThreads) {
+            const unsigned oi = ptOwner[size_t(psr_bid) * kTilePoints + t];
+            lVal[t] = oi != ~0u ? cval[oi] : 0.f;
+        }
+    } else
     for (int t = threadIdx.x; t < tg.nRoots * PP; t += kThreads) {
         const int rr = t / PP, u = t - rr * PP;
         const int ux = u / (ey * ez), uy = (u / ez) % ey, uz = u % ez;
@@ -1313,8 +1328,9 @@ __device__ __forceinline__ void sub_classify_body(int psr_bid, int psr_nbid, con
 __global__ void __launch_bounds__(kThreads) sub_classify(Grid g, const Tile* __restrict__ tiles, int nVD,
                                                          const float* __restrict__ cval, float* __restrict__ vals,
                                                          unsigned* __restrict__ cellWord,
-                                                         int* __restrict__ crossPrefix, int4* __restrict__ tileCnt) {
-    sub_classify_body(blockIdx.x, gridDim.x, g, tiles, nVD, cval, vals, cellWord, crossPrefix, tileCnt);
+                                                         int* __restrict__ crossPrefix, int4* __restrict__ tileCnt,
+                                                         const unsigned* __restrict__ ptOwner, int nMapped) {
+    sub_classify_body(blockIdx.x, gridDim.x, g, tiles, nVD, cval, vals, cellWord, crossPrefix, tileCnt, ptOwner, nMapped);
 }
 PSR_BATCHABLE(sub_classify, kThreads)
 
@@ -1844,11 +1860,20 @@ void extract_mesh_b(Ctx& c) {
             int* crossPre = c.buf<int>("sub_cross_prefix", nVD);
             int4* tileCnt = c.buf<int4>("sub_tile_cnt", nTiles);
             int4* tileBase = c.buf<int4>("sub_tile_base", nTiles);
+            // the one-root tiles' point owners, found by sub_corner_values_grid
+            // and reused by sub_classify (value indices slot * nVD + cell fit 32 bits)
+            unsigned* ptOwner = nullptr;
+            int nMapped = 0;
             if (nOne > 0 && denseHalo) {
                 int words = 0;
                 for (int k = 0; k < nCs; ++k)
                     if (D - cs[k].rd >= 3) words = std::max(words, grid_tile_words(D, cs[k].rd));
-                PSR_LAUNCH(sub_corner_values_grid, nOne, sizeof(float) * words, st, gr, dtiles, nVD, isoDev, cval);
+                if (size_t(8) * size_t(nVD) <= size_t(UINT32_MAX)) {
+                    ptOwner = c.buf<unsigned>("sub_point_owner", size_t(nOne) * kTilePoints);
+                    nMapped = nOne;
+                }
+                PSR_LAUNCH(sub_corner_values_grid, nOne, sizeof(float) * words, st, gr, dtiles, nVD, isoDev, cval,
+                           ptOwner);
             }
             else if (nOne > 0)
                 PSR_LAUNCH_K(sub_corner_values_one, sub_corner_values_one_k, nOne, 0, st, gr, dtiles, nVD, isoDev, cval);
@@ -1861,7 +1886,8 @@ void extract_mesh_b(Ctx& c) {
             } else if (nSmall > 0) {
                 PSR_LAUNCH_K(sub_corner_values_small, sub_corner_values_small_k, nSmall, 0, st, gr, dsmall, nVD, isoDev, cval);
             }
-            PSR_LAUNCH(sub_classify, nTiles, 0, st, gr, dtiles, nVD, cval, vals, cellWord, crossPre, tileCnt);
+            PSR_LAUNCH(sub_classify, nTiles, 0, st, gr, dtiles, nVD, cval, vals, cellWord, crossPre, tileCnt,
+                       static_cast<const unsigned*>(ptOwner), nMapped);
             // totals: slots 0-3
             PSR_LAUNCH(sub_tile_scan, 1, 0, st, tileCnt, nTiles, tileBase, keptMain, c.mail);
             if (deferred) {
