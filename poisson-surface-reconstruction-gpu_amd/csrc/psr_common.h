@@ -76,6 +76,16 @@ __device__ __forceinline__ double wave_sum_f64(double v) {
     v += dpp_f64<kDppMirror>(v);
     return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
 }
+// Sum over the half-wave h (lanes 32 h .. 32 h + 31), the same in every lane
+// of that half.
+__device__ __forceinline__ float half_sum_f32(float v, int h) {
+    v += dpp_f32<kDppSwap1>(v);
+    v += dpp_f32<kDppSwap2>(v);
+    v += dpp_f32<kDppHalfMirror>(v);
+    v += dpp_f32<kDppMirror>(v);
+    const float lo = readlane_f32(v, 0) + readlane_f32(v, 16), hi = readlane_f32(v, 32) + readlane_f32(v, 48);
+    return h ? hi : lo;
+}
 __device__ __forceinline__ float wave_sum_f32(float v) {
     v += dpp_f32<kDppSwap1>(v);
     v += dpp_f32<kDppSwap2>(v);

@@ -114,6 +114,31 @@ __device__ __forceinline__ int wave_ranges(NbRanges& r, const int* __restrict__ 
     return total;
 }
 
+// wave_ranges for two nodes, one per half-wave (hl: the lane within the half;
+// an invalid node has no range).
+template <class LenFn>
+__device__ __forceinline__ int half_ranges(NbRanges& r, const int* __restrict__ neigh, int M, int o, int hl,
+                                           bool valid, LenFn fn) {
+    constexpr int kHalf = PSR_WAVE / 2;
+    int len = 0, first = 0;
+    if (valid && hl < 27) {
+        const int nb = neigh[hl * M + o];
+        if (nb != -1) fn(nb, first, len);
+    }
+    int inc = len;
+    for (int off = 1; off < kHalf; off <<= 1) {
+        const int t = __shfl_up(inc, off, kHalf);
+        if (hl >= off) inc += t;
+    }
+    if (hl < 27) {
+        r.start[hl] = inc - len;
+        r.first[hl] = first;
+    }
+    const int total = __shfl(inc, 26, kHalf);
+    if (hl == 0) r.start[27] = total;
+    return total;
+}
+
 __device__ __forceinline__ int range_of(const NbRanges& r, int t) {
     int i = 0;
     for (int step = 16; step > 0; step >>= 1)
@@ -123,7 +148,9 @@ __device__ __forceinline__ int range_of(const NbRanges& r, int t) {
 
 // --- vector field (ComputeVectorField.cu:61-100) ---------------------------
 // V(o) = sum over the 27 neighbours' points p of F_o(x)F_o(y)F_o(z) n_p,
-// one wave per depth-D node.
+// two depth-D nodes per wave, one per half: a node's chain of dependent loads
+// (neighbours, their point ranges, the points) is latency-bound, and the two
+// chains of a wave overlap.
 // Field storage: FP32, or FP16 holding V * 2^-(D+1) (the normals' scale, an
 // exact power of two, so the stored value is the FP16 rounding of the sum of
 // weighted unit normals).
@@ -138,13 +165,16 @@ __device__ __forceinline__ void vector_field_kernel_body(int psr_bid, int psr_nb
     int nD, int offD, int M, int D, const int* __restrict__ neigh, const int* __restrict__ pidx,
     const int* __restrict__ pnum, const int* __restrict__ key, const float4* __restrict__ pos,
     const float4* __restrict__ nrm, FieldT* __restrict__ vf) {
-    __shared__ NbRanges R[kVfWaves];
-    int wid = threadIdx.x / PSR_WAVE, lane = __lane_id();
-    int node = psr_bid * kVfWaves + wid;
-    if (node >= nD) return;
-    int o = offD + node;
-    NbRanges& r = R[wid];
-    int total = wave_ranges(r, neigh, M, o, lane, [&](int nb, int& first, int& len) {
+    __shared__ NbRanges R[kVfWaves][2];
+    constexpr int kHalf = PSR_WAVE / 2;
+    const int wid = threadIdx.x / PSR_WAVE, lane = __lane_id(), half = lane / kHalf, hl = lane % kHalf;
+    const int node0 = (psr_bid * kVfWaves + wid) * 2;
+    if (node0 >= nD) return;
+    const int node = node0 + half;
+    const bool valid = node < nD;
+    const int o = offD + (valid ? node : node0);
+    NbRanges& r = R[wid][half];
+    const int total = half_ranges(r, neigh, M, o, hl, valid, [&](int nb, int& first, int& len) {
         first = pidx[nb];
         len = pnum[nb];
     });
@@ -155,7 +185,7 @@ __device__ __forceinline__ void vector_field_kernel_body(int psr_bid, int psr_nb
     func_center_invw(f.y, cy, invw);
     func_center_invw(f.z, cz, invw);
     float ax = 0.f, ay = 0.f, az = 0.f;
-    for (int t = lane; t < total; t += PSR_WAVE) {
+    for (int t = hl; t < total; t += kHalf) {
         int i = range_of(r, t);
         int p = r.first[i] + (t - r.start[i]);
         float4 q = pos[p];
@@ -165,10 +195,10 @@ __device__ __forceinline__ void vector_field_kernel_body(int psr_bid, int psr_nb
         ay += w * n.y;
         az += w * n.z;
     }
-    ax = wave_sum_f32(ax);
-    ay = wave_sum_f32(ay);
-    az = wave_sum_f32(az);
-    if (lane == 0) {
+    ax = half_sum_f32(ax, half);
+    ay = half_sum_f32(ay, half);
+    az = half_sum_f32(az, half);
+    if (hl == 0 && valid) {
         const float inv = 1.0f / float(2 << D);
         store_field(&vf[node], ax, inv);
         store_field(&vf[nD + node], ay, inv);
@@ -211,6 +241,8 @@ __device__ __forceinline__ void divergence_fine_body(int psr_bid, int psr_nbid,
     int begin, int count, int M, int offD, int nD, int res, int D, const int* __restrict__ neigh,
     const int* __restrict__ didx, const int* __restrict__ dnum, const int* __restrict__ enc,
     const FieldT* __restrict__ vf, const double* __restrict__ ddot, float* __restrict__ div) {
+    // (two nodes per wave, as the vector field splats them, measured 64%
+    // slower here: a node at depth 4 or 5 has hundreds to thousands of terms)
     __shared__ NbRanges R[kDivWaves];
     int wid = threadIdx.x / PSR_WAVE, lane = __lane_id();
     int node = psr_bid * kDivWaves + wid;
@@ -439,7 +471,7 @@ void field_stages(Ctx& c, FieldT* vf) {
     const Levels& L = o.L;
     const int D = L.D, nD = L.count[D], offD = L.base[D], M = o.M;
     hipStream_t s = c.stream;
-    PSR_LAUNCH_K(vector_field_kernel<FieldT>, vector_field_kernel_k<FieldT>, div_up(nD, kVfWaves), 0, s,
+    PSR_LAUNCH_K(vector_field_kernel<FieldT>, vector_field_kernel_k<FieldT>, div_up(nD, 2 * kVfWaves), 0, s,
         nD, offD, M, D, o.neigh, o.pidx, o.pnum, o.key, o.spos, o.snrm, vf);
     // depths 0..kCoarse: several workgroups per node, per depth as many as the
     // node's expected share of depth D asks for; depth D: a thread per node;
