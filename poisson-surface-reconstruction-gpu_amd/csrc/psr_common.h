@@ -209,12 +209,13 @@ __device__ __forceinline__ float chi_node_sum(int node, float invw, float3 p, in
 // node's plus (i / 9 - 1, (i / 3) % 3 - 1, i % 3 - 1) widths, exactly (the
 // centres are dyadic), so the 27 terms take 3 B-spline values per axis.  Each
 // term is formed as col_term forms it, so the sum equals chi_node_sum's.
+// The d_x come from the dense [27][M] copy of the columns' d_x (a wave's 27
+// loads then touch a quarter of the cache lines).
 __device__ __forceinline__ float chi_node_sum_regular(int node, float4 b, float3 p, int M,
-                                                      const float4* __restrict__ nbrCol) {
-    const float* dcol = reinterpret_cast<const float*>(nbrCol) + 3;  // d_x of each column entry
+                                                      const float* __restrict__ dCol) {
     float d[27];
 #pragma unroll
-    for (int i = 0; i < 27; ++i) d[i] = dcol[4 * (size_t(i) * M + node)];
+    for (int i = 0; i < 27; ++i) d[i] = dCol[size_t(i) * M + node];
     const float w = 1.0f / b.w;  // exact: b.w is a power of two
     float fx[3], fy[3], fz[3];
 #pragma unroll
@@ -240,6 +241,7 @@ struct PointsChiArgs {
     const int* p2n;
     const int* parent;
     const float4* nbrCol;
+    const float* dCol;
     const float4* basis;
     const int* q1Fired;
     float* chi;
@@ -256,7 +258,7 @@ __device__ __forceinline__ void points_chi_block(const PointsChiArgs& a, int blo
         int now = a.offD + a.p2n[i];
         float v = 0.f;
         for (int d = a.D; now != -1; --d) {  // a parent is one level up
-            v += regular ? chi_node_sum_regular(now, a.basis[now], p, a.M, a.nbrCol)
+            v += regular ? chi_node_sum_regular(now, a.basis[now], p, a.M, a.dCol)
                          : chi_node_sum(now, float(1 << d), p, a.M, a.nbrCol);
             now = a.parent[now];
         }

@@ -101,6 +101,7 @@ struct Field {
     float* dx = nullptr;      // [M]
     float* chiPts = nullptr;  // [N]
     float4* nbrCol = nullptr;  // [27][M]: centre (x, y, z) and d_x of neighbour i of each node (0 where absent)
+    float* dCol = nullptr;     // [27][M]: d_x of neighbour i of each node (0 where absent), 4 B a term
     double* isoSum = nullptr; // device scalar
     float iso = 0.f;
     PointsChiArgs chiArgs{};  // the points' implicit values, launched with the mesh stage's first kernel

@@ -359,6 +359,7 @@ __device__ __forceinline__ void vertex_values_body(int psr_bid, int psr_nbid, co
                                                           const int* __restrict__ parent,
                                                           const int* __restrict__ childBase,
                                                           const float4* __restrict__ nbrCol,
+                                                          const float* __restrict__ dCol,
                                                           const int* __restrict__ q1Fired,
                                                           const float* __restrict__ iso, float* __restrict__ vv,
                                                           int* __restrict__ mail, const int* __restrict__ results,
@@ -377,7 +378,7 @@ __device__ __forceinline__ void vertex_values_body(int psr_bid, int psr_nbid, co
     if (owner > 0) {
         int now = owner;
         for (int d = depth; now != -1; --d) {  // a parent is one level up
-            val += regular ? chi_node_sum_regular(now, v.basis[now], p, v.M, nbrCol)
+            val += regular ? chi_node_sum_regular(now, v.basis[now], p, v.M, dCol)
                            : chi_node_sum(now, float(1 << d), p, v.M, nbrCol);
             now = parent[now];
         }
@@ -389,7 +390,7 @@ __device__ __forceinline__ void vertex_values_body(int psr_bid, int psr_nbid, co
             int cb = childBase[now];
             if (cb < 0) break;
             now = cb + ex;
-            val += regular ? chi_node_sum_regular(now, v.basis[now], p, v.M, nbrCol)
+            val += regular ? chi_node_sum_regular(now, v.basis[now], p, v.M, dCol)
                            : chi_node_sum(now, float(1 << depth), p, v.M, nbrCol);
         }
     }
@@ -402,11 +403,12 @@ __global__ void __launch_bounds__(kValueThreads) vertex_values(NodeView v, const
                                                           const int* __restrict__ parent,
                                                           const int* __restrict__ childBase,
                                                           const float4* __restrict__ nbrCol,
+                                                          const float* __restrict__ dCol,
                                                           const int* __restrict__ q1Fired,
                                                           const float* __restrict__ iso, float* __restrict__ vv,
                                                           int* __restrict__ mail, const int* __restrict__ results,
                                                           int* __restrict__ resultsHost, int resultWords) {
-    vertex_values_body(blockIdx.x, gridDim.x, v, count, vOwner, vKind, vPos, parent, childBase, nbrCol, q1Fired, iso, vv, mail, results, resultsHost, resultWords);
+    vertex_values_body(blockIdx.x, gridDim.x, v, count, vOwner, vKind, vPos, parent, childBase, nbrCol, dCol, q1Fired, iso, vv, mail, results, resultsHost, resultWords);
 }
 PSR_BATCHABLE(vertex_values, kValueThreads)
 
@@ -648,7 +650,7 @@ void build_geometry(Ctx& c) {
     g.vvalue = c.buf<float>("vvalue", size_t(8) * M);
     PSR_LAUNCH(vertex_values, div_up(size_t(8) * M, kValueThreads), 0, s, v, g.counts + 0, g.vtxOwner, g.vtxKind,
                                                                        g.vtxPos, o.parent, o.childBase,
-                                                                       c.fld.nbrCol, o.q1Fired, isoDev, g.vvalue, c.mail,
+                                                                       c.fld.nbrCol, c.fld.dCol, o.q1Fired, isoDev, g.vvalue, c.mail,
                                                                        reinterpret_cast<const int*>(c.results()),
                                                                        reinterpret_cast<int*>(c.mailResults),
                                                                        int(sizeof(DevResults) / sizeof(int)));

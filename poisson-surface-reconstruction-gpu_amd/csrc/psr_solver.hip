@@ -1440,7 +1440,7 @@ PSR_BATCHABLE_WAVES(cg_mixed_minres, kCgThreads, kCgMinresWaves)
 __device__ __forceinline__ void neighbour_columns_body(int psr_bid, int psr_nbid, int M, const int* __restrict__ neigh,
                                                               const float4* __restrict__ basis,
                                                               const float* __restrict__ dx,
-                                                              float4* __restrict__ nbrCol,
+                                                              float4* __restrict__ nbrCol, float* __restrict__ dCol,
                                                               unsigned* __restrict__ zero, size_t nZero,
                                                               float* __restrict__ dense, size_t nDense,
                                                               const int* __restrict__ q1Fired,
@@ -1457,21 +1457,24 @@ __device__ __forceinline__ void neighbour_columns_body(int psr_bid, int psr_nbid
     const int nb = neigh[t];
     if (nb < 0) {
         nbrCol[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+        dCol[t] = 0.f;
     } else {
         const float4 b = basis[nb];
-        nbrCol[t] = make_float4(b.x, b.y, b.z, dx[nb]);
+        const float d = dx[nb];
+        nbrCol[t] = make_float4(b.x, b.y, b.z, d);
+        dCol[t] = d;
     }
 }
 __global__ void __launch_bounds__(kThreads) neighbour_columns(int M, const int* __restrict__ neigh,
                                                               const float4* __restrict__ basis,
                                                               const float* __restrict__ dx,
-                                                              float4* __restrict__ nbrCol,
+                                                              float4* __restrict__ nbrCol, float* __restrict__ dCol,
                                                               unsigned* __restrict__ zero, size_t nZero,
                                                               float* __restrict__ dense, size_t nDense,
                                                               const int* __restrict__ q1Fired,
                                                               unsigned* __restrict__ ctl, size_t nCtl,
                                                               unsigned* __restrict__ slots, size_t nSlots) {
-    neighbour_columns_body(blockIdx.x, gridDim.x, M, neigh, basis, dx, nbrCol, zero, nZero, dense, nDense, q1Fired, ctl, nCtl, slots, nSlots);
+    neighbour_columns_body(blockIdx.x, gridDim.x, M, neigh, basis, dx, nbrCol, dCol, zero, nZero, dense, nDense, q1Fired, ctl, nCtl, slots, nSlots);
 }
 PSR_BATCHABLE(neighbour_columns, kThreads)
 
@@ -1761,6 +1764,7 @@ void points_implicit(Ctx& c) {
     double* part = c.buf<double>("iso_part", nb);
     c.fld.isoSum = c.buf<double>("iso_sum", 1);
     c.fld.nbrCol = c.buf<float4>("nbr_col", size_t(27) * o.M);
+    c.fld.dCol = c.buf<float>("nbr_dcol", size_t(27) * o.M);
     // the mesh stage's node tables and face marks, cleared with the columns:
     // nodeVert [8][M], nodeFace [6][M], nodeEdge [12][n_D], face marks (6M, the face bound)
     const size_t M = size_t(o.M), nD = size_t(L.count[L.D]), nZero = 8 * M + 6 * M + 12 * nD + 6 * M;
@@ -1774,13 +1778,13 @@ void points_implicit(Ctx& c) {
     const size_t nDense = L.D <= kDenseEarlyDepth ? dense_grid_cells(L.D) : 0;
     c.geo.dxGrid = nDense ? c.buf<float>("sub_dx_grid", nDense) : nullptr;
     PSR_LAUNCH(neighbour_columns, div_up(size_t(27) * o.M, kThreads), 0, s,
-        o.M, o.neigh, o.basis, c.fld.dx, c.fld.nbrCol, reinterpret_cast<unsigned*>(zeroed), nZero, c.geo.dxGrid,
+        o.M, o.neigh, o.basis, c.fld.dx, c.fld.nbrCol, c.fld.dCol, reinterpret_cast<unsigned*>(zeroed), nZero, c.geo.dxGrid,
         nDense, o.q1Fired, reinterpret_cast<unsigned*>(c.cgCtl), c.cgCtlWords,
         reinterpret_cast<unsigned*>(c.cgSlots), c.cgSlotWords);
     // the implicit function at the points runs beside the mesh stage's first
     // kernel (build_geometry), then the iso value
     c.fld.chiArgs = PointsChiArgs{N,        L.base[L.D], o.M,          L.D,      o.spos,        o.point2node,
-                                  o.parent, c.fld.nbrCol, o.basis, o.q1Fired, c.fld.chiPts, part};
+                                  o.parent, c.fld.nbrCol, c.fld.dCol, o.basis, o.q1Fired, c.fld.chiPts, part};
     c.fld.chiBlocks = nb;
     PSR_HIP_CHECK(hipGetLastError());
 }
