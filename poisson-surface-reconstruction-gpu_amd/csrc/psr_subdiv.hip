@@ -344,6 +344,11 @@ constexpr int kLdsWordsOne = 7168;     // one-root tiles: list, halo, vbox, d_x,
 constexpr int kLdsWordsSmall = 5376;   // small-root tiles of 64 cells: halo, vbox, staged blocks (<= 5344 words)
 constexpr int kStagedBlocks = 32;  // contributor blocks of 27 staged in LDS per small-root tile
 constexpr int kSmallThreads = 128;  // threads of a small-root tile (4 x 27 or 125 corner points)
+// threads of a small-root tile in sub_corner_values_small_grid: the points take
+// at most 125; the others share out the tile's tables (the launch has only a
+// few blocks per CU, so the tables' set-up is latency-bound: 128 threads 31.1,
+// 256 25.4, 512 23.6 us on the torus)
+constexpr int kSmallGridThreads = 512;
 constexpr int kVboxWords = 2048;
 
 __device__ __forceinline__ int3 box_ext(int3 extD, int D, int l) {
@@ -1036,8 +1041,8 @@ __device__ __forceinline__ void sub_corner_values_small_grid_body(int psr_bid, i
     const int D = g.D, sh = D - S.rd, n = (1 << sh) + 1, P = n * n * n;
     const TileGeom tg = tile_geom(T, S, D);
     const int nR = tg.nRoots, K = tg.K, Kp = (K + 3) & ~3;
-    for (int t = threadIdx.x; t < nR * 27; t += kSmallThreads) lNbr[t] = g.rootNbr[27 * size_t(T.root) + t];
-    for (int t = threadIdx.x; t < nR * kChain; t += kSmallThreads)
+    for (int t = threadIdx.x; t < nR * 27; t += kSmallGridThreads) lNbr[t] = g.rootNbr[27 * size_t(T.root) + t];
+    for (int t = threadIdx.x; t < nR * kChain; t += kSmallGridThreads)
         lChain[t] = g.rootChain[kChain * size_t(T.root) + t];
     if (int(threadIdx.x) < nR) lSet[threadIdx.x] = g.rootSet[T.root + threadIdx.x];
     fill_roots_rec(g, T, S, tg, lRoot);
@@ -1045,7 +1050,7 @@ __device__ __forceinline__ void sub_corner_values_small_grid_body(int psr_bid, i
     // per root: list x (with d_x) [n][Kp] | list y | list z | halo d_x | halo factors
     const int listWords = 3 * n * Kp, perRoot = small_tile_words(D, S.rd, 1);
     const float inv = 1.0f / float(1 << D);
-    for (int t = threadIdx.x; t < nR * Kp; t += kSmallThreads) {
+    for (int t = threadIdx.x; t < nR * Kp; t += kSmallGridThreads) {
         const int r = t / Kp, k = t - r * Kp;
         const int node = k < K ? lChain[r * kChain + k / 27] : -1;
         float4 b = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -1064,7 +1069,7 @@ __device__ __forceinline__ void sub_corner_values_small_grid_body(int psr_bid, i
         }
     }
     // halo d_x, depth D first: 0 inside the roots of the set and where no node is
-    for (int t = threadIdx.x; t < nR * tg.haloPerRoot; t += kSmallThreads) {
+    for (int t = threadIdx.x; t < nR * tg.haloPerRoot; t += kSmallGridThreads) {
         const int r = t / tg.haloPerRoot;
         int u = t - r * tg.haloPerRoot, l = D;
         int3 x = box_ext(tg.ext, D, l);
@@ -1095,7 +1100,7 @@ __device__ __forceinline__ void sub_corner_values_small_grid_body(int psr_bid, i
     // halo factors: per depth, per axis, node cell (box-relative) x corner coordinate
     int facPerRoot = 0;
     for (int l = D; l > S.rd; --l) facPerRoot += 3 * (box_ext(tg.ext, D, l).x + 2) * n;
-    for (int t = threadIdx.x; t < nR * facPerRoot; t += kSmallThreads) {
+    for (int t = threadIdx.x; t < nR * facPerRoot; t += kSmallGridThreads) {
         const int r = t / facPerRoot;
         int u = t - r * facPerRoot, l = D;
         int3 x = box_ext(tg.ext, D, l);
@@ -1165,12 +1170,12 @@ __device__ __forceinline__ void sub_corner_values_small_grid_body(int psr_bid, i
     }
     cval[size_t(slot) * nVD + best] = val - *iso;
 }
-__global__ void __launch_bounds__(kSmallThreads) sub_corner_values_small_grid(Grid g, const Tile* __restrict__ tiles,
+__global__ void __launch_bounds__(kSmallGridThreads) sub_corner_values_small_grid(Grid g, const Tile* __restrict__ tiles,
                                                                               int nVD, const float* __restrict__ iso,
                                                                               float* __restrict__ cval) {
     sub_corner_values_small_grid_body(blockIdx.x, gridDim.x, g, tiles, nVD, iso, cval);
 }
-PSR_BATCHABLE(sub_corner_values_small_grid, kSmallThreads)
+PSR_BATCHABLE(sub_corner_values_small_grid, kSmallGridThreads)
 
 
 // Exclusive prefix of v over the block's threads and the block total (every
