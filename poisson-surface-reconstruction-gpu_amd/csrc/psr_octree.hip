@@ -500,60 +500,72 @@ __device__ __forceinline__ void encode_node(int o, int d, int D, const int* __re
     basis[o] = node_basis_params(f);
 }
 
-// Two depths per launch: the first count0 threads take depth `depth` from
-// their parents' final neighbours; the rest take depth + 1, whose parents'
-// neighbours this launch is still writing, so the 8 each one needs are formed
-// again from the grandparent's by the same recurrence (count1 = 0: one depth).
-// The launch for depth 1 also writes the root's row (itself at 13, nothing
-// else), reading it as such rather than from memory.  Every thread also
-// encodes its node (and the first, the root).
-__device__ __forceinline__ void neighbors_level_body(int psr_bid, int psr_nbid, int begin, int count0, int count1, int D, int depth,
-                                                            int M, const int* __restrict__ key,
-                                                            const int* __restrict__ parent,
-                                                            const int* __restrict__ childBase,
-                                                            int* __restrict__ neigh, int* __restrict__ enc,
-                                                            int* __restrict__ depthOf, float4* __restrict__ basis) {
-    const int t = psr_bid * kThreads + threadIdx.x;
-    const bool fromRoot = depth == 1;
-    if (fromRoot && t < 27) neigh[t * M] = t == 13 ? 0 : -1;
-    if (fromRoot && t == 0) encode_node(0, 0, D, key, enc, depthOf, basis);
-    if (t >= count0 + count1) return;
-    const int o = begin + t;
-    const int d = t < count0 ? depth : depth + 1;
-    encode_node(o, d, D, key, enc, depthOf, basis);
-    const int s = (key[o] >> (3 * (D - d))) & 7;
-    const int p = parent[o];
-    // neighbour at `slot` of an ancestor at depth - 1 (the root when fromRoot)
-    auto upper = [&](int slot, int node) { return fromRoot ? (slot == 13 ? 0 : -1) : neigh[slot * M + node]; };
-    int np[8];  // the parent's neighbours at slots parent_slot(s, u)
-    if (t < count0) {
-#pragma unroll
-        for (int u = 0; u < 8; ++u) np[u] = upper(parent_slot(s, u), p);
-    } else {
-        const int sp = (key[p] >> (3 * (D - depth))) & 7, gp = parent[p];
-        int cbp[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) cbp[u] = upper(parent_slot(sp, u), gp);
-#pragma unroll
-        for (int u = 0; u < 8; ++u) cbp[u] = cbp[u] == -1 ? -1 : childBase[cbp[u]];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) np[u] = child_neighbour(cbp, sp, parent_slot(s, u));
+// Every node in one launch: a node at depth d walks its ancestors from the
+// root down, keeping at each level only the 8 neighbours of the ancestor that
+// its next descendant's 27 entries touch, and ends with its own 27.  The
+// recurrence is the one above, applied d times from the root's row (itself at
+// 13, nothing else) instead of reading the parent's finished row, so no level
+// waits for the one before it; each level costs one round of 8 independent
+// childBase reads.  A row depends on the chain of parent links up to the root
+// and on the son key of each node on it.  Unless the level-local child 0 rule
+// changed a parent (q1Fired), that chain is the node's true ancestry and its
+// son keys are the bits of the node's own key; otherwise they are read along
+// the parent links, as the reference's level-by-level build uses them.  Every
+// thread also encodes its node, and thread 0 writes the root's row.
+__device__ __forceinline__ void neighbors_all_body(int psr_bid, int psr_nbid, const Levels& L, int M,
+                                                   const int* __restrict__ key, const int* __restrict__ parent,
+                                                   const int* __restrict__ childBase,
+                                                   const int* __restrict__ q1Fired, int* __restrict__ neigh,
+                                                   int* __restrict__ enc, int* __restrict__ depthOf,
+                                                   float4* __restrict__ basis) {
+    const int o = psr_bid * kThreads + threadIdx.x;
+    if (o >= M) return;
+    const int D = L.D;
+    if (o == 0) {
+        encode_node(0, 0, D, key, enc, depthOf, basis);
+        for (int i = 0; i < 27; ++i) neigh[i * M] = i == 13 ? 0 : -1;
+        return;
     }
+    const int d = depth_of(L, o);
+    encode_node(o, d, D, key, enc, depthOf, basis);
+    const int k = key[o];
+    static_assert(3 * PSR_MAX_DEPTH <= 32, "son keys of a chain fit one word");
+    unsigned sons = 0;  // son key of the chain's depth-l node in bits 3(l - 1) .. 3l - 1
+    if (*q1Fired == 0) {
+        for (int l = 1; l <= d; ++l) sons |= unsigned((k >> (3 * (D - l))) & 7) << (3 * (l - 1));
+    } else {
+        sons = unsigned((k >> (3 * (D - d))) & 7) << (3 * (d - 1));
+        for (int l = d - 1, p = parent[o]; l >= 1; --l, p = parent[p])
+            sons |= unsigned((key[p] >> (3 * (D - l))) & 7) << (3 * (l - 1));
+    }
+    auto son = [&](int l) { return int(sons >> (3 * (l - 1))) & 7; };
+    int s = son(1);
+    int np[8];  // neighbours of the depth l - 1 ancestor at slots parent_slot(s, u)
+#pragma unroll
+    for (int u = 0; u < 8; ++u) np[u] = parent_slot(s, u) == 13 ? 0 : -1;
     int cb[8];
+    for (int l = 1; l < d; ++l) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) cb[u] = np[u] == -1 ? -1 : childBase[np[u]];
+        const int sn = son(l + 1);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) np[u] = child_neighbour(cb, s, parent_slot(sn, u));
+        s = sn;
+    }
 #pragma unroll
     for (int u = 0; u < 8; ++u) cb[u] = np[u] == -1 ? -1 : childBase[np[u]];
 #pragma unroll
     for (int i = 0; i < 27; ++i) neigh[i * M + o] = child_neighbour(cb, s, i);
 }
-__global__ void __launch_bounds__(kThreads) neighbors_level(int begin, int count0, int count1, int D, int depth,
-                                                            int M, const int* __restrict__ key,
-                                                            const int* __restrict__ parent,
-                                                            const int* __restrict__ childBase,
-                                                            int* __restrict__ neigh, int* __restrict__ enc,
-                                                            int* __restrict__ depthOf, float4* __restrict__ basis) {
-    neighbors_level_body(blockIdx.x, gridDim.x, begin, count0, count1, D, depth, M, key, parent, childBase, neigh, enc, depthOf, basis);
+__global__ void __launch_bounds__(kThreads) neighbors_all(Levels L, int M, const int* __restrict__ key,
+                                                          const int* __restrict__ parent,
+                                                          const int* __restrict__ childBase,
+                                                          const int* __restrict__ q1Fired,
+                                                          int* __restrict__ neigh, int* __restrict__ enc,
+                                                          int* __restrict__ depthOf, float4* __restrict__ basis) {
+    neighbors_all_body(blockIdx.x, gridDim.x, L, M, key, parent, childBase, q1Fired, neigh, enc, depthOf, basis);
 }
-PSR_BATCHABLE(neighbors_level, kThreads)
+PSR_BATCHABLE(neighbors_all, kThreads)
 
 
 }  // namespace
@@ -651,14 +663,11 @@ void build_octree_b(Ctx& c) {
     PSR_LAUNCH(fill_blocks, div_up(totalBlocks + 1, kThreads), 0, s, L, totalBlocks, N, o.U[D - 1], o.key, o.pidx,
                                                                         o.pnum, o.parent, o.childBase, o.didx,
                                                                         o.dnum);
-    // neighbours (two depths per launch), packed index, depth and basis of every node
+    // neighbours, packed index, depth and basis of every node
     o.depth = c.buf<int>("node_depth", M);
     o.basis = c.buf<float4>("node_basis", M);
-    for (int d = 1; d <= D; d += 2) {
-        const int c1 = d + 1 <= D ? L.count[d + 1] : 0;
-        PSR_LAUNCH(neighbors_level, div_up(std::max(L.count[d] + c1, 27), kThreads), 0, s,
-            L.base[d], L.count[d], c1, D, d, M, o.key, o.parent, o.childBase, o.neigh, o.encIdx, o.depth, o.basis);
-    }
+    PSR_LAUNCH(neighbors_all, div_up(M, kThreads), 0, s, L, M, o.key, o.parent, o.childBase, o.q1Fired, o.neigh,
+               o.encIdx, o.depth, o.basis);
     PSR_HIP_CHECK(hipGetLastError());
 }
 
