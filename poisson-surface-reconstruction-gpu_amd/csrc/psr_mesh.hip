@@ -427,16 +427,26 @@ __device__ __forceinline__ void edge_crossings_body(int psr_bid, int psr_nbid, i
     const int tile = ticket_tile(t);
     const int n = tile * kThreads + threadIdx.x, o = offD + n;
     const int2 eo = n < nD ? edgeOf[n] : make_int2(0, 0);
+    // the node's 8 corner vertices and their values, read once for all its
+    // edges (two memory round trips)
+    int cv[8];
+    float cval[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) cv[i] = eo.y ? nodeVert[i * M + o] - 1 : -1;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) cval[i] = cv[i] >= 0 ? vv[cv[i]] : 0.f;
     unsigned cross = 0;
+#pragma unroll
     for (int k = 0; k < 12; ++k) {
         if (!((eo.y >> k) & 1)) continue;
-        const int v1 = nodeVert[kEdgeVertex[k][0] * M + o] - 1, v2 = nodeVert[kEdgeVertex[k][1] * M + o] - 1;
+        const int c1 = kEdgeVertex[k][0], c2 = kEdgeVertex[k][1];
         // a corner slot left unassigned (the reference's level-local child 0
         // parent rule breaks neighbour tables) makes the edge absent, not a crossing
-        if (v1 >= 0 && v2 >= 0 && vv[v1] * vv[v2] <= 0.f) cross |= 1u << k;
+        if (cv[c1] >= 0 && cv[c2] >= 0 && cval[c1] * cval[c2] <= 0.f) cross |= 1u << k;
     }
     int c = ticket_prefix<kThreads>(t, tile, __popc(cross));
     int a = eo.x;
+#pragma unroll
     for (int k = 0; k < 12; ++k) {
         if (!((eo.y >> k) & 1)) continue;
         const bool x = (cross >> k) & 1u;
@@ -444,10 +454,10 @@ __device__ __forceinline__ void edge_crossings_body(int psr_bid, int psr_nbid, i
         addr[a] = c;
         ++a;
         if (!x) continue;
-        const int v1 = nodeVert[kEdgeVertex[k][0] * M + o] - 1, v2 = nodeVert[kEdgeVertex[k][1] * M + o] - 1;
-        const float4 p1 = vPos[v1], p2 = vPos[v2];
-        const float3 p =
-            interpolate_point(make_float3(p1.x, p1.y, p1.z), make_float3(p2.x, p2.y, p2.z), k >> 2, vv[v1], vv[v2]);
+        const int c1 = kEdgeVertex[k][0], c2 = kEdgeVertex[k][1];
+        const float4 p1 = vPos[cv[c1]], p2 = vPos[cv[c2]];
+        const float3 p = interpolate_point(make_float3(p1.x, p1.y, p1.z), make_float3(p2.x, p2.y, p2.z), k >> 2,
+                                           cval[c1], cval[c2]);
         out[3 * c] = p.x;
         out[3 * c + 1] = p.y;
         out[3 * c + 2] = p.z;
@@ -534,11 +544,14 @@ __device__ __forceinline__ void cube_triangles_body(int psr_bid, int psr_nbid,
         int cf = nodeFace[f * M + o] - 1;
         if (cf < 0) continue;  // face slot left unassigned (broken tables): nothing to mark
         hasInter[cf] = 1;
-        // the walk stops where the reference would leave the tree or read an unassigned slot
-        while (faceHasParent[cf] != -1 && pn >= 0) {
-            cf = nodeFace[f * M + pn] - 1;
-            if (cf < 0) break;
-            pn = parent[pn];
+        // the walk stops where the reference would leave the tree or read an
+        // unassigned slot; a step's three reads depend only on the step before,
+        // so they are issued together (one memory round trip a level)
+        while (pn >= 0) {
+            const int up = faceHasParent[cf], cfUp = nodeFace[f * M + pn] - 1, pnUp = parent[pn];
+            if (up == -1 || cfUp < 0) break;
+            cf = cfUp;
+            pn = pnUp;
             hasInter[cf] = 1;
         }
     }
@@ -573,23 +586,33 @@ __device__ __forceinline__ void root_select_body(int psr_bid, int psr_nbid, int 
     int flag = 0;
     if (o < count && childBase[o] < 0 && neigh[13 * M + o] == o) {
         // unassigned corner and face slots are skipped; the sign reference is
-        // the first present corner (corner 0 in a consistent tree)
-        int i0 = 0;
-        while (i0 < 8 && nodeVert[i0 * M + o] == 0) ++i0;
-        if (i0 < 8) {
-            const float sign = vv[nodeVert[i0 * M + o] - 1] < 0.f ? -1.f : 1.f;
-            for (int i = i0 + 1; i < 8; ++i) {
-                const int v = nodeVert[i * M + o];
-                if (v != 0 && sign * vv[v - 1] < 0.f) {
-                    flag = 1;
-                    break;
-                }
+        // the first present corner (corner 0 in a consistent tree).  All slots
+        // are read at once, then all values they name: two memory round trips
+        // instead of one or two per corner and face.
+        int vtx[8], fs[6];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) vtx[i] = nodeVert[i * M + o];
+#pragma unroll
+        for (int f = 0; f < 6; ++f) fs[f] = nodeFace[f * M + o];
+        float val[8];
+        int hit = 0;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) val[i] = vtx[i] != 0 ? vv[vtx[i] - 1] : 0.f;
+#pragma unroll
+        for (int f = 0; f < 6; ++f) hit |= fs[f] != 0 ? hasInter[fs[f] - 1] : 0;
+        bool seen = false;
+        float sign = 1.f;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            if (vtx[i] == 0) continue;
+            if (!seen) {
+                seen = true;
+                sign = val[i] < 0.f ? -1.f : 1.f;
+            } else if (sign * val[i] < 0.f) {
+                flag = 1;
             }
         }
-        for (int f = 0; f < 6 && !flag; ++f) {
-            const int fs = nodeFace[f * M + o];
-            if (fs != 0 && hasInter[fs - 1]) flag = 1;
-        }
+        if (hit) flag = 1;
     }
     const int a = ticket_prefix<kThreads>(t, tile, flag);
     if (o >= count) return;
