@@ -494,20 +494,38 @@ __device__ __forceinline__ void cube_triangles_body(int psr_bid, int psr_nbid,
     const int tile = ticket_tile(t);
     const int n = tile * kThreads + threadIdx.x, o = offD + n;
     if (n == 0) mail[kMailQ1] = *q1Fired;
+    // The reads come in three rounds, each issued together: the node's corner
+    // and edge slots; the corner values; the 12 edges' crossing flags and
+    // crossing indices (before the look-back, which they then overlap).
     int c = 0;
+    int va[12];  // crossing index of each edge, read when every slot is assigned
+#pragma unroll
+    for (int e = 0; e < 12; ++e) va[e] = 0;
     if (n < nD) {
+        int vtx[8], ne[12];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) vtx[i] = nodeVert[i * M + o];
+#pragma unroll
+        for (int e = 0; e < 12; ++e) ne[e] = nodeEdge[e * nD + n];
         bool ok = true;
+#pragma unroll
         for (int i = 0; i < 8; ++i) {
-            const int v = nodeVert[i * M + o];
-            if (v == 0) ok = false;
-            else if (vv[v - 1] < 0.f) c |= 1 << i;
+            if (vtx[i] == 0) ok = false;
+            else if (vv[vtx[i] - 1] < 0.f) c |= 1 << i;
         }
+#pragma unroll
         for (int e = 0; e < 12; ++e)
-            if (nodeEdge[e * nD + n] == 0) ok = false;
+            if (ne[e] == 0) ok = false;
         if (ok) {
+            unsigned crossed = 0;
+#pragma unroll
+            for (int e = 0; e < 12; ++e) {
+                crossed |= (crossFlag[ne[e] - 1] ? 1u : 0u) << e;
+                va[e] = vexAddr[ne[e] - 1];
+            }
             const uint64_t w = mc_word(c);
             for (int k = 0, m = 3 * mc_tri_count(c); k < m; ++k)
-                if (!crossFlag[nodeEdge[int((w >> (4 * k)) & 0xF) * nD + n] - 1]) ok = false;
+                if (!((crossed >> int((w >> (4 * k)) & 0xF)) & 1u)) ok = false;
         }
         if (!ok) c = 0;
         cube[n] = c;
@@ -521,7 +539,10 @@ __device__ __forceinline__ void cube_triangles_body(int psr_bid, int psr_nbid,
         for (int k = 0; k < 3 * nt; ++k) {
             const int e = int((w >> (4 * k)) & 0xF);
             edgeHas |= 1 << e;
-            tri[3 * base + k] = vexAddr[nodeEdge[e * nD + n] - 1];
+            int a = 0;  // va[e], by selects rather than an indexed register read
+#pragma unroll
+            for (int j = 0; j < 12; ++j) a = e == j ? va[j] : a;
+            tri[3 * base + k] = a;
         }
         for (int f = 0; f < 6; ++f) {
             bool mark = false;
