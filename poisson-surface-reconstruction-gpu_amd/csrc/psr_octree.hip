@@ -151,7 +151,7 @@ __device__ __forceinline__ void normalize_encode_body(int psr_bid, int psr_nbid,
                                                              float4* __restrict__ nrmOut,
                                                              unsigned* __restrict__ keys,
                                                              unsigned* __restrict__ vals, int tiles,
-                                                             int* __restrict__ hist) {
+                                                             int* __restrict__ hist, unsigned digitMask) {
     __shared__ int lh[256];
     static_assert(kThreads == 256 && kSortTileKeys == 2 * kThreads, "one thread per digit, two keys each");
     lh[threadIdx.x] = 0;
@@ -162,10 +162,10 @@ __device__ __forceinline__ void normalize_encode_body(int psr_bid, int psr_nbid,
         const unsigned code = encode_point(xyz, xs, nrm, ns, i, D, prm, posOut, nrmOut);
         keys[i] = code;
         vals[i] = unsigned(i);
-        atomicAdd(&lh[code & 255u], 1);
+        atomicAdd(&lh[code & digitMask], 1);
     }
     __syncthreads();
-    hist[threadIdx.x * tiles + psr_bid] = lh[threadIdx.x];
+    if (threadIdx.x <= digitMask) hist[threadIdx.x * tiles + psr_bid] = lh[threadIdx.x];
 }
 __global__ void __launch_bounds__(kThreads) normalize_encode(const char* __restrict__ xyz, unsigned xs,
                                                              const char* __restrict__ nrm, unsigned ns, int n,
@@ -175,8 +175,8 @@ __global__ void __launch_bounds__(kThreads) normalize_encode(const char* __restr
                                                              float4* __restrict__ nrmOut,
                                                              unsigned* __restrict__ keys,
                                                              unsigned* __restrict__ vals, int tiles,
-                                                             int* __restrict__ hist) {
-    normalize_encode_body(blockIdx.x, gridDim.x, xyz, xs, nrm, ns, n, D, part, nb, params, mail, posOut, nrmOut, keys, vals, tiles, hist);
+                                                             int* __restrict__ hist, unsigned digitMask) {
+    normalize_encode_body(blockIdx.x, gridDim.x, xyz, xs, nrm, ns, n, D, part, nb, params, mail, posOut, nrmOut, keys, vals, tiles, hist, digitMask);
 }
 PSR_BATCHABLE(normalize_encode, kThreads)
 
@@ -593,7 +593,8 @@ void build_octree_a(Ctx& c, const psr_points& pts) {
     const int sortTiles = int(div_up(N, kSortTileKeys));
     PSR_LAUNCH(normalize_encode, sortTiles, 0, s,
         reinterpret_cast<const char*>(pts.xyz), xs, reinterpret_cast<const char*>(pts.normal), ns, N, D, part, nbb,
-        o.normParams, reinterpret_cast<float*>(c.mail + kMailNorm), posU, nrmU, keys, vals, sortTiles, sortHist);
+        o.normParams, reinterpret_cast<float*>(c.mail + kMailNorm), posU, nrmU, keys, vals, sortTiles, sortHist,
+        (1u << sort_digit_bits(3 * D)) - 1u);
     // sorted by code; the last pass gathers the points into code order
     o.spos = c.buf<float4>("spos", N);
     o.snrm = c.buf<float4>("snrm", N);

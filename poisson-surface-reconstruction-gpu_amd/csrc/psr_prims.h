@@ -51,8 +51,11 @@ size_t sort_scratch_ints(size_t n);
 size_t sort_scan_elems(size_t n);
 // Tiles of the sort: the first pass's histogram, when its producer takes it
 // (firstHist), holds at scratch[d * tiles + t] the keys of tile t (keys
-// [512 t, 512 t + 512)) whose low 8 bits are d.
+// [512 t, 512 t + 512)) whose low sort_digit_bits(bits) bits are d.
 constexpr int kSortTileKeys = 512;
+// Width of each pass's digit for keys of `bits` bits: the fewest passes of at
+// most 8 bits, the bits spread evenly over them (the last may be narrower).
+int sort_digit_bits(int bits);
 // Optional gather fused into the last pass: posOut[i] = posIn[v], nrmOut[i] =
 // nrmIn[v] for the value v that lands at i.
 struct SortGather {

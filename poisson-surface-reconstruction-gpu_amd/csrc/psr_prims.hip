@@ -106,8 +106,10 @@ void scan_exclusive(const int* in, int* out, int n, int* total, ScanState& st, h
 }
 
 // ---------------------------------------------------------------------------
-// Stable LSD radix sort, 8-bit digits.
-//   hist:    per tile 256-bin histogram, stored digit-major (digit * tiles + tile)
+// Stable LSD radix sort, digits of at most 8 bits: the fewest passes of 8
+// bits, with the key's bits spread evenly over them (21 bits: 7 + 7 + 7), so
+// that each pass scans as few histogram entries as it can.
+//   hist:    per tile 2^b-bin histogram, stored digit-major (digit * tiles + tile)
 //   scan:    exclusive scan of the digit-major histogram -> global offsets
 //   scatter: per tile, stable rank by wave ballots (multisplit) + offsets;
 //            the last pass can also gather the points into the sorted order
@@ -119,7 +121,7 @@ static_assert(kSortTile == kSortTileKeys, "tile of the sort's interface");
 static constexpr int kSortWaves = kSortThreads / PSR_WAVE;
 
 __device__ __forceinline__ void radix_hist_body(int psr_bid, int psr_nbid, const unsigned* __restrict__ keys, int n, int shift,
-                                                           int tiles, int* __restrict__ hist) {
+                                                           unsigned mask, int tiles, int* __restrict__ hist) {
     __shared__ int h[256];
     h[threadIdx.x] = 0;
     __syncthreads();
@@ -127,14 +129,14 @@ __device__ __forceinline__ void radix_hist_body(int psr_bid, int psr_nbid, const
 #pragma unroll 4
     for (int k = 0; k < kSortIters; ++k) {
         int i = base + k * kSortThreads + threadIdx.x;
-        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255], 1);
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & mask], 1);
     }
     __syncthreads();
-    hist[threadIdx.x * tiles + psr_bid] = h[threadIdx.x];
+    if (threadIdx.x <= mask) hist[threadIdx.x * tiles + psr_bid] = h[threadIdx.x];
 }
 __global__ void __launch_bounds__(kSortThreads) radix_hist(const unsigned* __restrict__ keys, int n, int shift,
-                                                           int tiles, int* __restrict__ hist) {
-    radix_hist_body(blockIdx.x, gridDim.x, keys, n, shift, tiles, hist);
+                                                           unsigned mask, int tiles, int* __restrict__ hist) {
+    radix_hist_body(blockIdx.x, gridDim.x, keys, n, shift, mask, tiles, hist);
 }
 PSR_BATCHABLE(radix_hist, kSortThreads)
 
@@ -143,12 +145,12 @@ __device__ __forceinline__ void radix_scatter_body(int psr_bid, int psr_nbid, co
                                                               const unsigned* __restrict__ vin,
                                                               unsigned* __restrict__ kout,
                                                               unsigned* __restrict__ vout, int n, int shift,
-                                                              int tiles, const int* __restrict__ offs,
+                                                              unsigned mask, int tiles, const int* __restrict__ offs,
                                                               const SortGather& g) {
     __shared__ int running[256];      // digit count of earlier iterations + global base
     __shared__ int waveCnt[kSortWaves][256];
     const int lane = __lane_id(), wid = threadIdx.x / PSR_WAVE;
-    running[threadIdx.x] = offs[threadIdx.x * tiles + psr_bid];
+    running[threadIdx.x] = threadIdx.x <= mask ? offs[threadIdx.x * tiles + psr_bid] : 0;
     const unsigned long long ltmask = (1ull << lane) - 1ull;
     int base = psr_bid * kSortTile;
     for (int it = 0; it < kSortIters; ++it) {
@@ -158,10 +160,11 @@ __device__ __forceinline__ void radix_scatter_body(int psr_bid, int psr_nbid, co
         bool valid = i < n;
         unsigned key = valid ? kin[i] : 0u;
         unsigned val = valid ? vin[i] : 0u;
-        int d = int((key >> shift) & 255u);
+        int d = int((key >> shift) & mask);
         unsigned long long peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
+            if (!((mask >> b) & 1u)) break;  // the digit's bits end here
             unsigned long long bb = __ballot((d >> b) & 1);
             peers &= ((d >> b) & 1) ? bb : ~bb;
         }
@@ -190,12 +193,17 @@ __global__ void __launch_bounds__(kSortThreads) radix_scatter(const unsigned* __
                                                               const unsigned* __restrict__ vin,
                                                               unsigned* __restrict__ kout,
                                                               unsigned* __restrict__ vout, int n, int shift,
-                                                              int tiles, const int* __restrict__ offs,
+                                                              unsigned mask, int tiles, const int* __restrict__ offs,
                                                               SortGather g) {
-    radix_scatter_body(blockIdx.x, gridDim.x, kin, vin, kout, vout, n, shift, tiles, offs, g);
+    radix_scatter_body(blockIdx.x, gridDim.x, kin, vin, kout, vout, n, shift, mask, tiles, offs, g);
 }
 PSR_BATCHABLE(radix_scatter, kSortThreads)
 
+
+int sort_digit_bits(int bits) {
+    const int passes = (bits + 7) / 8;
+    return passes > 0 ? (bits + passes - 1) / passes : 0;
+}
 
 size_t sort_scan_elems(size_t n) { return 256 * size_t(div_up(n, kSortTile)); }
 
@@ -208,16 +216,19 @@ size_t sort_scratch_ints(size_t n) {
 void radix_sort_pairs(unsigned*& keys, unsigned*& vals, unsigned*& ktmp, unsigned*& vtmp, int n, int bits,
                       int* scratch, bool firstHist, ScanState& st, const SortGather& g, hipStream_t s) {
     if (n <= 0) return;
+    if (bits <= 0) return;
     int tiles = int(div_up(n, kSortTile));
     int* hist = scratch;
-    int passes = (bits + 7) / 8;
+    const int passes = (bits + 7) / 8, width = sort_digit_bits(bits);
     unsigned *ka = keys, *va = vals, *kb = ktmp, *vb = vtmp;
     for (int p = 0; p < passes; ++p) {
-        int shift = 8 * p;
-        if (p > 0 || !firstHist) PSR_LAUNCH(radix_hist, tiles, 0, s, ka, n, shift, tiles, hist);
-        scan_exclusive(hist, hist, 256 * tiles, nullptr, st, s);
+        const int shift = width * p, b = std::min(width, bits - shift);
+        const unsigned mask = (1u << b) - 1u;
+        if (p > 0 || !firstHist) PSR_LAUNCH(radix_hist, tiles, 0, s, ka, n, shift, mask, tiles, hist);
+        scan_exclusive(hist, hist, int(mask + 1) * tiles, nullptr, st, s);
         const SortGather none{};
-        PSR_LAUNCH(radix_scatter, tiles, 0, s, ka, va, kb, vb, n, shift, tiles, hist, p == passes - 1 ? g : none);
+        PSR_LAUNCH(radix_scatter, tiles, 0, s, ka, va, kb, vb, n, shift, mask, tiles, hist,
+                   p == passes - 1 ? g : none);
         std::swap(ka, kb);
         std::swap(va, vb);
     }
