@@ -100,7 +100,8 @@ struct Field {
     float* div = nullptr;     // [M]
     float* dx = nullptr;      // [M]
     float* chiPts = nullptr;  // [N]
-    float4* nbrCol = nullptr;  // [27][M]: centre (x, y, z) and d_x of neighbour i of each node (0 where absent)
+    float4* nbrCol = nullptr;  // [27][M]: centre (x, y, z) and d_x of neighbour i of each node (0 where absent);
+                               // written only when the level-local child 0 rule changed a parent
     float* dCol = nullptr;     // [27][M]: d_x of neighbour i of each node (0 where absent), 4 B a term
     double* isoSum = nullptr; // device scalar
     float iso = 0.f;

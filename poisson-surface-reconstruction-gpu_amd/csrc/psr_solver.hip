@@ -1432,8 +1432,11 @@ PSR_BATCHABLE_WAVES(cg_mixed_minres, kCgThreads, kCgMinresWaves)
 
 
 // --- implicit function at points (LaplacianSolver.cu:85-118, 269-305) ------
-// Neighbour columns: basis parameters and d_x of neighbour i of every node, so
-// the 27 terms of a node need no dependent index loads.
+// Neighbour columns: d_x of neighbour i of every node, and, when the
+// level-local child 0 rule changed a parent, its basis parameters as well, so
+// the 27 terms of a node need no dependent index loads.  (The kernels of a
+// regular octree take the centres from the node grid; the few that need an
+// ancestor's neighbour centres read them through the neighbour table.)
 // Also clears the nZero words at zero (the mesh stage's node tables and face
 // marks) and, when the octree is regular, the nDense dense d_x grid cells,
 // which saves the mesh stage launches of its own.
@@ -1454,14 +1457,17 @@ __device__ __forceinline__ void neighbour_columns_body(int psr_bid, int psr_nbid
     if (nDense && *q1Fired == 0)
         for (size_t i = t; i < nDense; i += size_t(psr_nbid) * kThreads) dense[i] = 0.f;
     if (t >= size_t(27) * M) return;
+    const bool cols = *q1Fired != 0;
     const int nb = neigh[t];
     if (nb < 0) {
-        nbrCol[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (cols) nbrCol[t] = make_float4(0.f, 0.f, 0.f, 0.f);
         dCol[t] = 0.f;
     } else {
-        const float4 b = basis[nb];
         const float d = dx[nb];
-        nbrCol[t] = make_float4(b.x, b.y, b.z, d);
+        if (cols) {
+            const float4 b = basis[nb];
+            nbrCol[t] = make_float4(b.x, b.y, b.z, d);
+        }
         dCol[t] = d;
     }
 }

@@ -55,13 +55,25 @@ struct Grid {
     const int* neigh;
     const float4* basis;
     const float* dx;
-    const float4* nbrCol;    // [27][M] centre (x, y, z) and d_x of neighbour i of each node (0 where absent)
+    const float4* nbrCol;    // [27][M] centre (x, y, z) and d_x of neighbour i of each node (0 where absent),
+                             // written only for octrees in which the level-local child 0 rule changed a parent
     const int* rootNbr;      // [root][27] position in roots of the root's real neighbour, or -1
     const int* rootChain;    // [root][kChain] the root, then its ancestors (-1 past the tree root)
     const int4* rootCell;    // [root] cell at the root depth (x, y, z) and the root node (w)
     const unsigned* rootSet; // [root] bit i: neighbour i of the root belongs to the root's set
     const float* dxAt;       // per depth l, at (8^l - 1) / 7: d_x of the node at each cell, x-major; 0 where none
 };
+
+// Neighbour i of node as a column (centre, d_x; zeros where absent), read
+// through the neighbour table: the regular octree's tiles (the dense-halo
+// kernels) have no float4 columns.
+__device__ __forceinline__ float4 nbr_col(const Grid& g, int i, int node) {
+    const int nb = g.neigh[size_t(i) * g.M + node];
+    if (nb < 0) return make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 b = g.basis[nb];
+    return make_float4(b.x, b.y, b.z, g.dx[nb]);
+}
+
 constexpr int kChain = PSR_MAX_DEPTH + 1;
 
 // Bourke corner order: slot -> (x, y, z) offset (BuildMeshGeometry.cu:133-144).
@@ -842,7 +854,7 @@ __device__ __forceinline__ void sub_corner_values_grid_body(int psr_bid, int psr
         float4 b = make_float4(0.f, 0.f, 0.f, 0.f);
         float d = 0.f;
         if (node >= 0) {
-            const float4 col = g.nbrCol[size_t(k % 27) * g.M + node];
+            const float4 col = nbr_col(g, k % 27, node);
             b = make_float4(col.x, col.y, col.z, float(1 << (S.rd - k / 27)));
             d = col.w;
         }
@@ -1056,7 +1068,7 @@ __device__ __forceinline__ void sub_corner_values_small_grid_body(int psr_bid, i
         float4 b = make_float4(0.f, 0.f, 0.f, 0.f);
         float d = 0.f;
         if (node >= 0) {
-            const float4 col = g.nbrCol[size_t(k % 27) * g.M + node];
+            const float4 col = nbr_col(g, k % 27, node);
             b = make_float4(col.x, col.y, col.z, float(1 << (S.rd - k / 27)));
             d = col.w;
         }
