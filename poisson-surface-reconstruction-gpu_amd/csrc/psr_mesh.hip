@@ -128,7 +128,10 @@ PSR_BATCHABLE(iso_final, 1024)
 // dense: the dense d_x grids (or null), where a regular octree's nodes put dx.
 // Blocks from geoBlocks on compute the points' implicit values instead
 // (points_chi_block): independent work that fills the GPU beside this
-// kernel's few workgroups.
+// kernel's few workgroups.  Compiled for 4 waves per SIMD (128 registers): the
+// general ownership path, which only octrees with a changed parent take,
+// would hold 229 and keep every block, the points' included, at 2; it spills
+// instead.
 __device__ __forceinline__ void geometry_emit_body(int psr_bid, int psr_nbid, const NodeView& v, int offD, int nD,
                                                           const int* __restrict__ parent,
                                                           const int* __restrict__ q1Fired, const ScanTicket& tv,
@@ -340,7 +343,8 @@ __device__ __forceinline__ void geometry_emit_body(int psr_bid, int psr_nbid, co
         }
     }
 }
-__global__ void __launch_bounds__(kThreads) geometry_emit(NodeView v, int offD, int nD,
+constexpr int kGeometryWaves = 4;
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kGeometryWaves))) geometry_emit(NodeView v, int offD, int nD,
                                                           const int* __restrict__ parent,
                                                           const int* __restrict__ q1Fired, ScanTicket tv,
                                                           ScanTicket te, ScanTicket tf, GeoOut out,
@@ -348,7 +352,7 @@ __global__ void __launch_bounds__(kThreads) geometry_emit(NodeView v, int offD, 
                                                           int geoBlocks, PointsChiArgs pc) {
     geometry_emit_body(blockIdx.x, gridDim.x, v, offD, nD, parent, q1Fired, tv, te, tf, out, dx, dense, geoBlocks, pc);
 }
-PSR_BATCHABLE(geometry_emit, kThreads)
+PSR_BATCHABLE_WAVES(geometry_emit, kThreads, kGeometryWaves)
 
 
 // --- vertex implicit values (ComputeTriangleIndices.cu:346-399), quirk Q8 --
