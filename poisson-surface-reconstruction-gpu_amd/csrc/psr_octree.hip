@@ -219,29 +219,41 @@ PSR_BATCHABLE(rank_tile_hist, kThreads)
 __device__ __forceinline__ void rank_tile_scan_body(int psr_bid, int psr_nbid, const int* __restrict__ hist, int tiles, int D,
                                                        int* __restrict__ tileOff, int* __restrict__ U,
                                                        int* __restrict__ mail) {
-    int d = psr_bid + 1;
-    __shared__ int run;
-    if (threadIdx.x == 0) run = 0;
-    __syncthreads();
+    const int d = psr_bid + 1;
+    const int lane = __lane_id(), w = threadIdx.x / PSR_WAVE;
+    __shared__ int wsum[1024 / PSR_WAVE];
+    int run = 0;  // the same in every thread
     for (int t0 = 0; t0 < tiles; t0 += 1024) {
-        int t = t0 + threadIdx.x;
+        const int t = t0 + threadIdx.x;
         int c = 0;
-        if (t < tiles)
-            for (int l = 1; l <= d; ++l) c += hist[l * tiles + t];
-        // block exclusive scan
-        __shared__ int sbuf[1024];
-        sbuf[threadIdx.x] = c;
-        __syncthreads();
-        for (int off = 1; off < 1024; off <<= 1) {
-            int v = threadIdx.x >= off ? sbuf[threadIdx.x - off] : 0;
-            __syncthreads();
-            sbuf[threadIdx.x] += v;
-            __syncthreads();
+        if (t < tiles) {
+            // every level's count read at once (the histogram holds PSR_MAX_DEPTH + 2
+            // rows, those past D unwritten), the levels past d dropped after
+            int v[PSR_MAX_DEPTH];
+#pragma unroll
+            for (int l = 1; l <= PSR_MAX_DEPTH; ++l) v[l - 1] = hist[l * tiles + t];
+#pragma unroll
+            for (int l = 1; l <= PSR_MAX_DEPTH; ++l) c += l <= d ? v[l - 1] : 0;
         }
-        if (t < tiles) tileOff[d * tiles + t] = run + sbuf[threadIdx.x] - c;
+        // block exclusive scan: wave scans, then the waves' totals (one barrier)
+        int inc = c;
+#pragma unroll
+        for (int off = 1; off < PSR_WAVE; off <<= 1) {
+            const int y = __shfl_up(inc, off, PSR_WAVE);
+            if (lane >= off) inc += y;
+        }
+        if (lane == PSR_WAVE - 1) wsum[w] = inc;
         __syncthreads();
-        if (threadIdx.x == 1023) run += sbuf[1023];
-        __syncthreads();
+        int pre = 0, tot = 0;
+#pragma unroll
+        for (int k = 0; k < 1024 / PSR_WAVE; ++k) {
+            const int sk = wsum[k];
+            pre += k < w ? sk : 0;
+            tot += sk;
+        }
+        if (t < tiles) tileOff[d * tiles + t] = run + pre + inc - c;
+        run += tot;
+        __syncthreads();  // wsum is written again by the next chunk
     }
     if (threadIdx.x == 0) U[d] = mail[d] = run;
 }

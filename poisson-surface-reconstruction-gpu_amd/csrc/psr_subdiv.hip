@@ -1363,12 +1363,27 @@ __device__ __forceinline__ void sub_tile_scan_body(int psr_bid, int psr_nbid, co
     __shared__ int3 lSum3[kW];
     const int per = (n + kScanThreads - 1) / kScanThreads;
     const int i0 = min(n, int(threadIdx.x) * per), i1 = min(n, i0 + per);
+    // up to kPerMax counts a thread are held in registers, read at once
+    constexpr int kPerMax = 8;
+    const bool held = per <= kPerMax;
+    int4 v[kPerMax];
     int a = 0, b = 0, c = 0;
-    for (int i = i0; i < i1; ++i) {
-        const int4 t = cnt[i];
-        a += t.x;
-        b += t.y;
-        c += t.z;
+    if (held) {
+#pragma unroll
+        for (int k = 0; k < kPerMax; ++k) v[k] = i0 + k < i1 ? cnt[i0 + k] : make_int4(0, 0, 0, 0);
+#pragma unroll
+        for (int k = 0; k < kPerMax; ++k) {
+            a += v[k].x;
+            b += v[k].y;
+            c += v[k].z;
+        }
+    } else {
+        for (int i = i0; i < i1; ++i) {
+            const int4 t = cnt[i];
+            a += t.x;
+            b += t.y;
+            c += t.z;
+        }
     }
     // the three exclusive scans in one pass over the block (one barrier pair)
     const int lane = __lane_id(), w = threadIdx.x / PSR_WAVE;
@@ -1390,12 +1405,23 @@ __device__ __forceinline__ void sub_tile_scan_body(int psr_bid, int psr_nbid, co
     a = pa + xa - a;
     b = pb + xb - b;
     c = pc + xc - c;
-    for (int i = i0; i < i1; ++i) {
-        const int4 t = cnt[i];
-        base[i] = make_int4(a, b, c, 0);
-        a += t.x;
-        b += t.y;
-        c += t.z;
+    if (held) {
+#pragma unroll
+        for (int k = 0; k < kPerMax; ++k) {
+            if (i0 + k >= i1) break;
+            base[i0 + k] = make_int4(a, b, c, 0);
+            a += v[k].x;
+            b += v[k].y;
+            c += v[k].z;
+        }
+    } else {
+        for (int i = i0; i < i1; ++i) {
+            const int4 t = cnt[i];
+            base[i] = make_int4(a, b, c, 0);
+            a += t.x;
+            b += t.y;
+            c += t.z;
+        }
     }
     if (threadIdx.x == 0) {
         totals[0] = ta;
