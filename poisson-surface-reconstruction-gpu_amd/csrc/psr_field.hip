@@ -263,19 +263,6 @@ __device__ __forceinline__ void divergence_fine_body(int psr_bid, int psr_nbid,
     acc = wave_sum_f64(acc);
     if (lane == 0) div[o] = float(acc);
 }
-template <class FieldT>
-__global__ void __launch_bounds__(kDivWaves * PSR_WAVE) divergence_fine(
-    int begin, int count, int M, int offD, int nD, int res, int D, const int* __restrict__ neigh,
-    const int* __restrict__ didx, const int* __restrict__ dnum, const int* __restrict__ enc,
-    const FieldT* __restrict__ vf, const double* __restrict__ ddot, float* __restrict__ div) {
-    divergence_fine_body<FieldT>(blockIdx.x, gridDim.x, begin, count, M, offD, nD, res, D, neigh, didx, dnum, enc, vf, ddot, div);
-}
-template <class FieldT>
-struct divergence_fine_k {
-    static constexpr int kBlock = kDivWaves * PSR_WAVE;
-    template <class... A>
-    __device__ __forceinline__ static void run(int bid, int nbid, const A&... a) { divergence_fine_body<FieldT>(bid, nbid, a...); }
-};
 
 
 // Coarse nodes (few, each covering a large share of depth D): `parts`
@@ -335,19 +322,41 @@ __device__ __forceinline__ void divergence_coarse_body(int psr_bid, int psr_nbid
         partial[psr_bid] = s;
     }
 }
+// The coarse and the middle depths in one launch: workgroups below
+// coarseBlocks take the coarse nodes' shares, the others a middle-depth node
+// per wave.  Both read only the vector field and write disjoint outputs, so
+// the middle depths' waves fill the GPU beside the coarse nodes' long sums
+// instead of waiting for them.
+static_assert(kDivCoarseThreads == kDivWaves * PSR_WAVE, "one block size for both roles");
 template <class FieldT>
-__global__ void __launch_bounds__(kDivCoarseThreads) divergence_coarse(
-    CoarsePlan P, int M, int offD, int nD, int res, int D, const int* __restrict__ neigh,
-    const int* __restrict__ didx, const int* __restrict__ dnum, const int* __restrict__ enc,
-    const FieldT* __restrict__ vf, const double* __restrict__ ddot, double* __restrict__ partial,
-    int* __restrict__ nnz) {
-    divergence_coarse_body<FieldT>(blockIdx.x, gridDim.x, P, M, offD, nD, res, D, neigh, didx, dnum, enc, vf, ddot, partial, nnz);
+__device__ __forceinline__ void divergence_mid_body(int psr_bid, int psr_nbid, const CoarsePlan& P, int coarseBlocks,
+                                                    int begin, int count, int M, int offD, int nD, int res, int D,
+                                                    const int* __restrict__ neigh, const int* __restrict__ didx,
+                                                    const int* __restrict__ dnum, const int* __restrict__ enc,
+                                                    const FieldT* __restrict__ vf, const double* __restrict__ ddot,
+                                                    double* __restrict__ partial, int* __restrict__ nnz,
+                                                    float* __restrict__ div) {
+    if (psr_bid < coarseBlocks)
+        divergence_coarse_body<FieldT>(psr_bid, coarseBlocks, P, M, offD, nD, res, D, neigh, didx, dnum, enc, vf, ddot,
+                                       partial, nnz);
+    else
+        divergence_fine_body<FieldT>(psr_bid - coarseBlocks, psr_nbid - coarseBlocks, begin, count, M, offD, nD, res, D,
+                                     neigh, didx, dnum, enc, vf, ddot, div);
 }
 template <class FieldT>
-struct divergence_coarse_k {
+__global__ void __launch_bounds__(kDivCoarseThreads) divergence_mid(
+    CoarsePlan P, int coarseBlocks, int begin, int count, int M, int offD, int nD, int res, int D,
+    const int* __restrict__ neigh, const int* __restrict__ didx, const int* __restrict__ dnum,
+    const int* __restrict__ enc, const FieldT* __restrict__ vf, const double* __restrict__ ddot,
+    double* __restrict__ partial, int* __restrict__ nnz, float* __restrict__ div) {
+    divergence_mid_body<FieldT>(blockIdx.x, gridDim.x, P, coarseBlocks, begin, count, M, offD, nD, res, D, neigh, didx,
+                                dnum, enc, vf, ddot, partial, nnz, div);
+}
+template <class FieldT>
+struct divergence_mid_k {
     static constexpr int kBlock = kDivCoarseThreads;
     template <class... A>
-    __device__ __forceinline__ static void run(int bid, int nbid, const A&... a) { divergence_coarse_body<FieldT>(bid, nbid, a...); }
+    __device__ __forceinline__ static void run(int bid, int nbid, const A&... a) { divergence_mid_body<FieldT>(bid, nbid, a...); }
 };
 
 
@@ -493,15 +502,13 @@ void field_stages(Ctx& c, FieldT* vf) {
         P.blockStart[kCoarse + 1] = blocks;
         P.base[kCoarse + 1] = L.base[kCoarse + 1];
         partial = c.buf<double>("div_partial", blocks);
-        PSR_LAUNCH_K(divergence_coarse<FieldT>, divergence_coarse_k<FieldT>, blocks, 0, s, P, M, offD, nD, c.tables.res, D, o.neigh, o.didx,
-                                                                      o.dnum, o.encIdx, vf, c.tables.ddot, partial,
-                                                                      c.results()->cgNnz);
+        // the middle depths (a wave per node) in the same launch
+        const int midCount = std::max(offD - coarseCount, 0);
+        const int midBlocks = int(div_up(midCount, kDivWaves));
+        PSR_LAUNCH_K(divergence_mid<FieldT>, divergence_mid_k<FieldT>, blocks + midBlocks, 0, s, P, blocks, coarseCount,
+                     midCount, M, offD, nD, c.tables.res, D, o.neigh, o.didx, o.dnum, o.encIdx, vf, c.tables.ddot,
+                     partial, c.results()->cgNnz, c.fld.div);
     }
-    const int midCount = offD - coarseCount;
-    if (midCount > 0)
-        PSR_LAUNCH_K(divergence_fine<FieldT>, divergence_fine_k<FieldT>, div_up(midCount, kDivWaves), 0, s,
-            coarseCount, midCount, M, offD, nD, c.tables.res, D, o.neigh, o.didx, o.dnum, o.encIdx, vf, c.tables.ddot,
-            c.fld.div);
     // the operator (it needs only the octree and the tables) is assembled in
     // the same launch
     const AssembleArgs aa{L,     M,     o.neigh, o.encIdx, c.tables.dot, c.tables.d2dot, c.tables.res,
