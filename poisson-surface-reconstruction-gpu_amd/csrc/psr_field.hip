@@ -164,7 +164,11 @@ template <class FieldT>
 __device__ __forceinline__ void vector_field_kernel_body(int psr_bid, int psr_nbid, 
     int nD, int offD, int M, int D, const int* __restrict__ neigh, const int* __restrict__ pidx,
     const int* __restrict__ pnum, const int* __restrict__ key, const float4* __restrict__ pos,
-    const float4* __restrict__ nrm, FieldT* __restrict__ vf) {
+    const float4* __restrict__ nrm, FieldT* __restrict__ vf, int* __restrict__ clear) {
+    // the operator's per-depth entry counts and slot masks, which the
+    // divergence launch's assembly adds to, and the CG error flag between
+    // them in the results block
+    if (psr_bid == 0 && threadIdx.x < kResultsClearWords) clear[threadIdx.x] = 0;
     __shared__ NbRanges R[kVfWaves][2];
     constexpr int kHalf = PSR_WAVE / 2;
     const int wid = threadIdx.x / PSR_WAVE, lane = __lane_id(), half = lane / kHalf, hl = lane % kHalf;
@@ -209,8 +213,8 @@ template <class FieldT>
 __global__ void __launch_bounds__(kVfWaves * PSR_WAVE) vector_field_kernel(
     int nD, int offD, int M, int D, const int* __restrict__ neigh, const int* __restrict__ pidx,
     const int* __restrict__ pnum, const int* __restrict__ key, const float4* __restrict__ pos,
-    const float4* __restrict__ nrm, FieldT* __restrict__ vf) {
-    vector_field_kernel_body<FieldT>(xcd_block(blockIdx.x, gridDim.x), gridDim.x, nD, offD, M, D, neigh, pidx, pnum, key, pos, nrm, vf);
+    const float4* __restrict__ nrm, FieldT* __restrict__ vf, int* __restrict__ clear) {
+    vector_field_kernel_body<FieldT>(xcd_block(blockIdx.x, gridDim.x), gridDim.x, nD, offD, M, D, neigh, pidx, pnum, key, pos, nrm, vf, clear);
 }
 template <class FieldT>
 struct vector_field_kernel_k {
@@ -267,7 +271,7 @@ __device__ __forceinline__ void divergence_fine_body(int psr_bid, int psr_nbid,
 
 // Coarse nodes (few, each covering a large share of depth D): `parts`
 // workgroups each, every one summing a strided share of the node's terms; the
-// shares are folded in a fixed order by the first threads of divergence_leaf.
+// shares are folded in a fixed order by divergence_fold.
 constexpr int kDivCoarseThreads = 256;
 constexpr int kMaxCoarse = 4;  // coarse depths 0..3
 struct CoarsePlan {
@@ -280,12 +284,7 @@ template <class FieldT>
 __device__ __forceinline__ void divergence_coarse_body(int psr_bid, int psr_nbid, 
     const CoarsePlan& P, int M, int offD, int nD, int res, int D, const int* __restrict__ neigh,
     const int* __restrict__ didx, const int* __restrict__ dnum, const int* __restrict__ enc,
-    const FieldT* __restrict__ vf, const double* __restrict__ ddot, double* __restrict__ partial,
-    int* __restrict__ nnz) {
-    // the operator's per-depth entry counts and slot masks, added to beside
-    // the last divergence kernel, and the CG error flag between them in the
-    // results block
-    if (psr_bid == 0 && threadIdx.x < kResultsClearWords) nnz[threadIdx.x] = 0;
+    const FieldT* __restrict__ vf, const double* __restrict__ ddot, double* __restrict__ partial) {
     int d = 0;
     while (d + 1 < P.depths && int(psr_bid) >= P.blockStart[d + 1]) ++d;
     const int parts = P.parts[d];
@@ -322,71 +321,19 @@ __device__ __forceinline__ void divergence_coarse_body(int psr_bid, int psr_nbid
         partial[psr_bid] = s;
     }
 }
-// The coarse and the middle depths in one launch: workgroups below
-// coarseBlocks take the coarse nodes' shares, the others a middle-depth node
-// per wave.  Both read only the vector field and write disjoint outputs, so
-// the middle depths' waves fill the GPU beside the coarse nodes' long sums
-// instead of waiting for them.
-static_assert(kDivCoarseThreads == kDivWaves * PSR_WAVE, "one block size for both roles");
-template <class FieldT>
-__device__ __forceinline__ void divergence_mid_body(int psr_bid, int psr_nbid, const CoarsePlan& P, int coarseBlocks,
-                                                    int begin, int count, int M, int offD, int nD, int res, int D,
-                                                    const int* __restrict__ neigh, const int* __restrict__ didx,
-                                                    const int* __restrict__ dnum, const int* __restrict__ enc,
-                                                    const FieldT* __restrict__ vf, const double* __restrict__ ddot,
-                                                    double* __restrict__ partial, int* __restrict__ nnz,
-                                                    float* __restrict__ div) {
-    if (psr_bid < coarseBlocks)
-        divergence_coarse_body<FieldT>(psr_bid, coarseBlocks, P, M, offD, nD, res, D, neigh, didx, dnum, enc, vf, ddot,
-                                       partial, nnz);
-    else
-        divergence_fine_body<FieldT>(psr_bid - coarseBlocks, psr_nbid - coarseBlocks, begin, count, M, offD, nD, res, D,
-                                     neigh, didx, dnum, enc, vf, ddot, div);
-}
-template <class FieldT>
-__global__ void __launch_bounds__(kDivCoarseThreads) divergence_mid(
-    CoarsePlan P, int coarseBlocks, int begin, int count, int M, int offD, int nD, int res, int D,
-    const int* __restrict__ neigh, const int* __restrict__ didx, const int* __restrict__ dnum,
-    const int* __restrict__ enc, const FieldT* __restrict__ vf, const double* __restrict__ ddot,
-    double* __restrict__ partial, int* __restrict__ nnz, float* __restrict__ div) {
-    divergence_mid_body<FieldT>(blockIdx.x, gridDim.x, P, coarseBlocks, begin, count, M, offD, nD, res, D, neigh, didx,
-                                dnum, enc, vf, ddot, partial, nnz, div);
-}
-template <class FieldT>
-struct divergence_mid_k {
-    static constexpr int kBlock = kDivCoarseThreads;
-    template <class... A>
-    __device__ __forceinline__ static void run(int bid, int nbid, const A&... a) { divergence_mid_body<FieldT>(bid, nbid, a...); }
-};
-
-
 // Depth-D nodes: their neighbours are depth-D nodes, each its own only
 // descendant, so a thread sums the 27 terms of its node.  The first threads
 // also fold the coarse nodes' shares (thread t: coarse node t), in order.
 // Blocks from leafBlocks on assemble the operator instead (assemble_rows):
 // independent work that fills the GPU beside this kernel's few workgroups.
+// Depth-D nodes: their neighbours are depth-D nodes, each its own only
+// descendant, so a thread sums the 27 terms of its node.
 template <class FieldT>
-__device__ __forceinline__ void divergence_leaf_body(int psr_bid, int psr_nbid, int offD, int nD, int M, int res, int D,
-                                                       const int* __restrict__ neigh, const int* __restrict__ enc,
-                                                       const FieldT* __restrict__ vf,
-                                                       const double* __restrict__ ddot, const CoarsePlan& P,
-                                                       const double* __restrict__ partial, float* __restrict__ div,
-                                                       int leafBlocks, const AssembleArgs& aa) {
-    static_assert(kAssembleThreads == 256, "one block size for both roles");
-    if (int(psr_bid) >= leafBlocks) {
-        assemble_rows(aa, int(psr_bid) - leafBlocks);
-        return;
-    }
-    const int t = psr_bid * 256 + threadIdx.x;
-    if (t < P.base[P.depths]) {
-        int d = 0;
-        while (t >= P.base[d + 1]) ++d;
-        const int parts = P.parts[d];
-        const double* pp = partial + P.blockStart[d] + size_t(t - P.base[d]) * parts;
-        double s = 0;
-        for (int k = 0; k < parts; ++k) s += pp[k];
-        div[t] = float(s);
-    }
+__device__ __forceinline__ void divergence_leaf_nodes(int block, int offD, int nD, int M, int res, int D,
+                                                      const int* __restrict__ neigh, const int* __restrict__ enc,
+                                                      const FieldT* __restrict__ vf,
+                                                      const double* __restrict__ ddot, float* __restrict__ div) {
+    const int t = block * 256 + threadIdx.x;
     if (t >= nD) return;
     const int o = offD + t;
     const int3 fo = decode_func_index(enc[o], D);
@@ -399,21 +346,79 @@ __device__ __forceinline__ void divergence_leaf_body(int psr_bid, int psr_nbid, 
         if (nb[i] != -1) acc += div_term(nb[i] - offD, offD, nD, res, D, fo, enc, vf, ddot);
     div[o] = float(acc);
 }
+
+// Every depth in one launch, each role a range of 256-thread workgroups: the
+// coarse nodes' shares, a middle-depth node per wave, a depth-D node per
+// thread, and the operator's assembly (assemble_rows).  They read only the
+// vector field, the octree and the tables and write disjoint outputs, so the
+// roles fill the GPU side by side instead of one launch after another; the
+// coarse shares are folded by divergence_fold.
+struct DivRoles {
+    int coarseBlocks, midBlocks, leafBlocks;
+};
+static_assert(kDivCoarseThreads == kDivWaves * PSR_WAVE && kAssembleThreads == kDivCoarseThreads,
+              "one block size for every role");
 template <class FieldT>
-__global__ void __launch_bounds__(256) divergence_leaf(int offD, int nD, int M, int res, int D,
-                                                       const int* __restrict__ neigh, const int* __restrict__ enc,
-                                                       const FieldT* __restrict__ vf,
-                                                       const double* __restrict__ ddot, CoarsePlan P,
-                                                       const double* __restrict__ partial, float* __restrict__ div,
-                                                       int leafBlocks, AssembleArgs aa) {
-    divergence_leaf_body<FieldT>(blockIdx.x, gridDim.x, offD, nD, M, res, D, neigh, enc, vf, ddot, P, partial, div, leafBlocks, aa);
+__device__ __forceinline__ void divergence_all_body(int psr_bid, int psr_nbid, const CoarsePlan& P, const DivRoles& R,
+                                                    int begin, int count, int M, int offD, int nD, int res, int D,
+                                                    const int* __restrict__ neigh, const int* __restrict__ didx,
+                                                    const int* __restrict__ dnum, const int* __restrict__ enc,
+                                                    const FieldT* __restrict__ vf, const double* __restrict__ ddot,
+                                                    double* __restrict__ partial, float* __restrict__ div,
+                                                    const AssembleArgs& aa) {
+    int b = psr_bid;
+    if (b < R.coarseBlocks) {
+        divergence_coarse_body<FieldT>(b, R.coarseBlocks, P, M, offD, nD, res, D, neigh, didx, dnum, enc, vf, ddot,
+                                       partial);
+        return;
+    }
+    b -= R.coarseBlocks;
+    if (b < R.midBlocks) {
+        divergence_fine_body<FieldT>(b, R.midBlocks, begin, count, M, offD, nD, res, D, neigh, didx, dnum, enc, vf,
+                                     ddot, div);
+        return;
+    }
+    b -= R.midBlocks;
+    if (b < R.leafBlocks) {
+        divergence_leaf_nodes<FieldT>(b, offD, nD, M, res, D, neigh, enc, vf, ddot, div);
+        return;
+    }
+    assemble_rows(aa, b - R.leafBlocks);
 }
 template <class FieldT>
-struct divergence_leaf_k {
-    static constexpr int kBlock = 256;
+__global__ void __launch_bounds__(kDivCoarseThreads) divergence_all(
+    CoarsePlan P, DivRoles R, int begin, int count, int M, int offD, int nD, int res, int D,
+    const int* __restrict__ neigh, const int* __restrict__ didx, const int* __restrict__ dnum,
+    const int* __restrict__ enc, const FieldT* __restrict__ vf, const double* __restrict__ ddot,
+    double* __restrict__ partial, float* __restrict__ div, AssembleArgs aa) {
+    divergence_all_body<FieldT>(blockIdx.x, gridDim.x, P, R, begin, count, M, offD, nD, res, D, neigh, didx, dnum, enc,
+                                vf, ddot, partial, div, aa);
+}
+template <class FieldT>
+struct divergence_all_k {
+    static constexpr int kBlock = kDivCoarseThreads;
     template <class... A>
-    __device__ __forceinline__ static void run(int bid, int nbid, const A&... a) { divergence_leaf_body<FieldT>(bid, nbid, a...); }
+    __device__ __forceinline__ static void run(int bid, int nbid, const A&... a) { divergence_all_body<FieldT>(bid, nbid, a...); }
 };
+
+// The coarse nodes' divergence: thread t folds node t's shares, in order.
+__device__ __forceinline__ void divergence_fold_body(int psr_bid, int psr_nbid, const CoarsePlan& P,
+                                                     const double* __restrict__ partial, float* __restrict__ div) {
+    const int t = psr_bid * 256 + threadIdx.x;
+    if (t >= P.base[P.depths]) return;
+    int d = 0;
+    while (t >= P.base[d + 1]) ++d;
+    const int parts = P.parts[d];
+    const double* pp = partial + P.blockStart[d] + size_t(t - P.base[d]) * parts;
+    double s = 0;
+    for (int k = 0; k < parts; ++k) s += pp[k];
+    div[t] = float(s);
+}
+__global__ void __launch_bounds__(256) divergence_fold(CoarsePlan P, const double* __restrict__ partial,
+                                                       float* __restrict__ div) {
+    divergence_fold_body(blockIdx.x, gridDim.x, P, partial, div);
+}
+PSR_BATCHABLE(divergence_fold, 256)
 
 
 }  // namespace
@@ -480,8 +485,9 @@ void field_stages(Ctx& c, FieldT* vf) {
     const Levels& L = o.L;
     const int D = L.D, nD = L.count[D], offD = L.base[D], M = o.M;
     hipStream_t s = c.stream;
-    PSR_LAUNCH_K(vector_field_kernel<FieldT>, vector_field_kernel_k<FieldT>, div_up(nD, 2 * kVfWaves), 0, s,
-        nD, offD, M, D, o.neigh, o.pidx, o.pnum, o.key, o.spos, o.snrm, vf);
+    PSR_LAUNCH_K(vector_field_kernel<FieldT>, vector_field_kernel_k<FieldT>,
+        std::max<size_t>(1, div_up(nD, 2 * kVfWaves)), 0, s,
+        nD, offD, M, D, o.neigh, o.pidx, o.pnum, o.key, o.spos, o.snrm, vf, c.results()->cgNnz);
     // depths 0..kCoarse: several workgroups per node, per depth as many as the
     // node's expected share of depth D asks for; depth D: a thread per node;
     // between: a wave per node
@@ -502,20 +508,19 @@ void field_stages(Ctx& c, FieldT* vf) {
         P.blockStart[kCoarse + 1] = blocks;
         P.base[kCoarse + 1] = L.base[kCoarse + 1];
         partial = c.buf<double>("div_partial", blocks);
-        // the middle depths (a wave per node) in the same launch
-        const int midCount = std::max(offD - coarseCount, 0);
-        const int midBlocks = int(div_up(midCount, kDivWaves));
-        PSR_LAUNCH_K(divergence_mid<FieldT>, divergence_mid_k<FieldT>, blocks + midBlocks, 0, s, P, blocks, coarseCount,
-                     midCount, M, offD, nD, c.tables.res, D, o.neigh, o.didx, o.dnum, o.encIdx, vf, c.tables.ddot,
-                     partial, c.results()->cgNnz, c.fld.div);
     }
-    // the operator (it needs only the octree and the tables) is assembled in
-    // the same launch
+    // every depth's divergence and the operator's assembly (it needs only the
+    // octree and the tables) in one launch, then the coarse shares' fold
+    const int midCount = std::max(offD - coarseCount, 0);
+    const DivRoles R{P.blockStart[kCoarse + 1], int(div_up(midCount, kDivWaves)), int(div_up(nD, 256))};
     const AssembleArgs aa{L,     M,     o.neigh, o.encIdx, c.tables.dot, c.tables.d2dot, c.tables.res,
                           c.buf<float>("lap_val", size_t(27) * M), c.results()->cgNnz, c.results()->cgSlotMask};
-    const int leafBlocks = int(div_up(std::max(nD, coarseCount), 256));
-    PSR_LAUNCH_K(divergence_leaf<FieldT>, divergence_leaf_k<FieldT>, leafBlocks + int(div_up(size_t(3) * M, kAssembleThreads)), 0, s,
-        offD, nD, M, c.tables.res, D, o.neigh, o.encIdx, vf, c.tables.ddot, P, partial, c.fld.div, leafBlocks, aa);
+    const int assembleBlocks = int(div_up(size_t(3) * M, kAssembleThreads));
+    PSR_LAUNCH_K(divergence_all<FieldT>, divergence_all_k<FieldT>,
+                 R.coarseBlocks + R.midBlocks + R.leafBlocks + assembleBlocks, 0, s, P, R, coarseCount, midCount, M,
+                 offD, nD, c.tables.res, D, o.neigh, o.didx, o.dnum, o.encIdx, vf, c.tables.ddot, partial, c.fld.div,
+                 aa);
+    PSR_LAUNCH(divergence_fold, div_up(coarseCount, 256), 0, s, P, partial, c.fld.div);
     PSR_HIP_CHECK(hipGetLastError());
 }
 

@@ -1671,7 +1671,7 @@ void solve_depths(Ctx& c) {
     // clears them after every solve, so they need a launch of their own only
     // when the last call on this context did not complete or the slots moved.
     // The CG writes every depth's results itself; its error flag is cleared
-    // with the operator counts (divergence_mid).
+    // with the operator counts (by the vector-field kernel).
     const size_t slotWords = slots ? size_t(16) * wg : 0;
     if (!c.cgClean || slots != c.cgSlots || slotWords != c.cgSlotWords)
         zero_spans({{ctl, size_t(ctlWords)}, {slots, slotWords}}, s);
